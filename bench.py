@@ -1,0 +1,215 @@
+"""Benchmark: Mcell-steps/s of the plain FP64 timestep on one MI355X.
+
+Workload (kernel-scale plain FP64, nz x nr = 2048 x 1024): z in [0, 2],
+dx = 2/nz, r in [0, 1]; vorticity = 64 seeded Gaussians (amplitudes N(0,1),
+widths U[0.01, 0.05], centres uniform); body mask = smooth Heaviside of the
+union of 8 on-axis spheres (radius U[0.02, 0.06], centre z U[0.2, 1.8]);
+lambda = 1e4, nu = 1.5e-3, free stream U = 1 (no ramp).  A step is the full
+kill -> Poisson -> velocity -> dt -> penalise -> drag -> ENO3 -> RK2 sequence,
+device resident and replayed from a hipGraph.
+
+Prints one JSON line (rank 0).  `value` is whole-job Mcell-steps/s with the
+fields resident in HBM.  `roofline` covers the dominant stage, timed live
+with HIP events on the stepper's stream; `cpu_baseline` times the CPU oracle
+(numpy restatement of the reference, dense eig-route Poisson) on a bounded
+sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mcell-steps/s @2048x1024 FP64 vs batched FP32; p50/p99 case time-to-solution"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per cell of each stage (compulsory reads + writes, FP64):
+STAGE_BYTES_PER_CELL = {
+    "kill_taper": 0,            # perimeter only
+    "dct2_rows": 16,            # read omega, write omega-hat
+    "tridiag_columns": 16,      # read omega-hat, write psi-hat
+    "dct3_rows": 16,            # read psi-hat, write psi
+    "velocity_freestream": 24,  # read psi, write u_z, u_r
+    "finalize_dt": 0,
+    "penalize_curl": 56,        # read u_z, u_r, chi, omega; write u_pen (2), omega
+    "finalize_step": 0,
+    "eno3": 32,                 # read omega, u_pen (2); write omega'
+    "rk2_diffusion": 40,        # stage 1: 8 + 8; stage 2: 8 + 8 + 8
+}
+
+
+def workload(nz, nr, seed=0):
+    from pyaxisymflow_amd.stepper import rigid_grid
+
+    z_len = 2.0
+    dx, z, r = rigid_grid(nz, nr, z_len)
+    Z, R = np.meshgrid(z, r)
+    rng = np.random.default_rng(seed)
+    n = 64
+    a = rng.standard_normal(n)
+    s = rng.uniform(0.01, 0.05, n)
+    cz = rng.uniform(0, z_len, n)
+    cr = rng.uniform(0, nr * dx, n)
+    w = np.zeros_like(Z)
+    for i in range(n):
+        w += a[i] * np.exp(-((Z - cz[i]) ** 2 + (R - cr[i]) ** 2) / (2 * s[i] ** 2))
+    rng2 = np.random.default_rng(seed + 1)
+    rad = rng2.uniform(0.02, 0.06, 8)
+    zc = rng2.uniform(0.2, 1.8, 8)
+    phi = np.full_like(Z, -np.inf)
+    for i in range(8):
+        phi = np.maximum(phi, rad[i] - np.sqrt((Z - zc[i]) ** 2 + R**2))
+    wdt = np.sqrt(2) * dx
+    chi = (phi >= wdt) + (np.fabs(phi) < wdt) * 0.5 * (1 + phi / wdt + np.sin(np.pi * phi / wdt) / np.pi)
+    return dict(z_len=z_len, dx=dx, vorticity=w, char_func=chi.astype(np.float64),
+                r_sph=float(rad.mean()), nu=1.5e-3, brink_lam=1e4, U_0=1.0)
+
+
+def time_gpu(nz, nr, steps, warmup, stage_reps):
+    import ctypes
+
+    import torch
+
+    from pyaxisymflow_amd import _lib
+    from pyaxisymflow_amd.stepper import RigidBodyStepper
+
+    wl = workload(nz, nr)
+    st = RigidBodyStepper(nz, nr, wl["z_len"], wl["nu"], wl["brink_lam"], wl["char_func"],
+                          U_0=wl["U_0"], T_ramp=None, r_sph=wl["r_sph"],
+                          vorticity=wl["vorticity"], history=steps + warmup + 8)
+    stream = st.stream
+    st.step(warmup)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    st.step(steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    w = st.vorticity()
+    finite = bool(np.isfinite(w).all())
+    cd = st.drag_history()
+    # per-stage timing, each stage launched alone on the same stream
+    lib = _lib.load()
+    stages = {}
+    for q in range(lib.axs_rigid_stepper_stage_count()):
+        name = lib.axs_rigid_stepper_stage_name(q).decode()
+        _lib.call("axs_rigid_stepper_run_stage", st._h, q, 3, ctypes.c_void_p(stream.cuda_stream))
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        _lib.call("axs_rigid_stepper_run_stage", st._h, q, stage_reps,
+                  ctypes.c_void_p(stream.cuda_stream))
+        b.record(stream)
+        torch.cuda.synchronize()
+        stages[name] = a.elapsed_time(b) * 1e3 / stage_reps  # us per launch
+    return ms, finite, float(cd[-1]) if len(cd) else None, stages
+
+
+def cpu_baseline(nz, nr, steps):
+    """The oracle (numpy restatement of the reference path with the dense
+    eig-route Poisson solve, BLAS on all cores) on the same workload."""
+    from oracle.drivers import RigidBodyCase
+
+    wl = workload(nz, nr)
+    t0 = time.perf_counter()
+    case = RigidBodyCase(nz, nr, wl["z_len"], wl["nu"], wl["brink_lam"], wl["char_func"],
+                         U_0=wl["U_0"], r_sph=wl["r_sph"], vorticity=wl["vorticity"],
+                         solver="eig")
+    setup = time.perf_counter() - t0
+    case.step()  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        case.step()
+    dt = time.perf_counter() - t0
+    try:
+        from threadpoolctl import threadpool_info
+
+        blas = [(i.get("internal_api"), i.get("version"), i.get("num_threads"))
+                for i in threadpool_info() if i.get("user_api") == "blas"]
+    except Exception:
+        blas = []
+    cores = len(os.sched_getaffinity(0))
+    threads = max([b[2] for b in blas if b[2]] or [cores])
+    return dict(
+        value=nr * nz * steps / dt / 1e6,
+        unit="Mcell-steps/s",
+        cores=int(threads),
+        kind="port",
+        sample=(f"oracle (numpy, eig-route dense Poisson) at {nz}x{nr}: {steps} steps after 1 "
+                f"warm-up, {dt:.2f} s; eig set-up {setup:.2f} s not included; "
+                f"affinity {cores} cores; blas {blas}"),
+    )
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nz", type=int, default=2048)
+    ap.add_argument("--nr", type=int, default=1024)
+    ap.add_argument("--stage-reps", type=int, default=50)
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every case is independent and this build runs on "
+              "one device (see DESIGN.md for how the path shards); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    nz, nr = args.nz, args.nr
+    ms, finite, cd, stages = time_gpu(nz, nr, args.steps, args.warmup, args.stage_reps)
+    if not finite:
+        print("non-finite vorticity after the timed steps", file=sys.stderr)
+        sys.exit(1)
+    ms_step = ms / args.steps
+    value = nr * nz * args.steps / (ms / 1e3) / 1e6
+    dom = max(stages, key=stages.get)
+    bpc = STAGE_BYTES_PER_CELL[dom]
+    achieved = bpc * nr * nz / (stages[dom] * 1e-6) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain FP64 step, seeded 64-Gaussian vorticity + 8-sphere mask, {nz}x{nr}",
+            "nz": nz, "nr": nr, "domain": "z in [0,2], r in [0,1]", "lambda": 1e4, "nu": 1.5e-3,
+            "U": 1.0, "graph": True, "last_Cd": cd,
+            "stage_us": {k: round(v, 2) for k, v in stages.items()},
+            "step_bytes_per_cell": sum(STAGE_BYTES_PER_CELL.values()),
+            "graded_bytes_per_cell": 88,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_cell": bpc,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(nz, nr, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,202 @@
+/*
+ * axisym.h -- C ABI of libaxisym_hip.so, the MI355X (gfx950) kernels for
+ * PyAxisymFlow's per-timestep vorticity-streamfunction path on the
+ * axisymmetric (z, r) grid.
+ *
+ * Conventions
+ *   - Fields are row-major (nr, nz) arrays in device memory: row i = radial
+ *     index (r), column j = axial index (z, contiguous).  "n" is nr*nz.
+ *   - Every pointer is a device pointer unless the name ends in _host.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Calls are stream-ordered and asynchronous unless stated otherwise.
+ *   - Every function returns axs_status; on failure axs_last_error() holds
+ *     a message (thread-local).  Arguments are validated before any launch.
+ *   - Scalars that the reference computes on the host (dt) are passed by
+ *     value; the *_dev variants used by the fused stepper read them from
+ *     device memory instead.
+ * Citations name the reference interface each entry point replaces
+ * (paths inside the PyAxisymFlow tree).
+ */
+#ifndef AXISYM_H_
+#define AXISYM_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int axs_status;
+#define AXS_OK 0
+#define AXS_ERR_INVALID 1
+#define AXS_ERR_HIP 2
+#define AXS_ERR_UNSUPPORTED 3
+#define AXS_ERR_ALLOC 4
+
+/* Thread-local message for the last failing call on this thread. */
+const char* axs_last_error(void);
+/* Library version string. */
+const char* axs_version(void);
+
+/* ---- A1: kernels/kill_boundary_vorticity_sine.py:4-14 (z) and :17-27 (r).
+ * Z and R are the full (nr, nz) coordinate arrays the reference receives.
+ * Requires nz >= width, nr >= width, width >= 2. */
+axs_status axs_kill_boundary_vorticity_sine_z_f64(double* vorticity, const double* Z,
+                                                  int nr, int nz, int width, double dx,
+                                                  void* stream);
+axs_status axs_kill_boundary_vorticity_sine_r_f64(double* vorticity, const double* R,
+                                                  int nr, int nz, int width, double dx,
+                                                  void* stream);
+
+/* ---- A3: kernels/compute_velocity_from_psi.py:4-17 (nr, nz >= 3). */
+axs_status axs_compute_velocity_from_psi_unb_f64(double* u_z, double* u_r,
+                                                 const double* psi, const double* R,
+                                                 int nr, int nz, double dx, void* stream);
+
+/* ---- A4: kernels/compute_vorticity_from_velocity.py:4-13 (interior only). */
+axs_status axs_compute_vorticity_from_velocity_unb_f64(double* vort, const double* u_z,
+                                                       const double* u_r, int nr, int nz,
+                                                       double dx, void* stream);
+
+/* ---- A5: kernels/brinkmann_penalize.py:4-16 (elementwise over n cells). */
+axs_status axs_brinkmann_penalize_f64(double lam, double dt, const double* char_func,
+                                      double U_z, double U_r, const double* grid_u_z,
+                                      const double* grid_u_r, double* penalized_u_z,
+                                      double* penalized_u_r, int64_t n, void* stream);
+
+/* ---- A6: kernels/smooth_Heaviside.py:5-14 (elementwise over n cells). */
+axs_status axs_smooth_heaviside_f64(double* H, const double* phi, double blend_w,
+                                    int64_t n, void* stream);
+
+/* ---- A7: kernels/advect_vorticity_via_eno3.py:8-44 with
+ * pyst_kernels/advection_timestep.py:14-56 and advection_flux.py:9-163.
+ * One conservative ENO3 Euler step of omega (odd about the axis), mirrored
+ * in-index (no doubled buffers).  `scratch` is an (nr, nz) device buffer.
+ * Requires nr >= 3, nz >= 5. */
+axs_status axs_advect_vorticity_eno3_f64(double* vorticity, const double* u_z,
+                                         const double* u_r, double* scratch, int nr,
+                                         int nz, double dt, double dx, void* stream);
+
+/* ---- A8: kernels/diffusion_RK2.py:4-44 (temp_vorticity is in/out scratch). */
+axs_status axs_diffusion_rk2_unb_f64(double* vorticity, double* temp_vorticity,
+                                     const double* R, int nr, int nz, double nu,
+                                     double dt, double dx, void* stream);
+
+/* ---- A9: kernels/compute_forces.py:4-17 (F_pen's sum) and the drag
+ * coefficient sum of examples/FlowPastSphere/flow_past_sphere.py:166-175.
+ * Writes sum(R * chi * (u_z - U)) to out_sum[0] (device, FP64 always).
+ * `partials` is device scratch of axs_reduce_partials_size(n) doubles. */
+int64_t axs_reduce_partials_size(int64_t n);
+axs_status axs_penalization_force_sum_f64(const double* R, const double* chi,
+                                          const double* u_z, double U, int64_t n,
+                                          double* partials, double* out_sum,
+                                          void* stream);
+
+/* ---- A10: the dt rule's reduction np.amax(np.fabs(u_z) + np.fabs(u_r))
+ * (examples/FlowPastSphere/flow_past_sphere.py:150-153).  Writes to
+ * out_max[0] (device). */
+axs_status axs_max_abs_velocity_sum_f64(const double* u_z, const double* u_r, int64_t n,
+                                        double* partials, double* out_max, void* stream);
+
+/* ---- A11: elasto_kernels/solid_sigma.py:4-29.  eta_1z..eta_2r are in/out
+ * scratch with the reference's partial-write semantics. */
+axs_status axs_solid_sigma_f64(double* sigma_s_11, double* sigma_s_12, double* sigma_s_22,
+                               double G, double dx, const double* eta_1,
+                               const double* eta_2, double* eta_1z, double* eta_1r,
+                               double* eta_2z, double* eta_2r, int nr, int nz,
+                               void* stream);
+
+/* ---- A12: elasto_kernels/div_tau.py:4-34 (tau_z, tau_r in/out scratch). */
+axs_status axs_update_vorticity_from_solid_stress_f64(double* vorticity, double* tau_z,
+                                                      double* tau_r, const double* tau11,
+                                                      const double* tau12,
+                                                      const double* tau22, const double* R,
+                                                      int nr, int nz, double dt, double dx,
+                                                      void* stream);
+
+/* ---- A13: elasto_kernels/advect_refmap_via_eno3.py:8-54 with the
+ * non-conservative flux of pyst_kernels/advection_flux.py:166-332.
+ * eta1 is even and eta2 odd about the axis; `scratch` holds 2 (nr, nz) buffers. */
+axs_status axs_advect_refmap_eno3_f64(double* eta1, double* eta2, const double* u_z,
+                                      const double* u_r, double* scratch, int nr, int nz,
+                                      double dt, double dx, void* stream);
+
+/* ---- A2: kernels/FastDiagonalisationStokesSolver.py:6-156.
+ * Solves A_r psi + psi P_z^T = r * rhs with P_z, P_r, D_r as built at
+ * :41-97 (Neumann z and r), by a DCT-II along z and one radial tridiagonal
+ * solve per z-mode.  The plan owns its device tables and scratch. */
+typedef struct axs_stokes_plan axs_stokes_plan;
+#define AXS_BC_NEUMANN_Z_AND_R 0
+axs_status axs_stokes_plan_create(int nr, int nz, double dx, int bc_type,
+                                  axs_stokes_plan** plan_out);
+axs_status axs_stokes_plan_destroy(axs_stokes_plan* plan);
+/* psi and rhs may alias. */
+axs_status axs_stokes_solve_f64(axs_stokes_plan* plan, double* psi, const double* rhs,
+                                void* stream);
+
+/* ---- A14: core/src/extrapolate_using_least_squares.hpp:450-467
+ * (extrapolate_using_least_squares_till_first_order<short, double>, bound
+ * by core/extrapolate_using_least_squares_bind.cpp:11-41).  Flags are int16
+ * (n0, n1); grid_x has n1 entries and grid_y n0.  Sweeps until no cell is
+ * extrapolated or max_sweeps is reached; sweeps_done_host (may be NULL)
+ * receives the count.  Synchronizes `stream` (the sweep count is data
+ * dependent).  Out-of-range window cells read as unknown (flag 0). */
+axs_status axs_extrapolate_lsq_first_order_f64(int16_t* current_flag,
+                                               const int16_t* target_flag, double* eta_x,
+                                               double* eta_y, const double* grid_x,
+                                               const double* grid_y, int n0, int n1,
+                                               int max_sweeps, int* sweeps_done_host,
+                                               void* stream);
+
+/* ---- A16: the rigid-body driver step, device resident.
+ * examples/FlowPastSphere/flow_past_sphere.py:107-188 (static body, free
+ * stream with the sine ramp, drag coefficient Cd) and the oscillating
+ * penalisation velocity of examples/SpheroidStreaming/spheroid_streaming.py:216-226.
+ * One step: kill taper -> Stokes solve -> velocity (+ free stream) -> dt ->
+ * Brinkman + curl -> Cd -> ENO3 -> RK2 diffusion, with t, dt and the Cd
+ * history kept on the device (slot layout: t, dt, dt/dx, lam dt, nu dt/2,
+ * nu dt, ramp factor, U_pen, max|u|, drag sum, step counter).
+ * The caller owns every buffer; all are (nr, nz) FP64 fields except
+ * scalars (16 doubles), drag (history doubles or NULL), r (nr), kill_wz
+ * (2*kill_width weights: left then right block) and kill_wr (kill_width). */
+typedef struct axs_rigid_stepper axs_rigid_stepper;
+typedef struct {
+  int nr, nz, kill_width, history;
+  double dx, nu, brink_lam;
+  double U0;           /* free stream added to u_z each step */
+  double T_ramp;       /* ramp sin(pi t / (2 T_ramp)) while t < T_ramp; <= 0: none */
+  double CFL, eps;     /* dt = min(dt_diffusive, CFL dx / (max|u| + eps)) */
+  double dt_diffusive; /* 0.9 dx^2 / 4 / nu as the driver forms it */
+  double osc_amp, osc_omega; /* penalisation velocity U_z(t) = osc_amp cos(osc_omega t) */
+  double drag_scale, drag_div; /* Cd = drag_scale * sum(r chi u_z) / drag_div */
+} axs_rigid_config;
+typedef struct {
+  double *vorticity, *psi, *u_z, *u_r, *u_pen_z, *u_pen_r, *char_func, *work0, *work1;
+  double* scalars;
+  double* drag;
+  const double* r;
+  const double* kill_wz;
+  const double* kill_wr;
+} axs_rigid_buffers;
+axs_status axs_rigid_stepper_create(const axs_rigid_config* cfg, const axs_rigid_buffers* bufs,
+                                    axs_rigid_stepper** out);
+axs_status axs_rigid_stepper_destroy(axs_rigid_stepper* st);
+/* Queues nsteps steps on `stream`; use_graph != 0 captures one step into a
+ * hipGraph on first use and replays it (stream must not be the null stream). */
+axs_status axs_rigid_stepper_run(axs_rigid_stepper* st, int nsteps, int use_graph, void* stream);
+/* Profiling hook: the step is a fixed sequence of stages (kill taper,
+ * dct2 rows, tridiagonal columns, dct3 rows, velocity, dt, penalisation,
+ * step scalars, ENO3, RK2).  run_stage queues `reps` launches of one stage
+ * alone, so a timer on `stream` measures that stage.  It advances no state
+ * consistently: use it only on a stepper whose results are discarded. */
+int axs_rigid_stepper_stage_count(void);
+const char* axs_rigid_stepper_stage_name(int stage);
+axs_status axs_rigid_stepper_run_stage(axs_rigid_stepper* st, int stage, int reps, void* stream);
+/* Recomputes the ramp factor and U_pen for the time now in scalars[0]. */
+axs_status axs_rigid_stepper_reset_scalars(axs_rigid_stepper* st, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AXISYM_H_ */

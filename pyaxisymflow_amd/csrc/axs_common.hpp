@@ -1,0 +1,68 @@
+// Shared definitions for the axisymmetric vorticity-streamfunction kernels.
+// Target: gfx950 (CDNA4, 64-wide wavefronts).  Fields are row-major (nr, nz):
+// row i is radial index r_i, column j is axial index z_j (contiguous).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/axisym.h"
+
+namespace axs {
+
+// Thread-local last-error string reported by axs_last_error().
+void set_error(const std::string& msg);
+void clear_error();
+
+#define AXS_CHECK_ARG(cond, msg)            \
+  do {                                      \
+    if (!(cond)) {                          \
+      ::axs::set_error(std::string(msg));   \
+      return AXS_ERR_INVALID;               \
+    }                                       \
+  } while (0)
+
+#define AXS_HIP(...)                                                         \
+  do {                                                                       \
+    hipError_t _e = (__VA_ARGS__);                                           \
+    if (_e != hipSuccess) {                                                  \
+      ::axs::set_error(std::string(#__VA_ARGS__) + ": " + hipGetErrorString(_e)); \
+      return AXS_ERR_HIP;                                                    \
+    }                                                                        \
+  } while (0)
+
+// Checks the launch just issued; returns from the calling C-ABI function.
+#define AXS_LAUNCH_CHECK(name) AXS_HIP(hipGetLastError())
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline unsigned ceil_div(int64_t a, int64_t b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+// A scalar that is either a host value or read from device memory (so a
+// whole timestep can run with dt resident on the device).
+template <typename T>
+struct Scalar {
+  T host;
+  const double* dev;
+  __device__ __forceinline__ T get() const { return dev ? static_cast<T>(*dev) : host; }
+};
+
+// Radial coordinate: either a full (nr, nz) array as the reference passes R,
+// or one value per row (the grid is row-constant in r).
+template <typename T>
+struct RFull {
+  const T* R;
+  int nz;
+  __device__ __forceinline__ T at(int i, int j) const { return R[(int64_t)i * nz + j]; }
+};
+template <typename T>
+struct RRow {
+  const double* r;
+  __device__ __forceinline__ T at(int i, int) const { return static_cast<T>(r[i]); }
+};
+
+}  // namespace axs

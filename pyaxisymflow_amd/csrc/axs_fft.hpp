@@ -1,0 +1,208 @@
+// In-LDS complex FFT (Stockham auto-sort, radices 4/2/3/5) and the real
+// DCT-II / DCT-III row transforms built on it.
+//
+// DCT-II of a real row x[0..N) (N even), X_k = sum_n x_n cos(pi k (2n+1) / (2N)):
+//   v_n = x_{2n}, v_{N-1-n} = x_{2n+1}                       (reorder)
+//   z_m = v_{2m} + i v_{2m+1}, Z = FFT_M(z), M = N/2           (half-length FFT)
+//   V_k = (Z_k + Z*_{M-k})/2 - i e^{-2 pi i k/N} (Z_k - Z*_{M-k})/2
+//   X_k = Re(e^{-i pi k/(2N)} V_k),  X_{N-k} = -Im(e^{-i pi k/(2N)} V_k)
+// The inverse runs the same steps backwards and divides by N, so
+// inverse(forward(x)) == x.
+#pragma once
+
+#include "axs_common.hpp"
+
+namespace axs {
+
+constexpr int kMaxFftStages = 24;
+
+struct FftPlanArgs {
+  int M;                       // complex FFT length (N/2)
+  int nstages;
+  int radix[kMaxFftStages];
+};
+
+template <typename T>
+struct Cplx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ Cplx<T> cmul(Cplx<T> a, Cplx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ Cplx<T> cadd(Cplx<T> a, Cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ Cplx<T> csub(Cplx<T> a, Cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ Cplx<T> cconj(Cplx<T> a) { return {a.x, -a.y}; }
+
+// In-place DFT of size R on a[0..R), exponent sign `sgn` (-1 forward).
+template <typename T, int R>
+__device__ __forceinline__ void dft_small(Cplx<T>* a, int sgn) {
+  if constexpr (R == 2) {
+    Cplx<T> t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  } else if constexpr (R == 4) {
+    Cplx<T> t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+    Cplx<T> t2 = cadd(a[1], a[3]), t3 = csub(a[1], a[3]);
+    // w * t3 with w = sgn * i
+    Cplx<T> wt3 = {-T(sgn) * t3.y, T(sgn) * t3.x};
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, wt3);
+    a[3] = csub(t1, wt3);
+  } else if constexpr (R == 3) {
+    const T s3 = T(0.86602540378443864676) * T(sgn);
+    Cplx<T> t = cadd(a[1], a[2]), d = csub(a[1], a[2]);
+    Cplx<T> m = {a[0].x - T(0.5) * t.x, a[0].y - T(0.5) * t.y};
+    Cplx<T> jd = {-s3 * d.y, s3 * d.x};
+    a[0] = cadd(a[0], t);
+    a[1] = cadd(m, jd);
+    a[2] = csub(m, jd);
+  } else {  // R == 5: direct
+    const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+    const T s1 = T(0.95105651629515357212) * T(sgn), s2 = T(0.58778525229247312917) * T(sgn);
+    Cplx<T> b1 = cadd(a[1], a[4]), d1 = csub(a[1], a[4]);
+    Cplx<T> b2 = cadd(a[2], a[3]), d2 = csub(a[2], a[3]);
+    Cplx<T> r1 = {a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y};
+    Cplx<T> r2 = {a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y};
+    // i * (s1 d1 + s2 d2) and i * (s2 d1 - s1 d2)
+    Cplx<T> i1 = {-(s1 * d1.y + s2 * d2.y), s1 * d1.x + s2 * d2.x};
+    Cplx<T> i2 = {-(s2 * d1.y - s1 * d2.y), s2 * d1.x - s1 * d2.x};
+    a[0] = {a[0].x + b1.x + b2.x, a[0].y + b1.y + b2.y};
+    a[1] = cadd(r1, i1);
+    a[4] = csub(r1, i1);
+    a[2] = cadd(r2, i2);
+    a[3] = csub(r2, i2);
+  }
+}
+
+// One Stockham stage of radix R over buf[0..M) in LDS (in place: all reads
+// go to registers before a barrier, then all writes).  Each thread handles
+// PER butterflies (compile-time, so the staging array stays in registers).
+// tw[t] = e^{-2 pi i t/M}; the inverse (sgn = +1) uses its conjugate.
+template <typename T, int R, int PER>
+__device__ __forceinline__ void stockham_stage(Cplx<T>* buf, int M, int Ns,
+                                               const Cplx<double>* __restrict__ tw, int sgn) {
+  const int nb = M / R;
+  const int tstride = M / (Ns * R);
+  Cplx<T> v[PER][R];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int j = threadIdx.x + q * blockDim.x;
+    if (j < nb) {
+      const int jm = j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        Cplx<T> a = buf[j + r * nb];
+        if (r > 0 && jm > 0) {
+          Cplx<double> w = tw[jm * r * tstride];
+          Cplx<T> wt = {T(w.x), T(sgn < 0 ? w.y : -w.y)};
+          a = cmul(a, wt);
+        }
+        v[q][r] = a;
+      }
+      dft_small<T, R>(v[q], sgn);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int j = threadIdx.x + q * blockDim.x;
+    if (j < nb) {
+      const int jm = j % Ns;
+      const int idxD = (j / Ns) * Ns * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[idxD + r * Ns] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Full FFT of length p.M in LDS.  Requires p.M / min(radix) <= PER * blockDim.
+template <typename T, int PER>
+__device__ void fft_lds(Cplx<T>* buf, const FftPlanArgs& p, const Cplx<double>* tw, int sgn) {
+  int Ns = 1;
+  for (int s = 0; s < p.nstages; ++s) {
+    const int R = p.radix[s];
+    if (R == 4)
+      stockham_stage<T, 4, PER>(buf, p.M, Ns, tw, sgn);
+    else if (R == 2)
+      stockham_stage<T, 2, PER>(buf, p.M, Ns, tw, sgn);
+    else if (R == 3)
+      stockham_stage<T, 3, PER>(buf, p.M, Ns, tw, sgn);
+    else
+      stockham_stage<T, 5, PER>(buf, p.M, Ns, tw, sgn);
+    Ns *= R;
+  }
+}
+
+// Column of x that feeds v_n (the DCT reorder): v_n = x_{2n} for n < N/2,
+// else x_{2(N-1-n)+1}.
+__device__ __forceinline__ int dct_src_index(int n, int N) {
+  return (n < N / 2) ? 2 * n : 2 * (N - 1 - n) + 1;
+}
+
+// Forward DCT-II post-processing from Z (in LDS, natural order) to X (out).
+// e1[k] = e^{-2 pi i k/N}, e2[k] = e^{-i pi k/(2N)}, k in [0, M].
+template <typename T, typename Out>
+__device__ void dct2_post(const Cplx<T>* Z, int M, const Cplx<double>* __restrict__ e1,
+                          const Cplx<double>* __restrict__ e2, Out out) {
+  const int N = 2 * M;
+  for (int k = threadIdx.x; k <= M / 2; k += blockDim.x) {
+    const int kp = M - k;  // partner
+    Cplx<T> zk = Z[k % M], zp = Z[kp % M];
+    // V_k and V_kp
+    auto makeV = [&](Cplx<T> za, Cplx<T> zb, int kk) {
+      Cplx<T> zbc = cconj(zb);
+      Cplx<T> s = cadd(za, zbc), d = csub(za, zbc);
+      Cplx<double> w = e1[kk];
+      Cplx<T> wd = cmul(Cplx<T>{T(w.x), T(w.y)}, d);
+      // V = s/2 - i * wd / 2
+      return Cplx<T>{T(0.5) * (s.x + wd.y), T(0.5) * (s.y - wd.x)};
+    };
+    Cplx<T> Vk = makeV(zk, zp, k);
+    Cplx<double> q = e2[k];
+    Cplx<T> wk = cmul(Cplx<T>{T(q.x), T(q.y)}, Vk);
+    out(k, wk.x);
+    if (k > 0) out(N - k, -wk.y);
+    if (kp != k) {
+      Cplx<T> Vp = makeV(zp, zk, kp);
+      Cplx<double> qp = e2[kp];
+      Cplx<T> wp = cmul(Cplx<T>{T(qp.x), T(qp.y)}, Vp);
+      out(kp, wp.x);
+      if (kp != M) out(N - kp, -wp.y);  // for kp == M, N-kp == M is already written
+    }
+  }
+}
+
+// Inverse pre-processing: Z_k = A_k + i B_k from X (read via `in`), k in [0, M).
+//   V_k = e^{+i pi k/(2N)} (X_k - i X_{N-k}),  V_{k+M} = conj(V_{M-k})
+//   A_k = V_k + V_{k+M},  B_k = (V_k - V_{k+M}) e^{+2 pi i k/N}
+template <typename T, typename In>
+__device__ void dct3_pre(Cplx<T>* Z, int M, const Cplx<double>* __restrict__ e1,
+                         const Cplx<double>* __restrict__ e2, In in) {
+  const int N = 2 * M;
+  for (int k = threadIdx.x; k < M; k += blockDim.x) {
+    const T xk = in(k);
+    const T xnk = (k == 0) ? T(0) : in(N - k);
+    Cplx<double> q = e2[k];
+    Cplx<T> Vk = cmul(Cplx<T>{T(q.x), T(-q.y)}, Cplx<T>{xk, -xnk});
+    const int mk = M - k;  // in [1, M]
+    const T xm = in(mk);
+    const T xnm = in(N - mk);  // N - mk in [M, N)
+    Cplx<double> qm = e2[mk];
+    Cplx<T> Vm = cmul(Cplx<T>{T(qm.x), T(-qm.y)}, Cplx<T>{xm, -xnm});
+    Cplx<T> Vkm = cconj(Vm);  // V_{k+M}
+    Cplx<T> A = cadd(Vk, Vkm);
+    Cplx<T> D = csub(Vk, Vkm);
+    Cplx<double> w = e1[k];
+    Cplx<T> B = cmul(D, Cplx<T>{T(w.x), T(-w.y)});
+    Z[k] = Cplx<T>{A.x - B.y, A.y + B.x};
+  }
+}
+
+}  // namespace axs

@@ -1,0 +1,227 @@
+// Least-squares extrapolation of the reference map (row A14):
+// core/src/extrapolate_using_least_squares.hpp:142-446 as instantiated by
+// extrapolate_using_least_squares_till_first_order<short, double> (:450-467).
+//
+// Each sweep of the reference scans every cell, collects the cells with
+// current ^ target set and at least one known 3x3 neighbour, fits
+// eta = c0 + c1 x + c2 y on a shifted 3x3 window of known cells and only
+// then marks the collected cells known (:441-444).  A sweep is therefore
+// data-parallel: here one thread per cell fits into side buffers, and a
+// second pass commits values and flags.  The arithmetic follows the
+// reference's order with FMA contraction off, so results match it bit for
+// bit.  Out-of-range cells read as flag 0 (the reference has no bounds
+// checks there, :205-219, 276-280).
+#include "axs_lsq.hpp"
+
+#include <vector>
+
+namespace axs {
+
+__device__ __forceinline__ short lsq_flag(const int16_t* f, int n0, int n1, int j, int i) {
+  return (j < 0 || j >= n0 || i < 0 || i >= n1) ? short(0) : short(f[(int64_t)j * n1 + i]);
+}
+
+// ExtrapolationConfig.hpp:17-19, 131-158 (grid size 3: start = code - 3).
+__device__ __forceinline__ short lsq_start_code(short positional_sum, short total_sum) {
+  float temp = (positional_sum > short(4)) ? float(ceil(0.5 * float(positional_sum)) * 2.0)
+                                           : float(positional_sum);
+  short code = short(2.0 * double(temp) / double(total_sum));
+  return short(short(2 * code - short(6)) / short(2));
+}
+
+// least_squares.hpp:10-59 for 3 coefficients and 2 right-hand sides.
+__device__ void lsq_gauss_3x2(double A[3][5], double sol[2][3]) {
+  for (int i = 0; i < 3; i++) {
+    int max_row = i;
+    double max_element = fabs(A[i][i]);
+    for (int k = i + 1; k < 3; k++) {
+      if (fabs(A[k][i]) > max_element) {
+        max_element = fabs(A[k][i]);
+        max_row = k;
+      }
+    }
+    for (int k = i; k < 5; k++) {
+      double tmp = A[max_row][k];
+      A[max_row][k] = A[i][k];
+      A[i][k] = tmp;
+    }
+    for (int k = i + 1; k < 3; k++) {
+      double c = -A[k][i] / A[i][i];
+      A[k][i] = 0.0;
+      for (int j = i + 1; j < 5; j++) A[k][j] += c * A[i][j];
+    }
+  }
+  for (int comp = 0; comp < 2; ++comp) {
+    for (int i = 2; i >= 0; --i) {
+      sol[comp][i] = A[i][3 + comp] / A[i][i];
+      for (int k = i - 1; k >= 0; --k) A[k][3 + comp] -= A[k][i] * sol[comp][i];
+    }
+  }
+}
+
+// One sweep, phase 1: select + fit.  pend[c] = 1 when cell c is extrapolated
+// this sweep (new values in nx/ny), else 0.
+__global__ void lsq_fit_kernel(const int16_t* __restrict__ cur, const int16_t* __restrict__ tgt,
+                               const double* __restrict__ ex, const double* __restrict__ ey,
+                               const double* __restrict__ gx, const double* __restrict__ gy,
+                               int n0, int n1, double* __restrict__ nx, double* __restrict__ ny,
+                               int8_t* __restrict__ pend, unsigned* __restrict__ counter,
+                               const unsigned* __restrict__ prev_counter) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // column (x)
+  const int j = blockIdx.y;                              // row (y)
+  if (i >= n1) return;
+  const int64_t c = (int64_t)j * n1 + i;
+  pend[c] = 0;
+  if (prev_counter && *prev_counter == 0u) return;  // previous sweep found nothing
+  if (!(cur[c] ^ tgt[c])) return;
+  short local_sum = 0, lx = 0, ly = 0;
+  for (int jl = -1; jl <= 1; ++jl)
+    for (int il = -1; il <= 1; ++il) {
+      const short f = lsq_flag(cur, n0, n1, j + jl, i + il);
+      local_sum += (jl == 0 && il == 0) ? short(0) : f;
+      lx += f * short(il + 1);
+      ly += f * short(jl + 1);
+    }
+  if (!local_sum) return;
+  const int s_x = i + lsq_start_code(lx, local_sum);
+  const int s_y = j + lsq_start_code(ly, local_sum);
+  double L0[9], L1[9], L2[9], r0[9], r1[9];
+  int n = 0;
+  for (int jl = s_y; jl < s_y + 3; ++jl) {
+    for (int il = s_x; il < s_x + 3; ++il, ++n) {
+      const bool in = (jl >= 0 && jl < n0 && il >= 0 && il < n1);
+      const double m = in ? double(cur[(int64_t)jl * n1 + il]) : 0.0;
+      const double xv = in ? gx[il] : 0.0;
+      const double yv = in ? gy[jl] : 0.0;
+      L0[n] = m;
+      L1[n] = m * xv;
+      L2[n] = m * yv;
+      const int64_t g = (int64_t)jl * n1 + il;
+      r0[n] = m * (in ? ex[g] : 0.0);
+      r1[n] = m * (in ? ey[g] : 0.0);
+    }
+  }
+  const double* Ls[3] = {L0, L1, L2};
+  double B[3][5];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double s = 0.0;
+      for (int q = 0; q < 9; ++q) s += Ls[a][q] * Ls[b][q];
+      B[a][b] = s;
+    }
+  for (int a = 0; a < 3; ++a) {
+    double s = 0.0;
+    for (int q = 0; q < 9; ++q) s += Ls[a][q] * r0[q];
+    B[a][3] = s;
+    s = 0.0;
+    for (int q = 0; q < 9; ++q) s += Ls[a][q] * r1[q];
+    B[a][4] = s;
+  }
+  double sol[2][3];
+  lsq_gauss_3x2(B, sol);
+  const double px = gx[i], py = gy[j];
+  const double E[3] = {1.0, 1.0 * px, 1.0 * py};
+  double e = 0.0;
+  for (int q = 0; q < 3; ++q) e += sol[0][q] * E[q];
+  nx[c] = e;
+  e = 0.0;
+  for (int q = 0; q < 3; ++q) e += sol[1][q] * E[q];
+  ny[c] = e;
+  pend[c] = 1;
+  atomicAdd(counter, 1u);
+}
+
+// Phase 2: commit values and flags of the cells fitted in phase 1.
+__global__ void lsq_commit_kernel(int16_t* __restrict__ cur, double* __restrict__ ex,
+                                  double* __restrict__ ey, const double* __restrict__ nx,
+                                  const double* __restrict__ ny, const int8_t* __restrict__ pend,
+                                  int64_t n) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n || !pend[c]) return;
+  ex[c] = nx[c];
+  ey[c] = ny[c];
+  cur[c] = 1;
+}
+
+__global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+hipError_t launch_lsq_sweep(int16_t* cur, const int16_t* tgt, double* ex, double* ey,
+                            const double* gx, const double* gy, int n0, int n1, LsqWork w,
+                            int sweep, hipStream_t s) {
+  dim3 grid(ceil_div(n1, 256), n0);
+  lsq_fit_kernel<<<grid, 256, 0, s>>>(cur, tgt, ex, ey, gx, gy, n0, n1, w.nx, w.ny, w.pend,
+                                      w.counters + sweep,
+                                      sweep > 0 ? w.counters + sweep - 1 : nullptr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t n = (int64_t)n0 * n1;
+  lsq_commit_kernel<<<ceil_div(n, 256), 256, 0, s>>>(cur, ex, ey, w.nx, w.ny, w.pend, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s) {
+  fill_u32_kernel<<<ceil_div(n, 256), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace axs
+
+using namespace axs;
+
+extern "C" {
+
+axs_status axs_extrapolate_lsq_first_order_f64(int16_t* current_flag, const int16_t* target_flag,
+                                               double* eta_x, double* eta_y,
+                                               const double* grid_x, const double* grid_y,
+                                               int n0, int n1, int max_sweeps,
+                                               int* sweeps_done_host, void* stream) {
+  AXS_CHECK_ARG(current_flag && target_flag && eta_x && eta_y && grid_x && grid_y,
+                "null pointer");
+  AXS_CHECK_ARG(n0 >= 1 && n1 >= 1, "bad shape");
+  AXS_CHECK_ARG(max_sweeps >= 1 && max_sweeps <= 1 << 20, "bad max_sweeps");
+  hipStream_t s = as_stream(stream);
+  const int64_t n = (int64_t)n0 * n1;
+  // Sweeps are issued in batches; a sweep whose predecessor found nothing
+  // exits at once, so the host only reads the counters once per batch.
+  constexpr int kBatch = 8;
+  LsqWork w{};
+  AXS_HIP(hipMalloc(reinterpret_cast<void**>(&w.nx), sizeof(double) * n));
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&w.ny), sizeof(double) * n);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.pend), n);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.counters), sizeof(unsigned) * kBatch);
+  int done = 0;
+  bool finished = false;
+  std::vector<unsigned> h(kBatch);
+  while (e == hipSuccess && !finished && done < max_sweeps) {
+    e = launch_fill_u32(w.counters, kBatch, 0u, s);
+    const int nb = std::min(kBatch, max_sweeps - done);
+    for (int q = 0; q < nb && e == hipSuccess; ++q)
+      e = launch_lsq_sweep(current_flag, target_flag, eta_x, eta_y, grid_x, grid_y, n0, n1, w,
+                           q, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h.data(), w.counters, sizeof(unsigned) * kBatch, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (int q = 0; q < nb && e == hipSuccess; ++q) {
+      if (h[q] == 0) {
+        finished = true;
+        break;
+      }
+      ++done;
+    }
+  }
+  (void)hipFree(w.nx);
+  (void)hipFree(w.ny);
+  (void)hipFree(w.pend);
+  (void)hipFree(w.counters);
+  if (e != hipSuccess) {
+    set_error(std::string("lsq extrapolation: ") + hipGetErrorString(e));
+    return AXS_ERR_HIP;
+  }
+  if (sweeps_done_host) *sweeps_done_host = done;
+  return AXS_OK;
+}
+
+}  // extern "C"

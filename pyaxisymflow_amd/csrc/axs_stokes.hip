@@ -1,0 +1,429 @@
+// Stream-function Poisson solve of FastDiagonalisationStokesSolver
+// (kernels/FastDiagonalisationStokesSolver.py:6-156), restated as
+//   1. row pass:    what = DCT-II_z(r * rhs)          (one workgroup per row)
+//   2. column pass: (A_r + lambda_k I) psihat_k = what_k for every z-mode k
+//   3. row pass:    psi = DCT-II^{-1}_z(psihat)
+// A_r = P_r - D_r is the reference's radial matrix (tridiagonal, built with
+// the same floating-point operations as :41-86); P_z (Neumann) has the
+// DCT-II eigenvectors cos(pi k (2n+1)/(2 nz)) and eigenvalues
+// lambda_k = (4/dx^2) sin^2(pi k / (2 nz)).
+// The transforms are a different algorithm from the reference's dense
+// route, so FMA contraction is allowed here (the library default is off).
+#pragma clang fp contract(fast)
+
+#include "axs_stokes.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace axs {
+
+// ------------------------------------------------------------------ host plan
+
+static bool factor_radices(int M, FftPlanArgs* p) {
+  p->M = M;
+  p->nstages = 0;
+  int m = M;
+  for (int r : {4, 2, 3, 5}) {
+    while (m % r == 0) {
+      if (p->nstages >= kMaxFftStages) return false;
+      p->radix[p->nstages++] = r;
+      m /= r;
+    }
+  }
+  return m == 1;
+}
+
+template <typename V>
+static hipError_t upload(V** dev, const std::vector<V>& host) {
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(dev), sizeof(V) * std::max<size_t>(host.size(), 1));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*dev, host.data(), sizeof(V) * host.size(), hipMemcpyHostToDevice);
+}
+
+StokesPlan::~StokesPlan() {
+  for (void* p : {(void*)tw, (void*)e1, (void*)e2, (void*)a, (void*)b, (void*)c,
+                  (void*)lam, (void*)rvec, (void*)cseg, (void*)spec})
+    if (p) (void)hipFree(p);
+}
+
+int StokesPlan::tri_threads() const { return tri_K * tri_P; }
+
+hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
+  nr = nr_;
+  nz = nz_;
+  dx = dx_;
+  const int M = nz / 2;
+  // FFT twiddles e^{-2 pi i t/M}
+  std::vector<Cplx<double>> h_tw(M), h_e1(M + 1), h_e2(M + 1);
+  for (int t = 0; t < M; ++t) {
+    const double ang = -2.0 * M_PI * (double)t / (double)M;
+    h_tw[t] = {std::cos(ang), std::sin(ang)};
+  }
+  for (int k = 0; k <= M; ++k) {
+    const double a1 = -2.0 * M_PI * (double)k / (double)nz;
+    const double a2 = -M_PI * (double)k / (2.0 * nz);
+    h_e1[k] = {std::cos(a1), std::sin(a1)};
+    h_e2[k] = {std::cos(a2), std::sin(a2)};
+  }
+  // Radial coordinate, FastDiagonalisationStokesSolver.py:20-24:
+  // numpy.linspace(dx/2, nr*dx - dx/2, nr) = start + i*step, last = stop.
+  std::vector<double> h_r(nr);
+  const double start = dx / 2, stop = nr * dx - dx / 2;
+  const double step = (nr > 1) ? (stop - start) / (nr - 1) : 0.0;
+  for (int i = 0; i < nr; ++i) h_r[i] = i * step + start;
+  if (nr > 1) h_r[nr - 1] = stop;
+  // A_r = P_r - D_r (:45-65, Neumann BCs :79-86), entries as the reference forms them.
+  const double inv_dx2 = 1.0 / dx / dx;
+  const double inv_2dx = 1.0 / 2.0 / dx;
+  std::vector<double> h_a(nr, 0.0), h_b(nr), h_c(nr, 0.0);
+  for (int i = 0; i < nr; ++i) {
+    const double P_sub = inv_dx2 * -1.0, P_sup = inv_dx2 * -1.0;
+    const double D_sub = (inv_2dx * 1.0) / h_r[i], D_sup = (inv_2dx * -1.0) / h_r[i];
+    h_b[i] = inv_dx2 * 2.0;
+    if (i > 0) h_a[i] = P_sub - D_sub;
+    if (i < nr - 1) h_c[i] = P_sup - D_sup;
+  }
+  h_b[nr - 1] = inv_dx2;                 // P_r[-1,-1] = inv_dx2
+  if (nr > 1) h_a[nr - 1] = inv_dx2 * -1.0 - 0.0;  // D_r[-1,-2] = 0
+  std::vector<double> h_lam(nz);
+  for (int k = 0; k < nz; ++k) {
+    const double s = std::sin(M_PI * (double)k / (2.0 * nz));
+    h_lam[k] = (4.0 / (dx * dx)) * s * s;
+  }
+  // Segment layout of the column pass and c'_{s-1} at every segment start.
+  tri_L = (nr <= 256) ? 8 : 16;
+  tri_P = (nr + tri_L - 1) / tri_L;
+  while (tri_P > 128) {  // keep the workgroup within 1024 threads
+    tri_L *= 2;
+    tri_P = (nr + tri_L - 1) / tri_L;
+  }
+  tri_K = 1;
+  while (tri_K * 2 * tri_P <= 1024 && tri_K < 64) tri_K *= 2;
+  std::vector<double> h_cseg((size_t)tri_P * nz, 0.0);
+  for (int k = 0; k < nz; ++k) {
+    double cp = 0.0;
+    for (int i = 0; i < nr; ++i) {
+      if (i % tri_L == 0) h_cseg[(size_t)(i / tri_L) * nz + k] = cp;
+      const double m = (h_b[i] + h_lam[k]) - h_a[i] * cp;
+      cp = h_c[i] / m;
+    }
+  }
+  hipError_t e;
+  if ((e = upload(&tw, h_tw)) != hipSuccess) return e;
+  if ((e = upload(&e1, h_e1)) != hipSuccess) return e;
+  if ((e = upload(&e2, h_e2)) != hipSuccess) return e;
+  if ((e = upload(&a, h_a)) != hipSuccess) return e;
+  if ((e = upload(&b, h_b)) != hipSuccess) return e;
+  if ((e = upload(&c, h_c)) != hipSuccess) return e;
+  if ((e = upload(&lam, h_lam)) != hipSuccess) return e;
+  if ((e = upload(&rvec, h_r)) != hipSuccess) return e;
+  if ((e = upload(&cseg, h_cseg)) != hipSuccess) return e;
+  e = hipMalloc(reinterpret_cast<void**>(&spec), sizeof(double) * (size_t)nr * nz);
+  return e;
+}
+
+// ------------------------------------------------------------------ kernels
+
+// Row pass: x_j = scale_i * src[i, j]  ->  X = DCT-II(x)  ->  dst[i, :].
+template <typename T, int PER>
+__global__ void __launch_bounds__(256) dct2_rows_kernel(
+    const T* __restrict__ src, T* __restrict__ dst, const double* __restrict__ rscale,
+    int nz, FftPlanArgs fp, const Cplx<double>* __restrict__ tw,
+    const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2) {
+  extern __shared__ unsigned char smem_raw[];
+  Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
+  const int i = blockIdx.x;
+  const int M = fp.M;
+  const T* row = src + (int64_t)i * nz;
+  const T s = rscale ? T(rscale[i]) : T(1);
+  for (int m = threadIdx.x; m < M; m += blockDim.x) {
+    buf[m] = {s * row[dct_src_index(2 * m, nz)], s * row[dct_src_index(2 * m + 1, nz)]};
+  }
+  __syncthreads();
+  fft_lds<T, PER>(buf, fp, tw, -1);
+  T* out = dst + (int64_t)i * nz;
+  dct2_post<T>(buf, M, e1, e2, [&](int k, T v) { out[k] = v; });
+}
+
+// Row pass: x = DCT-II^{-1}(src[i, :]) -> dst[i, :].
+template <typename T, int PER>
+__global__ void __launch_bounds__(256) dct3_rows_kernel(
+    const T* __restrict__ src, T* __restrict__ dst, int nz, FftPlanArgs fp,
+    const Cplx<double>* __restrict__ tw, const Cplx<double>* __restrict__ e1,
+    const Cplx<double>* __restrict__ e2) {
+  extern __shared__ unsigned char smem_raw[];
+  Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
+  const int i = blockIdx.x;
+  const int M = fp.M;
+  const T* row = src + (int64_t)i * nz;
+  dct3_pre<T>(buf, M, e1, e2, [&](int k) { return row[k]; });
+  __syncthreads();
+  fft_lds<T, PER>(buf, fp, tw, +1);
+  const T invN = T(1) / T(nz);
+  T* out = dst + (int64_t)i * nz;
+  for (int m = threadIdx.x; m < M; m += blockDim.x) {
+    const Cplx<T> z = buf[m];
+    out[dct_src_index(2 * m, nz)] = z.x * invN;
+    out[dct_src_index(2 * m + 1, nz)] = z.y * invN;
+  }
+}
+
+// Column pass: for each z-mode k solve the tridiagonal system
+//   a_i x_{i-1} + (b_i + lam_k) x_i + c_i x_{i+1} = d_i,   i in [0, nr)
+// in place on d (row-major (nr, nz), k contiguous).  Workgroup = K modes x
+// P radial segments of L rows; thread (kk, p) owns rows [pL, pL+L) of mode
+// k in registers.  Thomas' elimination is split into segment-local
+// recurrences joined by prefix scans of affine maps across segments; the
+// data-independent pivot recurrence starts each segment from c'_{pL-1},
+// tabulated once per plan.
+template <typename T, int L>
+__global__ void __launch_bounds__(1024) tridiag_columns_kernel(
+    T* __restrict__ d, const double* __restrict__ a, const double* __restrict__ b,
+    const double* __restrict__ c, const double* __restrict__ lam,
+    const double* __restrict__ cseg, int nr, int nz, int K, int P) {
+  extern __shared__ double sm[];
+  // two ping-pong buffers of (G, y) pairs, each K*P entries
+  double* sG0 = sm;
+  double* sY0 = sm + K * P;
+  double* sG1 = sm + 2 * K * P;
+  double* sY1 = sm + 3 * K * P;
+  const int kk = threadIdx.x % K;
+  const int p = threadIdx.x / K;
+  const int k = blockIdx.x * K + kk;
+  const bool active = (k < nz) && (p < P);
+  const int s = p * L;
+  const int kc = active ? k : 0;
+
+  double x[L];
+  double im[L];
+  // -------- forward: load, local elimination with zero carry-in
+  double G = 1.0, y = 0.0;
+  {
+    double cp = (p == 0 || !active) ? 0.0 : cseg[(size_t)p * nz + kc];
+    const double lk = lam[kc];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const int i = s + l;
+      if (active && i < nr) {
+        const double ai = a[i];
+        const double m = (b[i] + lk) - ai * cp;
+        const double inv = 1.0 / m;
+        im[l] = inv;
+        cp = c[i] * inv;
+        const double di = static_cast<double>(d[(int64_t)i * nz + kc]);
+        y = (di - ai * y) * inv;
+        G = G * (-ai * inv);
+        x[l] = y;
+      } else {
+        im[l] = 0.0;
+        x[l] = 0.0;
+      }
+    }
+  }
+  // inclusive scan over segments of the affine maps D -> G*D + y
+  const int me = p * K + kk;
+  sG0[me] = G;
+  sY0[me] = y;
+  __syncthreads();
+  double* inG = sG0; double* inY = sY0; double* outG = sG1; double* outY = sY1;
+  for (int off = 1; off < P; off <<= 1) {
+    double g = inG[me], yy = inY[me];
+    if (p >= off) {
+      const int o = (p - off) * K + kk;
+      // compose: this o (earlier) then me: D -> g*(G_o D + y_o) + yy
+      yy = g * inY[o] + yy;
+      g = g * inG[o];
+    }
+    outG[me] = g;
+    outY[me] = yy;
+    __syncthreads();
+    double* t;
+    t = inG; inG = outG; outG = t;
+    t = inY; inY = outY; outY = t;
+  }
+  const double Din = (p == 0) ? 0.0 : inY[(p - 1) * K + kk];
+  __syncthreads();
+  // fix-up: d'_i = y_i + G_i * Din
+  {
+    double g = 1.0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const int i = s + l;
+      if (active && i < nr) {
+        g = g * (-a[i] * im[l]);
+        x[l] = x[l] + g * Din;
+      }
+    }
+  }
+  // -------- backward: local substitution with zero carry-in from above
+  double H = 1.0, z = 0.0;
+#pragma unroll
+  for (int l = L - 1; l >= 0; --l) {
+    const int i = s + l;
+    if (active && i < nr) {
+      const double cpi = c[i] * im[l];
+      z = x[l] - cpi * z;
+      H = H * (-cpi);
+      x[l] = z;
+    }
+  }
+  // suffix scan over segments (later segment feeds the earlier one)
+  sG0[me] = H;
+  sY0[me] = z;
+  __syncthreads();
+  inG = sG0; inY = sY0; outG = sG1; outY = sY1;
+  for (int off = 1; off < P; off <<= 1) {
+    double h = inG[me], zz = inY[me];
+    if (p + off < P) {
+      const int o = (p + off) * K + kk;
+      zz = h * inY[o] + zz;
+      h = h * inG[o];
+    }
+    outG[me] = h;
+    outY[me] = zz;
+    __syncthreads();
+    double* t;
+    t = inG; inG = outG; outG = t;
+    t = inY; inY = outY; outY = t;
+  }
+  const double Xin = (p == P - 1) ? 0.0 : inY[(p + 1) * K + kk];
+  // fix-up and store: x_i = z_i + H_i * Xin, H_i = prod_{j=i}^{e-1} (-c'_j)
+  {
+    double h = 1.0;
+#pragma unroll
+    for (int l = L - 1; l >= 0; --l) {
+      const int i = s + l;
+      if (active && i < nr) {
+        h = h * (-(c[i] * im[l]));
+        d[(int64_t)i * nz + kc] = static_cast<T>(x[l] + h * Xin);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+
+static int fft_block(int M) {
+  // threads so that every stage needs at most PER butterflies per thread
+  return std::min(256, std::max(64, M / 2));
+}
+
+template <typename T>
+hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const double* rscale,
+                            int rows, hipStream_t s) {
+  const int M = pl.nz / 2;
+  const int bs = fft_block(M);
+  const int per = (M / 2 + bs - 1) / bs;  // worst stage is radix 2
+  const size_t shm = sizeof(Cplx<T>) * M;
+  if (per <= 1)
+    dct2_rows_kernel<T, 1><<<rows, bs, shm, s>>>(src, dst, rscale, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
+  else if (per <= 2)
+    dct2_rows_kernel<T, 2><<<rows, bs, shm, s>>>(src, dst, rscale, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
+  else if (per <= 4)
+    dct2_rows_kernel<T, 4><<<rows, bs, shm, s>>>(src, dst, rscale, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
+  else
+    dct2_rows_kernel<T, 8><<<rows, bs, shm, s>>>(src, dst, rscale, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, hipStream_t s) {
+  const int M = pl.nz / 2;
+  const int bs = fft_block(M);
+  const int per = (M / 2 + bs - 1) / bs;
+  const size_t shm = sizeof(Cplx<T>) * M;
+  if (per <= 1)
+    dct3_rows_kernel<T, 1><<<rows, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
+  else if (per <= 2)
+    dct3_rows_kernel<T, 2><<<rows, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
+  else if (per <= 4)
+    dct3_rows_kernel<T, 4><<<rows, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
+  else
+    dct3_rows_kernel<T, 8><<<rows, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s) {
+  const int K = pl.tri_K, P = pl.tri_P;
+  const unsigned blocks = ceil_div(pl.nz, K);
+  const size_t shm = sizeof(double) * 4 * K * P;
+  switch (pl.tri_L) {
+    case 8:
+      tridiag_columns_kernel<T, 8><<<blocks, K * P, shm, s>>>(d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P);
+      break;
+    case 16:
+      tridiag_columns_kernel<T, 16><<<blocks, K * P, shm, s>>>(d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P);
+      break;
+    case 32:
+      tridiag_columns_kernel<T, 32><<<blocks, K * P, shm, s>>>(d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template hipError_t launch_dct2_rows<double>(const StokesPlan&, const double*, double*, const double*, int, hipStream_t);
+template hipError_t launch_dct3_rows<double>(const StokesPlan&, const double*, double*, int, hipStream_t);
+template hipError_t launch_tridiag<double>(const StokesPlan&, double*, hipStream_t);
+
+}  // namespace axs
+
+using namespace axs;
+
+struct axs_stokes_plan {
+  StokesPlan impl;
+};
+
+extern "C" {
+
+axs_status axs_stokes_plan_create(int nr, int nz, double dx, int bc_type,
+                                  axs_stokes_plan** plan_out) {
+  AXS_CHECK_ARG(plan_out, "null plan_out");
+  *plan_out = nullptr;
+  AXS_CHECK_ARG(bc_type == AXS_BC_NEUMANN_Z_AND_R,
+                "only homogenous_neumann_along_z_and_r is supported");
+  AXS_CHECK_ARG(nr >= 3 && nz >= 4 && nz % 2 == 0, "need nr >= 3 and even nz >= 4");
+  AXS_CHECK_ARG(nz <= 8192 && nr <= 16384, "grid too large for the row/column passes");
+  AXS_CHECK_ARG(dx > 0.0, "dx must be positive");
+  FftPlanArgs fp;
+  if (!factor_radices(nz / 2, &fp)) {
+    set_error("nz/2 must factor into 2, 3 and 5");
+    return AXS_ERR_UNSUPPORTED;
+  }
+  axs_stokes_plan* p = new (std::nothrow) axs_stokes_plan();
+  if (!p) {
+    set_error("out of host memory");
+    return AXS_ERR_ALLOC;
+  }
+  p->impl.fft = fp;
+  hipError_t e = p->impl.init(nr, nz, dx);
+  if (e != hipSuccess) {
+    set_error(std::string("stokes plan: ") + hipGetErrorString(e));
+    delete p;
+    return AXS_ERR_HIP;
+  }
+  *plan_out = p;
+  return AXS_OK;
+}
+
+axs_status axs_stokes_plan_destroy(axs_stokes_plan* plan) {
+  delete plan;
+  return AXS_OK;
+}
+
+axs_status axs_stokes_solve_f64(axs_stokes_plan* plan, double* psi, const double* rhs,
+                                void* stream) {
+  AXS_CHECK_ARG(plan && psi && rhs, "null pointer");
+  StokesPlan& pl = plan->impl;
+  hipStream_t s = as_stream(stream);
+  AXS_HIP(launch_dct2_rows<double>(pl, rhs, pl.spec, pl.rvec, pl.nr, s));
+  AXS_HIP(launch_tridiag<double>(pl, pl.spec, s));
+  AXS_HIP(launch_dct3_rows<double>(pl, pl.spec, psi, pl.nr, s));
+  return AXS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,39 @@
+// Stokes stream-function plan shared by the drop-in solver and the fused
+// stepper.
+#pragma once
+
+#include "axs_fft.hpp"
+
+namespace axs {
+
+struct StokesPlan {
+  int nr = 0, nz = 0;
+  double dx = 0.0;
+  FftPlanArgs fft{};
+  // device tables
+  Cplx<double>* tw = nullptr;  // e^{-2 pi i t/M}, t < M
+  Cplx<double>* e1 = nullptr;  // e^{-2 pi i k/nz}, k <= M
+  Cplx<double>* e2 = nullptr;  // e^{-i pi k/(2 nz)}, k <= M
+  double* a = nullptr;         // sub-diagonal of A_r (a[0] = 0)
+  double* b = nullptr;         // diagonal of A_r
+  double* c = nullptr;         // super-diagonal of A_r (c[nr-1] = 0)
+  double* lam = nullptr;       // z eigenvalues, nz
+  double* rvec = nullptr;      // radial coordinate r_i, nr
+  double* cseg = nullptr;      // c'_{pL-1} per segment p and mode k, (P, nz)
+  double* spec = nullptr;      // spectral work buffer (nr, nz)
+  int tri_L = 16, tri_P = 1, tri_K = 1;
+
+  hipError_t init(int nr, int nz, double dx);
+  int tri_threads() const;
+  ~StokesPlan();
+};
+
+template <typename T>
+hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const double* rscale,
+                            int rows, hipStream_t s);
+template <typename T>
+hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, hipStream_t s);
+template <typename T>
+hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s);
+
+}  // namespace axs
