@@ -109,7 +109,23 @@ def time_gpu(nz, nr, steps, warmup, stage_reps):
         b.record(stream)
         torch.cuda.synchronize()
         stages[name] = a.elapsed_time(b) * 1e3 / stage_reps  # us per launch
-    return ms, finite, float(cd[-1]) if len(cd) else None, stages
+    # plain copy kernel: achievable-bandwidth reference and byte-counter calibration
+    n = 1 << 24  # 128 MiB per buffer
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    args = (ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()), n,
+            ctypes.c_void_p(stream.cuda_stream))
+    _lib.call("axs_device_copy_f64", *args)
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    reps = 20
+    a.record(stream)
+    for _ in range(reps):
+        _lib.call("axs_device_copy_f64", *args)
+    b.record(stream)
+    torch.cuda.synchronize()
+    copy_gbs = 2 * 8 * n * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+    return ms, finite, float(cd[-1]) if len(cd) else None, stages, copy_gbs
 
 
 def cpu_baseline(nz, nr, steps):
@@ -161,11 +177,12 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every case is independent and this build runs on "
-              "one device (see DESIGN.md for how the path shards); run with --gpus 1",
+              "one device; independent cases shard across GPUs without communication; "
+              "run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
     nz, nr = args.nz, args.nr
-    ms, finite, cd, stages = time_gpu(nz, nr, args.steps, args.warmup, args.stage_reps)
+    ms, finite, cd, stages, copy_gbs = time_gpu(nz, nr, args.steps, args.warmup, args.stage_reps)
     if not finite:
         print("non-finite vorticity after the timed steps", file=sys.stderr)
         sys.exit(1)
@@ -194,6 +211,7 @@ def main():
             "stage_us": {k: round(v, 2) for k, v in stages.items()},
             "step_bytes_per_cell": sum(STAGE_BYTES_PER_CELL.values()),
             "graded_bytes_per_cell": 88,
+            "copy_kernel_GBs": round(copy_gbs, 1),
         },
         "roofline": {
             "bound": "hbm",

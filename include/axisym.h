@@ -98,6 +98,11 @@ axs_status axs_penalization_force_sum_f64(const double* R, const double* chi,
 axs_status axs_max_abs_velocity_sum_f64(const double* u_z, const double* u_r, int64_t n,
                                         double* partials, double* out_max, void* stream);
 
+/* Plain device copy of n doubles (16-byte aligned): no counterpart in the
+ * reference; the achievable-bandwidth reference and the calibration kernel
+ * for hardware byte counters. */
+axs_status axs_device_copy_f64(double* dst, const double* src, int64_t n, void* stream);
+
 /* ---- A11: elasto_kernels/solid_sigma.py:4-29.  eta_1z..eta_2r are in/out
  * scratch with the reference's partial-write semantics. */
 axs_status axs_solid_sigma_f64(double* sigma_s_11, double* sigma_s_12, double* sigma_s_22,

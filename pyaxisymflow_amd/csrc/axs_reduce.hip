@@ -4,6 +4,8 @@
 // fixed order, so repeated runs give identical bits.
 #include "axs_reduce.hpp"
 
+#include <algorithm>
+
 namespace axs {
 
 int reduce_blocks(int64_t n) {
@@ -103,6 +105,22 @@ template hipError_t launch_force_sum<double, RRow<double>>(RRow<double>, const d
 template hipError_t launch_max_velocity<double>(const double*, const double*, int64_t, double*,
                                                 double*, hipStream_t);
 
+// Plain streaming copy (one read, one write per element): the reference
+// point for achievable HBM bandwidth and for calibrating byte counters.
+__global__ void __launch_bounds__(256) copy_f64_kernel(double* __restrict__ dst,
+                                                       const double* __restrict__ src,
+                                                       int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * 256 * 2;
+  for (int64_t k = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 2; k < n; k += stride) {
+    if (k + 1 < n) {
+      const double2 v = *reinterpret_cast<const double2*>(src + k);
+      *reinterpret_cast<double2*>(dst + k) = v;
+    } else {
+      dst[k] = src[k];
+    }
+  }
+}
+
 }  // namespace axs
 
 using namespace axs;
@@ -110,6 +128,18 @@ using namespace axs;
 extern "C" {
 
 int64_t axs_reduce_partials_size(int64_t n) { return reduce_blocks(n); }
+
+axs_status axs_device_copy_f64(double* dst, const double* src, int64_t n, void* stream) {
+  AXS_CHECK_ARG(dst && src, "null pointer");
+  AXS_CHECK_ARG(n >= 0, "negative size");
+  AXS_CHECK_ARG((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) % 16 == 0,
+                "pointers must be 16-byte aligned");
+  if (n == 0) return AXS_OK;
+  const int64_t blocks = std::min<int64_t>((n + 511) / 512, 256 * 32);
+  copy_f64_kernel<<<(unsigned)blocks, 256, 0, as_stream(stream)>>>(dst, src, n);
+  AXS_LAUNCH_CHECK("copy_f64");
+  return AXS_OK;
+}
 
 axs_status axs_penalization_force_sum_f64(const double* R, const double* chi, const double* u_z,
                                           double U, int64_t n, double* partials,
