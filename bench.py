@@ -30,16 +30,13 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per cell of each stage (compulsory reads + writes, FP64):
 STAGE_BYTES_PER_CELL = {
-    "kill_taper": 0,            # perimeter only
-    "dct2_rows": 16,            # read omega, write omega-hat
-    "tridiag_columns": 16,      # read omega-hat, write psi-hat
-    "dct3_rows": 16,            # read psi-hat, write psi
-    "velocity_freestream": 24,  # read psi, write u_z, u_r
-    "finalize_dt": 0,
-    "penalize_curl": 56,        # read u_z, u_r, chi, omega; write u_pen (2), omega
-    "finalize_step": 0,
-    "eno3": 32,                 # read omega, u_pen (2); write omega'
-    "rk2_diffusion": 40,        # stage 1: 8 + 8; stage 2: 8 + 8 + 8
+    "taper_dct2_rows": 16,   # read omega (taper applied on the fly); write omega-hat
+    "tridiag_columns": 16,   # read omega-hat, write psi-hat
+    "dct3_rows": 16,         # read psi-hat, write psi
+    "velocity_max": 8,       # read psi
+    "body_mask": 8,          # write chi (moving body only)
+    "step_fused": 32,        # read psi, chi, omega; write omega
+    "finalize_step": 0,      # one block over the force partials
 }
 
 
@@ -209,7 +206,7 @@ def main():
             "nz": nz, "nr": nr, "domain": "z in [0,2], r in [0,1]", "lambda": 1e4, "nu": 1.5e-3,
             "U": 1.0, "graph": True, "last_Cd": cd,
             "stage_us": {k: round(v, 2) for k, v in stages.items()},
-            "step_bytes_per_cell": sum(STAGE_BYTES_PER_CELL.values()),
+            "step_bytes_per_cell": sum(v for k, v in STAGE_BYTES_PER_CELL.items() if k != "body_mask"),
             "graded_bytes_per_cell": 88,
             "copy_kernel_GBs": round(copy_gbs, 1),
         },

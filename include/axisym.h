@@ -155,48 +155,81 @@ axs_status axs_extrapolate_lsq_first_order_f64(int16_t* current_flag,
 
 /* ---- A16: the rigid-body driver step, device resident.
  * examples/FlowPastSphere/flow_past_sphere.py:107-188 (static body, free
- * stream with the sine ramp, drag coefficient Cd) and the oscillating
- * penalisation velocity of examples/SpheroidStreaming/spheroid_streaming.py:216-226.
+ * stream with the sine ramp, drag coefficient Cd) and
+ * examples/SpheroidStreaming/spheroid_streaming.py:129-246 (oscillating
+ * spheroid: mask rebuilt each step, penalisation velocity U_0 cos(omega t),
+ * dt clipped to the period and tEnd, psi averaged per period).
  * One step: kill taper -> Stokes solve -> velocity (+ free stream) -> dt ->
- * Brinkman + curl -> Cd -> ENO3 -> RK2 diffusion, with t, dt and the Cd
- * history kept on the device (slot layout: t, dt, dt/dx, lam dt, nu dt/2,
- * nu dt, ramp factor, U_pen, max|u|, drag sum, step counter).
- * The caller owns every buffer; all are (nr, nz) FP64 fields except
- * scalars (16 doubles), drag (history doubles or NULL), r (nr), kill_wz
- * (2*kill_width weights: left then right block) and kill_wr (kill_width). */
+ * [psi average] -> [body mask] -> Brinkman + curl -> force -> ENO3 -> RK2,
+ * with t, dt and the force history on the device (scalar slots: t, dt,
+ * dt/dx, lam dt, nu dt/2, nu dt, ramp factor, U_pen, max|u|, force sum,
+ * step counter, period timer, average reset flag, body centre, period timer
+ * of the step, ramp factor and U_pen of the last completed step).
+ * work0 holds the other half of a ping-pong pair with `vorticity`; when
+ * axs_rigid_stepper_run returns, `vorticity` holds the current field.
+ * u_pen_z / u_pen_r are filled on demand by axs_rigid_stepper_velocity;
+ * u_z, u_r and work1 are unused by the step (kept for the layout).
+ * precision AXS_F64: every field FP64.  AXS_F32: fields, stencils and
+ * z-transforms FP32; radial tridiagonal, mask level set, reductions and
+ * step scalars FP64.  The caller owns every buffer: (nr, nz) fields of the
+ * chosen precision, scalars (AXS_RIGID_SCALARS doubles), drag (history doubles or NULL),
+ * r (nr doubles), z (nz doubles; moving body only), kill_wz (2*kill_width
+ * doubles: left then right block), kill_wr (kill_width doubles). */
+#define AXS_F64 0
+#define AXS_F32 1
+#define AXS_RIGID_SCALARS 24
 typedef struct axs_rigid_stepper axs_rigid_stepper;
 typedef struct {
   int nr, nz, kill_width, history;
+  int precision;       /* AXS_F64 or AXS_F32 */
+  int moving_body;     /* 1: chi = H(phi) of the oscillating spheroid, rebuilt each step */
+  int average_psi;     /* 1: avg_psi += psi dt, restarted at each period boundary */
+  int pad;
   double dx, nu, brink_lam;
   double U0;           /* free stream added to u_z each step */
   double T_ramp;       /* ramp sin(pi t / (2 T_ramp)) while t < T_ramp; <= 0: none */
-  double CFL, eps;     /* dt = min(dt_diffusive, CFL dx / (max|u| + eps)) */
+  double CFL, eps;     /* dt = min(dt_diffusive, CFL dx / (max|u| + eps), dt_cap) */
   double dt_diffusive; /* 0.9 dx^2 / 4 / nu as the driver forms it */
   double osc_amp, osc_omega; /* penalisation velocity U_z(t) = osc_amp cos(osc_omega t) */
-  double drag_scale, drag_div; /* Cd = drag_scale * sum(r chi u_z) / drag_div */
+  double drag_scale, drag_div; /* force = drag_scale * sum(r chi (u_z - U_z(t))) / drag_div */
+  double dt_cap;       /* third term of the dt min (HUGE_VAL: none) */
+  double period;       /* clip dt to period boundaries (<= 0: none) */
+  double t_end;        /* clip dt at t_end (<= 0: none) */
+  /* moving spheroid: phi = radius - sqrt((z - zc(t))^2 / aspect2 + (r - r0)^2),
+   * zc(t) = z0 + disp sin(osc_omega t), chi = smooth Heaviside(phi, moll_zone) */
+  double body_z0, body_r0, body_aspect2, body_radius, body_disp, moll_zone;
 } axs_rigid_config;
 typedef struct {
-  double *vorticity, *psi, *u_z, *u_r, *u_pen_z, *u_pen_r, *char_func, *work0, *work1;
+  void *vorticity, *psi, *u_z, *u_r, *u_pen_z, *u_pen_r, *char_func, *work0, *work1, *avg_psi;
   double* scalars;
   double* drag;
   const double* r;
+  const double* z;
   const double* kill_wz;
   const double* kill_wr;
 } axs_rigid_buffers;
 axs_status axs_rigid_stepper_create(const axs_rigid_config* cfg, const axs_rigid_buffers* bufs,
                                     axs_rigid_stepper** out);
 axs_status axs_rigid_stepper_destroy(axs_rigid_stepper* st);
-/* Queues nsteps steps on `stream`; use_graph != 0 captures one step into a
- * hipGraph on first use and replays it (stream must not be the null stream). */
+/* Queues nsteps steps on `stream`; use_graph != 0 captures the step (one
+ * graph per ping-pong direction) on first use and replays it (stream must
+ * not be the null stream). */
 axs_status axs_rigid_stepper_run(axs_rigid_stepper* st, int nsteps, int use_graph, void* stream);
-/* Profiling hook: the step is a fixed sequence of stages (kill taper,
- * dct2 rows, tridiagonal columns, dct3 rows, velocity, dt, penalisation,
- * step scalars, ENO3, RK2).  run_stage queues `reps` launches of one stage
- * alone, so a timer on `stream` measures that stage.  It advances no state
- * consistently: use it only on a stepper whose results are discarded. */
+/* Profiling hook: the step is a fixed sequence of stages (taper + dct2 rows,
+ * tridiagonal columns, dct3 rows, velocity maximum, body mask, fused
+ * penalisation / advection / diffusion, step scalars; disabled stages are
+ * empty).  run_stage queues `reps` launches of one stage alone, so a timer
+ * on `stream` measures that stage.  It advances no state consistently: use
+ * it only on a stepper whose results are discarded. */
 int axs_rigid_stepper_stage_count(void);
 const char* axs_rigid_stepper_stage_name(int stage);
 axs_status axs_rigid_stepper_run_stage(axs_rigid_stepper* st, int stage, int reps, void* stream);
+/* Writes the penalised velocity of the last completed step (the velocity
+ * ENO3 advected with: brinkmann_penalize.py:4-16 applied to
+ * compute_velocity_from_psi.py:4-17 plus the free stream) into u_pen_z and
+ * u_pen_r.  Not part of the step, so a caller that never reads the
+ * velocity pays nothing for it. */
+axs_status axs_rigid_stepper_velocity(axs_rigid_stepper* st, void* stream);
 /* Recomputes the ramp factor and U_pen for the time now in scalars[0]. */
 axs_status axs_rigid_stepper_reset_scalars(axs_rigid_stepper* st, void* stream);
 

@@ -125,3 +125,223 @@ class RigidBodyCase:
         self.it += 1
         self.dts.append(dt)
         return dt
+
+
+class OscillatingSpheroidCase:
+    """examples/SpheroidStreaming/spheroid_streaming.py:29-246 (explicit
+    diffusion path): a spheroid oscillating along z in quiescent fluid,
+    mask rebuilt every step, dt clipped to the oscillation period and to
+    tEnd, psi time-averaged over each period.  Defaults are the reference's
+    (womersley_square = 100, radius 0.1, spheroid_AR = 1, freq = 16)."""
+
+    def __init__(self, grid_size_z, womersley_square=100.0, radius=0.1, spheroid_AR=1.0,
+                 freq=16, domain_AR=0.5, no_cycles=25, solver="eig", brink_lam=1e4):
+        # :40-68
+        dx = 1.0 / grid_size_z
+        grid_size_r = int(domain_AR * grid_size_z)
+        self.dx, self.nz, self.nr = dx, grid_size_z, grid_size_r
+        self.CFL = 0.1
+        self.eps = np.finfo(float).eps
+        self.brink_lam = brink_lam
+        self.moll_zone = dx * 2**0.5
+        self.radius_along = radius
+        self.radius_normal = radius / spheroid_AR
+        rad_eq = (self.radius_normal**2 * self.radius_along) ** (1 / 3)
+        self.length_scale = spheroid_AR * rad_eq
+        self.spheroid_AR = spheroid_AR
+        self.omega = 2 * np.pi * freq
+        nond_AC = 1.0 / np.sqrt(womersley_square)
+        self.e = 0.1
+        self.U_0 = self.e * self.length_scale * self.omega
+        Rs = (self.e / nond_AC) ** 2
+        self.nu = self.e * self.U_0 * self.length_scale / Rs
+        self.tEnd = no_cycles / freq
+        # :71-89
+        self.z = np.linspace(0 + dx / 2, 1 - dx / 2, grid_size_z)
+        self.r = np.linspace(0 + dx / 2, domain_AR - dx / 2, grid_size_r)
+        self.Z, self.R = np.meshgrid(self.z, self.r)
+        Z = self.Z
+        self.vorticity = 0 * Z
+        self.penal_vorticity = 0 * Z
+        self.temp_vorticity = 0 * Z
+        self.psi = 0 * Z
+        self.u_z = 0 * Z
+        self.u_r = 0 * Z
+        self.avg_psi = 0 * Z
+        self.Z_cm, self.R_cm = 0.5, 0.0
+        self.t = 0
+        self.it = 0
+        self.freqTimer = 0.0
+        # :104-112
+        self.freqTimer_limit = 1 / freq
+        diffusion_dt_limit = 0.9 * dx**2 / 4 / self.nu
+        num_diffusion_substeps = int(self.freqTimer_limit / diffusion_dt_limit)
+        self.diffusion_dt_limit = self.freqTimer_limit / num_diffusion_substeps
+        self.solver_kind = solver
+        if solver == "eig":
+            self.solver = FastDiagonalisationStokesSolver(grid_size_r, grid_size_z, dx)
+        self.dts = []
+        self.force = []
+        self.char_func = 0 * Z
+
+    def step(self):
+        """spheroid_streaming.py:129-246, one iteration (plus the Brinkman
+        force sum of compute_forces.py:14 on the penalised velocity)."""
+        dx, Z, R = self.dx, self.Z, self.R
+        K.kill_boundary_vorticity_sine_z(self.vorticity, Z, 3, dx)
+        K.kill_boundary_vorticity_sine_r(self.vorticity, R, 3, dx)
+        if self.solver_kind == "eig":
+            self.solver.solve(solution_field=self.psi, rhs_field=self.vorticity)
+        else:
+            self.psi[...] = exact_stokes_solve(self.vorticity, dx)
+        K.compute_velocity_from_psi_unb(self.u_z, self.u_r, self.psi, R, dx)
+        if np.isclose(self.freqTimer, self.freqTimer_limit):
+            self.freqTimer = 0.0
+            self.avg_psi[...] = 0 * Z
+        dt = min(
+            0.9 * dx**2 / 4 / self.nu,
+            self.CFL * dx / (np.amax(np.fabs(self.u_z) + np.fabs(self.u_r)) + self.eps),
+            0.01 * self.freqTimer_limit,
+        )
+        if self.freqTimer + dt > self.freqTimer_limit:
+            dt = self.freqTimer_limit - self.freqTimer
+        if self.t + dt > self.tEnd:
+            dt = self.tEnd - self.t
+        self.avg_psi[...] += self.psi * dt
+        Z_cm_t = self.Z_cm + self.e * self.length_scale * np.sin(self.omega * self.t)
+        phi = self.radius_normal - np.sqrt(
+            (Z - Z_cm_t) ** 2 / self.spheroid_AR**2 + (R - self.R_cm) ** 2
+        )
+        self.char_func = 0 * Z
+        K.smooth_Heaviside(self.char_func, phi, self.moll_zone)
+        u_z_upen = self.u_z.copy()
+        u_r_upen = self.u_r.copy()
+        U_pen = self.U_0 * np.cos(self.omega * self.t)
+        K.brinkmann_penalize(self.brink_lam, dt, self.char_func, U_pen, 0.0,
+                             u_z_upen, u_r_upen, self.u_z, self.u_r)
+        K.compute_vorticity_from_velocity_unb(
+            self.penal_vorticity, -u_z_upen + self.u_z, -u_r_upen + self.u_r, dx
+        )
+        self.vorticity[...] += self.penal_vorticity
+        # compute_forces.py:14 with rho_f = 1, times 2 pi dx^2 (falling_rigid_sphere.py:155)
+        self.force.append(2 * np.pi * dx**2 * 1.0 * self.brink_lam
+                          * np.sum(R * self.char_func * (self.u_z - U_pen)))
+        K.advect_vorticity_via_eno3(self.vorticity, self.u_z, self.u_r, dt, dx)
+        K.diffusion_RK2_unb(self.vorticity, self.temp_vorticity, R, self.nu, dt, dx)
+        self.t += dt
+        self.it += 1
+        self.freqTimer = self.freqTimer + dt
+        self.dts.append(dt)
+        return dt
+
+
+class SoftSphereCase:
+    """examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:27-215:
+    an elastic sphere (reference map eta, neo-Hookean-like stress with
+    shear modulus G) in an axisymmetric Taylor-Green-like vortex, no
+    Brinkman term.  scikit-fmm's ``distance`` is replaced by
+    oracle.reinit.distance (its restatement; parity with scikit-fmm itself
+    is unpinned)."""
+
+    def __init__(self, grid_size_z=256, G=1.0, nu=1e-3, rho_f=1.0, r_ball=0.15, domain_AR=0.5,
+                 solver="eig", C=5e-2):
+        from oracle.lsq import extrapolate_using_least_squares_till_first_order  # noqa: F401
+
+        dx = 1.0 / grid_size_z
+        grid_size_r = int(domain_AR * grid_size_z)
+        self.dx, self.nz, self.nr = dx, grid_size_z, grid_size_r
+        self.CFL = 0.1
+        self.eps = np.finfo(float).eps
+        # :36-45
+        self.moll_zone = dx * 4
+        self.extrap_zone = self.moll_zone + 4 * dx
+        self.reinit_band = self.extrap_zone
+        self.r_ball = r_ball
+        self.rho_f = rho_f
+        self.nu = nu
+        self.G = G
+        # :48-98
+        self.z = np.linspace(0 + dx / 2, 1 - dx / 2, grid_size_z)
+        self.r = np.linspace(0 + dx / 2, domain_AR - dx / 2, grid_size_r)
+        self.Z, self.R = np.meshgrid(self.z, self.r)
+        Z, R = self.Z, self.R
+        self.Z_cm, self.R_cm = 0.5, 0.0
+        self.ball_phi = -np.sqrt((Z - self.Z_cm) ** 2 + (R - self.R_cm) ** 2) + r_ball
+        self.ball_char_func = 0 * Z
+        K.smooth_Heaviside(self.ball_char_func, self.ball_phi, self.moll_zone)
+        self.inside_solid = self.ball_char_func > 0.5
+        self.vorticity = 0 * Z
+        self.temp_vorticity = 0 * Z
+        self.psi = 0 * Z
+        self.u_z = 0 * Z
+        self.u_r = 0 * Z
+        Z_double, R_double = np.meshgrid(self.z, self.z)
+        self.eta1 = Z.copy()
+        self.eta2 = R.copy()
+        self.eta1_double = Z_double.copy()
+        self.eta2_double = R_double.copy()
+        self.ball_phi_double = 0 * Z_double
+        kz = 2 * np.pi
+        kr = kz
+        self.vorticity[...] = C * (kz**2 + kr**2) * np.sin(kz * Z) * np.sin(kr * R)
+        self.t = 0
+        self.it = 0
+        self.bad_phi = 0 * Z
+        self.phi_orig = 0 * Z
+        self.sigma = [0 * Z for _ in range(3)]
+        self.eta_grad = [0 * Z for _ in range(4)]  # eta1z, eta1r, eta2z, eta2r
+        self.tau_z = 0 * Z
+        self.tau_r = 0 * Z
+        self.solver_kind = solver
+        if solver == "eig":
+            self.solver = FastDiagonalisationStokesSolver(grid_size_r, grid_size_z, dx)
+        self.dts = []
+        self.sweeps = []
+
+    def step(self):
+        """soft_sphere_in_taylor_green_vortex.py:101-213, one iteration."""
+        from oracle.reinit import distance
+
+        dx, Z, R = self.dx, self.Z, self.R
+        K.kill_boundary_vorticity_sine_z(self.vorticity, Z, 3, dx)
+        K.kill_boundary_vorticity_sine_r(self.vorticity, R, 3, dx)
+        if self.solver_kind == "eig":
+            self.solver.solve(solution_field=self.psi, rhs_field=self.vorticity)
+        else:
+            self.psi[...] = exact_stokes_solve(self.vorticity, dx)
+        K.compute_velocity_from_psi_unb(self.u_z, self.u_r, self.psi, R, dx)
+        dt = min(
+            self.CFL * dx / (np.amax(np.fabs(self.u_z) + np.fabs(self.u_r)) + self.eps),
+            self.CFL * dx / np.sqrt(self.G / self.rho_f),
+            0.9 * dx**2 / 4 / self.nu,
+        )
+        K.advect_refmap_via_eno3(self.eta1, self.eta2, self.u_z, self.u_r, dt, dx)
+        self.phi_orig[...] = (
+            -np.sqrt((self.eta1 - self.Z_cm) ** 2 + (self.eta2 - self.R_cm) ** 2) + self.r_ball
+        )
+        band = np.where(self.ball_phi > -3 * dx)
+        self.ball_phi[band] = self.phi_orig[band]
+        self.bad_phi[...] = self.ball_phi
+        d, masked = distance(self.ball_phi, dx, self.reinit_band)
+        self.ball_phi = np.where(masked, self.bad_phi, d)
+        self.ball_char_func *= 0
+        K.smooth_Heaviside(self.ball_char_func, self.ball_phi, self.moll_zone)
+        self.inside_solid[...] = self.ball_char_func > 0.5
+        K.extrapolate_eta_with_least_squares(
+            self.inside_solid, self.ball_phi, self.eta1, self.eta2, self.ball_phi_double,
+            self.eta1_double, self.eta2_double, self.extrap_zone, self.nr, self.z,
+        )
+        s11, s12, s22 = self.sigma
+        K.solid_sigma(s11, s12, s22, self.G, dx, self.eta1, self.eta2, *self.eta_grad)
+        s11[...] = self.ball_char_func * s11
+        s12[...] = self.ball_char_func * s12
+        s22[...] = self.ball_char_func * s22
+        K.update_vorticity_from_solid_stress(
+            self.vorticity, self.tau_z, self.tau_r, s11, s12, s22, R, dt, dx
+        )
+        K.advect_vorticity_via_eno3(self.vorticity, self.u_z, self.u_r, dt, dx)
+        K.diffusion_RK2_unb(self.vorticity, self.temp_vorticity, R, self.nu, dt, dx)
+        self.t += dt
+        self.it += 1
+        self.dts.append(dt)
+        return dt

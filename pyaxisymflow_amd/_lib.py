@@ -33,15 +33,12 @@ _D = ctypes.c_double
 class RigidConfig(ctypes.Structure):
     """axs_rigid_config (include/axisym.h)."""
 
-    _fields_ = [
-        ("nr", ctypes.c_int), ("nz", ctypes.c_int), ("kill_width", ctypes.c_int),
-        ("history", ctypes.c_int),
-        ("dx", ctypes.c_double), ("nu", ctypes.c_double), ("brink_lam", ctypes.c_double),
-        ("U0", ctypes.c_double), ("T_ramp", ctypes.c_double), ("CFL", ctypes.c_double),
-        ("eps", ctypes.c_double), ("dt_diffusive", ctypes.c_double),
-        ("osc_amp", ctypes.c_double), ("osc_omega", ctypes.c_double),
-        ("drag_scale", ctypes.c_double), ("drag_div", ctypes.c_double),
-    ]
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "nr", "nz", "kill_width", "history", "precision", "moving_body", "average_psi", "pad")] + [
+        (n, ctypes.c_double) for n in (
+            "dx", "nu", "brink_lam", "U0", "T_ramp", "CFL", "eps", "dt_diffusive", "osc_amp",
+            "osc_omega", "drag_scale", "drag_div", "dt_cap", "period", "t_end", "body_z0",
+            "body_r0", "body_aspect2", "body_radius", "body_disp", "moll_zone")]
 
 
 class RigidBuffers(ctypes.Structure):
@@ -49,7 +46,11 @@ class RigidBuffers(ctypes.Structure):
 
     _fields_ = [(n, ctypes.c_void_p) for n in (
         "vorticity", "psi", "u_z", "u_r", "u_pen_z", "u_pen_r", "char_func", "work0", "work1",
-        "scalars", "drag", "r", "kill_wz", "kill_wr")]
+        "avg_psi", "scalars", "drag", "r", "z", "kill_wz", "kill_wr")]
+
+
+AXS_F64 = 0
+AXS_F32 = 1
 
 
 # name -> argument types (restype axs_status = int unless listed in _RESTYPE)
@@ -81,6 +82,7 @@ SIGNATURES = {
     "axs_rigid_stepper_destroy": [_P],
     "axs_rigid_stepper_run": [_P, _I, _I, _P],
     "axs_rigid_stepper_reset_scalars": [_P, _P],
+    "axs_rigid_stepper_velocity": [_P, _P],
     "axs_rigid_stepper_stage_count": [],
     "axs_rigid_stepper_stage_name": [_I],
     "axs_rigid_stepper_run_stage": [_P, _I, _I, _P],

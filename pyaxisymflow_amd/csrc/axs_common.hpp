@@ -65,4 +65,19 @@ struct RRow {
   __device__ __forceinline__ T at(int i, int) const { return static_cast<T>(r[i]); }
 };
 
+// x / d rounded to nearest, given y = RN(1 / d): q = RN(x y) is within one
+// ulp of x / d, the residual x - q d is exact under fma, and q + RN(r y) is
+// then the correctly rounded quotient (Markstein's correction step,
+// P. Markstein, "Computation of elementary functions on the IBM RISC
+// System/6000 processor", IBM J. Res. Dev. 34(1), 1990).  Holds while
+// neither the quotient nor the residual overflows or underflows; a zero
+// quotient comes back as +0.  Three FP ops instead of the IEEE division
+// sequence when the divisor is constant per launch or per row.
+template <typename T>
+__device__ __forceinline__ T div_rn(T x, T d, T y) {
+  const T q = x * y;
+  const T r = fma(-q, d, x);
+  return fma(r, y, q);
+}
+
 }  // namespace axs

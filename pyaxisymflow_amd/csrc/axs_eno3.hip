@@ -155,6 +155,12 @@ hipError_t launch_eno3_refmap(const T* e1, const T* e2, const T* uz, const T* ur
 template hipError_t launch_eno3_conservative<double>(const double*, const double*, const double*,
                                                      double*, int, int, Scalar<double>, double,
                                                      hipStream_t);
+template hipError_t launch_eno3_conservative<float>(const float*, const float*, const float*,
+                                                    float*, int, int, Scalar<float>, float,
+                                                    hipStream_t);
+template hipError_t launch_eno3_refmap<float>(const float*, const float*, const float*,
+                                              const float*, float*, float*, int, int,
+                                              Scalar<float>, hipStream_t);
 template hipError_t launch_eno3_refmap<double>(const double*, const double*, const double*,
                                                const double*, double*, double*, int, int,
                                                Scalar<double>, hipStream_t);

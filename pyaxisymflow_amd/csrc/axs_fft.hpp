@@ -145,36 +145,66 @@ __device__ void fft_lds(Cplx<T>* buf, const FftPlanArgs& p, const Cplx<double>* 
 __device__ __forceinline__ int dct_src_index(int n, int N) {
   return (n < N / 2) ? 2 * n : 2 * (N - 1 - n) + 1;
 }
+// Inverse map: position n in v of column j of x.
+__device__ __forceinline__ int dct_dst_index(int j, int N) {
+  return (j % 2 == 0) ? j / 2 : N - 1 - (j - 1) / 2;
+}
+// Real component n of the packed sequence z_m = v_{2m} + i v_{2m+1}.
+template <typename T>
+__device__ __forceinline__ T& packed_component(Cplx<T>* z, int n) {
+  return (n & 1) ? z[n >> 1].y : z[n >> 1].x;
+}
 
 // Forward DCT-II post-processing from Z (in LDS, natural order) to X (out).
-// e1[k] = e^{-2 pi i k/N}, e2[k] = e^{-i pi k/(2N)}, k in [0, M].
-template <typename T, typename Out>
-__device__ void dct2_post(const Cplx<T>* Z, int M, const Cplx<double>* __restrict__ e1,
-                          const Cplx<double>* __restrict__ e2, Out out) {
+// Thread handles k = threadIdx.x + q * blockDim.x (q < KQ, k <= M/2) and its
+// partner M - k.  The twiddles e1[k] = e^{-2 pi i k/N} and
+// e2[k] = e^{-i pi k/(2N)} are fetched by dct2_post_twiddles before the FFT.
+template <int KQ>
+struct PostTwiddles {
+  Cplx<double> e1k[KQ], e1p[KQ], e2k[KQ], e2p[KQ];
+};
+
+template <int KQ>
+__device__ __forceinline__ void dct2_post_twiddles(PostTwiddles<KQ>& t, int M,
+                                                   const Cplx<double>* __restrict__ e1,
+                                                   const Cplx<double>* __restrict__ e2) {
+#pragma unroll
+  for (int q = 0; q < KQ; ++q) {
+    const int k = threadIdx.x + q * blockDim.x;
+    if (k <= M / 2) {
+      t.e1k[q] = e1[k];
+      t.e1p[q] = e1[M - k];
+      t.e2k[q] = e2[k];
+      t.e2p[q] = e2[M - k];
+    }
+  }
+}
+
+template <typename T, int KQ, typename Out>
+__device__ __forceinline__ void dct2_post(const Cplx<T>* Z, int M, const PostTwiddles<KQ>& t,
+                                          Out out) {
   const int N = 2 * M;
-  for (int k = threadIdx.x; k <= M / 2; k += blockDim.x) {
-    const int kp = M - k;  // partner
-    Cplx<T> zk = Z[k % M], zp = Z[kp % M];
-    // V_k and V_kp
-    auto makeV = [&](Cplx<T> za, Cplx<T> zb, int kk) {
-      Cplx<T> zbc = cconj(zb);
-      Cplx<T> s = cadd(za, zbc), d = csub(za, zbc);
-      Cplx<double> w = e1[kk];
-      Cplx<T> wd = cmul(Cplx<T>{T(w.x), T(w.y)}, d);
-      // V = s/2 - i * wd / 2
-      return Cplx<T>{T(0.5) * (s.x + wd.y), T(0.5) * (s.y - wd.x)};
-    };
-    Cplx<T> Vk = makeV(zk, zp, k);
-    Cplx<double> q = e2[k];
-    Cplx<T> wk = cmul(Cplx<T>{T(q.x), T(q.y)}, Vk);
-    out(k, wk.x);
-    if (k > 0) out(N - k, -wk.y);
-    if (kp != k) {
-      Cplx<T> Vp = makeV(zp, zk, kp);
-      Cplx<double> qp = e2[kp];
-      Cplx<T> wp = cmul(Cplx<T>{T(qp.x), T(qp.y)}, Vp);
-      out(kp, wp.x);
-      if (kp != M) out(N - kp, -wp.y);  // for kp == M, N-kp == M is already written
+#pragma unroll
+  for (int q = 0; q < KQ; ++q) {
+    const int k = threadIdx.x + q * blockDim.x;
+    if (k <= M / 2) {
+      const int kp = M - k;  // partner
+      const Cplx<T> zk = Z[k % M], zp = Z[kp % M];
+      // V_k = s/2 - i w d / 2 with s = Z_k + conj(Z_kp), d = Z_k - conj(Z_kp)
+      const Cplx<T> sk = cadd(zk, cconj(zp)), dk = csub(zk, cconj(zp));
+      const Cplx<T> wdk = cmul(Cplx<T>{T(t.e1k[q].x), T(t.e1k[q].y)}, dk);
+      const Cplx<T> Vk = {T(0.5) * (sk.x + wdk.y), T(0.5) * (sk.y - wdk.x)};
+      const Cplx<T> wk = cmul(Cplx<T>{T(t.e2k[q].x), T(t.e2k[q].y)}, Vk);
+      out(k, wk.x);
+      if (k > 0) out(N - k, -wk.y);
+      if (kp != k) {
+        const Cplx<T> sp = cadd(zp, cconj(zk)), dp = csub(zp, cconj(zk));
+        const Cplx<T> wdp = cmul(Cplx<T>{T(t.e1p[q].x), T(t.e1p[q].y)}, dp);
+        const Cplx<T> Vp = {T(0.5) * (sp.x + wdp.y), T(0.5) * (sp.y - wdp.x)};
+        const Cplx<T> wp = cmul(Cplx<T>{T(t.e2p[q].x), T(t.e2p[q].y)}, Vp);
+        out(kp, wp.x);
+        if (kp != M) out(N - kp, -wp.y);  // for kp == M, N-kp == M is already written
+      }
     }
   }
 }
@@ -182,26 +212,40 @@ __device__ void dct2_post(const Cplx<T>* Z, int M, const Cplx<double>* __restric
 // Inverse pre-processing: Z_k = A_k + i B_k from X (read via `in`), k in [0, M).
 //   V_k = e^{+i pi k/(2N)} (X_k - i X_{N-k}),  V_{k+M} = conj(V_{M-k})
 //   A_k = V_k + V_{k+M},  B_k = (V_k - V_{k+M}) e^{+2 pi i k/N}
-template <typename T, typename In>
-__device__ void dct3_pre(Cplx<T>* Z, int M, const Cplx<double>* __restrict__ e1,
-                         const Cplx<double>* __restrict__ e2, In in) {
+// Thread handles k = threadIdx.x + q * blockDim.x, q < KQ.  All global reads
+// (four X values and three twiddles per k) are issued before any arithmetic.
+template <typename T, int KQ, typename In>
+__device__ __forceinline__ void dct3_pre(Cplx<T>* Z, int M, const Cplx<double>* __restrict__ e1,
+                                         const Cplx<double>* __restrict__ e2, In in) {
   const int N = 2 * M;
-  for (int k = threadIdx.x; k < M; k += blockDim.x) {
-    const T xk = in(k);
-    const T xnk = (k == 0) ? T(0) : in(N - k);
-    Cplx<double> q = e2[k];
-    Cplx<T> Vk = cmul(Cplx<T>{T(q.x), T(-q.y)}, Cplx<T>{xk, -xnk});
-    const int mk = M - k;  // in [1, M]
-    const T xm = in(mk);
-    const T xnm = in(N - mk);  // N - mk in [M, N)
-    Cplx<double> qm = e2[mk];
-    Cplx<T> Vm = cmul(Cplx<T>{T(qm.x), T(-qm.y)}, Cplx<T>{xm, -xnm});
-    Cplx<T> Vkm = cconj(Vm);  // V_{k+M}
-    Cplx<T> A = cadd(Vk, Vkm);
-    Cplx<T> D = csub(Vk, Vkm);
-    Cplx<double> w = e1[k];
-    Cplx<T> B = cmul(D, Cplx<T>{T(w.x), T(-w.y)});
-    Z[k] = Cplx<T>{A.x - B.y, A.y + B.x};
+  T xk[KQ], xnk[KQ], xm[KQ], xnm[KQ];
+  Cplx<double> q2k[KQ], q2m[KQ], q1k[KQ];
+#pragma unroll
+  for (int q = 0; q < KQ; ++q) {
+    const int k = threadIdx.x + q * blockDim.x;
+    if (k < M) {
+      const int mk = M - k;  // in [1, M]
+      xk[q] = in(k);
+      xnk[q] = (k == 0) ? T(0) : in(N - k);
+      xm[q] = in(mk);
+      xnm[q] = in(N - mk);  // N - mk in [M, N)
+      q2k[q] = e2[k];
+      q2m[q] = e2[mk];
+      q1k[q] = e1[k];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < KQ; ++q) {
+    const int k = threadIdx.x + q * blockDim.x;
+    if (k < M) {
+      const Cplx<T> Vk = cmul(Cplx<T>{T(q2k[q].x), T(-q2k[q].y)}, Cplx<T>{xk[q], -xnk[q]});
+      const Cplx<T> Vm = cmul(Cplx<T>{T(q2m[q].x), T(-q2m[q].y)}, Cplx<T>{xm[q], -xnm[q]});
+      const Cplx<T> Vkm = cconj(Vm);  // V_{k+M}
+      const Cplx<T> A = cadd(Vk, Vkm);
+      const Cplx<T> D = csub(Vk, Vkm);
+      const Cplx<T> B = cmul(D, Cplx<T>{T(q1k[q].x), T(-q1k[q].y)});
+      Z[k] = Cplx<T>{A.x - B.y, A.y + B.x};
+    }
   }
 }
 

@@ -1,0 +1,565 @@
+// Fused stencil passes of the device-resident step (row A16).
+//
+//   velocity_max   u from psi (+ free stream) on 16 x 64 tiles ->
+//                  max(|u_z| + |u_r|), folded with one atomicMax per block
+//                  into one of kMaxSlots device slots (order-free, so the
+//                  result is deterministic; slots on separate lines keep
+//                  the atomics from serialising on one address).
+//   step_fused     everything from the penalisation to the end of the step
+//                  on a 16 x 54 output tile with a 6-cell halo:
+//                    dt rule from the slot maximum (flow_past_sphere.py:
+//                    150-153, spheroid_streaming.py:139-201),
+//                    u from psi (compute_velocity_from_psi.py:4-17),
+//                    Brinkman (brinkmann_penalize.py:4-16),
+//                    kill taper of the stored field (kill_boundary_
+//                    vorticity_sine.py:4-27), omega += curl(u_pen - u)
+//                    (compute_vorticity_from_velocity.py:4-13),
+//                    force partial sums (compute_forces.py:14), psi average,
+//                    ENO3 step (advection_timestep.py:45-54, in-index axis
+//                    mirror) and both RK2 stages (diffusion_RK2.py:4-44);
+//                  reads psi, chi and the stored omega, writes only the new
+//                  omega (into the other buffer of a ping-pong pair).
+//   velocity_out   u_pen of the last step, on demand (not part of a step).
+//   finalize_step  one block: fixed-order force sum, t += dt, it += 1.
+// No per-block device-scope fences: the L2s of the XCDs are not coherent, so
+// such a fence costs an L2 write-back; kernel boundaries order the passes.
+//
+// Per-cell arithmetic and its order are those of the single-purpose kernels.
+// Divisions by dx^2, 2 dx and r_i use div_rn (correctly rounded, so the
+// results are unchanged); ENO3 faces shared by two cells are evaluated once
+// (the backward face of a cell is the forward face of its lower neighbour,
+// bit for bit: same comparison, same products, same sums).
+#include "axs_fused.hpp"
+
+#include <algorithm>
+
+namespace axs {
+
+// ------------------------------------------------------------ helpers
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off);
+  return v;
+}
+
+// u_z (with the free stream) and u_r at global (i, j); p points at that
+// cell in a row-major array of row stride ld (global psi or an LDS tile)
+// (compute_velocity_from_psi.py:10-17, one-sided at the edges).
+template <typename T>
+__device__ __forceinline__ void velocity_rn(const T* p, int ld, int i, int j, int nr, int nz,
+                                            T two_dx, T inv2dx, T ri, T inv_ri, T ufree, T& uz,
+                                            T& ur) {
+  T nzv, nrv;
+  if (i == 0)
+    nzv = -p[2 * ld] + T(4) * p[ld] - T(3) * p[0];
+  else if (i == nr - 1)
+    nzv = p[-2 * ld] - T(4) * p[-ld] + T(3) * p[0];
+  else
+    nzv = p[ld] - p[-ld];
+  if (j == 0)
+    nrv = -(-p[2] + T(4) * p[1] - T(3) * p[0]);
+  else if (j == nz - 1)
+    nrv = -(p[-2] - T(4) * p[-1] + T(3) * p[0]);
+  else
+    nrv = -(p[1] - p[-1]);
+  uz = div_rn(div_rn(nzv, two_dx, inv2dx), ri, inv_ri) + ufree;
+  ur = div_rn(div_rn(nrv, two_dx, inv2dx), ri, inv_ri);
+}
+
+// (u + ldc U) / (1 + ldc); outside the body (ldc == 0) the quotient is the
+// numerator itself, so the division is skipped without changing a bit.
+template <typename T>
+__device__ __forceinline__ T brinkman(T u, T ldc, T Uv) {
+  T t = u + ldc * Uv;
+  if (ldc != T(0)) t = t / (T(1) + ldc);
+  return t;
+}
+
+__device__ __forceinline__ double load_slot(const double* p) {
+  const unsigned long long b = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double(static_cast<long long>(b));
+}
+
+// Kill taper (kill_boundary_vorticity_sine.py:4-27): z ends of every row
+// from columns w-1 and nz-w, then rows nr-w..nr-1 scaled copies of row
+// nr-w, then row 0 = 0.  The tapered value of (i, j) is a weight times one
+// stored value, the one at taper_source(i, j).
+__device__ __forceinline__ int64_t taper_source(int i, int j, int nr, int nz, int w) {
+  const int srow = (i >= nr - w) ? nr - w : i;
+  const int jsrc = (j < w) ? w - 1 : ((j >= nz - w) ? nz - w : j);
+  return (int64_t)srow * nz + jsrc;
+}
+template <typename T>
+__device__ __forceinline__ T taper_apply(T v, int i, int j, int nr, int nz, int w,
+                                         const double* __restrict__ wz,
+                                         const double* __restrict__ wr) {
+  T t = v;
+  if (j < w)
+    t = T(wz[j]) * v;
+  else if (j >= nz - w)
+    t = T(wz[w + (j - (nz - w))]) * v;
+  if (i >= nr - w) t = T(wr[i - (nr - w)]) * t;
+  if (i == 0) t = T(0);
+  return t;
+}
+
+template <typename T>
+__device__ __forceinline__ T eno3_face_f(T fm1, T f0, T fp1, T fp2, T um1, T u0, T up1, T up2) {
+  // conservative face between cells 0 and +1 (advection_flux.py:32-44)
+  return (u0 > -up1) ? (T(1) / T(3)) * fp1 * up1 + (T(5) / T(6)) * f0 * u0 - (T(1) / T(6)) * fm1 * um1
+                     : (T(1) / T(3)) * f0 * u0 + (T(5) / T(6)) * fp1 * up1 - (T(1) / T(6)) * fp2 * up2;
+}
+template <typename T>
+__device__ __forceinline__ T eno3_face_b(T fm2, T fm1, T f0, T fp1, T um2, T um1, T u0, T up1) {
+  // conservative face between cells -1 and 0 (advection_flux.py:61-73)
+  return (u0 > -um1) ? (T(1) / T(3)) * f0 * u0 + (T(5) / T(6)) * fm1 * um1 - (T(1) / T(6)) * fm2 * um2
+                     : (T(1) / T(3)) * fm1 * um1 + (T(5) / T(6)) * f0 * u0 - (T(1) / T(6)) * fp1 * up1;
+}
+
+// ------------------------------------------------------------ velocity_max
+
+constexpr int kVTZ = 64, kVTR = 16;                  // tile
+constexpr int kVPZ = kVTZ + 4, kVPR = kVTR + 4;      // psi tile (halo 2)
+
+template <typename T>
+__global__ void __launch_bounds__(256) velocity_max_kernel(const T* __restrict__ psi,
+                                                           const double* __restrict__ r, int nr,
+                                                           int nz, T dx,
+                                                           const double* __restrict__ sc,
+                                                           double* __restrict__ slots,
+                                                           RigidConfig cfg) {
+  __shared__ T sp[kVPR][kVPZ];
+  __shared__ T sri[kVTR], sinv[kVTR];
+  __shared__ double wm[4];
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.x * kVTZ, i0 = blockIdx.y * kVTR;
+  constexpr int NP = (kVPR * kVPZ + 255) / 256;
+  T v[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const int e = tid + q * 256;
+    const int a = e / kVPZ, b = e % kVPZ;
+    const int i = i0 - 2 + a, j = j0 - 2 + b;
+    v[q] = T(0);
+    if (e < kVPR * kVPZ && i >= 0 && i < nr && j >= 0 && j < nz) v[q] = psi[(int64_t)i * nz + j];
+  }
+  if (tid < kVTR) {
+    const int i = i0 + tid;
+    const T ri = (i < nr) ? T(r[i]) : T(1);
+    sri[tid] = ri;
+    sinv[tid] = T(1) / ri;
+  }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const int e = tid + q * 256;
+    if (e < kVPR * kVPZ) sp[e / kVPZ][e % kVPZ] = v[q];
+  }
+  __syncthreads();
+  const T two_dx = T(2) * dx, inv2dx = T(1) / two_dx;
+  const T ufree = T(cfg.U0 * sc[kScPrefac]);
+  double m = 0.0;
+#pragma unroll
+  for (int q = 0; q < (kVTR * kVTZ) / 256; ++q) {
+    const int e = tid + q * 256;
+    const int a = e / kVTZ, b = e % kVTZ;
+    const int i = i0 + a, j = j0 + b;
+    if (i < nr && j < nz) {
+      T uz, ur;
+      velocity_rn<T>(&sp[a + 2][b + 2], kVPZ, i, j, nr, nz, two_dx, inv2dx, sri[a], sinv[a], ufree,
+                     uz, ur);
+      m = fmax(m, double(fabs(uz) + fabs(ur)));
+    }
+  }
+  m = wave_max(m);
+  if ((tid & 63) == 0) wm[tid >> 6] = m;
+  __syncthreads();
+  if (tid == 0) {
+    m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+    const int slot = (blockIdx.y * gridDim.x + blockIdx.x) % kMaxSlots;
+    atomicMax(reinterpret_cast<unsigned long long*>(slots + slot * kSlotStride),
+              static_cast<unsigned long long>(__double_as_longlong(m)));
+  }
+}
+
+// ------------------------------------------------------------ step_fused
+
+// Output tile kFR x kFZ.  Regions by halo around it: R6 psi, R5 velocity and
+// penalisation, R4 ENO3 input (f, u_pen), R2 ENO3 output, R1 RK2 stage 1.
+constexpr int kFZ = 54, kFR = 16;
+constexpr int kR6Z = kFZ + 12, kR6R = kFR + 12;
+constexpr int kR5Z = kFZ + 10, kR5R = kFR + 10;
+constexpr int kR4Z = kFZ + 8, kR4R = kFR + 8;
+constexpr int kR2Z = kFZ + 4, kR2R = kFR + 4;
+constexpr int kR1Z = kFZ + 2, kR1R = kFR + 2;
+// Thread strips: penalisation 4 x 64 (kVS owned R4 rows + one R5 row above
+// and below), ENO3 rows kEH, stage 1 rows kSH, stage 2 rows kOH.
+constexpr int kVS = 6, kEH = 5, kSH = 6, kOH = 4;
+static_assert(kR5Z == 64 && 4 * kVS == kR4R, "penalisation strips cover R4 rows x R5 columns");
+static_assert(kR2R % kEH == 0 && kR1R % kSH == 0 && kFR % kOH == 0, "strip heights");
+static_assert((kR2R / kEH) * kR2Z <= 256 && (kR1R / kSH) * kR1Z <= 256 && (kFR / kOH) * kFZ <= 256,
+              "one strip per thread");
+// LDS buffers and what they hold phase by phase.
+constexpr int kBufA = kR6R * kR6Z;  // psi (R6), then f (R4)
+constexpr int kBufB = kR4R * kR5Z;  // u_pen_r - u_r (R4 rows x R5 cols), then ENO3 out (R2)
+constexpr int kBufC = kR4R * kR4Z;  // u_pen_z (R4), then stage-1 out (R1)
+constexpr int kBufD = kR4R * kR4Z;  // u_pen_r (R4)
+static_assert(kR4R * kR4Z <= kBufA && kR2R * kR2Z <= kBufB && kR1R * kR1Z <= kBufC, "aliases");
+
+int fused_blocks(int nr, int nz) { return (int)(ceil_div(nz, kFZ) * ceil_div(nr, kFR)); }
+
+template <typename T>
+__global__ void __launch_bounds__(256) step_fused_kernel(
+    const T* __restrict__ psi, const T* __restrict__ chi, const T* __restrict__ wsrc,
+    T* __restrict__ wdst, T* __restrict__ avg, const double* __restrict__ r,
+    const double* __restrict__ wz, const double* __restrict__ wr, int kw, int nr, int nz, T dx,
+    double* __restrict__ sc, const double* __restrict__ slots, double* __restrict__ partials,
+    RigidConfig cfg) {
+  __shared__ T bufA[kBufA], bufB[kBufB], bufC[kBufC], bufD[kBufD];
+  __shared__ T sri[kR6R], sinv[kR6R], sinv2[kR6R];
+  __shared__ double wpart[4];
+  __shared__ double smax;
+  T(*sp)[kR6Z] = reinterpret_cast<T(*)[kR6Z]>(bufA);
+  T(*sf)[kR4Z] = reinterpret_cast<T(*)[kR4Z]>(bufA);
+  T(*sdr)[kR5Z] = reinterpret_cast<T(*)[kR5Z]>(bufB);
+  T(*sa)[kR2Z] = reinterpret_cast<T(*)[kR2Z]>(bufB);
+  T(*sx)[kR4Z] = reinterpret_cast<T(*)[kR4Z]>(bufC);
+  T(*st)[kR1Z] = reinterpret_cast<T(*)[kR1Z]>(bufC);
+  T(*sy)[kR4Z] = reinterpret_cast<T(*)[kR4Z]>(bufD);
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.x * kFZ, i0 = blockIdx.y * kFR;
+  // penalisation strip of this thread: R5 column pb, R5 rows 6ps .. 6ps+7
+  const int pb = tid & 63, ps = tid >> 6;
+  const int pj = j0 - 5 + pb;
+  const bool pjin = pj >= 0 && pj < nz;
+
+  // ---- 1. loads: psi tile, chi and stored omega of the strip, slots, rows
+  constexpr int NP = (kBufA + 255) / 256;
+  T vp[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const int e = tid + q * 256;
+    const int a = e / kR6Z, b = e % kR6Z;
+    const int i = i0 - 6 + a, j = j0 - 6 + b;
+    vp[q] = T(0);
+    if (e < kBufA && i >= 0 && i < nr && j >= 0 && j < nz) vp[q] = psi[(int64_t)i * nz + j];
+  }
+  T vc[kVS + 2], vw[kVS];
+#pragma unroll
+  for (int k = 0; k < kVS + 2; ++k) {
+    const int i = i0 - 5 + kVS * ps + k;
+    vc[k] = T(0);
+    if (pjin && i >= 0 && i < nr) vc[k] = chi[(int64_t)i * nz + pj];
+  }
+#pragma unroll
+  for (int k = 0; k < kVS; ++k) {
+    const int i = i0 - 4 + kVS * ps + k;
+    vw[k] = T(0);
+    if (pjin && i >= 0 && i < nr) vw[k] = wsrc[taper_source(i, pj, nr, nz, kw)];
+  }
+  if (tid < 64) {
+    const double m = wave_max(load_slot(slots + tid * kSlotStride));
+    if (tid == 0) smax = m;
+  }
+  if (tid >= 64 && tid < 64 + kR6R) {
+    const int i = i0 - 6 + (tid - 64);
+    const T ri = (i >= 0 && i < nr) ? T(r[i]) : T(1);
+    sri[tid - 64] = ri;
+    sinv[tid - 64] = T(1) / ri;
+    sinv2[tid - 64] = T(1) / (ri * ri);
+  }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const int e = tid + q * 256;
+    if (e < kBufA) bufA[e] = vp[q];
+  }
+  __syncthreads();
+
+  // ---- 2. dt, then u and u_pen on the strip's R5 cells
+  const double mx = smax;
+  const DtScalars ds = compute_dt(sc, mx, cfg);
+  const T two_dx = T(2) * dx, inv2dx = T(1) / two_dx;
+  const T ufree = T(cfg.U0 * sc[kScPrefac]);
+  const T lam_dt = T(cfg.brink_lam * ds.dt);
+  const double Ud = sc[kScUpen];
+  const T U = T(Ud);
+  const T tdt = T(ds.dt);
+  T dz[kVS + 2];  // u_pen_z - u_z on R5 rows 6ps-1+1 .. (column pb)
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < kVS + 2; ++k) {
+    const int q5 = kVS * ps + k;  // R5 row
+    const int i = i0 - 5 + q5;
+    T dzv = T(0), drv = T(0), pzv = T(0), prv = T(0);
+    if (pjin && i >= 0 && i < nr) {
+      const T* p = &sp[q5 + 1][pb + 1];
+      T uz, ur;
+      velocity_rn<T>(p, kR6Z, i, pj, nr, nz, two_dx, inv2dx, sri[q5 + 1], sinv[q5 + 1], ufree,
+                     uz, ur);
+      const T ldc = lam_dt * vc[k];
+      pzv = brinkman(uz, ldc, U);
+      prv = brinkman(ur, ldc, T(0));
+      dzv = pzv - uz;
+      drv = prv - ur;
+      // output tile cells: R5 rows 5..5+kFR-1, columns 5..5+kFZ-1
+      if (k >= 1 && k <= kVS && q5 >= 5 && q5 < 5 + kFR && pb >= 5 && pb < 5 + kFZ) {
+        const int64_t c = (int64_t)i * nz + pj;
+        if (avg) {
+          const T base = (ds.reset != 0.0) ? T(0) : avg[c];
+          avg[c] = base + p[0] * tdt;
+        }
+        acc += (r[i] * double(vc[k])) * (double(pzv) - Ud);
+      }
+    }
+    dz[k] = dzv;
+    if (k >= 1 && k <= kVS) {
+      const int a4 = q5 - 1;
+      sdr[a4][pb] = drv;
+      if (pb >= 1 && pb <= kR4Z) {
+        sx[a4][pb - 1] = pzv;
+        sy[a4][pb - 1] = prv;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 3. f = taper(omega) + curl(u_pen - u) on the strip's R4 cells
+  if (pb >= 1 && pb <= kR4Z) {
+#pragma unroll
+    for (int k = 1; k <= kVS; ++k) {
+      const int a4 = kVS * ps + k - 1;
+      const int i = i0 - 4 + a4;
+      T fv = T(0);
+      if (pjin && i >= 0 && i < nr) {
+        fv = taper_apply(vw[k - 1], i, pj, nr, nz, kw, wz, wr);
+        if (i >= 1 && i <= nr - 2 && pj >= 1 && pj <= nz - 2) {
+          const T pv = div_rn(sdr[a4][pb + 1] - sdr[a4][pb - 1], two_dx, inv2dx) -
+                       div_rn(dz[k + 1] - dz[k - 1], two_dx, inv2dx);
+          fv = fv + pv;
+        }
+      }
+      sf[a4][pb - 1] = fv;
+    }
+  }
+  __syncthreads();
+  if (i0 == 0) {
+    // rows -4..-1 of the ENO3 input are the axis mirror of rows 3..0
+    // (f odd, u_z even, u_r odd: advect_vorticity_via_eno3.py:28-33)
+    for (int e = tid; e < 4 * kR4Z; e += 256) {
+      const int m = e / kR4Z, c = e % kR4Z;
+      sf[3 - m][c] = -sf[4 + m][c];
+      sx[3 - m][c] = sx[4 + m][c];
+      sy[3 - m][c] = -sy[4 + m][c];
+    }
+    __syncthreads();
+  }
+
+  // ---- 4. ENO3 on R2 (advection_timestep.py:45-54); r faces slide through
+  //         registers, each evaluated once
+  const T s = -T(ds.dt / cfg.dx);
+  if (tid < (kR2R / kEH) * kR2Z) {
+    const int b = tid % kR2Z, a0 = (tid / kR2Z) * kEH;
+    const int ib = b + 2;        // R4 column
+    T wf[kEH + 4], wy[kEH + 4];  // R4 rows a0 .. a0 + kEH + 3
+#pragma unroll
+    for (int k = 0; k < kEH + 4; ++k) {
+      wf[k] = sf[a0 + k][ib];
+      wy[k] = sy[a0 + k][ib];
+    }
+    T fr[kEH + 1];  // fr[m]: face between R2 rows a0+m-1 and a0+m
+#pragma unroll
+    for (int m = 0; m <= kEH; ++m)
+      fr[m] = eno3_face_f(wf[m], wf[m + 1], wf[m + 2], wf[m + 3], wy[m], wy[m + 1], wy[m + 2],
+                          wy[m + 3]);
+    const int j = j0 - 2 + b;
+    const bool jok = j >= 2 && j < nz - 2;
+    T out[kEH];
+#pragma unroll
+    for (int k = 0; k < kEH; ++k) {
+      const int ia = a0 + k + 2;
+      const int i = i0 - 2 + a0 + k;
+      T v = wf[k + 2];
+      if (jok && i >= 0 && i < nr - 2) {
+        const T fm2 = sf[ia][ib - 2], fm1 = sf[ia][ib - 1], fp1 = sf[ia][ib + 1],
+                fp2 = sf[ia][ib + 2];
+        const T xm2 = sx[ia][ib - 2], xm1 = sx[ia][ib - 1], x0 = sx[ia][ib],
+                xp1 = sx[ia][ib + 1], xp2 = sx[ia][ib + 2];
+        T flux = T(0) + s * eno3_face_f(fm1, v, fp1, fp2, xm1, x0, xp1, xp2);
+        flux = flux - s * eno3_face_b(fm2, fm1, v, fp1, xm2, xm1, x0, xp1);
+        flux = flux + s * fr[k + 1];
+        flux = flux - s * fr[k];
+        v = v + flux;
+      }
+      out[k] = v;
+    }
+    // bufB (u_pen_r - u_r) was last read before the previous barrier
+#pragma unroll
+    for (int k = 0; k < kEH; ++k) sa[a0 + k][b] = out[k];
+  }
+  __syncthreads();
+
+  // ---- 5. RK2 stage 1 on R1 (diffusion_RK2.py:9-26)
+  const T dx2 = dx * dx, inv_dx2 = T(1) / dx2;
+  const T half_nu_dt = T(0.5 * cfg.nu * ds.dt), nu_dt = T(cfg.nu * ds.dt);
+  if (tid < (kR1R / kSH) * kR1Z) {
+    const int b = tid % kR1Z, a0 = (tid / kR1Z) * kSH;
+    const int eb = b + 1;  // R2 column
+    T ws[kSH + 2];         // R2 rows a0 .. a0 + kSH + 1
+#pragma unroll
+    for (int k = 0; k < kSH + 2; ++k) ws[k] = sa[a0 + k][eb];
+    const int j = j0 - 1 + b;
+    const bool jok = j >= 1 && j <= nz - 2;
+#pragma unroll
+    for (int k = 0; k < kSH; ++k) {
+      const int a = a0 + k, ea = a + 1, a6 = a + 5;
+      const int i = i0 - 1 + a;
+      T v = ws[k + 1];
+      if (jok && i >= 1 && i <= nr - 2) {
+        const T up = ws[k + 2], dn = ws[k];
+        const T L = div_rn(up + dn + sa[ea][eb + 1] + sa[ea][eb - 1] - T(4) * v, dx2, inv_dx2) +
+                    div_rn(div_rn(up - dn, two_dx, inv2dx), sri[a6], sinv[a6]) - v * sinv2[a6];
+        v = v + half_nu_dt * L;
+      }
+      st[a][b] = v;  // bufC (u_pen_z) was last read before the previous barrier
+    }
+  }
+  __syncthreads();
+
+  // ---- 6. RK2 stage 2 on the output tile (diffusion_RK2.py:27-44)
+  if (tid < (kFR / kOH) * kFZ) {
+    const int b = tid % kFZ, a0 = (tid / kFZ) * kOH;
+    const int tb = b + 1;  // R1 column
+    T wt[kOH + 2];         // R1 rows a0 .. a0 + kOH + 1
+#pragma unroll
+    for (int k = 0; k < kOH + 2; ++k) wt[k] = st[a0 + k][tb];
+    const int j = j0 + b;
+    const bool jin = j < nz, jok = j >= 1 && j <= nz - 2;
+#pragma unroll
+    for (int k = 0; k < kOH; ++k) {
+      const int a = a0 + k, ta = a + 1, a6 = a + 6;
+      const int i = i0 + a;
+      if (!jin || i >= nr) continue;
+      T v = sa[a + 2][b + 2];
+      if (jok && i >= 1 && i <= nr - 2) {
+        const T tc = wt[k + 1], up = wt[k + 2], dn = wt[k];
+        const T L = div_rn(up + dn + st[ta][tb + 1] + st[ta][tb - 1] - T(4) * tc, dx2, inv_dx2) +
+                    div_rn(div_rn(up - dn, two_dx, inv2dx), sri[a6], sinv[a6]) - tc * sinv2[a6];
+        v = v + nu_dt * L;
+      }
+      wdst[(int64_t)i * nz + j] = v;
+    }
+  }
+
+  // ---- force partial of this tile; block (0, 0) publishes the dt scalars
+  acc = wave_sum(acc);
+  if ((tid & 63) == 0) wpart[tid >> 6] = acc;
+  __syncthreads();
+  if (tid == 0) {
+    partials[blockIdx.y * gridDim.x + blockIdx.x] = ((wpart[0] + wpart[1]) + wpart[2]) + wpart[3];
+    if (blockIdx.x == 0 && blockIdx.y == 0) store_dt_scalars(sc, ds, mx, cfg);
+  }
+}
+
+// ------------------------------------------------------------ velocity_out
+
+// u_pen of the last step from psi and chi with that step's dt, ramp factor
+// and penalisation velocity (the values step_fused used, bit for bit).
+template <typename T>
+__global__ void __launch_bounds__(256) velocity_out_kernel(const T* __restrict__ psi,
+                                                           const T* __restrict__ chi,
+                                                           T* __restrict__ pz, T* __restrict__ pr,
+                                                           const double* __restrict__ r, int nr,
+                                                           int nz, T dx,
+                                                           const double* __restrict__ sc,
+                                                           RigidConfig cfg) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= nz) return;
+  const T two_dx = T(2) * dx, inv2dx = T(1) / two_dx;
+  const T ri = T(r[i]);
+  const int64_t c = (int64_t)i * nz + j;
+  T uz, ur;
+  velocity_rn<T>(psi + c, nz, i, j, nr, nz, two_dx, inv2dx, ri, T(1) / ri,
+                 T(cfg.U0 * sc[kScPrefacStep]), uz, ur);
+  const T ldc = T(sc[kScLamDt]) * chi[c];
+  pz[c] = brinkman(uz, ldc, T(sc[kScUpenStep]));
+  pr[c] = brinkman(ur, ldc, T(0));
+}
+
+// ------------------------------------------------------------ finalize
+
+__global__ void __launch_bounds__(1024) finalize_step_kernel(const double* __restrict__ partials,
+                                                             int count, double* __restrict__ slots,
+                                                             double* __restrict__ sc,
+                                                             double* __restrict__ drag,
+                                                             RigidConfig cfg) {
+  __shared__ double red[1024];
+  double s = 0.0;
+  for (int q = threadIdx.x; q < count; q += 1024) s += partials[q];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 512; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x < kMaxSlots) slots[threadIdx.x * kSlotStride] = 0.0;
+  if (threadIdx.x == 0) finalize_step_scalars(sc, drag, red[0], cfg);
+}
+
+// ------------------------------------------------------------ launchers
+
+template <typename T>
+hipError_t launch_velocity_max(const T* psi, const double* r, int nr, int nz, T dx,
+                               const double* sc, double* slots, const RigidConfig& cfg,
+                               hipStream_t s) {
+  velocity_max_kernel<T><<<dim3(ceil_div(nz, kVTZ), ceil_div(nr, kVTR)), 256, 0, s>>>(
+      psi, r, nr, nz, dx, sc, slots, cfg);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_step_fused(const T* psi, const T* chi, const T* wsrc, T* wdst, T* avg,
+                             const double* r, const double* wz, const double* wr, int kw, int nr,
+                             int nz, T dx, double* sc, const double* slots, double* partials,
+                             const RigidConfig& cfg, hipStream_t s) {
+  step_fused_kernel<T><<<dim3(ceil_div(nz, kFZ), ceil_div(nr, kFR)), 256, 0, s>>>(
+      psi, chi, wsrc, wdst, avg, r, wz, wr, kw, nr, nz, dx, sc, slots, partials, cfg);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_velocity_out(const T* psi, const T* chi, T* pz, T* pr, const double* r, int nr,
+                               int nz, T dx, const double* sc, const RigidConfig& cfg,
+                               hipStream_t s) {
+  velocity_out_kernel<T><<<dim3(ceil_div(nz, 256), nr), 256, 0, s>>>(psi, chi, pz, pr, r, nr, nz,
+                                                                      dx, sc, cfg);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_step(const double* partials, int count, double* slots, double* sc,
+                                double* drag, const RigidConfig& cfg, hipStream_t s) {
+  finalize_step_kernel<<<1, 1024, 0, s>>>(partials, count, slots, sc, drag, cfg);
+  return hipGetLastError();
+}
+
+#define AXS_FUSED_INST(T)                                                                     \
+  template hipError_t launch_velocity_max<T>(const T*, const double*, int, int, T,          \
+                                             const double*, double*, const RigidConfig&,    \
+                                             hipStream_t);                                   \
+  template hipError_t launch_step_fused<T>(const T*, const T*, const T*, T*, T*,            \
+                                           const double*, const double*, const double*, int, \
+                                           int, int, T, double*, const double*, double*,    \
+                                           const RigidConfig&, hipStream_t);                 \
+  template hipError_t launch_velocity_out<T>(const T*, const T*, T*, T*, const double*, int, \
+                                             int, T, const double*, const RigidConfig&,     \
+                                             hipStream_t);
+AXS_FUSED_INST(double)
+AXS_FUSED_INST(float)
+#undef AXS_FUSED_INST
+
+}  // namespace axs

@@ -1,0 +1,99 @@
+// Fused stencil passes and the device-side step scalars they finalize.
+#pragma once
+
+#include "axs_stepper.hpp"
+
+namespace axs {
+
+__device__ __forceinline__ void next_step_scalars(double* sc, const RigidConfig& cfg, double t) {
+  sc[kScPrefac] = (cfg.T_ramp > 0.0 && t < cfg.T_ramp) ? sin(0.5 * M_PI * t / cfg.T_ramp) : 1.0;
+  sc[kScUpen] = cfg.osc_amp * cos(cfg.osc_omega * t);
+  sc[kScBodyZ] = cfg.body_z0 + cfg.body_disp * sin(cfg.osc_omega * t);
+}
+
+// Scalars of one step derived from max(|u_z| + |u_r|).
+struct DtScalars {
+  double dt, fr, reset;
+};
+
+// dt rule: flow_past_sphere.py:150-153 and spheroid_streaming.py:139-201
+// (period timer reset via numpy.isclose, 3-term min taking the first
+// minimal value, period and tEnd clipping).  Pure: every block that needs
+// dt evaluates it from the same inputs.
+__device__ __forceinline__ DtScalars compute_dt(const double* sc, double m,
+                                                const RigidConfig& cfg) {
+  double fr = sc[kScFreqTimer];
+  double reset = 0.0;
+  if (cfg.period > 0.0 && fabs(fr - cfg.period) <= 1e-8 + 1e-5 * fabs(cfg.period)) {
+    fr = 0.0;
+    reset = 1.0;
+  }
+  double dt = cfg.dt_diffusive;
+  const double cfl_dt = cfg.CFL * cfg.dx / (m + cfg.eps);
+  if (cfl_dt < dt) dt = cfl_dt;
+  if (cfg.dt_cap < dt) dt = cfg.dt_cap;
+  if (cfg.period > 0.0 && fr + dt > cfg.period) dt = cfg.period - fr;
+  if (cfg.t_end > 0.0 && sc[kScT] + dt > cfg.t_end) dt = cfg.t_end - sc[kScT];
+  return DtScalars{dt, fr, reset};
+}
+
+// Stores the step's dt-derived scalars (one thread, during the pass whose
+// blocks read only the slots it does not write).
+__device__ __forceinline__ void store_dt_scalars(double* sc, const DtScalars& d, double m,
+                                                 const RigidConfig& cfg) {
+  sc[kScFreqStep] = d.fr;
+  sc[kScPrefacStep] = sc[kScPrefac];
+  sc[kScUpenStep] = sc[kScUpen];
+  sc[kScAvgReset] = d.reset;
+  sc[kScMaxU] = m;
+  sc[kScDt] = d.dt;
+  sc[kScDtByDx] = d.dt / cfg.dx;
+  sc[kScLamDt] = cfg.brink_lam * d.dt;
+  sc[kScHalfNuDt] = 0.5 * cfg.nu * d.dt;
+  sc[kScNuDt] = cfg.nu * d.dt;
+}
+
+// Force / drag coefficient into the history, then t += dt, it += 1, the
+// period timer, and the next step's ramp factor, U_pen and body centre.
+__device__ __forceinline__ void finalize_step_scalars(double* sc, double* drag, double s,
+                                                      const RigidConfig& cfg) {
+  const long it = (long)sc[kScIt];
+  if (drag && it < cfg.history) drag[it] = (cfg.drag_scale * s) / cfg.drag_div;
+  sc[kScDragSum] = s;
+  const double t = sc[kScT] + sc[kScDt];
+  sc[kScT] = t;
+  sc[kScFreqTimer] = sc[kScFreqStep] + sc[kScDt];
+  sc[kScIt] = double(it + 1);
+  next_step_scalars(sc, cfg, t);
+}
+
+// Velocity maximum slots: kMaxSlots doubles, kSlotStride apart (4 KiB, so
+// they spread over memory channels), holding bit patterns of non-negative doubles; all zero between
+// steps (finalize_step clears them).
+constexpr int kMaxSlots = 64, kSlotStride = 512;
+
+template <typename T>
+hipError_t launch_velocity_max(const T* psi, const double* r, int nr, int nz, T dx,
+                               const double* sc, double* slots, const RigidConfig& cfg,
+                               hipStream_t s);
+// Number of step_fused blocks (= force partials) for a grid.
+int fused_blocks(int nr, int nz);
+// Penalisation through the end of the step: reads psi, chi and the stored
+// vorticity wsrc (kill taper applied on the fly; wz: 2 kw weights, wr: kw
+// weights), writes the new vorticity to wdst (must not alias wsrc), the psi
+// average (if avg) and one force partial per block.
+template <typename T>
+hipError_t launch_step_fused(const T* psi, const T* chi, const T* wsrc, T* wdst, T* avg,
+                             const double* r, const double* wz, const double* wr, int kw, int nr,
+                             int nz, T dx, double* sc, const double* slots, double* partials,
+                             const RigidConfig& cfg, hipStream_t s);
+template <typename T>
+hipError_t launch_velocity_out(const T* psi, const T* chi, T* pz, T* pr, const double* r, int nr,
+                               int nz, T dx, const double* sc, const RigidConfig& cfg,
+                               hipStream_t s);
+// One block: force sum of the partials, then t, it, period timer, the next
+// step's scalars, and cleared maximum slots.
+hipError_t launch_finalize_step(const double* partials, int count, double* slots, double* sc,
+                                double* drag, const RigidConfig& cfg, hipStream_t s);
+
+}  // namespace axs

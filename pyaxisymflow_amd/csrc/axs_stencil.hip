@@ -188,6 +188,9 @@ hipError_t launch_rk2(const T* w_in, T* tmp, T* w_out, RA R, int nr, int nz, Sca
 template hipError_t launch_rk2<double, RRow<double>>(const double*, double*, double*, RRow<double>,
                                                      int, int, Scalar<double>, Scalar<double>,
                                                      double, hipStream_t);
+template hipError_t launch_rk2<float, RRow<float>>(const float*, float*, float*, RRow<float>, int,
+                                                   int, Scalar<float>, Scalar<float>, float,
+                                                   hipStream_t);
 
 // ---------------------------------------------------------------- A11
 // solid_sigma.py:4-29.  Gradient scratch entries the reference never writes

@@ -1,219 +1,92 @@
 // Device-resident timestep of the rigid-body drivers (row A16):
 // examples/FlowPastSphere/flow_past_sphere.py:107-188 (static body, free
-// stream with sine ramp, drag coefficient) and, with an oscillating
-// penalisation velocity, examples/SpheroidStreaming/spheroid_streaming.py:129-246.
+// stream with sine ramp, drag coefficient) and
+// examples/SpheroidStreaming/spheroid_streaming.py:129-246 (oscillating
+// spheroid in quiescent fluid: mask rebuilt every step, penalisation
+// velocity U_0 cos(omega t), dt clipped to the period and to tEnd, psi
+// averaged over each period).
 //
-// One step, in the reference's order:
-//   kill taper (z, then r)                       kill_z_tab / kill_r_tab
-//   psi = Stokes solve(omega)                    dct2 rows, tridiagonal columns, dct3 rows
-//   u = velocity(psi) (+ free stream), max|u|    velocity_freestream
-//   dt = min(diffusive, CFL dx / (max + eps))    finalize_dt (one block)
-//   u <- Brinkman(u); omega += curl(u - u_raw);
-//   drag partial sums                            penalize_curl
-//   Cd, t += dt, it += 1                         finalize_step (one block)
-//   omega <- ENO3(omega, u)                      eno3 (out of place)
-//   omega <- RK2 diffusion                       rk2 stage 1, stage 2 (back home)
-// Every scalar (t, dt, ramp factor, penalisation velocity, drag history)
-// stays in device memory, so a step can be captured once into a hipGraph and
-// replayed.  No memset nodes are recorded (fills are kernels).
-#include "axs_eno3.hpp"
-#include "axs_reduce.hpp"
-#include "axs_stokes.hpp"
+// One step, in the reference's order, as seven passes (a stage is skipped
+// when its feature is off):
+//   taper_dct2_rows   kill taper of the stored field on the fly, DCT-II of
+//                     r * omega along z
+//   tridiag_columns   one radial tridiagonal solve per z-mode
+//   dct3_rows         inverse DCT -> psi
+//   velocity_max      u from psi (+ free stream), max|u| (atomicMax slots)
+//   body_mask         chi = H(phi(t)) of the moving body
+//   step_fused        dt rule, u from psi, Brinkman, taper + curl(u_pen - u),
+//                     force partials, psi average, ENO3, both RK2 stages ->
+//                     the other buffer of the vorticity pair
+//   finalize_step     force record, t, period timer, step count
+// The stored vorticity alternates between the home buffer and work0; one
+// graph is captured per direction, and a run that ends on work0 copies it
+// home, so the home buffer holds the field whenever a run returns.
+// Every scalar (t, dt, ramp factor, penalisation velocity, period timer,
+// force history) stays in device memory, so a step is captured once into a
+// hipGraph and replayed.  No memset nodes are recorded (fills are kernels).
+//
+// Precision: T = double is the plain FP64 path.  T = float stores every
+// field in FP32 and runs the stencils and z-transforms in FP32, while the
+// radial tridiagonal elimination, the mask level set, and all reductions
+// and step scalars stay FP64.
+#include "axs_fused.hpp"
 #include "axs_stepper.hpp"
 
 #include <new>
 
 namespace axs {
 
-template <typename T, class RA>
-hipError_t launch_rk2(const T* w_in, T* tmp, T* w_out, RA R, int nr, int nz, Scalar<T> half_nu_dt,
-                      Scalar<T> nu_dt, T dx, hipStream_t s);
-
 // ------------------------------------------------------------------ kernels
 
-// kill_boundary_vorticity_sine_z with tabulated weights (wz[0..w) left,
-// wz[w..2w) right; the host forms them with the reference's expression).
+// chi = smooth_Heaviside(phi) of the spheroid centred at z = sc[BodyZ]
+// (spheroid_streaming.py:207-212), level set and blend in FP64.
 template <typename T>
-__global__ void kill_z_tab_kernel(T* __restrict__ w, const double* __restrict__ wz, int nr,
-                                  int nz, int width) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nr) return;
-  T* row = w + (int64_t)i * nz;
-  const T left = row[width - 1];
-  for (int j = 0; j < width; ++j) row[j] = T(wz[j]) * left;
-  const T right = row[nz - width];
-  for (int j = 0; j < width; ++j) row[nz - width + j] = T(wz[width + j]) * right;
-}
-
-// kill_boundary_vorticity_sine_r with tabulated weights wr[0..w) for rows
-// nr-w..nr-1; then the axis row is zeroed.
-template <typename T>
-__global__ void kill_r_tab_kernel(T* __restrict__ w, const double* __restrict__ wr, int nr,
-                                  int nz, int width) {
+__global__ void body_mask_kernel(T* __restrict__ chi, const double* __restrict__ r,
+                                 const double* __restrict__ z, int nr, int nz,
+                                 const double* __restrict__ sc, RigidConfig cfg) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
   if (j >= nz) return;
-  const T src = w[(int64_t)(nr - width) * nz + j];
-  for (int q = 0; q < width; ++q) w[(int64_t)(nr - width + q) * nz + j] = T(wr[q]) * src;
-  w[j] = T(0);
+  const double zc = sc[kScBodyZ];
+  const double dz = z[j] - zc, dr = r[i] - cfg.body_r0;
+  const double phi = cfg.body_radius - sqrt(dz * dz / cfg.body_aspect2 + dr * dr);
+  const double w = cfg.moll_zone;
+  double h = (phi >= w) ? 1.0 : 0.0;
+  if (fabs(phi) < w) h = h + 0.5 * (1.0 + phi / w + sin(M_PI * phi / w) / M_PI);
+  chi[(int64_t)i * nz + j] = T(h);
 }
 
-template <typename T>
-__device__ __forceinline__ double block_max(double v) {
-  __shared__ double sm[256];
-  sm[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
-    __syncthreads();
-  }
-  return sm[0];
-}
-
-__device__ __forceinline__ double block_sum(double v) {
-  __shared__ double sm[256];
-  sm[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + off];
-    __syncthreads();
-  }
-  return sm[0];
-}
-
-// compute_velocity_from_psi_unb + u_z += U0 * prefac + per-block max(|u_z|+|u_r|).
-// Block = 256 columns of one row; partial index = row * gridDim.x + blockIdx.x.
-template <typename T>
-__global__ void __launch_bounds__(256) velocity_freestream_kernel(
-    T* __restrict__ u_z, T* __restrict__ u_r, const T* __restrict__ psi,
-    const double* __restrict__ r, int nr, int nz, T dx, const double* __restrict__ sc,
-    double U0, double* __restrict__ partials) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  double m = -1.0;
-  if (j < nz) {
-    const int64_t c = (int64_t)i * nz + j;
-    const T two_dx = T(2) * dx;
-    const T ri = T(r[i]);
-    T uz;
-    if (i == 0)
-      uz = (-psi[c + 2 * nz] + T(4) * psi[c + nz] - T(3) * psi[c]) / two_dx / ri;
-    else if (i == nr - 1)
-      uz = (psi[c - 2 * nz] - T(4) * psi[c - nz] + T(3) * psi[c]) / two_dx / ri;
-    else
-      uz = (psi[c + nz] - psi[c - nz]) / two_dx / ri;
-    T ur;
-    if (j == 0)
-      ur = -(-psi[c + 2] + T(4) * psi[c + 1] - T(3) * psi[c]) / two_dx / ri;
-    else if (j == nz - 1)
-      ur = -(psi[c - 2] - T(4) * psi[c - 1] + T(3) * psi[c]) / two_dx / ri;
-    else
-      ur = -(psi[c + 1] - psi[c - 1]) / two_dx / ri;
-    uz = uz + T(U0 * sc[kScPrefac]);
-    u_z[c] = uz;
-    u_r[c] = ur;
-    m = double(fabs(uz) + fabs(ur));
-  }
-  const double b = block_max<T>(m);
-  if (threadIdx.x == 0) partials[(int64_t)i * gridDim.x + blockIdx.x] = b;
-}
-
-// dt rule (flow_past_sphere.py:150-153) and the derived step scalars.
-__global__ void __launch_bounds__(256) finalize_dt_kernel(const double* __restrict__ partials,
-                                                          int count, double* __restrict__ sc,
-                                                          RigidConfig cfg) {
-  double m = -1.0;
-  for (int q = threadIdx.x; q < count; q += 256) m = fmax(m, partials[q]);
-  m = block_max<double>(m);
-  if (threadIdx.x == 0) {
-    const double cfl_dt = cfg.CFL * cfg.dx / (m + cfg.eps);
-    const double dt = (cfl_dt < cfg.dt_diffusive) ? cfl_dt : cfg.dt_diffusive;
-    sc[kScMaxU] = m;
-    sc[kScDt] = dt;
-    sc[kScDtByDx] = dt / cfg.dx;
-    sc[kScLamDt] = cfg.brink_lam * dt;
-    sc[kScHalfNuDt] = 0.5 * cfg.nu * dt;
-    sc[kScNuDt] = cfg.nu * dt;
-  }
-}
-
-// brinkmann_penalize (U_z = sc[U_pen], U_r = 0) into (pz, pr); omega's
-// interior += curl(u_pen - u_raw) (compute_vorticity_from_velocity_unb on
-// the difference); per-block partial of sum(r * chi * u_pen_z).
-template <typename T>
-__global__ void __launch_bounds__(256) penalize_curl_kernel(
-    const T* __restrict__ uz, const T* __restrict__ ur, const T* __restrict__ chi,
-    T* __restrict__ pz, T* __restrict__ pr, T* __restrict__ w, const double* __restrict__ r,
-    int nr, int nz, T dx, const double* __restrict__ sc, double* __restrict__ partials) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  double acc = 0.0;
-  if (j < nz) {
-    const T lam_dt = T(sc[kScLamDt]);
-    const T U = T(sc[kScUpen]);
-    const int64_t c = (int64_t)i * nz + j;
-    auto pen = [&](int64_t q, T u, T Uv) {
-      const T ldc = lam_dt * chi[q];
-      return (u + ldc * Uv) / (T(1) + ldc);
-    };
-    const T uzc = uz[c], urc = ur[c];
-    const T pzc = pen(c, uzc, U);
-    pz[c] = pzc;
-    pr[c] = pen(c, urc, T(0));
-    if (i >= 1 && i <= nr - 2 && j >= 1 && j <= nz - 2) {
-      const T two_dx = T(2) * dx;
-      const T dr_p = pen(c + 1, ur[c + 1], T(0)) - ur[c + 1];
-      const T dr_m = pen(c - 1, ur[c - 1], T(0)) - ur[c - 1];
-      const T dz_p = pen(c + nz, uz[c + nz], U) - uz[c + nz];
-      const T dz_m = pen(c - nz, uz[c - nz], U) - uz[c - nz];
-      const T pv = (dr_p - dr_m) / two_dx - (dz_p - dz_m) / two_dx;
-      w[c] = w[c] + pv;
-    }
-    acc = (r[i] * double(chi[c])) * double(pzc);
-  }
-  const double b = block_sum(acc);
-  if (threadIdx.x == 0) partials[(int64_t)i * gridDim.x + blockIdx.x] = b;
-}
-
-// Drag coefficient into the history, then t += dt, it += 1, and the next
-// step's ramp factor and penalisation velocity.
-__global__ void __launch_bounds__(256) finalize_step_kernel(const double* __restrict__ partials,
-                                                            int count, double* __restrict__ sc,
-                                                            double* __restrict__ drag,
-                                                            RigidConfig cfg) {
-  double s = 0.0;
-  for (int q = threadIdx.x; q < count; q += 256) s += partials[q];
-  s = block_sum(s);
-  if (threadIdx.x == 0) {
-    const long it = (long)sc[kScIt];
-    if (drag && it < cfg.history) drag[it] = (cfg.drag_scale * s) / cfg.drag_div;
-    sc[kScDragSum] = s;
-    const double t = sc[kScT] + sc[kScDt];
-    sc[kScT] = t;
-    sc[kScIt] = double(it + 1);
-    sc[kScPrefac] = (cfg.T_ramp > 0.0 && t < cfg.T_ramp) ? sin(0.5 * M_PI * t / cfg.T_ramp) : 1.0;
-    sc[kScUpen] = cfg.osc_amp * cos(cfg.osc_omega * t);
-  }
-}
-
-// Initial scalars for t = sc[T] (prefac, U_pen) without advancing.
-__global__ void init_scalars_kernel(double* __restrict__ sc, RigidConfig cfg) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const double t = sc[kScT];
-    sc[kScPrefac] = (cfg.T_ramp > 0.0 && t < cfg.T_ramp) ? sin(0.5 * M_PI * t / cfg.T_ramp) : 1.0;
-    sc[kScUpen] = cfg.osc_amp * cos(cfg.osc_omega * t);
-  }
+// Step scalars for the time now in sc[T] (without advancing); clears the
+// velocity maximum slots.
+__global__ void init_scalars_kernel(double* __restrict__ sc, double* __restrict__ slots,
+                                    RigidConfig cfg) {
+  if (threadIdx.x < kMaxSlots) slots[threadIdx.x * kSlotStride] = 0.0;
+  if (threadIdx.x == 0) next_step_scalars(sc, cfg, sc[kScT]);
 }
 
 // ------------------------------------------------------------------ stepper
 
-RigidStepper::~RigidStepper() {
-  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-  if (graph) (void)hipGraphDestroy(graph);
+const char* const kRigidStageNames[kRigidStages] = {
+    "taper_dct2_rows", "tridiag_columns", "dct3_rows",    "velocity_max",
+    "body_mask",       "step_fused",      "finalize_step"};
+
+template <typename T>
+__global__ void copy_field_kernel(T* __restrict__ dst, const T* __restrict__ src, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = src[k];
+}
+
+RigidStepperBase::~RigidStepperBase() {
+  for (int q = 0; q < 2; ++q) {
+    if (graph_exec[q]) (void)hipGraphExecDestroy(graph_exec[q]);
+    if (graph[q]) (void)hipGraphDestroy(graph[q]);
+  }
   if (partials) (void)hipFree(partials);
+  if (slots) (void)hipFree(slots);
   delete plan;
 }
 
-hipError_t RigidStepper::init(const RigidConfig& c, const RigidBuffers& b) {
+hipError_t RigidStepperBase::init(const RigidConfig& c, const RigidBuffers& b) {
   cfg = c;
   buf = b;
   plan = new (std::nothrow) StokesPlan();
@@ -232,66 +105,77 @@ hipError_t RigidStepper::init(const RigidConfig& c, const RigidBuffers& b) {
   hipError_t e = plan->init(cfg.nr, cfg.nz, cfg.dx);
   if (e != hipSuccess) return e;
   blocks_x = ceil_div(cfg.nz, 256);
-  npartials = blocks_x * cfg.nr;
-  e = hipMalloc(reinterpret_cast<void**>(&partials), sizeof(double) * (size_t)npartials * 2);
+  npartials = fused_blocks(cfg.nr, cfg.nz);
+  e = hipMalloc(reinterpret_cast<void**>(&partials), sizeof(double) * (size_t)npartials);
   if (e != hipSuccess) return e;
-  init_scalars_kernel<<<1, 64>>>(buf.scalars, cfg);
+  e = hipMalloc(reinterpret_cast<void**>(&slots), sizeof(double) * kMaxSlots * kSlotStride);
+  if (e != hipSuccess) return e;
+  init_scalars_kernel<<<1, 64>>>(buf.scalars, slots, cfg);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   return hipDeviceSynchronize();
 }
 
-// Stage table of one step (also used to time each stage on its own).
-const char* const kRigidStageNames[kRigidStages] = {
-    "kill_taper", "dct2_rows", "tridiag_columns", "dct3_rows", "velocity_freestream",
-    "finalize_dt", "penalize_curl", "finalize_step", "eno3", "rk2_diffusion"};
+bool RigidStepperBase::stage_enabled(int stage) const {
+  if (stage == 4) return cfg.moving_body != 0;
+  return stage >= 0 && stage < kRigidStages;
+}
 
-hipError_t RigidStepper::enqueue_stage(int stage, hipStream_t s) {
+template <typename T>
+hipError_t RigidStepper<T>::enqueue_stage(int stage, hipStream_t s) {
   const int nr = cfg.nr, nz = cfg.nz, w = cfg.kill_width;
   double* sc = buf.scalars;
-  double* part_u = partials;
-  double* part_d = partials + npartials;
-  const dim3 g2(blocks_x, nr);
+  T* home = static_cast<T*>(buf.vorticity);
+  T* other = static_cast<T*>(buf.work0);
+  T* cur = parity ? other : home;
+  T* nxt = parity ? home : other;
+  T* psi = static_cast<T*>(buf.psi);
+  T* chi = static_cast<T*>(buf.char_func);
+  T* spec = reinterpret_cast<T*>(plan->spec);
+  if (!stage_enabled(stage)) return hipSuccess;
   switch (stage) {
     case 0:
-      kill_z_tab_kernel<double><<<ceil_div(nr, 64), 64, 0, s>>>(buf.vorticity, buf.kill_wz, nr, nz, w);
-      kill_r_tab_kernel<double><<<ceil_div(nz, 256), 256, 0, s>>>(buf.vorticity, buf.kill_wr, nr, nz, w);
-      return hipGetLastError();
+      return launch_taper_dct2_rows<T>(*plan, cur, nullptr, spec, buf.kill_wz, buf.kill_wr, w, s);
     case 1:
-      return launch_dct2_rows<double>(*plan, buf.vorticity, plan->spec, plan->rvec, nr, s);
+      return launch_tridiag<T>(*plan, spec, s);
     case 2:
-      return launch_tridiag<double>(*plan, plan->spec, s);
+      return launch_dct3_rows<T>(*plan, spec, psi, nr, s);
     case 3:
-      return launch_dct3_rows<double>(*plan, plan->spec, buf.psi, nr, s);
+      return launch_velocity_max<T>(psi, buf.r, nr, nz, T(cfg.dx), sc, slots, cfg, s);
     case 4:
-      velocity_freestream_kernel<double><<<g2, 256, 0, s>>>(buf.u_z, buf.u_r, buf.psi, buf.r, nr,
-                                                            nz, cfg.dx, sc, cfg.U0, part_u);
+      body_mask_kernel<T><<<dim3(blocks_x, nr), 256, 0, s>>>(chi, buf.r, buf.z, nr, nz, sc, cfg);
       return hipGetLastError();
     case 5:
-      finalize_dt_kernel<<<1, 256, 0, s>>>(part_u, npartials, sc, cfg);
-      return hipGetLastError();
+      return launch_step_fused<T>(psi, chi, cur, nxt,
+                                  cfg.average_psi ? static_cast<T*>(buf.avg_psi) : nullptr,
+                                  buf.r, buf.kill_wz, buf.kill_wr, w, nr, nz, T(cfg.dx), sc, slots,
+                                  partials, cfg, s);
     case 6:
-      penalize_curl_kernel<double><<<g2, 256, 0, s>>>(buf.u_z, buf.u_r, buf.char_func,
-                                                      buf.u_pen_z, buf.u_pen_r, buf.vorticity,
-                                                      buf.r, nr, nz, cfg.dx, sc, part_d);
-      return hipGetLastError();
-    case 7:
-      finalize_step_kernel<<<1, 256, 0, s>>>(part_d, npartials, sc, buf.drag, cfg);
-      return hipGetLastError();
-    case 8:
-      return launch_eno3_conservative<double>(buf.vorticity, buf.u_pen_z, buf.u_pen_r, buf.work0,
-                                              nr, nz, Scalar<double>{0.0, sc + kScDtByDx}, -1.0, s);
-    case 9:
-      return launch_rk2<double, RRow<double>>(buf.work0, buf.work1, buf.vorticity,
-                                              RRow<double>{buf.r}, nr, nz,
-                                              Scalar<double>{0.0, sc + kScHalfNuDt},
-                                              Scalar<double>{0.0, sc + kScNuDt}, cfg.dx, s);
+      return launch_finalize_step(partials, npartials, slots, sc, buf.drag, cfg, s);
     default:
       return hipErrorInvalidValue;
   }
 }
 
-hipError_t RigidStepper::enqueue_step(hipStream_t s) {
+template <typename T>
+hipError_t RigidStepper<T>::settle(hipStream_t s) {
+  if (!parity) return hipSuccess;
+  const int64_t n = (int64_t)cfg.nr * cfg.nz;
+  copy_field_kernel<T><<<ceil_div(n, 256), 256, 0, s>>>(static_cast<T*>(buf.vorticity),
+                                                        static_cast<const T*>(buf.work0), n);
+  parity = 0;
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t RigidStepper<T>::velocity_out(hipStream_t s) {
+  return launch_velocity_out<T>(static_cast<const T*>(buf.psi),
+                                static_cast<const T*>(buf.char_func),
+                                static_cast<T*>(buf.u_pen_z), static_cast<T*>(buf.u_pen_r), buf.r,
+                                cfg.nr, cfg.nz, T(cfg.dx), buf.scalars, cfg, s);
+}
+
+hipError_t RigidStepperBase::enqueue_step(hipStream_t s) {
   for (int q = 0; q < kRigidStages; ++q) {
     hipError_t e = enqueue_stage(q, s);
     if (e != hipSuccess) return e;
@@ -299,46 +183,60 @@ hipError_t RigidStepper::enqueue_step(hipStream_t s) {
   return hipSuccess;
 }
 
-hipError_t RigidStepper::run(int nsteps, bool use_graph, hipStream_t s) {
+hipError_t RigidStepperBase::run(int nsteps, bool use_graph, hipStream_t s) {
   if (!use_graph) {
     for (int q = 0; q < nsteps; ++q) {
       hipError_t e = enqueue_step(s);
       if (e != hipSuccess) return e;
+      parity ^= 1;
     }
-    return hipSuccess;
+    return settle(s);
   }
-  if (!graph_exec || graph_stream != s) {
-    if (graph_exec) {
-      (void)hipGraphExecDestroy(graph_exec);
-      graph_exec = nullptr;
+  if (graph_stream != s) {
+    for (int q = 0; q < 2; ++q) {
+      if (graph_exec[q]) (void)hipGraphExecDestroy(graph_exec[q]);
+      if (graph[q]) (void)hipGraphDestroy(graph[q]);
+      graph_exec[q] = nullptr;
+      graph[q] = nullptr;
     }
-    if (graph) {
-      (void)hipGraphDestroy(graph);
-      graph = nullptr;
+    const int saved = parity;
+    for (int q = 0; q < 2; ++q) {
+      parity = q;
+      hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) {
+        parity = saved;
+        return e;
+      }
+      hipError_t le = enqueue_step(s);
+      e = hipStreamEndCapture(s, &graph[q]);
+      if (le == hipSuccess && e == hipSuccess)
+        e = hipGraphInstantiate(&graph_exec[q], graph[q], nullptr, nullptr, 0);
+      if (le != hipSuccess || e != hipSuccess) {
+        parity = saved;
+        return le != hipSuccess ? le : e;
+      }
     }
-    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return e;
-    hipError_t le = enqueue_step(s);
-    e = hipStreamEndCapture(s, &graph);
-    if (le != hipSuccess) return le;
-    if (e != hipSuccess) return e;
-    e = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) return e;
+    parity = saved;
     graph_stream = s;
   }
   for (int q = 0; q < nsteps; ++q) {
-    hipError_t e = hipGraphLaunch(graph_exec, s);
+    hipError_t e = hipGraphLaunch(graph_exec[parity], s);
     if (e != hipSuccess) return e;
+    parity ^= 1;
   }
-  return hipSuccess;
+  return settle(s);
 }
+
+template struct RigidStepper<double>;
+template struct RigidStepper<float>;
 
 }  // namespace axs
 
 using namespace axs;
 
 struct axs_rigid_stepper {
-  RigidStepper impl;
+  RigidStepperBase* impl = nullptr;
+  ~axs_rigid_stepper() { delete impl; }
 };
 
 extern "C" {
@@ -352,20 +250,37 @@ axs_status axs_rigid_stepper_create(const axs_rigid_config* cfg, const axs_rigid
                     cfg->kill_width < cfg->nr,
                 "bad kill_width for the grid");
   AXS_CHECK_ARG(cfg->dx > 0 && cfg->nu > 0 && cfg->CFL > 0, "dx, nu and CFL must be positive");
+  AXS_CHECK_ARG(cfg->precision == AXS_F64 || cfg->precision == AXS_F32, "bad precision");
+  AXS_CHECK_ARG(!cfg->moving_body || (cfg->body_aspect2 > 0 && cfg->moll_zone > 0),
+                "moving body needs aspect^2 > 0 and moll_zone > 0");
   AXS_CHECK_ARG(bufs->vorticity && bufs->psi && bufs->u_z && bufs->u_r && bufs->u_pen_z &&
                     bufs->u_pen_r && bufs->char_func && bufs->work0 && bufs->work1 &&
                     bufs->scalars && bufs->r && bufs->kill_wz && bufs->kill_wr,
                 "null buffer");
+  AXS_CHECK_ARG(!cfg->average_psi || bufs->avg_psi, "average_psi needs the avg_psi buffer");
+  AXS_CHECK_ARG(!cfg->moving_body || bufs->z, "moving body needs the z table");
   axs_rigid_stepper* st = new (std::nothrow) axs_rigid_stepper();
   if (!st) {
     set_error("out of host memory");
     return AXS_ERR_ALLOC;
   }
-  RigidConfig c;
+  if (cfg->precision == AXS_F64)
+    st->impl = new (std::nothrow) RigidStepper<double>();
+  else
+    st->impl = new (std::nothrow) RigidStepper<float>();
+  if (!st->impl) {
+    delete st;
+    set_error("out of host memory");
+    return AXS_ERR_ALLOC;
+  }
+  RigidConfig c{};
   c.nr = cfg->nr;
   c.nz = cfg->nz;
   c.kill_width = cfg->kill_width;
   c.history = cfg->history;
+  c.precision = cfg->precision;
+  c.moving_body = cfg->moving_body;
+  c.average_psi = cfg->average_psi;
   c.dx = cfg->dx;
   c.nu = cfg->nu;
   c.brink_lam = cfg->brink_lam;
@@ -378,11 +293,20 @@ axs_status axs_rigid_stepper_create(const axs_rigid_config* cfg, const axs_rigid
   c.osc_omega = cfg->osc_omega;
   c.drag_scale = cfg->drag_scale;
   c.drag_div = cfg->drag_div;
-  RigidBuffers b{bufs->vorticity, bufs->psi,     bufs->u_z,    bufs->u_r,
-                 bufs->u_pen_z,   bufs->u_pen_r, bufs->char_func, bufs->work0,
-                 bufs->work1,     bufs->scalars, bufs->drag,   bufs->r,
-                 bufs->kill_wz,   bufs->kill_wr};
-  hipError_t e = st->impl.init(c, b);
+  c.dt_cap = cfg->dt_cap;
+  c.period = cfg->period;
+  c.t_end = cfg->t_end;
+  c.body_z0 = cfg->body_z0;
+  c.body_r0 = cfg->body_r0;
+  c.body_aspect2 = cfg->body_aspect2;
+  c.body_radius = cfg->body_radius;
+  c.body_disp = cfg->body_disp;
+  c.moll_zone = cfg->moll_zone;
+  RigidBuffers b{bufs->vorticity, bufs->psi,    bufs->u_z,     bufs->u_r,    bufs->u_pen_z,
+                 bufs->u_pen_r,   bufs->char_func, bufs->work0, bufs->work1,  bufs->avg_psi,
+                 bufs->scalars,   bufs->drag,   bufs->r,       bufs->z,      bufs->kill_wz,
+                 bufs->kill_wr};
+  hipError_t e = st->impl->init(c, b);
   if (e != hipSuccess) {
     set_error(std::string("rigid stepper: ") + hipGetErrorString(e));
     delete st;
@@ -400,7 +324,8 @@ axs_status axs_rigid_stepper_destroy(axs_rigid_stepper* st) {
 axs_status axs_rigid_stepper_run(axs_rigid_stepper* st, int nsteps, int use_graph, void* stream) {
   AXS_CHECK_ARG(st, "null stepper");
   AXS_CHECK_ARG(nsteps >= 0, "negative step count");
-  AXS_HIP(st->impl.run(nsteps, use_graph != 0, as_stream(stream)));
+  AXS_CHECK_ARG(!use_graph || stream, "graph capture needs a non-null stream");
+  AXS_HIP(st->impl->run(nsteps, use_graph != 0, as_stream(stream)));
   return AXS_OK;
 }
 
@@ -414,13 +339,20 @@ axs_status axs_rigid_stepper_run_stage(axs_rigid_stepper* st, int stage, int rep
   AXS_CHECK_ARG(st, "null stepper");
   AXS_CHECK_ARG(stage >= 0 && stage < kRigidStages, "bad stage");
   AXS_CHECK_ARG(reps >= 0, "negative repetition count");
-  for (int q = 0; q < reps; ++q) AXS_HIP(st->impl.enqueue_stage(stage, as_stream(stream)));
+  for (int q = 0; q < reps; ++q) AXS_HIP(st->impl->enqueue_stage(stage, as_stream(stream)));
+  return AXS_OK;
+}
+
+axs_status axs_rigid_stepper_velocity(axs_rigid_stepper* st, void* stream) {
+  AXS_CHECK_ARG(st, "null stepper");
+  AXS_HIP(st->impl->velocity_out(as_stream(stream)));
   return AXS_OK;
 }
 
 axs_status axs_rigid_stepper_reset_scalars(axs_rigid_stepper* st, void* stream) {
   AXS_CHECK_ARG(st, "null stepper");
-  init_scalars_kernel<<<1, 64, 0, as_stream(stream)>>>(st->impl.buf.scalars, st->impl.cfg);
+  init_scalars_kernel<<<1, 64, 0, as_stream(stream)>>>(st->impl->buf.scalars, st->impl->slots,
+                                                       st->impl->cfg);
   AXS_LAUNCH_CHECK("init_scalars");
   return AXS_OK;
 }

@@ -7,7 +7,7 @@ namespace axs {
 
 // Device scalar slots (double array of kScCount entries).
 enum : int {
-  kScT = 0,       // time
+  kScT = 0,       // time at the start of the current step
   kScDt,          // dt of the current step
   kScDtByDx,      // dt / dx
   kScLamDt,       // brink_lam * dt
@@ -16,44 +16,72 @@ enum : int {
   kScPrefac,      // free-stream ramp factor for the current step
   kScUpen,        // penalisation velocity U_z(t) for the current step
   kScMaxU,        // max(|u_z| + |u_r|)
-  kScDragSum,     // sum(r * chi * u_pen_z)
+  kScDragSum,     // sum(r * chi * (u_pen_z - U_pen))
   kScIt,          // step counter
+  kScFreqTimer,   // time since the last period boundary
+  kScAvgReset,    // 1 when the psi average restarts this step
+  kScBodyZ,       // centre of the moving body for the current step
+  kScFreqStep,    // period timer after this step's reset test (before + dt)
+  kScPrefacStep,  // ramp factor the last completed step used
+  kScUpenStep,    // penalisation velocity the last completed step used
   kScCount
 };
 
+static_assert(kScCount <= AXS_RIGID_SCALARS, "scalar slots exceed the documented array size");
+
 struct RigidConfig {
   int nr, nz, kill_width, history;
+  int precision, moving_body, average_psi, pad;
   double dx, nu, brink_lam, U0, T_ramp, CFL, eps, dt_diffusive;
   double osc_amp, osc_omega, drag_scale, drag_div;
+  double dt_cap, period, t_end;
+  double body_z0, body_r0, body_aspect2, body_radius, body_disp, moll_zone;
 };
 
 struct RigidBuffers {
-  double *vorticity, *psi, *u_z, *u_r, *u_pen_z, *u_pen_r, *char_func, *work0, *work1;
+  void *vorticity, *psi, *u_z, *u_r, *u_pen_z, *u_pen_r, *char_func, *work0, *work1, *avg_psi;
   double* scalars;
   double* drag;
   const double* r;
+  const double* z;
   const double* kill_wz;
   const double* kill_wr;
 };
 
-constexpr int kRigidStages = 10;
+constexpr int kRigidStages = 7;
 extern const char* const kRigidStageNames[kRigidStages];
 
-struct RigidStepper {
+struct RigidStepperBase {
   RigidConfig cfg{};
   RigidBuffers buf{};
   StokesPlan* plan = nullptr;
-  double* partials = nullptr;
+  double* partials = nullptr;  // force partial sums of the penalize blocks
+  double* slots = nullptr;     // velocity maximum slots
   int blocks_x = 0, npartials = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
+  // The step reads the stored vorticity from one buffer of the pair
+  // (vorticity, work0) and writes the other; parity says which holds it.
+  int parity = 0;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   hipStream_t graph_stream = nullptr;
 
   hipError_t init(const RigidConfig& c, const RigidBuffers& b);
-  hipError_t enqueue_stage(int stage, hipStream_t s);
+  virtual hipError_t enqueue_stage(int stage, hipStream_t s) = 0;
+  bool stage_enabled(int stage) const;
   hipError_t enqueue_step(hipStream_t s);
   hipError_t run(int nsteps, bool use_graph, hipStream_t s);
-  ~RigidStepper();
+  // Brings the current vorticity back to the home buffer (after odd counts).
+  virtual hipError_t settle(hipStream_t s) = 0;
+  // u_pen of the last step into the u_pen buffers (on demand).
+  virtual hipError_t velocity_out(hipStream_t s) = 0;
+  virtual ~RigidStepperBase();
+};
+
+template <typename T>
+struct RigidStepper : RigidStepperBase {
+  hipError_t enqueue_stage(int stage, hipStream_t s) override;
+  hipError_t settle(hipStream_t s) override;
+  hipError_t velocity_out(hipStream_t s) override;
 };
 
 }  // namespace axs

@@ -93,14 +93,21 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
     h_lam[k] = (4.0 / (dx * dx)) * s * s;
   }
   // Segment layout of the column pass and c'_{s-1} at every segment start.
-  tri_L = (nr <= 256) ? 8 : 16;
+  // Segments of L = 8 rows (16 for very tall grids) keep the per-thread
+  // registers small; K modes per workgroup so that K * P <= 1024 and, when
+  // the grid allows, at least 512 workgroups are in flight.
+  tri_L = 8;
   tri_P = (nr + tri_L - 1) / tri_L;
-  while (tri_P > 128) {  // keep the workgroup within 1024 threads
+  while (tri_P > 256) {
     tri_L *= 2;
     tri_P = (nr + tri_L - 1) / tri_L;
   }
   tri_K = 1;
-  while (tri_K * 2 * tri_P <= 1024 && tri_K < 64) tri_K *= 2;
+  while (tri_K * 2 * tri_P <= 1024 && tri_K < 64 && nz / (tri_K * 2) >= 512) tri_K *= 2;
+  if (tri_K < 2 && 2 * tri_P <= 1024) tri_K = 2;
+  // keep the column kernel's LDS (scan buffers + coefficients) within 64 KiB
+  while (tri_K > 1 && sizeof(double) * (4 * tri_K * tri_P + 3 * nr) > 65536) tri_K /= 2;
+  if (sizeof(double) * (4 * tri_K * tri_P + 3 * nr) > 65536) return hipErrorInvalidValue;
   std::vector<double> h_cseg((size_t)tri_P * nz, 0.0);
   for (int k = 0; k < nz; ++k) {
     double cp = 0.0;
@@ -127,46 +134,116 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
 // ------------------------------------------------------------------ kernels
 
 // Row pass: x_j = scale_i * src[i, j]  ->  X = DCT-II(x)  ->  dst[i, :].
+// The row is read coalesced and scattered into the packed LDS sequence; the
+// post-processing twiddles are fetched before the FFT starts.
 template <typename T, int PER>
 __global__ void __launch_bounds__(256) dct2_rows_kernel(
     const T* __restrict__ src, T* __restrict__ dst, const double* __restrict__ rscale,
     int nz, FftPlanArgs fp, const Cplx<double>* __restrict__ tw,
     const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2) {
   extern __shared__ unsigned char smem_raw[];
+  const int M = fp.M;
   Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int i = blockIdx.x;
-  const int M = fp.M;
   const T* row = src + (int64_t)i * nz;
   const T s = rscale ? T(rscale[i]) : T(1);
-  for (int m = threadIdx.x; m < M; m += blockDim.x) {
-    buf[m] = {s * row[dct_src_index(2 * m, nz)], s * row[dct_src_index(2 * m + 1, nz)]};
+  constexpr int NJ = 4 * PER;  // row entries per thread (4 * PER * blockDim >= nz)
+  T v[NJ];
+#pragma unroll
+  for (int q = 0; q < NJ; ++q) {
+    const int j = threadIdx.x + q * blockDim.x;
+    if (j < nz) v[q] = row[j];
+  }
+  PostTwiddles<PER + 1> pt;
+  dct2_post_twiddles<PER + 1>(pt, M, e1, e2);
+#pragma unroll
+  for (int q = 0; q < NJ; ++q) {
+    const int j = threadIdx.x + q * blockDim.x;
+    if (j < nz) packed_component(buf, dct_dst_index(j, nz)) = s * v[q];
   }
   __syncthreads();
   fft_lds<T, PER>(buf, fp, tw, -1);
   T* out = dst + (int64_t)i * nz;
-  dct2_post<T>(buf, M, e1, e2, [&](int k, T v) { out[k] = v; });
+  dct2_post<T, PER + 1>(buf, M, pt, [&](int k, T val) { out[k] = val; });
 }
 
-// Row pass: x = DCT-II^{-1}(src[i, :]) -> dst[i, :].
+// Fused first pass of a step: the reference's kill taper
+// (kill_boundary_vorticity_sine.py:4-27: z ends of every row, then rows
+// nr-w..nr-1 from row nr-w, then row 0 = 0) applied on the fly to the
+// untouched field `src`; the tapered row goes to `tapered` (when not null)
+// and r_i * row goes through the DCT-II into `dst`.  Reading
+// src and writing a separate buffer keeps rows independent (the top rows
+// read row nr-w).  wz: 2w weights (left block, right block), wr: w weights.
+template <typename T, int PER>
+__global__ void __launch_bounds__(256) taper_dct2_rows_kernel(
+    const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
+    const double* __restrict__ rscale, const double* __restrict__ wz,
+    const double* __restrict__ wr, int nr, int nz, int w, FftPlanArgs fp,
+    const Cplx<double>* __restrict__ tw, const Cplx<double>* __restrict__ e1,
+    const Cplx<double>* __restrict__ e2) {
+  extern __shared__ unsigned char smem_raw[];
+  const int M = fp.M;
+  Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
+  const int i = blockIdx.x;
+  const int srow = (i >= nr - w) ? nr - w : i;
+  const T* row = src + (int64_t)srow * nz;
+  constexpr int NJ = 4 * PER;
+  T v[NJ];
+#pragma unroll
+  for (int q = 0; q < NJ; ++q) {
+    const int j = threadIdx.x + q * blockDim.x;
+    if (j < nz) v[q] = row[j];
+  }
+  const T left = row[w - 1], right = row[nz - w];
+  PostTwiddles<PER + 1> pt;
+  dct2_post_twiddles<PER + 1>(pt, M, e1, e2);
+  const T rs = T(rscale[i]);
+  const T rowf = (i == 0) ? T(0) : ((i >= nr - w) ? T(wr[i - (nr - w)]) : T(1));
+  T* out_t = tapered ? tapered + (int64_t)i * nz : nullptr;
+#pragma unroll
+  for (int q = 0; q < NJ; ++q) {
+    const int j = threadIdx.x + q * blockDim.x;
+    if (j < nz) {
+      T t = v[q];
+      if (j < w)
+        t = T(wz[j]) * left;
+      else if (j >= nz - w)
+        t = T(wz[w + (j - (nz - w))]) * right;
+      if (i >= nr - w) t = rowf * t;
+      if (i == 0) t = T(0);
+      if (out_t) out_t[j] = t;
+      packed_component(buf, dct_dst_index(j, nz)) = rs * t;
+    }
+  }
+  __syncthreads();
+  fft_lds<T, PER>(buf, fp, tw, -1);
+  T* out = dst + (int64_t)i * nz;
+  dct2_post<T, PER + 1>(buf, M, pt, [&](int k, T val) { out[k] = val; });
+}
+
+// Row pass: x = DCT-II^{-1}(src[i, :]) -> dst[i, :], stored coalesced by
+// gathering each column's value from the packed LDS sequence.
 template <typename T, int PER>
 __global__ void __launch_bounds__(256) dct3_rows_kernel(
     const T* __restrict__ src, T* __restrict__ dst, int nz, FftPlanArgs fp,
     const Cplx<double>* __restrict__ tw, const Cplx<double>* __restrict__ e1,
     const Cplx<double>* __restrict__ e2) {
   extern __shared__ unsigned char smem_raw[];
+  const int M = fp.M;
   Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int i = blockIdx.x;
-  const int M = fp.M;
   const T* row = src + (int64_t)i * nz;
-  dct3_pre<T>(buf, M, e1, e2, [&](int k) { return row[k]; });
+  constexpr int NQ = 2 * PER;
+  dct3_pre<T, NQ>(buf, M, e1, e2, [&](int k) { return row[k]; });
   __syncthreads();
   fft_lds<T, PER>(buf, fp, tw, +1);
   const T invN = T(1) / T(nz);
   T* out = dst + (int64_t)i * nz;
-  for (int m = threadIdx.x; m < M; m += blockDim.x) {
-    const Cplx<T> z = buf[m];
-    out[dct_src_index(2 * m, nz)] = z.x * invN;
-    out[dct_src_index(2 * m + 1, nz)] = z.y * invN;
+  constexpr int NJ = 4 * PER;
+#pragma unroll
+  for (int q = 0; q < NJ; ++q) {
+    const int j = threadIdx.x + q * blockDim.x;
+    if (j < nz) out[j] = packed_component(buf, dct_dst_index(j, nz)) * invN;
   }
 }
 
@@ -177,49 +254,66 @@ __global__ void __launch_bounds__(256) dct3_rows_kernel(
 // k in registers.  Thomas' elimination is split into segment-local
 // recurrences joined by prefix scans of affine maps across segments; the
 // data-independent pivot recurrence starts each segment from c'_{pL-1},
-// tabulated once per plan.
+// tabulated once per plan.  All of a thread's column values are loaded
+// before the recurrence starts and the coefficients sit in LDS, so the
+// serial chains never wait on global memory.  Workgroups are mapped so that
+// neighbouring mode chunks run on the same XCD and share its L2 lines.
 template <typename T, int L>
 __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
     T* __restrict__ d, const double* __restrict__ a, const double* __restrict__ b,
     const double* __restrict__ c, const double* __restrict__ lam,
     const double* __restrict__ cseg, int nr, int nz, int K, int P) {
   extern __shared__ double sm[];
-  // two ping-pong buffers of (G, y) pairs, each K*P entries
-  double* sG0 = sm;
-  double* sY0 = sm + K * P;
-  double* sG1 = sm + 2 * K * P;
-  double* sY1 = sm + 3 * K * P;
+  double* sA = sm;
+  double* sB = sm + nr;
+  double* sC = sm + 2 * nr;
+  double* sG0 = sm + 3 * nr;
+  double* sY0 = sG0 + K * P;
+  double* sG1 = sY0 + K * P;
+  double* sY1 = sG1 + K * P;
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+    sA[i] = a[i];
+    sB[i] = b[i];
+    sC[i] = c[i];
+  }
+  // XCD-aware chunk order: block b runs on XCD b % 8; give each XCD a
+  // contiguous range of mode chunks.
+  const int nb = gridDim.x;
+  int chunk = blockIdx.x;
+  if (nb % 8 == 0) chunk = (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
   const int kk = threadIdx.x % K;
   const int p = threadIdx.x / K;
-  const int k = blockIdx.x * K + kk;
+  const int k = chunk * K + kk;
   const bool active = (k < nz) && (p < P);
   const int s = p * L;
   const int kc = active ? k : 0;
 
   double x[L];
   double im[L];
-  // -------- forward: load, local elimination with zero carry-in
-  double G = 1.0, y = 0.0;
-  {
-    double cp = (p == 0 || !active) ? 0.0 : cseg[(size_t)p * nz + kc];
-    const double lk = lam[kc];
 #pragma unroll
-    for (int l = 0; l < L; ++l) {
-      const int i = s + l;
-      if (active && i < nr) {
-        const double ai = a[i];
-        const double m = (b[i] + lk) - ai * cp;
-        const double inv = 1.0 / m;
-        im[l] = inv;
-        cp = c[i] * inv;
-        const double di = static_cast<double>(d[(int64_t)i * nz + kc]);
-        y = (di - ai * y) * inv;
-        G = G * (-ai * inv);
-        x[l] = y;
-      } else {
-        im[l] = 0.0;
-        x[l] = 0.0;
-      }
+  for (int l = 0; l < L; ++l) {
+    const int i = s + l;
+    x[l] = (active && i < nr) ? static_cast<double>(d[(int64_t)i * nz + kc]) : 0.0;
+  }
+  const double lk = lam[kc];
+  double cp = (p == 0 || !active) ? 0.0 : cseg[(size_t)p * nz + kc];
+  __syncthreads();
+  // -------- forward: local elimination with zero carry-in
+  double G = 1.0, y = 0.0;
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const int i = s + l;
+    if (active && i < nr) {
+      const double ai = sA[i];
+      const double m = (sB[i] + lk) - ai * cp;
+      const double inv = 1.0 / m;
+      im[l] = inv;
+      cp = sC[i] * inv;
+      y = (x[l] - ai * y) * inv;
+      G = G * (-ai * inv);
+      x[l] = y;
+    } else {
+      im[l] = 0.0;
     }
   }
   // inclusive scan over segments of the affine maps D -> G*D + y
@@ -252,7 +346,7 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
     for (int l = 0; l < L; ++l) {
       const int i = s + l;
       if (active && i < nr) {
-        g = g * (-a[i] * im[l]);
+        g = g * (-sA[i] * im[l]);
         x[l] = x[l] + g * Din;
       }
     }
@@ -263,7 +357,7 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
   for (int l = L - 1; l >= 0; --l) {
     const int i = s + l;
     if (active && i < nr) {
-      const double cpi = c[i] * im[l];
+      const double cpi = sC[i] * im[l];
       z = x[l] - cpi * z;
       H = H * (-cpi);
       x[l] = z;
@@ -296,7 +390,7 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
     for (int l = L - 1; l >= 0; --l) {
       const int i = s + l;
       if (active && i < nr) {
-        h = h * (-(c[i] * im[l]));
+        h = h * (-(sC[i] * im[l]));
         d[(int64_t)i * nz + kc] = static_cast<T>(x[l] + h * Xin);
       }
     }
@@ -315,7 +409,8 @@ hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const do
                             int rows, hipStream_t s) {
   const int M = pl.nz / 2;
   const int bs = fft_block(M);
-  const int per = (M / 2 + bs - 1) / bs;  // worst stage is radix 2
+  // worst stage is radix 2 (M/2 butterflies); 2*per*bs >= M covers the row loads
+  const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
   if (per <= 1)
     dct2_rows_kernel<T, 1><<<rows, bs, shm, s>>>(src, dst, rscale, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
@@ -329,10 +424,32 @@ hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const do
 }
 
 template <typename T>
+hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered, T* dst,
+                                  const double* wz, const double* wr, int w, hipStream_t s) {
+  const int M = pl.nz / 2;
+  const int bs = fft_block(M);
+  const int per = std::max(1, (M / 2 + bs - 1) / bs);
+  const size_t shm = sizeof(Cplx<T>) * M;
+#define AXS_TAPER_LAUNCH(P)                                                                    \
+  taper_dct2_rows_kernel<T, P><<<pl.nr, bs, shm, s>>>(src, tapered, dst, pl.rvec, wz, wr, pl.nr, \
+                                                      pl.nz, w, pl.fft, pl.tw, pl.e1, pl.e2)
+  if (per <= 1)
+    AXS_TAPER_LAUNCH(1);
+  else if (per <= 2)
+    AXS_TAPER_LAUNCH(2);
+  else if (per <= 4)
+    AXS_TAPER_LAUNCH(4);
+  else
+    AXS_TAPER_LAUNCH(8);
+#undef AXS_TAPER_LAUNCH
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, hipStream_t s) {
   const int M = pl.nz / 2;
   const int bs = fft_block(M);
-  const int per = (M / 2 + bs - 1) / bs;
+  const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
   if (per <= 1)
     dct3_rows_kernel<T, 1><<<rows, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
@@ -349,7 +466,7 @@ template <typename T>
 hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s) {
   const int K = pl.tri_K, P = pl.tri_P;
   const unsigned blocks = ceil_div(pl.nz, K);
-  const size_t shm = sizeof(double) * 4 * K * P;
+  const size_t shm = sizeof(double) * (4 * K * P + 3 * pl.nr);
   switch (pl.tri_L) {
     case 8:
       tridiag_columns_kernel<T, 8><<<blocks, K * P, shm, s>>>(d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P);
@@ -369,6 +486,14 @@ hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s) {
 template hipError_t launch_dct2_rows<double>(const StokesPlan&, const double*, double*, const double*, int, hipStream_t);
 template hipError_t launch_dct3_rows<double>(const StokesPlan&, const double*, double*, int, hipStream_t);
 template hipError_t launch_tridiag<double>(const StokesPlan&, double*, hipStream_t);
+template hipError_t launch_dct2_rows<float>(const StokesPlan&, const float*, float*, const double*, int, hipStream_t);
+template hipError_t launch_dct3_rows<float>(const StokesPlan&, const float*, float*, int, hipStream_t);
+template hipError_t launch_tridiag<float>(const StokesPlan&, float*, hipStream_t);
+template hipError_t launch_taper_dct2_rows<double>(const StokesPlan&, const double*, double*,
+                                                   double*, const double*, const double*, int,
+                                                   hipStream_t);
+template hipError_t launch_taper_dct2_rows<float>(const StokesPlan&, const float*, float*, float*,
+                                                  const double*, const double*, int, hipStream_t);
 
 }  // namespace axs
 
@@ -387,7 +512,7 @@ axs_status axs_stokes_plan_create(int nr, int nz, double dx, int bc_type,
   AXS_CHECK_ARG(bc_type == AXS_BC_NEUMANN_Z_AND_R,
                 "only homogenous_neumann_along_z_and_r is supported");
   AXS_CHECK_ARG(nr >= 3 && nz >= 4 && nz % 2 == 0, "need nr >= 3 and even nz >= 4");
-  AXS_CHECK_ARG(nz <= 8192 && nr <= 16384, "grid too large for the row/column passes");
+  AXS_CHECK_ARG(nz <= 4096 && nr <= 4096, "grid too large for the row/column passes");
   AXS_CHECK_ARG(dx > 0.0, "dx must be positive");
   FftPlanArgs fp;
   if (!factor_radices(nz / 2, &fp)) {

@@ -31,6 +31,11 @@ struct StokesPlan {
 template <typename T>
 hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const double* rscale,
                             int rows, hipStream_t s);
+// Kill taper + DCT-II (the first pass of a fused step): src untouched,
+// tapered field to `tapered`, spectrum of r * tapered to `dst`.
+template <typename T>
+hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered, T* dst,
+                                  const double* wz, const double* wr, int w, hipStream_t s);
 template <typename T>
 hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, hipStream_t s);
 template <typename T>

@@ -1,0 +1,70 @@
+"""GPU parity of the oscillating-spheroid step (spheroid_streaming.py:129-246:
+moving mask, penalisation velocity U_0 cos(omega t), period-aligned dt,
+per-period psi average) against the oracle, and the FP32 precision form's
+error against the FP64 path."""
+import numpy as np
+import pytest
+
+from oracle.drivers import OscillatingSpheroidCase
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_spheroid_streaming_fp64_matches_oracle():
+    """250 steps at nz = 128 cross two period boundaries (dt is capped at
+    period/100 there), so the psi average restarts twice."""
+    from pyaxisymflow_amd.stepper import OscillatingSpheroidStepper
+
+    cpu = OscillatingSpheroidCase(128, solver="exact")
+    gpu = OscillatingSpheroidStepper(128)
+    n = 250
+    for _ in range(n):
+        cpu.step()
+    gpu.step(n)
+    assert gpu.steps_done() == n
+    assert abs(gpu.time() - cpu.t) <= 1e-12 * cpu.t
+    assert rel_l2(gpu.char_func(), cpu.char_func) < 1e-14
+    assert rel_l2(gpu.vorticity(), cpu.vorticity) < 1e-10
+    assert rel_l2(gpu.psi(), cpu.psi) < 1e-10
+    assert rel_l2(gpu.avg_psi(), cpu.avg_psi) < 1e-10
+    np.testing.assert_allclose(gpu.drag_history(), np.array(cpu.force), rtol=1e-9,
+                               atol=1e-10 * np.abs(cpu.force).max())
+
+
+def test_spheroid_streaming_dt_rule_and_end_time():
+    """With few cycles the last dt is clipped to tEnd, and every period
+    boundary is hit exactly (freqTimer reset)."""
+    from pyaxisymflow_amd.stepper import OscillatingSpheroidStepper
+
+    cpu = OscillatingSpheroidCase(64, solver="exact", no_cycles=1, freq=16)
+    gpu = OscillatingSpheroidStepper(64, no_cycles=1, freq=16)
+    while cpu.t < cpu.tEnd:
+        cpu.step()
+    gpu.step(cpu.it)
+    assert abs(gpu.time() - cpu.tEnd) <= 1e-15
+    assert rel_l2(gpu.vorticity(), cpu.vorticity) < 1e-10
+
+
+@pytest.mark.parametrize("nz", [256, 512])
+def test_precision_form_error_vs_fp64(nz):
+    """FP32 bulk with FP64 radial solve and reductions: relative L2 of u and
+    psi and relative drag error against the FP64 path after 400 steps."""
+    from pyaxisymflow_amd.stepper import OscillatingSpheroidStepper
+
+    a = OscillatingSpheroidStepper(nz, precision="f64")
+    b = OscillatingSpheroidStepper(nz, precision="f32")
+    a.step(400)
+    b.step(400)
+    assert a.steps_done() == b.steps_done() == 400
+    ua, ub = a.velocity(), b.velocity()
+    err_u = rel_l2(np.hypot(*ub), np.hypot(*ua))
+    err_psi = rel_l2(b.psi(), a.psi())
+    fa, fb = a.drag_history(), b.drag_history()
+    err_f = np.abs(fb - fa).max() / np.abs(fa).max()
+    print(f"nz={nz}: u {err_u:.2e} psi {err_psi:.2e} force {err_f:.2e}")
+    assert err_f < 1e-4
+    assert err_u < 1e-4 and err_psi < 1e-4
