@@ -125,6 +125,58 @@ def time_gpu(nz, nr, steps, warmup, stage_reps):
     return ms, finite, float(cd[-1]) if len(cd) else None, stages, copy_gbs
 
 
+def time_batched(steps, warmup, seeds=range(64)):
+    """The batched form: the 64-case set of pyaxisymflow_amd.workloads
+    (grids nr in {128, 192, 256, 384, 512}, nz = 2 nr; sphere, prolate
+    spheroid and torus bodies; Re in [10, 500]), one RigidBatchStepper per
+    grid, all groups in flight at once on their own streams.  FP32 is timed;
+    the same cases in FP64 give the drag error."""
+    import torch
+
+    from pyaxisymflow_amd import workloads
+    from pyaxisymflow_amd.stepper import RigidBatchStepper
+
+    groups = workloads.grouped_cases(seeds)
+    built = []
+    for (nz, nr), specs in groups.items():
+        args = [workloads.rigid_case_args(sp, workloads.case_inputs(sp, nz, nr)[1]) for sp in specs]
+        hist = steps + warmup + 4
+        built.append(((nz, nr), specs,
+                      RigidBatchStepper(nz, nr, 1.0, args, r_len=0.5, precision="f32",
+                                        history=hist),
+                      RigidBatchStepper(nz, nr, 1.0, args, r_len=0.5, precision="f64",
+                                        history=hist)))
+    for _, _, b32, _ in built:
+        b32.step(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _, _, b32, _ in built:
+        b32.step(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    for _, _, _, b64 in built:
+        b64.step(warmup + steps)
+    torch.cuda.synchronize()
+    cells = sum(len(sp) * nz * nr for (nz, nr), sp, _, _ in built)
+    err = 0.0
+    for _, specs, b32, b64 in built:
+        for q in range(len(specs)):
+            d32, d64 = b32.drag_history(q), b64.drag_history(q)
+            err = max(err, float(np.max(np.abs(d32 - d64)) / np.max(np.abs(d64))))
+    return dict(
+        value=round(cells * steps / wall / 1e6, 2),
+        unit="Mcell-steps/s",
+        dtype="f32 fields/stencils/z-transforms; f64 radial solve, reductions, scalars",
+        cases=sum(len(sp) for _, sp, _, _ in built),
+        groups={f"{nz}x{nr}": len(sp) for (nz, nr), sp, _, _ in built},
+        cells=cells,
+        steps=steps,
+        warmup=warmup,
+        wall_s=round(wall, 4),
+        drag_max_rel_err_vs_f64=err,
+    )
+
+
 def cpu_baseline(nz, nr, steps):
     """The oracle (numpy restatement of the reference path with the dense
     eig-route Poisson solve, BLAS on all cores) on the same workload."""
@@ -171,6 +223,8 @@ def main():
     ap.add_argument("--stage-reps", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every case is independent and this build runs on "
@@ -221,6 +275,8 @@ def main():
             "bytes_per_cell": bpc,
         },
     }
+    if not args.no_batched:
+        out["forms"] = {"batched_fp32": time_batched(args.batched_steps, 2)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nz, nr, args.cpu_steps)
     print(json.dumps(out), flush=True)

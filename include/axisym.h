@@ -210,6 +210,19 @@ typedef struct {
 } axs_rigid_buffers;
 axs_status axs_rigid_stepper_create(const axs_rigid_config* cfg, const axs_rigid_buffers* bufs,
                                     axs_rigid_stepper** out);
+/* The batched form: `batch` cases of one grid stepped together by the same
+ * launches (the case index is a grid dimension of every kernel).  cfgs[batch]
+ * share nr, nz, kill_width, history, precision, dx and the moving_body /
+ * average_psi flags, and may differ in everything else (viscosity, body
+ * mask, ramp, dt rule, force scale).  Per-case buffers hold the cases back
+ * to back: fields batch * nr * nz elements, scalars batch *
+ * AXS_RIGID_SCALARS, drag batch * history; r, z, kill_wz and kill_wr are
+ * shared.  Each case advances its own t and dt.  The reference steps one
+ * case per process (flow_past_sphere.py:107-188); this runs that loop body
+ * for many cases at once.  axs_rigid_stepper_create is batch_create with
+ * batch = 1. */
+axs_status axs_rigid_batch_create(const axs_rigid_config* cfgs, int batch,
+                                  const axs_rigid_buffers* bufs, axs_rigid_stepper** out);
 axs_status axs_rigid_stepper_destroy(axs_rigid_stepper* st);
 /* Queues nsteps steps on `stream`; use_graph != 0 captures the step (one
  * graph per ping-pong direction) on first use and replays it (stream must

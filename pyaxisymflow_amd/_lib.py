@@ -79,6 +79,8 @@ SIGNATURES = {
                                             ctypes.POINTER(_I), _P],
     "axs_rigid_stepper_create": [ctypes.POINTER(RigidConfig), ctypes.POINTER(RigidBuffers),
                                  ctypes.POINTER(_P)],
+    "axs_rigid_batch_create": [ctypes.POINTER(RigidConfig), _I, ctypes.POINTER(RigidBuffers),
+                               ctypes.POINTER(_P)],
     "axs_rigid_stepper_destroy": [_P],
     "axs_rigid_stepper_run": [_P, _I, _I, _P],
     "axs_rigid_stepper_reset_scalars": [_P, _P],

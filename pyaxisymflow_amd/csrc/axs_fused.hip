@@ -133,11 +133,16 @@ __global__ void __launch_bounds__(256) velocity_max_kernel(const T* __restrict__
                                                            int nz, T dx,
                                                            const double* __restrict__ sc,
                                                            double* __restrict__ slots,
-                                                           RigidConfig cfg) {
+                                                           const RigidConfig* __restrict__ cfgs) {
   __shared__ T sp[kVPR][kVPZ];
   __shared__ T sri[kVTR], sinv[kVTR];
   __shared__ double wm[4];
   const int tid = threadIdx.x;
+  const int cb = blockIdx.z;  // case of a batch
+  const RigidConfig& cfg = cfgs[cb];
+  psi += (int64_t)cb * nr * nz;
+  sc += (int64_t)cb * AXS_RIGID_SCALARS;
+  slots += (int64_t)cb * kSlotsPerCase;
   const int j0 = blockIdx.x * kVTZ, i0 = blockIdx.y * kVTR;
   constexpr int NP = (kVPR * kVPZ + 255) / 256;
   T v[NP];
@@ -219,8 +224,21 @@ __global__ void __launch_bounds__(256) step_fused_kernel(
     T* __restrict__ wdst, T* __restrict__ avg, const double* __restrict__ r,
     const double* __restrict__ wz, const double* __restrict__ wr, int kw, int nr, int nz, T dx,
     double* __restrict__ sc, const double* __restrict__ slots, double* __restrict__ partials,
-    RigidConfig cfg) {
+    const RigidConfig* __restrict__ cfgs) {
   __shared__ T bufA[kBufA], bufB[kBufB], bufC[kBufC], bufD[kBufD];
+  const int cb = blockIdx.z;  // case of a batch
+  const RigidConfig& cfg = cfgs[cb];
+  {
+    const int64_t fo = (int64_t)cb * nr * nz;
+    psi += fo;
+    chi += fo;
+    wsrc += fo;
+    wdst += fo;
+    if (avg) avg += fo;
+    sc += (int64_t)cb * AXS_RIGID_SCALARS;
+    slots += (int64_t)cb * kSlotsPerCase;
+    partials += (int64_t)cb * gridDim.x * gridDim.y;
+  }
   __shared__ T sri[kR6R], sinv[kR6R], sinv2[kR6R];
   __shared__ double wpart[4];
   __shared__ double smax;
@@ -476,10 +494,18 @@ __global__ void __launch_bounds__(256) velocity_out_kernel(const T* __restrict__
                                                            const double* __restrict__ r, int nr,
                                                            int nz, T dx,
                                                            const double* __restrict__ sc,
-                                                           RigidConfig cfg) {
+                                                           const RigidConfig* __restrict__ cfgs) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= nz) return;
+  const int cb = blockIdx.z;  // case of a batch
+  const RigidConfig& cfg = cfgs[cb];
+  const int64_t fo = (int64_t)cb * nr * nz;
+  psi += fo;
+  chi += fo;
+  pz += fo;
+  pr += fo;
+  sc += (int64_t)cb * AXS_RIGID_SCALARS;
   const T two_dx = T(2) * dx, inv2dx = T(1) / two_dx;
   const T ri = T(r[i]);
   const int64_t c = (int64_t)i * nz + j;
@@ -497,8 +523,14 @@ __global__ void __launch_bounds__(1024) finalize_step_kernel(const double* __res
                                                              int count, double* __restrict__ slots,
                                                              double* __restrict__ sc,
                                                              double* __restrict__ drag,
-                                                             RigidConfig cfg) {
+                                                             const RigidConfig* __restrict__ cfgs) {
   __shared__ double red[1024];
+  const int cb = blockIdx.x;  // one block per case
+  const RigidConfig& cfg = cfgs[cb];
+  partials += (int64_t)cb * count;
+  slots += (int64_t)cb * kSlotsPerCase;
+  sc += (int64_t)cb * AXS_RIGID_SCALARS;
+  if (drag) drag += (int64_t)cb * cfg.history;
   double s = 0.0;
   for (int q = threadIdx.x; q < count; q += 1024) s += partials[q];
   red[threadIdx.x] = s;
@@ -515,10 +547,10 @@ __global__ void __launch_bounds__(1024) finalize_step_kernel(const double* __res
 
 template <typename T>
 hipError_t launch_velocity_max(const T* psi, const double* r, int nr, int nz, T dx,
-                               const double* sc, double* slots, const RigidConfig& cfg,
-                               hipStream_t s) {
-  velocity_max_kernel<T><<<dim3(ceil_div(nz, kVTZ), ceil_div(nr, kVTR)), 256, 0, s>>>(
-      psi, r, nr, nz, dx, sc, slots, cfg);
+                               const double* sc, double* slots, const RigidConfig* cfgs,
+                               int batch, hipStream_t s) {
+  velocity_max_kernel<T><<<dim3(ceil_div(nz, kVTZ), ceil_div(nr, kVTR), batch), 256, 0, s>>>(
+      psi, r, nr, nz, dx, sc, slots, cfgs);
   return hipGetLastError();
 }
 
@@ -526,37 +558,37 @@ template <typename T>
 hipError_t launch_step_fused(const T* psi, const T* chi, const T* wsrc, T* wdst, T* avg,
                              const double* r, const double* wz, const double* wr, int kw, int nr,
                              int nz, T dx, double* sc, const double* slots, double* partials,
-                             const RigidConfig& cfg, hipStream_t s) {
-  step_fused_kernel<T><<<dim3(ceil_div(nz, kFZ), ceil_div(nr, kFR)), 256, 0, s>>>(
-      psi, chi, wsrc, wdst, avg, r, wz, wr, kw, nr, nz, dx, sc, slots, partials, cfg);
+                             const RigidConfig* cfgs, int batch, hipStream_t s) {
+  step_fused_kernel<T><<<dim3(ceil_div(nz, kFZ), ceil_div(nr, kFR), batch), 256, 0, s>>>(
+      psi, chi, wsrc, wdst, avg, r, wz, wr, kw, nr, nz, dx, sc, slots, partials, cfgs);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_velocity_out(const T* psi, const T* chi, T* pz, T* pr, const double* r, int nr,
-                               int nz, T dx, const double* sc, const RigidConfig& cfg,
+                               int nz, T dx, const double* sc, const RigidConfig* cfgs, int batch,
                                hipStream_t s) {
-  velocity_out_kernel<T><<<dim3(ceil_div(nz, 256), nr), 256, 0, s>>>(psi, chi, pz, pr, r, nr, nz,
-                                                                      dx, sc, cfg);
+  velocity_out_kernel<T><<<dim3(ceil_div(nz, 256), nr, batch), 256, 0, s>>>(psi, chi, pz, pr, r,
+                                                                             nr, nz, dx, sc, cfgs);
   return hipGetLastError();
 }
 
 hipError_t launch_finalize_step(const double* partials, int count, double* slots, double* sc,
-                                double* drag, const RigidConfig& cfg, hipStream_t s) {
-  finalize_step_kernel<<<1, 1024, 0, s>>>(partials, count, slots, sc, drag, cfg);
+                                double* drag, const RigidConfig* cfgs, int batch, hipStream_t s) {
+  finalize_step_kernel<<<batch, 1024, 0, s>>>(partials, count, slots, sc, drag, cfgs);
   return hipGetLastError();
 }
 
 #define AXS_FUSED_INST(T)                                                                     \
   template hipError_t launch_velocity_max<T>(const T*, const double*, int, int, T,          \
-                                             const double*, double*, const RigidConfig&,    \
+                                             const double*, double*, const RigidConfig*, int, \
                                              hipStream_t);                                   \
   template hipError_t launch_step_fused<T>(const T*, const T*, const T*, T*, T*,            \
                                            const double*, const double*, const double*, int, \
                                            int, int, T, double*, const double*, double*,    \
-                                           const RigidConfig&, hipStream_t);                 \
+                                           const RigidConfig*, int, hipStream_t);            \
   template hipError_t launch_velocity_out<T>(const T*, const T*, T*, T*, const double*, int, \
-                                             int, T, const double*, const RigidConfig&,     \
+                                             int, T, const double*, const RigidConfig*, int, \
                                              hipStream_t);
 AXS_FUSED_INST(double)
 AXS_FUSED_INST(float)

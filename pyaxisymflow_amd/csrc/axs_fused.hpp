@@ -71,11 +71,16 @@ __device__ __forceinline__ void finalize_step_scalars(double* sc, double* drag, 
 // they spread over memory channels), holding bit patterns of non-negative doubles; all zero between
 // steps (finalize_step clears them).
 constexpr int kMaxSlots = 64, kSlotStride = 512;
+constexpr int kSlotsPerCase = kMaxSlots * kSlotStride;  // doubles per case of a batch
 
+// Every launcher runs `batch` cases of one (nr, nz) grid: fields back to
+// back (nr * nz each), AXS_RIGID_SCALARS scalars, kSlotsPerCase slots, one
+// block's worth of partials and `history` drag entries per case, and the
+// per-case configurations cfgs[batch] in device memory.
 template <typename T>
 hipError_t launch_velocity_max(const T* psi, const double* r, int nr, int nz, T dx,
-                               const double* sc, double* slots, const RigidConfig& cfg,
-                               hipStream_t s);
+                               const double* sc, double* slots, const RigidConfig* cfgs,
+                               int batch, hipStream_t s);
 // Number of step_fused blocks (= force partials) for a grid.
 int fused_blocks(int nr, int nz);
 // Penalisation through the end of the step: reads psi, chi and the stored
@@ -86,14 +91,14 @@ template <typename T>
 hipError_t launch_step_fused(const T* psi, const T* chi, const T* wsrc, T* wdst, T* avg,
                              const double* r, const double* wz, const double* wr, int kw, int nr,
                              int nz, T dx, double* sc, const double* slots, double* partials,
-                             const RigidConfig& cfg, hipStream_t s);
+                             const RigidConfig* cfgs, int batch, hipStream_t s);
 template <typename T>
 hipError_t launch_velocity_out(const T* psi, const T* chi, T* pz, T* pr, const double* r, int nr,
-                               int nz, T dx, const double* sc, const RigidConfig& cfg,
+                               int nz, T dx, const double* sc, const RigidConfig* cfgs, int batch,
                                hipStream_t s);
 // One block: force sum of the partials, then t, it, period timer, the next
 // step's scalars, and cleared maximum slots.
 hipError_t launch_finalize_step(const double* partials, int count, double* slots, double* sc,
-                                double* drag, const RigidConfig& cfg, hipStream_t s);
+                                double* drag, const RigidConfig* cfgs, int batch, hipStream_t s);
 
 }  // namespace axs

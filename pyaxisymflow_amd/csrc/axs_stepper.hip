@@ -33,6 +33,7 @@
 #include "axs_stepper.hpp"
 
 #include <new>
+#include <vector>
 
 namespace axs {
 
@@ -43,10 +44,15 @@ namespace axs {
 template <typename T>
 __global__ void body_mask_kernel(T* __restrict__ chi, const double* __restrict__ r,
                                  const double* __restrict__ z, int nr, int nz,
-                                 const double* __restrict__ sc, RigidConfig cfg) {
+                                 const double* __restrict__ sc,
+                                 const RigidConfig* __restrict__ cfgs) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= nz) return;
+  const int cb = blockIdx.z;  // case of a batch
+  const RigidConfig& cfg = cfgs[cb];
+  chi += (int64_t)cb * nr * nz;
+  sc += (int64_t)cb * AXS_RIGID_SCALARS;
   const double zc = sc[kScBodyZ];
   const double dz = z[j] - zc, dr = r[i] - cfg.body_r0;
   const double phi = cfg.body_radius - sqrt(dz * dz / cfg.body_aspect2 + dr * dr);
@@ -58,10 +64,14 @@ __global__ void body_mask_kernel(T* __restrict__ chi, const double* __restrict__
 
 // Step scalars for the time now in sc[T] (without advancing); clears the
 // velocity maximum slots.
+// One block per case.
 __global__ void init_scalars_kernel(double* __restrict__ sc, double* __restrict__ slots,
-                                    RigidConfig cfg) {
+                                    const RigidConfig* __restrict__ cfgs) {
+  const int cb = blockIdx.x;
+  sc += (int64_t)cb * AXS_RIGID_SCALARS;
+  slots += (int64_t)cb * kSlotsPerCase;
   if (threadIdx.x < kMaxSlots) slots[threadIdx.x * kSlotStride] = 0.0;
-  if (threadIdx.x == 0) next_step_scalars(sc, cfg, sc[kScT]);
+  if (threadIdx.x == 0) next_step_scalars(sc, cfgs[cb], sc[kScT]);
 }
 
 // ------------------------------------------------------------------ stepper
@@ -83,11 +93,14 @@ RigidStepperBase::~RigidStepperBase() {
   }
   if (partials) (void)hipFree(partials);
   if (slots) (void)hipFree(slots);
+  if (spec) (void)hipFree(spec);
+  if (dcfg) (void)hipFree(dcfg);
   delete plan;
 }
 
-hipError_t RigidStepperBase::init(const RigidConfig& c, const RigidBuffers& b) {
-  cfg = c;
+hipError_t RigidStepperBase::init(const RigidConfig* cfgs, int nbatch, const RigidBuffers& b) {
+  cfg = cfgs[0];
+  batch = nbatch;
   buf = b;
   plan = new (std::nothrow) StokesPlan();
   if (!plan) return hipErrorOutOfMemory;
@@ -106,11 +119,18 @@ hipError_t RigidStepperBase::init(const RigidConfig& c, const RigidBuffers& b) {
   if (e != hipSuccess) return e;
   blocks_x = ceil_div(cfg.nz, 256);
   npartials = fused_blocks(cfg.nr, cfg.nz);
-  e = hipMalloc(reinterpret_cast<void**>(&partials), sizeof(double) * (size_t)npartials);
+  const size_t cells = (size_t)cfg.nr * cfg.nz * batch;
+  e = hipMalloc(reinterpret_cast<void**>(&partials), sizeof(double) * (size_t)npartials * batch);
   if (e != hipSuccess) return e;
-  e = hipMalloc(reinterpret_cast<void**>(&slots), sizeof(double) * kMaxSlots * kSlotStride);
+  e = hipMalloc(reinterpret_cast<void**>(&slots), sizeof(double) * kSlotsPerCase * batch);
   if (e != hipSuccess) return e;
-  init_scalars_kernel<<<1, 64>>>(buf.scalars, slots, cfg);
+  e = hipMalloc(&spec, sizeof(double) * cells);
+  if (e != hipSuccess) return e;
+  e = hipMalloc(reinterpret_cast<void**>(&dcfg), sizeof(RigidConfig) * batch);
+  if (e != hipSuccess) return e;
+  e = hipMemcpy(dcfg, cfgs, sizeof(RigidConfig) * batch, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  init_scalars_kernel<<<batch, 64>>>(buf.scalars, slots, dcfg);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   return hipDeviceSynchronize();
@@ -131,27 +151,29 @@ hipError_t RigidStepper<T>::enqueue_stage(int stage, hipStream_t s) {
   T* nxt = parity ? home : other;
   T* psi = static_cast<T*>(buf.psi);
   T* chi = static_cast<T*>(buf.char_func);
-  T* spec = reinterpret_cast<T*>(plan->spec);
+  T* sp = static_cast<T*>(spec);
+  const int B = batch;
   if (!stage_enabled(stage)) return hipSuccess;
   switch (stage) {
     case 0:
-      return launch_taper_dct2_rows<T>(*plan, cur, nullptr, spec, buf.kill_wz, buf.kill_wr, w, s);
+      return launch_taper_dct2_rows<T>(*plan, cur, nullptr, sp, buf.kill_wz, buf.kill_wr, w, s, B);
     case 1:
-      return launch_tridiag<T>(*plan, spec, s);
+      return launch_tridiag<T>(*plan, sp, s, B);
     case 2:
-      return launch_dct3_rows<T>(*plan, spec, psi, nr, s);
+      return launch_dct3_rows<T>(*plan, sp, psi, nr, s, B);
     case 3:
-      return launch_velocity_max<T>(psi, buf.r, nr, nz, T(cfg.dx), sc, slots, cfg, s);
+      return launch_velocity_max<T>(psi, buf.r, nr, nz, T(cfg.dx), sc, slots, dcfg, B, s);
     case 4:
-      body_mask_kernel<T><<<dim3(blocks_x, nr), 256, 0, s>>>(chi, buf.r, buf.z, nr, nz, sc, cfg);
+      body_mask_kernel<T><<<dim3(blocks_x, nr, B), 256, 0, s>>>(chi, buf.r, buf.z, nr, nz, sc,
+                                                                 dcfg);
       return hipGetLastError();
     case 5:
       return launch_step_fused<T>(psi, chi, cur, nxt,
                                   cfg.average_psi ? static_cast<T*>(buf.avg_psi) : nullptr,
                                   buf.r, buf.kill_wz, buf.kill_wr, w, nr, nz, T(cfg.dx), sc, slots,
-                                  partials, cfg, s);
+                                  partials, dcfg, B, s);
     case 6:
-      return launch_finalize_step(partials, npartials, slots, sc, buf.drag, cfg, s);
+      return launch_finalize_step(partials, npartials, slots, sc, buf.drag, dcfg, B, s);
     default:
       return hipErrorInvalidValue;
   }
@@ -160,7 +182,7 @@ hipError_t RigidStepper<T>::enqueue_stage(int stage, hipStream_t s) {
 template <typename T>
 hipError_t RigidStepper<T>::settle(hipStream_t s) {
   if (!parity) return hipSuccess;
-  const int64_t n = (int64_t)cfg.nr * cfg.nz;
+  const int64_t n = (int64_t)cfg.nr * cfg.nz * batch;
   copy_field_kernel<T><<<ceil_div(n, 256), 256, 0, s>>>(static_cast<T*>(buf.vorticity),
                                                         static_cast<const T*>(buf.work0), n);
   parity = 0;
@@ -172,7 +194,7 @@ hipError_t RigidStepper<T>::velocity_out(hipStream_t s) {
   return launch_velocity_out<T>(static_cast<const T*>(buf.psi),
                                 static_cast<const T*>(buf.char_func),
                                 static_cast<T*>(buf.u_pen_z), static_cast<T*>(buf.u_pen_r), buf.r,
-                                cfg.nr, cfg.nz, T(cfg.dx), buf.scalars, cfg, s);
+                                cfg.nr, cfg.nz, T(cfg.dx), buf.scalars, dcfg, batch, s);
 }
 
 hipError_t RigidStepperBase::enqueue_step(hipStream_t s) {
@@ -241,30 +263,74 @@ struct axs_rigid_stepper {
 
 extern "C" {
 
-axs_status axs_rigid_stepper_create(const axs_rigid_config* cfg, const axs_rigid_buffers* bufs,
-                                    axs_rigid_stepper** out) {
-  AXS_CHECK_ARG(cfg && bufs && out, "null pointer");
+static RigidConfig to_internal(const axs_rigid_config& x) {
+  RigidConfig c{};
+  c.nr = x.nr;
+  c.nz = x.nz;
+  c.kill_width = x.kill_width;
+  c.history = x.history;
+  c.precision = x.precision;
+  c.moving_body = x.moving_body;
+  c.average_psi = x.average_psi;
+  c.dx = x.dx;
+  c.nu = x.nu;
+  c.brink_lam = x.brink_lam;
+  c.U0 = x.U0;
+  c.T_ramp = x.T_ramp;
+  c.CFL = x.CFL;
+  c.eps = x.eps;
+  c.dt_diffusive = x.dt_diffusive;
+  c.osc_amp = x.osc_amp;
+  c.osc_omega = x.osc_omega;
+  c.drag_scale = x.drag_scale;
+  c.drag_div = x.drag_div;
+  c.dt_cap = x.dt_cap;
+  c.period = x.period;
+  c.t_end = x.t_end;
+  c.body_z0 = x.body_z0;
+  c.body_r0 = x.body_r0;
+  c.body_aspect2 = x.body_aspect2;
+  c.body_radius = x.body_radius;
+  c.body_disp = x.body_disp;
+  c.moll_zone = x.moll_zone;
+  return c;
+}
+
+axs_status axs_rigid_batch_create(const axs_rigid_config* cfgs, int batch,
+                                  const axs_rigid_buffers* bufs, axs_rigid_stepper** out) {
+  AXS_CHECK_ARG(cfgs && bufs && out, "null pointer");
   *out = nullptr;
-  AXS_CHECK_ARG(cfg->nr >= 4 && cfg->nz >= 8 && cfg->nz % 2 == 0, "need nr >= 4 and even nz >= 8");
-  AXS_CHECK_ARG(cfg->kill_width >= 2 && 2 * cfg->kill_width <= cfg->nz &&
-                    cfg->kill_width < cfg->nr,
+  AXS_CHECK_ARG(batch >= 1 && batch <= 4096, "batch must be in [1, 4096]");
+  const axs_rigid_config& c0 = cfgs[0];
+  AXS_CHECK_ARG(c0.nr >= 4 && c0.nz >= 8 && c0.nz % 2 == 0, "need nr >= 4 and even nz >= 8");
+  AXS_CHECK_ARG(c0.kill_width >= 2 && 2 * c0.kill_width <= c0.nz && c0.kill_width < c0.nr,
                 "bad kill_width for the grid");
-  AXS_CHECK_ARG(cfg->dx > 0 && cfg->nu > 0 && cfg->CFL > 0, "dx, nu and CFL must be positive");
-  AXS_CHECK_ARG(cfg->precision == AXS_F64 || cfg->precision == AXS_F32, "bad precision");
-  AXS_CHECK_ARG(!cfg->moving_body || (cfg->body_aspect2 > 0 && cfg->moll_zone > 0),
-                "moving body needs aspect^2 > 0 and moll_zone > 0");
-  AXS_CHECK_ARG(bufs->vorticity && bufs->psi && bufs->u_z && bufs->u_r && bufs->u_pen_z &&
-                    bufs->u_pen_r && bufs->char_func && bufs->work0 && bufs->work1 &&
-                    bufs->scalars && bufs->r && bufs->kill_wz && bufs->kill_wr,
+  AXS_CHECK_ARG(c0.precision == AXS_F64 || c0.precision == AXS_F32, "bad precision");
+  AXS_CHECK_ARG(c0.history >= 0, "negative history");
+  for (int q = 0; q < batch; ++q) {
+    const axs_rigid_config& c = cfgs[q];
+    AXS_CHECK_ARG(c.nr == c0.nr && c.nz == c0.nz && c.kill_width == c0.kill_width &&
+                      c.history == c0.history && c.precision == c0.precision &&
+                      c.moving_body == c0.moving_body && c.average_psi == c0.average_psi &&
+                      c.dx == c0.dx,
+                  "cases of a batch must share grid, kill width, history, precision and flags");
+    AXS_CHECK_ARG(c.dx > 0 && c.nu > 0 && c.CFL > 0, "dx, nu and CFL must be positive");
+    AXS_CHECK_ARG(!c.moving_body || (c.body_aspect2 > 0 && c.moll_zone > 0),
+                  "moving body needs aspect^2 > 0 and moll_zone > 0");
+  }
+  AXS_CHECK_ARG(bufs->vorticity && bufs->psi && bufs->u_pen_z && bufs->u_pen_r &&
+                    bufs->char_func && bufs->work0 && bufs->scalars && bufs->r && bufs->kill_wz &&
+                    bufs->kill_wr,
                 "null buffer");
-  AXS_CHECK_ARG(!cfg->average_psi || bufs->avg_psi, "average_psi needs the avg_psi buffer");
-  AXS_CHECK_ARG(!cfg->moving_body || bufs->z, "moving body needs the z table");
+  AXS_CHECK_ARG(!c0.average_psi || bufs->avg_psi, "average_psi needs the avg_psi buffer");
+  AXS_CHECK_ARG(!c0.moving_body || bufs->z, "moving body needs the z table");
+  AXS_CHECK_ARG(c0.history == 0 || bufs->drag, "history > 0 needs the drag buffer");
   axs_rigid_stepper* st = new (std::nothrow) axs_rigid_stepper();
   if (!st) {
     set_error("out of host memory");
     return AXS_ERR_ALLOC;
   }
-  if (cfg->precision == AXS_F64)
+  if (c0.precision == AXS_F64)
     st->impl = new (std::nothrow) RigidStepper<double>();
   else
     st->impl = new (std::nothrow) RigidStepper<float>();
@@ -273,40 +339,13 @@ axs_status axs_rigid_stepper_create(const axs_rigid_config* cfg, const axs_rigid
     set_error("out of host memory");
     return AXS_ERR_ALLOC;
   }
-  RigidConfig c{};
-  c.nr = cfg->nr;
-  c.nz = cfg->nz;
-  c.kill_width = cfg->kill_width;
-  c.history = cfg->history;
-  c.precision = cfg->precision;
-  c.moving_body = cfg->moving_body;
-  c.average_psi = cfg->average_psi;
-  c.dx = cfg->dx;
-  c.nu = cfg->nu;
-  c.brink_lam = cfg->brink_lam;
-  c.U0 = cfg->U0;
-  c.T_ramp = cfg->T_ramp;
-  c.CFL = cfg->CFL;
-  c.eps = cfg->eps;
-  c.dt_diffusive = cfg->dt_diffusive;
-  c.osc_amp = cfg->osc_amp;
-  c.osc_omega = cfg->osc_omega;
-  c.drag_scale = cfg->drag_scale;
-  c.drag_div = cfg->drag_div;
-  c.dt_cap = cfg->dt_cap;
-  c.period = cfg->period;
-  c.t_end = cfg->t_end;
-  c.body_z0 = cfg->body_z0;
-  c.body_r0 = cfg->body_r0;
-  c.body_aspect2 = cfg->body_aspect2;
-  c.body_radius = cfg->body_radius;
-  c.body_disp = cfg->body_disp;
-  c.moll_zone = cfg->moll_zone;
+  std::vector<RigidConfig> cs((size_t)batch);
+  for (int q = 0; q < batch; ++q) cs[q] = to_internal(cfgs[q]);
   RigidBuffers b{bufs->vorticity, bufs->psi,    bufs->u_z,     bufs->u_r,    bufs->u_pen_z,
                  bufs->u_pen_r,   bufs->char_func, bufs->work0, bufs->work1,  bufs->avg_psi,
                  bufs->scalars,   bufs->drag,   bufs->r,       bufs->z,      bufs->kill_wz,
                  bufs->kill_wr};
-  hipError_t e = st->impl->init(c, b);
+  hipError_t e = st->impl->init(cs.data(), batch, b);
   if (e != hipSuccess) {
     set_error(std::string("rigid stepper: ") + hipGetErrorString(e));
     delete st;
@@ -314,6 +353,11 @@ axs_status axs_rigid_stepper_create(const axs_rigid_config* cfg, const axs_rigid
   }
   *out = st;
   return AXS_OK;
+}
+
+axs_status axs_rigid_stepper_create(const axs_rigid_config* cfg, const axs_rigid_buffers* bufs,
+                                    axs_rigid_stepper** out) {
+  return axs_rigid_batch_create(cfg, 1, bufs, out);
 }
 
 axs_status axs_rigid_stepper_destroy(axs_rigid_stepper* st) {
@@ -351,8 +395,8 @@ axs_status axs_rigid_stepper_velocity(axs_rigid_stepper* st, void* stream) {
 
 axs_status axs_rigid_stepper_reset_scalars(axs_rigid_stepper* st, void* stream) {
   AXS_CHECK_ARG(st, "null stepper");
-  init_scalars_kernel<<<1, 64, 0, as_stream(stream)>>>(st->impl->buf.scalars, st->impl->slots,
-                                                       st->impl->cfg);
+  init_scalars_kernel<<<st->impl->batch, 64, 0, as_stream(stream)>>>(
+      st->impl->buf.scalars, st->impl->slots, st->impl->dcfg);
   AXS_LAUNCH_CHECK("init_scalars");
   return AXS_OK;
 }

@@ -51,11 +51,17 @@ struct RigidBuffers {
 constexpr int kRigidStages = 7;
 extern const char* const kRigidStageNames[kRigidStages];
 
+// `batch` cases of one (nr, nz) grid stepped together; every field buffer
+// holds the cases back to back, and each case has its own configuration
+// (viscosity, body, ramp, dt rule, force scale) and device scalars.
 struct RigidStepperBase {
-  RigidConfig cfg{};
+  RigidConfig cfg{};                // case 0 (grid-level fields are shared)
+  RigidConfig* dcfg = nullptr;      // per-case configurations, device
+  int batch = 1;
   RigidBuffers buf{};
   StokesPlan* plan = nullptr;
-  double* partials = nullptr;  // force partial sums of the penalize blocks
+  void* spec = nullptr;        // spectral work buffer, batch * nr * nz doubles
+  double* partials = nullptr;  // force partial sums of the fused blocks
   double* slots = nullptr;     // velocity maximum slots
   int blocks_x = 0, npartials = 0;
   // The step reads the stored vorticity from one buffer of the pair
@@ -65,7 +71,7 @@ struct RigidStepperBase {
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   hipStream_t graph_stream = nullptr;
 
-  hipError_t init(const RigidConfig& c, const RigidBuffers& b);
+  hipError_t init(const RigidConfig* cfgs, int nbatch, const RigidBuffers& b);
   virtual hipError_t enqueue_stage(int stage, hipStream_t s) = 0;
   bool stage_enabled(int stage) const;
   hipError_t enqueue_step(hipStream_t s);

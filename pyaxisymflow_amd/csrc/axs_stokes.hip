@@ -185,6 +185,10 @@ __global__ void __launch_bounds__(256) taper_dct2_rows_kernel(
   const int M = fp.M;
   Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int i = blockIdx.x;
+  const int64_t cs = (int64_t)blockIdx.y * nr * nz;  // case of a batch
+  src += cs;
+  dst += cs;
+  if (tapered) tapered += cs;
   const int srow = (i >= nr - w) ? nr - w : i;
   const T* row = src + (int64_t)srow * nz;
   constexpr int NJ = 4 * PER;
@@ -232,6 +236,9 @@ __global__ void __launch_bounds__(256) dct3_rows_kernel(
   const int M = fp.M;
   Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int i = blockIdx.x;
+  const int64_t cs = (int64_t)blockIdx.y * gridDim.x * nz;  // case of a batch
+  src += cs;
+  dst += cs;
   const T* row = src + (int64_t)i * nz;
   constexpr int NQ = 2 * PER;
   dct3_pre<T, NQ>(buf, M, e1, e2, [&](int k) { return row[k]; });
@@ -264,6 +271,7 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
     const double* __restrict__ c, const double* __restrict__ lam,
     const double* __restrict__ cseg, int nr, int nz, int K, int P) {
   extern __shared__ double sm[];
+  d += (int64_t)blockIdx.y * nr * nz;  // case of a batch
   double* sA = sm;
   double* sB = sm + nr;
   double* sC = sm + 2 * nr;
@@ -425,13 +433,15 @@ hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const do
 
 template <typename T>
 hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered, T* dst,
-                                  const double* wz, const double* wr, int w, hipStream_t s) {
+                                  const double* wz, const double* wr, int w, hipStream_t s,
+                                  int batch) {
   const int M = pl.nz / 2;
   const int bs = fft_block(M);
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
 #define AXS_TAPER_LAUNCH(P)                                                                    \
-  taper_dct2_rows_kernel<T, P><<<pl.nr, bs, shm, s>>>(src, tapered, dst, pl.rvec, wz, wr, pl.nr, \
+  taper_dct2_rows_kernel<T, P><<<dim3(pl.nr, batch), bs, shm, s>>>(                          \
+      src, tapered, dst, pl.rvec, wz, wr, pl.nr,                                               \
                                                       pl.nz, w, pl.fft, pl.tw, pl.e1, pl.e2)
   if (per <= 1)
     AXS_TAPER_LAUNCH(1);
@@ -446,26 +456,28 @@ hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered
 }
 
 template <typename T>
-hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, hipStream_t s) {
+hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, hipStream_t s,
+                            int batch) {
   const int M = pl.nz / 2;
   const int bs = fft_block(M);
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
+  const dim3 grid(rows, batch);
   if (per <= 1)
-    dct3_rows_kernel<T, 1><<<rows, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
+    dct3_rows_kernel<T, 1><<<grid, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
   else if (per <= 2)
-    dct3_rows_kernel<T, 2><<<rows, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
+    dct3_rows_kernel<T, 2><<<grid, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
   else if (per <= 4)
-    dct3_rows_kernel<T, 4><<<rows, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
+    dct3_rows_kernel<T, 4><<<grid, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
   else
-    dct3_rows_kernel<T, 8><<<rows, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
+    dct3_rows_kernel<T, 8><<<grid, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s) {
+hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch) {
   const int K = pl.tri_K, P = pl.tri_P;
-  const unsigned blocks = ceil_div(pl.nz, K);
+  const dim3 blocks(ceil_div(pl.nz, K), batch);
   const size_t shm = sizeof(double) * (4 * K * P + 3 * pl.nr);
   switch (pl.tri_L) {
     case 8:
@@ -484,16 +496,17 @@ hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s) {
 }
 
 template hipError_t launch_dct2_rows<double>(const StokesPlan&, const double*, double*, const double*, int, hipStream_t);
-template hipError_t launch_dct3_rows<double>(const StokesPlan&, const double*, double*, int, hipStream_t);
-template hipError_t launch_tridiag<double>(const StokesPlan&, double*, hipStream_t);
+template hipError_t launch_dct3_rows<double>(const StokesPlan&, const double*, double*, int, hipStream_t, int);
+template hipError_t launch_tridiag<double>(const StokesPlan&, double*, hipStream_t, int);
 template hipError_t launch_dct2_rows<float>(const StokesPlan&, const float*, float*, const double*, int, hipStream_t);
-template hipError_t launch_dct3_rows<float>(const StokesPlan&, const float*, float*, int, hipStream_t);
-template hipError_t launch_tridiag<float>(const StokesPlan&, float*, hipStream_t);
+template hipError_t launch_dct3_rows<float>(const StokesPlan&, const float*, float*, int, hipStream_t, int);
+template hipError_t launch_tridiag<float>(const StokesPlan&, float*, hipStream_t, int);
 template hipError_t launch_taper_dct2_rows<double>(const StokesPlan&, const double*, double*,
                                                    double*, const double*, const double*, int,
-                                                   hipStream_t);
+                                                   hipStream_t, int);
 template hipError_t launch_taper_dct2_rows<float>(const StokesPlan&, const float*, float*, float*,
-                                                  const double*, const double*, int, hipStream_t);
+                                                  const double*, const double*, int, hipStream_t,
+                                                  int);
 
 }  // namespace axs
 

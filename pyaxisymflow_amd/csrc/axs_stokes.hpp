@@ -32,13 +32,16 @@ template <typename T>
 hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const double* rscale,
                             int rows, hipStream_t s);
 // Kill taper + DCT-II (the first pass of a fused step): src untouched,
-// tapered field to `tapered`, spectrum of r * tapered to `dst`.
+// tapered field to `tapered` (if not null), spectrum of r * tapered to `dst`.
+// `batch` cases of (nr, nz) lie back to back in every buffer.
 template <typename T>
 hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered, T* dst,
-                                  const double* wz, const double* wr, int w, hipStream_t s);
+                                  const double* wz, const double* wr, int w, hipStream_t s,
+                                  int batch = 1);
 template <typename T>
-hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, hipStream_t s);
+hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, hipStream_t s,
+                            int batch = 1);
 template <typename T>
-hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s);
+hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch = 1);
 
 }  // namespace axs

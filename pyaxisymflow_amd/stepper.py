@@ -48,7 +48,215 @@ def kill_weights(z, r, dx, width=3):
     return np.concatenate([wz_l, wz_r]), wr
 
 
-class RigidBodyStepper:
+_CASE_KEYS = ("nu", "brink_lam", "char_func", "U_0", "T_ramp", "r_sph", "CFL", "vorticity",
+              "oscillation", "moving_body", "average_psi", "dt_cap", "period", "t_end",
+              "force_scale")
+
+
+def _case_config(nr, nz, dx, kill_width, history, precision, nu, brink_lam, char_func=None,
+                 U_0=1.0, T_ramp=None, r_sph=None, CFL=0.1, vorticity=None, oscillation=None,
+                 moving_body=None, average_psi=False, dt_cap=None, period=None, t_end=None,
+                 force_scale=None):
+    """axs_rigid_config of one case (flow_past_sphere.py:39-56,
+    spheroid_streaming.py:40-112 for the streaming terms)."""
+    if char_func is None and moving_body is None:
+        raise ValueError("need char_func or moving_body")
+    cfg = _Config()
+    cfg.nr, cfg.nz, cfg.kill_width, cfg.history = nr, nz, kill_width, history
+    cfg.precision = _lib.AXS_F64 if precision == "f64" else _lib.AXS_F32
+    cfg.dx, cfg.nu, cfg.brink_lam = dx, nu, brink_lam
+    cfg.CFL = CFL
+    cfg.eps = np.finfo(float).eps
+    cfg.dt_diffusive = 0.9 * dx**2 / 4 / nu  # flow_past_sphere.py:151
+    cfg.dt_cap = np.inf if dt_cap is None else dt_cap
+    cfg.period = 0.0 if period is None else period
+    cfg.t_end = 0.0 if t_end is None else t_end
+    cfg.average_psi = 1 if average_psi else 0
+    if oscillation is None:
+        cfg.U0 = U_0
+        cfg.T_ramp = T_ramp if T_ramp is not None else 0.0
+        cfg.osc_amp, cfg.osc_omega = 0.0, 0.0
+    else:
+        cfg.U0, cfg.T_ramp = 0.0, 0.0
+        cfg.osc_amp, cfg.osc_omega = oscillation
+    if moving_body is not None:
+        cfg.moving_body = 1
+        cfg.body_z0 = moving_body["z0"]
+        cfg.body_r0 = moving_body.get("r0", 0.0)
+        cfg.body_aspect2 = moving_body.get("aspect", 1.0) ** 2
+        cfg.body_radius = moving_body["radius"]
+        cfg.body_disp = moving_body["disp"]
+        cfg.moll_zone = moving_body["moll_zone"]
+    if force_scale is not None:
+        cfg.drag_scale, cfg.drag_div = force_scale, 1.0
+    elif r_sph is not None:
+        # flow_past_sphere.py:166-175
+        cfg.drag_scale = 2 * 2 * np.pi * dx * dx * brink_lam
+        cfg.drag_div = np.pi * r_sph**2
+    else:
+        cfg.drag_scale, cfg.drag_div = 0.0, 1.0
+    return cfg
+
+
+class RigidBatchStepper:
+    """Many rigid-body cases of one grid, stepped together on the GPU (the
+    batched form of examples/FlowPastSphere/flow_past_sphere.py:107-188).
+
+    ``cases`` is a list of dicts with the per-case arguments of
+    ``RigidBodyStepper`` (``nu``, ``brink_lam``, ``char_func``, ``U_0``,
+    ``T_ramp``, ``r_sph``, ``CFL``, ``vorticity``, ``oscillation``,
+    ``moving_body``, ``average_psi``, ``dt_cap``, ``period``, ``t_end``,
+    ``force_scale``).  Every case keeps its own t, dt and force history;
+    one launch per stage advances all of them.  Accessors take the case
+    index.
+    """
+
+    def __init__(self, nz, nr, z_len, cases, kill_width=3, r_len=None, use_graph=True,
+                 precision="f64", history=1 << 16):
+        torch = _lib.torch()
+        dev = _lib.device()
+        if precision not in ("f64", "f32"):
+            raise ValueError("precision must be 'f64' or 'f32'")
+        cases = list(cases)
+        if not cases:
+            raise ValueError("need at least one case")
+        for c in cases:
+            bad = set(c) - set(_CASE_KEYS)
+            if bad:
+                raise TypeError(f"unknown case arguments {sorted(bad)}")
+        self.batch = B = len(cases)
+        self.precision = precision
+        self.nz, self.nr = int(nz), int(nr)
+        self.dx, self.z, self.r = rigid_grid(nz, nr, z_len, r_len)
+        dx = self.dx
+        self.use_graph = use_graph
+        self.stream = torch.cuda.Stream(device=dev)
+        ft = torch.float64 if precision == "f64" else torch.float32
+        self.field_dtype = ft
+        self.history = int(history)
+
+        def fields(key=None):
+            t = torch.zeros((B, nr, nz), dtype=ft, device=dev)
+            if key is not None:
+                for q, c in enumerate(cases):
+                    a = c.get(key)
+                    if a is None:
+                        continue
+                    a = np.asarray(a, dtype=np.float64)
+                    if a.shape != (nr, nz):
+                        raise ValueError(f"{key}: expected shape {(nr, nz)}, got {a.shape}")
+                    t[q].copy_(torch.from_numpy(np.ascontiguousarray(a)).to(dev).to(ft))
+            return t
+
+        def table(a):
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+        cfgs = (_Config * B)()
+        for q, c in enumerate(cases):
+            cfgs[q] = _case_config(self.nr, self.nz, dx, kill_width, self.history, precision, **c)
+        self.cfgs = cfgs
+        average_psi = bool(cfgs[0].average_psi)
+        self.t_vorticity = fields("vorticity")
+        self.t_psi = fields()
+        self.t_u_pen_z = fields()
+        self.t_u_pen_r = fields()
+        self.t_char_func = fields("char_func")
+        self.t_work0 = fields()
+        self.t_avg_psi = fields() if average_psi else None
+        self.t_scalars = torch.zeros((B, N_SCALARS), dtype=torch.float64, device=dev)
+        self.t_drag = torch.zeros((B, max(self.history, 1)), dtype=torch.float64, device=dev)
+        self.t_r = table(self.r)
+        self.t_z = table(self.z)
+        wz, wr = kill_weights(self.z, self.r, dx, kill_width)
+        self.t_wz = table(wz)
+        self.t_wr = table(wr)
+
+        bufs = _Buffers()
+        bufs.vorticity = self.t_vorticity.data_ptr()
+        bufs.psi = self.t_psi.data_ptr()
+        bufs.u_pen_z = self.t_u_pen_z.data_ptr()
+        bufs.u_pen_r = self.t_u_pen_r.data_ptr()
+        bufs.char_func = self.t_char_func.data_ptr()
+        bufs.work0 = self.t_work0.data_ptr()
+        bufs.scalars = self.t_scalars.data_ptr()
+        bufs.drag = self.t_drag.data_ptr()
+        bufs.avg_psi = self.t_avg_psi.data_ptr() if average_psi else None
+        bufs.r = self.t_r.data_ptr()
+        bufs.z = self.t_z.data_ptr()
+        bufs.kill_wz = self.t_wz.data_ptr()
+        bufs.kill_wr = self.t_wr.data_ptr()
+        self._bufs = bufs
+        torch.cuda.synchronize(dev)
+        handle = ctypes.c_void_p()
+        _lib.call("axs_rigid_batch_create", cfgs, B, ctypes.byref(bufs), ctypes.byref(handle))
+        self._h = handle
+
+    # -------------------------------------------------------------- stepping
+    def step(self, n=1):
+        """Queues n timesteps of every case on this stepper's stream
+        (asynchronous)."""
+        torch = _lib.torch()
+        cur = torch.cuda.current_stream()
+        self.stream.wait_stream(cur)
+        _lib.call("axs_rigid_stepper_run", self._h, int(n), 1 if self.use_graph else 0,
+                  ctypes.c_void_p(self.stream.cuda_stream))
+        cur.wait_stream(self.stream)
+
+    def synchronize(self):
+        self.stream.synchronize()
+
+    # -------------------------------------------------------------- state
+    def _host(self, t):
+        self.synchronize()
+        return t.double().cpu().numpy()
+
+    def vorticity(self, case=0):
+        return self._host(self.t_vorticity[case])
+
+    def psi(self, case=0):
+        return self._host(self.t_psi[case])
+
+    def avg_psi(self, case=0):
+        return None if self.t_avg_psi is None else self._host(self.t_avg_psi[case])
+
+    def char_func(self, case=0):
+        return self._host(self.t_char_func[case])
+
+    def velocity(self, case=0):
+        """(u_z, u_r) after penalisation (the velocity ENO3 used in the last
+        step), computed on demand from psi and chi."""
+        _lib.call("axs_rigid_stepper_velocity", self._h, ctypes.c_void_p(self.stream.cuda_stream))
+        return self._host(self.t_u_pen_z[case]), self._host(self.t_u_pen_r[case])
+
+    def scalars(self, case=0):
+        return self._host(self.t_scalars[case])
+
+    def time(self, case=0):
+        return float(self.scalars(case)[SC_T])
+
+    def steps_done(self, case=0):
+        return int(self.scalars(case)[SC_IT])
+
+    def dt(self, case=0):
+        return float(self.scalars(case)[SC_DT])
+
+    def drag_history(self, case=0):
+        """Drag coefficient (or force) of every step so far, FP64."""
+        n = min(self.steps_done(case), self.history)
+        return self._host(self.t_drag[case])[:n].copy()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self.synchronize()
+                _lib.load().axs_rigid_stepper_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class RigidBodyStepper(RigidBatchStepper):
     """Rigid body in an axisymmetric flow, stepped on the GPU.
 
     Parameters follow examples/FlowPastSphere/flow_past_sphere.py:39-56:
@@ -73,164 +281,12 @@ class RigidBodyStepper:
                  r_sph=None, CFL=0.1, vorticity=None, oscillation=None, history=1 << 16,
                  kill_width=3, r_len=None, use_graph=True, precision="f64", moving_body=None,
                  average_psi=False, dt_cap=None, period=None, t_end=None, force_scale=None):
-        torch = _lib.torch()
-        dev = _lib.device()
-        if precision not in ("f64", "f32"):
-            raise ValueError("precision must be 'f64' or 'f32'")
-        self.precision = precision
-        self.nz, self.nr = int(nz), int(nr)
-        self.dx, self.z, self.r = rigid_grid(nz, nr, z_len, r_len)
-        dx = self.dx
-        self.use_graph = use_graph
-        self.stream = torch.cuda.Stream(device=dev)
-        ft = torch.float64 if precision == "f64" else torch.float32
-        self.field_dtype = ft
-
-        def field(a=None):
-            if a is None:
-                return torch.zeros((nr, nz), dtype=ft, device=dev)
-            a = np.asarray(a, dtype=np.float64)
-            if a.shape != (nr, nz):
-                raise ValueError(f"expected shape {(nr, nz)}, got {a.shape}")
-            return torch.from_numpy(np.ascontiguousarray(a)).to(dev).to(ft)
-
-        def table(a):
-            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
-
-        if char_func is None and moving_body is None:
-            raise ValueError("need char_func or moving_body")
-        self.t_vorticity = field(vorticity)
-        self.t_psi = field()
-        self.t_u_z = field()
-        self.t_u_r = field()
-        self.t_u_pen_z = field()
-        self.t_u_pen_r = field()
-        self.t_char_func = field(char_func)
-        self.t_work0 = field()
-        self.t_work1 = field()
-        self.t_avg_psi = field() if average_psi else None
-        self.t_scalars = torch.zeros(N_SCALARS, dtype=torch.float64, device=dev)
-        self.history = int(history)
-        self.t_drag = torch.zeros(max(self.history, 1), dtype=torch.float64, device=dev)
-        self.t_r = table(self.r)
-        self.t_z = table(self.z)
-        wz, wr = kill_weights(self.z, self.r, dx, kill_width)
-        self.t_wz = table(wz)
-        self.t_wr = table(wr)
-
-        cfg = _Config()
-        cfg.nr, cfg.nz, cfg.kill_width, cfg.history = self.nr, self.nz, kill_width, self.history
-        cfg.precision = _lib.AXS_F64 if precision == "f64" else _lib.AXS_F32
-        cfg.dx, cfg.nu, cfg.brink_lam = dx, nu, brink_lam
-        cfg.CFL = CFL
-        cfg.eps = np.finfo(float).eps
-        cfg.dt_diffusive = 0.9 * dx**2 / 4 / nu  # flow_past_sphere.py:151
-        cfg.dt_cap = np.inf if dt_cap is None else dt_cap
-        cfg.period = 0.0 if period is None else period
-        cfg.t_end = 0.0 if t_end is None else t_end
-        cfg.average_psi = 1 if average_psi else 0
-        if oscillation is None:
-            cfg.U0 = U_0
-            cfg.T_ramp = T_ramp if T_ramp is not None else 0.0
-            cfg.osc_amp, cfg.osc_omega = 0.0, 0.0
-        else:
-            cfg.U0, cfg.T_ramp = 0.0, 0.0
-            cfg.osc_amp, cfg.osc_omega = oscillation
-        if moving_body is not None:
-            cfg.moving_body = 1
-            cfg.body_z0 = moving_body["z0"]
-            cfg.body_r0 = moving_body.get("r0", 0.0)
-            cfg.body_aspect2 = moving_body.get("aspect", 1.0) ** 2
-            cfg.body_radius = moving_body["radius"]
-            cfg.body_disp = moving_body["disp"]
-            cfg.moll_zone = moving_body["moll_zone"]
-        if force_scale is not None:
-            cfg.drag_scale, cfg.drag_div = force_scale, 1.0
-        elif r_sph is not None:
-            # flow_past_sphere.py:166-175
-            cfg.drag_scale = 2 * 2 * np.pi * dx * dx * brink_lam
-            cfg.drag_div = np.pi * r_sph**2
-        else:
-            cfg.drag_scale, cfg.drag_div = 0.0, 1.0
-        self.cfg = cfg
-        bufs = _Buffers()
-        for name in ("vorticity", "psi", "u_z", "u_r", "u_pen_z", "u_pen_r", "char_func",
-                     "work0", "work1", "scalars", "drag"):
-            setattr(bufs, name, getattr(self, "t_" + name).data_ptr())
-        bufs.avg_psi = self.t_avg_psi.data_ptr() if average_psi else None
-        bufs.r = self.t_r.data_ptr()
-        bufs.z = self.t_z.data_ptr()
-        bufs.kill_wz = self.t_wz.data_ptr()
-        bufs.kill_wr = self.t_wr.data_ptr()
-        self._bufs = bufs
-        torch.cuda.synchronize(dev)
-        handle = ctypes.c_void_p()
-        _lib.call("axs_rigid_stepper_create", ctypes.byref(cfg), ctypes.byref(bufs),
-                  ctypes.byref(handle))
-        self._h = handle
-
-    # -------------------------------------------------------------- stepping
-    def step(self, n=1):
-        """Queues n timesteps on this stepper's stream (asynchronous)."""
-        torch = _lib.torch()
-        cur = torch.cuda.current_stream()
-        self.stream.wait_stream(cur)
-        _lib.call("axs_rigid_stepper_run", self._h, int(n), 1 if self.use_graph else 0,
-                  ctypes.c_void_p(self.stream.cuda_stream))
-        cur.wait_stream(self.stream)
-
-    def synchronize(self):
-        self.stream.synchronize()
-
-    # -------------------------------------------------------------- state
-    def _host(self, t):
-        self.synchronize()
-        return t.double().cpu().numpy()
-
-    def vorticity(self):
-        return self._host(self.t_vorticity)
-
-    def psi(self):
-        return self._host(self.t_psi)
-
-    def avg_psi(self):
-        return None if self.t_avg_psi is None else self._host(self.t_avg_psi)
-
-    def char_func(self):
-        return self._host(self.t_char_func)
-
-    def velocity(self):
-        """(u_z, u_r) after penalisation (the velocity ENO3 used in the last
-        step), computed on demand from psi and chi."""
-        _lib.call("axs_rigid_stepper_velocity", self._h, ctypes.c_void_p(self.stream.cuda_stream))
-        return self._host(self.t_u_pen_z), self._host(self.t_u_pen_r)
-
-    def scalars(self):
-        return self._host(self.t_scalars)
-
-    def time(self):
-        return float(self.scalars()[SC_T])
-
-    def steps_done(self):
-        return int(self.scalars()[SC_IT])
-
-    def dt(self):
-        return float(self.scalars()[SC_DT])
-
-    def drag_history(self):
-        """Drag coefficient (or force) of every step so far, FP64."""
-        n = min(self.steps_done(), self.history)
-        return self._host(self.t_drag)[:n].copy()
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            try:
-                self.synchronize()
-                _lib.load().axs_rigid_stepper_destroy(h)
-            except Exception:
-                pass
-            self._h = None
+        case = dict(nu=nu, brink_lam=brink_lam, char_func=char_func, U_0=U_0, T_ramp=T_ramp,
+                    r_sph=r_sph, CFL=CFL, vorticity=vorticity, oscillation=oscillation,
+                    moving_body=moving_body, average_psi=average_psi, dt_cap=dt_cap,
+                    period=period, t_end=t_end, force_scale=force_scale)
+        super().__init__(nz, nr, z_len, [case], kill_width=kill_width, r_len=r_len,
+                         use_graph=use_graph, precision=precision, history=history)
 
 
 class OscillatingSpheroidStepper(RigidBodyStepper):

@@ -1,0 +1,81 @@
+"""Synthetic case sets for the batched rigid-body form.
+
+Each case is flow past an axisymmetric body started from rest with the
+free-stream ramp of examples/FlowPastSphere/flow_past_sphere.py:39-56, on
+z in [0, 1], r in [0, 0.5] with nz = 2 nr.  Bodies are level sets from the
+reference drivers, blended by the smooth Heaviside of
+kernels/smooth_Heaviside.py:5-14 (computed on the GPU by the drop-in):
+
+* sphere:            a - |x - c|                     (flow_past_sphere.py:44-46)
+* prolate spheroid:  b - sqrt((z - zc)^2 / AR^2 + r^2)
+                     (spheroid_streaming.py:92-95), b = a / AR^(1/3) so the
+                     volume equals the sphere's
+* torus:             a_t - sqrt((z - zc)^2 + (r - R_c)^2)
+                     (torus_streaming.py:94 with cross-section AR = 1)
+
+All bodies sit at z = 0.5.  Choices: equivalent radius a = 0.075, torus
+minor radius a_t = a / 2 and R_c = ratio * a_t, ratio ~ U[2, 4];
+nu = 2 a U / Re with Re ~ U[10, 500]; moll_zone = sqrt(2) dx as the rigid
+drivers use.
+"""
+import numpy as np
+
+A_EQ = 0.075
+SIZES_NR = (128, 192, 256, 384, 512)
+BODIES = ("sphere", "spheroid", "torus")
+
+
+def case_spec(seed):
+    """Parameters of case `seed` (numpy.random.default_rng(seed))."""
+    rng = np.random.default_rng(seed)
+    nr = int(rng.choice(SIZES_NR))
+    re = float(rng.uniform(10.0, 500.0))
+    body = str(rng.choice(BODIES))
+    spec = dict(seed=seed, nr=nr, nz=2 * nr, Re=re, nu=2 * A_EQ * 1.0 / re, body=body)
+    if body == "spheroid":
+        spec["AR"] = float(rng.uniform(1.5, 3.0))
+    elif body == "torus":
+        spec["ratio"] = float(rng.uniform(2.0, 4.0))
+    return spec
+
+
+def body_phi(spec, Z, R):
+    """Level set of the case's body on the grid (positive inside)."""
+    zc = 0.5
+    if spec["body"] == "sphere":
+        return A_EQ - np.sqrt((Z - zc) ** 2 + R**2)
+    if spec["body"] == "spheroid":
+        ar = spec["AR"]
+        b = A_EQ / ar ** (1.0 / 3.0)
+        return -np.sqrt((Z - zc) ** 2 / ar**2 + R**2) + b
+    a_t = 0.5 * A_EQ
+    rc = spec["ratio"] * a_t
+    return -np.sqrt((Z - zc) ** 2 + (R - rc) ** 2) + a_t
+
+
+def case_inputs(spec, nz, nr):
+    """(dx, char_func) of a case on its grid, the mask from the GPU drop-in."""
+    from pyaxisymflow_amd.kernels.smooth_Heaviside import smooth_Heaviside
+    from pyaxisymflow_amd.stepper import rigid_grid
+
+    dx, z, r = rigid_grid(nz, nr, 1.0, 0.5)
+    Z, R = np.meshgrid(z, r)
+    phi = body_phi(spec, Z, R)
+    chi = np.zeros_like(phi)
+    smooth_Heaviside(chi, phi, np.sqrt(2.0) * dx)
+    return dx, chi
+
+
+def rigid_case_args(spec, char_func):
+    """Per-case arguments of RigidBatchStepper for a case spec."""
+    return dict(nu=spec["nu"], brink_lam=1e4, char_func=char_func, U_0=1.0,
+                T_ramp=20 * A_EQ / 1.0, r_sph=A_EQ, CFL=0.1)
+
+
+def grouped_cases(seeds):
+    """Case specs grouped by grid: {(nz, nr): [spec, ...]}, groups in size order."""
+    groups = {}
+    for s in seeds:
+        sp = case_spec(s)
+        groups.setdefault((sp["nz"], sp["nr"]), []).append(sp)
+    return dict(sorted(groups.items(), key=lambda kv: kv[0][1]))
