@@ -153,6 +153,19 @@ axs_status axs_extrapolate_lsq_first_order_f64(int16_t* current_flag,
                                                int max_sweeps, int* sweeps_done_host,
                                                void* stream);
 
+/* ---- A15: narrow-band signed-distance reinitialisation.  Replaces the
+ * call skfmm.distance(phi, dx=dx, narrow=band) (scikit-fmm 2022.8.15,
+ * order 2) at examples/SoftSphereInTaylorGreenVortex/
+ * soft_sphere_in_taylor_green_vortex.py:159-162.  out = signed distance to
+ * phi's zero level set where |d| <= narrow and phi elsewhere; masked[c] = 1
+ * (if masked is not null) where out keeps phi, as skfmm's masked array.
+ * The discrete equations (linear interface distances, second-order upwind
+ * eikonal update) are solved by `iterations` Jacobi sweeps instead of fast
+ * marching (< 0: 2 ceil(narrow / dx) + 8).  phi and out may alias. */
+axs_status axs_reinit_distance_f64(const double* phi, double* out, uint8_t* masked, int nr,
+                                   int nz, double dx, double narrow, int iterations,
+                                   void* stream);
+
 /* ---- A16: the rigid-body driver step, device resident.
  * examples/FlowPastSphere/flow_past_sphere.py:107-188 (static body, free
  * stream with the sine ramp, drag coefficient Cd) and
@@ -245,6 +258,41 @@ axs_status axs_rigid_stepper_run_stage(axs_rigid_stepper* st, int stage, int rep
 axs_status axs_rigid_stepper_velocity(axs_rigid_stepper* st, void* stream);
 /* Recomputes the ramp factor and U_pen for the time now in scalars[0]. */
 axs_status axs_rigid_stepper_reset_scalars(axs_rigid_stepper* st, void* stream);
+
+/* ---- A16 (soft body): the soft-sphere driver step, device resident.
+ * examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:101-213:
+ * kill taper -> Stokes solve -> velocity -> dt = min(CFL dx / (max|u| +
+ * eps), CFL dx / sqrt(G / rho_f), 0.9 dx^2 / 4 / nu) -> reference-map ENO3
+ * -> phi pinned to the sphere of eta where phi > -3 dx -> narrow-band
+ * reinit (as axs_reinit_distance_f64) -> chi = smooth Heaviside(phi,
+ * moll_zone) -> least-squares extrapolation of eta on the doubled grid
+ * (sweeps until one fits nothing, at most lsq_sweeps) -> solid stress
+ * blended by chi -> omega += dt curl(div sigma) -> ENO3 -> RK2.  FP64.
+ * The caller owns the state buffers ((nr, nz) doubles; scalars
+ * AXS_RIGID_SCALARS doubles: t, dt, dt/dx, nu dt/2, nu dt, max|u|, step
+ * counter, sweeps used in the last step, steps that exhausted lsq_sweeps;
+ * r (nr), z (nz), kill_wz (2 kill_width), kill_wr (kill_width)); the
+ * stepper owns its scratch.  nz must equal 2 nr (the extrapolation uses z
+ * as both axes of the doubled grid, as the reference does). */
+typedef struct axs_soft_stepper axs_soft_stepper;
+typedef struct {
+  int nr, nz, kill_width;
+  int lsq_sweeps;        /* upper bound on extrapolation sweeps per step */
+  int reinit_iterations; /* < 0: 2 ceil(reinit_band / dx) + 8 */
+  int pad;
+  double dx, nu, G, rho_f, CFL, eps;
+  double r_ball, z_cm, r_cm;               /* phi pin: r_ball - |eta - (z_cm, r_cm)| */
+  double moll_zone, extrap_zone, reinit_band;
+} axs_soft_config;
+typedef struct {
+  double *vorticity, *psi, *u_z, *u_r, *eta1, *eta2, *phi, *char_func, *scalars;
+  const double *r, *z, *kill_wz, *kill_wr;
+} axs_soft_buffers;
+axs_status axs_soft_stepper_create(const axs_soft_config* cfg, const axs_soft_buffers* bufs,
+                                   axs_soft_stepper** out);
+axs_status axs_soft_stepper_destroy(axs_soft_stepper* st);
+/* Queues nsteps steps on `stream` (use_graph: captured once, replayed). */
+axs_status axs_soft_stepper_run(axs_soft_stepper* st, int nsteps, int use_graph, void* stream);
 
 #ifdef __cplusplus
 }

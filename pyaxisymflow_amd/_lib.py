@@ -49,8 +49,27 @@ class RigidBuffers(ctypes.Structure):
         "avg_psi", "scalars", "drag", "r", "z", "kill_wz", "kill_wr")]
 
 
+class SoftConfig(ctypes.Structure):
+    """axs_soft_config (include/axisym.h)."""
+
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "nr", "nz", "kill_width", "lsq_sweeps", "reinit_iterations", "pad")] + [
+        (n, ctypes.c_double) for n in (
+            "dx", "nu", "G", "rho_f", "CFL", "eps", "r_ball", "z_cm", "r_cm", "moll_zone",
+            "extrap_zone", "reinit_band")]
+
+
+class SoftBuffers(ctypes.Structure):
+    """axs_soft_buffers (include/axisym.h)."""
+
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "vorticity", "psi", "u_z", "u_r", "eta1", "eta2", "phi", "char_func", "scalars", "r", "z",
+        "kill_wz", "kill_wr")]
+
+
 AXS_F64 = 0
 AXS_F32 = 1
+AXS_RIGID_SCALARS = 24
 
 
 # name -> argument types (restype axs_status = int unless listed in _RESTYPE)
@@ -79,6 +98,11 @@ SIGNATURES = {
                                             ctypes.POINTER(_I), _P],
     "axs_rigid_stepper_create": [ctypes.POINTER(RigidConfig), ctypes.POINTER(RigidBuffers),
                                  ctypes.POINTER(_P)],
+    "axs_reinit_distance_f64": [_P, _P, _P, _I, _I, _D, _D, _I, _P],
+    "axs_soft_stepper_create": [ctypes.POINTER(SoftConfig), ctypes.POINTER(SoftBuffers),
+                                ctypes.POINTER(_P)],
+    "axs_soft_stepper_destroy": [_P],
+    "axs_soft_stepper_run": [_P, _I, _I, _P],
     "axs_rigid_batch_create": [ctypes.POINTER(RigidConfig), _I, ctypes.POINTER(RigidBuffers),
                                ctypes.POINTER(_P)],
     "axs_rigid_stepper_destroy": [_P],
@@ -179,7 +203,8 @@ class Staged:
 
     def _tdtype(self, dtype):
         t = torch()
-        return {np.float64: t.float64, np.float32: t.float32, np.int16: t.int16}[dtype]
+        return {np.float64: t.float64, np.float32: t.float32, np.int16: t.int16,
+                np.uint8: t.uint8}[dtype]
 
     def tensor(self, a, name, dtype=None, out=False, copy_in=True):
         dtype = dtype or self.dtype

@@ -1,0 +1,205 @@
+// Narrow-band signed-distance reinitialisation (row A15).
+//
+// The reference calls scikit-fmm 2022.8.15 (skfmm.distance with order 2,
+// soft_sphere_in_taylor_green_vortex.py:159-162), a sequential fast
+// marching method.  Marching in order of distance does not map onto a GPU;
+// here the same discrete equations are solved by Jacobi iteration, which is
+// data parallel:
+//   init    interface cells (phi == 0, or a sign change towards an axis
+//           neighbour) are frozen at the distance interpolated linearly to
+//           the crossing along each axis, d_a = dx phi / (phi - phi_nb),
+//           combined as 1 / sqrt(sum_a 1 / d_a^2), with the sign of phi;
+//   sweep   every other cell takes the second-order upwind eikonal update
+//           |grad d| = 1 from its known axis neighbours (values relative to
+//           its own side of the interface), keeping the smaller of old and
+//           new (Jacobi, ping-pong buffers);
+//   final   |d| > narrow is masked and keeps phi.
+// FP64 with the operation order of the CPU restatement used by the tests.
+#include "axs_reinit.hpp"
+
+#include <cmath>
+
+namespace axs {
+
+constexpr double kBig = 1e300;
+
+int reinit_iterations(double narrow, double dx) {
+  return 2 * static_cast<int>(std::ceil(narrow / dx)) + 8;
+}
+
+__global__ void reinit_init_kernel(const double* __restrict__ phi, double* __restrict__ D,
+                                   double* __restrict__ dist, uint8_t* __restrict__ frozen,
+                                   int nr, int nz, double dx) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= nz) return;
+  const int64_t c = (int64_t)i * nz + j;
+  const double p = phi[c];
+  double inv = 0.0;
+  bool border = false;
+#pragma unroll
+  for (int axis = 0; axis < 2; ++axis) {
+    double dmin = HUGE_VAL;
+#pragma unroll
+    for (int s = -1; s <= 1; s += 2) {
+      const int ii = axis == 0 ? i + s : i, jj = axis == 0 ? j : j + s;
+      if (ii < 0 || ii >= nr || jj < 0 || jj >= nz) continue;
+      const double nb = phi[(int64_t)ii * nz + jj];
+      if (p * nb < 0.0) dmin = fmin(dmin, dx * p / (p - nb));
+    }
+    if (dmin < HUGE_VAL) {
+      border = true;
+      inv = inv + 1.0 / dmin / dmin;
+    }
+  }
+  double dv = 0.0;
+  bool fr = false;
+  if (border) {
+    const double mag = sqrt(1.0 / inv);
+    dv = (p < 0.0) ? -mag : mag;
+    fr = true;
+  }
+  if (p == 0.0) {
+    dv = 0.0;
+    fr = true;
+  }
+  dist[c] = dv;
+  frozen[c] = fr ? 1 : 0;
+  D[c] = fr ? dv : kBig;
+}
+
+// Upwind eikonal update of cell (i, j) from the iterate D (kBig = unknown),
+// neighbour values taken relative to the cell's side.
+__device__ __forceinline__ double eikonal_update(const double* __restrict__ D, int i, int j,
+                                                 int nr, int nz, double side, double dx) {
+  const double idx2 = 1.0 / (dx * dx);
+  const double aa = 9.0 / 4.0;
+  double a = 0.0, b = 0.0, c = 0.0, vmax = -kBig, vmin = kBig;
+#pragma unroll
+  for (int axis = 0; axis < 2; ++axis) {
+    double v1 = kBig, v2 = kBig;
+#pragma unroll
+    for (int s = -1; s <= 1; s += 2) {
+      const int i1 = axis == 0 ? i + s : i, j1 = axis == 0 ? j : j + s;
+      const int i2 = axis == 0 ? i + 2 * s : i, j2 = axis == 0 ? j : j + 2 * s;
+      double n1 = kBig, n2 = kBig;
+      if (i1 >= 0 && i1 < nr && j1 >= 0 && j1 < nz) {
+        const double v = D[(int64_t)i1 * nz + j1];
+        if (v < kBig) n1 = side * v;
+      }
+      if (i2 >= 0 && i2 < nr && j2 >= 0 && j2 < nz) {
+        const double v = D[(int64_t)i2 * nz + j2];
+        if (v < kBig) n2 = side * v;
+      }
+      if (n1 < v1) {
+        v2 = (n2 <= n1) ? n2 : kBig;
+        v1 = n1;
+      }
+    }
+    const bool second = v1 < kBig && v2 < kBig;
+    const bool first = v1 < kBig && !second;
+    const double w1 = (v1 < kBig) ? v1 : 0.0;
+    const double w2 = second ? v2 : 0.0;
+    const double tp = (4.0 * w1 - w2) / 3.0;
+    a = a + (second ? idx2 * aa : (first ? idx2 : 0.0));
+    b = b - (second ? idx2 * 2.0 * aa * tp : (first ? idx2 * 2.0 * w1 : 0.0));
+    c = c + (second ? idx2 * aa * tp * tp : (first ? idx2 * w1 * w1 : 0.0));
+    vmax = fmax(vmax, second ? tp : (first ? w1 : -kBig));
+    vmin = fmin(vmin, (second || first) ? w1 : kBig);
+  }
+  if (!(a > 0.0)) return kBig;
+  c = c - 1.0;
+  const double det = b * b - 4.0 * a * c;
+  double x = (-b + sqrt(fmax(det, 0.0))) / (2.0 * a);
+  if (det < 0.0 || x < vmax) x = vmin + dx;
+  return x;
+}
+
+__global__ void reinit_sweep_kernel(const double* __restrict__ Din, double* __restrict__ Dout,
+                                    const double* __restrict__ phi,
+                                    const uint8_t* __restrict__ frozen, int nr, int nz,
+                                    double dx) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= nz) return;
+  const int64_t c = (int64_t)i * nz + j;
+  const double dc = Din[c];
+  if (frozen[c]) {
+    Dout[c] = dc;
+    return;
+  }
+  const double side = (phi[c] < 0.0) ? -1.0 : 1.0;
+  const double m = eikonal_update(Din, i, j, nr, nz, side, dx);
+  const double cur = (dc < kBig) ? side * dc : kBig;
+  Dout[c] = (m >= cur) ? dc : side * m;
+}
+
+__global__ void reinit_final_kernel(const double* __restrict__ phi, const double* __restrict__ D,
+                                    const double* __restrict__ dist,
+                                    const uint8_t* __restrict__ frozen, double* __restrict__ out,
+                                    uint8_t* __restrict__ masked, int64_t n, double narrow) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double d = frozen[c] ? dist[c] : D[c];
+  const bool m = !(fabs(d) <= narrow);
+  out[c] = m ? phi[c] : d;
+  if (masked) masked[c] = m ? 1 : 0;
+}
+
+hipError_t launch_reinit(const double* phi, double* out, uint8_t* masked, int nr, int nz,
+                         double dx, double narrow, int iterations, const ReinitWork& w,
+                         hipStream_t s) {
+  const dim3 grid(ceil_div(nz, 256), nr);
+  reinit_init_kernel<<<grid, 256, 0, s>>>(phi, w.d0, w.dist, w.frozen, nr, nz, dx);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  double* a = w.d0;
+  double* b = w.d1;
+  for (int q = 0; q < iterations; ++q) {
+    reinit_sweep_kernel<<<grid, 256, 0, s>>>(a, b, phi, w.frozen, nr, nz, dx);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    double* t = a;
+    a = b;
+    b = t;
+  }
+  const int64_t n = (int64_t)nr * nz;
+  reinit_final_kernel<<<ceil_div(n, 256), 256, 0, s>>>(phi, a, w.dist, w.frozen, out, masked, n,
+                                                       narrow);
+  return hipGetLastError();
+}
+
+}  // namespace axs
+
+using namespace axs;
+
+extern "C" {
+
+axs_status axs_reinit_distance_f64(const double* phi, double* out, uint8_t* masked, int nr,
+                                   int nz, double dx, double narrow, int iterations,
+                                   void* stream) {
+  AXS_CHECK_ARG(phi && out, "null pointer");
+  AXS_CHECK_ARG(nr >= 1 && nz >= 1, "empty grid");
+  AXS_CHECK_ARG(dx > 0.0 && narrow > 0.0, "dx and narrow must be positive");
+  if (iterations < 0) iterations = reinit_iterations(narrow, dx);
+  AXS_CHECK_ARG(iterations <= 1 << 16, "too many iterations");
+  const size_t n = (size_t)nr * nz;
+  ReinitWork w{};
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&w.d0), sizeof(double) * n);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.d1), sizeof(double) * n);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.dist), sizeof(double) * n);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.frozen), n);
+  hipStream_t s = as_stream(stream);
+  if (e == hipSuccess) e = launch_reinit(phi, out, masked, nr, nz, dx, narrow, iterations, w, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (w.d0) (void)hipFree(w.d0);
+  if (w.d1) (void)hipFree(w.d1);
+  if (w.dist) (void)hipFree(w.dist);
+  if (w.frozen) (void)hipFree(w.frozen);
+  if (e != hipSuccess) {
+    set_error(std::string("reinit: ") + hipGetErrorString(e));
+    return AXS_ERR_HIP;
+  }
+  return AXS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,433 @@
+// Device-resident timestep of the soft-body driver (row A16, soft case):
+// examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:101-213.
+// An elastic sphere tracked by its reference map (eta1, eta2) and level set
+// phi in an axisymmetric vortex; no Brinkman term.  One step, in the
+// reference's order:
+//   taper_dct2_rows   kill taper (kill_boundary_vorticity_sine.py:4-27) into
+//                     work, DCT-II of r * omega
+//   tridiag_columns   radial tridiagonal per z-mode
+//   dct3_rows         -> psi
+//   velocity          u_z, u_r from psi (compute_velocity_from_psi.py:4-17)
+//   velocity_max      max(|u_z| + |u_r|) into spread atomicMax slots
+//   dt                min(CFL dx / (max + eps), CFL dx / sqrt(G / rho_f),
+//                     0.9 dx^2 / 4 / nu) (:139-143), derived scalars
+//   refmap_eno3       eta advected (advect_refmap_via_eno3.py:8-54)
+//   pin_phi           phi = sphere of eta where phi > -3 dx (:152-156)
+//   reinit            narrow-band signed distance, masked cells keep phi
+//                     (:159-162; axs_reinit.hip)
+//   char_func         chi = smooth Heaviside(phi) (:165-166)
+//   lsq_mirror        doubled-grid phi, eta, flags
+//                     (extrapolate_eta_using_least_squares_unb.py:19-24,
+//                     extrapolate_using_least_squares.py:33-35)
+//   lsq_sweeps        least-squares extrapolation sweeps until a sweep
+//                     fits nothing (later sweeps exit at once)
+//   lsq_back          eta = doubled eta's physical half (:29-30)
+//   solid_sigma       stresses (solid_sigma.py:4-29), blended by chi (:197-199)
+//   div_tau           omega += dt curl(div sigma) (div_tau.py:4-34)
+//   advect            ENO3 conservative (advect_vorticity_via_eno3.py:8-44)
+//   diffuse           RK2 (diffusion_RK2.py:4-44) -> home buffer
+//   finalize          t += dt, it += 1, LS convergence record
+// FP64 throughout.  Every scalar stays on the device; the step is captured
+// into a hipGraph (fills are kernels, no memset nodes).
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "axs_eno3.hpp"
+#include "axs_fused.hpp"
+#include "axs_lsq.hpp"
+#include "axs_reinit.hpp"
+#include "axs_stencil.hpp"
+#include "axs_stokes.hpp"
+
+namespace axs {
+
+// scalar slots of a soft step (AXS_RIGID_SCALARS doubles)
+enum : int {
+  kSsT = 0,         // time
+  kSsDt,            // dt of the current step
+  kSsDtByDx,        // dt / dx
+  kSsHalfNuDt,      // 0.5 nu dt
+  kSsNuDt,          // nu dt
+  kSsMaxU,          // max(|u_z| + |u_r|)
+  kSsIt,            // step counter
+  kSsLsqSweeps,     // sweeps that fitted cells in the last step
+  kSsLsqUnconverged,  // steps whose last allowed sweep still fitted cells
+  kSsCount
+};
+static_assert(kSsCount <= AXS_RIGID_SCALARS, "soft scalar slots exceed the array");
+
+struct SoftConfig {
+  int nr, nz, kill_width, lsq_sweeps, reinit_iterations;
+  double dx, nu, G, rho_f, CFL, eps, r_ball, z_cm, r_cm, moll_zone, extrap_zone, reinit_band;
+};
+
+// ------------------------------------------------------------------ kernels
+
+__global__ void soft_velocity_max_kernel(const double* __restrict__ uz,
+                                         const double* __restrict__ ur, int64_t n,
+                                         double* __restrict__ slots) {
+  __shared__ double wm[4];
+  double m = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    m = fmax(m, fabs(uz[k]) + fabs(ur[k]));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+    atomicMax(reinterpret_cast<unsigned long long*>(slots + (blockIdx.x % kMaxSlots) * kSlotStride),
+              static_cast<unsigned long long>(__double_as_longlong(m)));
+  }
+}
+
+// One block of 64: the dt rule, derived scalars, cleared slots.
+__global__ void soft_dt_kernel(double* __restrict__ sc, double* __restrict__ slots,
+                               SoftConfig cfg) {
+  double m = __longlong_as_double(static_cast<long long>(
+      __hip_atomic_load(reinterpret_cast<unsigned long long*>(slots + threadIdx.x * kSlotStride),
+                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  slots[threadIdx.x * kSlotStride] = 0.0;
+  if (threadIdx.x != 0) return;
+  double dt = cfg.CFL * cfg.dx / (m + cfg.eps);
+  const double d2 = cfg.CFL * cfg.dx / sqrt(cfg.G / cfg.rho_f);
+  const double d3 = 0.9 * (cfg.dx * cfg.dx) / 4.0 / cfg.nu;  // 0.9 * dx**2 / 4 / nu
+  if (d2 < dt) dt = d2;
+  if (d3 < dt) dt = d3;
+  sc[kSsMaxU] = m;
+  sc[kSsDt] = dt;
+  sc[kSsDtByDx] = dt / cfg.dx;
+  sc[kSsHalfNuDt] = 0.5 * cfg.nu * dt;
+  sc[kSsNuDt] = cfg.nu * dt;
+}
+
+// eta <- advected eta; phi pinned to the sphere of eta in phi > -3 dx.
+__global__ void soft_pin_kernel(double* __restrict__ e1, double* __restrict__ e2,
+                                const double* __restrict__ e1n, const double* __restrict__ e2n,
+                                double* __restrict__ phi, int64_t n, SoftConfig cfg) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double a = e1n[c], b = e2n[c];
+  e1[c] = a;
+  e2[c] = b;
+  const double da = a - cfg.z_cm, db = b - cfg.r_cm;
+  const double orig = -sqrt(da * da + db * db) + cfg.r_ball;
+  if (phi[c] > -3 * cfg.dx) phi[c] = orig;
+}
+
+// Doubled grid of extrapolate_eta_using_least_squares_unb.py:19-24 and the
+// flags of extrapolate_using_least_squares.py:33-35 (lower bound 0).
+__global__ void soft_lsq_mirror_kernel(const double* __restrict__ phi,
+                                       const double* __restrict__ chi,
+                                       const double* __restrict__ e1,
+                                       const double* __restrict__ e2, double* __restrict__ e1d,
+                                       double* __restrict__ e2d, int16_t* __restrict__ cur,
+                                       int16_t* __restrict__ tgt, int nr, int nz,
+                                       double extrap_zone) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int I = blockIdx.y;  // doubled row, [0, 2 nr)
+  if (j >= nz) return;
+  const bool upper = I >= nr;
+  const int i = upper ? I - nr : nr - 1 - I;
+  const int64_t c = (int64_t)i * nz + j;
+  const double in = (chi[c] > 0.5) ? 1.0 : 0.0;
+  const double pd = -phi[c];
+  const double a = in * e1[c];
+  const double b = in * e2[c];
+  const int64_t d = (int64_t)I * nz + j;
+  e1d[d] = a;
+  e2d[d] = upper ? b : -b;
+  cur[d] = (pd < 0.0) ? 1 : 0;
+  tgt[d] = (pd < extrap_zone) ? 1 : 0;
+}
+
+__global__ void soft_lsq_back_kernel(double* __restrict__ e1, double* __restrict__ e2,
+                                     const double* __restrict__ e1d,
+                                     const double* __restrict__ e2d, int64_t n, int64_t off) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  e1[c] = e1d[off + c];
+  e2[c] = e2d[off + c];
+}
+
+__global__ void soft_blend_kernel(double* __restrict__ s11, double* __restrict__ s12,
+                                  double* __restrict__ s22, const double* __restrict__ chi,
+                                  int64_t n) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double x = chi[c];
+  s11[c] = x * s11[c];
+  s12[c] = x * s12[c];
+  s22[c] = x * s22[c];
+}
+
+__global__ void soft_finalize_kernel(double* __restrict__ sc,
+                                     const unsigned* __restrict__ counters, int sweeps) {
+  if (threadIdx.x != 0) return;
+  int used = 0;
+  while (used < sweeps && counters[used] != 0u) ++used;
+  sc[kSsLsqSweeps] = double(used);
+  if (counters[sweeps - 1] != 0u) sc[kSsLsqUnconverged] = sc[kSsLsqUnconverged] + 1.0;
+  sc[kSsT] = sc[kSsT] + sc[kSsDt];
+  sc[kSsIt] = sc[kSsIt] + 1.0;
+}
+
+__global__ void soft_zero_kernel(double* __restrict__ p, int64_t n, int64_t stride) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k * stride] = 0.0;
+}
+
+// ------------------------------------------------------------------ stepper
+
+struct SoftBuffers {
+  double *vorticity, *psi, *u_z, *u_r, *eta1, *eta2, *phi, *char_func, *scalars;
+  const double *r, *z, *kill_wz, *kill_wr;
+};
+
+struct SoftStepper {
+  SoftConfig cfg{};
+  SoftBuffers buf{};
+  StokesPlan* plan = nullptr;
+  std::vector<void*> allocs;
+  double *w0 = nullptr, *w1 = nullptr, *tmp = nullptr, *spec = nullptr;
+  double *e1n = nullptr, *e2n = nullptr, *slots = nullptr;
+  double *e1d = nullptr, *e2d = nullptr;
+  int16_t *cur = nullptr, *tgt = nullptr;
+  double *s11 = nullptr, *s12 = nullptr, *s22 = nullptr;
+  double *g1z = nullptr, *g1r = nullptr, *g2z = nullptr, *g2r = nullptr;
+  double *tz = nullptr, *tr = nullptr;
+  ReinitWork rw{};
+  LsqWork lw{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  hipStream_t graph_stream = nullptr;
+
+  template <typename P>
+  hipError_t alloc(P** p, size_t bytes) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return e;
+    allocs.push_back(q);
+    *p = static_cast<P*>(q);
+    return hipSuccess;
+  }
+
+  hipError_t init(const SoftConfig& c, const SoftBuffers& b) {
+    cfg = c;
+    buf = b;
+    plan = new (std::nothrow) StokesPlan();
+    if (!plan) return hipErrorOutOfMemory;
+    FftPlanArgs fp{};
+    fp.M = cfg.nz / 2;
+    int m = fp.M;
+    for (int rdx : {4, 2, 3, 5})
+      while (m % rdx == 0 && fp.nstages < kMaxFftStages) {
+        fp.radix[fp.nstages++] = rdx;
+        m /= rdx;
+      }
+    if (m != 1) return hipErrorInvalidValue;
+    plan->fft = fp;
+    hipError_t e = plan->init(cfg.nr, cfg.nz, cfg.dx);
+    if (e != hipSuccess) return e;
+    const size_t n = (size_t)cfg.nr * cfg.nz, nd = 2 * n, B = sizeof(double);
+    double** fields[] = {&w0, &w1, &tmp, &spec, &e1n, &e2n, &s11, &s12, &s22,
+                         &g1z, &g1r, &g2z, &g2r, &tz, &tr, &rw.d0, &rw.d1, &rw.dist};
+    for (double** f : fields)
+      if ((e = alloc(f, n * B)) != hipSuccess) return e;
+    if ((e = alloc(&rw.frozen, n)) != hipSuccess) return e;
+    if ((e = alloc(&e1d, nd * B)) != hipSuccess) return e;
+    if ((e = alloc(&e2d, nd * B)) != hipSuccess) return e;
+    if ((e = alloc(&cur, nd * sizeof(int16_t))) != hipSuccess) return e;
+    if ((e = alloc(&tgt, nd * sizeof(int16_t))) != hipSuccess) return e;
+    if ((e = alloc(&lw.nx, nd * B)) != hipSuccess) return e;
+    if ((e = alloc(&lw.ny, nd * B)) != hipSuccess) return e;
+    if ((e = alloc(&lw.pend, nd)) != hipSuccess) return e;
+    if ((e = alloc(&lw.counters, sizeof(unsigned) * cfg.lsq_sweeps)) != hipSuccess) return e;
+    if ((e = alloc(&slots, B * kSlotsPerCase)) != hipSuccess) return e;
+    // scratch the reference allocates once and never fully rewrites
+    // (gradient and tau scratch keep zeros where they are never written)
+    for (double* f : {g1z, g1r, g2z, g2r, tz, tr, s11, s12, s22}) {
+      soft_zero_kernel<<<ceil_div((int64_t)n, 256), 256>>>(f, (int64_t)n, 1);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    soft_zero_kernel<<<1, 64>>>(slots, kMaxSlots, kSlotStride);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return hipDeviceSynchronize();
+  }
+
+  ~SoftStepper() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void* p : allocs) (void)hipFree(p);
+    delete plan;
+  }
+
+  hipError_t enqueue_step(hipStream_t s) {
+    const int nr = cfg.nr, nz = cfg.nz;
+    const int64_t n = (int64_t)nr * nz;
+    const dim3 g2(ceil_div(nz, 256), nr);
+    double* sc = buf.scalars;
+    hipError_t e;
+#define AXS_TRY_STEP(...)                              \
+  do {                                                 \
+    if ((e = (__VA_ARGS__)) != hipSuccess) return e;   \
+  } while (0)
+#define AXS_TRY_LAUNCH()                                       \
+  do {                                                         \
+    if ((e = hipGetLastError()) != hipSuccess) return e;       \
+  } while (0)
+    AXS_TRY_STEP(launch_taper_dct2_rows<double>(*plan, buf.vorticity, w0, spec, buf.kill_wz,
+                                                buf.kill_wr, cfg.kill_width, s));
+    AXS_TRY_STEP(launch_tridiag<double>(*plan, spec, s));
+    AXS_TRY_STEP(launch_dct3_rows<double>(*plan, spec, buf.psi, nr, s));
+    AXS_TRY_STEP(launch_velocity_from_psi<double>(buf.u_z, buf.u_r, buf.psi, buf.r, nr, nz,
+                                                  cfg.dx, s));
+    const int mb = std::min<int64_t>(ceil_div(n, 256), 1024);
+    soft_velocity_max_kernel<<<mb, 256, 0, s>>>(buf.u_z, buf.u_r, n, slots);
+    AXS_TRY_LAUNCH();
+    soft_dt_kernel<<<1, 64, 0, s>>>(sc, slots, cfg);
+    AXS_TRY_LAUNCH();
+    AXS_TRY_STEP(launch_eno3_refmap<double>(buf.eta1, buf.eta2, buf.u_z, buf.u_r, e1n, e2n, nr,
+                                            nz, Scalar<double>{0.0, sc + kSsDtByDx}, s));
+    soft_pin_kernel<<<ceil_div(n, 256), 256, 0, s>>>(buf.eta1, buf.eta2, e1n, e2n, buf.phi, n,
+                                                     cfg);
+    AXS_TRY_LAUNCH();
+    AXS_TRY_STEP(launch_reinit(buf.phi, buf.phi, nullptr, nr, nz, cfg.dx, cfg.reinit_band,
+                               cfg.reinit_iterations, rw, s));
+    AXS_TRY_STEP(launch_heaviside<double>(buf.char_func, buf.phi, cfg.moll_zone, n, s));
+    soft_lsq_mirror_kernel<<<dim3(ceil_div(nz, 256), 2 * nr), 256, 0, s>>>(
+        buf.phi, buf.char_func, buf.eta1, buf.eta2, e1d, e2d, cur, tgt, nr, nz, cfg.extrap_zone);
+    AXS_TRY_LAUNCH();
+    AXS_TRY_STEP(launch_fill_u32(lw.counters, cfg.lsq_sweeps, 0u, s));
+    for (int q = 0; q < cfg.lsq_sweeps; ++q)
+      AXS_TRY_STEP(launch_lsq_sweep(cur, tgt, e1d, e2d, buf.z, buf.z, 2 * nr, nz, lw, q, s));
+    soft_lsq_back_kernel<<<ceil_div(n, 256), 256, 0, s>>>(buf.eta1, buf.eta2, e1d, e2d, n, n);
+    AXS_TRY_LAUNCH();
+    AXS_TRY_STEP(launch_solid_sigma<double>(s11, s12, s22, cfg.G, cfg.dx, buf.eta1, buf.eta2, g1z,
+                                            g1r, g2z, g2r, nr, nz, s));
+    soft_blend_kernel<<<ceil_div(n, 256), 256, 0, s>>>(s11, s12, s22, buf.char_func, n);
+    AXS_TRY_LAUNCH();
+    AXS_TRY_STEP(launch_div_tau<double>(w0, tz, tr, s11, s12, s22, buf.r, nr, nz,
+                                        Scalar<double>{0.0, sc + kSsDt}, cfg.dx, s));
+    AXS_TRY_STEP(launch_eno3_conservative<double>(w0, buf.u_z, buf.u_r, w1, nr, nz,
+                                                  Scalar<double>{0.0, sc + kSsDtByDx}, -1.0, s));
+    AXS_TRY_STEP(launch_rk2<double, RRow<double>>(
+        w1, tmp, buf.vorticity, RRow<double>{buf.r}, nr, nz, Scalar<double>{0.0, sc + kSsHalfNuDt},
+        Scalar<double>{0.0, sc + kSsNuDt}, cfg.dx, s));
+    soft_finalize_kernel<<<1, 64, 0, s>>>(sc, lw.counters, cfg.lsq_sweeps);
+    AXS_TRY_LAUNCH();
+#undef AXS_TRY_STEP
+#undef AXS_TRY_LAUNCH
+    return hipSuccess;
+  }
+
+  hipError_t run(int nsteps, bool use_graph, hipStream_t s) {
+    if (!use_graph) {
+      for (int q = 0; q < nsteps; ++q) {
+        hipError_t e = enqueue_step(s);
+        if (e != hipSuccess) return e;
+      }
+      return hipSuccess;
+    }
+    if (!graph_exec || graph_stream != s) {
+      if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      graph_exec = nullptr;
+      graph = nullptr;
+      hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) return e;
+      hipError_t le = enqueue_step(s);
+      e = hipStreamEndCapture(s, &graph);
+      if (le != hipSuccess) return le;
+      if (e != hipSuccess) return e;
+      e = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) return e;
+      graph_stream = s;
+    }
+    for (int q = 0; q < nsteps; ++q) {
+      hipError_t e = hipGraphLaunch(graph_exec, s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+};
+
+}  // namespace axs
+
+using namespace axs;
+
+struct axs_soft_stepper {
+  SoftStepper impl;
+};
+
+extern "C" {
+
+axs_status axs_soft_stepper_create(const axs_soft_config* c, const axs_soft_buffers* b,
+                                   axs_soft_stepper** out) {
+  AXS_CHECK_ARG(c && b && out, "null pointer");
+  *out = nullptr;
+  AXS_CHECK_ARG(c->nr >= 4 && c->nz >= 8 && c->nz % 2 == 0, "need nr >= 4 and even nz >= 8");
+  AXS_CHECK_ARG(c->nz == 2 * c->nr,
+                "the least-squares extrapolation uses z as both grid axes: need nz = 2 nr");
+  AXS_CHECK_ARG(c->kill_width >= 2 && 2 * c->kill_width <= c->nz && c->kill_width < c->nr,
+                "bad kill_width for the grid");
+  AXS_CHECK_ARG(c->lsq_sweeps >= 1 && c->lsq_sweeps <= 4096, "lsq_sweeps must be in [1, 4096]");
+  AXS_CHECK_ARG(c->dx > 0 && c->nu > 0 && c->G > 0 && c->rho_f > 0 && c->CFL > 0,
+                "dx, nu, G, rho_f and CFL must be positive");
+  AXS_CHECK_ARG(c->moll_zone > 0 && c->extrap_zone > 0 && c->reinit_band > 0,
+                "moll_zone, extrap_zone and reinit_band must be positive");
+  AXS_CHECK_ARG(b->vorticity && b->psi && b->u_z && b->u_r && b->eta1 && b->eta2 && b->phi &&
+                    b->char_func && b->scalars && b->r && b->z && b->kill_wz && b->kill_wr,
+                "null buffer");
+  axs_soft_stepper* st = new (std::nothrow) axs_soft_stepper();
+  if (!st) {
+    set_error("out of host memory");
+    return AXS_ERR_ALLOC;
+  }
+  SoftConfig cfg{};
+  cfg.nr = c->nr;
+  cfg.nz = c->nz;
+  cfg.kill_width = c->kill_width;
+  cfg.lsq_sweeps = c->lsq_sweeps;
+  cfg.reinit_iterations = c->reinit_iterations;
+  cfg.dx = c->dx;
+  cfg.nu = c->nu;
+  cfg.G = c->G;
+  cfg.rho_f = c->rho_f;
+  cfg.CFL = c->CFL;
+  cfg.eps = c->eps;
+  cfg.r_ball = c->r_ball;
+  cfg.z_cm = c->z_cm;
+  cfg.r_cm = c->r_cm;
+  cfg.moll_zone = c->moll_zone;
+  cfg.extrap_zone = c->extrap_zone;
+  cfg.reinit_band = c->reinit_band;
+  if (cfg.reinit_iterations < 0) cfg.reinit_iterations = reinit_iterations(cfg.reinit_band, cfg.dx);
+  SoftBuffers bb{b->vorticity, b->psi,     b->u_z, b->u_r, b->eta1,    b->eta2,   b->phi,
+                 b->char_func, b->scalars, b->r,   b->z,   b->kill_wz, b->kill_wr};
+  hipError_t e = st->impl.init(cfg, bb);
+  if (e != hipSuccess) {
+    set_error(std::string("soft stepper: ") + hipGetErrorString(e));
+    delete st;
+    return AXS_ERR_HIP;
+  }
+  *out = st;
+  return AXS_OK;
+}
+
+axs_status axs_soft_stepper_destroy(axs_soft_stepper* st) {
+  delete st;
+  return AXS_OK;
+}
+
+axs_status axs_soft_stepper_run(axs_soft_stepper* st, int nsteps, int use_graph, void* stream) {
+  AXS_CHECK_ARG(st, "null stepper");
+  AXS_CHECK_ARG(nsteps >= 0, "negative step count");
+  AXS_CHECK_ARG(!use_graph || stream, "graph capture needs a non-null stream");
+  AXS_HIP(st->impl.run(nsteps, use_graph != 0, as_stream(stream)));
+  return AXS_OK;
+}
+
+}  // extern "C"

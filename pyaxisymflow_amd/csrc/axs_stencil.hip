@@ -1,7 +1,7 @@
 // Pointwise and small-stencil kernels of the timestep (drop-in forms).
 // Each kernel restates one reference function; the expression order follows
 // the reference so FP64 results agree to round-off.
-#include "axs_common.hpp"
+#include "axs_stencil.hpp"
 
 #include <mutex>
 
@@ -256,14 +256,64 @@ __global__ void div_tau_stage1_kernel(T* __restrict__ tz, T* __restrict__ tr,
 // ... then omega's interior gains dt * curl(tau).
 template <typename T>
 __global__ void div_tau_stage2_kernel(T* __restrict__ w, const T* __restrict__ tz,
-                                      const T* __restrict__ tr, int nr, int nz, T dt,
+                                      const T* __restrict__ tr, int nr, int nz, Scalar<T> dt_s,
                                       T two_dx) {
   int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
   int i = blockIdx.y + 1;
   if (j >= nz - 1) return;
   const int64_t c = (int64_t)i * nz + j;
+  const T dt = dt_s.get();
   w[c] = w[c] + dt * (tr[c + 1] - tr[c - 1] - tz[c + nz] + tz[c - nz]) / two_dx;
 }
+
+// ---------------------------------------------------------------- launchers
+// (device-resident steps pass dt-derived scalars from device memory)
+
+template <typename T>
+hipError_t launch_velocity_from_psi(T* u_z, T* u_r, const T* psi, const double* r, int nr, int nz,
+                                    T dx, hipStream_t s) {
+  velocity_from_psi_kernel<T, RRow<T>><<<dim3(ceil_div(nz, 256), nr), 256, 0, s>>>(
+      u_z, u_r, psi, RRow<T>{r}, nr, nz, dx);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_heaviside(T* H, const T* phi, T w, int64_t n, hipStream_t s) {
+  heaviside_kernel<T><<<ceil_div(n, 256), 256, 0, s>>>(H, phi, w, n);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_solid_sigma(T* s11, T* s12, T* s22, T G, T dx, const T* e1, const T* e2, T* e1z,
+                              T* e1r, T* e2z, T* e2r, int nr, int nz, hipStream_t s) {
+  solid_sigma_kernel<T><<<dim3(ceil_div(nz, 256), nr), 256, 0, s>>>(s11, s12, s22, G, dx, e1, e2,
+                                                                    e1z, e1r, e2z, e2r, nr, nz);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_div_tau(T* w, T* tz, T* tr, const T* t11, const T* t12, const T* t22,
+                          const double* r, int nr, int nz, Scalar<T> dt, T dx, hipStream_t s) {
+  div_tau_stage1_kernel<T, RRow<T>><<<dim3(ceil_div(nz - 2, 256), nr - 1), 256, 0, s>>>(
+      tz, tr, t11, t12, t22, RRow<T>{r}, nr, nz, T(2) * dx);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  div_tau_stage2_kernel<T><<<dim3(ceil_div(nz - 2, 256), nr - 2), 256, 0, s>>>(w, tz, tr, nr, nz,
+                                                                              dt, T(2) * dx);
+  return hipGetLastError();
+}
+
+template hipError_t launch_velocity_from_psi<double>(double*, double*, const double*,
+                                                     const double*, int, int, double,
+                                                     hipStream_t);
+template hipError_t launch_heaviside<double>(double*, const double*, double, int64_t,
+                                             hipStream_t);
+template hipError_t launch_solid_sigma<double>(double*, double*, double*, double, double,
+                                               const double*, const double*, double*, double*,
+                                               double*, double*, int, int, hipStream_t);
+template hipError_t launch_div_tau<double>(double*, double*, double*, const double*,
+                                           const double*, const double*, const double*, int, int,
+                                           Scalar<double>, double, hipStream_t);
 
 }  // namespace axs
 
@@ -393,7 +443,7 @@ axs_status axs_update_vorticity_from_solid_stress_f64(double* vorticity, double*
       tau_z, tau_r, tau11, tau12, tau22, RFull<double>{R, nz}, nr, nz, 2.0 * dx);
   AXS_LAUNCH_CHECK("div_tau_stage1");
   div_tau_stage2_kernel<double><<<dim3(ceil_div(nz - 2, 256), nr - 2), 256, 0, s>>>(
-      vorticity, tau_z, tau_r, nr, nz, dt, 2.0 * dx);
+      vorticity, tau_z, tau_r, nr, nz, Scalar<double>{dt, nullptr}, 2.0 * dx);
   AXS_LAUNCH_CHECK("div_tau_stage2");
   return AXS_OK;
 }

@@ -1,0 +1,30 @@
+// Launchers of the stencil kernels for the device-resident steps (the
+// drop-in C entry points in axs_stencil.hip use the same kernels).
+#pragma once
+
+#include "axs_common.hpp"
+
+namespace axs {
+
+// diffusion_RK2.py:4-44: w_out = RK2 step of w_in (border copied), tmp is
+// the stage-1 field.
+template <typename T, class RA>
+hipError_t launch_rk2(const T* w_in, T* tmp, T* w_out, RA R, int nr, int nz, Scalar<T> half_nu_dt,
+                      Scalar<T> nu_dt, T dx, hipStream_t s);
+// compute_velocity_from_psi.py:4-17 with r per row.
+template <typename T>
+hipError_t launch_velocity_from_psi(T* u_z, T* u_r, const T* psi, const double* r, int nr, int nz,
+                                    T dx, hipStream_t s);
+// smooth_Heaviside.py:5-14 over n values.
+template <typename T>
+hipError_t launch_heaviside(T* H, const T* phi, T w, int64_t n, hipStream_t s);
+// solid_sigma.py:4-29 (gradient scratch persists between calls).
+template <typename T>
+hipError_t launch_solid_sigma(T* s11, T* s12, T* s22, T G, T dx, const T* e1, const T* e2, T* e1z,
+                              T* e1r, T* e2z, T* e2r, int nr, int nz, hipStream_t s);
+// div_tau.py:4-34: tau (scratch persists), then omega's interior += dt curl tau.
+template <typename T>
+hipError_t launch_div_tau(T* w, T* tz, T* tr, const T* t11, const T* t12, const T* t22,
+                          const double* r, int nr, int nz, Scalar<T> dt, T dx, hipStream_t s);
+
+}  // namespace axs

@@ -322,3 +322,142 @@ class OscillatingSpheroidStepper(RigidBodyStepper):
                          dt_cap=0.01 * self.freqTimer_limit, period=self.freqTimer_limit,
                          t_end=self.tEnd, force_scale=2 * np.pi * dx**2 * 1.0 * brink_lam,
                          r_len=domain_AR, precision=precision, history=history, **kw)
+
+
+# soft-step scalar slots (axs_soft.hip)
+SS_T, SS_DT, SS_DT_BY_DX, SS_HALF_NU_DT, SS_NU_DT, SS_MAXU, SS_IT, SS_LSQ_SWEEPS, \
+    SS_LSQ_UNCONVERGED = range(9)
+
+
+class SoftSphereStepper:
+    """examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:27-215
+    on the GPU: an elastic sphere (reference map eta, shear modulus G) in an
+    axisymmetric Taylor-Green-like vortex, every stage of the loop body
+    (:101-213) device resident and replayed from a hipGraph.
+
+    Parameters and derived quantities follow the driver (:27-98): dx =
+    1 / grid_size_z, nr = domain_AR * nz, moll_zone = 4 dx, extrap_zone =
+    reinit_band = moll_zone + 4 dx, sphere of radius r_ball at (0.5, 0),
+    vorticity C (kz^2 + kr^2) sin(kz z) sin(kr r) with kz = kr = 2 pi.
+    ``lsq_sweeps`` bounds the extrapolation sweeps per step (the reference
+    loops until a sweep fits nothing; ``lsq_unconverged()`` counts steps that
+    hit the bound)."""
+
+    def __init__(self, grid_size_z=256, G=1.0, nu=1e-3, rho_f=1.0, r_ball=0.15, domain_AR=0.5,
+                 C=5e-2, lsq_sweeps=None, use_graph=True, kill_width=3):
+        torch = _lib.torch()
+        dev = _lib.device()
+        dx = 1.0 / grid_size_z
+        nz, nr = int(grid_size_z), int(domain_AR * grid_size_z)
+        self.nz, self.nr, self.dx = nz, nr, dx
+        self.use_graph = use_graph
+        self.stream = torch.cuda.Stream(device=dev)
+        moll_zone = dx * 4
+        extrap_zone = moll_zone + 4 * dx
+        z = np.linspace(0 + dx / 2, 1 - dx / 2, nz)
+        r = np.linspace(0 + dx / 2, domain_AR - dx / 2, nr)
+        self.z, self.r = z, r
+        Z, R = np.meshgrid(z, r)
+        phi = -np.sqrt((Z - 0.5) ** 2 + (R - 0.0) ** 2) + r_ball
+        kz = kr = 2 * np.pi
+        w = C * (kz**2 + kr**2) * np.sin(kz * Z) * np.sin(kr * R)
+
+        def t(a):
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+        self.t_vorticity = t(w)
+        self.t_psi = t(0 * Z)
+        self.t_u_z = t(0 * Z)
+        self.t_u_r = t(0 * Z)
+        self.t_eta1 = t(Z)
+        self.t_eta2 = t(R)
+        self.t_phi = t(phi)
+        self.t_char_func = t(0 * Z)
+        from pyaxisymflow_amd.kernels.smooth_Heaviside import smooth_Heaviside
+
+        smooth_Heaviside(self.t_char_func, self.t_phi, moll_zone)
+        self.t_scalars = torch.zeros(N_SCALARS, dtype=torch.float64, device=dev)
+        self.t_r, self.t_z = t(r), t(z)
+        wz, wr = kill_weights(z, r, dx, kill_width)
+        self.t_wz, self.t_wr = t(wz), t(wr)
+        cfg = _lib.SoftConfig()
+        cfg.nr, cfg.nz, cfg.kill_width = nr, nz, kill_width
+        cfg.lsq_sweeps = int(lsq_sweeps) if lsq_sweeps else int(np.ceil(extrap_zone / dx)) + 8
+        cfg.reinit_iterations = -1
+        cfg.dx, cfg.nu, cfg.G, cfg.rho_f, cfg.CFL = dx, nu, G, rho_f, 0.1
+        cfg.eps = np.finfo(float).eps
+        cfg.r_ball, cfg.z_cm, cfg.r_cm = r_ball, 0.5, 0.0
+        cfg.moll_zone, cfg.extrap_zone, cfg.reinit_band = moll_zone, extrap_zone, extrap_zone
+        self.cfg = cfg
+        bufs = _lib.SoftBuffers()
+        for name in ("vorticity", "psi", "u_z", "u_r", "eta1", "eta2", "phi", "char_func",
+                     "scalars", "r", "z"):
+            setattr(bufs, name, getattr(self, "t_" + name).data_ptr())
+        bufs.kill_wz, bufs.kill_wr = self.t_wz.data_ptr(), self.t_wr.data_ptr()
+        self._bufs = bufs
+        torch.cuda.synchronize(dev)
+        handle = ctypes.c_void_p()
+        _lib.call("axs_soft_stepper_create", ctypes.byref(cfg), ctypes.byref(bufs),
+                  ctypes.byref(handle))
+        self._h = handle
+
+    def step(self, n=1):
+        torch = _lib.torch()
+        cur = torch.cuda.current_stream()
+        self.stream.wait_stream(cur)
+        _lib.call("axs_soft_stepper_run", self._h, int(n), 1 if self.use_graph else 0,
+                  ctypes.c_void_p(self.stream.cuda_stream))
+        cur.wait_stream(self.stream)
+
+    def synchronize(self):
+        self.stream.synchronize()
+
+    def _host(self, t):
+        self.synchronize()
+        return t.cpu().numpy()
+
+    def vorticity(self):
+        return self._host(self.t_vorticity)
+
+    def psi(self):
+        return self._host(self.t_psi)
+
+    def velocity(self):
+        return self._host(self.t_u_z), self._host(self.t_u_r)
+
+    def eta(self):
+        return self._host(self.t_eta1), self._host(self.t_eta2)
+
+    def phi(self):
+        return self._host(self.t_phi)
+
+    def char_func(self):
+        return self._host(self.t_char_func)
+
+    def scalars(self):
+        return self._host(self.t_scalars)
+
+    def time(self):
+        return float(self.scalars()[SS_T])
+
+    def dt(self):
+        return float(self.scalars()[SS_DT])
+
+    def steps_done(self):
+        return int(self.scalars()[SS_IT])
+
+    def lsq_sweeps_last(self):
+        return int(self.scalars()[SS_LSQ_SWEEPS])
+
+    def lsq_unconverged(self):
+        return int(self.scalars()[SS_LSQ_UNCONVERGED])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self.synchronize()
+                _lib.load().axs_soft_stepper_destroy(h)
+            except Exception:
+                pass
+            self._h = None
