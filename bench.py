@@ -40,6 +40,24 @@ STAGE_BYTES_PER_CELL = {
 }
 
 
+def profile_traffic(stage, nz, nr, dtype="double"):
+    """HBM bytes per launch of a stage's kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE and WRITE_SIZE, each calibrated against the
+    copy kernel), or None when no summary covers this grid and kernel."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_fp64_{nz}x{nr}.md")
+    if not os.path.exists(path):
+        return None
+    key = f"{stage}_kernel<{dtype}"
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and key in cells[0].replace("`", ""):
+            try:
+                return round(float(cells[3]) * nz * nr)
+            except ValueError:
+                return None
+    return None
+
+
 def workload(nz, nr, seed=0):
     from pyaxisymflow_amd.stepper import rigid_grid
 
@@ -177,6 +195,40 @@ def time_batched(steps, warmup, seeds=range(64)):
     )
 
 
+def time_serving(n_requests, steps, max_active=16, seed=0):
+    """The serving form: soft-sphere requests (pyaxisymflow_amd.serve:
+    nr lognormal around 256 in [128, 1024], 5-smooth; G in [0.1, 10]).  A
+    closed-loop pass (every request queued at t = 0) measures the capacity;
+    the open-loop pass offers Poisson arrivals at 80% of it and reports the
+    time-to-solution percentiles."""
+    from pyaxisymflow_amd.serve import SoftServer, latency_summary, request_set
+
+    srv = SoftServer(max_active=max_active, chunk=100)
+    closed = request_set(n_requests, seed=seed, rate=1e9, steps=steps)
+    t0 = time.perf_counter()
+    recs = srv.run(closed)
+    t_closed = time.perf_counter() - t0
+    capacity = n_requests / t_closed
+    rate = 0.8 * capacity
+    open_set = request_set(n_requests, seed=seed, rate=rate, steps=steps)
+    t0 = time.perf_counter()
+    recs2 = srv.run(open_set)
+    t_open = time.perf_counter() - t0
+    s = latency_summary(recs2)
+    cells = sum(r["nr"] * r["grid_size_z"] for r in open_set) * steps
+    return dict(
+        p50_s=round(s["p50_s"], 4), p99_s=round(s["p99_s"], 4), mean_s=round(s["mean_s"], 4),
+        unit="s (case time-to-solution: arrival to last step done)",
+        requests=n_requests, steps_per_request=steps, max_active=max_active,
+        nr_set=sorted({r["nr"] for r in open_set}),
+        capacity_requests_per_s=round(capacity, 3), offered_rate_per_s=round(rate, 3),
+        offered_load=0.8, open_loop_wall_s=round(t_open, 3),
+        closed_loop_wall_s=round(t_closed, 3),
+        closed_loop_mcell_steps_per_s=round(cells / t_closed / 1e6, 2),
+        lsq_unconverged_steps=int(sum(r["lsq_unconverged"] for r in recs + recs2)),
+    )
+
+
 def cpu_baseline(nz, nr, steps):
     """The oracle (numpy restatement of the reference path with the dense
     eig-route Poisson solve, BLAS on all cores) on the same workload."""
@@ -225,6 +277,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-steps", type=int, default=2000)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every case is independent and this build runs on "
@@ -271,12 +326,18 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profile_traffic(dom, nz, nr),
+            "traffic_unit": "B per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)",
             "bytes_per_cell": bpc,
         },
     }
+    forms = {}
     if not args.no_batched:
-        out["forms"] = {"batched_fp32": time_batched(args.batched_steps, 2)}
+        forms["batched_fp32"] = time_batched(args.batched_steps, 2)
+    if not args.no_serve:
+        forms["serving_soft"] = time_serving(args.serve_requests, args.serve_steps)
+    if forms:
+        out["forms"] = forms
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nz, nr, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -67,12 +67,14 @@ __global__ void lsq_fit_kernel(const int16_t* __restrict__ cur, const int16_t* _
                                int n0, int n1, double* __restrict__ nx, double* __restrict__ ny,
                                int8_t* __restrict__ pend, unsigned* __restrict__ counter,
                                const unsigned* __restrict__ prev_counter) {
+  // A sweep after one that fitted nothing does nothing (the reference loop
+  // has ended); its commit sees a zero counter and exits too.
+  if (prev_counter && *prev_counter == 0u) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;  // column (x)
   const int j = blockIdx.y;                              // row (y)
   if (i >= n1) return;
   const int64_t c = (int64_t)j * n1 + i;
   pend[c] = 0;
-  if (prev_counter && *prev_counter == 0u) return;  // previous sweep found nothing
   if (!(cur[c] ^ tgt[c])) return;
   short local_sum = 0, lx = 0, ly = 0;
   for (int jl = -1; jl <= 1; ++jl)
@@ -135,7 +137,8 @@ __global__ void lsq_fit_kernel(const int16_t* __restrict__ cur, const int16_t* _
 __global__ void lsq_commit_kernel(int16_t* __restrict__ cur, double* __restrict__ ex,
                                   double* __restrict__ ey, const double* __restrict__ nx,
                                   const double* __restrict__ ny, const int8_t* __restrict__ pend,
-                                  int64_t n) {
+                                  int64_t n, const unsigned* __restrict__ counter) {
+  if (*counter == 0u) return;  // nothing fitted (or the loop has ended)
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n || !pend[c]) return;
   ex[c] = nx[c];
@@ -158,7 +161,8 @@ hipError_t launch_lsq_sweep(int16_t* cur, const int16_t* tgt, double* ex, double
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int64_t n = (int64_t)n0 * n1;
-  lsq_commit_kernel<<<ceil_div(n, 256), 256, 0, s>>>(cur, ex, ey, w.nx, w.ny, w.pend, n);
+  lsq_commit_kernel<<<ceil_div(n, 256), 256, 0, s>>>(cur, ex, ey, w.nx, w.ny, w.pend, n,
+                                                      w.counters + sweep);
   return hipGetLastError();
 }
 

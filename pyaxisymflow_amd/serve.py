@@ -1,0 +1,157 @@
+"""Serving form: soft-sphere simulations as requests, continuously batched
+onto one GPU.
+
+A request is one run of the soft-sphere loop
+(examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:101-213)
+for a fixed number of steps, with its own grid and shear modulus.  The
+server admits queued requests while a slot is free and their device
+footprint fits the HBM budget, gives each admitted request its own
+device-resident SoftSphereStepper on its own stream, and keeps every active
+request supplied with work in chunks of graph-replayed steps; a request
+that finishes frees its slot and memory at once, so new arrivals join
+between chunks of the running ones (continuous batching).  Requests share
+nothing but the device, so a request's results do not depend on what else
+runs beside it.
+
+Time-to-solution of a request = completion time - arrival time (queueing,
+set-up, and stepping), measured on the host clock, completion observed via
+an event recorded after the request's last chunk.
+"""
+import time
+
+import numpy as np
+
+from pyaxisymflow_amd import _lib
+
+
+def smooth_size(n, lo=None, hi=None):
+    """The 5-smooth integer nearest n within [lo, hi] (the z-transform
+    factors nz / 2 = nr into radices 2, 3, 5)."""
+    def smooth(m):
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        return m == 1
+
+    lo = 1 if lo is None else lo
+    hi = 1 << 30 if hi is None else hi
+    for d in range(0, n):
+        for m in (n - d, n + d):
+            if lo <= m <= hi and smooth(m):
+                return m
+    raise ValueError(f"no 5-smooth size near {n} in [{lo}, {hi}]")
+
+
+def request_set(n, seed=0, rate=1.0, steps=2000, nr_range=(128, 1024)):
+    """Synthetic requests: Poisson arrivals at `rate` per second; nr =
+    clip(round(lognormal(ln 256, 0.5)), 128, 1024) rounded to the nearest
+    5-smooth size, nz = 2 nr; G ~ U[0.1, 10]; rho_f = 1, nu = 1e-3,
+    r_ball = 0.15, the driver's Taylor-Green vorticity."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    out = []
+    for q in range(n):
+        nr = int(np.clip(round(rng.lognormal(np.log(256), 0.5)), *nr_range))
+        nr = smooth_size(nr, *nr_range)
+        out.append(dict(id=q, arrival=float(arrivals[q]), grid_size_z=2 * nr, nr=nr,
+                        G=float(rng.uniform(0.1, 10.0)), nu=1e-3, rho_f=1.0, r_ball=0.15,
+                        steps=int(steps)))
+    return out
+
+
+def footprint_bytes(req):
+    """Device bytes of one soft request: 9 caller-owned (nr, nz) FP64 fields,
+    18 scratch fields, the doubled grid (eta x2, LS fits x2: FP64; flags x2:
+    int16; pending: int8) and the spectral tables."""
+    n = req["nr"] * req["grid_size_z"]
+    return 8 * n * (9 + 18) + 2 * n * (4 * 8 + 2 * 2 + 1) + 8 * n
+
+
+class SoftServer:
+    """Continuous batching of soft-sphere requests on one GPU.
+
+    max_active: slots (concurrent requests); hbm_budget: bytes the active
+    requests may hold (default: 80% of free device memory); chunk: steps
+    queued per request between scheduling decisions."""
+
+    def __init__(self, max_active=16, hbm_budget=None, chunk=100):
+        torch = _lib.torch()
+        dev = _lib.device()
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info(dev)
+            hbm_budget = int(0.8 * free)
+        self.max_active = int(max_active)
+        self.hbm_budget = int(hbm_budget)
+        self.chunk = int(chunk)
+
+    def run(self, requests, keep_state=False, poll_s=2e-4):
+        """Serves `requests` (dicts from request_set) with arrivals relative
+        to the start of the call; returns one record per request (arrival,
+        admitted, finished, time_to_solution, steps, final time) and, with
+        keep_state, each request's final fields."""
+        torch = _lib.torch()
+        from pyaxisymflow_amd.stepper import SoftSphereStepper
+
+        pending = sorted(requests, key=lambda r: r["arrival"])
+        for r in pending:
+            if footprint_bytes(r) > self.hbm_budget:
+                raise ValueError(f"request {r['id']} needs {footprint_bytes(r)} bytes, more "
+                                 f"than the budget {self.hbm_budget}")
+        active, records, used = [], {}, 0
+        t0 = time.perf_counter()
+        while pending or active:
+            now = time.perf_counter() - t0
+            # admission: arrived, slot free, memory fits (first come first served)
+            while (pending and pending[0]["arrival"] <= now and len(active) < self.max_active
+                   and used + footprint_bytes(pending[0]) <= self.hbm_budget):
+                r = pending.pop(0)
+                st = SoftSphereStepper(grid_size_z=r["grid_size_z"], G=r["G"], nu=r["nu"],
+                                       rho_f=r["rho_f"], r_ball=r["r_ball"])
+                used += footprint_bytes(r)
+                active.append(dict(req=r, st=st, done=0, ev=None,
+                                   admitted=time.perf_counter() - t0))
+            progressed = False
+            for a in list(active):
+                if a["ev"] is not None and not a["ev"].query():
+                    continue
+                r = a["req"]
+                if a["done"] >= r["steps"]:
+                    st = a["st"]
+                    rec = dict(id=r["id"], nr=r["nr"], arrival=r["arrival"],
+                               admitted=a["admitted"], finished=time.perf_counter() - t0,
+                               steps=a["done"], sim_time=st.time(),
+                               lsq_unconverged=st.lsq_unconverged())
+                    rec["time_to_solution"] = rec["finished"] - rec["arrival"]
+                    if keep_state:
+                        e1, e2 = st.eta()
+                        rec["state"] = dict(vorticity=st.vorticity(), psi=st.psi(), eta1=e1,
+                                            eta2=e2, phi=st.phi(), scalars=st.scalars())
+                    records[r["id"]] = rec
+                    used -= footprint_bytes(r)
+                    active.remove(a)
+                    del st
+                    progressed = True
+                    continue
+                k = min(self.chunk, r["steps"] - a["done"])
+                a["st"].step(k)
+                ev = torch.cuda.Event()
+                ev.record(a["st"].stream)
+                a["ev"] = ev
+                a["done"] += k
+                progressed = True
+            if not progressed:
+                if pending and not active:
+                    wait = pending[0]["arrival"] - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
+                else:
+                    time.sleep(poll_s)
+        return [records[k] for k in sorted(records)]
+
+
+def latency_summary(records):
+    tts = np.array([r["time_to_solution"] for r in records])
+    return dict(requests=len(records), p50_s=float(np.percentile(tts, 50)),
+                p99_s=float(np.percentile(tts, 99)), mean_s=float(tts.mean()),
+                max_s=float(tts.max()))

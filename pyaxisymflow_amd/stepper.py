@@ -382,7 +382,10 @@ class SoftSphereStepper:
         self.t_wz, self.t_wr = t(wz), t(wr)
         cfg = _lib.SoftConfig()
         cfg.nr, cfg.nz, cfg.kill_width = nr, nz, kill_width
-        cfg.lsq_sweeps = int(lsq_sweeps) if lsq_sweeps else int(np.ceil(extrap_zone / dx)) + 8
+        # the reference's loop needs ~ band / dx sweeps at first and more as
+        # the solid deforms (18 after 200 steps at nz = 256); sweeps after
+        # the loop has ended exit on one counter read
+        cfg.lsq_sweeps = int(lsq_sweeps) if lsq_sweeps else 48
         cfg.reinit_iterations = -1
         cfg.dx, cfg.nu, cfg.G, cfg.rho_f, cfg.CFL = dx, nu, G, rho_f, 0.1
         cfg.eps = np.finfo(float).eps

@@ -1,0 +1,47 @@
+"""Serving form (continuous batching of soft-sphere requests).
+
+* A request's results are the same alone and in the mix: requests share only
+  the device, so the final fields of a request run beside others equal, bit
+  for bit, those of the same request run alone.
+* Every request completes its steps; time-to-solution >= its stepping; the
+  slot and memory limits hold (admission waits instead of overcommitting).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _reqs():
+    base = dict(nu=1e-3, rho_f=1.0, r_ball=0.15, steps=40)
+    return [dict(id=0, arrival=0.0, grid_size_z=128, nr=64, G=1.0, **base),
+            dict(id=1, arrival=0.0, grid_size_z=96, nr=48, G=5.0, **base),
+            dict(id=2, arrival=0.001, grid_size_z=160, nr=80, G=0.3, **base),
+            dict(id=3, arrival=0.002, grid_size_z=128, nr=64, G=8.0, **base)]
+
+
+def test_request_same_alone_and_in_the_mix():
+    from pyaxisymflow_amd.serve import SoftServer
+
+    reqs = _reqs()
+    alone = SoftServer(max_active=1, chunk=7).run([reqs[0]], keep_state=True)[0]
+    mix = SoftServer(max_active=4, chunk=5).run(reqs, keep_state=True)
+    got = [r for r in mix if r["id"] == 0][0]
+    assert got["steps"] == alone["steps"] == 40
+    for k in ("vorticity", "psi", "eta1", "eta2", "phi", "scalars"):
+        assert np.array_equal(got["state"][k], alone["state"][k]), k
+
+
+def test_serving_limits_and_records():
+    from pyaxisymflow_amd.serve import SoftServer, footprint_bytes, latency_summary
+
+    reqs = _reqs()
+    budget = footprint_bytes(reqs[2]) + footprint_bytes(reqs[0])  # at most two at once
+    recs = SoftServer(max_active=3, hbm_budget=budget, chunk=10).run(reqs)
+    assert [r["id"] for r in recs] == [0, 1, 2, 3]
+    for r in recs:
+        assert r["steps"] == 40 and r["sim_time"] > 0
+        assert r["finished"] >= r["admitted"] >= r["arrival"] - 1e-9
+        assert r["time_to_solution"] == pytest.approx(r["finished"] - r["arrival"])
+    s = latency_summary(recs)
+    assert s["requests"] == 4 and s["p99_s"] >= s["p50_s"] > 0
