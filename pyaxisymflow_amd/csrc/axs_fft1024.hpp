@@ -1,0 +1,245 @@
+// DCT-II / DCT-III of rows with nz = 2048 (half-length complex FFT of
+// M = 1024 points): one wavefront per row, the FFT in registers.
+//
+// Four-step split M = 16 x 64 (n = 64 n1 + n2, k = k1 + 16 k2):
+//   pass 1  lane n2 holds x[64 n1 + n2], n1 < 16: DFT-16 over n1 in
+//           registers, times W_1024^(n2 k1)             -> Y[k1][n2]
+//   pass 2  lane (k1, m) holds Y[k1][m + 4c], c < 16: DFT-16 over c, times
+//           W_64^(m k2a)                                 -> E[k2a][m][k1]
+//   pass 3  lane (g, k1) holds E[4g + t][m][k1]: four DFT-4 over m, giving
+//           X[k1 + 16 (4g + t) + 256 k2b]
+// with k2 = k2a + 16 k2b.  Between passes the wave exchanges values through
+// LDS; row strides are padded so each group of 16 lanes (one LDS cycle of
+// 16-byte accesses) touches 16 distinct bank quads.  All index arithmetic
+// is compile-time except the lane number.  The DCT pre/post steps around
+// the FFT are the ones of axs_fft.hpp (Makhoul's reorder and twiddles).
+#pragma once
+
+#include "axs_fft.hpp"
+
+namespace axs {
+
+constexpr int kF1024M = 1024;
+constexpr int kYStride = 68;                          // Y row (k1) stride, complex
+constexpr int kEStrideM = 20, kEStrideK = 80;         // E[k2a][m][k1] strides
+constexpr int kF1024Lds = 16 * kEStrideK;             // complex entries (>= M, >= 16*68)
+static_assert(kF1024Lds >= kF1024M && kF1024Lds >= 16 * kYStride, "LDS buffer");
+
+template <typename T, int SGN>
+__device__ __forceinline__ void dft4n(Cplx<T>& a0, Cplx<T>& a1, Cplx<T>& a2, Cplx<T>& a3) {
+  const Cplx<T> y0 = cadd(a0, a2), y1 = csub(a0, a2), y2 = cadd(a1, a3), d = csub(a1, a3);
+  const Cplx<T> y3 = {SGN < 0 ? d.y : -d.y, SGN < 0 ? -d.x : d.x};  // (SGN i) d
+  a0 = cadd(y0, y2);
+  a2 = csub(y0, y2);
+  a1 = cadd(y1, y3);
+  a3 = csub(y1, y3);
+}
+
+// a * W_16^E, W_16 = e^{SGN 2 pi i / 16}
+template <typename T, int SGN, int E>
+__device__ __forceinline__ Cplx<T> w16(Cplx<T> a) {
+  constexpr int e = E & 15;
+  if constexpr (e == 0) {
+    return a;
+  } else if constexpr (e == 4) {
+    return SGN < 0 ? Cplx<T>{a.y, -a.x} : Cplx<T>{-a.y, a.x};
+  } else if constexpr (e == 8) {
+    return Cplx<T>{-a.x, -a.y};
+  } else if constexpr (e == 12) {
+    return SGN < 0 ? Cplx<T>{-a.y, a.x} : Cplx<T>{a.y, -a.x};
+  } else {
+    constexpr double C[16] = {1.0,
+                              0.92387953251128675613,
+                              0.70710678118654752440,
+                              0.38268343236508977173,
+                              0.0,
+                              -0.38268343236508977173,
+                              -0.70710678118654752440,
+                              -0.92387953251128675613,
+                              -1.0,
+                              -0.92387953251128675613,
+                              -0.70710678118654752440,
+                              -0.38268343236508977173,
+                              0.0,
+                              0.38268343236508977173,
+                              0.70710678118654752440,
+                              0.92387953251128675613};
+    constexpr double S[16] = {0.0,
+                              0.38268343236508977173,
+                              0.70710678118654752440,
+                              0.92387953251128675613,
+                              1.0,
+                              0.92387953251128675613,
+                              0.70710678118654752440,
+                              0.38268343236508977173,
+                              0.0,
+                              -0.38268343236508977173,
+                              -0.70710678118654752440,
+                              -0.92387953251128675613,
+                              -1.0,
+                              -0.92387953251128675613,
+                              -0.70710678118654752440,
+                              -0.38268343236508977173};
+    return cmul(a, Cplx<T>{T(C[e]), T(SGN * S[e])});
+  }
+}
+
+// DFT-16 in place; on return x[4 ka + kb] holds X[ka + 4 kb].
+template <typename T, int SGN>
+__device__ __forceinline__ void dft16(Cplx<T> (&x)[16]) {
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb) dft4n<T, SGN>(x[nb], x[4 + nb], x[8 + nb], x[12 + nb]);
+  // x[4 ka + nb] = t[nb][ka]; twiddle by W_16^(nb ka)
+  x[5] = w16<T, SGN, 1>(x[5]);
+  x[6] = w16<T, SGN, 2>(x[6]);
+  x[7] = w16<T, SGN, 3>(x[7]);
+  x[9] = w16<T, SGN, 2>(x[9]);
+  x[10] = w16<T, SGN, 4>(x[10]);
+  x[11] = w16<T, SGN, 6>(x[11]);
+  x[13] = w16<T, SGN, 3>(x[13]);
+  x[14] = w16<T, SGN, 6>(x[14]);
+  x[15] = w16<T, SGN, 9>(x[15]);
+#pragma unroll
+  for (int ka = 0; ka < 4; ++ka) dft4n<T, SGN>(x[4 * ka], x[4 * ka + 1], x[4 * ka + 2], x[4 * ka + 3]);
+}
+
+template <int SGN>
+__device__ __forceinline__ Cplx<double> twid(const Cplx<double>* __restrict__ tw, int t) {
+  const Cplx<double> w = tw[t];
+  return SGN < 0 ? w : Cplx<double>{w.x, -w.y};
+}
+
+// FFT of the M = 1024 complex values in buf (natural order, lane-shared
+// LDS); result in natural order in buf.  blockDim.x must be 64.
+template <typename T, int SGN>
+__device__ __forceinline__ void fft1024_wave(Cplx<T>* buf, const Cplx<double>* __restrict__ tw) {
+  const int l = threadIdx.x;
+  Cplx<T> v[16];
+  // ---- pass 1: lane n2 = l
+  Cplx<double> t1[15];
+#pragma unroll
+  for (int k1 = 1; k1 < 16; ++k1) t1[k1 - 1] = twid<SGN>(tw, l * k1);
+#pragma unroll
+  for (int n1 = 0; n1 < 16; ++n1) v[n1] = buf[64 * n1 + l];
+  dft16<T, SGN>(v);
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < 16; ++k1) {
+    Cplx<T> b = v[4 * (k1 & 3) + (k1 >> 2)];
+    if (k1 > 0) b = cmul(b, Cplx<T>{T(t1[k1 - 1].x), T(t1[k1 - 1].y)});
+    buf[k1 * kYStride + l] = b;
+  }
+  __syncthreads();
+  // ---- pass 2: lane (k1, m)
+  {
+    const int k1 = l >> 2, m = l & 3;
+    Cplx<double> t2[15];
+#pragma unroll
+    for (int k2a = 1; k2a < 16; ++k2a) t2[k2a - 1] = twid<SGN>(tw, 16 * m * k2a);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) v[c] = buf[k1 * kYStride + m + 4 * c];
+    dft16<T, SGN>(v);
+    __syncthreads();
+#pragma unroll
+    for (int k2a = 0; k2a < 16; ++k2a) {
+      Cplx<T> e = v[4 * (k2a & 3) + (k2a >> 2)];
+      if (k2a > 0) e = cmul(e, Cplx<T>{T(t2[k2a - 1].x), T(t2[k2a - 1].y)});
+      buf[k2a * kEStrideK + m * kEStrideM + k1] = e;
+    }
+  }
+  __syncthreads();
+  // ---- pass 3: lane (g, k1)
+  {
+    const int g = l >> 4, k1 = l & 15;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) v[4 * t + m] = buf[(4 * g + t) * kEStrideK + m * kEStrideM + k1];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) dft4n<T, SGN>(v[4 * t], v[4 * t + 1], v[4 * t + 2], v[4 * t + 3]);
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) buf[k1 + 16 * (4 * g + t) + 256 * kb] = v[4 * t + kb];
+  }
+  __syncthreads();
+}
+
+}  // namespace axs
+
+namespace axs {
+
+// DCT-II post step (as dct2_post) for k = threadIdx.x + 64 q, q in [Q0, Q0 + KQ).
+template <typename T, int Q0, int KQ, typename Out>
+__device__ __forceinline__ void dct2_post_w64(const Cplx<T>* Z, const Cplx<double>* __restrict__ e1,
+                                              const Cplx<double>* __restrict__ e2, Out out) {
+  constexpr int M = kF1024M, N = 2 * kF1024M;
+  Cplx<double> a1[KQ], b1[KQ], a2[KQ], b2[KQ];
+#pragma unroll
+  for (int q = 0; q < KQ; ++q) {
+    const int k = threadIdx.x + 64 * (Q0 + q);
+    if (k <= M / 2) {
+      a1[q] = e1[k];
+      b1[q] = e1[M - k];
+      a2[q] = e2[k];
+      b2[q] = e2[M - k];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < KQ; ++q) {
+    const int k = threadIdx.x + 64 * (Q0 + q);
+    if (k <= M / 2) {
+      const int kp = M - k;
+      const Cplx<T> zk = Z[k % M], zp = Z[kp % M];
+      const Cplx<T> sk = cadd(zk, cconj(zp)), dk = csub(zk, cconj(zp));
+      const Cplx<T> wdk = cmul(Cplx<T>{T(a1[q].x), T(a1[q].y)}, dk);
+      const Cplx<T> Vk = {T(0.5) * (sk.x + wdk.y), T(0.5) * (sk.y - wdk.x)};
+      const Cplx<T> wk = cmul(Cplx<T>{T(a2[q].x), T(a2[q].y)}, Vk);
+      out(k, wk.x);
+      if (k > 0) out(N - k, -wk.y);
+      if (kp != k) {
+        const Cplx<T> sp = cadd(zp, cconj(zk)), dp = csub(zp, cconj(zk));
+        const Cplx<T> wdp = cmul(Cplx<T>{T(b1[q].x), T(b1[q].y)}, dp);
+        const Cplx<T> Vp = {T(0.5) * (sp.x + wdp.y), T(0.5) * (sp.y - wdp.x)};
+        const Cplx<T> wp = cmul(Cplx<T>{T(b2[q].x), T(b2[q].y)}, Vp);
+        out(kp, wp.x);
+        if (kp != M) out(N - kp, -wp.y);
+      }
+    }
+  }
+}
+
+// DCT-III pre step (as dct3_pre) for k = threadIdx.x + 64 q, q in [Q0, Q0 + KQ).
+template <typename T, int Q0, int KQ, typename In>
+__device__ __forceinline__ void dct3_pre_w64(Cplx<T>* Z, const Cplx<double>* __restrict__ e1,
+                                             const Cplx<double>* __restrict__ e2, In in) {
+  constexpr int M = kF1024M, N = 2 * kF1024M;
+  T xk[KQ], xnk[KQ], xm[KQ], xnm[KQ];
+  Cplx<double> q2k[KQ], q2m[KQ], q1k[KQ];
+#pragma unroll
+  for (int q = 0; q < KQ; ++q) {
+    const int k = threadIdx.x + 64 * (Q0 + q);
+    const int mk = M - k;
+    xk[q] = in(k);
+    xnk[q] = (k == 0) ? T(0) : in(N - k);
+    xm[q] = in(mk);
+    xnm[q] = in(N - mk);
+    q2k[q] = e2[k];
+    q2m[q] = e2[mk];
+    q1k[q] = e1[k];
+  }
+#pragma unroll
+  for (int q = 0; q < KQ; ++q) {
+    const int k = threadIdx.x + 64 * (Q0 + q);
+    const Cplx<T> Vk = cmul(Cplx<T>{T(q2k[q].x), T(-q2k[q].y)}, Cplx<T>{xk[q], -xnk[q]});
+    const Cplx<T> Vm = cmul(Cplx<T>{T(q2m[q].x), T(-q2m[q].y)}, Cplx<T>{xm[q], -xnm[q]});
+    const Cplx<T> Vkm = cconj(Vm);
+    const Cplx<T> A = cadd(Vk, Vkm);
+    const Cplx<T> D = csub(Vk, Vkm);
+    const Cplx<T> B = cmul(D, Cplx<T>{T(q1k[q].x), T(-q1k[q].y)});
+    Z[k] = Cplx<T>{A.x - B.y, A.y + B.x};
+  }
+}
+
+}  // namespace axs

@@ -12,6 +12,7 @@
 #pragma clang fp contract(fast)
 
 #include "axs_stokes.hpp"
+#include "axs_fft1024.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -254,6 +255,84 @@ __global__ void __launch_bounds__(256) dct3_rows_kernel(
   }
 }
 
+// nz = 2048 row passes: one wavefront per row, FFT in registers
+// (axs_fft1024.hpp).  Same inputs, outputs and taper as the kernels above.
+template <typename T, bool TAPER>
+__global__ void __launch_bounds__(64) dct2_rows_m1024_kernel(
+    const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
+    const double* __restrict__ rscale, const double* __restrict__ wz,
+    const double* __restrict__ wr, int nr, int w, const Cplx<double>* __restrict__ tw,
+    const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2) {
+  constexpr int N = 2 * kF1024M;
+  __shared__ Cplx<T> buf[kF1024Lds];
+  const int l = threadIdx.x;
+  const int i = blockIdx.x;
+  const int64_t cs = (int64_t)blockIdx.y * nr * N;  // case of a batch
+  src += cs;
+  dst += cs;
+  const int srow = (TAPER && i >= nr - w) ? nr - w : i;
+  const T* row = src + (int64_t)srow * N;
+  T v[N / 64];
+#pragma unroll
+  for (int q = 0; q < N / 64; ++q) v[q] = row[l + 64 * q];
+  T left = T(0), right = T(0), rowf = T(1);
+  if (TAPER) {
+    left = row[w - 1];
+    right = row[N - w];
+    rowf = (i == 0) ? T(0) : ((i >= nr - w) ? T(wr[i - (nr - w)]) : T(1));
+  }
+  const T rs = rscale ? T(rscale[i]) : T(1);
+  T* out_t = (TAPER && tapered) ? tapered + cs + (int64_t)i * N : nullptr;
+#pragma unroll
+  for (int q = 0; q < N / 64; ++q) {
+    const int j = l + 64 * q;
+    T t = v[q];
+    if (TAPER) {
+      if (j < w)
+        t = T(wz[j]) * left;
+      else if (j >= N - w)
+        t = T(wz[w + (j - (N - w))]) * right;
+      if (i >= nr - w) t = rowf * t;
+      if (i == 0) t = T(0);
+      if (out_t) out_t[j] = t;
+    }
+    packed_component(buf, dct_dst_index(j, N)) = rs * t;
+  }
+  __syncthreads();
+  fft1024_wave<T, -1>(buf, tw);
+  T* out = dst + (int64_t)i * N;
+  auto put = [&](int k, T val) { out[k] = val; };
+  dct2_post_w64<T, 0, 3>(buf, e1, e2, put);
+  dct2_post_w64<T, 3, 3>(buf, e1, e2, put);
+  dct2_post_w64<T, 6, 3>(buf, e1, e2, put);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) dct3_rows_m1024_kernel(
+    const T* __restrict__ src, T* __restrict__ dst, const Cplx<double>* __restrict__ tw,
+    const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2) {
+  constexpr int N = 2 * kF1024M;
+  __shared__ Cplx<T> buf[kF1024Lds];
+  const int l = threadIdx.x;
+  const int i = blockIdx.x;
+  const int64_t cs = (int64_t)blockIdx.y * gridDim.x * N;  // case of a batch
+  const T* row = src + cs + (int64_t)i * N;
+  auto get = [&](int k) { return row[k]; };
+  dct3_pre_w64<T, 0, 4>(buf, e1, e2, get);
+  dct3_pre_w64<T, 4, 4>(buf, e1, e2, get);
+  dct3_pre_w64<T, 8, 4>(buf, e1, e2, get);
+  dct3_pre_w64<T, 12, 4>(buf, e1, e2, get);
+  __syncthreads();
+  fft1024_wave<T, +1>(buf, tw);
+  const T invN = T(1) / T(N);
+  T* out = dst + cs + (int64_t)i * N;
+#pragma unroll
+  for (int q = 0; q < N / 64; ++q) {
+    const int j = l + 64 * q;
+    out[j] = packed_component(buf, dct_dst_index(j, N)) * invN;
+  }
+}
+
 // Column pass: for each z-mode k solve the tridiagonal system
 //   a_i x_{i-1} + (b_i + lam_k) x_i + c_i x_{i+1} = d_i,   i in [0, nr)
 // in place on d (row-major (nr, nz), k contiguous).  Workgroup = K modes x
@@ -420,6 +499,11 @@ hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const do
   // worst stage is radix 2 (M/2 butterflies); 2*per*bs >= M covers the row loads
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
+  if (M == kF1024M) {
+    dct2_rows_m1024_kernel<T, false><<<rows, 64, 0, s>>>(src, nullptr, dst, rscale, nullptr,
+                                                          nullptr, rows, 0, pl.tw, pl.e1, pl.e2);
+    return hipGetLastError();
+  }
   if (per <= 1)
     dct2_rows_kernel<T, 1><<<rows, bs, shm, s>>>(src, dst, rscale, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
   else if (per <= 2)
@@ -439,6 +523,11 @@ hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered
   const int bs = fft_block(M);
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
+  if (M == kF1024M) {
+    dct2_rows_m1024_kernel<T, true><<<dim3(pl.nr, batch), 64, 0, s>>>(
+        src, tapered, dst, pl.rvec, wz, wr, pl.nr, w, pl.tw, pl.e1, pl.e2);
+    return hipGetLastError();
+  }
 #define AXS_TAPER_LAUNCH(P)                                                                    \
   taper_dct2_rows_kernel<T, P><<<dim3(pl.nr, batch), bs, shm, s>>>(                          \
       src, tapered, dst, pl.rvec, wz, wr, pl.nr,                                               \
@@ -463,6 +552,10 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
   const dim3 grid(rows, batch);
+  if (M == kF1024M) {
+    dct3_rows_m1024_kernel<T><<<grid, 64, 0, s>>>(src, dst, pl.tw, pl.e1, pl.e2);
+    return hipGetLastError();
+  }
   if (per <= 1)
     dct3_rows_kernel<T, 1><<<grid, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
   else if (per <= 2)

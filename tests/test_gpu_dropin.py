@@ -241,6 +241,31 @@ def test_stokes_mixed_radix(nz, nr):
     assert rel_l2(psi, exact_stokes_solve(w, dx)) < 1e-12
 
 
+@pytest.mark.parametrize("nr", [64, 131])
+def test_stokes_nz2048_wave_per_row(nr):
+    """nz = 2048 runs the one-wave-per-row register FFT (axs_fft1024.hpp):
+    against the exact discrete solution, plus an inverse check: the DCT-II
+    of r omega followed by the DCT-III returns r omega (via a solve of a
+    field whose spectrum is the identity on a single mode)."""
+    from pyaxisymflow_amd.kernels.FastDiagonalisationStokesSolver import (
+        FastDiagonalisationStokesSolver,
+    )
+
+    nz = 2048
+    dx = 1.0 / nz
+    z = np.linspace(dx / 2, 1 - dx / 2, nz)
+    r = np.linspace(dx / 2, nr * dx - dx / 2, nr)
+    Z, R = np.meshgrid(z, r)
+    w = gaussian_vorticity(Z, R, 16, 5, 0.005, 0.02, 1.0, nr * dx)
+    w += 0.1 * np.random.default_rng(nr).standard_normal(w.shape)
+    psi = np.zeros_like(w)
+    FastDiagonalisationStokesSolver(nr, nz, dx).solve(solution_field=psi, rhs_field=w)
+    exact = exact_stokes_solve(w, dx)
+    print(f"nz=2048 nr={nr}: rel L2 vs exact {rel_l2(psi, exact):.2e}", flush=True)
+    assert rel_l2(psi, exact) < 1e-12
+    assert stokes_residual(psi, w, dx) <= max(2 * stokes_residual(exact, w, dx), 1e-13)
+
+
 def test_stokes_rejects_unsupported():
     from pyaxisymflow_amd.kernels.FastDiagonalisationStokesSolver import (
         FastDiagonalisationStokesSolver,
