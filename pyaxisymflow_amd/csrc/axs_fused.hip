@@ -218,88 +218,106 @@ static_assert(kR4R * kR4Z <= kBufA && kR2R * kR2Z <= kBufB && kR1R * kR1Z <= kBu
 
 int fused_blocks(int nr, int nz) { return (int)(ceil_div(nz, kFZ) * ceil_div(nr, kFR)); }
 
+// LDS of one step_fused block (passed to the tile body by pointer so both
+// tile variants share one allocation).
 template <typename T>
-__global__ void __launch_bounds__(256) step_fused_kernel(
+struct FusedLds {
+  T* A;  // psi (R6), then f (R4)
+  T* B;  // u_pen_r - u_r (R4 rows x R5 cols), then ENO3 out (R2)
+  T* C;  // u_pen_z (R4), then stage-1 out (R1)
+  T* D;  // u_pen_r (R4)
+  T *sri, *sinv, *sinv2;  // per R6 row: r_i, 1 / r_i, 1 / r_i^2
+  double* srd;            // per R6 row: r_i (FP64, for the force)
+  double* wpart;
+  double* smax;
+};
+
+// One output tile.  IN: the tile and its 6-cell halo lie at least kill_width
+// cells inside every domain edge (checked by the caller), so no cell needs a
+// bounds test, a one-sided velocity formula, the kill taper, the axis mirror
+// or a border exemption; the other tiles take every test.
+template <typename T, bool IN>
+__device__ __forceinline__ void step_fused_tile(
     const T* __restrict__ psi, const T* __restrict__ chi, const T* __restrict__ wsrc,
     T* __restrict__ wdst, T* __restrict__ avg, const double* __restrict__ r,
     const double* __restrict__ wz, const double* __restrict__ wr, int kw, int nr, int nz, T dx,
     double* __restrict__ sc, const double* __restrict__ slots, double* __restrict__ partials,
-    const RigidConfig* __restrict__ cfgs) {
-  __shared__ T bufA[kBufA], bufB[kBufB], bufC[kBufC], bufD[kBufD];
-  const int cb = blockIdx.z;  // case of a batch
-  const RigidConfig& cfg = cfgs[cb];
-  {
-    const int64_t fo = (int64_t)cb * nr * nz;
-    psi += fo;
-    chi += fo;
-    wsrc += fo;
-    wdst += fo;
-    if (avg) avg += fo;
-    sc += (int64_t)cb * AXS_RIGID_SCALARS;
-    slots += (int64_t)cb * kSlotsPerCase;
-    partials += (int64_t)cb * gridDim.x * gridDim.y;
-  }
-  __shared__ T sri[kR6R], sinv[kR6R], sinv2[kR6R];
-  __shared__ double wpart[4];
-  __shared__ double smax;
-  T(*sp)[kR6Z] = reinterpret_cast<T(*)[kR6Z]>(bufA);
-  T(*sf)[kR4Z] = reinterpret_cast<T(*)[kR4Z]>(bufA);
-  T(*sdr)[kR5Z] = reinterpret_cast<T(*)[kR5Z]>(bufB);
-  T(*sa)[kR2Z] = reinterpret_cast<T(*)[kR2Z]>(bufB);
-  T(*sx)[kR4Z] = reinterpret_cast<T(*)[kR4Z]>(bufC);
-  T(*st)[kR1Z] = reinterpret_cast<T(*)[kR1Z]>(bufC);
-  T(*sy)[kR4Z] = reinterpret_cast<T(*)[kR4Z]>(bufD);
+    const RigidConfig& cfg, const FusedLds<T>& L, int i0, int j0, int tile) {
+  T(*sp)[kR6Z] = reinterpret_cast<T(*)[kR6Z]>(L.A);
+  T(*sf)[kR4Z] = reinterpret_cast<T(*)[kR4Z]>(L.A);
+  T(*sdr)[kR5Z] = reinterpret_cast<T(*)[kR5Z]>(L.B);
+  T(*sa)[kR2Z] = reinterpret_cast<T(*)[kR2Z]>(L.B);
+  T(*sx)[kR4Z] = reinterpret_cast<T(*)[kR4Z]>(L.C);
+  T(*st)[kR1Z] = reinterpret_cast<T(*)[kR1Z]>(L.C);
+  T(*sy)[kR4Z] = reinterpret_cast<T(*)[kR4Z]>(L.D);
+  const T* sri = L.sri;
+  const T* sinv = L.sinv;
+  const T* sinv2 = L.sinv2;
   const int tid = threadIdx.x;
-  const int j0 = blockIdx.x * kFZ, i0 = blockIdx.y * kFR;
   // penalisation strip of this thread: R5 column pb, R5 rows 6ps .. 6ps+7
   const int pb = tid & 63, ps = tid >> 6;
   const int pj = j0 - 5 + pb;
-  const bool pjin = pj >= 0 && pj < nz;
+  const bool pjin = IN || (pj >= 0 && pj < nz);
 
   // ---- 1. loads: psi tile, chi and stored omega of the strip, slots, rows
   constexpr int NP = (kBufA + 255) / 256;
   T vp[NP];
+  const T* ptile = psi + ((i0 - 6) * nz + (j0 - 6));  // interior tiles only
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
     const int e = tid + q * 256;
     const int a = e / kR6Z, b = e % kR6Z;
-    const int i = i0 - 6 + a, j = j0 - 6 + b;
     vp[q] = T(0);
-    if (e < kBufA && i >= 0 && i < nr && j >= 0 && j < nz) vp[q] = psi[(int64_t)i * nz + j];
+    if constexpr (IN) {
+      if (e < kBufA) vp[q] = ptile[a * nz + b];
+    } else {
+      const int i = i0 - 6 + a, j = j0 - 6 + b;
+      if (e < kBufA && i >= 0 && i < nr && j >= 0 && j < nz) vp[q] = psi[(int64_t)i * nz + j];
+    }
   }
   T vc[kVS + 2], vw[kVS];
 #pragma unroll
   for (int k = 0; k < kVS + 2; ++k) {
     const int i = i0 - 5 + kVS * ps + k;
-    vc[k] = T(0);
-    if (pjin && i >= 0 && i < nr) vc[k] = chi[(int64_t)i * nz + pj];
+    if constexpr (IN) {
+      vc[k] = chi[i * nz + pj];
+    } else {
+      vc[k] = T(0);
+      if (pjin && i >= 0 && i < nr) vc[k] = chi[(int64_t)i * nz + pj];
+    }
   }
 #pragma unroll
   for (int k = 0; k < kVS; ++k) {
     const int i = i0 - 4 + kVS * ps + k;
-    vw[k] = T(0);
-    if (pjin && i >= 0 && i < nr) vw[k] = wsrc[taper_source(i, pj, nr, nz, kw)];
+    if constexpr (IN) {
+      vw[k] = wsrc[i * nz + pj];
+    } else {
+      vw[k] = T(0);
+      if (pjin && i >= 0 && i < nr) vw[k] = wsrc[taper_source(i, pj, nr, nz, kw)];
+    }
   }
   if (tid < 64) {
     const double m = wave_max(load_slot(slots + tid * kSlotStride));
-    if (tid == 0) smax = m;
+    if (tid == 0) *L.smax = m;
   }
   if (tid >= 64 && tid < 64 + kR6R) {
     const int i = i0 - 6 + (tid - 64);
-    const T ri = (i >= 0 && i < nr) ? T(r[i]) : T(1);
-    sri[tid - 64] = ri;
-    sinv[tid - 64] = T(1) / ri;
-    sinv2[tid - 64] = T(1) / (ri * ri);
+    const double rd = (IN || (i >= 0 && i < nr)) ? r[i] : 1.0;
+    const T ri = T(rd);
+    L.srd[tid - 64] = rd;
+    L.sri[tid - 64] = ri;
+    L.sinv[tid - 64] = T(1) / ri;
+    L.sinv2[tid - 64] = T(1) / (ri * ri);
   }
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
     const int e = tid + q * 256;
-    if (e < kBufA) bufA[e] = vp[q];
+    if (e < kBufA) L.A[e] = vp[q];
   }
   __syncthreads();
 
   // ---- 2. dt, then u and u_pen on the strip's R5 cells
-  const double mx = smax;
+  const double mx = *L.smax;
   const DtScalars ds = compute_dt(sc, mx, cfg);
   const T two_dx = T(2) * dx, inv2dx = T(1) / two_dx;
   const T ufree = T(cfg.U0 * sc[kScPrefac]);
@@ -307,31 +325,38 @@ __global__ void __launch_bounds__(256) step_fused_kernel(
   const double Ud = sc[kScUpen];
   const T U = T(Ud);
   const T tdt = T(ds.dt);
-  T dz[kVS + 2];  // u_pen_z - u_z on R5 rows 6ps-1+1 .. (column pb)
+  T dz[kVS + 2];  // u_pen_z - u_z on the strip's R5 rows
   double acc = 0.0;
 #pragma unroll
   for (int k = 0; k < kVS + 2; ++k) {
     const int q5 = kVS * ps + k;  // R5 row
     const int i = i0 - 5 + q5;
     T dzv = T(0), drv = T(0), pzv = T(0), prv = T(0);
-    if (pjin && i >= 0 && i < nr) {
+    if (IN || (pjin && i >= 0 && i < nr)) {
       const T* p = &sp[q5 + 1][pb + 1];
       T uz, ur;
-      velocity_rn<T>(p, kR6Z, i, pj, nr, nz, two_dx, inv2dx, sri[q5 + 1], sinv[q5 + 1], ufree,
-                     uz, ur);
+      if constexpr (IN) {
+        // compute_velocity_from_psi.py:10-17, central differences
+        uz = div_rn(div_rn(p[kR6Z] - p[-kR6Z], two_dx, inv2dx), sri[q5 + 1], sinv[q5 + 1]) + ufree;
+        ur = div_rn(div_rn(-(p[1] - p[-1]), two_dx, inv2dx), sri[q5 + 1], sinv[q5 + 1]);
+      } else {
+        velocity_rn<T>(p, kR6Z, i, pj, nr, nz, two_dx, inv2dx, sri[q5 + 1], sinv[q5 + 1], ufree,
+                       uz, ur);
+      }
       const T ldc = lam_dt * vc[k];
       pzv = brinkman(uz, ldc, U);
       prv = brinkman(ur, ldc, T(0));
       dzv = pzv - uz;
       drv = prv - ur;
       // output tile cells: R5 rows 5..5+kFR-1, columns 5..5+kFZ-1
-      if (k >= 1 && k <= kVS && q5 >= 5 && q5 < 5 + kFR && pb >= 5 && pb < 5 + kFZ) {
-        const int64_t c = (int64_t)i * nz + pj;
+      if (k >= 1 && k <= kVS && q5 >= 5 && q5 < 5 + kFR && pb >= 5 && pb < 5 + kFZ &&
+          (IN || (i < nr && pj < nz))) {
         if (avg) {
+          const int64_t c = (int64_t)i * nz + pj;
           const T base = (ds.reset != 0.0) ? T(0) : avg[c];
           avg[c] = base + p[0] * tdt;
         }
-        acc += (r[i] * double(vc[k])) * (double(pzv) - Ud);
+        acc += (L.srd[q5 + 1] * double(vc[k])) * (double(pzv) - Ud);
       }
     }
     dz[k] = dzv;
@@ -353,28 +378,36 @@ __global__ void __launch_bounds__(256) step_fused_kernel(
       const int a4 = kVS * ps + k - 1;
       const int i = i0 - 4 + a4;
       T fv = T(0);
-      if (pjin && i >= 0 && i < nr) {
-        fv = taper_apply(vw[k - 1], i, pj, nr, nz, kw, wz, wr);
-        if (i >= 1 && i <= nr - 2 && pj >= 1 && pj <= nz - 2) {
-          const T pv = div_rn(sdr[a4][pb + 1] - sdr[a4][pb - 1], two_dx, inv2dx) -
-                       div_rn(dz[k + 1] - dz[k - 1], two_dx, inv2dx);
-          fv = fv + pv;
+      if constexpr (IN) {
+        const T pv = div_rn(sdr[a4][pb + 1] - sdr[a4][pb - 1], two_dx, inv2dx) -
+                     div_rn(dz[k + 1] - dz[k - 1], two_dx, inv2dx);
+        fv = vw[k - 1] + pv;
+      } else {
+        if (pjin && i >= 0 && i < nr) {
+          fv = taper_apply(vw[k - 1], i, pj, nr, nz, kw, wz, wr);
+          if (i >= 1 && i <= nr - 2 && pj >= 1 && pj <= nz - 2) {
+            const T pv = div_rn(sdr[a4][pb + 1] - sdr[a4][pb - 1], two_dx, inv2dx) -
+                         div_rn(dz[k + 1] - dz[k - 1], two_dx, inv2dx);
+            fv = fv + pv;
+          }
         }
       }
       sf[a4][pb - 1] = fv;
     }
   }
   __syncthreads();
-  if (i0 == 0) {
-    // rows -4..-1 of the ENO3 input are the axis mirror of rows 3..0
-    // (f odd, u_z even, u_r odd: advect_vorticity_via_eno3.py:28-33)
-    for (int e = tid; e < 4 * kR4Z; e += 256) {
-      const int m = e / kR4Z, c = e % kR4Z;
-      sf[3 - m][c] = -sf[4 + m][c];
-      sx[3 - m][c] = sx[4 + m][c];
-      sy[3 - m][c] = -sy[4 + m][c];
+  if constexpr (!IN) {
+    if (i0 == 0) {
+      // rows -4..-1 of the ENO3 input are the axis mirror of rows 3..0
+      // (f odd, u_z even, u_r odd: advect_vorticity_via_eno3.py:28-33)
+      for (int e = tid; e < 4 * kR4Z; e += 256) {
+        const int m = e / kR4Z, c = e % kR4Z;
+        sf[3 - m][c] = -sf[4 + m][c];
+        sx[3 - m][c] = sx[4 + m][c];
+        sy[3 - m][c] = -sy[4 + m][c];
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 
   // ---- 4. ENO3 on R2 (advection_timestep.py:45-54); r faces slide through
@@ -395,14 +428,14 @@ __global__ void __launch_bounds__(256) step_fused_kernel(
       fr[m] = eno3_face_f(wf[m], wf[m + 1], wf[m + 2], wf[m + 3], wy[m], wy[m + 1], wy[m + 2],
                           wy[m + 3]);
     const int j = j0 - 2 + b;
-    const bool jok = j >= 2 && j < nz - 2;
+    const bool jok = IN || (j >= 2 && j < nz - 2);
     T out[kEH];
 #pragma unroll
     for (int k = 0; k < kEH; ++k) {
       const int ia = a0 + k + 2;
       const int i = i0 - 2 + a0 + k;
       T v = wf[k + 2];
-      if (jok && i >= 0 && i < nr - 2) {
+      if (IN || (jok && i >= 0 && i < nr - 2)) {
         const T fm2 = sf[ia][ib - 2], fm1 = sf[ia][ib - 1], fp1 = sf[ia][ib + 1],
                 fp2 = sf[ia][ib + 2];
         const T xm2 = sx[ia][ib - 2], xm1 = sx[ia][ib - 1], x0 = sx[ia][ib],
@@ -431,17 +464,17 @@ __global__ void __launch_bounds__(256) step_fused_kernel(
 #pragma unroll
     for (int k = 0; k < kSH + 2; ++k) ws[k] = sa[a0 + k][eb];
     const int j = j0 - 1 + b;
-    const bool jok = j >= 1 && j <= nz - 2;
+    const bool jok = IN || (j >= 1 && j <= nz - 2);
 #pragma unroll
     for (int k = 0; k < kSH; ++k) {
       const int a = a0 + k, ea = a + 1, a6 = a + 5;
       const int i = i0 - 1 + a;
       T v = ws[k + 1];
-      if (jok && i >= 1 && i <= nr - 2) {
+      if (IN || (jok && i >= 1 && i <= nr - 2)) {
         const T up = ws[k + 2], dn = ws[k];
-        const T L = div_rn(up + dn + sa[ea][eb + 1] + sa[ea][eb - 1] - T(4) * v, dx2, inv_dx2) +
-                    div_rn(div_rn(up - dn, two_dx, inv2dx), sri[a6], sinv[a6]) - v * sinv2[a6];
-        v = v + half_nu_dt * L;
+        const T Lp = div_rn(up + dn + sa[ea][eb + 1] + sa[ea][eb - 1] - T(4) * v, dx2, inv_dx2) +
+                     div_rn(div_rn(up - dn, two_dx, inv2dx), sri[a6], sinv[a6]) - v * sinv2[a6];
+        v = v + half_nu_dt * Lp;
       }
       st[a][b] = v;  // bufC (u_pen_z) was last read before the previous barrier
     }
@@ -456,31 +489,78 @@ __global__ void __launch_bounds__(256) step_fused_kernel(
 #pragma unroll
     for (int k = 0; k < kOH + 2; ++k) wt[k] = st[a0 + k][tb];
     const int j = j0 + b;
-    const bool jin = j < nz, jok = j >= 1 && j <= nz - 2;
+    const bool jin = IN || j < nz, jok = IN || (j >= 1 && j <= nz - 2);
 #pragma unroll
     for (int k = 0; k < kOH; ++k) {
       const int a = a0 + k, ta = a + 1, a6 = a + 6;
       const int i = i0 + a;
-      if (!jin || i >= nr) continue;
+      if (!IN && (!jin || i >= nr)) continue;
       T v = sa[a + 2][b + 2];
-      if (jok && i >= 1 && i <= nr - 2) {
+      if (IN || (jok && i >= 1 && i <= nr - 2)) {
         const T tc = wt[k + 1], up = wt[k + 2], dn = wt[k];
-        const T L = div_rn(up + dn + st[ta][tb + 1] + st[ta][tb - 1] - T(4) * tc, dx2, inv_dx2) +
-                    div_rn(div_rn(up - dn, two_dx, inv2dx), sri[a6], sinv[a6]) - tc * sinv2[a6];
-        v = v + nu_dt * L;
+        const T Lp = div_rn(up + dn + st[ta][tb + 1] + st[ta][tb - 1] - T(4) * tc, dx2, inv_dx2) +
+                     div_rn(div_rn(up - dn, two_dx, inv2dx), sri[a6], sinv[a6]) - tc * sinv2[a6];
+        v = v + nu_dt * Lp;
       }
-      wdst[(int64_t)i * nz + j] = v;
+      if constexpr (IN)
+        wdst[i * nz + j] = v;
+      else
+        wdst[(int64_t)i * nz + j] = v;
     }
   }
 
   // ---- force partial of this tile; block (0, 0) publishes the dt scalars
   acc = wave_sum(acc);
-  if ((tid & 63) == 0) wpart[tid >> 6] = acc;
+  if ((tid & 63) == 0) L.wpart[tid >> 6] = acc;
   __syncthreads();
   if (tid == 0) {
-    partials[blockIdx.y * gridDim.x + blockIdx.x] = ((wpart[0] + wpart[1]) + wpart[2]) + wpart[3];
-    if (blockIdx.x == 0 && blockIdx.y == 0) store_dt_scalars(sc, ds, mx, cfg);
+    partials[tile] = ((L.wpart[0] + L.wpart[1]) + L.wpart[2]) + L.wpart[3];
+    if (tile == 0) store_dt_scalars(sc, ds, mx, cfg);
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) step_fused_kernel(
+    const T* __restrict__ psi, const T* __restrict__ chi, const T* __restrict__ wsrc,
+    T* __restrict__ wdst, T* __restrict__ avg, const double* __restrict__ r,
+    const double* __restrict__ wz, const double* __restrict__ wr, int kw, int nr, int nz, T dx,
+    double* __restrict__ sc, const double* __restrict__ slots, double* __restrict__ partials,
+    const RigidConfig* __restrict__ cfgs) {
+  __shared__ T bufA[kBufA], bufB[kBufB], bufC[kBufC], bufD[kBufD];
+  __shared__ T sri[kR6R], sinv[kR6R], sinv2[kR6R];
+  __shared__ double srd[kR6R];
+  __shared__ double wpart[4];
+  __shared__ double smax;
+  const int cb = blockIdx.z;  // case of a batch
+  const RigidConfig& cfg = cfgs[cb];
+  const int64_t fo = (int64_t)cb * nr * nz;
+  psi += fo;
+  chi += fo;
+  wsrc += fo;
+  wdst += fo;
+  if (avg) avg += fo;
+  sc += (int64_t)cb * AXS_RIGID_SCALARS;
+  slots += (int64_t)cb * kSlotsPerCase;
+  partials += (int64_t)cb * gridDim.x * gridDim.y;
+  const FusedLds<T> L{bufA, bufB, bufC, bufD, sri, sinv, sinv2, srd, wpart, &smax};
+  // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin by
+  // linear id; give each XCD a contiguous band of tiles so a tile's halo
+  // rows are still in that XCD's L2 when its neighbours read them.
+  const int gx = gridDim.x, ntile = gx * gridDim.y;
+  int tile = blockIdx.y * gx + blockIdx.x;
+  if ((ntile & 7) == 0) tile = (tile & 7) * (ntile >> 3) + (tile >> 3);
+  const int j0 = (tile % gx) * kFZ, i0 = (tile / gx) * kFR;
+  // interior: every R6 cell exists and every R4 cell is clear of the taper
+  // (kill_width columns at each z end, kill_width rows below r_max, row 0)
+  const int m = kw > 1 ? kw : 1;
+  const bool interior = i0 - 6 >= m && i0 + kFR + 6 <= nr - m && j0 - 6 >= m &&
+                        j0 + kFZ + 6 <= nz - m;
+  if (interior)
+    step_fused_tile<T, true>(psi, chi, wsrc, wdst, avg, r, wz, wr, kw, nr, nz, dx, sc, slots,
+                             partials, cfg, L, i0, j0, tile);
+  else
+    step_fused_tile<T, false>(psi, chi, wsrc, wdst, avg, r, wz, wr, kw, nr, nz, dx, sc, slots,
+                              partials, cfg, L, i0, j0, tile);
 }
 
 // ------------------------------------------------------------ velocity_out

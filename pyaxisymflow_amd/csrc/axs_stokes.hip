@@ -15,7 +15,10 @@
 #include "axs_fft1024.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 namespace axs {
@@ -50,6 +53,101 @@ StokesPlan::~StokesPlan() {
 }
 
 int StokesPlan::tri_threads() const { return tri_K * tri_P; }
+
+// c'_{pL-1} of the forward elimination at every segment start p L, per mode
+// (the pivot recurrence depends only on the matrix, not on the data).
+static std::vector<double> pivot_table(const std::vector<double>& a, const std::vector<double>& b,
+                                       const std::vector<double>& c,
+                                       const std::vector<double>& lam, int nr, int nz, int L) {
+  const int P = (nr + L - 1) / L;
+  std::vector<double> t((size_t)P * nz, 0.0);
+  for (int k = 0; k < nz; ++k) {
+    double cp = 0.0;
+    for (int i = 0; i < nr; ++i) {
+      if (i % L == 0) t[(size_t)(i / L) * nz + k] = cp;
+      const double m = (b[i] + lam[k]) - a[i] * cp;
+      cp = c[i] / m;
+    }
+  }
+  return t;
+}
+
+// Autotuner of the column pass: times every admissible (modes per
+// workgroup K, rows per segment L) on this device with the plan's own
+// matrices and keeps the fastest; results are cached per (nr, nz).
+hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::vector<double>& hb,
+                                    const std::vector<double>& hc,
+                                    const std::vector<double>& hlam) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, std::pair<int, int>> cache;
+  std::pair<int, int> best{tri_K, tri_L};
+  bool cached = false;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find({nr, nz});
+    if (it != cache.end()) {
+      best = it->second;
+      cached = true;
+    }
+  }
+  if (!cached) {
+    hipError_t e = hipMemset(spec, 0, sizeof(double) * (size_t)nr * nz);
+    if (e != hipSuccess) return e;
+    hipEvent_t t0, t1;
+    if ((e = hipEventCreate(&t0)) != hipSuccess) return e;
+    if ((e = hipEventCreate(&t1)) != hipSuccess) return e;
+    float best_ms = 1e30f;
+    double* saved = cseg;
+    const int saved_k = tri_K, saved_l = tri_L, saved_p = tri_P;
+    for (int L : {4, 8, 16}) {
+      const int P = (nr + L - 1) / L;
+      if (P > 512) continue;
+      double* tab = nullptr;
+      if ((e = upload(&tab, pivot_table(ha, hb, hc, hlam, nr, nz, L))) != hipSuccess) break;
+      for (int K = 1; K <= 64; K *= 2) {
+        if (K * P > 1024 || K * P < 64) continue;
+        if (sizeof(double) * (4 * K * P + 3 * nr) > 65536) continue;
+        cseg = tab;
+        tri_K = K;
+        tri_L = L;
+        tri_P = P;
+        float ms = 0.f;
+        for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) e = launch_tridiag<double>(*this, spec, 0);
+        if (e == hipSuccess) e = hipEventRecord(t0, 0);
+        for (int rep = 0; rep < 5 && e == hipSuccess; ++rep) e = launch_tridiag<double>(*this, spec, 0);
+        if (e == hipSuccess) e = hipEventRecord(t1, 0);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        if (e != hipSuccess) break;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = {K, L};
+        }
+      }
+      (void)hipFree(tab);
+      if (e != hipSuccess) break;
+    }
+    cseg = saved;
+    tri_K = saved_k;
+    tri_L = saved_l;
+    tri_P = saved_p;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> g(mu);
+    cache[{nr, nz}] = best;
+  }
+  if (best.first == tri_K && best.second == tri_L) return hipSuccess;
+  double* tab = nullptr;
+  hipError_t e = upload(&tab, pivot_table(ha, hb, hc, hlam, nr, nz, best.second));
+  if (e != hipSuccess) return e;
+  (void)hipFree(cseg);
+  cseg = tab;
+  tri_K = best.first;
+  tri_L = best.second;
+  tri_P = (nr + tri_L - 1) / tri_L;
+  return hipSuccess;
+}
 
 hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
   nr = nr_;
@@ -109,15 +207,7 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
   // keep the column kernel's LDS (scan buffers + coefficients) within 64 KiB
   while (tri_K > 1 && sizeof(double) * (4 * tri_K * tri_P + 3 * nr) > 65536) tri_K /= 2;
   if (sizeof(double) * (4 * tri_K * tri_P + 3 * nr) > 65536) return hipErrorInvalidValue;
-  std::vector<double> h_cseg((size_t)tri_P * nz, 0.0);
-  for (int k = 0; k < nz; ++k) {
-    double cp = 0.0;
-    for (int i = 0; i < nr; ++i) {
-      if (i % tri_L == 0) h_cseg[(size_t)(i / tri_L) * nz + k] = cp;
-      const double m = (h_b[i] + h_lam[k]) - h_a[i] * cp;
-      cp = h_c[i] / m;
-    }
-  }
+  std::vector<double> h_cseg = pivot_table(h_a, h_b, h_c, h_lam, nr, nz, tri_L);
   hipError_t e;
   if ((e = upload(&tw, h_tw)) != hipSuccess) return e;
   if ((e = upload(&e1, h_e1)) != hipSuccess) return e;
@@ -129,7 +219,10 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
   if ((e = upload(&rvec, h_r)) != hipSuccess) return e;
   if ((e = upload(&cseg, h_cseg)) != hipSuccess) return e;
   e = hipMalloc(reinterpret_cast<void**>(&spec), sizeof(double) * (size_t)nr * nz);
-  return e;
+  if (e != hipSuccess) return e;
+  if ((size_t)nr * nz >= (size_t(1) << 18) && !std::getenv("AXS_NO_TUNE"))
+    return tune_tridiag(h_a, h_b, h_c, h_lam);
+  return hipSuccess;
 }
 
 // ------------------------------------------------------------------ kernels
@@ -573,6 +666,9 @@ hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch) 
   const dim3 blocks(ceil_div(pl.nz, K), batch);
   const size_t shm = sizeof(double) * (4 * K * P + 3 * pl.nr);
   switch (pl.tri_L) {
+    case 4:
+      tridiag_columns_kernel<T, 4><<<blocks, K * P, shm, s>>>(d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P);
+      break;
     case 8:
       tridiag_columns_kernel<T, 8><<<blocks, K * P, shm, s>>>(d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P);
       break;

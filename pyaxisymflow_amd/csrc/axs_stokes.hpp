@@ -4,6 +4,8 @@
 
 #include "axs_fft.hpp"
 
+#include <vector>
+
 namespace axs {
 
 struct StokesPlan {
@@ -24,6 +26,8 @@ struct StokesPlan {
   int tri_L = 16, tri_P = 1, tri_K = 1;
 
   hipError_t init(int nr, int nz, double dx);
+  hipError_t tune_tridiag(const std::vector<double>& a, const std::vector<double>& b,
+                          const std::vector<double>& c, const std::vector<double>& lam);
   int tri_threads() const;
   ~StokesPlan();
 };
