@@ -36,7 +36,6 @@ STAGE_BYTES_PER_CELL = {
     "velocity_max": 8,       # read psi
     "body_mask": 8,          # write chi (moving body only)
     "step_fused": 32,        # read psi, chi, omega; write omega
-    "finalize_step": 0,      # one block over the force partials
 }
 
 

@@ -20,7 +20,9 @@
 //                  reads psi, chi and the stored omega, writes only the new
 //                  omega (into the other buffer of a ping-pong pair).
 //   velocity_out   u_pen of the last step, on demand (not part of a step).
-//   finalize_step  one block: fixed-order force sum, t += dt, it += 1.
+//   finalize       completes a step (fixed-order force sum, t += dt, it += 1)
+//                  as an extra workgroup of the next step's first pass, or
+//                  standalone at the end of a run.
 // No per-block device-scope fences: the L2s of the XCDs are not coherent, so
 // such a fence costs an L2 write-back; kernel boundaries order the passes.
 //
@@ -607,20 +609,9 @@ __global__ void __launch_bounds__(1024) finalize_step_kernel(const double* __res
   __shared__ double red[1024];
   const int cb = blockIdx.x;  // one block per case
   const RigidConfig& cfg = cfgs[cb];
-  partials += (int64_t)cb * count;
-  slots += (int64_t)cb * kSlotsPerCase;
-  sc += (int64_t)cb * AXS_RIGID_SCALARS;
-  if (drag) drag += (int64_t)cb * cfg.history;
-  double s = 0.0;
-  for (int q = threadIdx.x; q < count; q += 1024) s += partials[q];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 512; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x < kMaxSlots) slots[threadIdx.x * kSlotStride] = 0.0;
-  if (threadIdx.x == 0) finalize_step_scalars(sc, drag, red[0], cfg);
+  finalize_pending(partials + (int64_t)cb * count, count, slots + (int64_t)cb * kSlotsPerCase,
+                   sc + (int64_t)cb * AXS_RIGID_SCALARS,
+                   drag ? drag + (int64_t)cb * cfg.history : nullptr, cfg, red);
 }
 
 // ------------------------------------------------------------ launchers

@@ -5,10 +5,24 @@
 
 namespace axs {
 
+// Velocity maximum slots: kMaxSlots doubles, kSlotStride apart (4 KiB, so
+// they spread over memory channels), holding bit patterns of non-negative doubles; all zero between
+// steps (finalize_step clears them).
+constexpr int kMaxSlots = 64, kSlotStride = 512;
+constexpr int kSlotsPerCase = kMaxSlots * kSlotStride;  // doubles per case of a batch
+
+// Ramp factor, penalisation velocity and body centre at time t.  With no
+// oscillation (omega = 0) cos and sin of 0 are 1 and 0 exactly, so they are
+// not evaluated.
 __device__ __forceinline__ void next_step_scalars(double* sc, const RigidConfig& cfg, double t) {
   sc[kScPrefac] = (cfg.T_ramp > 0.0 && t < cfg.T_ramp) ? sin(0.5 * M_PI * t / cfg.T_ramp) : 1.0;
-  sc[kScUpen] = cfg.osc_amp * cos(cfg.osc_omega * t);
-  sc[kScBodyZ] = cfg.body_z0 + cfg.body_disp * sin(cfg.osc_omega * t);
+  if (cfg.osc_omega == 0.0) {
+    sc[kScUpen] = cfg.osc_amp;
+    sc[kScBodyZ] = cfg.body_z0;
+  } else {
+    sc[kScUpen] = cfg.osc_amp * cos(cfg.osc_omega * t);
+    sc[kScBodyZ] = cfg.body_z0 + cfg.body_disp * sin(cfg.osc_omega * t);
+  }
 }
 
 // Scalars of one step derived from max(|u_z| + |u_r|).
@@ -44,6 +58,7 @@ __device__ __forceinline__ void store_dt_scalars(double* sc, const DtScalars& d,
   sc[kScFreqStep] = d.fr;
   sc[kScPrefacStep] = sc[kScPrefac];
   sc[kScUpenStep] = sc[kScUpen];
+  sc[kScPending] = 1.0;
   sc[kScAvgReset] = d.reset;
   sc[kScMaxU] = m;
   sc[kScDt] = d.dt;
@@ -67,16 +82,52 @@ __device__ __forceinline__ void finalize_step_scalars(double* sc, double* drag, 
   next_step_scalars(sc, cfg, t);
 }
 
-// Velocity maximum slots: kMaxSlots doubles, kSlotStride apart (4 KiB, so
-// they spread over memory channels), holding bit patterns of non-negative doubles; all zero between
-// steps (finalize_step clears them).
-constexpr int kMaxSlots = 64, kSlotStride = 512;
-constexpr int kSlotsPerCase = kMaxSlots * kSlotStride;  // doubles per case of a batch
 
 // Every launcher runs `batch` cases of one (nr, nz) grid: fields back to
 // back (nr * nz each), AXS_RIGID_SCALARS scalars, kSlotsPerCase slots, one
 // block's worth of partials and `history` drag entries per case, and the
 // per-case configurations cfgs[batch] in device memory.
+// Completes a case's last step if its pending flag is set: fixed-order sum
+// of the force partials (thread t adds t, t + blockDim, ...; then a tree in
+// red[blockDim]), force record, t += dt, it += 1, period timer, next step's
+// scalars, cleared maximum slots.  Called by one whole block.
+__device__ __forceinline__ void finalize_pending(const double* __restrict__ partials, int count,
+                                                 double* __restrict__ slots,
+                                                 double* __restrict__ sc,
+                                                 double* __restrict__ drag,
+                                                 const RigidConfig& cfg, double* red) {
+  if (sc[kScPending] == 0.0) return;
+  const int nt = blockDim.x;
+  double s = 0.0;
+  // sixteen independent loads in flight per batch; added in index order
+  for (int base = threadIdx.x; base < count; base += 16 * nt) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int q = base + u * nt;
+      v[u] = (q < count) ? partials[q] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s += v[u];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < nt; off <<= 1) {
+    // pairwise in a fixed pattern (works for any block size)
+    double v = 0.0;
+    const bool take = (threadIdx.x % (2 * off) == 0) && (threadIdx.x + off < nt);
+    if (take) v = red[threadIdx.x + off];
+    __syncthreads();
+    if (take) red[threadIdx.x] = red[threadIdx.x] + v;
+    __syncthreads();
+  }
+  for (int q = threadIdx.x; q < kMaxSlots; q += nt) slots[q * kSlotStride] = 0.0;
+  if (threadIdx.x == 0) {
+    finalize_step_scalars(sc, drag, red[0], cfg);
+    sc[kScPending] = 0.0;
+  }
+}
+
 template <typename T>
 hipError_t launch_velocity_max(const T* psi, const double* r, int nr, int nz, T dx,
                                const double* sc, double* slots, const RigidConfig* cfgs,

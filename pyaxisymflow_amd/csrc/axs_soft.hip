@@ -260,6 +260,8 @@ struct SoftStepper {
   ~SoftStepper() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
+    if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
     for (void* p : allocs) (void)hipFree(p);
     delete plan;
   }
@@ -323,6 +325,12 @@ struct SoftStepper {
     return hipSuccess;
   }
 
+  // kChunkSteps steps per graph (a graph launch costs microseconds) and a
+  // one-step graph for the remainder.
+  static constexpr int kChunkSteps = 8;
+  hipGraph_t chunk_graph = nullptr;
+  hipGraphExec_t chunk_exec = nullptr;
+
   hipError_t run(int nsteps, bool use_graph, hipStream_t s) {
     if (!use_graph) {
       for (int q = 0; q < nsteps; ++q) {
@@ -334,19 +342,33 @@ struct SoftStepper {
     if (!graph_exec || graph_stream != s) {
       if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
       if (graph) (void)hipGraphDestroy(graph);
+      if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
+      if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
       graph_exec = nullptr;
       graph = nullptr;
-      hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-      if (e != hipSuccess) return e;
-      hipError_t le = enqueue_step(s);
-      e = hipStreamEndCapture(s, &graph);
-      if (le != hipSuccess) return le;
-      if (e != hipSuccess) return e;
-      e = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
-      if (e != hipSuccess) return e;
+      chunk_exec = nullptr;
+      chunk_graph = nullptr;
+      for (int q = 0; q < 2; ++q) {
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        hipError_t le = hipSuccess;
+        for (int k = 0; k < (q == 0 ? 1 : kChunkSteps) && le == hipSuccess; ++k)
+          le = enqueue_step(s);
+        hipGraph_t* g = q == 0 ? &graph : &chunk_graph;
+        hipGraphExec_t* x = q == 0 ? &graph_exec : &chunk_exec;
+        e = hipStreamEndCapture(s, g);
+        if (le != hipSuccess) return le;
+        if (e != hipSuccess) return e;
+        if ((e = hipGraphInstantiate(x, *g, nullptr, nullptr, 0)) != hipSuccess) return e;
+      }
       graph_stream = s;
     }
-    for (int q = 0; q < nsteps; ++q) {
+    int q = 0;
+    for (; q + kChunkSteps <= nsteps; q += kChunkSteps) {
+      hipError_t e = hipGraphLaunch(chunk_exec, s);
+      if (e != hipSuccess) return e;
+    }
+    for (; q < nsteps; ++q) {
       hipError_t e = hipGraphLaunch(graph_exec, s);
       if (e != hipSuccess) return e;
     }

@@ -6,10 +6,11 @@
 // velocity U_0 cos(omega t), dt clipped to the period and to tEnd, psi
 // averaged over each period).
 //
-// One step, in the reference's order, as seven passes (a stage is skipped
+// One step, in the reference's order, as six passes (a stage is skipped
 // when its feature is off):
 //   taper_dct2_rows   kill taper of the stored field on the fly, DCT-II of
-//                     r * omega along z
+//                     r * omega along z; one extra workgroup per case
+//                     completes the previous step (force record, t, it)
 //   tridiag_columns   one radial tridiagonal solve per z-mode
 //   dct3_rows         inverse DCT -> psi
 //   velocity_max      u from psi (+ free stream), max|u| (atomicMax slots)
@@ -17,10 +18,10 @@
 //   step_fused        dt rule, u from psi, Brinkman, taper + curl(u_pen - u),
 //                     force partials, psi average, ENO3, both RK2 stages ->
 //                     the other buffer of the vorticity pair
-//   finalize_step     force record, t, period timer, step count
 // The stored vorticity alternates between the home buffer and work0; one
-// graph is captured per direction, and a run that ends on work0 copies it
-// home, so the home buffer holds the field whenever a run returns.
+// graph is captured per direction.  A run ends with a standalone completion
+// of its last step and, if it ended on work0, a copy home, so the home
+// buffer, t, it and the force history are current whenever a run returns.
 // Every scalar (t, dt, ramp factor, penalisation velocity, period timer,
 // force history) stays in device memory, so a step is captured once into a
 // hipGraph and replayed.  No memset nodes are recorded (fills are kernels).
@@ -71,14 +72,17 @@ __global__ void init_scalars_kernel(double* __restrict__ sc, double* __restrict_
   sc += (int64_t)cb * AXS_RIGID_SCALARS;
   slots += (int64_t)cb * kSlotsPerCase;
   if (threadIdx.x < kMaxSlots) slots[threadIdx.x * kSlotStride] = 0.0;
-  if (threadIdx.x == 0) next_step_scalars(sc, cfgs[cb], sc[kScT]);
+  if (threadIdx.x == 0) {
+    sc[kScPending] = 0.0;
+    next_step_scalars(sc, cfgs[cb], sc[kScT]);
+  }
 }
 
 // ------------------------------------------------------------------ stepper
 
 const char* const kRigidStageNames[kRigidStages] = {
-    "taper_dct2_rows", "tridiag_columns", "dct3_rows",    "velocity_max",
-    "body_mask",       "step_fused",      "finalize_step"};
+    "taper_dct2_rows", "tridiag_columns", "dct3_rows",
+    "velocity_max",    "body_mask",       "step_fused"};
 
 template <typename T>
 __global__ void copy_field_kernel(T* __restrict__ dst, const T* __restrict__ src, int64_t n) {
@@ -91,6 +95,8 @@ RigidStepperBase::~RigidStepperBase() {
     if (graph_exec[q]) (void)hipGraphExecDestroy(graph_exec[q]);
     if (graph[q]) (void)hipGraphDestroy(graph[q]);
   }
+  if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
+  if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
   if (partials) (void)hipFree(partials);
   if (slots) (void)hipFree(slots);
   if (spec) (void)hipFree(spec);
@@ -155,8 +161,18 @@ hipError_t RigidStepper<T>::enqueue_stage(int stage, hipStream_t s) {
   const int B = batch;
   if (!stage_enabled(stage)) return hipSuccess;
   switch (stage) {
-    case 0:
-      return launch_taper_dct2_rows<T>(*plan, cur, nullptr, sp, buf.kill_wz, buf.kill_wr, w, s, B);
+    case 0: {
+      // the previous step completes in an extra workgroup of this pass
+      StepFinalize fin;
+      fin.partials = partials;
+      fin.count = npartials;
+      fin.slots = slots;
+      fin.sc = sc;
+      fin.drag = buf.drag;
+      fin.cfgs = dcfg;
+      return launch_taper_dct2_rows<T>(*plan, cur, nullptr, sp, buf.kill_wz, buf.kill_wr, w, s, B,
+                                       &fin);
+    }
     case 1:
       return launch_tridiag<T>(*plan, sp, s, B);
     case 2:
@@ -172,8 +188,6 @@ hipError_t RigidStepper<T>::enqueue_stage(int stage, hipStream_t s) {
                                   cfg.average_psi ? static_cast<T*>(buf.avg_psi) : nullptr,
                                   buf.r, buf.kill_wz, buf.kill_wr, w, nr, nz, T(cfg.dx), sc, slots,
                                   partials, dcfg, B, s);
-    case 6:
-      return launch_finalize_step(partials, npartials, slots, sc, buf.drag, dcfg, B, s);
     default:
       return hipErrorInvalidValue;
   }
@@ -181,6 +195,10 @@ hipError_t RigidStepper<T>::enqueue_stage(int stage, hipStream_t s) {
 
 template <typename T>
 hipError_t RigidStepper<T>::settle(hipStream_t s) {
+  // complete the last step (a no-op when nothing is pending)
+  hipError_t e = launch_finalize_step(partials, npartials, slots, buf.scalars, buf.drag, dcfg,
+                                      batch, s);
+  if (e != hipSuccess) return e;
   if (!parity) return hipSuccess;
   const int64_t n = (int64_t)cfg.nr * cfg.nz * batch;
   copy_field_kernel<T><<<ceil_div(n, 256), 256, 0, s>>>(static_cast<T*>(buf.vorticity),
@@ -221,18 +239,33 @@ hipError_t RigidStepperBase::run(int nsteps, bool use_graph, hipStream_t s) {
       graph_exec[q] = nullptr;
       graph[q] = nullptr;
     }
+    if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
+    if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
+    chunk_exec = nullptr;
+    chunk_graph = nullptr;
     const int saved = parity;
-    for (int q = 0; q < 2; ++q) {
-      parity = q;
+    // graphs: one step from each parity, and kChunkSteps steps from parity 0
+    for (int q = 0; q < 3; ++q) {
       hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
       if (e != hipSuccess) {
         parity = saved;
         return e;
       }
-      hipError_t le = enqueue_step(s);
-      e = hipStreamEndCapture(s, &graph[q]);
-      if (le == hipSuccess && e == hipSuccess)
-        e = hipGraphInstantiate(&graph_exec[q], graph[q], nullptr, nullptr, 0);
+      hipError_t le = hipSuccess;
+      if (q < 2) {
+        parity = q;
+        le = enqueue_step(s);
+      } else {
+        parity = 0;
+        for (int k = 0; k < kChunkSteps && le == hipSuccess; ++k) {
+          le = enqueue_step(s);
+          parity ^= 1;
+        }
+      }
+      hipGraph_t* g = (q < 2) ? &graph[q] : &chunk_graph;
+      hipGraphExec_t* x = (q < 2) ? &graph_exec[q] : &chunk_exec;
+      e = hipStreamEndCapture(s, g);
+      if (le == hipSuccess && e == hipSuccess) e = hipGraphInstantiate(x, *g, nullptr, nullptr, 0);
       if (le != hipSuccess || e != hipSuccess) {
         parity = saved;
         return le != hipSuccess ? le : e;
@@ -241,7 +274,14 @@ hipError_t RigidStepperBase::run(int nsteps, bool use_graph, hipStream_t s) {
     parity = saved;
     graph_stream = s;
   }
-  for (int q = 0; q < nsteps; ++q) {
+  int q = 0;
+  if (parity == 0) {
+    for (; q + kChunkSteps <= nsteps; q += kChunkSteps) {
+      hipError_t e = hipGraphLaunch(chunk_exec, s);
+      if (e != hipSuccess) return e;
+    }
+  }
+  for (; q < nsteps; ++q) {
     hipError_t e = hipGraphLaunch(graph_exec[parity], s);
     if (e != hipSuccess) return e;
     parity ^= 1;

@@ -24,6 +24,7 @@ enum : int {
   kScFreqStep,    // period timer after this step's reset test (before + dt)
   kScPrefacStep,  // ramp factor the last completed step used
   kScUpenStep,    // penalisation velocity the last completed step used
+  kScPending,     // 1 while a step's force partials await finalisation
   kScCount
 };
 
@@ -48,7 +49,7 @@ struct RigidBuffers {
   const double* kill_wr;
 };
 
-constexpr int kRigidStages = 7;
+constexpr int kRigidStages = 6;
 extern const char* const kRigidStageNames[kRigidStages];
 
 // `batch` cases of one (nr, nz) grid stepped together; every field buffer
@@ -69,6 +70,12 @@ struct RigidStepperBase {
   int parity = 0;
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  // kChunkSteps consecutive steps (an even count, so the vorticity pair
+  // returns to the parity it started from) in one graph: a graph launch
+  // costs microseconds, as much as a small kernel.
+  static constexpr int kChunkSteps = 16;
+  hipGraph_t chunk_graph = nullptr;
+  hipGraphExec_t chunk_exec = nullptr;
   hipStream_t graph_stream = nullptr;
 
   hipError_t init(const RigidConfig* cfgs, int nbatch, const RigidBuffers& b);

@@ -11,8 +11,9 @@
 // route, so FMA contraction is allowed here (the library default is off).
 #pragma clang fp contract(fast)
 
-#include "axs_stokes.hpp"
 #include "axs_fft1024.hpp"
+#include "axs_fused.hpp"
+#include "axs_stokes.hpp"
 
 #include <algorithm>
 #include <cstdlib>
@@ -274,7 +275,16 @@ __global__ void __launch_bounds__(256) taper_dct2_rows_kernel(
     const double* __restrict__ rscale, const double* __restrict__ wz,
     const double* __restrict__ wr, int nr, int nz, int w, FftPlanArgs fp,
     const Cplx<double>* __restrict__ tw, const Cplx<double>* __restrict__ e1,
-    const Cplx<double>* __restrict__ e2) {
+    const Cplx<double>* __restrict__ e2, StepFinalize fin) {
+  if (fin.partials && blockIdx.x == nr) {  // completes the previous step
+    __shared__ double fin_red[256];
+    const int cb = blockIdx.y;
+    const RigidConfig& cfg = fin.cfgs[cb];
+    finalize_pending(fin.partials + (int64_t)cb * fin.count, fin.count,
+                     fin.slots + (int64_t)cb * kSlotsPerCase, fin.sc + (int64_t)cb * AXS_RIGID_SCALARS,
+                     fin.drag ? fin.drag + (int64_t)cb * cfg.history : nullptr, cfg, fin_red);
+    return;
+  }
   extern __shared__ unsigned char smem_raw[];
   const int M = fp.M;
   Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
@@ -355,8 +365,17 @@ __global__ void __launch_bounds__(64) dct2_rows_m1024_kernel(
     const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
     const double* __restrict__ rscale, const double* __restrict__ wz,
     const double* __restrict__ wr, int nr, int w, const Cplx<double>* __restrict__ tw,
-    const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2) {
+    const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2, StepFinalize fin) {
   constexpr int N = 2 * kF1024M;
+  if (fin.partials && blockIdx.x == nr) {  // completes the previous step
+    __shared__ double fin_red[256];
+    const int cb = blockIdx.y;
+    const RigidConfig& cfg = fin.cfgs[cb];
+    finalize_pending(fin.partials + (int64_t)cb * fin.count, fin.count,
+                     fin.slots + (int64_t)cb * kSlotsPerCase, fin.sc + (int64_t)cb * AXS_RIGID_SCALARS,
+                     fin.drag ? fin.drag + (int64_t)cb * cfg.history : nullptr, cfg, fin_red);
+    return;
+  }
   __shared__ Cplx<T> buf[kF1024Lds];
   const int l = threadIdx.x;
   const int i = blockIdx.x;
@@ -594,7 +613,8 @@ hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const do
   const size_t shm = sizeof(Cplx<T>) * M;
   if (M == kF1024M) {
     dct2_rows_m1024_kernel<T, false><<<rows, 64, 0, s>>>(src, nullptr, dst, rscale, nullptr,
-                                                          nullptr, rows, 0, pl.tw, pl.e1, pl.e2);
+                                                          nullptr, rows, 0, pl.tw, pl.e1, pl.e2,
+                                                          StepFinalize{});
     return hipGetLastError();
   }
   if (per <= 1)
@@ -611,20 +631,21 @@ hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const do
 template <typename T>
 hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered, T* dst,
                                   const double* wz, const double* wr, int w, hipStream_t s,
-                                  int batch) {
+                                  int batch, const StepFinalize* fin) {
+  const StepFinalize f = fin ? *fin : StepFinalize{};
+  const int gx = pl.nr + (f.partials ? 1 : 0);
   const int M = pl.nz / 2;
   const int bs = fft_block(M);
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
   if (M == kF1024M) {
-    dct2_rows_m1024_kernel<T, true><<<dim3(pl.nr, batch), 64, 0, s>>>(
-        src, tapered, dst, pl.rvec, wz, wr, pl.nr, w, pl.tw, pl.e1, pl.e2);
+    dct2_rows_m1024_kernel<T, true><<<dim3(gx, batch), 64, 0, s>>>(
+        src, tapered, dst, pl.rvec, wz, wr, pl.nr, w, pl.tw, pl.e1, pl.e2, f);
     return hipGetLastError();
   }
 #define AXS_TAPER_LAUNCH(P)                                                                    \
-  taper_dct2_rows_kernel<T, P><<<dim3(pl.nr, batch), bs, shm, s>>>(                          \
-      src, tapered, dst, pl.rvec, wz, wr, pl.nr,                                               \
-                                                      pl.nz, w, pl.fft, pl.tw, pl.e1, pl.e2)
+  taper_dct2_rows_kernel<T, P><<<dim3(gx, batch), bs, shm, s>>>(                              \
+      src, tapered, dst, pl.rvec, wz, wr, pl.nr, pl.nz, w, pl.fft, pl.tw, pl.e1, pl.e2, f)
   if (per <= 1)
     AXS_TAPER_LAUNCH(1);
   else if (per <= 2)
@@ -692,10 +713,10 @@ template hipError_t launch_dct3_rows<float>(const StokesPlan&, const float*, flo
 template hipError_t launch_tridiag<float>(const StokesPlan&, float*, hipStream_t, int);
 template hipError_t launch_taper_dct2_rows<double>(const StokesPlan&, const double*, double*,
                                                    double*, const double*, const double*, int,
-                                                   hipStream_t, int);
+                                                   hipStream_t, int, const StepFinalize*);
 template hipError_t launch_taper_dct2_rows<float>(const StokesPlan&, const float*, float*, float*,
                                                   const double*, const double*, int, hipStream_t,
-                                                  int);
+                                                  int, const StepFinalize*);
 
 }  // namespace axs
 

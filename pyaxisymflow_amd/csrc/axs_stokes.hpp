@@ -32,6 +32,20 @@ struct StokesPlan {
   ~StokesPlan();
 };
 
+struct RigidConfig;
+
+// Extra work of the first pass of a rigid step: one more workgroup per case
+// completes the previous step (force record, t, it, next scalars) while the
+// row transforms run (finalize_pending in axs_fused.hpp).
+struct StepFinalize {
+  const double* partials = nullptr;  // null: no extra workgroup
+  int count = 0;                     // partials per case
+  double* slots = nullptr;
+  double* sc = nullptr;
+  double* drag = nullptr;
+  const RigidConfig* cfgs = nullptr;
+};
+
 template <typename T>
 hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const double* rscale,
                             int rows, hipStream_t s);
@@ -41,7 +55,7 @@ hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const do
 template <typename T>
 hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered, T* dst,
                                   const double* wz, const double* wr, int w, hipStream_t s,
-                                  int batch = 1);
+                                  int batch = 1, const StepFinalize* fin = nullptr);
 template <typename T>
 hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, hipStream_t s,
                             int batch = 1);

@@ -20,7 +20,7 @@ from pyaxisymflow_amd import _lib
 # device scalar slots (axs_stepper.hpp)
 SC_T, SC_DT, SC_DT_BY_DX, SC_LAM_DT, SC_HALF_NU_DT, SC_NU_DT, SC_PREFAC, SC_UPEN, SC_MAXU, \
     SC_DRAG_SUM, SC_IT, SC_FREQ_TIMER, SC_AVG_RESET, SC_BODY_Z, SC_FREQ_STEP, SC_PREFAC_STEP, \
-    SC_UPEN_STEP = range(17)
+    SC_UPEN_STEP, SC_PENDING = range(18)
 N_SCALARS = 24  # AXS_RIGID_SCALARS
 
 

@@ -93,6 +93,11 @@ def test_graph_replay_matches_eager():
         a.synchronize()
         b.step(5)
     assert np.array_equal(a.vorticity(), b.vorticity())
+    # multi-step graphs (16 steps per launch) plus single-step remainders
+    a.step(37)
+    b.step(37)
+    assert a.steps_done() == b.steps_done() == 57
+    assert np.array_equal(a.vorticity(), b.vorticity())
     assert np.array_equal(a.drag_history(), b.drag_history())
     assert np.array_equal(a.scalars(), b.scalars())
 
