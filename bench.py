@@ -276,6 +276,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--large-steps", type=int, default=100,
+                    help="steps of the 4096x2048 FP64 point (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-steps", type=int, default=2000)
     ap.add_argument("--no-serve", action="store_true")
@@ -331,6 +333,14 @@ def main():
         },
     }
     forms = {}
+    if args.large_steps > 0 and (nz, nr) != (4096, 2048):
+        ms2, finite2, _, st2, _ = time_gpu(4096, 2048, args.large_steps, 8, 5)
+        forms["fp64_4096x2048"] = dict(
+            value=round(2048 * 4096 * args.large_steps / (ms2 / 1e3) / 1e6, 2),
+            unit="Mcell-steps/s", ms_per_step=round(ms2 / args.large_steps, 5),
+            steps=args.large_steps, finite=finite2,
+            stage_us={k: round(v, 2) for k, v in st2.items()},
+            note="same workload generator at nz x nr = 4096 x 2048 (z in [0,2], r in [0,1])")
     if not args.no_batched:
         forms["batched_fp32"] = time_batched(args.batched_steps, 2)
     if not args.no_serve:

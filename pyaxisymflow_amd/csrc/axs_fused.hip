@@ -316,9 +316,14 @@ __device__ __forceinline__ void step_fused_tile(
     const int e = tid + q * 256;
     if (e < kBufA) L.A[e] = vp[q];
   }
-  __syncthreads();
+  // Does the body reach this tile's R5 region?  Where chi = 0, Brinkman
+  // leaves u unchanged bit for bit (u + 0 U, no division), so u_pen - u = 0,
+  // the penalisation curl is 0 and the force term is a signed zero.
+  bool touches = false;
+#pragma unroll
+  for (int k = 0; k < kVS + 2; ++k) touches |= (vc[k] != T(0));
+  const bool body = __syncthreads_or(touches) != 0;
 
-  // ---- 2. dt, then u and u_pen on the strip's R5 cells
   const double mx = *L.smax;
   const DtScalars ds = compute_dt(sc, mx, cfg);
   const T two_dx = T(2) * dx, inv2dx = T(1) / two_dx;
@@ -327,8 +332,40 @@ __device__ __forceinline__ void step_fused_tile(
   const double Ud = sc[kScUpen];
   const T U = T(Ud);
   const T tdt = T(ds.dt);
-  T dz[kVS + 2];  // u_pen_z - u_z on the strip's R5 rows
   double acc = 0.0;
+  if (IN && !body) {
+    // ---- 2'. body-free interior tile: u on the R4 cells only (the ENO3
+    //          velocity), f = omega + 0 (the zero curl, so signed zeros
+    //          match the general path)
+    if (pb >= 1 && pb <= kR4Z) {
+#pragma unroll
+      for (int k = 1; k <= kVS; ++k) {
+        const int q5 = kVS * ps + k, a4 = q5 - 1;
+        const T* p = &sp[q5 + 1][pb + 1];
+        const T uz =
+            div_rn(div_rn(p[kR6Z] - p[-kR6Z], two_dx, inv2dx), sri[q5 + 1], sinv[q5 + 1]) + ufree;
+        const T ur = div_rn(div_rn(-(p[1] - p[-1]), two_dx, inv2dx), sri[q5 + 1], sinv[q5 + 1]);
+        const T ldc = lam_dt * vc[k];
+        const T pzv = brinkman(uz, ldc, U);
+        const T prv = brinkman(ur, ldc, T(0));
+        if (avg && q5 >= 5 && q5 < 5 + kFR && pb >= 5 && pb < 5 + kFZ) {
+          const int c = (i0 - 5 + q5) * nz + pj;
+          const T base = (ds.reset != 0.0) ? T(0) : avg[c];
+          avg[c] = base + p[0] * tdt;
+        }
+        sx[a4][pb - 1] = pzv;
+        sy[a4][pb - 1] = prv;
+        vw[k - 1] = vw[k - 1] + T(0);
+      }
+    }
+    __syncthreads();  // the psi tile (bufA) is read above and overwritten below
+    if (pb >= 1 && pb <= kR4Z) {
+#pragma unroll
+      for (int k = 1; k <= kVS; ++k) sf[kVS * ps + k - 1][pb - 1] = vw[k - 1];
+    }
+    __syncthreads();
+  } else {
+  T dz[kVS + 2];  // u_pen_z - u_z on the strip's R5 rows
 #pragma unroll
   for (int k = 0; k < kVS + 2; ++k) {
     const int q5 = kVS * ps + k;  // R5 row
@@ -398,6 +435,7 @@ __device__ __forceinline__ void step_fused_tile(
     }
   }
   __syncthreads();
+  }  // body path
   if constexpr (!IN) {
     if (i0 == 0) {
       // rows -4..-1 of the ENO3 input are the axis mirror of rows 3..0
