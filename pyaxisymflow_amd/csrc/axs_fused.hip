@@ -171,16 +171,30 @@ __global__ void __launch_bounds__(256) velocity_max_kernel(const T* __restrict__
   const T two_dx = T(2) * dx, inv2dx = T(1) / two_dx;
   const T ufree = T(cfg.U0 * sc[kScPrefac]);
   double m = 0.0;
+  if (i0 >= 1 && i0 + kVTR <= nr - 1 && j0 >= 1 && j0 + kVTZ <= nz - 1) {
+    // no cell of the tile lies on a domain edge: central differences only
+    // (the formulas velocity_rn takes away from the edges)
 #pragma unroll
-  for (int q = 0; q < (kVTR * kVTZ) / 256; ++q) {
-    const int e = tid + q * 256;
-    const int a = e / kVTZ, b = e % kVTZ;
-    const int i = i0 + a, j = j0 + b;
-    if (i < nr && j < nz) {
-      T uz, ur;
-      velocity_rn<T>(&sp[a + 2][b + 2], kVPZ, i, j, nr, nz, two_dx, inv2dx, sri[a], sinv[a], ufree,
-                     uz, ur);
+    for (int q = 0; q < (kVTR * kVTZ) / 256; ++q) {
+      const int e = tid + q * 256;
+      const int a = e / kVTZ, b = e % kVTZ;
+      const T* p = &sp[a + 2][b + 2];
+      const T uz = div_rn(div_rn(p[kVPZ] - p[-kVPZ], two_dx, inv2dx), sri[a], sinv[a]) + ufree;
+      const T ur = div_rn(div_rn(-(p[1] - p[-1]), two_dx, inv2dx), sri[a], sinv[a]);
       m = fmax(m, double(fabs(uz) + fabs(ur)));
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < (kVTR * kVTZ) / 256; ++q) {
+      const int e = tid + q * 256;
+      const int a = e / kVTZ, b = e % kVTZ;
+      const int i = i0 + a, j = j0 + b;
+      if (i < nr && j < nz) {
+        T uz, ur;
+        velocity_rn<T>(&sp[a + 2][b + 2], kVPZ, i, j, nr, nz, two_dx, inv2dx, sri[a], sinv[a],
+                       ufree, uz, ur);
+        m = fmax(m, double(fabs(uz) + fabs(ur)));
+      }
     }
   }
   m = wave_max(m);
@@ -213,10 +227,9 @@ static_assert((kR2R / kEH) * kR2Z <= 256 && (kR1R / kSH) * kR1Z <= 256 && (kFR /
               "one strip per thread");
 // LDS buffers and what they hold phase by phase.
 constexpr int kBufA = kR6R * kR6Z;  // psi (R6), then f (R4)
-constexpr int kBufB = kR4R * kR5Z;  // u_pen_r - u_r (R4 rows x R5 cols), then ENO3 out (R2)
 constexpr int kBufC = kR4R * kR4Z;  // u_pen_z (R4), then stage-1 out (R1)
-constexpr int kBufD = kR4R * kR4Z;  // u_pen_r (R4)
-static_assert(kR4R * kR4Z <= kBufA && kR2R * kR2Z <= kBufB && kR1R * kR1Z <= kBufC, "aliases");
+constexpr int kBufD = kR4R * kR4Z;  // u_pen_r (R4), then ENO3 out (R2)
+static_assert(kR4R * kR4Z <= kBufA && kR2R * kR2Z <= kBufD && kR1R * kR1Z <= kBufC, "aliases");
 
 int fused_blocks(int nr, int nz) { return (int)(ceil_div(nz, kFZ) * ceil_div(nr, kFR)); }
 
@@ -225,9 +238,8 @@ int fused_blocks(int nr, int nz) { return (int)(ceil_div(nz, kFZ) * ceil_div(nr,
 template <typename T>
 struct FusedLds {
   T* A;  // psi (R6), then f (R4)
-  T* B;  // u_pen_r - u_r (R4 rows x R5 cols), then ENO3 out (R2)
   T* C;  // u_pen_z (R4), then stage-1 out (R1)
-  T* D;  // u_pen_r (R4)
+  T* D;  // u_pen_r (R4), then ENO3 out (R2)
   T *sri, *sinv, *sinv2;  // per R6 row: r_i, 1 / r_i, 1 / r_i^2
   double* srd;            // per R6 row: r_i (FP64, for the force)
   double* wpart;
@@ -247,8 +259,7 @@ __device__ __forceinline__ void step_fused_tile(
     const RigidConfig& cfg, const FusedLds<T>& L, int i0, int j0, int tile) {
   T(*sp)[kR6Z] = reinterpret_cast<T(*)[kR6Z]>(L.A);
   T(*sf)[kR4Z] = reinterpret_cast<T(*)[kR4Z]>(L.A);
-  T(*sdr)[kR5Z] = reinterpret_cast<T(*)[kR5Z]>(L.B);
-  T(*sa)[kR2Z] = reinterpret_cast<T(*)[kR2Z]>(L.B);
+  T(*sa)[kR2Z] = reinterpret_cast<T(*)[kR2Z]>(L.D);
   T(*sx)[kR4Z] = reinterpret_cast<T(*)[kR4Z]>(L.C);
   T(*st)[kR1Z] = reinterpret_cast<T(*)[kR1Z]>(L.C);
   T(*sy)[kR4Z] = reinterpret_cast<T(*)[kR4Z]>(L.D);
@@ -366,6 +377,7 @@ __device__ __forceinline__ void step_fused_tile(
     __syncthreads();
   } else {
   T dz[kVS + 2];  // u_pen_z - u_z on the strip's R5 rows
+  T dr[kVS];      // u_pen_r - u_r on the strip's R4 rows
 #pragma unroll
   for (int k = 0; k < kVS + 2; ++k) {
     const int q5 = kVS * ps + k;  // R5 row
@@ -401,7 +413,7 @@ __device__ __forceinline__ void step_fused_tile(
     dz[k] = dzv;
     if (k >= 1 && k <= kVS) {
       const int a4 = q5 - 1;
-      sdr[a4][pb] = drv;
+      dr[k - 1] = drv;
       if (pb >= 1 && pb <= kR4Z) {
         sx[a4][pb - 1] = pzv;
         sy[a4][pb - 1] = prv;
@@ -410,7 +422,14 @@ __device__ __forceinline__ void step_fused_tile(
   }
   __syncthreads();
 
-  // ---- 3. f = taper(omega) + curl(u_pen - u) on the strip's R4 cells
+  // ---- 3. f = taper(omega) + curl(u_pen - u) on the strip's R4 cells; the
+  //         z neighbours of a strip column are the neighbouring lanes
+  T drp[kVS], drm[kVS];
+#pragma unroll
+  for (int k = 0; k < kVS; ++k) {
+    drp[k] = __shfl_down(dr[k], 1);
+    drm[k] = __shfl_up(dr[k], 1);
+  }
   if (pb >= 1 && pb <= kR4Z) {
 #pragma unroll
     for (int k = 1; k <= kVS; ++k) {
@@ -418,14 +437,14 @@ __device__ __forceinline__ void step_fused_tile(
       const int i = i0 - 4 + a4;
       T fv = T(0);
       if constexpr (IN) {
-        const T pv = div_rn(sdr[a4][pb + 1] - sdr[a4][pb - 1], two_dx, inv2dx) -
+        const T pv = div_rn(drp[k - 1] - drm[k - 1], two_dx, inv2dx) -
                      div_rn(dz[k + 1] - dz[k - 1], two_dx, inv2dx);
         fv = vw[k - 1] + pv;
       } else {
         if (pjin && i >= 0 && i < nr) {
           fv = taper_apply(vw[k - 1], i, pj, nr, nz, kw, wz, wr);
           if (i >= 1 && i <= nr - 2 && pj >= 1 && pj <= nz - 2) {
-            const T pv = div_rn(sdr[a4][pb + 1] - sdr[a4][pb - 1], two_dx, inv2dx) -
+            const T pv = div_rn(drp[k - 1] - drm[k - 1], two_dx, inv2dx) -
                          div_rn(dz[k + 1] - dz[k - 1], two_dx, inv2dx);
             fv = fv + pv;
           }
@@ -453,8 +472,11 @@ __device__ __forceinline__ void step_fused_tile(
   // ---- 4. ENO3 on R2 (advection_timestep.py:45-54); r faces slide through
   //         registers, each evaluated once
   const T s = -T(ds.dt / cfg.dx);
-  if (tid < (kR2R / kEH) * kR2Z) {
-    const int b = tid % kR2Z, a0 = (tid / kR2Z) * kEH;
+  const bool eno = tid < (kR2R / kEH) * kR2Z;
+  const int eb = tid % kR2Z, ea0 = (tid / kR2Z) * kEH;
+  T out[kEH];
+  if (eno) {
+    const int b = eb, a0 = ea0;
     const int ib = b + 2;        // R4 column
     T wf[kEH + 4], wy[kEH + 4];  // R4 rows a0 .. a0 + kEH + 3
 #pragma unroll
@@ -469,7 +491,6 @@ __device__ __forceinline__ void step_fused_tile(
                           wy[m + 3]);
     const int j = j0 - 2 + b;
     const bool jok = IN || (j >= 2 && j < nz - 2);
-    T out[kEH];
 #pragma unroll
     for (int k = 0; k < kEH; ++k) {
       const int ia = a0 + k + 2;
@@ -488,9 +509,11 @@ __device__ __forceinline__ void step_fused_tile(
       }
       out[k] = v;
     }
-    // bufB (u_pen_r - u_r) was last read before the previous barrier
+  }
+  __syncthreads();  // u_pen_r (bufD) is read above and overwritten below
+  if (eno) {
 #pragma unroll
-    for (int k = 0; k < kEH; ++k) sa[a0 + k][b] = out[k];
+    for (int k = 0; k < kEH; ++k) sa[ea0 + k][eb] = out[k];
   }
   __syncthreads();
 
@@ -559,14 +582,15 @@ __device__ __forceinline__ void step_fused_tile(
   }
 }
 
+// 256 threads, at most 128 VGPRs and under 40 KiB of LDS: four blocks per CU.
 template <typename T>
-__global__ void __launch_bounds__(256) step_fused_kernel(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) step_fused_kernel(
     const T* __restrict__ psi, const T* __restrict__ chi, const T* __restrict__ wsrc,
     T* __restrict__ wdst, T* __restrict__ avg, const double* __restrict__ r,
     const double* __restrict__ wz, const double* __restrict__ wr, int kw, int nr, int nz, T dx,
     double* __restrict__ sc, const double* __restrict__ slots, double* __restrict__ partials,
     const RigidConfig* __restrict__ cfgs) {
-  __shared__ T bufA[kBufA], bufB[kBufB], bufC[kBufC], bufD[kBufD];
+  __shared__ T bufA[kBufA], bufC[kBufC], bufD[kBufD];
   __shared__ T sri[kR6R], sinv[kR6R], sinv2[kR6R];
   __shared__ double srd[kR6R];
   __shared__ double wpart[4];
@@ -582,7 +606,7 @@ __global__ void __launch_bounds__(256) step_fused_kernel(
   sc += (int64_t)cb * AXS_RIGID_SCALARS;
   slots += (int64_t)cb * kSlotsPerCase;
   partials += (int64_t)cb * gridDim.x * gridDim.y;
-  const FusedLds<T> L{bufA, bufB, bufC, bufD, sri, sinv, sinv2, srd, wpart, &smax};
+  const FusedLds<T> L{bufA, bufC, bufD, sri, sinv, sinv2, srd, wpart, &smax};
   // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin by
   // linear id; give each XCD a contiguous band of tiles so a tile's halo
   // rows are still in that XCD's L2 when its neighbours read them.
