@@ -423,17 +423,69 @@ template <typename T>
 __global__ void __launch_bounds__(64) dct3_rows_m1024_kernel(
     const T* __restrict__ src, T* __restrict__ dst, const Cplx<double>* __restrict__ tw,
     const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2) {
-  constexpr int N = 2 * kF1024M;
+  constexpr int M = kF1024M, N = 2 * kF1024M;
+  constexpr int KP = M / 2 / 64;  // index pairs (k, M - k) per lane
+  typedef T T2 __attribute__((ext_vector_type(2)));
   __shared__ Cplx<T> buf[kF1024Lds];
+  T* raw = reinterpret_cast<T*>(buf);  // the input row, staged before the pre step
   const int l = threadIdx.x;
   const int i = blockIdx.x;
   const int64_t cs = (int64_t)blockIdx.y * gridDim.x * N;  // case of a batch
-  const T* row = src + cs + (int64_t)i * N;
-  auto get = [&](int k) { return row[k]; };
-  dct3_pre_w64<T, 0, 4>(buf, e1, e2, get);
-  dct3_pre_w64<T, 4, 4>(buf, e1, e2, get);
-  dct3_pre_w64<T, 8, 4>(buf, e1, e2, get);
-  dct3_pre_w64<T, 12, 4>(buf, e1, e2, get);
+  const T2* row = reinterpret_cast<const T2*>(src + cs + (int64_t)i * N);
+  // Every global load of the pre step is issued before the first wait: the
+  // row once (two values per lane and load) and the twiddles of the pairs.
+  T2 v[N / 128];
+#pragma unroll
+  for (int q = 0; q < N / 128; ++q) v[q] = row[l + 64 * q];
+  Cplx<double> q2k[KP], q2m[KP], q1k[KP], q1m[KP];
+#pragma unroll
+  for (int q = 0; q < KP; ++q) {
+    const int k = l + 64 * q;
+    q2k[q] = e2[k];
+    q2m[q] = e2[M - k];
+    q1k[q] = e1[k];
+    q1m[q] = e1[M - k];
+  }
+  const Cplx<double> q2h = e2[M / 2], q1h = e1[M / 2];
+#pragma unroll
+  for (int q = 0; q < N / 128; ++q) reinterpret_cast<T2*>(raw)[l + 64 * q] = v[q];
+  __syncthreads();
+  T xk[KP], xnk[KP], xm[KP], xnm[KP];
+#pragma unroll
+  for (int q = 0; q < KP; ++q) {
+    const int k = l + 64 * q;
+    xk[q] = raw[k];
+    xnk[q] = (k == 0) ? T(0) : raw[N - k];
+    xm[q] = raw[M - k];
+    xnm[q] = raw[M + k];
+  }
+  const T xh = raw[M / 2], xnh = raw[N - M / 2];
+  __syncthreads();  // raw is read above and overwritten by Z below
+  // DCT-III pre step (as dct3_pre): Z[k] and Z[M - k] share V_k and V_{M-k}
+  // with their roles swapped, so each pair is formed from one set of loads
+  // with the arithmetic the single-index form does for each of the two.
+#pragma unroll
+  for (int q = 0; q < KP; ++q) {
+    const int k = l + 64 * q;
+    const Cplx<T> Vk = cmul(Cplx<T>{T(q2k[q].x), T(-q2k[q].y)}, Cplx<T>{xk[q], -xnk[q]});
+    const Cplx<T> Vm = cmul(Cplx<T>{T(q2m[q].x), T(-q2m[q].y)}, Cplx<T>{xm[q], -xnm[q]});
+    {
+      const Cplx<T> A = cadd(Vk, cconj(Vm)), D = csub(Vk, cconj(Vm));
+      const Cplx<T> B = cmul(D, Cplx<T>{T(q1k[q].x), T(-q1k[q].y)});
+      buf[k] = Cplx<T>{A.x - B.y, A.y + B.x};
+    }
+    if (k != 0) {
+      const Cplx<T> A = cadd(Vm, cconj(Vk)), D = csub(Vm, cconj(Vk));
+      const Cplx<T> B = cmul(D, Cplx<T>{T(q1m[q].x), T(-q1m[q].y)});
+      buf[M - k] = Cplx<T>{A.x - B.y, A.y + B.x};
+    }
+  }
+  if (l == 0) {  // k = M / 2 is its own partner
+    const Cplx<T> Vk = cmul(Cplx<T>{T(q2h.x), T(-q2h.y)}, Cplx<T>{xh, -xnh});
+    const Cplx<T> A = cadd(Vk, cconj(Vk)), D = csub(Vk, cconj(Vk));
+    const Cplx<T> B = cmul(D, Cplx<T>{T(q1h.x), T(-q1h.y)});
+    buf[M / 2] = Cplx<T>{A.x - B.y, A.y + B.x};
+  }
   __syncthreads();
   fft1024_wave<T, +1>(buf, tw);
   const T invN = T(1) / T(N);
