@@ -210,36 +210,4 @@ __device__ __forceinline__ void dct2_post_w64(const Cplx<T>* Z, const Cplx<doubl
   }
 }
 
-// DCT-III pre step (as dct3_pre) for k = threadIdx.x + 64 q, q in [Q0, Q0 + KQ).
-template <typename T, int Q0, int KQ, typename In>
-__device__ __forceinline__ void dct3_pre_w64(Cplx<T>* Z, const Cplx<double>* __restrict__ e1,
-                                             const Cplx<double>* __restrict__ e2, In in) {
-  constexpr int M = kF1024M, N = 2 * kF1024M;
-  T xk[KQ], xnk[KQ], xm[KQ], xnm[KQ];
-  Cplx<double> q2k[KQ], q2m[KQ], q1k[KQ];
-#pragma unroll
-  for (int q = 0; q < KQ; ++q) {
-    const int k = threadIdx.x + 64 * (Q0 + q);
-    const int mk = M - k;
-    xk[q] = in(k);
-    xnk[q] = (k == 0) ? T(0) : in(N - k);
-    xm[q] = in(mk);
-    xnm[q] = in(N - mk);
-    q2k[q] = e2[k];
-    q2m[q] = e2[mk];
-    q1k[q] = e1[k];
-  }
-#pragma unroll
-  for (int q = 0; q < KQ; ++q) {
-    const int k = threadIdx.x + 64 * (Q0 + q);
-    const Cplx<T> Vk = cmul(Cplx<T>{T(q2k[q].x), T(-q2k[q].y)}, Cplx<T>{xk[q], -xnk[q]});
-    const Cplx<T> Vm = cmul(Cplx<T>{T(q2m[q].x), T(-q2m[q].y)}, Cplx<T>{xm[q], -xnm[q]});
-    const Cplx<T> Vkm = cconj(Vm);
-    const Cplx<T> A = cadd(Vk, Vkm);
-    const Cplx<T> D = csub(Vk, Vkm);
-    const Cplx<T> B = cmul(D, Cplx<T>{T(q1k[q].x), T(-q1k[q].y)});
-    Z[k] = Cplx<T>{A.x - B.y, A.y + B.x};
-  }
-}
-
 }  // namespace axs

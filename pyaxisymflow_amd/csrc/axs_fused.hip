@@ -243,7 +243,7 @@ struct FusedLds {
   T *sri, *sinv, *sinv2;  // per R6 row: r_i, 1 / r_i, 1 / r_i^2
   double* srd;            // per R6 row: r_i (FP64, for the force)
   double* wpart;
-  double* smax;
+  double* smax;  // slot maximum, then the step's dt, timer and reset flag
 };
 
 // One output tile.  IN: the tile and its 6-cell halo lie at least kill_width
@@ -311,7 +311,15 @@ __device__ __forceinline__ void step_fused_tile(
   }
   if (tid < 64) {
     const double m = wave_max(load_slot(slots + tid * kSlotStride));
-    if (tid == 0) *L.smax = m;
+    if (tid == 0) {
+      // the dt rule once per block (not per thread: its temporaries would
+      // overlap the tile loads still in flight)
+      const DtScalars d = compute_dt(sc, m, cfg);
+      L.smax[0] = m;
+      L.smax[1] = d.dt;
+      L.smax[2] = d.fr;
+      L.smax[3] = d.reset;
+    }
   }
   if (tid >= 64 && tid < 64 + kR6R) {
     const int i = i0 - 6 + (tid - 64);
@@ -335,8 +343,8 @@ __device__ __forceinline__ void step_fused_tile(
   for (int k = 0; k < kVS + 2; ++k) touches |= (vc[k] != T(0));
   const bool body = __syncthreads_or(touches) != 0;
 
-  const double mx = *L.smax;
-  const DtScalars ds = compute_dt(sc, mx, cfg);
+  const double mx = L.smax[0];
+  const DtScalars ds{L.smax[1], L.smax[2], L.smax[3]};
   const T two_dx = T(2) * dx, inv2dx = T(1) / two_dx;
   const T ufree = T(cfg.U0 * sc[kScPrefac]);
   const T lam_dt = T(cfg.brink_lam * ds.dt);
@@ -594,7 +602,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) s
   __shared__ T sri[kR6R], sinv[kR6R], sinv2[kR6R];
   __shared__ double srd[kR6R];
   __shared__ double wpart[4];
-  __shared__ double smax;
+  __shared__ double smax[4];
   const int cb = blockIdx.z;  // case of a batch
   const RigidConfig& cfg = cfgs[cb];
   const int64_t fo = (int64_t)cb * nr * nz;
@@ -606,7 +614,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) s
   sc += (int64_t)cb * AXS_RIGID_SCALARS;
   slots += (int64_t)cb * kSlotsPerCase;
   partials += (int64_t)cb * gridDim.x * gridDim.y;
-  const FusedLds<T> L{bufA, bufC, bufD, sri, sinv, sinv2, srd, wpart, &smax};
+  const FusedLds<T> L{bufA, bufC, bufD, sri, sinv, sinv2, srd, wpart, smax};
   // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin by
   // linear id; give each XCD a contiguous band of tiles so a tile's halo
   // rows are still in that XCD's L2 when its neighbours read them.
