@@ -508,7 +508,20 @@ __global__ void __launch_bounds__(64) dct3_rows_m1024_kernel(
 // before the recurrence starts and the coefficients sit in LDS, so the
 // serial chains never wait on global memory.  Workgroups are mapped so that
 // neighbouring mode chunks run on the same XCD and share its L2 lines.
-template <typename T, int L>
+// 1 / m to within an ulp: the hardware estimate and two Newton steps (the
+// solve is not a correctly rounded quotient anyway; pivots are O(1/dx^2),
+// far from the range the IEEE sequence's scaling guards).
+__device__ __forceinline__ double recip_nr(double m) {
+  double y = __builtin_amdgcn_rcp(m);
+  double e = fma(-m, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-m, y, 1.0);
+  return fma(y, e, y);
+}
+
+// FULL: nr = P L and every chunk holds K modes, so no thread or row is idle
+// and no bounds test is needed.
+template <typename T, int L, bool FULL>
 __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
     T* __restrict__ d, const double* __restrict__ a, const double* __restrict__ b,
     const double* __restrict__ c, const double* __restrict__ lam,
@@ -535,16 +548,17 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
   const int kk = threadIdx.x % K;
   const int p = threadIdx.x / K;
   const int k = chunk * K + kk;
-  const bool active = (k < nz) && (p < P);
+  const bool active = FULL || ((k < nz) && (p < P));
   const int s = p * L;
   const int kc = active ? k : 0;
+  T* col = d + ((int64_t)s * nz + kc);  // row s of this thread's mode
 
   double x[L];
   double im[L];
 #pragma unroll
   for (int l = 0; l < L; ++l) {
     const int i = s + l;
-    x[l] = (active && i < nr) ? static_cast<double>(d[(int64_t)i * nz + kc]) : 0.0;
+    x[l] = (FULL || (active && i < nr)) ? static_cast<double>(col[(int64_t)l * nz]) : 0.0;
   }
   const double lk = lam[kc];
   double cp = (p == 0 || !active) ? 0.0 : cseg[(size_t)p * nz + kc];
@@ -554,10 +568,10 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
 #pragma unroll
   for (int l = 0; l < L; ++l) {
     const int i = s + l;
-    if (active && i < nr) {
+    if (FULL || (active && i < nr)) {
       const double ai = sA[i];
       const double m = (sB[i] + lk) - ai * cp;
-      const double inv = 1.0 / m;
+      const double inv = recip_nr(m);
       im[l] = inv;
       cp = sC[i] * inv;
       y = (x[l] - ai * y) * inv;
@@ -596,7 +610,7 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       const int i = s + l;
-      if (active && i < nr) {
+      if (FULL || (active && i < nr)) {
         g = g * (-sA[i] * im[l]);
         x[l] = x[l] + g * Din;
       }
@@ -607,7 +621,7 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
 #pragma unroll
   for (int l = L - 1; l >= 0; --l) {
     const int i = s + l;
-    if (active && i < nr) {
+    if (FULL || (active && i < nr)) {
       const double cpi = sC[i] * im[l];
       z = x[l] - cpi * z;
       H = H * (-cpi);
@@ -640,9 +654,9 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
 #pragma unroll
     for (int l = L - 1; l >= 0; --l) {
       const int i = s + l;
-      if (active && i < nr) {
+      if (FULL || (active && i < nr)) {
         h = h * (-(sC[i] * im[l]));
-        d[(int64_t)i * nz + kc] = static_cast<T>(x[l] + h * Xin);
+        col[(int64_t)l * nz] = static_cast<T>(x[l] + h * Xin);
       }
     }
   }
@@ -738,22 +752,31 @@ hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch) 
   const int K = pl.tri_K, P = pl.tri_P;
   const dim3 blocks(ceil_div(pl.nz, K), batch);
   const size_t shm = sizeof(double) * (4 * K * P + 3 * pl.nr);
+  const bool full = (pl.nr % pl.tri_L == 0) && (pl.nz % K == 0);
+#define AXS_TRI(LL)                                                                              \
+  (full ? (tridiag_columns_kernel<T, LL, true><<<blocks, K * P, shm, s>>>(                        \
+               d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P),                          \
+           0)                                                                                     \
+        : (tridiag_columns_kernel<T, LL, false><<<blocks, K * P, shm, s>>>(                       \
+               d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P),                          \
+           0))
   switch (pl.tri_L) {
     case 4:
-      tridiag_columns_kernel<T, 4><<<blocks, K * P, shm, s>>>(d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P);
+      AXS_TRI(4);
       break;
     case 8:
-      tridiag_columns_kernel<T, 8><<<blocks, K * P, shm, s>>>(d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P);
+      AXS_TRI(8);
       break;
     case 16:
-      tridiag_columns_kernel<T, 16><<<blocks, K * P, shm, s>>>(d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P);
+      AXS_TRI(16);
       break;
     case 32:
-      tridiag_columns_kernel<T, 32><<<blocks, K * P, shm, s>>>(d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P);
+      AXS_TRI(32);
       break;
     default:
       return hipErrorInvalidValue;
   }
+#undef AXS_TRI
   return hipGetLastError();
 }
 
