@@ -170,43 +170,54 @@ __device__ __forceinline__ void fft1024_wave(Cplx<T>* buf, const Cplx<double>* _
 
 namespace axs {
 
-// DCT-II post step (as dct2_post) for k = threadIdx.x + 64 q, q in [Q0, Q0 + KQ).
-template <typename T, int Q0, int KQ, typename Out>
-__device__ __forceinline__ void dct2_post_w64(const Cplx<T>* Z, const Cplx<double>* __restrict__ e1,
-                                              const Cplx<double>* __restrict__ e2, Out out) {
-  constexpr int M = kF1024M, N = 2 * kF1024M;
-  Cplx<double> a1[KQ], b1[KQ], a2[KQ], b2[KQ];
+// DCT-II post step (as dct2_post): lane l takes k = l + 64 q (q < 8) with
+// its partner M - k, writing outputs k, N - k, M - k and N - (M - k); lane 0
+// also takes k = M / 2, its own partner.  The twiddles are loaded before
+// any is used.
+template <typename T>
+__device__ __forceinline__ void dct2_post_pairs(const Cplx<T>* Z, const Cplx<double>* __restrict__ e1,
+                                                const Cplx<double>* __restrict__ e2,
+                                                T* __restrict__ out) {
+  constexpr int M = kF1024M, N = 2 * kF1024M, KP = M / 2 / 64;
+  const int l = threadIdx.x;
+  Cplx<double> a1[KP], b1[KP], a2[KP], b2[KP];
 #pragma unroll
-  for (int q = 0; q < KQ; ++q) {
-    const int k = threadIdx.x + 64 * (Q0 + q);
-    if (k <= M / 2) {
-      a1[q] = e1[k];
-      b1[q] = e1[M - k];
-      a2[q] = e2[k];
-      b2[q] = e2[M - k];
+  for (int q = 0; q < KP; ++q) {
+    const int k = l + 64 * q;
+    a1[q] = e1[k];
+    b1[q] = e1[M - k];
+    a2[q] = e2[k];
+    b2[q] = e2[M - k];
+  }
+  const Cplx<double> ah = e1[M / 2], bh = e2[M / 2];
+#pragma unroll
+  for (int q = 0; q < KP; ++q) {
+    const int k = l + 64 * q, kp = M - k;
+    const Cplx<T> zk = Z[k], zp = Z[kp & (M - 1)];
+    const Cplx<T> sk = cadd(zk, cconj(zp)), dk = csub(zk, cconj(zp));
+    const Cplx<T> wdk = cmul(Cplx<T>{T(a1[q].x), T(a1[q].y)}, dk);
+    const Cplx<T> Vk = {T(0.5) * (sk.x + wdk.y), T(0.5) * (sk.y - wdk.x)};
+    const Cplx<T> wk = cmul(Cplx<T>{T(a2[q].x), T(a2[q].y)}, Vk);
+    const Cplx<T> sp = cadd(zp, cconj(zk)), dp = csub(zp, cconj(zk));
+    const Cplx<T> wdp = cmul(Cplx<T>{T(b1[q].x), T(b1[q].y)}, dp);
+    const Cplx<T> Vp = {T(0.5) * (sp.x + wdp.y), T(0.5) * (sp.y - wdp.x)};
+    const Cplx<T> wp = cmul(Cplx<T>{T(b2[q].x), T(b2[q].y)}, Vp);
+    out[k] = wk.x;
+    out[kp] = wp.x;
+    if (k > 0) {
+      out[N - k] = -wk.y;
+      out[N - kp] = -wp.y;
     }
   }
-#pragma unroll
-  for (int q = 0; q < KQ; ++q) {
-    const int k = threadIdx.x + 64 * (Q0 + q);
-    if (k <= M / 2) {
-      const int kp = M - k;
-      const Cplx<T> zk = Z[k % M], zp = Z[kp % M];
-      const Cplx<T> sk = cadd(zk, cconj(zp)), dk = csub(zk, cconj(zp));
-      const Cplx<T> wdk = cmul(Cplx<T>{T(a1[q].x), T(a1[q].y)}, dk);
-      const Cplx<T> Vk = {T(0.5) * (sk.x + wdk.y), T(0.5) * (sk.y - wdk.x)};
-      const Cplx<T> wk = cmul(Cplx<T>{T(a2[q].x), T(a2[q].y)}, Vk);
-      out(k, wk.x);
-      if (k > 0) out(N - k, -wk.y);
-      if (kp != k) {
-        const Cplx<T> sp = cadd(zp, cconj(zk)), dp = csub(zp, cconj(zk));
-        const Cplx<T> wdp = cmul(Cplx<T>{T(b1[q].x), T(b1[q].y)}, dp);
-        const Cplx<T> Vp = {T(0.5) * (sp.x + wdp.y), T(0.5) * (sp.y - wdp.x)};
-        const Cplx<T> wp = cmul(Cplx<T>{T(b2[q].x), T(b2[q].y)}, Vp);
-        out(kp, wp.x);
-        if (kp != M) out(N - kp, -wp.y);
-      }
-    }
+  if (l == 0) {
+    constexpr int k = M / 2;
+    const Cplx<T> zk = Z[k];
+    const Cplx<T> sk = cadd(zk, cconj(zk)), dk = csub(zk, cconj(zk));
+    const Cplx<T> wdk = cmul(Cplx<T>{T(ah.x), T(ah.y)}, dk);
+    const Cplx<T> Vk = {T(0.5) * (sk.x + wdk.y), T(0.5) * (sk.y - wdk.x)};
+    const Cplx<T> wk = cmul(Cplx<T>{T(bh.x), T(bh.y)}, Vk);
+    out[k] = wk.x;
+    out[N - k] = -wk.y;
   }
 }
 

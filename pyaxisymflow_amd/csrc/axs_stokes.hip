@@ -391,32 +391,38 @@ __global__ void __launch_bounds__(64) dct2_rows_m1024_kernel(
   if (TAPER) {
     left = row[w - 1];
     right = row[N - w];
-    rowf = (i == 0) ? T(0) : ((i >= nr - w) ? T(wr[i - (nr - w)]) : T(1));
+    if (i >= nr - w) rowf = T(wr[i - (nr - w)]);
   }
   const T rs = rscale ? T(rscale[i]) : T(1);
   T* out_t = (TAPER && tapered) ? tapered + cs + (int64_t)i * N : nullptr;
+  // Makhoul's reorder: element j goes to real slot j / 2 (j even) or
+  // N - 1 - (j - 1) / 2 (j odd) of the packed sequence; j = l + 64 q has the
+  // parity of l, so the slot is a per-lane base plus or minus 32 q.
+  T* zr = reinterpret_cast<T*>(buf);
+  const int d0 = (l & 1) ? N - 1 - (l >> 1) : (l >> 1);
+  const int dq = (l & 1) ? -32 : 32;
+  const bool wide = w > 64;  // the z taper reaches past the first/last 64 columns
 #pragma unroll
   for (int q = 0; q < N / 64; ++q) {
     const int j = l + 64 * q;
     T t = v[q];
     if (TAPER) {
-      if (j < w)
-        t = T(wz[j]) * left;
-      else if (j >= N - w)
-        t = T(wz[w + (j - (N - w))]) * right;
+      if (q == 0 || q == N / 64 - 1 || wide) {
+        if (j < w)
+          t = T(wz[j]) * left;
+        else if (j >= N - w)
+          t = T(wz[w + (j - (N - w))]) * right;
+      }
       if (i >= nr - w) t = rowf * t;
       if (i == 0) t = T(0);
       if (out_t) out_t[j] = t;
     }
-    packed_component(buf, dct_dst_index(j, N)) = rs * t;
+    zr[d0 + dq * q] = rs * t;
   }
   __syncthreads();
   fft1024_wave<T, -1>(buf, tw);
   T* out = dst + (int64_t)i * N;
-  auto put = [&](int k, T val) { out[k] = val; };
-  dct2_post_w64<T, 0, 3>(buf, e1, e2, put);
-  dct2_post_w64<T, 3, 3>(buf, e1, e2, put);
-  dct2_post_w64<T, 6, 3>(buf, e1, e2, put);
+  dct2_post_pairs<T>(buf, e1, e2, out);
 }
 
 template <typename T>
