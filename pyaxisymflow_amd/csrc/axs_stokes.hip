@@ -496,11 +496,13 @@ __global__ void __launch_bounds__(64) dct3_rows_m1024_kernel(
   fft1024_wave<T, +1>(buf, tw);
   const T invN = T(1) / T(N);
   T* out = dst + cs + (int64_t)i * N;
+  // output j = l + 64 q reads packed slot j / 2 or N - 1 - (j - 1) / 2 (the
+  // parity of j is that of l): a per-lane base plus or minus 32 q
+  const T* zr = reinterpret_cast<const T*>(buf);
+  const int d0 = (l & 1) ? N - 1 - (l >> 1) : (l >> 1);
+  const int dq = (l & 1) ? -32 : 32;
 #pragma unroll
-  for (int q = 0; q < N / 64; ++q) {
-    const int j = l + 64 * q;
-    out[j] = packed_component(buf, dct_dst_index(j, N)) * invN;
-  }
+  for (int q = 0; q < N / 64; ++q) out[l + 64 * q] = zr[d0 + dq * q] * invN;
 }
 
 // Column pass: for each z-mode k solve the tridiagonal system

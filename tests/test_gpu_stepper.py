@@ -77,6 +77,16 @@ def test_flow_past_sphere_100_steps(solver):
     assert abs(gpu.time() - cpu.t) <= 1e-12 * cpu.t
 
 
+def test_flow_past_sphere_dx_not_power_of_two():
+    """nz x nr = 240 x 120 on z in [0, 2]: dx = 1/120, so the fused step takes
+    its general quotient path (the power-of-two grids above take products)."""
+    case = sphere_case(nz=240, nr=120, seed=1)
+    cpu, gpu = run_pair(case, 40)
+    assert rel_l2(gpu.vorticity(), cpu.vorticity) < 1e-10
+    assert rel_l2(gpu.psi(), cpu.psi) < 1e-10
+    np.testing.assert_allclose(gpu.drag_history(), np.array(cpu.drag), rtol=1e-10, atol=1e-12)
+
+
 def test_graph_replay_matches_eager():
     """A captured step replayed 20 times equals 20 eagerly launched steps bit
     for bit (no memset nodes; every replay re-reads the device scalars)."""
