@@ -107,7 +107,7 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
       if ((e = upload(&tab, pivot_table(ha, hb, hc, hlam, nr, nz, L))) != hipSuccess) break;
       for (int K = 1; K <= 64; K *= 2) {
         if (K * P > 1024 || K * P < 64) continue;
-        if (sizeof(double) * (4 * K * P + 3 * nr) > 65536) continue;
+        if (sizeof(double) * (4 * K * P + 3 * tri_coef_stride(nr, L)) > 65536) continue;
         cseg = tab;
         tri_K = K;
         tri_L = L;
@@ -206,8 +206,11 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
   while (tri_K * 2 * tri_P <= 1024 && tri_K < 64 && nz / (tri_K * 2) >= 512) tri_K *= 2;
   if (tri_K < 2 && 2 * tri_P <= 1024) tri_K = 2;
   // keep the column kernel's LDS (scan buffers + coefficients) within 64 KiB
-  while (tri_K > 1 && sizeof(double) * (4 * tri_K * tri_P + 3 * nr) > 65536) tri_K /= 2;
-  if (sizeof(double) * (4 * tri_K * tri_P + 3 * nr) > 65536) return hipErrorInvalidValue;
+  while (tri_K > 1 &&
+         sizeof(double) * (4 * tri_K * tri_P + 3 * tri_coef_stride(nr, tri_L)) > 65536)
+    tri_K /= 2;
+  if (sizeof(double) * (4 * tri_K * tri_P + 3 * tri_coef_stride(nr, tri_L)) > 65536)
+    return hipErrorInvalidValue;
   std::vector<double> h_cseg = pivot_table(h_a, h_b, h_c, h_lam, nr, nz, tri_L);
   hipError_t e;
   if ((e = upload(&tw, h_tw)) != hipSuccess) return e;
@@ -536,17 +539,21 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
     const double* __restrict__ cseg, int nr, int nz, int K, int P) {
   extern __shared__ double sm[];
   d += (int64_t)blockIdx.y * nr * nz;  // case of a batch
+  // coefficient rows are padded by one double per L rows: the threads of a
+  // wave read rows p L + l of different segments p in the same instruction,
+  // and a stride of L doubles would put them all in one LDS bank
+  const int nrp = tri_coef_stride(nr, L);
   double* sA = sm;
-  double* sB = sm + nr;
-  double* sC = sm + 2 * nr;
-  double* sG0 = sm + 3 * nr;
+  double* sB = sm + nrp;
+  double* sC = sm + 2 * nrp;
+  double* sG0 = sm + 3 * nrp;
   double* sY0 = sG0 + K * P;
   double* sG1 = sY0 + K * P;
   double* sY1 = sG1 + K * P;
   for (int i = threadIdx.x; i < nr; i += blockDim.x) {
-    sA[i] = a[i];
-    sB[i] = b[i];
-    sC[i] = c[i];
+    sA[i + i / L] = a[i];
+    sB[i + i / L] = b[i];
+    sC[i + i / L] = c[i];
   }
   // XCD-aware chunk order: block b runs on XCD b % 8; give each XCD a
   // contiguous range of mode chunks.
@@ -577,11 +584,11 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
   for (int l = 0; l < L; ++l) {
     const int i = s + l;
     if (FULL || (active && i < nr)) {
-      const double ai = sA[i];
-      const double m = (sB[i] + lk) - ai * cp;
+      const double ai = sA[i + p];
+      const double m = (sB[i + p] + lk) - ai * cp;
       const double inv = recip_nr(m);
       im[l] = inv;
-      cp = sC[i] * inv;
+      cp = sC[i + p] * inv;
       y = (x[l] - ai * y) * inv;
       G = G * (-ai * inv);
       x[l] = y;
@@ -619,7 +626,7 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
     for (int l = 0; l < L; ++l) {
       const int i = s + l;
       if (FULL || (active && i < nr)) {
-        g = g * (-sA[i] * im[l]);
+        g = g * (-sA[i + p] * im[l]);
         x[l] = x[l] + g * Din;
       }
     }
@@ -630,7 +637,7 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
   for (int l = L - 1; l >= 0; --l) {
     const int i = s + l;
     if (FULL || (active && i < nr)) {
-      const double cpi = sC[i] * im[l];
+      const double cpi = sC[i + p] * im[l];
       z = x[l] - cpi * z;
       H = H * (-cpi);
       x[l] = z;
@@ -663,7 +670,7 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
     for (int l = L - 1; l >= 0; --l) {
       const int i = s + l;
       if (FULL || (active && i < nr)) {
-        h = h * (-(sC[i] * im[l]));
+        h = h * (-(sC[i + p] * im[l]));
         col[(int64_t)l * nz] = static_cast<T>(x[l] + h * Xin);
       }
     }
@@ -759,7 +766,7 @@ template <typename T>
 hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch) {
   const int K = pl.tri_K, P = pl.tri_P;
   const dim3 blocks(ceil_div(pl.nz, K), batch);
-  const size_t shm = sizeof(double) * (4 * K * P + 3 * pl.nr);
+  const size_t shm = sizeof(double) * (4 * K * P + 3 * tri_coef_stride(pl.nr, pl.tri_L));
   const bool full = (pl.nr % pl.tri_L == 0) && (pl.nz % K == 0);
 #define AXS_TRI(LL)                                                                              \
   (full ? (tridiag_columns_kernel<T, LL, true><<<blocks, K * P, shm, s>>>(                        \

@@ -8,6 +8,10 @@
 
 namespace axs {
 
+// Length of one tridiagonal coefficient array in LDS: nr rows padded by one
+// double per L rows.
+__host__ __device__ constexpr int tri_coef_stride(int nr, int L) { return nr + nr / L + 1; }
+
 struct StokesPlan {
   int nr = 0, nz = 0;
   double dx = 0.0;
