@@ -16,6 +16,7 @@
 #include "axs_stokes.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -80,10 +81,27 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
                                     const std::vector<double>& hc,
                                     const std::vector<double>& hlam) {
   static std::mutex mu;
-  static std::map<std::pair<int, int>, std::pair<int, int>> cache;
-  std::pair<int, int> best{tri_K, tri_L};
+  struct Choice {
+    int K, L;
+    bool G;
+  };
+  static std::map<std::pair<int, int>, Choice> cache;
+  Choice best{tri_K, tri_L, tri_G};
   bool cached = false;
-  {
+  // AXS_TRIDIAG_LK="L,K" pins the choice (the search times the candidates,
+  // so two processes may pick different ones when timings tie, and the
+  // radial solve's round-off depends on the choice)
+  if (const char* pin = std::getenv("AXS_TRIDIAG_LK")) {  // "L,K" or "L,K,G" (G = 1: global)
+    int L = 0, K = 0, G = 0;
+    const int got = std::sscanf(pin, "%d,%d,%d", &L, &K, &G);
+    if (got < 2 || (L != 4 && L != 8 && L != 16) || K < 1 || G < 0 || G > 1)
+      return hipErrorInvalidValue;
+    const int P = (nr + L - 1) / L;
+    if (K * P > 1024 || tri_lds_bytes(nr, L, K, P, G == 1) > 65536) return hipErrorInvalidValue;
+    best = {K, L, G == 1};
+    cached = true;
+  }
+  if (!cached) {
     std::lock_guard<std::mutex> g(mu);
     auto it = cache.find({nr, nz});
     if (it != cache.end()) {
@@ -100,18 +118,21 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
     float best_ms = 1e30f;
     double* saved = cseg;
     const int saved_k = tri_K, saved_l = tri_L, saved_p = tri_P;
+    const bool saved_g = tri_G;
     for (int L : {4, 8, 16}) {
       const int P = (nr + L - 1) / L;
       if (P > 512) continue;
       double* tab = nullptr;
       if ((e = upload(&tab, pivot_table(ha, hb, hc, hlam, nr, nz, L))) != hipSuccess) break;
-      for (int K = 1; K <= 64; K *= 2) {
+      for (int K = 1; K <= 64; K *= 2)
+      for (int G = 0; G < 2; ++G) {
         if (K * P > 1024 || K * P < 64) continue;
-        if (sizeof(double) * (4 * K * P + 3 * tri_coef_stride(nr, L)) > 65536) continue;
+        if (tri_lds_bytes(nr, L, K, P, G == 1) > 65536) continue;
         cseg = tab;
         tri_K = K;
         tri_L = L;
         tri_P = P;
+        tri_G = G == 1;
         float ms = 0.f;
         for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) e = launch_tridiag<double>(*this, spec, 0);
         if (e == hipSuccess) e = hipEventRecord(t0, 0);
@@ -122,7 +143,7 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
         if (e != hipSuccess) break;
         if (ms < best_ms) {
           best_ms = ms;
-          best = {K, L};
+          best = {K, L, G == 1};
         }
       }
       (void)hipFree(tab);
@@ -132,20 +153,22 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
     tri_K = saved_k;
     tri_L = saved_l;
     tri_P = saved_p;
+    tri_G = saved_g;
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     if (e != hipSuccess) return e;
     std::lock_guard<std::mutex> g(mu);
     cache[{nr, nz}] = best;
   }
-  if (best.first == tri_K && best.second == tri_L) return hipSuccess;
+  tri_G = best.G;
+  if (best.K == tri_K && best.L == tri_L) return hipSuccess;
   double* tab = nullptr;
-  hipError_t e = upload(&tab, pivot_table(ha, hb, hc, hlam, nr, nz, best.second));
+  hipError_t e = upload(&tab, pivot_table(ha, hb, hc, hlam, nr, nz, best.L));
   if (e != hipSuccess) return e;
   (void)hipFree(cseg);
   cseg = tab;
-  tri_K = best.first;
-  tri_L = best.second;
+  tri_K = best.K;
+  tri_L = best.L;
   tri_P = (nr + tri_L - 1) / tri_L;
   return hipSuccess;
 }
@@ -206,11 +229,9 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
   while (tri_K * 2 * tri_P <= 1024 && tri_K < 64 && nz / (tri_K * 2) >= 512) tri_K *= 2;
   if (tri_K < 2 && 2 * tri_P <= 1024) tri_K = 2;
   // keep the column kernel's LDS (scan buffers + coefficients) within 64 KiB
-  while (tri_K > 1 &&
-         sizeof(double) * (4 * tri_K * tri_P + 3 * tri_coef_stride(nr, tri_L)) > 65536)
-    tri_K /= 2;
-  if (sizeof(double) * (4 * tri_K * tri_P + 3 * tri_coef_stride(nr, tri_L)) > 65536)
-    return hipErrorInvalidValue;
+  tri_G = tri_lds_bytes(nr, tri_L, tri_K, tri_P, false) > 65536;
+  while (tri_K > 1 && tri_lds_bytes(nr, tri_L, tri_K, tri_P, tri_G) > 65536) tri_K /= 2;
+  if (tri_lds_bytes(nr, tri_L, tri_K, tri_P, tri_G) > 65536) return hipErrorInvalidValue;
   std::vector<double> h_cseg = pivot_table(h_a, h_b, h_c, h_lam, nr, nz, tri_L);
   hipError_t e;
   if ((e = upload(&tw, h_tw)) != hipSuccess) return e;
@@ -224,7 +245,8 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
   if ((e = upload(&cseg, h_cseg)) != hipSuccess) return e;
   e = hipMalloc(reinterpret_cast<void**>(&spec), sizeof(double) * (size_t)nr * nz);
   if (e != hipSuccess) return e;
-  if ((size_t)nr * nz >= (size_t(1) << 18) && !std::getenv("AXS_NO_TUNE"))
+  if (std::getenv("AXS_TRIDIAG_LK") ||
+      ((size_t)nr * nz >= (size_t(1) << 18) && !std::getenv("AXS_NO_TUNE")))
     return tune_tridiag(h_a, h_b, h_c, h_lam);
   return hipSuccess;
 }
@@ -531,8 +553,11 @@ __device__ __forceinline__ double recip_nr(double m) {
 }
 
 // FULL: nr = P L and every chunk holds K modes, so no thread or row is idle
-// and no bounds test is needed.
-template <typename T, int L, bool FULL>
+// and no bounds test is needed.  CG: the coefficients are read from global
+// memory (every mode reads the same rows, so they stay in L1/L2) instead of
+// being staged in LDS, which leaves the LDS to the scans and allows more
+// modes per workgroup on tall grids.
+template <typename T, int L, bool FULL, bool CG>
 __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
     T* __restrict__ d, const double* __restrict__ a, const double* __restrict__ b,
     const double* __restrict__ c, const double* __restrict__ lam,
@@ -542,7 +567,7 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
   // coefficient rows are padded by one double per L rows: the threads of a
   // wave read rows p L + l of different segments p in the same instruction,
   // and a stride of L doubles would put them all in one LDS bank
-  const int nrp = tri_coef_stride(nr, L);
+  const int nrp = CG ? 0 : tri_coef_stride(nr, L);
   double* sA = sm;
   double* sB = sm + nrp;
   double* sC = sm + 2 * nrp;
@@ -550,10 +575,12 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
   double* sY0 = sG0 + K * P;
   double* sG1 = sY0 + K * P;
   double* sY1 = sG1 + K * P;
-  for (int i = threadIdx.x; i < nr; i += blockDim.x) {
-    sA[i + i / L] = a[i];
-    sB[i + i / L] = b[i];
-    sC[i + i / L] = c[i];
+  if constexpr (!CG) {
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+      sA[i + i / L] = a[i];
+      sB[i + i / L] = b[i];
+      sC[i + i / L] = c[i];
+    }
   }
   // XCD-aware chunk order: block b runs on XCD b % 8; give each XCD a
   // contiguous range of mode chunks.
@@ -567,6 +594,16 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
   const int s = p * L;
   const int kc = active ? k : 0;
   T* col = d + ((int64_t)s * nz + kc);  // row s of this thread's mode
+  // coefficients of row i = s + l (padded LDS index i + p)
+  auto ca = [&](int i) -> double {
+    if constexpr (CG) return a[i]; else return sA[i + p];
+  };
+  auto cb = [&](int i) -> double {
+    if constexpr (CG) return b[i]; else return sB[i + p];
+  };
+  auto cc = [&](int i) -> double {
+    if constexpr (CG) return c[i]; else return sC[i + p];
+  };
 
   double x[L];
   double im[L];
@@ -584,11 +621,11 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
   for (int l = 0; l < L; ++l) {
     const int i = s + l;
     if (FULL || (active && i < nr)) {
-      const double ai = sA[i + p];
-      const double m = (sB[i + p] + lk) - ai * cp;
+      const double ai = ca(i);
+      const double m = (cb(i) + lk) - ai * cp;
       const double inv = recip_nr(m);
       im[l] = inv;
-      cp = sC[i + p] * inv;
+      cp = cc(i) * inv;
       y = (x[l] - ai * y) * inv;
       G = G * (-ai * inv);
       x[l] = y;
@@ -626,7 +663,7 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
     for (int l = 0; l < L; ++l) {
       const int i = s + l;
       if (FULL || (active && i < nr)) {
-        g = g * (-sA[i + p] * im[l]);
+        g = g * (-ca(i) * im[l]);
         x[l] = x[l] + g * Din;
       }
     }
@@ -637,7 +674,7 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
   for (int l = L - 1; l >= 0; --l) {
     const int i = s + l;
     if (FULL || (active && i < nr)) {
-      const double cpi = sC[i + p] * im[l];
+      const double cpi = cc(i) * im[l];
       z = x[l] - cpi * z;
       H = H * (-cpi);
       x[l] = z;
@@ -670,7 +707,7 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
     for (int l = L - 1; l >= 0; --l) {
       const int i = s + l;
       if (FULL || (active && i < nr)) {
-        h = h * (-(sC[i + p] * im[l]));
+        h = h * (-(cc(i) * im[l]));
         col[(int64_t)l * nz] = static_cast<T>(x[l] + h * Xin);
       }
     }
@@ -766,15 +803,23 @@ template <typename T>
 hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch) {
   const int K = pl.tri_K, P = pl.tri_P;
   const dim3 blocks(ceil_div(pl.nz, K), batch);
-  const size_t shm = sizeof(double) * (4 * K * P + 3 * tri_coef_stride(pl.nr, pl.tri_L));
   const bool full = (pl.nr % pl.tri_L == 0) && (pl.nz % K == 0);
+  const bool cg = pl.tri_G;
+  const size_t shm = tri_lds_bytes(pl.nr, pl.tri_L, K, P, cg);
+#define AXS_TRI_ONE(LL, F, G)                                                                    \
+  tridiag_columns_kernel<T, LL, F, G><<<blocks, K * P, shm, s>>>(d, pl.a, pl.b, pl.c, pl.lam,    \
+                                                                 pl.cseg, pl.nr, pl.nz, K, P)
 #define AXS_TRI(LL)                                                                              \
-  (full ? (tridiag_columns_kernel<T, LL, true><<<blocks, K * P, shm, s>>>(                        \
-               d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P),                          \
-           0)                                                                                     \
-        : (tridiag_columns_kernel<T, LL, false><<<blocks, K * P, shm, s>>>(                       \
-               d, pl.a, pl.b, pl.c, pl.lam, pl.cseg, pl.nr, pl.nz, K, P),                          \
-           0))
+  do {                                                                                           \
+    if (full && cg)                                                                              \
+      AXS_TRI_ONE(LL, true, true);                                                               \
+    else if (full)                                                                               \
+      AXS_TRI_ONE(LL, true, false);                                                              \
+    else if (cg)                                                                                 \
+      AXS_TRI_ONE(LL, false, true);                                                              \
+    else                                                                                         \
+      AXS_TRI_ONE(LL, false, false);                                                             \
+  } while (0)
   switch (pl.tri_L) {
     case 4:
       AXS_TRI(4);
@@ -791,6 +836,7 @@ hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch) 
     default:
       return hipErrorInvalidValue;
   }
+#undef AXS_TRI_ONE
 #undef AXS_TRI
   return hipGetLastError();
 }

@@ -11,6 +11,11 @@ namespace axs {
 // Length of one tridiagonal coefficient array in LDS: nr rows padded by one
 // double per L rows.
 __host__ __device__ constexpr int tri_coef_stride(int nr, int L) { return nr + nr / L + 1; }
+// LDS bytes of one tridiagonal workgroup: four scan buffers of K P doubles,
+// plus the three coefficient arrays unless they are read from global memory.
+inline size_t tri_lds_bytes(int nr, int L, int K, int P, bool coef_global) {
+  return sizeof(double) * (4 * (size_t)K * P + (coef_global ? 0 : 3 * (size_t)tri_coef_stride(nr, L)));
+}
 
 struct StokesPlan {
   int nr = 0, nz = 0;
@@ -28,6 +33,7 @@ struct StokesPlan {
   double* cseg = nullptr;      // c'_{pL-1} per segment p and mode k, (P, nz)
   double* spec = nullptr;      // spectral work buffer (nr, nz)
   int tri_L = 16, tri_P = 1, tri_K = 1;
+  bool tri_G = false;  // tridiagonal coefficients read from global memory, not LDS
 
   hipError_t init(int nr, int nz, double dx);
   hipError_t tune_tridiag(const std::vector<double>& a, const std::vector<double>& b,

@@ -87,6 +87,34 @@ def test_flow_past_sphere_dx_not_power_of_two():
     np.testing.assert_allclose(gpu.drag_history(), np.array(cpu.drag), rtol=1e-10, atol=1e-12)
 
 
+def test_pinned_radial_solve_configuration(monkeypatch):
+    """AXS_TRIDIAG_LK pins the tridiagonal pass's segment length and modes per
+    workgroup: runs with the same pin are bit-identical, every pin meets the
+    FP64 bar against the oracle, and a pin the kernel cannot run is refused."""
+    from pyaxisymflow_amd.stepper import RigidBodyStepper
+
+    case = sphere_case(seed=2)
+    keys = ("nz", "nr", "z_len", "nu", "brink_lam", "char_func")
+    kw = dict(U_0=case["U_0"], T_ramp=case["T_ramp"], r_sph=case["r_sph"],
+              vorticity=case["vorticity"])
+    cpu = RigidBodyCase(*(case[k] for k in keys), solver="exact", **kw)
+    for _ in range(20):
+        cpu.step()
+    seen = {}
+    for pin in ("8,2", "16,4", "4,1", "8,2"):
+        monkeypatch.setenv("AXS_TRIDIAG_LK", pin)
+        gpu = RigidBodyStepper(*(case[k] for k in keys), **kw)
+        gpu.step(20)
+        w = gpu.vorticity()
+        assert rel_l2(w, cpu.vorticity) < 1e-10, pin
+        if pin in seen:
+            assert np.array_equal(seen[pin], w)
+        seen[pin] = w
+    monkeypatch.setenv("AXS_TRIDIAG_LK", "8,4096")  # 4096 x 16 threads per workgroup
+    with pytest.raises(RuntimeError):
+        RigidBodyStepper(*(case[k] for k in keys), **kw)
+
+
 def test_graph_replay_matches_eager():
     """A captured step replayed 20 times equals 20 eagerly launched steps bit
     for bit (no memset nodes; every replay re-reads the device scalars)."""
