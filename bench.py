@@ -194,6 +194,59 @@ def time_batched(steps, warmup, seeds=range(64)):
     )
 
 
+def time_precision(grid_size_z=1024, no_cycles=20):
+    """The precision form: an oscillating sphere in quiescent fluid (steady
+    streaming; spheroid_streaming.py with aspect ratio 1) at nz x nr =
+    1024 x 512, radius 0.1, amplitude 0.01, Womersley number 10, 20 periods.
+    FP32 fields, stencils and z-transforms with the radial solve and force
+    reduction in FP64, against the FP64 path over the same steps (the FP64
+    path meets the oracle within 1e-10 in the test suite; the oracle itself
+    would need ~an hour per run at this size)."""
+    import torch
+
+    from pyaxisymflow_amd.stepper import OscillatingSpheroidStepper
+
+    def make(prec):
+        return OscillatingSpheroidStepper(grid_size_z, womersley_square=100.0, radius=0.1,
+                                          spheroid_AR=1.0, freq=16, no_cycles=no_cycles,
+                                          precision=prec)
+
+    # steps to the end time (the dt rule depends on the flow, so count them)
+    probe = make("f64")
+    while probe.time() < probe.tEnd * (1 - 1e-13):
+        left = probe.tEnd - probe.time()
+        probe.step(64 if left > 80 * probe.dt() else 1)
+    n = probe.steps_done()
+    del probe
+    out = {}
+    runs = {}
+    for prec in ("f64", "f32"):
+        st = make(prec)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st.stream)
+        st.step(n)
+        e1.record(st.stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        out[prec] = round(st.nr * st.nz * n / (ms / 1e3) / 1e6, 2)
+        runs[prec] = st
+    a, b = runs["f64"], runs["f32"]
+    rel = lambda x, y: float(np.linalg.norm(x - y) / np.linalg.norm(y))
+    ua, ub = a.velocity(), b.velocity()
+    fa, fb = a.drag_history(), b.drag_history()
+    return dict(
+        value=out["f32"], unit="Mcell-steps/s", fp64_value=out["f64"],
+        workload="oscillating sphere, steady streaming, 1024x512, 20 periods",
+        steps=n, sim_time=b.time(),
+        u_rel_l2_vs_f64=rel(np.hypot(*ub), np.hypot(*ua)),
+        psi_rel_l2_vs_f64=rel(b.psi(), a.psi()),
+        psi_avg_rel_l2_vs_f64=rel(b.avg_psi(), a.avg_psi()),
+        drag_max_rel_err_vs_f64=float(np.abs(fb - fa).max() / np.abs(fa).max()),
+    )
+
+
 def time_serving(n_requests, steps, max_active=16, seed=0):
     """The serving form: soft-sphere requests (pyaxisymflow_amd.serve:
     nr lognormal around 256 in [128, 1024], 5-smooth; G in [0.1, 10]).  A
@@ -281,6 +334,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-steps", type=int, default=2000)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every case is independent and this build runs on "
@@ -341,6 +395,8 @@ def main():
             steps=args.large_steps, finite=finite2,
             stage_us={k: round(v, 2) for k, v in st2.items()},
             note="same workload generator at nz x nr = 4096 x 2048 (z in [0,2], r in [0,1])")
+    if not args.no_precision:
+        forms["precision_fp32"] = time_precision()
     if not args.no_batched:
         forms["batched_fp32"] = time_batched(args.batched_steps, 2)
     if not args.no_serve:
