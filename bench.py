@@ -145,52 +145,69 @@ def time_gpu(nz, nr, steps, warmup, stage_reps):
 def time_batched(steps, warmup, seeds=range(64)):
     """The batched form: the 64-case set of pyaxisymflow_amd.workloads
     (grids nr in {128, 192, 256, 384, 512}, nz = 2 nr; sphere, prolate
-    spheroid and torus bodies; Re in [10, 500]), one RigidBatchStepper per
-    grid, all groups in flight at once on their own streams.  FP32 is timed;
-    the same cases in FP64 give the drag error."""
+    spheroid and torus bodies; Re in [10, 500]) as one UnevenBatchStepper:
+    every pass is one launch over all 64 cases, 16 steps per graph launch.
+    FP32 is timed; the same cases in FP64 give the drag error.  The grouped
+    layout (one batched stepper per grid, each on its own stream) is timed
+    beside it."""
     import torch
 
     from pyaxisymflow_amd import workloads
-    from pyaxisymflow_amd.stepper import RigidBatchStepper
+    from pyaxisymflow_amd.stepper import RigidBatchStepper, UnevenBatchStepper
 
     groups = workloads.grouped_cases(seeds)
-    built = []
+    hist = steps + warmup + 4
+    cases = []
     for (nz, nr), specs in groups.items():
-        args = [workloads.rigid_case_args(sp, workloads.case_inputs(sp, nz, nr)[1]) for sp in specs]
-        hist = steps + warmup + 4
-        built.append(((nz, nr), specs,
-                      RigidBatchStepper(nz, nr, 1.0, args, r_len=0.5, precision="f32",
-                                        history=hist),
-                      RigidBatchStepper(nz, nr, 1.0, args, r_len=0.5, precision="f64",
-                                        history=hist)))
-    for _, _, b32, _ in built:
-        b32.step(warmup)
+        for sp in specs:
+            a = workloads.rigid_case_args(sp, workloads.case_inputs(sp, nz, nr)[1])
+            a.update(nz=nz, nr=nr, z_len=1.0, r_len=0.5)
+            cases.append(a)
+    b32 = UnevenBatchStepper(cases, precision="f32", history=hist)
+    b64 = UnevenBatchStepper(cases, precision="f64", history=hist)
+    b32.step(warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _, _, b32, _ in built:
-        b32.step(steps)
+    b32.step(steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    for _, _, _, b64 in built:
-        b64.step(warmup + steps)
+    b64.step(warmup + steps)
     torch.cuda.synchronize()
-    cells = sum(len(sp) * nz * nr for (nz, nr), sp, _, _ in built)
     err = 0.0
-    for _, specs, b32, b64 in built:
-        for q in range(len(specs)):
-            d32, d64 = b32.drag_history(q), b64.drag_history(q)
-            err = max(err, float(np.max(np.abs(d32 - d64)) / np.max(np.abs(d64))))
+    for q in range(len(cases)):
+        d32, d64 = b32.drag_history(q), b64.drag_history(q)
+        err = max(err, float(np.max(np.abs(d32 - d64)) / np.max(np.abs(d64))))
+    cells = sum(c["nz"] * c["nr"] for c in cases)
+    del b64
+    # the grouped layout, for comparison
+    grouped = []
+    for (nz, nr), specs in groups.items():
+        args = [workloads.rigid_case_args(sp, workloads.case_inputs(sp, nz, nr)[1])
+                for sp in specs]
+        grouped.append(RigidBatchStepper(nz, nr, 1.0, args, r_len=0.5, precision="f32",
+                                         history=hist))
+    for g in grouped:
+        g.step(warmup)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for g in grouped:
+        g.step(steps)
+    torch.cuda.synchronize()
+    wall_g = time.perf_counter() - t1
     return dict(
         value=round(cells * steps / wall / 1e6, 2),
         unit="Mcell-steps/s",
         dtype="f32 fields/stencils/z-transforms; f64 radial solve, reductions, scalars",
-        cases=sum(len(sp) for _, sp, _, _ in built),
-        groups={f"{nz}x{nr}": len(sp) for (nz, nr), sp, _, _ in built},
+        layout="one launch per pass over all cases (uneven grids)",
+        cases=len(cases),
+        groups={f"{nz}x{nr}": len(sp) for (nz, nr), sp in groups.items()},
         cells=cells,
         steps=steps,
         warmup=warmup,
         wall_s=round(wall, 4),
         drag_max_rel_err_vs_f64=err,
+        grouped_value=round(cells * steps / wall_g / 1e6, 2),
+        grouped_note="one batched stepper per grid, each on its own stream",
     )
 
 

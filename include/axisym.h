@@ -223,17 +223,25 @@ typedef struct {
 } axs_rigid_buffers;
 axs_status axs_rigid_stepper_create(const axs_rigid_config* cfg, const axs_rigid_buffers* bufs,
                                     axs_rigid_stepper** out);
-/* The batched form: `batch` cases of one grid stepped together by the same
- * launches (the case index is a grid dimension of every kernel).  cfgs[batch]
- * share nr, nz, kill_width, history, precision, dx and the moving_body /
- * average_psi flags, and may differ in everything else (viscosity, body
+/* The batched form: `batch` cases stepped together by the same launches
+ * (the case index is a grid dimension of every kernel).  cfgs[batch] share
+ * kill_width, history, precision and the moving_body / average_psi flags;
+ * on one grid they share nr, nz and dx and may differ in everything else
+ * (viscosity, body
  * mask, ramp, dt rule, force scale).  Per-case buffers hold the cases back
  * to back: fields batch * nr * nz elements, scalars batch *
  * AXS_RIGID_SCALARS, drag batch * history; r, z, kill_wz and kill_wr are
  * shared.  Each case advances its own t and dt.  The reference steps one
  * case per process (flow_past_sphere.py:107-188); this runs that loop body
  * for many cases at once.  axs_rigid_stepper_create is batch_create with
- * batch = 1. */
+ * batch = 1.
+ *
+ * Uneven batches: cases may also differ in nr, nz and dx when none has a
+ * moving body, every nz <= 1024, and the cases come in at most 16 runs of
+ * one grid.  Every pass is then one launch over all cases.  Fields hold
+ * case q's nr_q * nz_q elements after those of cases 0..q-1; r holds each
+ * case's nr_q radii back to back, kill_wz each case's 2 kill_width
+ * weights, kill_wr each case's kill_width weights (z is unused). */
 axs_status axs_rigid_batch_create(const axs_rigid_config* cfgs, int batch,
                                   const axs_rigid_buffers* bufs, axs_rigid_stepper** out);
 axs_status axs_rigid_stepper_destroy(axs_rigid_stepper* st);

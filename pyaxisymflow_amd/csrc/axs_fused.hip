@@ -129,23 +129,20 @@ __device__ __forceinline__ T eno3_face_b(T fm2, T fm1, T f0, T fp1, T um2, T um1
 constexpr int kVTZ = 64, kVTR = 16;                  // tile
 constexpr int kVPZ = kVTZ + 4, kVPR = kVTR + 4;      // psi tile (halo 2)
 
+// Tile (bx, by) of one case (psi, sc, slots at the case); the maximum goes
+// to slot `slot`.
 template <typename T>
-__global__ void __launch_bounds__(256) velocity_max_kernel(const T* __restrict__ psi,
-                                                           const double* __restrict__ r, int nr,
-                                                           int nz, T dx,
-                                                           const double* __restrict__ sc,
-                                                           double* __restrict__ slots,
-                                                           const RigidConfig* __restrict__ cfgs) {
+__device__ __forceinline__ void velocity_max_tile(const T* __restrict__ psi,
+                                                  const double* __restrict__ r, int nr, int nz,
+                                                  T dx, const double* __restrict__ sc,
+                                                  double* __restrict__ slots,
+                                                  const RigidConfig& cfg, int bx, int by,
+                                                  int slot) {
   __shared__ T sp[kVPR][kVPZ];
   __shared__ T sri[kVTR], sinv[kVTR];
   __shared__ double wm[4];
   const int tid = threadIdx.x;
-  const int cb = blockIdx.z;  // case of a batch
-  const RigidConfig& cfg = cfgs[cb];
-  psi += (int64_t)cb * nr * nz;
-  sc += (int64_t)cb * AXS_RIGID_SCALARS;
-  slots += (int64_t)cb * kSlotsPerCase;
-  const int j0 = blockIdx.x * kVTZ, i0 = blockIdx.y * kVTR;
+  const int j0 = bx * kVTZ, i0 = by * kVTR;
   constexpr int NP = (kVPR * kVPZ + 255) / 256;
   T v[NP];
 #pragma unroll
@@ -202,10 +199,36 @@ __global__ void __launch_bounds__(256) velocity_max_kernel(const T* __restrict__
   __syncthreads();
   if (tid == 0) {
     m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
-    const int slot = (blockIdx.y * gridDim.x + blockIdx.x) % kMaxSlots;
     atomicMax(reinterpret_cast<unsigned long long*>(slots + slot * kSlotStride),
               static_cast<unsigned long long>(__double_as_longlong(m)));
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) velocity_max_kernel(const T* __restrict__ psi,
+                                                           const double* __restrict__ r, int nr,
+                                                           int nz, T dx,
+                                                           const double* __restrict__ sc,
+                                                           double* __restrict__ slots,
+                                                           const RigidConfig* __restrict__ cfgs) {
+  const int cb = blockIdx.z;  // case of a batch
+  velocity_max_tile<T>(psi + (int64_t)cb * nr * nz, r, nr, nz, dx,
+                       sc + (int64_t)cb * AXS_RIGID_SCALARS, slots + (int64_t)cb * kSlotsPerCase,
+                       cfgs[cb], blockIdx.x, blockIdx.y,
+                       (blockIdx.y * gridDim.x + blockIdx.x) % kMaxSlots);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) velocity_max_ragged_kernel(
+    const T* __restrict__ psi, const double* __restrict__ sc, double* __restrict__ slots,
+    const RigidConfig* __restrict__ cfgs, const RaggedGroup* __restrict__ grp, int ngrp) {
+  const int b = blockIdx.x;
+  const RaggedGroup& G = grp[ragged_group<&RaggedGroup::vt0>(grp, ngrp, b)];
+  const int q = b - G.vt0, cl = q / G.vtiles, t = q % G.vtiles;
+  const int cb = G.case0 + cl;
+  velocity_max_tile<T>(psi + G.cell0 + (int64_t)cl * G.nr * G.nz, G.r, G.nr, G.nz, T(G.dx),
+                       sc + (int64_t)cb * AXS_RIGID_SCALARS, slots + (int64_t)cb * kSlotsPerCase,
+                       cfgs[cb], t % G.vz, t / G.vz, t % kMaxSlots);
 }
 
 // ------------------------------------------------------------ step_fused
@@ -635,6 +658,45 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) s
                               partials, cfg, L, i0, j0, tile);
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+step_fused_ragged_kernel(const T* __restrict__ psi, const T* __restrict__ chi,
+                         const T* __restrict__ wsrc, T* __restrict__ wdst, T* __restrict__ avg,
+                         int kw, double* __restrict__ sc, const double* __restrict__ slots,
+                         double* __restrict__ partials, const RigidConfig* __restrict__ cfgs,
+                         const RaggedGroup* __restrict__ grp, int ngrp) {
+  __shared__ T bufA[kBufA], bufC[kBufC], bufD[kBufD];
+  __shared__ T sri[kR6R], sinv[kR6R], sinv2[kR6R];
+  __shared__ double srd[kR6R];
+  __shared__ double wpart[4];
+  __shared__ double smax[4];
+  // the same XCD-aware order as the uniform launch, over all tiles of all cases
+  const int nb = gridDim.x;
+  int b = blockIdx.x;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  const RaggedGroup& G = grp[ragged_group<&RaggedGroup::ft0>(grp, ngrp, b)];
+  const int q = b - G.ft0, cl = q / G.ftiles, tile = q % G.ftiles;
+  const int cb = G.case0 + cl;
+  const int nr = G.nr, nz = G.nz;
+  const int64_t fo = G.cell0 + (int64_t)cl * nr * nz;
+  const FusedLds<T> L{bufA, bufC, bufD, sri, sinv, sinv2, srd, wpart, smax};
+  const int j0 = (tile % G.fz) * kFZ, i0 = (tile / G.fz) * kFR;
+  const int m = kw > 1 ? kw : 1;
+  const bool interior = i0 - 6 >= m && i0 + kFR + 6 <= nr - m && j0 - 6 >= m &&
+                        j0 + kFZ + 6 <= nz - m;
+  double* part = partials + G.part0 + (int64_t)cl * G.ftiles;
+  if (interior)
+    step_fused_tile<T, true>(psi + fo, chi + fo, wsrc + fo, wdst + fo, avg ? avg + fo : nullptr,
+                             G.r, G.kill_wz, G.kill_wr, kw, nr, nz, T(G.dx),
+                             sc + (int64_t)cb * AXS_RIGID_SCALARS,
+                             slots + (int64_t)cb * kSlotsPerCase, part, cfgs[cb], L, i0, j0, tile);
+  else
+    step_fused_tile<T, false>(psi + fo, chi + fo, wsrc + fo, wdst + fo, avg ? avg + fo : nullptr,
+                              G.r, G.kill_wz, G.kill_wr, kw, nr, nz, T(G.dx),
+                              sc + (int64_t)cb * AXS_RIGID_SCALARS,
+                              slots + (int64_t)cb * kSlotsPerCase, part, cfgs[cb], L, i0, j0, tile);
+}
+
 // ------------------------------------------------------------ velocity_out
 
 // u_pen of the last step from psi and chi with that step's dt, ramp factor
@@ -684,6 +746,19 @@ __global__ void __launch_bounds__(1024) finalize_step_kernel(const double* __res
                    drag ? drag + (int64_t)cb * cfg.history : nullptr, cfg, red);
 }
 
+__global__ void __launch_bounds__(1024) finalize_ragged_kernel(
+    const double* __restrict__ partials, double* __restrict__ slots, double* __restrict__ sc,
+    double* __restrict__ drag, const RigidConfig* __restrict__ cfgs,
+    const RaggedGroup* __restrict__ grp, int ngrp) {
+  __shared__ double red[1024];
+  const int cb = blockIdx.x;  // one block per case
+  const RaggedGroup& G = grp[ragged_group<&RaggedGroup::case0>(grp, ngrp, cb)];
+  const RigidConfig& cfg = cfgs[cb];
+  finalize_pending(partials + G.part0 + (int64_t)(cb - G.case0) * G.ftiles, G.ftiles,
+                   slots + (int64_t)cb * kSlotsPerCase, sc + (int64_t)cb * AXS_RIGID_SCALARS,
+                   drag ? drag + (int64_t)cb * cfg.history : nullptr, cfg, red);
+}
+
 // ------------------------------------------------------------ launchers
 
 template <typename T>
@@ -720,6 +795,35 @@ hipError_t launch_finalize_step(const double* partials, int count, double* slots
   return hipGetLastError();
 }
 
+int fused_tiles_z(int nz) { return (int)ceil_div(nz, kFZ); }
+int vmax_tiles_z(int nz) { return (int)ceil_div(nz, kVTZ); }
+int vmax_tiles(int nr, int nz) { return (int)(ceil_div(nz, kVTZ) * ceil_div(nr, kVTR)); }
+
+template <typename T>
+hipError_t launch_velocity_max_ragged(const T* psi, const double* sc, double* slots,
+                                      const RigidConfig* cfgs, const RaggedGroup* grp, int ngrp,
+                                      int tiles, hipStream_t s) {
+  velocity_max_ragged_kernel<T><<<tiles, 256, 0, s>>>(psi, sc, slots, cfgs, grp, ngrp);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_step_fused_ragged(const T* psi, const T* chi, const T* wsrc, T* wdst, T* avg,
+                                    int kw, double* sc, const double* slots, double* partials,
+                                    const RigidConfig* cfgs, const RaggedGroup* grp, int ngrp,
+                                    int tiles, hipStream_t s) {
+  step_fused_ragged_kernel<T><<<tiles, 256, 0, s>>>(psi, chi, wsrc, wdst, avg, kw, sc, slots,
+                                                    partials, cfgs, grp, ngrp);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_ragged(const double* partials, double* slots, double* sc, double* drag,
+                                  const RigidConfig* cfgs, const RaggedGroup* grp, int ngrp,
+                                  int ncases, hipStream_t s) {
+  finalize_ragged_kernel<<<ncases, 1024, 0, s>>>(partials, slots, sc, drag, cfgs, grp, ngrp);
+  return hipGetLastError();
+}
+
 #define AXS_FUSED_INST(T)                                                                     \
   template hipError_t launch_velocity_max<T>(const T*, const double*, int, int, T,          \
                                              const double*, double*, const RigidConfig*, int, \
@@ -733,6 +837,20 @@ hipError_t launch_finalize_step(const double* partials, int count, double* slots
                                              hipStream_t);
 AXS_FUSED_INST(double)
 AXS_FUSED_INST(float)
+template hipError_t launch_velocity_max_ragged<double>(const double*, const double*, double*,
+                                                       const RigidConfig*, const RaggedGroup*,
+                                                       int, int, hipStream_t);
+template hipError_t launch_velocity_max_ragged<float>(const float*, const double*, double*,
+                                                      const RigidConfig*, const RaggedGroup*, int,
+                                                      int, hipStream_t);
+template hipError_t launch_step_fused_ragged<double>(const double*, const double*, const double*,
+                                                     double*, double*, int, double*,
+                                                     const double*, double*, const RigidConfig*,
+                                                     const RaggedGroup*, int, int, hipStream_t);
+template hipError_t launch_step_fused_ragged<float>(const float*, const float*, const float*,
+                                                    float*, float*, int, double*, const double*,
+                                                    double*, const RigidConfig*,
+                                                    const RaggedGroup*, int, int, hipStream_t);
 #undef AXS_FUSED_INST
 
 }  // namespace axs

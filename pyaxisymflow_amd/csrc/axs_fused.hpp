@@ -147,6 +147,24 @@ template <typename T>
 hipError_t launch_velocity_out(const T* psi, const T* chi, T* pz, T* pr, const double* r, int nr,
                                int nz, T dx, const double* sc, const RigidConfig* cfgs, int batch,
                                hipStream_t s);
+// Uneven batches (axs_ragged.hpp): tiles of every case of every group in
+// one launch; per-case scalars, slots and configurations at the global case
+// index, force partials at part0 + (case - case0) * ftiles of the group.
+int fused_tiles_z(int nz);
+int vmax_tiles_z(int nz);
+int vmax_tiles(int nr, int nz);
+template <typename T>
+hipError_t launch_velocity_max_ragged(const T* psi, const double* sc, double* slots,
+                                      const RigidConfig* cfgs, const RaggedGroup* grp, int ngrp,
+                                      int tiles, hipStream_t s);
+template <typename T>
+hipError_t launch_step_fused_ragged(const T* psi, const T* chi, const T* wsrc, T* wdst, T* avg,
+                                    int kw, double* sc, const double* slots, double* partials,
+                                    const RigidConfig* cfgs, const RaggedGroup* grp, int ngrp,
+                                    int tiles, hipStream_t s);
+hipError_t launch_finalize_ragged(const double* partials, double* slots, double* sc, double* drag,
+                                  const RigidConfig* cfgs, const RaggedGroup* grp, int ngrp,
+                                  int ncases, hipStream_t s);
 // One block: force sum of the partials, then t, it, period timer, the next
 // step's scalars, and cleared maximum slots.
 hipError_t launch_finalize_step(const double* partials, int count, double* slots, double* sc,

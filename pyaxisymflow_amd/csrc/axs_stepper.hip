@@ -33,6 +33,7 @@
 #include "axs_fused.hpp"
 #include "axs_stepper.hpp"
 
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -289,8 +290,194 @@ hipError_t RigidStepperBase::run(int nsteps, bool use_graph, hipStream_t s) {
   return settle(s);
 }
 
+// ------------------------------------------------------------ uneven batches
+
+static bool same_grid(const RigidConfig& a, const RigidConfig& b) {
+  return a.nr == b.nr && a.nz == b.nz && a.dx == b.dx;
+}
+
+// Radix plan of the half-length transform (radices 4, 2, 3, 5); false if
+// nz / 2 has another prime factor.
+static bool fft_args(int nz, FftPlanArgs& fp) {
+  fp = FftPlanArgs{};
+  fp.M = nz / 2;
+  int m = fp.M;
+  for (int rdx : {4, 2, 3, 5})
+    while (m % rdx == 0 && fp.nstages < kMaxFftStages) {
+      fp.radix[fp.nstages++] = rdx;
+      m /= rdx;
+    }
+  return m == 1;
+}
+
+template <typename T>
+hipError_t RaggedStepper<T>::init_ragged(const RigidConfig* cfgs, int nbatch,
+                                         const RigidBuffers& b) {
+  cfg = cfgs[0];
+  batch = nbatch;
+  buf = b;
+  const int kw = cfg.kill_width;
+  int64_t cell = 0, r_off = 0, w_off = 0;
+  int row = 0, tri = 0, vt = 0, ft = 0;
+  for (int q = 0; q < nbatch;) {
+    int e = q;
+    while (e < nbatch && same_grid(cfgs[e], cfgs[q])) ++e;
+    if ((int)groups.size() == kMaxRaggedGroups) return hipErrorInvalidValue;
+    const int nr = cfgs[q].nr, nz = cfgs[q].nz, nc = e - q;
+    StokesPlan* pl = new (std::nothrow) StokesPlan();
+    if (!pl) return hipErrorOutOfMemory;
+    plans.push_back(pl);
+    if (!fft_args(nz, pl->fft)) return hipErrorInvalidValue;
+    pl->force_L = kRaggedL;
+    hipError_t err = pl->init(nr, nz, cfgs[q].dx);
+    if (err != hipSuccess) return err;
+    if (pl->tri_G) return hipErrorInvalidValue;  // coefficients must fit the LDS
+    RaggedGroup G{};
+    G.nr = nr;
+    G.nz = nz;
+    G.ncase = nc;
+    G.case0 = q;
+    G.rows0 = row;
+    G.K = pl->tri_K;
+    G.P = pl->tri_P;
+    G.tchunks = (int)ceil_div(nz, G.K);
+    G.tri0 = tri;
+    G.vz = vmax_tiles_z(nz);
+    G.vtiles = vmax_tiles(nr, nz);
+    G.vt0 = vt;
+    G.fz = fused_tiles_z(nz);
+    G.ftiles = fused_blocks(nr, nz);
+    G.ft0 = ft;
+    G.part0 = ft;
+    G.cell0 = cell;
+    G.dx = cfgs[q].dx;
+    G.fp = pl->fft;
+    G.tw = pl->tw;
+    G.e1 = pl->e1;
+    G.e2 = pl->e2;
+    G.rvec = pl->rvec;
+    G.a = pl->a;
+    G.b = pl->b;
+    G.c = pl->c;
+    G.lam = pl->lam;
+    G.cseg = pl->cseg;
+    G.r = b.r + r_off;  // the group's first case's tables
+    G.kill_wz = b.kill_wz + 2 * w_off;
+    G.kill_wr = b.kill_wr + w_off;
+    row += nc * nr;
+    tri += nc * G.tchunks;
+    vt += nc * G.vtiles;
+    ft += nc * G.ftiles;
+    cell += (int64_t)nc * nr * nz;
+    r_off += (int64_t)nc * nr;
+    w_off += (int64_t)nc * kw;
+    maxM = std::max(maxM, nz / 2);
+    tri_shm = std::max(tri_shm, tri_lds_bytes(nr, kRaggedL, G.K, G.P, false));
+    groups.push_back(G);
+    q = e;
+  }
+  rows = row;
+  tri_blocks = tri;
+  vt_tiles = vt;
+  ft_tiles = ft;
+  cells = cell;
+  npartials = ft;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&partials), sizeof(double) * (size_t)ft);
+  if (e != hipSuccess) return e;
+  e = hipMalloc(reinterpret_cast<void**>(&slots), sizeof(double) * kSlotsPerCase * batch);
+  if (e != hipSuccess) return e;
+  e = hipMalloc(&spec, sizeof(double) * (size_t)cells);
+  if (e != hipSuccess) return e;
+  e = hipMalloc(reinterpret_cast<void**>(&dcfg), sizeof(RigidConfig) * batch);
+  if (e != hipSuccess) return e;
+  e = hipMemcpy(dcfg, cfgs, sizeof(RigidConfig) * batch, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  e = hipMalloc(reinterpret_cast<void**>(&dgroups), sizeof(RaggedGroup) * groups.size());
+  if (e != hipSuccess) return e;
+  e = hipMemcpy(dgroups, groups.data(), sizeof(RaggedGroup) * groups.size(),
+                hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  init_scalars_kernel<<<batch, 64>>>(buf.scalars, slots, dcfg);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return hipDeviceSynchronize();
+}
+
+template <typename T>
+RaggedStepper<T>::~RaggedStepper() {
+  for (StokesPlan* p : plans) delete p;
+  if (dgroups) (void)hipFree(dgroups);
+}
+
+template <typename T>
+hipError_t RaggedStepper<T>::enqueue_stage(int stage, hipStream_t s) {
+  const int w = cfg.kill_width, ng = (int)groups.size();
+  double* sc = buf.scalars;
+  T* home = static_cast<T*>(buf.vorticity);
+  T* other = static_cast<T*>(buf.work0);
+  T* cur = parity ? other : home;
+  T* nxt = parity ? home : other;
+  T* psi = static_cast<T*>(buf.psi);
+  T* chi = static_cast<T*>(buf.char_func);
+  T* sp = static_cast<T*>(spec);
+  if (!stage_enabled(stage)) return hipSuccess;
+  switch (stage) {
+    case 0: {
+      StepFinalize fin;
+      fin.partials = partials;
+      fin.slots = slots;
+      fin.sc = sc;
+      fin.drag = buf.drag;
+      fin.cfgs = dcfg;
+      return launch_taper_dct2_ragged<T>(cur, sp, dgroups, ng, batch, rows, w, maxM, &fin, s);
+    }
+    case 1:
+      return launch_tridiag_ragged<T>(sp, dgroups, ng, tri_blocks, tri_shm, s);
+    case 2:
+      return launch_dct3_ragged<T>(sp, psi, dgroups, ng, rows, maxM, s);
+    case 3:
+      return launch_velocity_max_ragged<T>(psi, sc, slots, dcfg, dgroups, ng, vt_tiles, s);
+    case 4:
+      return hipSuccess;  // no moving body in uneven batches
+    case 5:
+      return launch_step_fused_ragged<T>(psi, chi, cur, nxt,
+                                         cfg.average_psi ? static_cast<T*>(buf.avg_psi) : nullptr,
+                                         w, sc, slots, partials, dcfg, dgroups, ng, ft_tiles, s);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t RaggedStepper<T>::settle(hipStream_t s) {
+  hipError_t e = launch_finalize_ragged(partials, slots, buf.scalars, buf.drag, dcfg, dgroups,
+                                        (int)groups.size(), batch, s);
+  if (e != hipSuccess) return e;
+  if (!parity) return hipSuccess;
+  copy_field_kernel<T><<<ceil_div(cells, 256), 256, 0, s>>>(static_cast<T*>(buf.vorticity),
+                                                            static_cast<const T*>(buf.work0),
+                                                            cells);
+  parity = 0;
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t RaggedStepper<T>::velocity_out(hipStream_t s) {
+  for (const RaggedGroup& G : groups) {
+    hipError_t e = launch_velocity_out<T>(
+        static_cast<const T*>(buf.psi) + G.cell0, static_cast<const T*>(buf.char_func) + G.cell0,
+        static_cast<T*>(buf.u_pen_z) + G.cell0, static_cast<T*>(buf.u_pen_r) + G.cell0, G.r, G.nr,
+        G.nz, T(G.dx), buf.scalars + (int64_t)G.case0 * AXS_RIGID_SCALARS, dcfg + G.case0,
+        G.ncase, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 template struct RigidStepper<double>;
 template struct RigidStepper<float>;
+template struct RaggedStepper<double>;
+template struct RaggedStepper<float>;
 
 }  // namespace axs
 
@@ -347,16 +534,31 @@ axs_status axs_rigid_batch_create(const axs_rigid_config* cfgs, int batch,
                 "bad kill_width for the grid");
   AXS_CHECK_ARG(c0.precision == AXS_F64 || c0.precision == AXS_F32, "bad precision");
   AXS_CHECK_ARG(c0.history >= 0, "negative history");
+  bool uneven = false;
+  int runs = 1;
   for (int q = 0; q < batch; ++q) {
     const axs_rigid_config& c = cfgs[q];
-    AXS_CHECK_ARG(c.nr == c0.nr && c.nz == c0.nz && c.kill_width == c0.kill_width &&
-                      c.history == c0.history && c.precision == c0.precision &&
-                      c.moving_body == c0.moving_body && c.average_psi == c0.average_psi &&
-                      c.dx == c0.dx,
-                  "cases of a batch must share grid, kill width, history, precision and flags");
+    AXS_CHECK_ARG(c.kill_width == c0.kill_width && c.history == c0.history &&
+                      c.precision == c0.precision && c.moving_body == c0.moving_body &&
+                      c.average_psi == c0.average_psi,
+                  "cases of a batch must share kill width, history, precision and flags");
+    AXS_CHECK_ARG(c.nr >= 4 && c.nz >= 8 && c.nz % 2 == 0, "need nr >= 4 and even nz >= 8");
+    AXS_CHECK_ARG(2 * c.kill_width <= c.nz && c.kill_width < c.nr, "bad kill_width for the grid");
     AXS_CHECK_ARG(c.dx > 0 && c.nu > 0 && c.CFL > 0, "dx, nu and CFL must be positive");
     AXS_CHECK_ARG(!c.moving_body || (c.body_aspect2 > 0 && c.moll_zone > 0),
                   "moving body needs aspect^2 > 0 and moll_zone > 0");
+    if (c.nr != c0.nr || c.nz != c0.nz || c.dx != c0.dx) uneven = true;
+    if (q > 0 && (c.nr != cfgs[q - 1].nr || c.nz != cfgs[q - 1].nz || c.dx != cfgs[q - 1].dx))
+      ++runs;
+  }
+  if (uneven) {
+    // one launch per pass over cases of several grids (axs_ragged.hpp)
+    AXS_CHECK_ARG(!c0.moving_body, "a batch of uneven grids takes no moving body");
+    AXS_CHECK_ARG(runs <= kMaxRaggedGroups,
+                  "a batch of uneven grids takes at most 16 runs of one grid (order the cases)");
+    for (int q = 0; q < batch; ++q)
+      AXS_CHECK_ARG(cfgs[q].nz <= 1024 && cfgs[q].nr <= 4096,
+                    "a batch of uneven grids takes nz <= 1024 and nr <= 4096");
   }
   AXS_CHECK_ARG(bufs->vorticity && bufs->psi && bufs->u_pen_z && bufs->u_pen_r &&
                     bufs->char_func && bufs->work0 && bufs->scalars && bufs->r && bufs->kill_wz &&
@@ -370,7 +572,11 @@ axs_status axs_rigid_batch_create(const axs_rigid_config* cfgs, int batch,
     set_error("out of host memory");
     return AXS_ERR_ALLOC;
   }
-  if (c0.precision == AXS_F64)
+  if (uneven && c0.precision == AXS_F64)
+    st->impl = new (std::nothrow) RaggedStepper<double>();
+  else if (uneven)
+    st->impl = new (std::nothrow) RaggedStepper<float>();
+  else if (c0.precision == AXS_F64)
     st->impl = new (std::nothrow) RigidStepper<double>();
   else
     st->impl = new (std::nothrow) RigidStepper<float>();
@@ -385,7 +591,13 @@ axs_status axs_rigid_batch_create(const axs_rigid_config* cfgs, int batch,
                  bufs->u_pen_r,   bufs->char_func, bufs->work0, bufs->work1,  bufs->avg_psi,
                  bufs->scalars,   bufs->drag,   bufs->r,       bufs->z,      bufs->kill_wz,
                  bufs->kill_wr};
-  hipError_t e = st->impl->init(cs.data(), batch, b);
+  hipError_t e;
+  if (!uneven)
+    e = st->impl->init(cs.data(), batch, b);
+  else if (c0.precision == AXS_F64)
+    e = static_cast<RaggedStepper<double>*>(st->impl)->init_ragged(cs.data(), batch, b);
+  else
+    e = static_cast<RaggedStepper<float>*>(st->impl)->init_ragged(cs.data(), batch, b);
   if (e != hipSuccess) {
     set_error(std::string("rigid stepper: ") + hipGetErrorString(e));
     delete st;

@@ -3,6 +3,8 @@
 
 #include "axs_stokes.hpp"
 
+#include <vector>
+
 namespace axs {
 
 // Device scalar slots (double array of kScCount entries).
@@ -95,6 +97,25 @@ struct RigidStepper : RigidStepperBase {
   hipError_t enqueue_stage(int stage, hipStream_t s) override;
   hipError_t settle(hipStream_t s) override;
   hipError_t velocity_out(hipStream_t s) override;
+};
+
+// Cases of several grids in one launch per pass (axs_ragged.hpp): runs of
+// cases with one (nr, nz, dx) form groups, each with its own plan; every
+// field holds all cases back to back, and the r and kill-weight tables hold
+// each case's table back to back.  No moving body; nz <= 1024.
+template <typename T>
+struct RaggedStepper : RigidStepperBase {
+  std::vector<StokesPlan*> plans;
+  std::vector<RaggedGroup> groups;
+  RaggedGroup* dgroups = nullptr;
+  int rows = 0, tri_blocks = 0, vt_tiles = 0, ft_tiles = 0, maxM = 0;
+  size_t tri_shm = 0;
+  int64_t cells = 0;
+  hipError_t init_ragged(const RigidConfig* cfgs, int nbatch, const RigidBuffers& b);
+  hipError_t enqueue_stage(int stage, hipStream_t s) override;
+  hipError_t settle(hipStream_t s) override;
+  hipError_t velocity_out(hipStream_t s) override;
+  ~RaggedStepper() override;
 };
 
 }  // namespace axs

@@ -133,13 +133,20 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
         tri_L = L;
         tri_P = P;
         tri_G = G == 1;
-        float ms = 0.f;
+        // the fastest of three batches of five launches (one batch is noisy
+        // enough to rank neighbours wrongly)
+        float ms = 1e30f;
         for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) e = launch_tridiag<double>(*this, spec, 0);
-        if (e == hipSuccess) e = hipEventRecord(t0, 0);
-        for (int rep = 0; rep < 5 && e == hipSuccess; ++rep) e = launch_tridiag<double>(*this, spec, 0);
-        if (e == hipSuccess) e = hipEventRecord(t1, 0);
-        if (e == hipSuccess) e = hipEventSynchronize(t1);
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        for (int trial = 0; trial < 3 && e == hipSuccess; ++trial) {
+          float t = 0.f;
+          e = hipEventRecord(t0, 0);
+          for (int rep = 0; rep < 5 && e == hipSuccess; ++rep)
+            e = launch_tridiag<double>(*this, spec, 0);
+          if (e == hipSuccess) e = hipEventRecord(t1, 0);
+          if (e == hipSuccess) e = hipEventSynchronize(t1);
+          if (e == hipSuccess) e = hipEventElapsedTime(&t, t0, t1);
+          if (e == hipSuccess && t < ms) ms = t;
+        }
         if (e != hipSuccess) break;
         if (ms < best_ms) {
           best_ms = ms;
@@ -229,6 +236,12 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
   while (tri_K * 2 * tri_P <= 1024 && tri_K < 64 && nz / (tri_K * 2) >= 512) tri_K *= 2;
   if (tri_K < 2 && 2 * tri_P <= 1024) tri_K = 2;
   // keep the column kernel's LDS (scan buffers + coefficients) within 64 KiB
+  if (force_L > 0) {
+    tri_L = force_L;
+    tri_P = (nr + tri_L - 1) / tri_L;
+    if (tri_P > 1024) return hipErrorInvalidValue;
+    tri_K = std::max(1, 1024 / tri_P);
+  }
   tri_G = tri_lds_bytes(nr, tri_L, tri_K, tri_P, false) > 65536;
   while (tri_K > 1 && tri_lds_bytes(nr, tri_L, tri_K, tri_P, tri_G) > 65536) tri_K /= 2;
   if (tri_lds_bytes(nr, tri_L, tri_K, tri_P, tri_G) > 65536) return hipErrorInvalidValue;
@@ -245,6 +258,7 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
   if ((e = upload(&cseg, h_cseg)) != hipSuccess) return e;
   e = hipMalloc(reinterpret_cast<void**>(&spec), sizeof(double) * (size_t)nr * nz);
   if (e != hipSuccess) return e;
+  if (force_L > 0) return hipSuccess;
   if (std::getenv("AXS_TRIDIAG_LK") ||
       ((size_t)nr * nz >= (size_t(1) << 18) && !std::getenv("AXS_NO_TUNE")))
     return tune_tridiag(h_a, h_b, h_c, h_lam);
@@ -294,30 +308,17 @@ __global__ void __launch_bounds__(256) dct2_rows_kernel(
 // and r_i * row goes through the DCT-II into `dst`.  Reading
 // src and writing a separate buffer keeps rows independent (the top rows
 // read row nr-w).  wz: 2w weights (left block, right block), wr: w weights.
+// One row of the tapered forward pass (src, tapered, dst at the case).
 template <typename T, int PER>
-__global__ void __launch_bounds__(256) taper_dct2_rows_kernel(
+__device__ __forceinline__ void taper_dct2_row(
     const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
     const double* __restrict__ rscale, const double* __restrict__ wz,
-    const double* __restrict__ wr, int nr, int nz, int w, FftPlanArgs fp,
+    const double* __restrict__ wr, int nr, int nz, int w, const FftPlanArgs& fp,
     const Cplx<double>* __restrict__ tw, const Cplx<double>* __restrict__ e1,
-    const Cplx<double>* __restrict__ e2, StepFinalize fin) {
-  if (fin.partials && blockIdx.x == nr) {  // completes the previous step
-    __shared__ double fin_red[256];
-    const int cb = blockIdx.y;
-    const RigidConfig& cfg = fin.cfgs[cb];
-    finalize_pending(fin.partials + (int64_t)cb * fin.count, fin.count,
-                     fin.slots + (int64_t)cb * kSlotsPerCase, fin.sc + (int64_t)cb * AXS_RIGID_SCALARS,
-                     fin.drag ? fin.drag + (int64_t)cb * cfg.history : nullptr, cfg, fin_red);
-    return;
-  }
+    const Cplx<double>* __restrict__ e2, int i) {
   extern __shared__ unsigned char smem_raw[];
   const int M = fp.M;
   Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
-  const int i = blockIdx.x;
-  const int64_t cs = (int64_t)blockIdx.y * nr * nz;  // case of a batch
-  src += cs;
-  dst += cs;
-  if (tapered) tapered += cs;
   const int srow = (i >= nr - w) ? nr - w : i;
   const T* row = src + (int64_t)srow * nz;
   constexpr int NJ = 4 * PER;
@@ -356,18 +357,37 @@ __global__ void __launch_bounds__(256) taper_dct2_rows_kernel(
 
 // Row pass: x = DCT-II^{-1}(src[i, :]) -> dst[i, :], stored coalesced by
 // gathering each column's value from the packed LDS sequence.
+
 template <typename T, int PER>
-__global__ void __launch_bounds__(256) dct3_rows_kernel(
-    const T* __restrict__ src, T* __restrict__ dst, int nz, FftPlanArgs fp,
+__global__ void __launch_bounds__(256) taper_dct2_rows_kernel(
+    const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
+    const double* __restrict__ rscale, const double* __restrict__ wz,
+    const double* __restrict__ wr, int nr, int nz, int w, FftPlanArgs fp,
     const Cplx<double>* __restrict__ tw, const Cplx<double>* __restrict__ e1,
-    const Cplx<double>* __restrict__ e2) {
+    const Cplx<double>* __restrict__ e2, StepFinalize fin) {
+  if (fin.partials && blockIdx.x == nr) {  // completes the previous step
+    __shared__ double fin_red[256];
+    const int cb = blockIdx.y;
+    const RigidConfig& cfg = fin.cfgs[cb];
+    finalize_pending(fin.partials + (int64_t)cb * fin.count, fin.count,
+                     fin.slots + (int64_t)cb * kSlotsPerCase, fin.sc + (int64_t)cb * AXS_RIGID_SCALARS,
+                     fin.drag ? fin.drag + (int64_t)cb * cfg.history : nullptr, cfg, fin_red);
+    return;
+  }
+  const int64_t cs = (int64_t)blockIdx.y * nr * nz;  // case of a batch
+  taper_dct2_row<T, PER>(src + cs, tapered ? tapered + cs : nullptr, dst + cs, rscale, wz, wr, nr,
+                         nz, w, fp, tw, e1, e2, blockIdx.x);
+}
+// One row of the inverse pass (src, dst at the case).
+template <typename T, int PER>
+__device__ __forceinline__ void dct3_row(const T* __restrict__ src, T* __restrict__ dst, int nz,
+                                         const FftPlanArgs& fp,
+                                         const Cplx<double>* __restrict__ tw,
+                                         const Cplx<double>* __restrict__ e1,
+                                         const Cplx<double>* __restrict__ e2, int i) {
   extern __shared__ unsigned char smem_raw[];
   const int M = fp.M;
   Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
-  const int i = blockIdx.x;
-  const int64_t cs = (int64_t)blockIdx.y * gridDim.x * nz;  // case of a batch
-  src += cs;
-  dst += cs;
   const T* row = src + (int64_t)i * nz;
   constexpr int NQ = 2 * PER;
   dct3_pre<T, NQ>(buf, M, e1, e2, [&](int k) { return row[k]; });
@@ -381,6 +401,14 @@ __global__ void __launch_bounds__(256) dct3_rows_kernel(
     const int j = threadIdx.x + q * blockDim.x;
     if (j < nz) out[j] = packed_component(buf, dct_dst_index(j, nz)) * invN;
   }
+}
+template <typename T, int PER>
+__global__ void __launch_bounds__(256) dct3_rows_kernel(
+    const T* __restrict__ src, T* __restrict__ dst, int nz, FftPlanArgs fp,
+    const Cplx<double>* __restrict__ tw, const Cplx<double>* __restrict__ e1,
+    const Cplx<double>* __restrict__ e2) {
+  const int64_t cs = (int64_t)blockIdx.y * gridDim.x * nz;  // case of a batch
+  dct3_row<T, PER>(src + cs, dst + cs, nz, fp, tw, e1, e2, blockIdx.x);
 }
 
 // nz = 2048 row passes: one wavefront per row, FFT in registers
@@ -558,12 +586,11 @@ __device__ __forceinline__ double recip_nr(double m) {
 // being staged in LDS, which leaves the LDS to the scans and allows more
 // modes per workgroup on tall grids.
 template <typename T, int L, bool FULL, bool CG>
-__global__ void __launch_bounds__(1024) tridiag_columns_kernel(
+__device__ __forceinline__ void tridiag_columns_body(
     T* __restrict__ d, const double* __restrict__ a, const double* __restrict__ b,
     const double* __restrict__ c, const double* __restrict__ lam,
-    const double* __restrict__ cseg, int nr, int nz, int K, int P) {
+    const double* __restrict__ cseg, int nr, int nz, int K, int P, int chunk) {
   extern __shared__ double sm[];
-  d += (int64_t)blockIdx.y * nr * nz;  // case of a batch
   // coefficient rows are padded by one double per L rows: the threads of a
   // wave read rows p L + l of different segments p in the same instruction,
   // and a stride of L doubles would put them all in one LDS bank
@@ -582,11 +609,6 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
       sC[i + i / L] = c[i];
     }
   }
-  // XCD-aware chunk order: block b runs on XCD b % 8; give each XCD a
-  // contiguous range of mode chunks.
-  const int nb = gridDim.x;
-  int chunk = blockIdx.x;
-  if (nb % 8 == 0) chunk = (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
   const int kk = threadIdx.x % K;
   const int p = threadIdx.x / K;
   const int k = chunk * K + kk;
@@ -635,26 +657,33 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
   }
   // inclusive scan over segments of the affine maps D -> G*D + y
   const int me = p * K + kk;
-  sG0[me] = G;
-  sY0[me] = y;
+  // threads past K P (a wider ragged launch) only join the barriers; with
+  // FULL the workgroup is exactly K P threads
+  const bool seg = FULL || p < P;
+  if (seg) {
+    sG0[me] = G;
+    sY0[me] = y;
+  }
   __syncthreads();
   double* inG = sG0; double* inY = sY0; double* outG = sG1; double* outY = sY1;
   for (int off = 1; off < P; off <<= 1) {
-    double g = inG[me], yy = inY[me];
-    if (p >= off) {
-      const int o = (p - off) * K + kk;
-      // compose: this o (earlier) then me: D -> g*(G_o D + y_o) + yy
-      yy = g * inY[o] + yy;
-      g = g * inG[o];
+    if (seg) {
+      double g = inG[me], yy = inY[me];
+      if (p >= off) {
+        const int o = (p - off) * K + kk;
+        // compose: this o (earlier) then me: D -> g*(G_o D + y_o) + yy
+        yy = g * inY[o] + yy;
+        g = g * inG[o];
+      }
+      outG[me] = g;
+      outY[me] = yy;
     }
-    outG[me] = g;
-    outY[me] = yy;
     __syncthreads();
     double* t;
     t = inG; inG = outG; outG = t;
     t = inY; inY = outY; outY = t;
   }
-  const double Din = (p == 0) ? 0.0 : inY[(p - 1) * K + kk];
+  const double Din = (p == 0 || !seg) ? 0.0 : inY[(p - 1) * K + kk];
   __syncthreads();
   // fix-up: d'_i = y_i + G_i * Din
   {
@@ -681,25 +710,29 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
     }
   }
   // suffix scan over segments (later segment feeds the earlier one)
-  sG0[me] = H;
-  sY0[me] = z;
+  if (seg) {
+    sG0[me] = H;
+    sY0[me] = z;
+  }
   __syncthreads();
   inG = sG0; inY = sY0; outG = sG1; outY = sY1;
   for (int off = 1; off < P; off <<= 1) {
-    double h = inG[me], zz = inY[me];
-    if (p + off < P) {
-      const int o = (p + off) * K + kk;
-      zz = h * inY[o] + zz;
-      h = h * inG[o];
+    if (seg) {
+      double h = inG[me], zz = inY[me];
+      if (p + off < P) {
+        const int o = (p + off) * K + kk;
+        zz = h * inY[o] + zz;
+        h = h * inG[o];
+      }
+      outG[me] = h;
+      outY[me] = zz;
     }
-    outG[me] = h;
-    outY[me] = zz;
     __syncthreads();
     double* t;
     t = inG; inG = outG; outG = t;
     t = inY; inY = outY; outY = t;
   }
-  const double Xin = (p == P - 1) ? 0.0 : inY[(p + 1) * K + kk];
+  const double Xin = (p >= P - 1) ? 0.0 : inY[(p + 1) * K + kk];
   // fix-up and store: x_i = z_i + H_i * Xin, H_i = prod_{j=i}^{e-1} (-c'_j)
   {
     double h = 1.0;
@@ -712,6 +745,70 @@ __global__ void __launch_bounds__(1024) tridiag_columns_kernel(
       }
     }
   }
+}
+template <typename T, int L, bool FULL, bool CG>
+__global__ void __launch_bounds__(1024) tridiag_columns_kernel(
+    T* __restrict__ d, const double* __restrict__ a, const double* __restrict__ b,
+    const double* __restrict__ c, const double* __restrict__ lam,
+    const double* __restrict__ cseg, int nr, int nz, int K, int P) {
+  // XCD-aware chunk order: block b runs on XCD b % 8; give each XCD a
+  // contiguous range of mode chunks.
+  const int nb = gridDim.x;
+  int chunk = blockIdx.x;
+  if (nb % 8 == 0) chunk = (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
+  tridiag_columns_body<T, L, FULL, CG>(d + (int64_t)blockIdx.y * nr * nz, a, b, c, lam, cseg, nr,
+                                       nz, K, P, chunk);
+}
+
+// ------------------------------------------------------------ uneven batches
+
+template <typename T>
+__global__ void __launch_bounds__(256) taper_dct2_ragged_kernel(
+    const T* __restrict__ src, T* __restrict__ dst, const RaggedGroup* __restrict__ grp, int ngrp,
+    int ncases, int w, StepFinalize fin) {
+  int b = blockIdx.x;
+  if (fin.partials) {
+    if (b < ncases) {  // completes case b's previous step
+      __shared__ double fin_red[256];
+      const RaggedGroup& G = grp[ragged_group<&RaggedGroup::case0>(grp, ngrp, b)];
+      const RigidConfig& cfg = fin.cfgs[b];
+      finalize_pending(fin.partials + G.part0 + (int64_t)(b - G.case0) * G.ftiles, G.ftiles,
+                       fin.slots + (int64_t)b * kSlotsPerCase,
+                       fin.sc + (int64_t)b * AXS_RIGID_SCALARS,
+                       fin.drag ? fin.drag + (int64_t)b * cfg.history : nullptr, cfg, fin_red);
+      return;
+    }
+    b -= ncases;
+  }
+  const RaggedGroup& G = grp[ragged_group<&RaggedGroup::rows0>(grp, ngrp, b)];
+  const int q = b - G.rows0, cl = q / G.nr, i = q % G.nr;
+  const int64_t cs = G.cell0 + (int64_t)cl * G.nr * G.nz;
+  taper_dct2_row<T, 1>(src + cs, nullptr, dst + cs, G.rvec, G.kill_wz, G.kill_wr, G.nr, G.nz, w,
+                       G.fp, G.tw, G.e1, G.e2, i);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) dct3_ragged_kernel(const T* __restrict__ src,
+                                                          T* __restrict__ dst,
+                                                          const RaggedGroup* __restrict__ grp,
+                                                          int ngrp) {
+  const int b = blockIdx.x;
+  const RaggedGroup& G = grp[ragged_group<&RaggedGroup::rows0>(grp, ngrp, b)];
+  const int q = b - G.rows0, cl = q / G.nr, i = q % G.nr;
+  const int64_t cs = G.cell0 + (int64_t)cl * G.nr * G.nz;
+  dct3_row<T, 1>(src + cs, dst + cs, G.nz, G.fp, G.tw, G.e1, G.e2, i);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) tridiag_ragged_kernel(T* __restrict__ d,
+                                                             const RaggedGroup* __restrict__ grp,
+                                                             int ngrp) {
+  const int b = blockIdx.x;
+  const RaggedGroup& G = grp[ragged_group<&RaggedGroup::tri0>(grp, ngrp, b)];
+  const int q = b - G.tri0, cl = q / G.tchunks, chunk = q % G.tchunks;
+  tridiag_columns_body<T, kRaggedL, false, false>(d + G.cell0 + (int64_t)cl * G.nr * G.nz, G.a,
+                                                  G.b, G.c, G.lam, G.cseg, G.nr, G.nz, G.K, G.P,
+                                                  chunk);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -841,6 +938,41 @@ hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch) 
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_taper_dct2_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp,
+                                    int ncases, int rows, int w, int maxM,
+                                    const StepFinalize* fin, hipStream_t s) {
+  const StepFinalize f = fin ? *fin : StepFinalize{};
+  const int blocks = rows + (f.partials ? ncases : 0);
+  taper_dct2_ragged_kernel<T><<<blocks, 256, sizeof(Cplx<T>) * maxM, s>>>(src, dst, grp, ngrp,
+                                                                          ncases, w, f);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_dct3_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp, int rows,
+                              int maxM, hipStream_t s) {
+  dct3_ragged_kernel<T><<<rows, 256, sizeof(Cplx<T>) * maxM, s>>>(src, dst, grp, ngrp);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_tridiag_ragged(T* d, const RaggedGroup* grp, int ngrp, int blocks, size_t shm,
+                                 hipStream_t s) {
+  tridiag_ragged_kernel<T><<<blocks, 1024, shm, s>>>(d, grp, ngrp);
+  return hipGetLastError();
+}
+
+#define AXS_RAGGED_INST(T)                                                                      \
+  template hipError_t launch_taper_dct2_ragged<T>(const T*, T*, const RaggedGroup*, int, int, int, \
+                                                  int, int, const StepFinalize*, hipStream_t);    \
+  template hipError_t launch_dct3_ragged<T>(const T*, T*, const RaggedGroup*, int, int, int,      \
+                                            hipStream_t);                                         \
+  template hipError_t launch_tridiag_ragged<T>(T*, const RaggedGroup*, int, int, size_t,          \
+                                               hipStream_t);
+AXS_RAGGED_INST(double)
+AXS_RAGGED_INST(float)
+#undef AXS_RAGGED_INST
 template hipError_t launch_dct2_rows<double>(const StokesPlan&, const double*, double*, const double*, int, hipStream_t);
 template hipError_t launch_dct3_rows<double>(const StokesPlan&, const double*, double*, int, hipStream_t, int);
 template hipError_t launch_tridiag<double>(const StokesPlan&, double*, hipStream_t, int);

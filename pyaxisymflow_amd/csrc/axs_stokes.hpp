@@ -3,6 +3,7 @@
 #pragma once
 
 #include "axs_fft.hpp"
+#include "axs_ragged.hpp"
 
 #include <vector>
 
@@ -34,6 +35,9 @@ struct StokesPlan {
   double* spec = nullptr;      // spectral work buffer (nr, nz)
   int tri_L = 16, tri_P = 1, tri_K = 1;
   bool tri_G = false;  // tridiagonal coefficients read from global memory, not LDS
+  // > 0: segments of force_L rows and as many modes per workgroup as fit
+  // 1024 threads, no timed search (the layout an uneven batch launches)
+  int force_L = 0;
 
   hipError_t init(int nr, int nz, double dx);
   hipError_t tune_tridiag(const std::vector<double>& a, const std::vector<double>& b,
@@ -71,5 +75,23 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
                             int batch = 1);
 template <typename T>
 hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch = 1);
+
+// Uneven batches (axs_ragged.hpp): one launch per pass over every group.
+// Row passes run 256 threads per row (nz <= 1024); the forward pass puts one
+// completion workgroup per case in front of the rows when fin is given
+// (force partials of case q at part0 + (q - case0) * ftiles of its group).
+template <typename T>
+hipError_t launch_taper_dct2_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp,
+                                    int ncases, int rows, int w, int maxM,
+                                    const StepFinalize* fin, hipStream_t s);
+template <typename T>
+hipError_t launch_dct3_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp, int rows,
+                              int maxM, hipStream_t s);
+// Radial solves with segments of kRaggedL rows, K and P per group, one
+// 1024-thread workgroup per K modes of a case.
+constexpr int kRaggedL = 8;
+template <typename T>
+hipError_t launch_tridiag_ragged(T* d, const RaggedGroup* grp, int ngrp, int blocks, size_t shm,
+                                 hipStream_t s);
 
 }  // namespace axs

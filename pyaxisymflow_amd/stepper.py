@@ -256,6 +256,171 @@ class RigidBatchStepper:
             self._h = None
 
 
+class UnevenBatchStepper:
+    """Rigid-body cases of several grids, stepped together with one launch
+    per pass over all of them (the batched form of
+    examples/FlowPastSphere/flow_past_sphere.py:107-188 over uneven grids).
+
+    ``cases`` is a list of dicts, each with the case's grid -- ``nz``,
+    ``nr``, ``z_len`` and optionally ``r_len`` -- and the per-case arguments
+    of ``RigidBodyStepper`` (no ``moving_body``; nz <= 1024).  Cases are
+    stored grouped by grid (at most 16 grids); accessors take the caller's
+    case index and return that case's (nr, nz) arrays.  Every case keeps its
+    own t, dt and force history.
+    """
+
+    def __init__(self, cases, kill_width=3, use_graph=True, precision="f64", history=1 << 16):
+        torch = _lib.torch()
+        dev = _lib.device()
+        if precision not in ("f64", "f32"):
+            raise ValueError("precision must be 'f64' or 'f32'")
+        cases = [dict(c) for c in cases]
+        if not cases:
+            raise ValueError("need at least one case")
+        grids = []
+        for c in cases:
+            for k in ("nz", "nr", "z_len"):
+                if k not in c:
+                    raise TypeError(f"each case needs {k}")
+            key = (int(c["nz"]), int(c["nr"]), float(c["z_len"]),
+                   None if c.get("r_len") is None else float(c["r_len"]))
+            c["_key"] = key
+            bad = set(c) - set(_CASE_KEYS) - {"nz", "nr", "z_len", "r_len", "_key"}
+            if bad:
+                raise TypeError(f"unknown case arguments {sorted(bad)}")
+            if key not in grids:
+                grids.append(key)
+        if len(grids) > 16:
+            raise ValueError("at most 16 distinct grids")
+        sig = {(nz, nr, z_len / nz) for nz, nr, z_len, _ in grids}
+        if len(sig) != len(grids):
+            raise ValueError("grids differing only in r_len cannot share one batch")
+        self.batch = B = len(cases)
+        self.precision = precision
+        self.use_graph = use_graph
+        self.history = int(history)
+        self.stream = torch.cuda.Stream(device=dev)
+        ft = torch.float64 if precision == "f64" else torch.float32
+        self.field_dtype = ft
+        # internal order: by grid, in order of first appearance
+        self.order = sorted(range(B), key=lambda q: grids.index(cases[q]["_key"]))
+        self.slot = {q: k for k, q in enumerate(self.order)}
+        self.shape, self.offset = {}, {}
+        self.grid = {}
+        cfgs = (_Config * B)()
+        r_tab, wz_tab, wr_tab = [], [], []
+        off = 0
+        for k, q in enumerate(self.order):
+            c = cases[q]
+            nz, nr, z_len, r_len = c["_key"]
+            dx, z, r = rigid_grid(nz, nr, z_len, r_len)
+            self.grid[q] = (dx, z, r)
+            self.shape[q] = (nr, nz)
+            self.offset[q] = off
+            off += nr * nz
+            args = {kk: v for kk, v in c.items() if kk in _CASE_KEYS}
+            cfgs[k] = _case_config(nr, nz, dx, kill_width, self.history, precision, **args)
+            wz, wr = kill_weights(z, r, dx, kill_width)
+            r_tab.append(r)
+            wz_tab.append(wz)
+            wr_tab.append(wr)
+        self.cells = off
+        self.cfgs = cfgs
+        average_psi = bool(cfgs[0].average_psi)
+
+        def fields(key=None):
+            t = torch.zeros(off, dtype=ft, device=dev)
+            if key is not None:
+                for q, c in enumerate(cases):
+                    a = c.get(key)
+                    if a is None:
+                        continue
+                    a = np.asarray(a, dtype=np.float64)
+                    if a.shape != self.shape[q]:
+                        raise ValueError(f"{key}: expected shape {self.shape[q]}, got {a.shape}")
+                    self._view(t, q).copy_(
+                        torch.from_numpy(np.ascontiguousarray(a)).to(dev).to(ft))
+            return t
+
+        def table(a):
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+        self.t_vorticity = fields("vorticity")
+        self.t_psi = fields()
+        self.t_u_pen_z = fields()
+        self.t_u_pen_r = fields()
+        self.t_char_func = fields("char_func")
+        self.t_work0 = fields()
+        self.t_avg_psi = fields() if average_psi else None
+        self.t_scalars = torch.zeros((B, N_SCALARS), dtype=torch.float64, device=dev)
+        self.t_drag = torch.zeros((B, max(self.history, 1)), dtype=torch.float64, device=dev)
+        self.t_r = table(np.concatenate(r_tab))
+        self.t_wz = table(np.concatenate(wz_tab))
+        self.t_wr = table(np.concatenate(wr_tab))
+        bufs = _Buffers()
+        bufs.vorticity = self.t_vorticity.data_ptr()
+        bufs.psi = self.t_psi.data_ptr()
+        bufs.u_pen_z = self.t_u_pen_z.data_ptr()
+        bufs.u_pen_r = self.t_u_pen_r.data_ptr()
+        bufs.char_func = self.t_char_func.data_ptr()
+        bufs.work0 = self.t_work0.data_ptr()
+        bufs.scalars = self.t_scalars.data_ptr()
+        bufs.drag = self.t_drag.data_ptr()
+        bufs.avg_psi = self.t_avg_psi.data_ptr() if average_psi else None
+        bufs.r = self.t_r.data_ptr()
+        bufs.z = None
+        bufs.kill_wz = self.t_wz.data_ptr()
+        bufs.kill_wr = self.t_wr.data_ptr()
+        self._bufs = bufs
+        torch.cuda.synchronize(dev)
+        handle = ctypes.c_void_p()
+        _lib.call("axs_rigid_batch_create", cfgs, B, ctypes.byref(bufs), ctypes.byref(handle))
+        self._h = handle
+
+    def _view(self, t, case):
+        nr, nz = self.shape[case]
+        o = self.offset[case]
+        return t[o:o + nr * nz].view(nr, nz)
+
+    step = RigidBatchStepper.step
+    synchronize = RigidBatchStepper.synchronize
+    _host = RigidBatchStepper._host
+    __del__ = RigidBatchStepper.__del__
+
+    def vorticity(self, case=0):
+        return self._host(self._view(self.t_vorticity, case))
+
+    def psi(self, case=0):
+        return self._host(self._view(self.t_psi, case))
+
+    def avg_psi(self, case=0):
+        return None if self.t_avg_psi is None else self._host(self._view(self.t_avg_psi, case))
+
+    def char_func(self, case=0):
+        return self._host(self._view(self.t_char_func, case))
+
+    def velocity(self, case=0):
+        _lib.call("axs_rigid_stepper_velocity", self._h, ctypes.c_void_p(self.stream.cuda_stream))
+        return (self._host(self._view(self.t_u_pen_z, case)),
+                self._host(self._view(self.t_u_pen_r, case)))
+
+    def scalars(self, case=0):
+        return self._host(self.t_scalars[self.slot[case]])
+
+    def time(self, case=0):
+        return float(self.scalars(case)[SC_T])
+
+    def steps_done(self, case=0):
+        return int(self.scalars(case)[SC_IT])
+
+    def dt(self, case=0):
+        return float(self.scalars(case)[SC_DT])
+
+    def drag_history(self, case=0):
+        n = min(self.steps_done(case), self.history)
+        return self._host(self.t_drag[self.slot[case]])[:n].copy()
+
+
 class RigidBodyStepper(RigidBatchStepper):
     """Rigid body in an axisymmetric flow, stepped on the GPU.
 
