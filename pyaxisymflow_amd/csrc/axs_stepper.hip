@@ -34,6 +34,8 @@
 #include "axs_stepper.hpp"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <new>
 #include <vector>
 
@@ -310,15 +312,14 @@ static bool fft_args(int nz, FftPlanArgs& fp) {
   return m == 1;
 }
 
-template <typename T>
-hipError_t RaggedStepper<T>::init_ragged(const RigidConfig* cfgs, int nbatch,
-                                         const RigidBuffers& b) {
-  cfg = cfgs[0];
-  batch = nbatch;
-  buf = b;
-  const int kw = cfg.kill_width;
+// Groups (runs of one grid) and their plans for radial segments of L rows.
+static hipError_t ragged_groups(const RigidConfig* cfgs, int nbatch, const RigidBuffers& b, int L,
+                                std::vector<StokesPlan*>& plans, std::vector<RaggedGroup>& groups,
+                                int& tri_blocks, size_t& tri_shm) {
+  const int kw = cfgs[0].kill_width;
   int64_t cell = 0, r_off = 0, w_off = 0;
   int row = 0, tri = 0, vt = 0, ft = 0;
+  tri_shm = 0;
   for (int q = 0; q < nbatch;) {
     int e = q;
     while (e < nbatch && same_grid(cfgs[e], cfgs[q])) ++e;
@@ -328,7 +329,7 @@ hipError_t RaggedStepper<T>::init_ragged(const RigidConfig* cfgs, int nbatch,
     if (!pl) return hipErrorOutOfMemory;
     plans.push_back(pl);
     if (!fft_args(nz, pl->fft)) return hipErrorInvalidValue;
-    pl->force_L = kRaggedL;
+    pl->force_L = L;
     hipError_t err = pl->init(nr, nz, cfgs[q].dx);
     if (err != hipSuccess) return err;
     if (pl->tri_G) return hipErrorInvalidValue;  // coefficients must fit the LDS
@@ -371,22 +372,115 @@ hipError_t RaggedStepper<T>::init_ragged(const RigidConfig* cfgs, int nbatch,
     cell += (int64_t)nc * nr * nz;
     r_off += (int64_t)nc * nr;
     w_off += (int64_t)nc * kw;
-    maxM = std::max(maxM, nz / 2);
-    tri_shm = std::max(tri_shm, tri_lds_bytes(nr, kRaggedL, G.K, G.P, false));
+    tri_shm = std::max(tri_shm, tri_lds_bytes(nr, L, G.K, G.P, false));
     groups.push_back(G);
     q = e;
   }
-  rows = row;
   tri_blocks = tri;
-  vt_tiles = vt;
-  ft_tiles = ft;
-  cells = cell;
-  npartials = ft;
-  hipError_t e = hipMalloc(reinterpret_cast<void**>(&partials), sizeof(double) * (size_t)ft);
+  return hipSuccess;
+}
+
+static void free_plans(std::vector<StokesPlan*>& plans) {
+  for (StokesPlan* p : plans) delete p;
+  plans.clear();
+}
+
+template <typename T>
+hipError_t RaggedStepper<T>::init_ragged(const RigidConfig* cfgs, int nbatch,
+                                         const RigidBuffers& b) {
+  cfg = cfgs[0];
+  batch = nbatch;
+  buf = b;
+  cells = 0;
+  for (int q = 0; q < nbatch; ++q) cells += (int64_t)cfgs[q].nr * cfgs[q].nz;
+  hipError_t e = hipMalloc(&spec, sizeof(double) * (size_t)cells);
+  if (e != hipSuccess) return e;
+  // Radial segment length: AXS_TRIDIAG_LK pins it (its K is ignored: an
+  // uneven launch fills 1024 threads per group), AXS_NO_TUNE takes 8, else
+  // the whole batch's radial pass is timed for each candidate.
+  std::vector<int> cands{8, 4, 16};
+  if (const char* pin = std::getenv("AXS_TRIDIAG_LK")) {
+    int L = 0;
+    if (std::sscanf(pin, "%d", &L) != 1 || (L != 4 && L != 8 && L != 16))
+      return hipErrorInvalidValue;
+    cands = {L};
+  } else if (std::getenv("AXS_NO_TUNE")) {
+    cands = {8};
+  }
+  float best = 1e30f;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (cands.size() > 1) {
+    if ((e = hipMemset(spec, 0, sizeof(double) * (size_t)cells)) != hipSuccess) return e;
+    if ((e = hipEventCreate(&t0)) != hipSuccess) return e;
+    if ((e = hipEventCreate(&t1)) != hipSuccess) return e;
+  }
+  for (int L : cands) {
+    std::vector<StokesPlan*> pl;
+    std::vector<RaggedGroup> gr;
+    int tri = 0;
+    size_t shm = 0;
+    e = ragged_groups(cfgs, nbatch, b, L, pl, gr, tri, shm);
+    if (e == hipErrorInvalidValue && cands.size() > 1) {  // L does not fit this batch
+      free_plans(pl);
+      continue;
+    }
+    if (e != hipSuccess) {
+      free_plans(pl);
+      break;
+    }
+    float ms = 0.f;
+    if (cands.size() > 1) {
+      RaggedGroup* dg = nullptr;
+      e = hipMalloc(reinterpret_cast<void**>(&dg), sizeof(RaggedGroup) * gr.size());
+      if (e == hipSuccess)
+        e = hipMemcpy(dg, gr.data(), sizeof(RaggedGroup) * gr.size(), hipMemcpyHostToDevice);
+      T* sp = static_cast<T*>(spec);
+      for (int rep = 0; rep < 2 && e == hipSuccess; ++rep)
+        e = launch_tridiag_ragged<T>(sp, dg, (int)gr.size(), tri, shm, L, 0);
+      ms = 1e30f;
+      for (int trial = 0; trial < 3 && e == hipSuccess; ++trial) {
+        float t = 0.f;
+        e = hipEventRecord(t0, 0);
+        for (int rep = 0; rep < 5 && e == hipSuccess; ++rep)
+          e = launch_tridiag_ragged<T>(sp, dg, (int)gr.size(), tri, shm, L, 0);
+        if (e == hipSuccess) e = hipEventRecord(t1, 0);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, t0, t1);
+        if (e == hipSuccess && t < ms) ms = t;
+      }
+      if (dg) (void)hipFree(dg);
+      if (e != hipSuccess) {
+        free_plans(pl);
+        break;
+      }
+    }
+    if (groups.empty() || ms < best) {
+      best = ms;
+      free_plans(plans);
+      plans.swap(pl);
+      groups.swap(gr);
+      tri_blocks = tri;
+      tri_shm = shm;
+      tri_L = L;
+    } else {
+      free_plans(pl);
+    }
+  }
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  if (e != hipSuccess && groups.empty()) return e;
+  if (groups.empty()) return hipErrorInvalidValue;
+  rows = vt_tiles = ft_tiles = maxM = 0;
+  for (const RaggedGroup& G : groups) {
+    rows += G.ncase * G.nr;
+    vt_tiles += G.ncase * G.vtiles;
+    ft_tiles += G.ncase * G.ftiles;
+    maxM = std::max(maxM, G.nz / 2);
+  }
+  npartials = ft_tiles;
+  e = hipMalloc(reinterpret_cast<void**>(&partials), sizeof(double) * (size_t)ft_tiles);
   if (e != hipSuccess) return e;
   e = hipMalloc(reinterpret_cast<void**>(&slots), sizeof(double) * kSlotsPerCase * batch);
-  if (e != hipSuccess) return e;
-  e = hipMalloc(&spec, sizeof(double) * (size_t)cells);
   if (e != hipSuccess) return e;
   e = hipMalloc(reinterpret_cast<void**>(&dcfg), sizeof(RigidConfig) * batch);
   if (e != hipSuccess) return e;
@@ -405,7 +499,7 @@ hipError_t RaggedStepper<T>::init_ragged(const RigidConfig* cfgs, int nbatch,
 
 template <typename T>
 RaggedStepper<T>::~RaggedStepper() {
-  for (StokesPlan* p : plans) delete p;
+  free_plans(plans);
   if (dgroups) (void)hipFree(dgroups);
 }
 
@@ -432,7 +526,7 @@ hipError_t RaggedStepper<T>::enqueue_stage(int stage, hipStream_t s) {
       return launch_taper_dct2_ragged<T>(cur, sp, dgroups, ng, batch, rows, w, maxM, &fin, s);
     }
     case 1:
-      return launch_tridiag_ragged<T>(sp, dgroups, ng, tri_blocks, tri_shm, s);
+      return launch_tridiag_ragged<T>(sp, dgroups, ng, tri_blocks, tri_shm, tri_L, s);
     case 2:
       return launch_dct3_ragged<T>(sp, psi, dgroups, ng, rows, maxM, s);
     case 3:

@@ -799,16 +799,15 @@ __global__ void __launch_bounds__(256) dct3_ragged_kernel(const T* __restrict__ 
   dct3_row<T, 1>(src + cs, dst + cs, G.nz, G.fp, G.tw, G.e1, G.e2, i);
 }
 
-template <typename T>
+template <typename T, int L>
 __global__ void __launch_bounds__(1024) tridiag_ragged_kernel(T* __restrict__ d,
                                                              const RaggedGroup* __restrict__ grp,
                                                              int ngrp) {
   const int b = blockIdx.x;
   const RaggedGroup& G = grp[ragged_group<&RaggedGroup::tri0>(grp, ngrp, b)];
   const int q = b - G.tri0, cl = q / G.tchunks, chunk = q % G.tchunks;
-  tridiag_columns_body<T, kRaggedL, false, false>(d + G.cell0 + (int64_t)cl * G.nr * G.nz, G.a,
-                                                  G.b, G.c, G.lam, G.cseg, G.nr, G.nz, G.K, G.P,
-                                                  chunk);
+  tridiag_columns_body<T, L, false, false>(d + G.cell0 + (int64_t)cl * G.nr * G.nz, G.a, G.b,
+                                           G.c, G.lam, G.cseg, G.nr, G.nz, G.K, G.P, chunk);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -958,8 +957,15 @@ hipError_t launch_dct3_ragged(const T* src, T* dst, const RaggedGroup* grp, int 
 
 template <typename T>
 hipError_t launch_tridiag_ragged(T* d, const RaggedGroup* grp, int ngrp, int blocks, size_t shm,
-                                 hipStream_t s) {
-  tridiag_ragged_kernel<T><<<blocks, 1024, shm, s>>>(d, grp, ngrp);
+                                 int L, hipStream_t s) {
+  if (L == 4)
+    tridiag_ragged_kernel<T, 4><<<blocks, 1024, shm, s>>>(d, grp, ngrp);
+  else if (L == 8)
+    tridiag_ragged_kernel<T, 8><<<blocks, 1024, shm, s>>>(d, grp, ngrp);
+  else if (L == 16)
+    tridiag_ragged_kernel<T, 16><<<blocks, 1024, shm, s>>>(d, grp, ngrp);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
@@ -968,7 +974,7 @@ hipError_t launch_tridiag_ragged(T* d, const RaggedGroup* grp, int ngrp, int blo
                                                   int, int, const StepFinalize*, hipStream_t);    \
   template hipError_t launch_dct3_ragged<T>(const T*, T*, const RaggedGroup*, int, int, int,      \
                                             hipStream_t);                                         \
-  template hipError_t launch_tridiag_ragged<T>(T*, const RaggedGroup*, int, int, size_t,          \
+  template hipError_t launch_tridiag_ragged<T>(T*, const RaggedGroup*, int, int, size_t, int,     \
                                                hipStream_t);
 AXS_RAGGED_INST(double)
 AXS_RAGGED_INST(float)

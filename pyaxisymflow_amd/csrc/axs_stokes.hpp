@@ -87,11 +87,11 @@ hipError_t launch_taper_dct2_ragged(const T* src, T* dst, const RaggedGroup* grp
 template <typename T>
 hipError_t launch_dct3_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp, int rows,
                               int maxM, hipStream_t s);
-// Radial solves with segments of kRaggedL rows, K and P per group, one
-// 1024-thread workgroup per K modes of a case.
-constexpr int kRaggedL = 8;
+// Radial solves with segments of L rows (4, 8 or 16; the groups' plans
+// must have been made for that L), K and P per group, one 1024-thread
+// workgroup per K modes of a case.
 template <typename T>
 hipError_t launch_tridiag_ragged(T* d, const RaggedGroup* grp, int ngrp, int blocks, size_t shm,
-                                 hipStream_t s);
+                                 int L, hipStream_t s);
 
 }  // namespace axs

@@ -1,8 +1,8 @@
 """The batched form over uneven grids: cases of several grids advanced by one
 launch per pass (axs_rigid_batch_create with differing nr, nz, dx).
 
-* With the radial solve pinned to the uneven launch's layout (segments of
-  8 rows, as many modes per workgroup as fit 1024 threads), every case's
+* With the radial solve of both pinned to one layout (segments of 8 rows,
+  as many modes per workgroup as fit 1024 threads), every case's
   vorticity and psi equal those of the same cases stepped as one-grid
   batches, bit for bit; the force record differs only by the summation
   order of the per-tile partials (the completion reduction runs with a
@@ -52,6 +52,7 @@ def test_uneven_equals_one_grid_batches(monkeypatch):
     cases = uneven_cases([[0, 3], [1, 4, 6], [2]])
     order = [0, 2, 1, 5, 3, 4]
     mixed = [cases[q] for q in order]
+    monkeypatch.setenv("AXS_TRIDIAG_LK", "8,1")  # segments of 8 rows (K is per group)
     uneven = UnevenBatchStepper(mixed)
     uneven.step(30)
     for g, (nz, nr) in enumerate(GRIDS):
