@@ -59,23 +59,15 @@ __device__ void lsq_gauss_3x2(double A[3][5], double sol[2][3]) {
   }
 }
 
-// One sweep, phase 1: select + fit.  pend[c] = 1 when cell c is extrapolated
-// this sweep (new values in nx/ny), else 0.
-__global__ void lsq_fit_kernel(const int16_t* __restrict__ cur, const int16_t* __restrict__ tgt,
-                               const double* __restrict__ ex, const double* __restrict__ ey,
-                               const double* __restrict__ gx, const double* __restrict__ gy,
-                               int n0, int n1, double* __restrict__ nx, double* __restrict__ ny,
-                               int8_t* __restrict__ pend, unsigned* __restrict__ counter,
-                               const unsigned* __restrict__ prev_counter) {
-  // A sweep after one that fitted nothing does nothing (the reference loop
-  // has ended); its commit sees a zero counter and exits too.
-  if (prev_counter && *prev_counter == 0u) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // column (x)
-  const int j = blockIdx.y;                              // row (y)
-  if (i >= n1) return;
-  const int64_t c = (int64_t)j * n1 + i;
-  pend[c] = 0;
-  if (!(cur[c] ^ tgt[c])) return;
+// Fit of cell (j, i) from the known cells around it
+// (extrapolate_using_least_squares.hpp:190-305); false when no 3x3
+// neighbour is known (the cell waits for a later sweep).
+__device__ __forceinline__ bool lsq_fit_cell(const int16_t* __restrict__ cur,
+                                             const double* __restrict__ ex,
+                                             const double* __restrict__ ey,
+                                             const double* __restrict__ gx,
+                                             const double* __restrict__ gy, int n0, int n1,
+                                             int j, int i, double& vx, double& vy) {
   short local_sum = 0, lx = 0, ly = 0;
   for (int jl = -1; jl <= 1; ++jl)
     for (int il = -1; il <= 1; ++il) {
@@ -84,7 +76,7 @@ __global__ void lsq_fit_kernel(const int16_t* __restrict__ cur, const int16_t* _
       lx += f * short(il + 1);
       ly += f * short(jl + 1);
     }
-  if (!local_sum) return;
+  if (!local_sum) return false;
   const int s_x = i + lsq_start_code(lx, local_sum);
   const int s_y = j + lsq_start_code(ly, local_sum);
   double L0[9], L1[9], L2[9], r0[9], r1[9];
@@ -125,12 +117,66 @@ __global__ void lsq_fit_kernel(const int16_t* __restrict__ cur, const int16_t* _
   const double E[3] = {1.0, 1.0 * px, 1.0 * py};
   double e = 0.0;
   for (int q = 0; q < 3; ++q) e += sol[0][q] * E[q];
-  nx[c] = e;
+  vx = e;
   e = 0.0;
   for (int q = 0; q < 3; ++q) e += sol[1][q] * E[q];
-  ny[c] = e;
+  vy = e;
+  return true;
+}
+
+// One sweep, phase 1: select + fit.  pend[c] = 1 when cell c is extrapolated
+// this sweep (new values in nx/ny), else 0.
+__global__ void lsq_fit_kernel(const int16_t* __restrict__ cur, const int16_t* __restrict__ tgt,
+                               const double* __restrict__ ex, const double* __restrict__ ey,
+                               const double* __restrict__ gx, const double* __restrict__ gy,
+                               int n0, int n1, double* __restrict__ nx, double* __restrict__ ny,
+                               int8_t* __restrict__ pend, unsigned* __restrict__ counter,
+                               const unsigned* __restrict__ prev_counter) {
+  // A sweep after one that fitted nothing does nothing (the reference loop
+  // has ended); its commit sees a zero counter and exits too.
+  if (prev_counter && *prev_counter == 0u) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // column (x)
+  const int j = blockIdx.y;                              // row (y)
+  if (i >= n1) return;
+  const int64_t c = (int64_t)j * n1 + i;
+  pend[c] = 0;
+  if (!(cur[c] ^ tgt[c])) return;
+  double vx, vy;
+  if (!lsq_fit_cell(cur, ex, ey, gx, gy, n0, n1, j, i, vx, vy)) return;
+  nx[c] = vx;
+  ny[c] = vy;
   pend[c] = 1;
   atomicAdd(counter, 1u);
+}
+
+// The same sweep over a list of cells (the ones with current ^ target set
+// when the list was made; entries that became known are skipped by the same
+// test).  pend is indexed by list entry.
+__global__ void lsq_fit_list_kernel(const int16_t* __restrict__ cur,
+                                    const int16_t* __restrict__ tgt,
+                                    const double* __restrict__ ex, const double* __restrict__ ey,
+                                    const double* __restrict__ gx, const double* __restrict__ gy,
+                                    int n0, int n1, const int* __restrict__ list,
+                                    const unsigned* __restrict__ count, double* __restrict__ nx,
+                                    double* __restrict__ ny, int8_t* __restrict__ pend,
+                                    unsigned* __restrict__ counter,
+                                    const unsigned* __restrict__ prev_counter) {
+  if (prev_counter && *prev_counter == 0u) return;
+  const int nl = (int)*count;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nl; k += gridDim.x * blockDim.x) {
+    const int c = list[k];
+    int8_t p = 0;
+    if (cur[c] ^ tgt[c]) {
+      double vx, vy;
+      if (lsq_fit_cell(cur, ex, ey, gx, gy, n0, n1, c / n1, c % n1, vx, vy)) {
+        nx[c] = vx;
+        ny[c] = vy;
+        p = 1;
+        atomicAdd(counter, 1u);
+      }
+    }
+    pend[k] = p;
+  }
 }
 
 // Phase 2: commit values and flags of the cells fitted in phase 1.
@@ -144,6 +190,32 @@ __global__ void lsq_commit_kernel(int16_t* __restrict__ cur, double* __restrict_
   ex[c] = nx[c];
   ey[c] = ny[c];
   cur[c] = 1;
+}
+
+__global__ void lsq_commit_list_kernel(int16_t* __restrict__ cur, double* __restrict__ ex,
+                                       double* __restrict__ ey, const double* __restrict__ nx,
+                                       const double* __restrict__ ny,
+                                       const int8_t* __restrict__ pend,
+                                       const int* __restrict__ list,
+                                       const unsigned* __restrict__ count,
+                                       const unsigned* __restrict__ counter) {
+  if (*counter == 0u) return;
+  const int nl = (int)*count;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nl; k += gridDim.x * blockDim.x) {
+    if (!pend[k]) continue;
+    const int c = list[k];
+    ex[c] = nx[c];
+    ey[c] = ny[c];
+    cur[c] = 1;
+  }
+}
+
+// Cells with current ^ target set, in any order (the set is what matters:
+// a sweep's fits read only the previous sweep's state); *count must be zero.
+__global__ void lsq_list_kernel(const int16_t* __restrict__ cur, const int16_t* __restrict__ tgt,
+                                int64_t n, int* __restrict__ list, unsigned* __restrict__ count) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n && (cur[c] ^ tgt[c])) list[atomicAdd(count, 1u)] = (int)c;
 }
 
 __global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
@@ -163,6 +235,26 @@ hipError_t launch_lsq_sweep(int16_t* cur, const int16_t* tgt, double* ex, double
   const int64_t n = (int64_t)n0 * n1;
   lsq_commit_kernel<<<ceil_div(n, 256), 256, 0, s>>>(cur, ex, ey, w.nx, w.ny, w.pend, n,
                                                       w.counters + sweep);
+  return hipGetLastError();
+}
+
+hipError_t launch_lsq_list(const int16_t* cur, const int16_t* tgt, int64_t n, LsqWork w,
+                           hipStream_t s) {
+  lsq_list_kernel<<<ceil_div(n, 256), 256, 0, s>>>(cur, tgt, n, w.list, w.count);
+  return hipGetLastError();
+}
+
+hipError_t launch_lsq_sweep_list(int16_t* cur, const int16_t* tgt, double* ex, double* ey,
+                                 const double* gx, const double* gy, int n0, int n1, LsqWork w,
+                                 int sweep, hipStream_t s) {
+  constexpr int kBlocks = 128;  // the band's cells, strided (a few thousand here)
+  lsq_fit_list_kernel<<<kBlocks, 256, 0, s>>>(cur, tgt, ex, ey, gx, gy, n0, n1, w.list, w.count,
+                                              w.nx, w.ny, w.pend, w.counters + sweep,
+                                              sweep > 0 ? w.counters + sweep - 1 : nullptr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  lsq_commit_list_kernel<<<kBlocks, 256, 0, s>>>(cur, ex, ey, w.nx, w.ny, w.pend, w.list, w.count,
+                                                 w.counters + sweep);
   return hipGetLastError();
 }
 

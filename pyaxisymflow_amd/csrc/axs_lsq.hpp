@@ -11,6 +11,8 @@ struct LsqWork {
   double* ny;          // fitted eta_y of this sweep, (n0, n1)
   int8_t* pend;        // 1 where a cell was fitted this sweep
   unsigned* counters;  // cells fitted per sweep (one slot per sweep)
+  int* list;           // cells with current ^ target set (list form), (n0, n1) ints
+  unsigned* count;     // entries of list
 };
 
 // One sweep (fit + commit).  Sweep `sweep` exits at once when
@@ -18,6 +20,15 @@ struct LsqWork {
 hipError_t launch_lsq_sweep(int16_t* cur, const int16_t* tgt, double* ex, double* ey,
                             const double* gx, const double* gy, int n0, int n1, LsqWork w,
                             int sweep, hipStream_t s);
+// List form: launch_lsq_list collects the cells with current ^ target set
+// (*w.count must be zero; the caller clears it after the last sweep), then
+// each launch_lsq_sweep_list sweeps only those cells, with the same results
+// as launch_lsq_sweep on the whole grid.
+hipError_t launch_lsq_list(const int16_t* cur, const int16_t* tgt, int64_t n, LsqWork w,
+                           hipStream_t s);
+hipError_t launch_lsq_sweep_list(int16_t* cur, const int16_t* tgt, double* ex, double* ey,
+                                 const double* gx, const double* gy, int n0, int n1, LsqWork w,
+                                 int sweep, hipStream_t s);
 hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s);
 
 }  // namespace axs

@@ -20,7 +20,8 @@
 //                     (extrapolate_eta_using_least_squares_unb.py:19-24,
 //                     extrapolate_using_least_squares.py:33-35)
 //   lsq_sweeps        least-squares extrapolation sweeps until a sweep
-//                     fits nothing (later sweeps exit at once)
+//                     fits nothing (later sweeps exit at once), over a
+//                     list of the cells to fill made once per step
 //   lsq_back          eta = doubled eta's physical half (:29-30)
 //   solid_sigma       stresses (solid_sigma.py:4-29), blended by chi (:197-199)
 //   div_tau           omega += dt curl(div sigma) (div_tau.py:4-34)
@@ -164,8 +165,10 @@ __global__ void soft_blend_kernel(double* __restrict__ s11, double* __restrict__
 }
 
 __global__ void soft_finalize_kernel(double* __restrict__ sc,
-                                     const unsigned* __restrict__ counters, int sweeps) {
+                                     const unsigned* __restrict__ counters, int sweeps,
+                                     unsigned* __restrict__ list_count) {
   if (threadIdx.x != 0) return;
+  *list_count = 0u;  // for the next step's list of cells to extrapolate
   int used = 0;
   while (used < sweeps && counters[used] != 0u) ++used;
   sc[kSsLsqSweeps] = double(used);
@@ -245,6 +248,9 @@ struct SoftStepper {
     if ((e = alloc(&lw.ny, nd * B)) != hipSuccess) return e;
     if ((e = alloc(&lw.pend, nd)) != hipSuccess) return e;
     if ((e = alloc(&lw.counters, sizeof(unsigned) * cfg.lsq_sweeps)) != hipSuccess) return e;
+    if ((e = alloc(&lw.list, nd * sizeof(int))) != hipSuccess) return e;
+    if ((e = alloc(&lw.count, sizeof(unsigned))) != hipSuccess) return e;
+    if ((e = hipMemset(lw.count, 0, sizeof(unsigned))) != hipSuccess) return e;
     if ((e = alloc(&slots, B * kSlotsPerCase)) != hipSuccess) return e;
     // scratch the reference allocates once and never fully rewrites
     // (gradient and tau scratch keep zeros where they are never written)
@@ -303,8 +309,10 @@ struct SoftStepper {
         buf.phi, buf.char_func, buf.eta1, buf.eta2, e1d, e2d, cur, tgt, nr, nz, cfg.extrap_zone);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_fill_u32(lw.counters, cfg.lsq_sweeps, 0u, s));
+    AXS_TRY_STEP(launch_lsq_list(cur, tgt, 2 * n, lw, s));
     for (int q = 0; q < cfg.lsq_sweeps; ++q)
-      AXS_TRY_STEP(launch_lsq_sweep(cur, tgt, e1d, e2d, buf.z, buf.z, 2 * nr, nz, lw, q, s));
+      AXS_TRY_STEP(
+          launch_lsq_sweep_list(cur, tgt, e1d, e2d, buf.z, buf.z, 2 * nr, nz, lw, q, s));
     soft_lsq_back_kernel<<<ceil_div(n, 256), 256, 0, s>>>(buf.eta1, buf.eta2, e1d, e2d, n, n);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_solid_sigma<double>(s11, s12, s22, cfg.G, cfg.dx, buf.eta1, buf.eta2, g1z,
@@ -318,7 +326,7 @@ struct SoftStepper {
     AXS_TRY_STEP(launch_rk2<double, RRow<double>>(
         w1, tmp, buf.vorticity, RRow<double>{buf.r}, nr, nz, Scalar<double>{0.0, sc + kSsHalfNuDt},
         Scalar<double>{0.0, sc + kSsNuDt}, cfg.dx, s));
-    soft_finalize_kernel<<<1, 64, 0, s>>>(sc, lw.counters, cfg.lsq_sweeps);
+    soft_finalize_kernel<<<1, 64, 0, s>>>(sc, lw.counters, cfg.lsq_sweeps, lw.count);
     AXS_TRY_LAUNCH();
 #undef AXS_TRY_STEP
 #undef AXS_TRY_LAUNCH
