@@ -285,7 +285,7 @@ axs_status axs_rigid_stepper_reset_scalars(axs_rigid_stepper* st, void* stream);
 typedef struct axs_soft_stepper axs_soft_stepper;
 typedef struct {
   int nr, nz, kill_width;
-  int lsq_sweeps;        /* upper bound on extrapolation sweeps per step */
+  int lsq_sweeps;        /* upper bound on extrapolation sweeps per step (rounded up to even) */
   int reinit_iterations; /* < 0: 2 ceil(reinit_band / dx) + 8 */
   int pad;
   double dx, nu, G, rho_f, CFL, eps;

@@ -218,6 +218,32 @@ __global__ void lsq_list_kernel(const int16_t* __restrict__ cur, const int16_t* 
   if (c < n && (cur[c] ^ tgt[c])) list[atomicAdd(count, 1u)] = (int)c;
 }
 
+__global__ void lsq_sweep_pingpong_kernel(
+    const int16_t* __restrict__ cur_in, const double* __restrict__ ex_in,
+    const double* __restrict__ ey_in, int16_t* __restrict__ cur_out, double* __restrict__ ex_out,
+    double* __restrict__ ey_out, const int16_t* __restrict__ tgt, const double* __restrict__ gx,
+    const double* __restrict__ gy, int n0, int n1, const int* __restrict__ list,
+    const unsigned* __restrict__ count, unsigned* __restrict__ counter,
+    const unsigned* __restrict__ prev_counter) {
+  const bool ended = prev_counter && *prev_counter == 0u;  // the reference loop has ended
+  const int nl = (int)*count;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nl; k += gridDim.x * blockDim.x) {
+    const int c = list[k];
+    double vx, vy;
+    if (!ended && (cur_in[c] ^ tgt[c]) &&
+        lsq_fit_cell(cur_in, ex_in, ey_in, gx, gy, n0, n1, c / n1, c % n1, vx, vy)) {
+      ex_out[c] = vx;
+      ey_out[c] = vy;
+      cur_out[c] = 1;
+      atomicAdd(counter, 1u);
+    } else {
+      ex_out[c] = ex_in[c];
+      ey_out[c] = ey_in[c];
+      cur_out[c] = cur_in[c];
+    }
+  }
+}
+
 __global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = v;
@@ -255,6 +281,18 @@ hipError_t launch_lsq_sweep_list(int16_t* cur, const int16_t* tgt, double* ex, d
   if (e != hipSuccess) return e;
   lsq_commit_list_kernel<<<kBlocks, 256, 0, s>>>(cur, ex, ey, w.nx, w.ny, w.pend, w.list, w.count,
                                                  w.counters + sweep);
+  return hipGetLastError();
+}
+
+hipError_t launch_lsq_sweep_pingpong(const int16_t* cur_in, const double* ex_in,
+                                     const double* ey_in, int16_t* cur_out, double* ex_out,
+                                     double* ey_out, const int16_t* tgt, const double* gx,
+                                     const double* gy, int n0, int n1, LsqWork w, int sweep,
+                                     hipStream_t s) {
+  constexpr int kBlocks = 128;
+  lsq_sweep_pingpong_kernel<<<kBlocks, 256, 0, s>>>(
+      cur_in, ex_in, ey_in, cur_out, ex_out, ey_out, tgt, gx, gy, n0, n1, w.list, w.count,
+      w.counters + sweep, sweep > 0 ? w.counters + sweep - 1 : nullptr);
   return hipGetLastError();
 }
 

@@ -13,6 +13,9 @@ struct LsqWork {
   unsigned* counters;  // cells fitted per sweep (one slot per sweep)
   int* list;           // cells with current ^ target set (list form), (n0, n1) ints
   unsigned* count;     // entries of list
+  int16_t* cur2;       // second flag and value buffers of the ping-pong sweeps
+  double* ex2;
+  double* ey2;
 };
 
 // One sweep (fit + commit).  Sweep `sweep` exits at once when
@@ -29,6 +32,15 @@ hipError_t launch_lsq_list(const int16_t* cur, const int16_t* tgt, int64_t n, Ls
 hipError_t launch_lsq_sweep_list(int16_t* cur, const int16_t* tgt, double* ex, double* ey,
                                  const double* gx, const double* gy, int n0, int n1, LsqWork w,
                                  int sweep, hipStream_t s);
+// One sweep over the listed cells in one launch: reads flags and values from
+// (cur_in, ex_in, ey_in), writes (cur_out, ex_out, ey_out) for every listed
+// cell (fitted, or copied through); cells off the list must already be equal
+// in both sets.  After the loop has ended a sweep only copies.
+hipError_t launch_lsq_sweep_pingpong(const int16_t* cur_in, const double* ex_in,
+                                     const double* ey_in, int16_t* cur_out, double* ex_out,
+                                     double* ey_out, const int16_t* tgt, const double* gx,
+                                     const double* gy, int n0, int n1, LsqWork w, int sweep,
+                                     hipStream_t s);
 hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s);
 
 }  // namespace axs

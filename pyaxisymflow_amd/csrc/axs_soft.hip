@@ -20,8 +20,10 @@
 //                     (extrapolate_eta_using_least_squares_unb.py:19-24,
 //                     extrapolate_using_least_squares.py:33-35)
 //   lsq_sweeps        least-squares extrapolation sweeps until a sweep
-//                     fits nothing (later sweeps exit at once), over a
-//                     list of the cells to fill made once per step
+//                     fits nothing (later sweeps only copy), over a list
+//                     of the cells to fill made once per step; each sweep
+//                     is one launch reading one buffer set and writing the
+//                     other
 //   lsq_back          eta = doubled eta's physical half (:29-30)
 //   solid_sigma       stresses (solid_sigma.py:4-29), blended by chi (:197-199)
 //   div_tau           omega += dt curl(div sigma) (div_tau.py:4-34)
@@ -125,8 +127,9 @@ __global__ void soft_lsq_mirror_kernel(const double* __restrict__ phi,
                                        const double* __restrict__ e1,
                                        const double* __restrict__ e2, double* __restrict__ e1d,
                                        double* __restrict__ e2d, int16_t* __restrict__ cur,
-                                       int16_t* __restrict__ tgt, int nr, int nz,
-                                       double extrap_zone) {
+                                       int16_t* __restrict__ tgt, double* __restrict__ e1d2,
+                                       double* __restrict__ e2d2, int16_t* __restrict__ cur2,
+                                       int nr, int nz, double extrap_zone) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int I = blockIdx.y;  // doubled row, [0, 2 nr)
   if (j >= nz) return;
@@ -138,10 +141,16 @@ __global__ void soft_lsq_mirror_kernel(const double* __restrict__ phi,
   const double a = in * e1[c];
   const double b = in * e2[c];
   const int64_t d = (int64_t)I * nz + j;
+  const int16_t known = (pd < 0.0) ? 1 : 0;
   e1d[d] = a;
   e2d[d] = upper ? b : -b;
-  cur[d] = (pd < 0.0) ? 1 : 0;
+  cur[d] = known;
   tgt[d] = (pd < extrap_zone) ? 1 : 0;
+  // the second buffers of the ping-pong sweeps (cells off the list never
+  // change, so both copies must hold them)
+  e1d2[d] = a;
+  e2d2[d] = upper ? b : -b;
+  cur2[d] = known;
 }
 
 __global__ void soft_lsq_back_kernel(double* __restrict__ e1, double* __restrict__ e2,
@@ -220,6 +229,8 @@ struct SoftStepper {
   hipError_t init(const SoftConfig& c, const SoftBuffers& b) {
     cfg = c;
     buf = b;
+    // the ping-pong sweeps end in the first buffer set after an even count
+    cfg.lsq_sweeps += cfg.lsq_sweeps & 1;
     plan = new (std::nothrow) StokesPlan();
     if (!plan) return hipErrorOutOfMemory;
     FftPlanArgs fp{};
@@ -249,6 +260,9 @@ struct SoftStepper {
     if ((e = alloc(&lw.pend, nd)) != hipSuccess) return e;
     if ((e = alloc(&lw.counters, sizeof(unsigned) * cfg.lsq_sweeps)) != hipSuccess) return e;
     if ((e = alloc(&lw.list, nd * sizeof(int))) != hipSuccess) return e;
+    if ((e = alloc(&lw.cur2, nd * sizeof(int16_t))) != hipSuccess) return e;
+    if ((e = alloc(&lw.ex2, nd * B)) != hipSuccess) return e;
+    if ((e = alloc(&lw.ey2, nd * B)) != hipSuccess) return e;
     if ((e = alloc(&lw.count, sizeof(unsigned))) != hipSuccess) return e;
     if ((e = hipMemset(lw.count, 0, sizeof(unsigned))) != hipSuccess) return e;
     if ((e = alloc(&slots, B * kSlotsPerCase)) != hipSuccess) return e;
@@ -306,13 +320,19 @@ struct SoftStepper {
                                cfg.reinit_iterations, rw, s));
     AXS_TRY_STEP(launch_heaviside<double>(buf.char_func, buf.phi, cfg.moll_zone, n, s));
     soft_lsq_mirror_kernel<<<dim3(ceil_div(nz, 256), 2 * nr), 256, 0, s>>>(
-        buf.phi, buf.char_func, buf.eta1, buf.eta2, e1d, e2d, cur, tgt, nr, nz, cfg.extrap_zone);
+        buf.phi, buf.char_func, buf.eta1, buf.eta2, e1d, e2d, cur, tgt, lw.ex2, lw.ey2, lw.cur2, nr,
+        nz, cfg.extrap_zone);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_fill_u32(lw.counters, cfg.lsq_sweeps, 0u, s));
     AXS_TRY_STEP(launch_lsq_list(cur, tgt, 2 * n, lw, s));
-    for (int q = 0; q < cfg.lsq_sweeps; ++q)
-      AXS_TRY_STEP(
-          launch_lsq_sweep_list(cur, tgt, e1d, e2d, buf.z, buf.z, 2 * nr, nz, lw, q, s));
+    // ping-pong sweeps; the bound is even, so the last one lands in cur, e1d, e2d
+    for (int q = 0; q < cfg.lsq_sweeps; ++q) {
+      const bool odd = q & 1;
+      AXS_TRY_STEP(launch_lsq_sweep_pingpong(odd ? lw.cur2 : cur, odd ? lw.ex2 : e1d,
+                                             odd ? lw.ey2 : e2d, odd ? cur : lw.cur2,
+                                             odd ? e1d : lw.ex2, odd ? e2d : lw.ey2, tgt, buf.z,
+                                             buf.z, 2 * nr, nz, lw, q, s));
+    }
     soft_lsq_back_kernel<<<ceil_div(n, 256), 256, 0, s>>>(buf.eta1, buf.eta2, e1d, e2d, n, n);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_solid_sigma<double>(s11, s12, s22, cfg.G, cfg.dx, buf.eta1, buf.eta2, g1z,
