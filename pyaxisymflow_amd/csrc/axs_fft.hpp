@@ -86,13 +86,14 @@ __device__ __forceinline__ void dft_small(Cplx<T>* a, int sgn) {
 // tw[t] = e^{-2 pi i t/M}; the inverse (sgn = +1) uses its conjugate.
 template <typename T, int R, int PER>
 __device__ __forceinline__ void stockham_stage(Cplx<T>* buf, int M, int Ns,
-                                               const Cplx<double>* __restrict__ tw, int sgn) {
+                                               const Cplx<double>* __restrict__ tw, int sgn,
+                                               int tid, int nth) {
   const int nb = M / R;
   const int tstride = M / (Ns * R);
   Cplx<T> v[PER][R];
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
-    const int j = threadIdx.x + q * blockDim.x;
+    const int j = tid + q * nth;
     if (j < nb) {
       const int jm = j % Ns;
 #pragma unroll
@@ -111,7 +112,7 @@ __device__ __forceinline__ void stockham_stage(Cplx<T>* buf, int M, int Ns,
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
-    const int j = threadIdx.x + q * blockDim.x;
+    const int j = tid + q * nth;
     if (j < nb) {
       const int jm = j % Ns;
       const int idxD = (j / Ns) * Ns * R + jm;
@@ -122,20 +123,23 @@ __device__ __forceinline__ void stockham_stage(Cplx<T>* buf, int M, int Ns,
   __syncthreads();
 }
 
-// Full FFT of length p.M in LDS.  Requires p.M / min(radix) <= PER * blockDim.
+// Full FFT of length p.M in LDS by nth threads (this one: tid).  Requires
+// p.M / min(radix) <= PER * nth.  Every thread of the block must call it
+// (stages end in block barriers).
 template <typename T, int PER>
-__device__ void fft_lds(Cplx<T>* buf, const FftPlanArgs& p, const Cplx<double>* tw, int sgn) {
+__device__ void fft_lds(Cplx<T>* buf, const FftPlanArgs& p, const Cplx<double>* tw, int sgn,
+                        int tid, int nth) {
   int Ns = 1;
   for (int s = 0; s < p.nstages; ++s) {
     const int R = p.radix[s];
     if (R == 4)
-      stockham_stage<T, 4, PER>(buf, p.M, Ns, tw, sgn);
+      stockham_stage<T, 4, PER>(buf, p.M, Ns, tw, sgn, tid, nth);
     else if (R == 2)
-      stockham_stage<T, 2, PER>(buf, p.M, Ns, tw, sgn);
+      stockham_stage<T, 2, PER>(buf, p.M, Ns, tw, sgn, tid, nth);
     else if (R == 3)
-      stockham_stage<T, 3, PER>(buf, p.M, Ns, tw, sgn);
+      stockham_stage<T, 3, PER>(buf, p.M, Ns, tw, sgn, tid, nth);
     else
-      stockham_stage<T, 5, PER>(buf, p.M, Ns, tw, sgn);
+      stockham_stage<T, 5, PER>(buf, p.M, Ns, tw, sgn, tid, nth);
     Ns *= R;
   }
 }
@@ -156,7 +160,7 @@ __device__ __forceinline__ T& packed_component(Cplx<T>* z, int n) {
 }
 
 // Forward DCT-II post-processing from Z (in LDS, natural order) to X (out).
-// Thread handles k = threadIdx.x + q * blockDim.x (q < KQ, k <= M/2) and its
+// Thread tid of nth handles k = tid + q * nth (q < KQ, k <= M/2) and its
 // partner M - k.  The twiddles e1[k] = e^{-2 pi i k/N} and
 // e2[k] = e^{-i pi k/(2N)} are fetched by dct2_post_twiddles before the FFT.
 template <int KQ>
@@ -167,10 +171,11 @@ struct PostTwiddles {
 template <int KQ>
 __device__ __forceinline__ void dct2_post_twiddles(PostTwiddles<KQ>& t, int M,
                                                    const Cplx<double>* __restrict__ e1,
-                                                   const Cplx<double>* __restrict__ e2) {
+                                                   const Cplx<double>* __restrict__ e2, int tid,
+                                                   int nth) {
 #pragma unroll
   for (int q = 0; q < KQ; ++q) {
-    const int k = threadIdx.x + q * blockDim.x;
+    const int k = tid + q * nth;
     if (k <= M / 2) {
       t.e1k[q] = e1[k];
       t.e1p[q] = e1[M - k];
@@ -182,11 +187,11 @@ __device__ __forceinline__ void dct2_post_twiddles(PostTwiddles<KQ>& t, int M,
 
 template <typename T, int KQ, typename Out>
 __device__ __forceinline__ void dct2_post(const Cplx<T>* Z, int M, const PostTwiddles<KQ>& t,
-                                          Out out) {
+                                          Out out, int tid, int nth) {
   const int N = 2 * M;
 #pragma unroll
   for (int q = 0; q < KQ; ++q) {
-    const int k = threadIdx.x + q * blockDim.x;
+    const int k = tid + q * nth;
     if (k <= M / 2) {
       const int kp = M - k;  // partner
       const Cplx<T> zk = Z[k % M], zp = Z[kp % M];
@@ -212,17 +217,18 @@ __device__ __forceinline__ void dct2_post(const Cplx<T>* Z, int M, const PostTwi
 // Inverse pre-processing: Z_k = A_k + i B_k from X (read via `in`), k in [0, M).
 //   V_k = e^{+i pi k/(2N)} (X_k - i X_{N-k}),  V_{k+M} = conj(V_{M-k})
 //   A_k = V_k + V_{k+M},  B_k = (V_k - V_{k+M}) e^{+2 pi i k/N}
-// Thread handles k = threadIdx.x + q * blockDim.x, q < KQ.  All global reads
+// Thread tid of nth handles k = tid + q * nth, q < KQ.  All global reads
 // (four X values and three twiddles per k) are issued before any arithmetic.
 template <typename T, int KQ, typename In>
 __device__ __forceinline__ void dct3_pre(Cplx<T>* Z, int M, const Cplx<double>* __restrict__ e1,
-                                         const Cplx<double>* __restrict__ e2, In in) {
+                                         const Cplx<double>* __restrict__ e2, In in, int tid,
+                                         int nth) {
   const int N = 2 * M;
   T xk[KQ], xnk[KQ], xm[KQ], xnm[KQ];
   Cplx<double> q2k[KQ], q2m[KQ], q1k[KQ];
 #pragma unroll
   for (int q = 0; q < KQ; ++q) {
-    const int k = threadIdx.x + q * blockDim.x;
+    const int k = tid + q * nth;
     if (k < M) {
       const int mk = M - k;  // in [1, M]
       xk[q] = in(k);
@@ -236,7 +242,7 @@ __device__ __forceinline__ void dct3_pre(Cplx<T>* Z, int M, const Cplx<double>* 
   }
 #pragma unroll
   for (int q = 0; q < KQ; ++q) {
-    const int k = threadIdx.x + q * blockDim.x;
+    const int k = tid + q * nth;
     if (k < M) {
       const Cplx<T> Vk = cmul(Cplx<T>{T(q2k[q].x), T(-q2k[q].y)}, Cplx<T>{xk[q], -xnk[q]});
       const Cplx<T> Vm = cmul(Cplx<T>{T(q2m[q].x), T(-q2m[q].y)}, Cplx<T>{xm[q], -xnm[q]});

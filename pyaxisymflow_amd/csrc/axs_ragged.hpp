@@ -18,7 +18,8 @@ constexpr int kMaxRaggedGroups = 16;
 
 struct RaggedGroup {
   int nr, nz, ncase, case0;  // grid, cases, global index of the first case
-  int rows0;                 // first row of the group in the row passes
+  int rb0, rpb, bs;          // row passes: first workgroup of the group, rows per
+                             // 256-thread workgroup, threads per row (rpb bs = 256)
   int tri0, K, P, tchunks;   // radial solve: first workgroup, modes per workgroup,
                              // segments, workgroups per case
   int vt0, vz, vtiles;       // velocity_max: first tile, tiles along z, tiles per case

@@ -338,7 +338,16 @@ static hipError_t ragged_groups(const RigidConfig* cfgs, int nbatch, const Rigid
     G.nz = nz;
     G.ncase = nc;
     G.case0 = q;
-    G.rows0 = row;
+    // rows of a 256-thread workgroup: bs threads per row cover every stage
+    // of the row transform (bs >= M / 2)
+    G.bs = 64;
+    while (G.bs < nz / 4 && G.bs < 256) G.bs *= 2;
+    G.rpb = 256 / G.bs;
+    while (G.rpb > 1 && nr % G.rpb) {
+      G.rpb /= 2;
+      G.bs *= 2;
+    }
+    G.rb0 = row;
     G.K = pl->tri_K;
     G.P = pl->tri_P;
     G.tchunks = (int)ceil_div(nz, G.K);
@@ -365,7 +374,7 @@ static hipError_t ragged_groups(const RigidConfig* cfgs, int nbatch, const Rigid
     G.r = b.r + r_off;  // the group's first case's tables
     G.kill_wz = b.kill_wz + 2 * w_off;
     G.kill_wr = b.kill_wr + w_off;
-    row += nc * nr;
+    row += nc * nr / G.rpb;
     tri += nc * G.tchunks;
     vt += nc * G.vtiles;
     ft += nc * G.ftiles;
@@ -472,10 +481,10 @@ hipError_t RaggedStepper<T>::init_ragged(const RigidConfig* cfgs, int nbatch,
   if (groups.empty()) return hipErrorInvalidValue;
   rows = vt_tiles = ft_tiles = maxM = 0;
   for (const RaggedGroup& G : groups) {
-    rows += G.ncase * G.nr;
+    rows += G.ncase * G.nr / G.rpb;  // row-pass workgroups
     vt_tiles += G.ncase * G.vtiles;
     ft_tiles += G.ncase * G.ftiles;
-    maxM = std::max(maxM, G.nz / 2);
+    maxM = std::max(maxM, G.rpb * (G.nz / 2));  // LDS complex entries per workgroup
   }
   npartials = ft_tiles;
   e = hipMalloc(reinterpret_cast<void**>(&partials), sizeof(double) * (size_t)ft_tiles);

@@ -289,16 +289,16 @@ __global__ void __launch_bounds__(256) dct2_rows_kernel(
     if (j < nz) v[q] = row[j];
   }
   PostTwiddles<PER + 1> pt;
-  dct2_post_twiddles<PER + 1>(pt, M, e1, e2);
+  dct2_post_twiddles<PER + 1>(pt, M, e1, e2, threadIdx.x, blockDim.x);
 #pragma unroll
   for (int q = 0; q < NJ; ++q) {
     const int j = threadIdx.x + q * blockDim.x;
     if (j < nz) packed_component(buf, dct_dst_index(j, nz)) = s * v[q];
   }
   __syncthreads();
-  fft_lds<T, PER>(buf, fp, tw, -1);
+  fft_lds<T, PER>(buf, fp, tw, -1, threadIdx.x, blockDim.x);
   T* out = dst + (int64_t)i * nz;
-  dct2_post<T, PER + 1>(buf, M, pt, [&](int k, T val) { out[k] = val; });
+  dct2_post<T, PER + 1>(buf, M, pt, [&](int k, T val) { out[k] = val; }, threadIdx.x, blockDim.x);
 }
 
 // Fused first pass of a step: the reference's kill taper
@@ -309,34 +309,34 @@ __global__ void __launch_bounds__(256) dct2_rows_kernel(
 // src and writing a separate buffer keeps rows independent (the top rows
 // read row nr-w).  wz: 2w weights (left block, right block), wr: w weights.
 // One row of the tapered forward pass (src, tapered, dst at the case).
+// Threads tid of nth (all of them call it: the FFT has block barriers)
+// work in buf, M complex entries of LDS.
 template <typename T, int PER>
 __device__ __forceinline__ void taper_dct2_row(
     const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
     const double* __restrict__ rscale, const double* __restrict__ wz,
     const double* __restrict__ wr, int nr, int nz, int w, const FftPlanArgs& fp,
     const Cplx<double>* __restrict__ tw, const Cplx<double>* __restrict__ e1,
-    const Cplx<double>* __restrict__ e2, int i) {
-  extern __shared__ unsigned char smem_raw[];
+    const Cplx<double>* __restrict__ e2, int i, Cplx<T>* buf, int tid, int nth) {
   const int M = fp.M;
-  Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int srow = (i >= nr - w) ? nr - w : i;
   const T* row = src + (int64_t)srow * nz;
   constexpr int NJ = 4 * PER;
   T v[NJ];
 #pragma unroll
   for (int q = 0; q < NJ; ++q) {
-    const int j = threadIdx.x + q * blockDim.x;
+    const int j = tid + q * nth;
     if (j < nz) v[q] = row[j];
   }
   const T left = row[w - 1], right = row[nz - w];
   PostTwiddles<PER + 1> pt;
-  dct2_post_twiddles<PER + 1>(pt, M, e1, e2);
+  dct2_post_twiddles<PER + 1>(pt, M, e1, e2, tid, nth);
   const T rs = T(rscale[i]);
   const T rowf = (i == 0) ? T(0) : ((i >= nr - w) ? T(wr[i - (nr - w)]) : T(1));
   T* out_t = tapered ? tapered + (int64_t)i * nz : nullptr;
 #pragma unroll
   for (int q = 0; q < NJ; ++q) {
-    const int j = threadIdx.x + q * blockDim.x;
+    const int j = tid + q * nth;
     if (j < nz) {
       T t = v[q];
       if (j < w)
@@ -350,9 +350,9 @@ __device__ __forceinline__ void taper_dct2_row(
     }
   }
   __syncthreads();
-  fft_lds<T, PER>(buf, fp, tw, -1);
+  fft_lds<T, PER>(buf, fp, tw, -1, tid, nth);
   T* out = dst + (int64_t)i * nz;
-  dct2_post<T, PER + 1>(buf, M, pt, [&](int k, T val) { out[k] = val; });
+  dct2_post<T, PER + 1>(buf, M, pt, [&](int k, T val) { out[k] = val; }, tid, nth);
 }
 
 // Row pass: x = DCT-II^{-1}(src[i, :]) -> dst[i, :], stored coalesced by
@@ -375,8 +375,10 @@ __global__ void __launch_bounds__(256) taper_dct2_rows_kernel(
     return;
   }
   const int64_t cs = (int64_t)blockIdx.y * nr * nz;  // case of a batch
+  extern __shared__ unsigned char smem_raw[];
   taper_dct2_row<T, PER>(src + cs, tapered ? tapered + cs : nullptr, dst + cs, rscale, wz, wr, nr,
-                         nz, w, fp, tw, e1, e2, blockIdx.x);
+                         nz, w, fp, tw, e1, e2, blockIdx.x,
+                         reinterpret_cast<Cplx<T>*>(smem_raw), threadIdx.x, blockDim.x);
 }
 // One row of the inverse pass (src, dst at the case).
 template <typename T, int PER>
@@ -384,21 +386,20 @@ __device__ __forceinline__ void dct3_row(const T* __restrict__ src, T* __restric
                                          const FftPlanArgs& fp,
                                          const Cplx<double>* __restrict__ tw,
                                          const Cplx<double>* __restrict__ e1,
-                                         const Cplx<double>* __restrict__ e2, int i) {
-  extern __shared__ unsigned char smem_raw[];
+                                         const Cplx<double>* __restrict__ e2, int i,
+                                         Cplx<T>* buf, int tid, int nth) {
   const int M = fp.M;
-  Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
   const T* row = src + (int64_t)i * nz;
   constexpr int NQ = 2 * PER;
-  dct3_pre<T, NQ>(buf, M, e1, e2, [&](int k) { return row[k]; });
+  dct3_pre<T, NQ>(buf, M, e1, e2, [&](int k) { return row[k]; }, tid, nth);
   __syncthreads();
-  fft_lds<T, PER>(buf, fp, tw, +1);
+  fft_lds<T, PER>(buf, fp, tw, +1, tid, nth);
   const T invN = T(1) / T(nz);
   T* out = dst + (int64_t)i * nz;
   constexpr int NJ = 4 * PER;
 #pragma unroll
   for (int q = 0; q < NJ; ++q) {
-    const int j = threadIdx.x + q * blockDim.x;
+    const int j = tid + q * nth;
     if (j < nz) out[j] = packed_component(buf, dct_dst_index(j, nz)) * invN;
   }
 }
@@ -408,7 +409,9 @@ __global__ void __launch_bounds__(256) dct3_rows_kernel(
     const Cplx<double>* __restrict__ tw, const Cplx<double>* __restrict__ e1,
     const Cplx<double>* __restrict__ e2) {
   const int64_t cs = (int64_t)blockIdx.y * gridDim.x * nz;  // case of a batch
-  dct3_row<T, PER>(src + cs, dst + cs, nz, fp, tw, e1, e2, blockIdx.x);
+  extern __shared__ unsigned char smem_raw[];
+  dct3_row<T, PER>(src + cs, dst + cs, nz, fp, tw, e1, e2, blockIdx.x,
+                   reinterpret_cast<Cplx<T>*>(smem_raw), threadIdx.x, blockDim.x);
 }
 
 // nz = 2048 row passes: one wavefront per row, FFT in registers
@@ -780,11 +783,15 @@ __global__ void __launch_bounds__(256) taper_dct2_ragged_kernel(
     }
     b -= ncases;
   }
-  const RaggedGroup& G = grp[ragged_group<&RaggedGroup::rows0>(grp, ngrp, b)];
-  const int q = b - G.rows0, cl = q / G.nr, i = q % G.nr;
+  const RaggedGroup& G = grp[ragged_group<&RaggedGroup::rb0>(grp, ngrp, b)];
+  const int sub = threadIdx.x / G.bs;  // row of this workgroup
+  const int row = (b - G.rb0) * G.rpb + sub, cl = row / G.nr, i = row % G.nr;
   const int64_t cs = G.cell0 + (int64_t)cl * G.nr * G.nz;
+  extern __shared__ unsigned char smem_raw[];
   taper_dct2_row<T, 1>(src + cs, nullptr, dst + cs, G.rvec, G.kill_wz, G.kill_wr, G.nr, G.nz, w,
-                       G.fp, G.tw, G.e1, G.e2, i);
+                       G.fp, G.tw, G.e1, G.e2, i,
+                       reinterpret_cast<Cplx<T>*>(smem_raw) + sub * G.fp.M, threadIdx.x % G.bs,
+                       G.bs);
 }
 
 template <typename T>
@@ -793,10 +800,13 @@ __global__ void __launch_bounds__(256) dct3_ragged_kernel(const T* __restrict__ 
                                                           const RaggedGroup* __restrict__ grp,
                                                           int ngrp) {
   const int b = blockIdx.x;
-  const RaggedGroup& G = grp[ragged_group<&RaggedGroup::rows0>(grp, ngrp, b)];
-  const int q = b - G.rows0, cl = q / G.nr, i = q % G.nr;
+  const RaggedGroup& G = grp[ragged_group<&RaggedGroup::rb0>(grp, ngrp, b)];
+  const int sub = threadIdx.x / G.bs;
+  const int row = (b - G.rb0) * G.rpb + sub, cl = row / G.nr, i = row % G.nr;
   const int64_t cs = G.cell0 + (int64_t)cl * G.nr * G.nz;
-  dct3_row<T, 1>(src + cs, dst + cs, G.nz, G.fp, G.tw, G.e1, G.e2, i);
+  extern __shared__ unsigned char smem_raw[];
+  dct3_row<T, 1>(src + cs, dst + cs, G.nz, G.fp, G.tw, G.e1, G.e2, i,
+                 reinterpret_cast<Cplx<T>*>(smem_raw) + sub * G.fp.M, threadIdx.x % G.bs, G.bs);
 }
 
 template <typename T, int L>
@@ -939,19 +949,19 @@ hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch) 
 
 template <typename T>
 hipError_t launch_taper_dct2_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp,
-                                    int ncases, int rows, int w, int maxM,
+                                    int ncases, int row_blocks, int w, int lds_cplx,
                                     const StepFinalize* fin, hipStream_t s) {
   const StepFinalize f = fin ? *fin : StepFinalize{};
-  const int blocks = rows + (f.partials ? ncases : 0);
-  taper_dct2_ragged_kernel<T><<<blocks, 256, sizeof(Cplx<T>) * maxM, s>>>(src, dst, grp, ngrp,
-                                                                          ncases, w, f);
+  const int blocks = row_blocks + (f.partials ? ncases : 0);
+  taper_dct2_ragged_kernel<T><<<blocks, 256, sizeof(Cplx<T>) * lds_cplx, s>>>(src, dst, grp, ngrp,
+                                                                              ncases, w, f);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_dct3_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp, int rows,
-                              int maxM, hipStream_t s) {
-  dct3_ragged_kernel<T><<<rows, 256, sizeof(Cplx<T>) * maxM, s>>>(src, dst, grp, ngrp);
+hipError_t launch_dct3_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp,
+                              int row_blocks, int lds_cplx, hipStream_t s) {
+  dct3_ragged_kernel<T><<<row_blocks, 256, sizeof(Cplx<T>) * lds_cplx, s>>>(src, dst, grp, ngrp);
   return hipGetLastError();
 }
 

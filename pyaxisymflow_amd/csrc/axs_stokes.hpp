@@ -77,16 +77,18 @@ template <typename T>
 hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch = 1);
 
 // Uneven batches (axs_ragged.hpp): one launch per pass over every group.
-// Row passes run 256 threads per row (nz <= 1024); the forward pass puts one
-// completion workgroup per case in front of the rows when fin is given
-// (force partials of case q at part0 + (q - case0) * ftiles of its group).
+// Row passes run 256-thread workgroups of rpb rows with bs threads each
+// (nz <= 1024), lds_cplx complex LDS entries per workgroup (the largest
+// rpb M); the forward pass puts one completion workgroup per case in front
+// of the rows when fin is given (force partials of case q at
+// part0 + (q - case0) * ftiles of its group).
 template <typename T>
 hipError_t launch_taper_dct2_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp,
-                                    int ncases, int rows, int w, int maxM,
+                                    int ncases, int row_blocks, int w, int lds_cplx,
                                     const StepFinalize* fin, hipStream_t s);
 template <typename T>
-hipError_t launch_dct3_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp, int rows,
-                              int maxM, hipStream_t s);
+hipError_t launch_dct3_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp,
+                              int row_blocks, int lds_cplx, hipStream_t s);
 // Radial solves with segments of L rows (4, 8 or 16; the groups' plans
 // must have been made for that L), K and P per group, one 1024-thread
 // workgroup per K modes of a case.
