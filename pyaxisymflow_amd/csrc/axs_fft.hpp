@@ -80,28 +80,60 @@ __device__ __forceinline__ void dft_small(Cplx<T>* a, int sgn) {
   }
 }
 
+// Copies the plan's twiddle table tw[0..M) into LDS at tl, converted to T,
+// by threads tid of nth; a block barrier must follow before the stages read
+// it.  (The stages' twiddle reads then wait on LDS, not on a global load
+// that cannot be issued before the stage's barrier.)
+template <typename T>
+__device__ __forceinline__ void stage_twiddles(Cplx<T>* tl, const Cplx<double>* __restrict__ tw,
+                                               int M, int tid, int nth) {
+  for (int t = tid; t < M; t += nth) {
+    const Cplx<double> w = tw[t];
+    tl[t] = Cplx<T>{T(w.x), T(w.y)};
+  }
+}
+
 // One Stockham stage of radix R over buf[0..M) in LDS (in place: all reads
 // go to registers before a barrier, then all writes).  Each thread handles
 // PER butterflies (compile-time, so the staging array stays in registers).
-// tw[t] = e^{-2 pi i t/M}; the inverse (sgn = +1) uses its conjugate.
+// tl[t] = e^{-2 pi i t/M} in LDS (stage_twiddles); the inverse (sgn = +1)
+// uses its conjugate.
 template <typename T, int R, int PER>
-__device__ __forceinline__ void stockham_stage(Cplx<T>* buf, int M, int Ns,
-                                               const Cplx<double>* __restrict__ tw, int sgn,
-                                               int tid, int nth) {
+__device__ __forceinline__ void stockham_stage(Cplx<T>* buf, int M, int Ns, const Cplx<T>* tl,
+                                               int sgn, int tid, int nth) {
   const int nb = M / R;
   const int tstride = M / (Ns * R);
+  // butterfly j: jm = j mod Ns, jq = j div Ns (a shift and a mask when Ns is
+  // a power of two, as it is for every stage when the radix-3 and -5
+  // stages come last)
+  int jm[PER], jq[PER];
+  if ((Ns & (Ns - 1)) == 0) {
+    const int sh = __builtin_ctz(Ns);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int j = tid + q * nth;
+      jm[q] = j & (Ns - 1);
+      jq[q] = j >> sh;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const unsigned j = tid + q * nth;
+      jq[q] = (int)(j / (unsigned)Ns);
+      jm[q] = (int)j - jq[q] * Ns;
+    }
+  }
   Cplx<T> v[PER][R];
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int j = tid + q * nth;
     if (j < nb) {
-      const int jm = j % Ns;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         Cplx<T> a = buf[j + r * nb];
-        if (r > 0 && jm > 0) {
-          Cplx<double> w = tw[jm * r * tstride];
-          Cplx<T> wt = {T(w.x), T(sgn < 0 ? w.y : -w.y)};
+        if (r > 0 && jm[q] > 0) {
+          const Cplx<T> w = tl[jm[q] * r * tstride];
+          const Cplx<T> wt = {w.x, sgn < 0 ? w.y : -w.y};
           a = cmul(a, wt);
         }
         v[q][r] = a;
@@ -114,8 +146,7 @@ __device__ __forceinline__ void stockham_stage(Cplx<T>* buf, int M, int Ns,
   for (int q = 0; q < PER; ++q) {
     const int j = tid + q * nth;
     if (j < nb) {
-      const int jm = j % Ns;
-      const int idxD = (j / Ns) * Ns * R + jm;
+      const int idxD = jq[q] * Ns * R + jm[q];
 #pragma unroll
       for (int r = 0; r < R; ++r) buf[idxD + r * Ns] = v[q][r];
     }
@@ -123,23 +154,23 @@ __device__ __forceinline__ void stockham_stage(Cplx<T>* buf, int M, int Ns,
   __syncthreads();
 }
 
-// Full FFT of length p.M in LDS by nth threads (this one: tid).  Requires
-// p.M / min(radix) <= PER * nth.  Every thread of the block must call it
-// (stages end in block barriers).
+// Full FFT of length p.M in LDS by nth threads (this one: tid), twiddles
+// tl in LDS.  Requires p.M / min(radix) <= PER * nth.  Every thread of the
+// block must call it (stages end in block barriers).
 template <typename T, int PER>
-__device__ void fft_lds(Cplx<T>* buf, const FftPlanArgs& p, const Cplx<double>* tw, int sgn,
-                        int tid, int nth) {
+__device__ void fft_lds(Cplx<T>* buf, const FftPlanArgs& p, const Cplx<T>* tl, int sgn, int tid,
+                        int nth) {
   int Ns = 1;
   for (int s = 0; s < p.nstages; ++s) {
     const int R = p.radix[s];
     if (R == 4)
-      stockham_stage<T, 4, PER>(buf, p.M, Ns, tw, sgn, tid, nth);
+      stockham_stage<T, 4, PER>(buf, p.M, Ns, tl, sgn, tid, nth);
     else if (R == 2)
-      stockham_stage<T, 2, PER>(buf, p.M, Ns, tw, sgn, tid, nth);
+      stockham_stage<T, 2, PER>(buf, p.M, Ns, tl, sgn, tid, nth);
     else if (R == 3)
-      stockham_stage<T, 3, PER>(buf, p.M, Ns, tw, sgn, tid, nth);
+      stockham_stage<T, 3, PER>(buf, p.M, Ns, tl, sgn, tid, nth);
     else
-      stockham_stage<T, 5, PER>(buf, p.M, Ns, tw, sgn, tid, nth);
+      stockham_stage<T, 5, PER>(buf, p.M, Ns, tl, sgn, tid, nth);
     Ns *= R;
   }
 }
@@ -193,8 +224,8 @@ __device__ __forceinline__ void dct2_post(const Cplx<T>* Z, int M, const PostTwi
   for (int q = 0; q < KQ; ++q) {
     const int k = tid + q * nth;
     if (k <= M / 2) {
-      const int kp = M - k;  // partner
-      const Cplx<T> zk = Z[k % M], zp = Z[kp % M];
+      const int kp = M - k;  // partner, in [M/2, M]; Z is periodic in M
+      const Cplx<T> zk = Z[k], zp = Z[kp == M ? 0 : kp];
       // V_k = s/2 - i w d / 2 with s = Z_k + conj(Z_kp), d = Z_k - conj(Z_kp)
       const Cplx<T> sk = cadd(zk, cconj(zp)), dk = csub(zk, cconj(zp));
       const Cplx<T> wdk = cmul(Cplx<T>{T(t.e1k[q].x), T(t.e1k[q].y)}, dk);

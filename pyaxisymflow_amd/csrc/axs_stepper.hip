@@ -484,7 +484,8 @@ hipError_t RaggedStepper<T>::init_ragged(const RigidConfig* cfgs, int nbatch,
     rows += G.ncase * G.nr / G.rpb;  // row-pass workgroups
     vt_tiles += G.ncase * G.vtiles;
     ft_tiles += G.ncase * G.ftiles;
-    maxM = std::max(maxM, G.rpb * (G.nz / 2));  // LDS complex entries per workgroup
+    // LDS complex entries per workgroup: rpb rows and the twiddle table
+    maxM = std::max(maxM, (G.rpb + 1) * (G.nz / 2));
   }
   npartials = ft_tiles;
   e = hipMalloc(reinterpret_cast<void**>(&partials), sizeof(double) * (size_t)ft_tiles);
