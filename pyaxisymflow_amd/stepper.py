@@ -504,9 +504,10 @@ class SoftSphereStepper:
     1 / grid_size_z, nr = domain_AR * nz, moll_zone = 4 dx, extrap_zone =
     reinit_band = moll_zone + 4 dx, sphere of radius r_ball at (0.5, 0),
     vorticity C (kz^2 + kr^2) sin(kz z) sin(kr r) with kz = kr = 2 pi.
-    ``lsq_sweeps`` bounds the extrapolation sweeps per step (default 32; the
-    reference loops until a sweep fits nothing, which takes 18-19 sweeps
-    here at every grid size, as the band is a fixed number of cells wide;
+    ``lsq_sweeps`` bounds the extrapolation sweeps per step (default 48; the
+    reference loops until a sweep fits nothing, which takes 18-19 sweeps at
+    first at every grid size, as the band is a fixed number of cells wide,
+    and more as a soft solid deforms over thousands of steps;
     ``lsq_unconverged()`` counts steps that hit the bound)."""
 
     def __init__(self, grid_size_z=256, G=1.0, nu=1e-3, rho_f=1.0, r_ball=0.15, domain_AR=0.5,
@@ -549,9 +550,10 @@ class SoftSphereStepper:
         cfg = _lib.SoftConfig()
         cfg.nr, cfg.nz, cfg.kill_width = nr, nz, kill_width
         # the reference's loop needs ~ band / dx sweeps at first and more as
-        # the solid deforms (18 after 200 steps at nz = 256); sweeps after
-        # the loop has ended exit on one counter read
-        cfg.lsq_sweeps = int(lsq_sweeps) if lsq_sweeps else 32
+        # the solid deforms (18 after 200 steps at nz = 256; a bound of 32
+        # is hit over 2000 steps of the softest solids, 48 is not); sweeps
+        # after the loop has ended exit on one counter read
+        cfg.lsq_sweeps = int(lsq_sweeps) if lsq_sweeps else 48
         cfg.reinit_iterations = -1
         cfg.dx, cfg.nu, cfg.G, cfg.rho_f, cfg.CFL = dx, nu, G, rho_f, 0.1
         cfg.eps = np.finfo(float).eps
