@@ -83,24 +83,39 @@ __device__ __forceinline__ void dft_small(Cplx<T>* a, int sgn) {
 // Copies the plan's twiddle table tw[0..M) into LDS at tl, converted to T,
 // by threads tid of nth; a block barrier must follow before the stages read
 // it.  (The stages' twiddle reads then wait on LDS, not on a global load
-// that cannot be issued before the stage's barrier.)
+// that cannot be issued before the stage's barrier.  This pays for short
+// FP32 rows; for long FP64 rows the strided 16-byte reads compete with the
+// row's own LDS traffic and the global table is faster.)  Loads go out two
+// at a time before their stores: one load latency when M <= 2 nth, and few
+// registers held.
 template <typename T>
 __device__ __forceinline__ void stage_twiddles(Cplx<T>* tl, const Cplx<double>* __restrict__ tw,
                                                int M, int tid, int nth) {
-  for (int t = tid; t < M; t += nth) {
-    const Cplx<double> w = tw[t];
-    tl[t] = Cplx<T>{T(w.x), T(w.y)};
+  constexpr int U = 2;
+  for (int base = tid; base < M; base += U * nth) {
+    Cplx<double> w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = base + u * nth;
+      if (t < M) w[u] = tw[t];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = base + u * nth;
+      if (t < M) tl[t] = Cplx<T>{T(w[u].x), T(w[u].y)};
+    }
   }
 }
 
 // One Stockham stage of radix R over buf[0..M) in LDS (in place: all reads
 // go to registers before a barrier, then all writes).  Each thread handles
 // PER butterflies (compile-time, so the staging array stays in registers).
-// tl[t] = e^{-2 pi i t/M} in LDS (stage_twiddles); the inverse (sgn = +1)
-// uses its conjugate.
-template <typename T, int R, int PER>
-__device__ __forceinline__ void stockham_stage(Cplx<T>* buf, int M, int Ns, const Cplx<T>* tl,
-                                               int sgn, int tid, int nth) {
+// tw[t] = e^{-2 pi i t/M}: the plan's FP64 table in global memory
+// (W = Cplx<double>) or its copy in LDS (W = Cplx<T>, stage_twiddles); both
+// give the same bits.  The inverse (sgn = +1) uses its conjugate.
+template <typename T, int R, int PER, typename W>
+__device__ __forceinline__ void stockham_stage(Cplx<T>* buf, int M, int Ns, const W* tw, int sgn,
+                                               int tid, int nth) {
   const int nb = M / R;
   const int tstride = M / (Ns * R);
   // butterfly j: jm = j mod Ns, jq = j div Ns (a shift and a mask when Ns is
@@ -132,8 +147,8 @@ __device__ __forceinline__ void stockham_stage(Cplx<T>* buf, int M, int Ns, cons
       for (int r = 0; r < R; ++r) {
         Cplx<T> a = buf[j + r * nb];
         if (r > 0 && jm[q] > 0) {
-          const Cplx<T> w = tl[jm[q] * r * tstride];
-          const Cplx<T> wt = {w.x, sgn < 0 ? w.y : -w.y};
+          const W w = tw[jm[q] * r * tstride];
+          const Cplx<T> wt = {T(w.x), T(sgn < 0 ? w.y : -w.y)};
           a = cmul(a, wt);
         }
         v[q][r] = a;
@@ -154,23 +169,23 @@ __device__ __forceinline__ void stockham_stage(Cplx<T>* buf, int M, int Ns, cons
   __syncthreads();
 }
 
-// Full FFT of length p.M in LDS by nth threads (this one: tid), twiddles
-// tl in LDS.  Requires p.M / min(radix) <= PER * nth.  Every thread of the
-// block must call it (stages end in block barriers).
-template <typename T, int PER>
-__device__ void fft_lds(Cplx<T>* buf, const FftPlanArgs& p, const Cplx<T>* tl, int sgn, int tid,
+// Full FFT of length p.M in LDS by nth threads (this one: tid), twiddles tw
+// as for stockham_stage.  Requires p.M / min(radix) <= PER * nth.  Every
+// thread of the block must call it (stages end in block barriers).
+template <typename T, int PER, typename W>
+__device__ void fft_lds(Cplx<T>* buf, const FftPlanArgs& p, const W* tw, int sgn, int tid,
                         int nth) {
   int Ns = 1;
   for (int s = 0; s < p.nstages; ++s) {
     const int R = p.radix[s];
     if (R == 4)
-      stockham_stage<T, 4, PER>(buf, p.M, Ns, tl, sgn, tid, nth);
+      stockham_stage<T, 4, PER, W>(buf, p.M, Ns, tw, sgn, tid, nth);
     else if (R == 2)
-      stockham_stage<T, 2, PER>(buf, p.M, Ns, tl, sgn, tid, nth);
+      stockham_stage<T, 2, PER, W>(buf, p.M, Ns, tw, sgn, tid, nth);
     else if (R == 3)
-      stockham_stage<T, 3, PER>(buf, p.M, Ns, tl, sgn, tid, nth);
+      stockham_stage<T, 3, PER, W>(buf, p.M, Ns, tw, sgn, tid, nth);
     else
-      stockham_stage<T, 5, PER>(buf, p.M, Ns, tl, sgn, tid, nth);
+      stockham_stage<T, 5, PER, W>(buf, p.M, Ns, tw, sgn, tid, nth);
     Ns *= R;
   }
 }

@@ -269,8 +269,7 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
 
 // Row pass: x_j = scale_i * src[i, j]  ->  X = DCT-II(x)  ->  dst[i, :].
 // The row is read coalesced and scattered into the packed LDS sequence; the
-// post-processing twiddles are fetched before the FFT starts, and the FFT's
-// twiddle table is staged in LDS after the row (2M complex entries of LDS).
+// post-processing twiddles are fetched before the FFT starts.
 template <typename T, int PER>
 __global__ void __launch_bounds__(256) dct2_rows_kernel(
     const T* __restrict__ src, T* __restrict__ dst, const double* __restrict__ rscale,
@@ -291,14 +290,13 @@ __global__ void __launch_bounds__(256) dct2_rows_kernel(
   }
   PostTwiddles<PER + 1> pt;
   dct2_post_twiddles<PER + 1>(pt, M, e1, e2, threadIdx.x, blockDim.x);
-  stage_twiddles(buf + M, tw, M, threadIdx.x, blockDim.x);
 #pragma unroll
   for (int q = 0; q < NJ; ++q) {
     const int j = threadIdx.x + q * blockDim.x;
     if (j < nz) packed_component(buf, dct_dst_index(j, nz)) = s * v[q];
   }
   __syncthreads();
-  fft_lds<T, PER>(buf, fp, buf + M, -1, threadIdx.x, blockDim.x);
+  fft_lds<T, PER>(buf, fp, tw, -1, threadIdx.x, blockDim.x);
   T* out = dst + (int64_t)i * nz;
   dct2_post<T, PER + 1>(buf, M, pt, [&](int k, T val) { out[k] = val; }, threadIdx.x, blockDim.x);
 }
@@ -312,10 +310,11 @@ __global__ void __launch_bounds__(256) dct2_rows_kernel(
 // read row nr-w).  wz: 2w weights (left block, right block), wr: w weights.
 // One row of the tapered forward pass (src, tapered, dst at the case).
 // Threads tid of nth (all of them call it: the FFT has block barriers)
-// work in buf, M complex entries of LDS.  tl (M entries of LDS) receives the
-// twiddle table from threads wtid of wnth: every row of a workgroup passes
-// the same tl and its workgroup-wide thread index.
-template <typename T, int PER>
+// work in buf, M complex entries of LDS.  STAGE: the FFT reads its twiddles
+// from tl (M entries of LDS), filled from tw by threads wtid of wnth (every
+// row of a workgroup passes the same tl and its workgroup-wide thread
+// index); else from tw in global memory (tl, wtid, wnth unused).
+template <typename T, int PER, bool STAGE>
 __device__ __forceinline__ void taper_dct2_row(
     const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
     const double* __restrict__ rscale, const double* __restrict__ wz,
@@ -336,7 +335,7 @@ __device__ __forceinline__ void taper_dct2_row(
   const T left = row[w - 1], right = row[nz - w];
   PostTwiddles<PER + 1> pt;
   dct2_post_twiddles<PER + 1>(pt, M, e1, e2, tid, nth);
-  stage_twiddles(tl, tw, M, wtid, wnth);
+  if constexpr (STAGE) stage_twiddles(tl, tw, M, wtid, wnth);
   const T rs = T(rscale[i]);
   const T rowf = (i == 0) ? T(0) : ((i >= nr - w) ? T(wr[i - (nr - w)]) : T(1));
   T* out_t = tapered ? tapered + (int64_t)i * nz : nullptr;
@@ -356,7 +355,10 @@ __device__ __forceinline__ void taper_dct2_row(
     }
   }
   __syncthreads();
-  fft_lds<T, PER>(buf, fp, tl, -1, tid, nth);
+  if constexpr (STAGE)
+    fft_lds<T, PER>(buf, fp, static_cast<const Cplx<T>*>(tl), -1, tid, nth);
+  else
+    fft_lds<T, PER>(buf, fp, tw, -1, tid, nth);
   T* out = dst + (int64_t)i * nz;
   dct2_post<T, PER + 1>(buf, M, pt, [&](int k, T val) { out[k] = val; }, tid, nth);
 }
@@ -382,14 +384,14 @@ __global__ void __launch_bounds__(256) taper_dct2_rows_kernel(
   }
   const int64_t cs = (int64_t)blockIdx.y * nr * nz;  // case of a batch
   extern __shared__ unsigned char smem_raw[];
-  Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
-  taper_dct2_row<T, PER>(src + cs, tapered ? tapered + cs : nullptr, dst + cs, rscale, wz, wr, nr,
-                         nz, w, fp, tw, e1, e2, blockIdx.x, buf, buf + fp.M, threadIdx.x,
-                         blockDim.x, threadIdx.x, blockDim.x);
+  taper_dct2_row<T, PER, false>(src + cs, tapered ? tapered + cs : nullptr, dst + cs, rscale, wz,
+                                wr, nr, nz, w, fp, tw, e1, e2, blockIdx.x,
+                                reinterpret_cast<Cplx<T>*>(smem_raw), nullptr, threadIdx.x,
+                                blockDim.x, 0, 0);
 }
 // One row of the inverse pass (src, dst at the case); buf, tl, tid, nth,
-// wtid, wnth as for taper_dct2_row.
-template <typename T, int PER>
+// wtid, wnth and STAGE as for taper_dct2_row.
+template <typename T, int PER, bool STAGE>
 __device__ __forceinline__ void dct3_row(const T* __restrict__ src, T* __restrict__ dst, int nz,
                                          const FftPlanArgs& fp,
                                          const Cplx<double>* __restrict__ tw,
@@ -400,10 +402,13 @@ __device__ __forceinline__ void dct3_row(const T* __restrict__ src, T* __restric
   const int M = fp.M;
   const T* row = src + (int64_t)i * nz;
   constexpr int NQ = 2 * PER;
-  stage_twiddles(tl, tw, M, wtid, wnth);
+  if constexpr (STAGE) stage_twiddles(tl, tw, M, wtid, wnth);
   dct3_pre<T, NQ>(buf, M, e1, e2, [&](int k) { return row[k]; }, tid, nth);
   __syncthreads();
-  fft_lds<T, PER>(buf, fp, tl, +1, tid, nth);
+  if constexpr (STAGE)
+    fft_lds<T, PER>(buf, fp, static_cast<const Cplx<T>*>(tl), +1, tid, nth);
+  else
+    fft_lds<T, PER>(buf, fp, tw, +1, tid, nth);
   const T invN = T(1) / T(nz);
   T* out = dst + (int64_t)i * nz;
   constexpr int NJ = 4 * PER;
@@ -420,9 +425,9 @@ __global__ void __launch_bounds__(256) dct3_rows_kernel(
     const Cplx<double>* __restrict__ e2) {
   const int64_t cs = (int64_t)blockIdx.y * gridDim.x * nz;  // case of a batch
   extern __shared__ unsigned char smem_raw[];
-  Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
-  dct3_row<T, PER>(src + cs, dst + cs, nz, fp, tw, e1, e2, blockIdx.x, buf, buf + fp.M,
-                   threadIdx.x, blockDim.x, threadIdx.x, blockDim.x);
+  dct3_row<T, PER, false>(src + cs, dst + cs, nz, fp, tw, e1, e2, blockIdx.x,
+                          reinterpret_cast<Cplx<T>*>(smem_raw), nullptr, threadIdx.x, blockDim.x,
+                          0, 0);
 }
 
 // nz = 2048 row passes: one wavefront per row, FFT in registers
@@ -800,7 +805,7 @@ __global__ void __launch_bounds__(256) taper_dct2_ragged_kernel(
   const int64_t cs = G.cell0 + (int64_t)cl * G.nr * G.nz;
   extern __shared__ unsigned char smem_raw[];
   Cplx<T>* lds = reinterpret_cast<Cplx<T>*>(smem_raw);
-  taper_dct2_row<T, 1>(src + cs, nullptr, dst + cs, G.rvec, G.kill_wz, G.kill_wr, G.nr, G.nz, w,
+  taper_dct2_row<T, 1, true>(src + cs, nullptr, dst + cs, G.rvec, G.kill_wz, G.kill_wr, G.nr, G.nz, w,
                        G.fp, G.tw, G.e1, G.e2, i, lds + sub * G.fp.M, lds + G.rpb * G.fp.M,
                        threadIdx.x % G.bs, G.bs, threadIdx.x, blockDim.x);
 }
@@ -817,7 +822,7 @@ __global__ void __launch_bounds__(256) dct3_ragged_kernel(const T* __restrict__ 
   const int64_t cs = G.cell0 + (int64_t)cl * G.nr * G.nz;
   extern __shared__ unsigned char smem_raw[];
   Cplx<T>* lds = reinterpret_cast<Cplx<T>*>(smem_raw);
-  dct3_row<T, 1>(src + cs, dst + cs, G.nz, G.fp, G.tw, G.e1, G.e2, i, lds + sub * G.fp.M,
+  dct3_row<T, 1, true>(src + cs, dst + cs, G.nz, G.fp, G.tw, G.e1, G.e2, i, lds + sub * G.fp.M,
                  lds + G.rpb * G.fp.M, threadIdx.x % G.bs, G.bs, threadIdx.x, blockDim.x);
 }
 
@@ -846,7 +851,7 @@ hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const do
   const int bs = fft_block(M);
   // worst stage is radix 2 (M/2 butterflies); 2*per*bs >= M covers the row loads
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
-  const size_t shm = sizeof(Cplx<T>) * 2 * M;  // row + twiddle table
+  const size_t shm = sizeof(Cplx<T>) * M;
   if (M == kF1024M) {
     dct2_rows_m1024_kernel<T, false><<<rows, 64, 0, s>>>(src, nullptr, dst, rscale, nullptr,
                                                           nullptr, rows, 0, pl.tw, pl.e1, pl.e2,
@@ -873,7 +878,7 @@ hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered
   const int M = pl.nz / 2;
   const int bs = fft_block(M);
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
-  const size_t shm = sizeof(Cplx<T>) * 2 * M;  // row + twiddle table
+  const size_t shm = sizeof(Cplx<T>) * M;
   if (M == kF1024M) {
     dct2_rows_m1024_kernel<T, true><<<dim3(gx, batch), 64, 0, s>>>(
         src, tapered, dst, pl.rvec, wz, wr, pl.nr, w, pl.tw, pl.e1, pl.e2, f);
@@ -900,7 +905,7 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
   const int M = pl.nz / 2;
   const int bs = fft_block(M);
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
-  const size_t shm = sizeof(Cplx<T>) * 2 * M;  // row + twiddle table
+  const size_t shm = sizeof(Cplx<T>) * M;
   const dim3 grid(rows, batch);
   if (M == kF1024M) {
     dct3_rows_m1024_kernel<T><<<grid, 64, 0, s>>>(src, dst, pl.tw, pl.e1, pl.e2);
