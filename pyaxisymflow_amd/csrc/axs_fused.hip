@@ -126,23 +126,25 @@ __device__ __forceinline__ T eno3_face_b(T fm2, T fm1, T f0, T fp1, T um2, T um1
 
 // ------------------------------------------------------------ velocity_max
 
-constexpr int kVTZ = 64, kVTR = 16;                  // tile
-constexpr int kVPZ = kVTZ + 4, kVPR = kVTR + 4;      // psi tile (halo 2)
+constexpr int kVTZ = 64, kVTR = 16;  // tile of the one-grid kernel (TR = kVTR)
+constexpr int kVPZ = kVTZ + 4;       // psi tile columns (halo 2)
 
-// Tile (bx, by) of one case (psi, sc, slots at the case); the maximum goes
-// to slot `slot`.
-template <typename T>
+// Tile (bx, by), TR rows by kVTZ columns (TR = 8, 16 or 32), of one case
+// (psi, sc, slots at the case); the maximum goes to slot `slot`.
+template <typename T, int TR>
 __device__ __forceinline__ void velocity_max_tile(const T* __restrict__ psi,
                                                   const double* __restrict__ r, int nr, int nz,
                                                   T dx, const double* __restrict__ sc,
                                                   double* __restrict__ slots,
                                                   const RigidConfig& cfg, int bx, int by,
                                                   int slot) {
+  static_assert(TR == 8 || TR == 16 || TR == 32, "tile rows");
+  constexpr int kVPR = TR + 4;  // psi tile rows (halo 2)
   __shared__ T sp[kVPR][kVPZ];
-  __shared__ T sri[kVTR], sinv[kVTR];
+  __shared__ T sri[TR], sinv[TR];
   __shared__ double wm[4];
   const int tid = threadIdx.x;
-  const int j0 = bx * kVTZ, i0 = by * kVTR;
+  const int j0 = bx * kVTZ, i0 = by * TR;
   constexpr int NP = (kVPR * kVPZ + 255) / 256;
   T v[NP];
 #pragma unroll
@@ -153,7 +155,7 @@ __device__ __forceinline__ void velocity_max_tile(const T* __restrict__ psi,
     v[q] = T(0);
     if (e < kVPR * kVPZ && i >= 0 && i < nr && j >= 0 && j < nz) v[q] = psi[(int64_t)i * nz + j];
   }
-  if (tid < kVTR) {
+  if (tid < TR) {
     const int i = i0 + tid;
     const T ri = (i < nr) ? T(r[i]) : T(1);
     sri[tid] = ri;
@@ -168,11 +170,11 @@ __device__ __forceinline__ void velocity_max_tile(const T* __restrict__ psi,
   const T two_dx = T(2) * dx, inv2dx = T(1) / two_dx;
   const T ufree = T(cfg.U0 * sc[kScPrefac]);
   double m = 0.0;
-  if (i0 >= 1 && i0 + kVTR <= nr - 1 && j0 >= 1 && j0 + kVTZ <= nz - 1) {
+  if (i0 >= 1 && i0 + TR <= nr - 1 && j0 >= 1 && j0 + kVTZ <= nz - 1) {
     // no cell of the tile lies on a domain edge: central differences only
     // (the formulas velocity_rn takes away from the edges)
 #pragma unroll
-    for (int q = 0; q < (kVTR * kVTZ) / 256; ++q) {
+    for (int q = 0; q < (TR * kVTZ) / 256; ++q) {
       const int e = tid + q * 256;
       const int a = e / kVTZ, b = e % kVTZ;
       const T* p = &sp[a + 2][b + 2];
@@ -182,7 +184,7 @@ __device__ __forceinline__ void velocity_max_tile(const T* __restrict__ psi,
     }
   } else {
 #pragma unroll
-    for (int q = 0; q < (kVTR * kVTZ) / 256; ++q) {
+    for (int q = 0; q < (TR * kVTZ) / 256; ++q) {
       const int e = tid + q * 256;
       const int a = e / kVTZ, b = e % kVTZ;
       const int i = i0 + a, j = j0 + b;
@@ -212,13 +214,13 @@ __global__ void __launch_bounds__(256) velocity_max_kernel(const T* __restrict__
                                                            double* __restrict__ slots,
                                                            const RigidConfig* __restrict__ cfgs) {
   const int cb = blockIdx.z;  // case of a batch
-  velocity_max_tile<T>(psi + (int64_t)cb * nr * nz, r, nr, nz, dx,
+  velocity_max_tile<T, kVTR>(psi + (int64_t)cb * nr * nz, r, nr, nz, dx,
                        sc + (int64_t)cb * AXS_RIGID_SCALARS, slots + (int64_t)cb * kSlotsPerCase,
                        cfgs[cb], blockIdx.x, blockIdx.y,
                        (blockIdx.y * gridDim.x + blockIdx.x) % kMaxSlots);
 }
 
-template <typename T>
+template <typename T, int TR>
 __global__ void __launch_bounds__(256) velocity_max_ragged_kernel(
     const T* __restrict__ psi, const double* __restrict__ sc, double* __restrict__ slots,
     const RigidConfig* __restrict__ cfgs, const RaggedGroup* __restrict__ grp, int ngrp) {
@@ -226,7 +228,7 @@ __global__ void __launch_bounds__(256) velocity_max_ragged_kernel(
   const RaggedGroup& G = grp[ragged_group<&RaggedGroup::vt0>(grp, ngrp, b)];
   const int q = b - G.vt0, cl = q / G.vtiles, t = q % G.vtiles;
   const int cb = G.case0 + cl;
-  velocity_max_tile<T>(psi + G.cell0 + (int64_t)cl * G.nr * G.nz, G.r, G.nr, G.nz, T(G.dx),
+  velocity_max_tile<T, TR>(psi + G.cell0 + (int64_t)cl * G.nr * G.nz, G.r, G.nr, G.nz, T(G.dx),
                        sc + (int64_t)cb * AXS_RIGID_SCALARS, slots + (int64_t)cb * kSlotsPerCase,
                        cfgs[cb], t % G.vz, t / G.vz, t % kMaxSlots);
 }
@@ -797,13 +799,20 @@ hipError_t launch_finalize_step(const double* partials, int count, double* slots
 
 int fused_tiles_z(int nz) { return (int)ceil_div(nz, kFZ); }
 int vmax_tiles_z(int nz) { return (int)ceil_div(nz, kVTZ); }
-int vmax_tiles(int nr, int nz) { return (int)(ceil_div(nz, kVTZ) * ceil_div(nr, kVTR)); }
+int vmax_tiles(int nr, int nz, int tr) { return (int)(ceil_div(nz, kVTZ) * ceil_div(nr, tr)); }
 
 template <typename T>
 hipError_t launch_velocity_max_ragged(const T* psi, const double* sc, double* slots,
                                       const RigidConfig* cfgs, const RaggedGroup* grp, int ngrp,
-                                      int tiles, hipStream_t s) {
-  velocity_max_ragged_kernel<T><<<tiles, 256, 0, s>>>(psi, sc, slots, cfgs, grp, ngrp);
+                                      int tiles, int tr, hipStream_t s) {
+  if (tr == 8)
+    velocity_max_ragged_kernel<T, 8><<<tiles, 256, 0, s>>>(psi, sc, slots, cfgs, grp, ngrp);
+  else if (tr == 16)
+    velocity_max_ragged_kernel<T, 16><<<tiles, 256, 0, s>>>(psi, sc, slots, cfgs, grp, ngrp);
+  else if (tr == 32)
+    velocity_max_ragged_kernel<T, 32><<<tiles, 256, 0, s>>>(psi, sc, slots, cfgs, grp, ngrp);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
@@ -839,10 +848,10 @@ AXS_FUSED_INST(double)
 AXS_FUSED_INST(float)
 template hipError_t launch_velocity_max_ragged<double>(const double*, const double*, double*,
                                                        const RigidConfig*, const RaggedGroup*,
-                                                       int, int, hipStream_t);
+                                                       int, int, int, hipStream_t);
 template hipError_t launch_velocity_max_ragged<float>(const float*, const double*, double*,
                                                       const RigidConfig*, const RaggedGroup*, int,
-                                                      int, hipStream_t);
+                                                      int, int, hipStream_t);
 template hipError_t launch_step_fused_ragged<double>(const double*, const double*, const double*,
                                                      double*, double*, int, double*,
                                                      const double*, double*, const RigidConfig*,

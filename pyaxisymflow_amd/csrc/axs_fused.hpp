@@ -152,11 +152,12 @@ hipError_t launch_velocity_out(const T* psi, const T* chi, T* pz, T* pr, const d
 // index, force partials at part0 + (case - case0) * ftiles of the group.
 int fused_tiles_z(int nz);
 int vmax_tiles_z(int nz);
-int vmax_tiles(int nr, int nz);
+// velocity_max tiles of one case with tiles of tr rows (8, 16 or 32) by 64
+int vmax_tiles(int nr, int nz, int tr);
 template <typename T>
 hipError_t launch_velocity_max_ragged(const T* psi, const double* sc, double* slots,
                                       const RigidConfig* cfgs, const RaggedGroup* grp, int ngrp,
-                                      int tiles, hipStream_t s);
+                                      int tiles, int tr, hipStream_t s);
 template <typename T>
 hipError_t launch_step_fused_ragged(const T* psi, const T* chi, const T* wsrc, T* wdst, T* avg,
                                     int kw, double* sc, const double* slots, double* partials,

@@ -312,6 +312,18 @@ static bool fft_args(int nz, FftPlanArgs& fp) {
   return m == 1;
 }
 
+// Lays out the velocity_max tiles of every group for tiles of tr rows;
+// returns the tiles of the whole batch.
+static int set_vmax_tiles(std::vector<RaggedGroup>& groups, int tr) {
+  int vt = 0;
+  for (RaggedGroup& G : groups) {
+    G.vtiles = vmax_tiles(G.nr, G.nz, tr);
+    G.vt0 = vt;
+    vt += G.ncase * G.vtiles;
+  }
+  return vt;
+}
+
 // Groups (runs of one grid) and their plans for radial segments of L rows.
 static hipError_t ragged_groups(const RigidConfig* cfgs, int nbatch, const RigidBuffers& b, int L,
                                 std::vector<StokesPlan*>& plans, std::vector<RaggedGroup>& groups,
@@ -353,7 +365,7 @@ static hipError_t ragged_groups(const RigidConfig* cfgs, int nbatch, const Rigid
     G.tchunks = (int)ceil_div(nz, G.K);
     G.tri0 = tri;
     G.vz = vmax_tiles_z(nz);
-    G.vtiles = vmax_tiles(nr, nz);
+    G.vtiles = vmax_tiles(nr, nz, 16);  // set_vmax_tiles re-lays them out
     G.vt0 = vt;
     G.fz = fused_tiles_z(nz);
     G.ftiles = fused_blocks(nr, nz);
@@ -498,6 +510,57 @@ hipError_t RaggedStepper<T>::init_ragged(const RigidConfig* cfgs, int nbatch,
   if (e != hipSuccess) return e;
   e = hipMalloc(reinterpret_cast<void**>(&dgroups), sizeof(RaggedGroup) * groups.size());
   if (e != hipSuccess) return e;
+  // velocity_max tile height: AXS_VMAX_TR pins it, AXS_NO_TUNE takes 16,
+  // else the whole batch's pass is timed for 16, 8 and 32 rows (before the
+  // slots it writes are cleared below)
+  std::vector<int> trs{16, 8, 32};
+  if (const char* pin = std::getenv("AXS_VMAX_TR")) {
+    int tr = 0;
+    if (std::sscanf(pin, "%d", &tr) != 1 || (tr != 8 && tr != 16 && tr != 32))
+      return hipErrorInvalidValue;
+    trs = {tr};
+  } else if (std::getenv("AXS_NO_TUNE")) {
+    trs = {16};
+  }
+  vmax_tr = trs[0];
+  if (trs.size() > 1) {
+    if ((e = hipEventCreate(&t0)) != hipSuccess) return e;
+    if ((e = hipEventCreate(&t1)) != hipSuccess) return e;
+    const T* psi = static_cast<const T*>(buf.psi);
+    float best_v = 1e30f;
+    for (int tr : trs) {
+      const int tiles = set_vmax_tiles(groups, tr);
+      e = hipMemcpy(dgroups, groups.data(), sizeof(RaggedGroup) * groups.size(),
+                    hipMemcpyHostToDevice);
+      for (int rep = 0; rep < 2 && e == hipSuccess; ++rep)
+        e = launch_velocity_max_ragged<T>(psi, buf.scalars, slots, dcfg, dgroups,
+                                          (int)groups.size(), tiles, tr, 0);
+      float ms = 1e30f;
+      for (int trial = 0; trial < 3 && e == hipSuccess; ++trial) {
+        float t = 0.f;
+        e = hipEventRecord(t0, 0);
+        for (int rep = 0; rep < 5 && e == hipSuccess; ++rep)
+          e = launch_velocity_max_ragged<T>(psi, buf.scalars, slots, dcfg, dgroups,
+                                            (int)groups.size(), tiles, tr, 0);
+        if (e == hipSuccess) e = hipEventRecord(t1, 0);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, t0, t1);
+        if (e == hipSuccess && t < ms) ms = t;
+      }
+      if (e != hipSuccess) break;
+      if (ms < best_v) {
+        best_v = ms;
+        vmax_tr = tr;
+      }
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (e != hipSuccess) return e;
+  }
+  vt_tiles = set_vmax_tiles(groups, vmax_tr);
+  if (std::getenv("AXS_TUNE_LOG"))
+    std::fprintf(stderr, "uneven batch: radial segments L = %d, velocity_max tiles %d x 64\n",
+                 tri_L, vmax_tr);
   e = hipMemcpy(dgroups, groups.data(), sizeof(RaggedGroup) * groups.size(),
                 hipMemcpyHostToDevice);
   if (e != hipSuccess) return e;
@@ -540,7 +603,8 @@ hipError_t RaggedStepper<T>::enqueue_stage(int stage, hipStream_t s) {
     case 2:
       return launch_dct3_ragged<T>(sp, psi, dgroups, ng, rows, maxM, s);
     case 3:
-      return launch_velocity_max_ragged<T>(psi, sc, slots, dcfg, dgroups, ng, vt_tiles, s);
+      return launch_velocity_max_ragged<T>(psi, sc, slots, dcfg, dgroups, ng, vt_tiles, vmax_tr,
+                                           s);
     case 4:
       return hipSuccess;  // no moving body in uneven batches
     case 5:

@@ -109,7 +109,8 @@ struct RaggedStepper : RigidStepperBase {
   std::vector<RaggedGroup> groups;
   RaggedGroup* dgroups = nullptr;
   int rows = 0, tri_blocks = 0, vt_tiles = 0, ft_tiles = 0, maxM = 0;
-  int tri_L = 8;  // radial segment length (timed search over 4, 8, 16 at set-up)
+  int tri_L = 8;    // radial segment length (timed search over 4, 8, 16 at set-up)
+  int vmax_tr = 16;  // velocity_max tile rows (timed search over 16, 8, 32 at set-up)
   size_t tri_shm = 0;
   int64_t cells = 0;
   hipError_t init_ragged(const RigidConfig* cfgs, int nbatch, const RigidBuffers& b);

@@ -7,6 +7,8 @@ launch per pass (axs_rigid_batch_create with differing nr, nz, dx).
   batches, bit for bit; the force record differs only by the summation
   order of the per-tile partials (the completion reduction runs with a
   different workgroup size).
+* The velocity_max tile height picked at set-up (8, 16 or 32 rows) leaves
+  every field and the drag unchanged, bit for bit.
 * Each case also matches the oracle's restatement of the driver loop
   (flow_past_sphere.py:107-188) within the FP64 bar.
 * FP32 uneven batches keep the drag within 1e-4 of FP64.
@@ -69,6 +71,27 @@ def test_uneven_equals_one_grid_batches(monkeypatch):
             assert uneven.time(q) == one.time(k) and uneven.steps_done(q) == 30
             np.testing.assert_allclose(uneven.drag_history(q), one.drag_history(k), rtol=1e-13,
                                        atol=1e-15)
+
+
+@pytest.mark.parametrize("tr", [8, 32])
+def test_uneven_vmax_tile_height_is_exact(monkeypatch, tr):
+    """The velocity_max tile height the set-up search picks (8, 16 or 32
+    rows) changes only the tiling of an exact maximum: vorticity, psi and
+    drag equal those of 16-row tiles bit for bit."""
+    from pyaxisymflow_amd.stepper import UnevenBatchStepper
+
+    cases = uneven_cases([[0], [1], [2]])
+    monkeypatch.setenv("AXS_TRIDIAG_LK", "8")
+    runs = []
+    for rows in (16, tr):
+        monkeypatch.setenv("AXS_VMAX_TR", str(rows))
+        b = UnevenBatchStepper(cases)
+        b.step(20)
+        runs.append(b)
+    for q in range(len(cases)):
+        assert np.array_equal(runs[0].vorticity(q), runs[1].vorticity(q))
+        assert np.array_equal(runs[0].psi(q), runs[1].psi(q))
+        assert np.array_equal(runs[0].drag_history(q), runs[1].drag_history(q))
 
 
 def test_uneven_cases_match_oracle():
