@@ -294,7 +294,9 @@ def time_serving(n_requests, steps, max_active=16, seed=0):
         offered_load=0.8, open_loop_wall_s=round(t_open, 3),
         closed_loop_wall_s=round(t_closed, 3),
         closed_loop_mcell_steps_per_s=round(cells / t_closed / 1e6, 2),
-        lsq_unconverged_steps=int(sum(r["lsq_unconverged"] for r in recs + recs2)),
+        # steps whose extrapolation loop outran the queued sweeps and was
+        # finished by the one-workgroup tail (results unchanged)
+        lsq_tail_steps=int(sum(r["lsq_unconverged"] for r in recs + recs2)),
     )
 
 

@@ -274,18 +274,20 @@ axs_status axs_rigid_stepper_reset_scalars(axs_rigid_stepper* st, void* stream);
  * -> phi pinned to the sphere of eta where phi > -3 dx -> narrow-band
  * reinit (as axs_reinit_distance_f64) -> chi = smooth Heaviside(phi,
  * moll_zone) -> least-squares extrapolation of eta on the doubled grid
- * (sweeps until one fits nothing, at most lsq_sweeps) -> solid stress
+ * (sweeps until one fits nothing: lsq_sweeps queued launches, a longer
+ * loop finished by one workgroup) -> solid stress
  * blended by chi -> omega += dt curl(div sigma) -> ENO3 -> RK2.  FP64.
  * The caller owns the state buffers ((nr, nz) doubles; scalars
  * AXS_RIGID_SCALARS doubles: t, dt, dt/dx, nu dt/2, nu dt, max|u|, step
- * counter, sweeps used in the last step, steps that exhausted lsq_sweeps;
+ * counter, sweeps used in the last step, steps whose loop outran the
+ * lsq_sweeps queued launches and was finished by the one-workgroup tail;
  * r (nr), z (nz), kill_wz (2 kill_width), kill_wr (kill_width)); the
  * stepper owns its scratch.  nz must equal 2 nr (the extrapolation uses z
  * as both axes of the doubled grid, as the reference does). */
 typedef struct axs_soft_stepper axs_soft_stepper;
 typedef struct {
   int nr, nz, kill_width;
-  int lsq_sweeps;        /* upper bound on extrapolation sweeps per step (rounded up to even) */
+  int lsq_sweeps;        /* extrapolation sweeps queued per step as launches */
   int reinit_iterations; /* < 0: 2 ceil(reinit_band / dx) + 8 */
   int pad;
   double dx, nu, G, rho_f, CFL, eps;

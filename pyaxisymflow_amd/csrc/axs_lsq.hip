@@ -225,12 +225,12 @@ __global__ void lsq_sweep_pingpong_kernel(
     const double* __restrict__ gy, int n0, int n1, const int* __restrict__ list,
     const unsigned* __restrict__ count, unsigned* __restrict__ counter,
     const unsigned* __restrict__ prev_counter) {
-  const bool ended = prev_counter && *prev_counter == 0u;  // the reference loop has ended
+  if (prev_counter && *prev_counter == 0u) return;  // the reference loop has ended
   const int nl = (int)*count;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nl; k += gridDim.x * blockDim.x) {
     const int c = list[k];
     double vx, vy;
-    if (!ended && (cur_in[c] ^ tgt[c]) &&
+    if ((cur_in[c] ^ tgt[c]) &&
         lsq_fit_cell(cur_in, ex_in, ey_in, gx, gy, n0, n1, c / n1, c % n1, vx, vy)) {
       ex_out[c] = vx;
       ey_out[c] = vy;
@@ -242,6 +242,73 @@ __global__ void lsq_sweep_pingpong_kernel(
       cur_out[c] = cur_in[c];
     }
   }
+}
+
+__global__ void __launch_bounds__(512) lsq_sweep_tail_kernel(
+    int16_t* __restrict__ cur_a, double* __restrict__ ex_a, double* __restrict__ ey_a,
+    int16_t* __restrict__ cur_b, double* __restrict__ ex_b, double* __restrict__ ey_b,
+    const int16_t* __restrict__ tgt, const double* __restrict__ gx, const double* __restrict__ gy,
+    int n0, int n1, const int* __restrict__ list, const unsigned* __restrict__ count,
+    unsigned* __restrict__ counters, int sweeps) {
+  __shared__ unsigned fits[2];
+  if (counters[sweeps - 1] == 0u) {  // the loop ended within the queued sweeps
+    if (threadIdx.x == 0) counters[sweeps] = 0u;
+    return;
+  }
+  // sweep sweeps-1 wrote the second set when sweeps-1 is even: the result
+  // must end there
+  const bool home_b = ((sweeps - 1) & 1) == 0;
+  int16_t* ci = home_b ? cur_b : cur_a;
+  double* xi = home_b ? ex_b : ex_a;
+  double* yi = home_b ? ey_b : ey_a;
+  int16_t* co = home_b ? cur_a : cur_b;
+  double* xo = home_b ? ex_a : ex_b;
+  double* yo = home_b ? ey_a : ey_b;
+  const int nl = (int)*count;
+  unsigned ran = 0;
+  for (;;) {
+    const int par = ran & 1;
+    if (threadIdx.x == 0) fits[par] = 0u;
+    __syncthreads();
+    for (int k = threadIdx.x; k < nl; k += blockDim.x) {
+      const int c = list[k];
+      double vx, vy;
+      if ((ci[c] ^ tgt[c]) && lsq_fit_cell(ci, xi, yi, gx, gy, n0, n1, c / n1, c % n1, vx, vy)) {
+        xo[c] = vx;
+        yo[c] = vy;
+        co[c] = 1;
+        atomicAdd(&fits[par], 1u);
+      } else {
+        xo[c] = xi[c];
+        yo[c] = yi[c];
+        co[c] = ci[c];
+      }
+    }
+    __syncthreads();
+    ++ran;
+    int16_t* tc = ci;
+    ci = co;
+    co = tc;
+    double* t = xi;
+    xi = xo;
+    xo = t;
+    t = yi;
+    yi = yo;
+    yo = t;
+    // the same values for every thread; with current flags inside target
+    // ones every sweep but the last fills a listed cell, so the second
+    // test only bounds the loop for flags that break that rule
+    if (fits[par] == 0u || ran > (unsigned)nl) break;
+  }
+  if (ran & 1) {  // the result is in the other set: copy the listed cells home
+    for (int k = threadIdx.x; k < nl; k += blockDim.x) {
+      const int c = list[k];
+      xo[c] = xi[c];
+      yo[c] = yi[c];
+      co[c] = ci[c];
+    }
+  }
+  if (threadIdx.x == 0) counters[sweeps] = ran;
 }
 
 __global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
@@ -293,6 +360,14 @@ hipError_t launch_lsq_sweep_pingpong(const int16_t* cur_in, const double* ex_in,
   lsq_sweep_pingpong_kernel<<<kBlocks, 256, 0, s>>>(
       cur_in, ex_in, ey_in, cur_out, ex_out, ey_out, tgt, gx, gy, n0, n1, w.list, w.count,
       w.counters + sweep, sweep > 0 ? w.counters + sweep - 1 : nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_lsq_sweep_tail(int16_t* cur, double* ex, double* ey, const int16_t* tgt,
+                                 const double* gx, const double* gy, int n0, int n1, LsqWork w,
+                                 int sweeps, hipStream_t s) {
+  lsq_sweep_tail_kernel<<<1, 512, 0, s>>>(cur, ex, ey, w.cur2, w.ex2, w.ey2, tgt, gx, gy, n0, n1,
+                                           w.list, w.count, w.counters, sweeps);
   return hipGetLastError();
 }
 

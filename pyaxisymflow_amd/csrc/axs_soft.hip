@@ -20,11 +20,13 @@
 //                     (extrapolate_eta_using_least_squares_unb.py:19-24,
 //                     extrapolate_using_least_squares.py:33-35)
 //   lsq_sweeps        least-squares extrapolation sweeps until a sweep
-//                     fits nothing (later sweeps only copy), over a list
-//                     of the cells to fill made once per step; each sweep
-//                     is one launch reading one buffer set and writing the
-//                     other
-//   lsq_back          eta = doubled eta's physical half (:29-30)
+//                     fits nothing, over a list of the cells to fill made
+//                     once per step; each sweep is one launch reading one
+//                     buffer set and writing the other, and returns at
+//                     once after the loop has ended; a loop longer than
+//                     the queued sweeps is finished by one workgroup
+//   lsq_back          eta = doubled eta's physical half (:29-30), from
+//                     the buffer set holding the result
 //   solid_sigma       stresses (solid_sigma.py:4-29), blended by chi (:197-199)
 //   div_tau           omega += dt curl(div sigma) (div_tau.py:4-34)
 //   advect            ENO3 conservative (advect_vorticity_via_eno3.py:8-44)
@@ -153,13 +155,20 @@ __global__ void soft_lsq_mirror_kernel(const double* __restrict__ phi,
   cur2[d] = known;
 }
 
+// Extrapolated eta back into the fields from whichever ping-pong buffer set
+// holds the sweeps' result.
 __global__ void soft_lsq_back_kernel(double* __restrict__ e1, double* __restrict__ e2,
                                      const double* __restrict__ e1d,
-                                     const double* __restrict__ e2d, int64_t n, int64_t off) {
+                                     const double* __restrict__ e2d,
+                                     const double* __restrict__ e1d2,
+                                     const double* __restrict__ e2d2,
+                                     const unsigned* __restrict__ counters, int sweeps, int64_t n,
+                                     int64_t off) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  e1[c] = e1d[off + c];
-  e2[c] = e2d[off + c];
+  const bool second = lsq_result_is_second(counters, sweeps);
+  e1[c] = (second ? e1d2 : e1d)[off + c];
+  e2[c] = (second ? e2d2 : e2d)[off + c];
 }
 
 __global__ void soft_blend_kernel(double* __restrict__ s11, double* __restrict__ s12,
@@ -178,10 +187,15 @@ __global__ void soft_finalize_kernel(double* __restrict__ sc,
                                      unsigned* __restrict__ list_count) {
   if (threadIdx.x != 0) return;
   *list_count = 0u;  // for the next step's list of cells to extrapolate
+  // sweeps that fitted cells: the queued ones, then the tail's (its last
+  // sweep fits nothing)
   int used = 0;
   while (used < sweeps && counters[used] != 0u) ++used;
+  if (used == sweeps && counters[sweeps] != 0u) {
+    used += (int)counters[sweeps] - 1;
+    sc[kSsLsqUnconverged] = sc[kSsLsqUnconverged] + 1.0;  // steps the tail finished
+  }
   sc[kSsLsqSweeps] = double(used);
-  if (counters[sweeps - 1] != 0u) sc[kSsLsqUnconverged] = sc[kSsLsqUnconverged] + 1.0;
   sc[kSsT] = sc[kSsT] + sc[kSsDt];
   sc[kSsIt] = sc[kSsIt] + 1.0;
 }
@@ -229,8 +243,6 @@ struct SoftStepper {
   hipError_t init(const SoftConfig& c, const SoftBuffers& b) {
     cfg = c;
     buf = b;
-    // the ping-pong sweeps end in the first buffer set after an even count
-    cfg.lsq_sweeps += cfg.lsq_sweeps & 1;
     plan = new (std::nothrow) StokesPlan();
     if (!plan) return hipErrorOutOfMemory;
     FftPlanArgs fp{};
@@ -258,7 +270,8 @@ struct SoftStepper {
     if ((e = alloc(&lw.nx, nd * B)) != hipSuccess) return e;
     if ((e = alloc(&lw.ny, nd * B)) != hipSuccess) return e;
     if ((e = alloc(&lw.pend, nd)) != hipSuccess) return e;
-    if ((e = alloc(&lw.counters, sizeof(unsigned) * cfg.lsq_sweeps)) != hipSuccess) return e;
+    if ((e = alloc(&lw.counters, sizeof(unsigned) * (cfg.lsq_sweeps + 1))) != hipSuccess)
+      return e;
     if ((e = alloc(&lw.list, nd * sizeof(int))) != hipSuccess) return e;
     if ((e = alloc(&lw.cur2, nd * sizeof(int16_t))) != hipSuccess) return e;
     if ((e = alloc(&lw.ex2, nd * B)) != hipSuccess) return e;
@@ -323,9 +336,10 @@ struct SoftStepper {
         buf.phi, buf.char_func, buf.eta1, buf.eta2, e1d, e2d, cur, tgt, lw.ex2, lw.ey2, lw.cur2, nr,
         nz, cfg.extrap_zone);
     AXS_TRY_LAUNCH();
-    AXS_TRY_STEP(launch_fill_u32(lw.counters, cfg.lsq_sweeps, 0u, s));
+    AXS_TRY_STEP(launch_fill_u32(lw.counters, cfg.lsq_sweeps + 1, 0u, s));
     AXS_TRY_STEP(launch_lsq_list(cur, tgt, 2 * n, lw, s));
-    // ping-pong sweeps; the bound is even, so the last one lands in cur, e1d, e2d
+    // ping-pong sweeps (sweep q reads the first set when q is even); those
+    // after the loop has ended return at once
     for (int q = 0; q < cfg.lsq_sweeps; ++q) {
       const bool odd = q & 1;
       AXS_TRY_STEP(launch_lsq_sweep_pingpong(odd ? lw.cur2 : cur, odd ? lw.ex2 : e1d,
@@ -333,7 +347,11 @@ struct SoftStepper {
                                              odd ? e1d : lw.ex2, odd ? e2d : lw.ey2, tgt, buf.z,
                                              buf.z, 2 * nr, nz, lw, q, s));
     }
-    soft_lsq_back_kernel<<<ceil_div(n, 256), 256, 0, s>>>(buf.eta1, buf.eta2, e1d, e2d, n, n);
+    // a loop longer than the queued sweeps ends in one workgroup
+    AXS_TRY_STEP(launch_lsq_sweep_tail(cur, e1d, e2d, tgt, buf.z, buf.z, 2 * nr, nz, lw,
+                                       cfg.lsq_sweeps, s));
+    soft_lsq_back_kernel<<<ceil_div(n, 256), 256, 0, s>>>(
+        buf.eta1, buf.eta2, e1d, e2d, lw.ex2, lw.ey2, lw.counters, cfg.lsq_sweeps, n, n);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_solid_sigma<double>(s11, s12, s22, cfg.G, cfg.dx, buf.eta1, buf.eta2, g1z,
                                             g1r, g2z, g2r, nr, nz, s));

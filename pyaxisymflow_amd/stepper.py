@@ -504,11 +504,13 @@ class SoftSphereStepper:
     1 / grid_size_z, nr = domain_AR * nz, moll_zone = 4 dx, extrap_zone =
     reinit_band = moll_zone + 4 dx, sphere of radius r_ball at (0.5, 0),
     vorticity C (kz^2 + kr^2) sin(kz z) sin(kr r) with kz = kr = 2 pi.
-    ``lsq_sweeps`` bounds the extrapolation sweeps per step (default 48; the
-    reference loops until a sweep fits nothing, which takes 18-19 sweeps at
-    first at every grid size, as the band is a fixed number of cells wide,
-    and more as a soft solid deforms over thousands of steps;
-    ``lsq_unconverged()`` counts steps that hit the bound)."""
+    ``lsq_sweeps`` is the number of extrapolation sweeps queued per step as
+    separate launches (default 48). The reference loops until a sweep fits
+    nothing, which takes 18-19 sweeps at first at every grid size (the band
+    is a fixed number of cells wide) and more as a soft solid deforms over
+    thousands of steps; a longer loop is finished by one workgroup, so the
+    result is always the reference loop's. ``lsq_unconverged()`` counts the
+    steps the one-workgroup tail finished."""
 
     def __init__(self, grid_size_z=256, G=1.0, nu=1e-3, rho_f=1.0, r_ball=0.15, domain_AR=0.5,
                  C=5e-2, lsq_sweeps=None, use_graph=True, kill_width=3):
@@ -550,9 +552,10 @@ class SoftSphereStepper:
         cfg = _lib.SoftConfig()
         cfg.nr, cfg.nz, cfg.kill_width = nr, nz, kill_width
         # the reference's loop needs ~ band / dx sweeps at first and more as
-        # the solid deforms (18 after 200 steps at nz = 256; a bound of 32
-        # is hit over 2000 steps of the softest solids, 48 is not); sweeps
-        # after the loop has ended exit on one counter read
+        # the solid deforms (18 after 200 steps at nz = 256; 32 queued
+        # sweeps fall short in 15% of the steps of the softest serving
+        # solids over 2000 steps, 48 in 1 of 48,000); queued sweeps after
+        # the loop has ended return on one counter read
         cfg.lsq_sweeps = int(lsq_sweeps) if lsq_sweeps else 48
         cfg.reinit_iterations = -1
         cfg.dx, cfg.nu, cfg.G, cfg.rho_f, cfg.CFL = dx, nu, G, rho_f, 0.1

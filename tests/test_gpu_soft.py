@@ -13,7 +13,9 @@ two Poisson routes (exact vs the reference's eig route) differ by 1.1e-7 in
 omega after 12 steps.  Bars: after one step every field within 1e-12
 relative L2 (dt within 1e-12); after 12 steps every field no further from
 the oracle's exact route than twice the oracle's exact-vs-eig spread.  The
-smooth Heaviside's sin() is the device's, within an ulp of numpy's.
+smooth Heaviside's sin() is the device's, within an ulp of numpy's.  A
+loop longer than the queued extrapolation sweeps, finished by one
+workgroup, gives the same bits as one that fits in them.
 """
 import numpy as np
 import pytest
@@ -94,3 +96,23 @@ def test_soft_graph_equals_eager():
     ea, eb = a.eta(), b.eta()
     assert np.array_equal(ea[0], eb[0]) and np.array_equal(ea[1], eb[1])
     assert np.array_equal(a.scalars(), b.scalars())
+
+
+@pytest.mark.parametrize("queued", [4, 5])
+def test_soft_tail_finishes_the_loop(queued):
+    """With fewer queued sweeps than the loop needs (18-19 here), the
+    one-workgroup tail carries the loop on: every field and scalar but the
+    tail-step counter equals the run with the default 48, bit for bit, for
+    an even and an odd count (the result returns to the right buffer set)."""
+    from pyaxisymflow_amd.stepper import SoftSphereStepper
+
+    a = SoftSphereStepper(grid_size_z=128, lsq_sweeps=queued)
+    b = SoftSphereStepper(grid_size_z=128)
+    a.step(10)
+    b.step(10)
+    assert a.lsq_unconverged() == 10 and b.lsq_unconverged() == 0
+    assert a.lsq_sweeps_last() == b.lsq_sweeps_last() > queued
+    assert np.array_equal(a.vorticity(), b.vorticity())
+    assert np.array_equal(a.phi(), b.phi())
+    ea, eb = a.eta(), b.eta()
+    assert np.array_equal(ea[0], eb[0]) and np.array_equal(ea[1], eb[1])
