@@ -289,7 +289,10 @@ typedef struct {
   int nr, nz, kill_width;
   int lsq_sweeps;        /* extrapolation sweeps queued per step as launches */
   int reinit_iterations; /* < 0: 2 ceil(reinit_band / dx) + 8 */
-  int pad;
+  int lsq_grow;          /* nonzero: graph runs raise lsq_sweeps to the most sweeps a step
+                            has needed so far + 8, rounded up to 8 (at most 4096), as they
+                            learn it (read back without blocking; the graphs are captured
+                            again); eager runs keep it */
   double dx, nu, G, rho_f, CFL, eps;
   double r_ball, z_cm, r_cm;               /* phi pin: r_ball - |eta - (z_cm, r_cm)| */
   double moll_zone, extrap_zone, reinit_band;

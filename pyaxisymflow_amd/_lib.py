@@ -53,7 +53,7 @@ class SoftConfig(ctypes.Structure):
     """axs_soft_config (include/axisym.h)."""
 
     _fields_ = [(n, ctypes.c_int) for n in (
-        "nr", "nz", "kill_width", "lsq_sweeps", "reinit_iterations", "pad")] + [
+        "nr", "nz", "kill_width", "lsq_sweeps", "reinit_iterations", "lsq_grow")] + [
         (n, ctypes.c_double) for n in (
             "dx", "nu", "G", "rho_f", "CFL", "eps", "r_ball", "z_cm", "r_cm", "moll_zone",
             "extrap_zone", "reinit_band")]

@@ -249,8 +249,12 @@ __global__ void __launch_bounds__(512) lsq_sweep_tail_kernel(
     int16_t* __restrict__ cur_b, double* __restrict__ ex_b, double* __restrict__ ey_b,
     const int16_t* __restrict__ tgt, const double* __restrict__ gx, const double* __restrict__ gy,
     int n0, int n1, const int* __restrict__ list, const unsigned* __restrict__ count,
-    unsigned* __restrict__ counters, int sweeps) {
-  __shared__ unsigned fits[2];
+    unsigned* __restrict__ counters, int sweeps, int* __restrict__ tail, int cap) {
+  // unknown cells (u) and cells fitted by the previous sweep (f), read from
+  // index `par` and written to index par ^ 1 of each
+  int* ul[2] = {tail, tail + cap};
+  int* fl[2] = {tail + 2 * (int64_t)cap, tail + 3 * (int64_t)cap};
+  __shared__ unsigned nu[2], nf[2];
   if (counters[sweeps - 1] == 0u) {  // the loop ended within the queued sweeps
     if (threadIdx.x == 0) counters[sweeps] = 0u;
     return;
@@ -267,21 +271,53 @@ __global__ void __launch_bounds__(512) lsq_sweep_tail_kernel(
   const int nl = (int)*count;
   unsigned ran = 0;
   for (;;) {
-    const int par = ran & 1;
-    if (threadIdx.x == 0) fits[par] = 0u;
+    const int par = ran & 1, nxt = par ^ 1;
+    if (threadIdx.x == 0) {
+      nu[nxt] = 0u;
+      nf[nxt] = 0u;
+    }
     __syncthreads();
-    for (int k = threadIdx.x; k < nl; k += blockDim.x) {
-      const int c = list[k];
-      double vx, vy;
-      if ((ci[c] ^ tgt[c]) && lsq_fit_cell(ci, xi, yi, gx, gy, n0, n1, c / n1, c % n1, vx, vy)) {
-        xo[c] = vx;
-        yo[c] = vy;
-        co[c] = 1;
-        atomicAdd(&fits[par], 1u);
-      } else {
+    if (ran == 0) {
+      // the whole list: fit, or copy through (the two sets differ on the
+      // cells the last queued sweep fitted)
+      for (int k = threadIdx.x; k < nl; k += blockDim.x) {
+        const int c = list[k];
+        double vx, vy;
+        const bool unknown = ci[c] ^ tgt[c];
+        if (unknown && lsq_fit_cell(ci, xi, yi, gx, gy, n0, n1, c / n1, c % n1, vx, vy)) {
+          xo[c] = vx;
+          yo[c] = vy;
+          co[c] = 1;
+          fl[nxt][atomicAdd(&nf[nxt], 1u)] = c;
+        } else {
+          xo[c] = xi[c];
+          yo[c] = yi[c];
+          co[c] = ci[c];
+          if (unknown) ul[nxt][atomicAdd(&nu[nxt], 1u)] = c;
+        }
+      }
+    } else {
+      // cells fitted by the previous sweep: into this sweep's output set
+      const int nfp = (int)nf[par];
+      for (int k = threadIdx.x; k < nfp; k += blockDim.x) {
+        const int c = fl[par][k];
         xo[c] = xi[c];
         yo[c] = yi[c];
         co[c] = ci[c];
+      }
+      // cells still unknown (equal, flag 0, in both sets): fit or keep
+      const int nup = (int)nu[par];
+      for (int k = threadIdx.x; k < nup; k += blockDim.x) {
+        const int c = ul[par][k];
+        double vx, vy;
+        if (lsq_fit_cell(ci, xi, yi, gx, gy, n0, n1, c / n1, c % n1, vx, vy)) {
+          xo[c] = vx;
+          yo[c] = vy;
+          co[c] = 1;
+          fl[nxt][atomicAdd(&nf[nxt], 1u)] = c;
+        } else {
+          ul[nxt][atomicAdd(&nu[nxt], 1u)] = c;
+        }
       }
     }
     __syncthreads();
@@ -298,16 +334,10 @@ __global__ void __launch_bounds__(512) lsq_sweep_tail_kernel(
     // the same values for every thread; with current flags inside target
     // ones every sweep but the last fills a listed cell, so the second
     // test only bounds the loop for flags that break that rule
-    if (fits[par] == 0u || ran > (unsigned)nl) break;
+    if (nf[nxt] == 0u || ran > (unsigned)nl) break;
   }
-  if (ran & 1) {  // the result is in the other set: copy the listed cells home
-    for (int k = threadIdx.x; k < nl; k += blockDim.x) {
-      const int c = list[k];
-      xo[c] = xi[c];
-      yo[c] = yi[c];
-      co[c] = ci[c];
-    }
-  }
+  // the last sweep fitted nothing and copied the previous sweep's fits
+  // across: both sets now hold every fit, the home set included
   if (threadIdx.x == 0) counters[sweeps] = ran;
 }
 
@@ -367,7 +397,7 @@ hipError_t launch_lsq_sweep_tail(int16_t* cur, double* ex, double* ey, const int
                                  const double* gx, const double* gy, int n0, int n1, LsqWork w,
                                  int sweeps, hipStream_t s) {
   lsq_sweep_tail_kernel<<<1, 512, 0, s>>>(cur, ex, ey, w.cur2, w.ex2, w.ey2, tgt, gx, gy, n0, n1,
-                                           w.list, w.count, w.counters, sweeps);
+                                           w.list, w.count, w.counters, sweeps, w.tail, w.cap);
   return hipGetLastError();
 }
 

@@ -16,6 +16,8 @@ struct LsqWork {
   int16_t* cur2;       // second flag and value buffers of the ping-pong sweeps
   double* ex2;
   double* ey2;
+  int* tail;           // the tail's four lists (unknown and just fitted, twice), list capacity each
+  int cap;             // list capacity (entries)
 };
 
 // One sweep (fit + commit).  Sweep `sweep` exits at once when
@@ -49,9 +51,12 @@ hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s);
 // the second (w.cur2, w.ex2, w.ey2): when the last of them still fitted
 // cells, one workgroup carries the reference's loop on over the listed cells
 // (block barriers between sweeps) until a sweep fits nothing, and leaves the
-// result in the set lsq_result_is_second names.  w.counters[sweeps]
-// receives the sweeps it ran (0 when the loop had ended).  It ends within
-// (list entries + 1) sweeps: every sweep but the last fills a listed cell.
+// result in the set lsq_result_is_second names.  Its first sweep covers the
+// list; later ones only the cells still unknown and those fitted by the
+// sweep before (copied across, so each set ends up with every fit), kept in
+// w.tail.  w.counters[sweeps] receives the sweeps it ran (0 when the loop
+// had ended).  It ends within (list entries + 1) sweeps: every sweep but
+// the last fills a listed cell.
 hipError_t launch_lsq_sweep_tail(int16_t* cur, double* ex, double* ey, const int16_t* tgt,
                                  const double* gx, const double* gy, int n0, int n1, LsqWork w,
                                  int sweeps, hipStream_t s);

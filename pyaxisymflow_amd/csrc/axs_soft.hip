@@ -63,7 +63,7 @@ enum : int {
 static_assert(kSsCount <= AXS_RIGID_SCALARS, "soft scalar slots exceed the array");
 
 struct SoftConfig {
-  int nr, nz, kill_width, lsq_sweeps, reinit_iterations;
+  int nr, nz, kill_width, lsq_sweeps, reinit_iterations, lsq_grow;
   double dx, nu, G, rho_f, CFL, eps, r_ball, z_cm, r_cm, moll_zone, extrap_zone, reinit_band;
 };
 
@@ -184,7 +184,8 @@ __global__ void soft_blend_kernel(double* __restrict__ s11, double* __restrict__
 
 __global__ void soft_finalize_kernel(double* __restrict__ sc,
                                      const unsigned* __restrict__ counters, int sweeps,
-                                     unsigned* __restrict__ list_count) {
+                                     unsigned* __restrict__ list_count,
+                                     unsigned* __restrict__ need) {
   if (threadIdx.x != 0) return;
   *list_count = 0u;  // for the next step's list of cells to extrapolate
   // sweeps that fitted cells: the queued ones, then the tail's (its last
@@ -196,6 +197,7 @@ __global__ void soft_finalize_kernel(double* __restrict__ sc,
     sc[kSsLsqUnconverged] = sc[kSsLsqUnconverged] + 1.0;  // steps the tail finished
   }
   sc[kSsLsqSweeps] = double(used);
+  if ((unsigned)used + 1u > *need) *need = (unsigned)used + 1u;  // sweeps this loop took
   sc[kSsT] = sc[kSsT] + sc[kSsDt];
   sc[kSsIt] = sc[kSsIt] + 1.0;
 }
@@ -229,6 +231,14 @@ struct SoftStepper {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   hipStream_t graph_stream = nullptr;
+  // queued-sweep growth: the most sweeps a step has taken (device), its
+  // pinned host copy and the event that says the copy has landed
+  static constexpr int kMaxQueued = 4096;
+  unsigned* need = nullptr;
+  unsigned* need_host = nullptr;
+  hipEvent_t need_ev = nullptr;
+  bool need_pending = false;
+  int since_request = 0;  // graph launches queued after the pending copy
 
   template <typename P>
   hipError_t alloc(P** p, size_t bytes) {
@@ -267,12 +277,18 @@ struct SoftStepper {
     if ((e = alloc(&e2d, nd * B)) != hipSuccess) return e;
     if ((e = alloc(&cur, nd * sizeof(int16_t))) != hipSuccess) return e;
     if ((e = alloc(&tgt, nd * sizeof(int16_t))) != hipSuccess) return e;
-    if ((e = alloc(&lw.nx, nd * B)) != hipSuccess) return e;
-    if ((e = alloc(&lw.ny, nd * B)) != hipSuccess) return e;
-    if ((e = alloc(&lw.pend, nd)) != hipSuccess) return e;
-    if ((e = alloc(&lw.counters, sizeof(unsigned) * (cfg.lsq_sweeps + 1))) != hipSuccess)
-      return e;
+    if ((e = alloc(&lw.counters, sizeof(unsigned) * (kMaxQueued + 1))) != hipSuccess) return e;
+    if ((e = alloc(&need, sizeof(unsigned))) != hipSuccess) return e;
+    if ((e = hipMemset(need, 0, sizeof(unsigned))) != hipSuccess) return e;
+    if (cfg.lsq_grow) {
+      if ((e = hipHostMalloc(reinterpret_cast<void**>(&need_host), sizeof(unsigned),
+                             hipHostMallocDefault)) != hipSuccess)
+        return e;
+      if ((e = hipEventCreateWithFlags(&need_ev, hipEventDisableTiming)) != hipSuccess) return e;
+    }
     if ((e = alloc(&lw.list, nd * sizeof(int))) != hipSuccess) return e;
+    if ((e = alloc(&lw.tail, 4 * nd * sizeof(int))) != hipSuccess) return e;
+    lw.cap = (int)nd;
     if ((e = alloc(&lw.cur2, nd * sizeof(int16_t))) != hipSuccess) return e;
     if ((e = alloc(&lw.ex2, nd * B)) != hipSuccess) return e;
     if ((e = alloc(&lw.ey2, nd * B)) != hipSuccess) return e;
@@ -291,10 +307,9 @@ struct SoftStepper {
   }
 
   ~SoftStepper() {
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
-    if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
+    drop_graphs();
+    if (need_ev) (void)hipEventDestroy(need_ev);
+    if (need_host) (void)hipHostFree(need_host);
     for (void* p : allocs) (void)hipFree(p);
     delete plan;
   }
@@ -364,7 +379,7 @@ struct SoftStepper {
     AXS_TRY_STEP(launch_rk2<double, RRow<double>>(
         w1, tmp, buf.vorticity, RRow<double>{buf.r}, nr, nz, Scalar<double>{0.0, sc + kSsHalfNuDt},
         Scalar<double>{0.0, sc + kSsNuDt}, cfg.dx, s));
-    soft_finalize_kernel<<<1, 64, 0, s>>>(sc, lw.counters, cfg.lsq_sweeps, lw.count);
+    soft_finalize_kernel<<<1, 64, 0, s>>>(sc, lw.counters, cfg.lsq_sweeps, lw.count, need);
     AXS_TRY_LAUNCH();
 #undef AXS_TRY_STEP
 #undef AXS_TRY_LAUNCH
@@ -377,6 +392,66 @@ struct SoftStepper {
   hipGraph_t chunk_graph = nullptr;
   hipGraphExec_t chunk_exec = nullptr;
 
+  void drop_graphs() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
+    if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    chunk_exec = nullptr;
+    chunk_graph = nullptr;
+  }
+
+  hipError_t capture(hipStream_t s) {
+    drop_graphs();
+    for (int q = 0; q < 2; ++q) {
+      hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) return e;
+      hipError_t le = hipSuccess;
+      for (int k = 0; k < (q == 0 ? 1 : kChunkSteps) && le == hipSuccess; ++k)
+        le = enqueue_step(s);
+      hipGraph_t* g = q == 0 ? &graph : &chunk_graph;
+      hipGraphExec_t* x = q == 0 ? &graph_exec : &chunk_exec;
+      e = hipStreamEndCapture(s, g);
+      if (le != hipSuccess) return le;
+      if (e != hipSuccess) return e;
+      if ((e = hipGraphInstantiate(x, *g, nullptr, nullptr, 0)) != hipSuccess) return e;
+    }
+    graph_stream = s;
+    return hipSuccess;
+  }
+
+  // Queued-sweep growth (cfg.lsq_grow): a graph launch is followed by an
+  // asynchronous copy of the most sweeps a step has taken; once it has
+  // landed (polled; waited for only when 16 graph launches are queued behind
+  // it, so the device keeps ~128 steps of work) the queue grows to that + 8,
+  // rounded up to 8, and the graphs are dropped to be captured again.  Loops
+  // that outrun the queue meanwhile are finished by the extrapolation tail.
+  hipError_t poll_need() {
+    if (!need_pending) return hipSuccess;
+    hipError_t e = since_request >= 16 ? hipEventSynchronize(need_ev) : hipEventQuery(need_ev);
+    if (e == hipErrorNotReady) return hipSuccess;
+    if (e != hipSuccess) return e;
+    need_pending = false;
+    const int want = std::min(kMaxQueued, (int)((*need_host + 8u + 7u) / 8u * 8u));
+    if (want > cfg.lsq_sweeps) {
+      cfg.lsq_sweeps = want;
+      drop_graphs();
+    }
+    return hipSuccess;
+  }
+  hipError_t request_need(hipStream_t s) {
+    if (need_pending) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(need_host, need, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipEventRecord(need_ev, s);
+    if (e == hipSuccess) {
+      need_pending = true;
+      since_request = 0;
+    }
+    return e;
+  }
+
   hipError_t run(int nsteps, bool use_graph, hipStream_t s) {
     if (!use_graph) {
       for (int q = 0; q < nsteps; ++q) {
@@ -385,38 +460,17 @@ struct SoftStepper {
       }
       return hipSuccess;
     }
-    if (!graph_exec || graph_stream != s) {
-      if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-      if (graph) (void)hipGraphDestroy(graph);
-      if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
-      if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
-      graph_exec = nullptr;
-      graph = nullptr;
-      chunk_exec = nullptr;
-      chunk_graph = nullptr;
-      for (int q = 0; q < 2; ++q) {
-        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) return e;
-        hipError_t le = hipSuccess;
-        for (int k = 0; k < (q == 0 ? 1 : kChunkSteps) && le == hipSuccess; ++k)
-          le = enqueue_step(s);
-        hipGraph_t* g = q == 0 ? &graph : &chunk_graph;
-        hipGraphExec_t* x = q == 0 ? &graph_exec : &chunk_exec;
-        e = hipStreamEndCapture(s, g);
-        if (le != hipSuccess) return le;
-        if (e != hipSuccess) return e;
-        if ((e = hipGraphInstantiate(x, *g, nullptr, nullptr, 0)) != hipSuccess) return e;
-      }
-      graph_stream = s;
-    }
+    if (graph_stream != s) drop_graphs();
+    hipError_t e = hipSuccess;
     int q = 0;
-    for (; q + kChunkSteps <= nsteps; q += kChunkSteps) {
-      hipError_t e = hipGraphLaunch(chunk_exec, s);
-      if (e != hipSuccess) return e;
-    }
-    for (; q < nsteps; ++q) {
-      hipError_t e = hipGraphLaunch(graph_exec, s);
-      if (e != hipSuccess) return e;
+    while (q < nsteps) {
+      if (cfg.lsq_grow && (e = poll_need()) != hipSuccess) return e;
+      if (!graph_exec && (e = capture(s)) != hipSuccess) return e;
+      const bool chunk = q + kChunkSteps <= nsteps;
+      if ((e = hipGraphLaunch(chunk ? chunk_exec : graph_exec, s)) != hipSuccess) return e;
+      q += chunk ? kChunkSteps : 1;
+      ++since_request;
+      if (cfg.lsq_grow && (e = request_need(s)) != hipSuccess) return e;
     }
     return hipSuccess;
   }
@@ -460,6 +514,7 @@ axs_status axs_soft_stepper_create(const axs_soft_config* c, const axs_soft_buff
   cfg.kill_width = c->kill_width;
   cfg.lsq_sweeps = c->lsq_sweeps;
   cfg.reinit_iterations = c->reinit_iterations;
+  cfg.lsq_grow = c->lsq_grow;
   cfg.dx = c->dx;
   cfg.nu = c->nu;
   cfg.G = c->G;

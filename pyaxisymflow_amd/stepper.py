@@ -504,13 +504,14 @@ class SoftSphereStepper:
     1 / grid_size_z, nr = domain_AR * nz, moll_zone = 4 dx, extrap_zone =
     reinit_band = moll_zone + 4 dx, sphere of radius r_ball at (0.5, 0),
     vorticity C (kz^2 + kr^2) sin(kz z) sin(kr r) with kz = kr = 2 pi.
-    ``lsq_sweeps`` is the number of extrapolation sweeps queued per step as
-    separate launches (default 48). The reference loops until a sweep fits
-    nothing, which takes 18-19 sweeps at first at every grid size (the band
-    is a fixed number of cells wide) and more as a soft solid deforms over
-    thousands of steps; a longer loop is finished by one workgroup, so the
-    result is always the reference loop's. ``lsq_unconverged()`` counts the
-    steps the one-workgroup tail finished."""
+    ``lsq_sweeps`` fixes the number of extrapolation sweeps queued per step
+    as separate launches; by default the queue starts at 24 and grows with
+    the most sweeps a step has needed (+ 8, rounded up to 8) as graph runs
+    learn it. The reference loops until a sweep fits nothing, which takes
+    8-21 sweeps for stiff solids and up to ~60 for the softest as they
+    deform; a loop longer than the queue is finished by one workgroup, so
+    the result is always the reference loop's. ``lsq_unconverged()`` counts
+    the steps the one-workgroup tail finished."""
 
     def __init__(self, grid_size_z=256, G=1.0, nu=1e-3, rho_f=1.0, r_ball=0.15, domain_AR=0.5,
                  C=5e-2, lsq_sweeps=None, use_graph=True, kill_width=3):
@@ -552,11 +553,13 @@ class SoftSphereStepper:
         cfg = _lib.SoftConfig()
         cfg.nr, cfg.nz, cfg.kill_width = nr, nz, kill_width
         # the reference's loop needs ~ band / dx sweeps at first and more as
-        # the solid deforms (18 after 200 steps at nz = 256; 32 queued
-        # sweeps fall short in 15% of the steps of the softest serving
-        # solids over 2000 steps, 48 in 1 of 48,000); queued sweeps after
-        # the loop has ended return on one counter read
-        cfg.lsq_sweeps = int(lsq_sweeps) if lsq_sweeps else 48
+        # the solid deforms (15-21 for G >= 1 at nz = 512, ~60 after 1000
+        # steps at G = 0.1); every queued sweep costs a graph node (~5 us)
+        # even after the loop has ended, a tail sweep ~50 us
+        if lsq_sweeps:
+            cfg.lsq_sweeps, cfg.lsq_grow = int(lsq_sweeps), 0
+        else:
+            cfg.lsq_sweeps, cfg.lsq_grow = 24, 1
         cfg.reinit_iterations = -1
         cfg.dx, cfg.nu, cfg.G, cfg.rho_f, cfg.CFL = dx, nu, G, rho_f, 0.1
         cfg.eps = np.finfo(float).eps
