@@ -116,3 +116,22 @@ def test_soft_tail_finishes_the_loop(queued):
     assert np.array_equal(a.phi(), b.phi())
     ea, eb = a.eta(), b.eta()
     assert np.array_equal(ea[0], eb[0]) and np.array_equal(ea[1], eb[1])
+
+
+def test_soft_growing_queue_equals_fixed():
+    """The default stepper starts with 24 queued sweeps and grows the queue
+    (re-capturing its graphs) as steps need more; over 200 steps every field
+    equals a stepper with a fixed queue of 64, bit for bit."""
+    from pyaxisymflow_amd.stepper import SoftSphereStepper
+
+    a = SoftSphereStepper(grid_size_z=128, G=0.1)
+    b = SoftSphereStepper(grid_size_z=128, G=0.1, lsq_sweeps=64)
+    for _ in range(4):
+        a.step(50)
+        b.step(50)
+    assert b.lsq_unconverged() == 0
+    assert a.lsq_sweeps_last() == b.lsq_sweeps_last()
+    assert np.array_equal(a.vorticity(), b.vorticity())
+    assert np.array_equal(a.phi(), b.phi())
+    ea, eb = a.eta(), b.eta()
+    assert np.array_equal(ea[0], eb[0]) and np.array_equal(ea[1], eb[1])
