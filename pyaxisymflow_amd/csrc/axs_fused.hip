@@ -368,7 +368,6 @@ __device__ __forceinline__ void step_fused_tile(
   for (int k = 0; k < kVS + 2; ++k) touches |= (vc[k] != T(0));
   const bool body = __syncthreads_or(touches) != 0;
 
-  const double mx = L.smax[0];
   const DtScalars ds{L.smax[1], L.smax[2], L.smax[3]};
   const T two_dx = T(2) * dx, inv2dx = T(1) / two_dx;
   const T ufree = T(cfg.U0 * sc[kScPrefac]);
@@ -488,6 +487,10 @@ __device__ __forceinline__ void step_fused_tile(
   }
   __syncthreads();
   }  // body path
+  // the tile's force sum leaves the registers before the ENO3 and RK2
+  // phases (the same wave reduction the end of the kernel would make)
+  acc = wave_sum(acc);
+  if ((tid & 63) == 0) L.wpart[tid >> 6] = acc;
   if constexpr (!IN) {
     if (i0 == 0) {
       // rows -4..-1 of the ENO3 input are the axis mirror of rows 3..0
@@ -510,25 +513,29 @@ __device__ __forceinline__ void step_fused_tile(
   T out[kEH];
   if (eno) {
     const int b = eb, a0 = ea0;
-    const int ib = b + 2;        // R4 column
-    T wf[kEH + 4], wy[kEH + 4];  // R4 rows a0 .. a0 + kEH + 3
-#pragma unroll
-    for (int k = 0; k < kEH + 4; ++k) {
-      wf[k] = sf[a0 + k][ib];
-      wy[k] = sy[a0 + k][ib];
-    }
-    T fr[kEH + 1];  // fr[m]: face between R2 rows a0+m-1 and a0+m
-#pragma unroll
-    for (int m = 0; m <= kEH; ++m)
-      fr[m] = eno3_face_f(wf[m], wf[m + 1], wf[m + 2], wf[m + 3], wy[m], wy[m + 1], wy[m + 2],
-                          wy[m + 3]);
+    const int ib = b + 2;  // R4 column
+    // R4 rows a0+k .. a0+k+3 of this column slide through a window of four;
+    // face m lies between R2 rows a0+m-1 and a0+m: cell k uses faces k and
+    // k+1, so only two are live
+    T w0 = sf[a0][ib], w1 = sf[a0 + 1][ib], w2 = sf[a0 + 2][ib], w3 = sf[a0 + 3][ib];
+    T y0 = sy[a0][ib], y1 = sy[a0 + 1][ib], y2 = sy[a0 + 2][ib], y3 = sy[a0 + 3][ib];
+    T face_lo = eno3_face_f(w0, w1, w2, w3, y0, y1, y2, y3);
     const int j = j0 - 2 + b;
     const bool jok = IN || (j >= 2 && j < nz - 2);
 #pragma unroll
     for (int k = 0; k < kEH; ++k) {
       const int ia = a0 + k + 2;
       const int i = i0 - 2 + a0 + k;
-      T v = wf[k + 2];
+      T v = w2;  // R4 row a0+k+2
+      w0 = w1;
+      w1 = w2;
+      w2 = w3;
+      w3 = sf[a0 + k + 4][ib];
+      y0 = y1;
+      y1 = y2;
+      y2 = y3;
+      y3 = sy[a0 + k + 4][ib];
+      const T face_hi = eno3_face_f(w0, w1, w2, w3, y0, y1, y2, y3);
       if (IN || (jok && i >= 0 && i < nr - 2)) {
         const T fm2 = sf[ia][ib - 2], fm1 = sf[ia][ib - 1], fp1 = sf[ia][ib + 1],
                 fp2 = sf[ia][ib + 2];
@@ -536,11 +543,12 @@ __device__ __forceinline__ void step_fused_tile(
                 xp1 = sx[ia][ib + 1], xp2 = sx[ia][ib + 2];
         T flux = T(0) + s * eno3_face_f(fm1, v, fp1, fp2, xm1, x0, xp1, xp2);
         flux = flux - s * eno3_face_b(fm2, fm1, v, fp1, xm2, xm1, x0, xp1);
-        flux = flux + s * fr[k + 1];
-        flux = flux - s * fr[k];
+        flux = flux + s * face_hi;
+        flux = flux - s * face_lo;
         v = v + flux;
       }
       out[k] = v;
+      face_lo = face_hi;
     }
   }
   __syncthreads();  // u_pen_r (bufD) is read above and overwritten below
@@ -606,12 +614,12 @@ __device__ __forceinline__ void step_fused_tile(
   }
 
   // ---- force partial of this tile; block (0, 0) publishes the dt scalars
-  acc = wave_sum(acc);
-  if ((tid & 63) == 0) L.wpart[tid >> 6] = acc;
+  //      (read back from LDS, not held in registers through the phases)
   __syncthreads();
   if (tid == 0) {
     partials[tile] = ((L.wpart[0] + L.wpart[1]) + L.wpart[2]) + L.wpart[3];
-    if (tile == 0) store_dt_scalars(sc, ds, mx, cfg);
+    if (tile == 0)
+      store_dt_scalars(sc, DtScalars{L.smax[1], L.smax[2], L.smax[3]}, L.smax[0], cfg);
   }
 }
 
