@@ -623,9 +623,21 @@ __device__ __forceinline__ void step_fused_tile(
   }
 }
 
-// 256 threads, at most 128 VGPRs and under 40 KiB of LDS: four blocks per CU.
+// Waves per SIMD the fused kernels are compiled for.  FP64: at most 128
+// VGPRs and under 40 KiB of LDS, four blocks per CU.  FP32 one-grid: the
+// same bound (82 VGPRs, five blocks per CU; a one-grid launch of the
+// precision form has two or three blocks per CU, so more occupancy buys
+// nothing and spills cost 4%).  FP32 uneven batches: at most 64 VGPRs and
+// 20 KiB, eight blocks per CU (the spills this costs are cheaper than the
+// lost occupancy: 367 -> 362 us per batched step).
 template <typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) step_fused_kernel(
+constexpr int kFusedWaves = 4;
+template <typename T>
+constexpr int kRaggedFusedWaves = sizeof(T) == 8 ? 4 : 8;
+
+// 256 threads, waves and LDS as kFusedWaves says.
+template <typename T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFusedWaves<T>))) step_fused_kernel(
     const T* __restrict__ psi, const T* __restrict__ chi, const T* __restrict__ wsrc,
     T* __restrict__ wdst, T* __restrict__ avg, const double* __restrict__ r,
     const double* __restrict__ wz, const double* __restrict__ wr, int kw, int nr, int nz, T dx,
@@ -669,7 +681,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) s
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kRaggedFusedWaves<T>)))
 step_fused_ragged_kernel(const T* __restrict__ psi, const T* __restrict__ chi,
                          const T* __restrict__ wsrc, T* __restrict__ wdst, T* __restrict__ avg,
                          int kw, double* __restrict__ sc, const double* __restrict__ slots,
