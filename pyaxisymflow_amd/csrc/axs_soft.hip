@@ -15,8 +15,8 @@
 //   pin_phi           phi = sphere of eta where phi > -3 dx (:152-156)
 //   reinit            narrow-band signed distance, masked cells keep phi
 //                     (:159-162; axs_reinit.hip)
-//   char_func         chi = smooth Heaviside(phi) (:165-166)
-//   lsq_mirror        doubled-grid phi, eta, flags
+//   lsq_mirror        chi = smooth Heaviside(phi) (:165-166); doubled-grid
+//                     phi, eta, flags; the sweep counters cleared
 //                     (extrapolate_eta_using_least_squares_unb.py:19-24,
 //                     extrapolate_using_least_squares.py:33-35)
 //   lsq_sweeps        least-squares extrapolation sweeps until a sweep
@@ -27,7 +27,8 @@
 //                     the queued sweeps is finished by one workgroup
 //   lsq_back          eta = doubled eta's physical half (:29-30), from
 //                     the buffer set holding the result
-//   solid_sigma       stresses (solid_sigma.py:4-29), blended by chi (:197-199)
+//   solid_sigma       stresses (solid_sigma.py:4-29), blended by chi as they
+//                     are stored (:197-199)
 //   div_tau           omega += dt curl(div sigma) (div_tau.py:4-34)
 //   advect            ENO3 conservative (advect_vorticity_via_eno3.py:8-44)
 //   diffuse           RK2 (diffusion_RK2.py:4-44) -> home buffer
@@ -122,24 +123,33 @@ __global__ void soft_pin_kernel(double* __restrict__ e1, double* __restrict__ e2
   if (phi[c] > -3 * cfg.dx) phi[c] = orig;
 }
 
-// Doubled grid of extrapolate_eta_using_least_squares_unb.py:19-24 and the
-// flags of extrapolate_using_least_squares.py:33-35 (lower bound 0).
+// chi = smooth Heaviside(phi) (:165-166, written from the upper half of the
+// doubled rows), the doubled grid of
+// extrapolate_eta_using_least_squares_unb.py:19-24 and the flags of
+// extrapolate_using_least_squares.py:33-35 (lower bound 0); block (0, 0)
+// also clears the extrapolation sweeps' counters.
 __global__ void soft_lsq_mirror_kernel(const double* __restrict__ phi,
-                                       const double* __restrict__ chi,
+                                       double* __restrict__ chi, double moll_zone,
                                        const double* __restrict__ e1,
                                        const double* __restrict__ e2, double* __restrict__ e1d,
                                        double* __restrict__ e2d, int16_t* __restrict__ cur,
                                        int16_t* __restrict__ tgt, double* __restrict__ e1d2,
                                        double* __restrict__ e2d2, int16_t* __restrict__ cur2,
-                                       int nr, int nz, double extrap_zone) {
+                                       int nr, int nz, double extrap_zone,
+                                       unsigned* __restrict__ counters, int ncounters) {
+  if (blockIdx.x == 0 && blockIdx.y == 0)
+    for (int k = threadIdx.x; k < ncounters; k += blockDim.x) counters[k] = 0u;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int I = blockIdx.y;  // doubled row, [0, 2 nr)
   if (j >= nz) return;
   const bool upper = I >= nr;
   const int i = upper ? I - nr : nr - 1 - I;
   const int64_t c = (int64_t)i * nz + j;
-  const double in = (chi[c] > 0.5) ? 1.0 : 0.0;
-  const double pd = -phi[c];
+  const double p = phi[c];
+  const double h = smooth_heaviside(p, moll_zone);
+  if (upper) chi[c] = h;
+  const double in = (h > 0.5) ? 1.0 : 0.0;
+  const double pd = -p;
   const double a = in * e1[c];
   const double b = in * e2[c];
   const int64_t d = (int64_t)I * nz + j;
@@ -169,17 +179,6 @@ __global__ void soft_lsq_back_kernel(double* __restrict__ e1, double* __restrict
   const bool second = lsq_result_is_second(counters, sweeps);
   e1[c] = (second ? e1d2 : e1d)[off + c];
   e2[c] = (second ? e2d2 : e2d)[off + c];
-}
-
-__global__ void soft_blend_kernel(double* __restrict__ s11, double* __restrict__ s12,
-                                  double* __restrict__ s22, const double* __restrict__ chi,
-                                  int64_t n) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const double x = chi[c];
-  s11[c] = x * s11[c];
-  s12[c] = x * s12[c];
-  s22[c] = x * s22[c];
 }
 
 __global__ void soft_finalize_kernel(double* __restrict__ sc,
@@ -346,12 +345,10 @@ struct SoftStepper {
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_reinit(buf.phi, buf.phi, nullptr, nr, nz, cfg.dx, cfg.reinit_band,
                                cfg.reinit_iterations, rw, s));
-    AXS_TRY_STEP(launch_heaviside<double>(buf.char_func, buf.phi, cfg.moll_zone, n, s));
     soft_lsq_mirror_kernel<<<dim3(ceil_div(nz, 256), 2 * nr), 256, 0, s>>>(
-        buf.phi, buf.char_func, buf.eta1, buf.eta2, e1d, e2d, cur, tgt, lw.ex2, lw.ey2, lw.cur2, nr,
-        nz, cfg.extrap_zone);
+        buf.phi, buf.char_func, cfg.moll_zone, buf.eta1, buf.eta2, e1d, e2d, cur, tgt, lw.ex2,
+        lw.ey2, lw.cur2, nr, nz, cfg.extrap_zone, lw.counters, cfg.lsq_sweeps + 1);
     AXS_TRY_LAUNCH();
-    AXS_TRY_STEP(launch_fill_u32(lw.counters, cfg.lsq_sweeps + 1, 0u, s));
     AXS_TRY_STEP(launch_lsq_list(cur, tgt, 2 * n, lw, s));
     // ping-pong sweeps (sweep q reads the first set when q is even); those
     // after the loop has ended return at once
@@ -369,9 +366,7 @@ struct SoftStepper {
         buf.eta1, buf.eta2, e1d, e2d, lw.ex2, lw.ey2, lw.counters, cfg.lsq_sweeps, n, n);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_solid_sigma<double>(s11, s12, s22, cfg.G, cfg.dx, buf.eta1, buf.eta2, g1z,
-                                            g1r, g2z, g2r, nr, nz, s));
-    soft_blend_kernel<<<ceil_div(n, 256), 256, 0, s>>>(s11, s12, s22, buf.char_func, n);
-    AXS_TRY_LAUNCH();
+                                            g1r, g2z, g2r, nr, nz, s, buf.char_func));
     AXS_TRY_STEP(launch_div_tau<double>(w0, tz, tr, s11, s12, s22, buf.r, nr, nz,
                                         Scalar<double>{0.0, sc + kSsDt}, cfg.dx, s));
     AXS_TRY_STEP(launch_eno3_conservative<double>(w0, buf.u_z, buf.u_r, w1, nr, nz,

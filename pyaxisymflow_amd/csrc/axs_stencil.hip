@@ -112,10 +112,7 @@ __global__ void heaviside_kernel(T* __restrict__ H, const T* __restrict__ phi, T
                                  int64_t n) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const T p = phi[k];
-  T h = (p >= w) ? T(1) : T(0);
-  if (fabs(p) < w) h = h + T(0.5) * (T(1) + p / w + sin(T(kPi) * p / w) / T(kPi));
-  H[k] = h;
+  H[k] = smooth_heaviside(phi[k], w);
 }
 
 // ---------------------------------------------------------------- A8
@@ -200,7 +197,8 @@ __global__ void solid_sigma_kernel(T* __restrict__ s11, T* __restrict__ s12,
                                    T* __restrict__ s22, T G, T dx,
                                    const T* __restrict__ e1, const T* __restrict__ e2,
                                    T* __restrict__ e1z, T* __restrict__ e1r,
-                                   T* __restrict__ e2z, T* __restrict__ e2r, int nr, int nz) {
+                                   T* __restrict__ e2z, T* __restrict__ e2r, int nr, int nz,
+                                   const T* __restrict__ chi) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   int i = blockIdx.y;
   if (j >= nz) return;
@@ -224,10 +222,18 @@ __global__ void solid_sigma_kernel(T* __restrict__ s11, T* __restrict__ s12,
     e1r[c] = a1r;
     e2r[c] = a2r;
   }
-  s12[c] = -G * (a1z * a1r + a2z * a2r);
+  const T t12 = -G * (a1z * a1r + a2z * a2r);
   const T s = (T(0.5) * G) * (a1r * a1r + a2r * a2r - a1z * a1z - a2z * a2z);
-  s11[c] = s;
-  s22[c] = -s;
+  if (chi) {
+    const T x = chi[c];
+    s12[c] = x * t12;
+    s11[c] = x * s;
+    s22[c] = x * -s;
+  } else {
+    s12[c] = t12;
+    s11[c] = s;
+    s22[c] = -s;
+  }
 }
 
 // ---------------------------------------------------------------- A12
@@ -285,9 +291,10 @@ hipError_t launch_heaviside(T* H, const T* phi, T w, int64_t n, hipStream_t s) {
 
 template <typename T>
 hipError_t launch_solid_sigma(T* s11, T* s12, T* s22, T G, T dx, const T* e1, const T* e2, T* e1z,
-                              T* e1r, T* e2z, T* e2r, int nr, int nz, hipStream_t s) {
-  solid_sigma_kernel<T><<<dim3(ceil_div(nz, 256), nr), 256, 0, s>>>(s11, s12, s22, G, dx, e1, e2,
-                                                                    e1z, e1r, e2z, e2r, nr, nz);
+                              T* e1r, T* e2z, T* e2r, int nr, int nz, hipStream_t s,
+                              const T* chi) {
+  solid_sigma_kernel<T><<<dim3(ceil_div(nz, 256), nr), 256, 0, s>>>(
+      s11, s12, s22, G, dx, e1, e2, e1z, e1r, e2z, e2r, nr, nz, chi);
   return hipGetLastError();
 }
 
@@ -310,7 +317,8 @@ template hipError_t launch_heaviside<double>(double*, const double*, double, int
                                              hipStream_t);
 template hipError_t launch_solid_sigma<double>(double*, double*, double*, double, double,
                                                const double*, const double*, double*, double*,
-                                               double*, double*, int, int, hipStream_t);
+                                               double*, double*, int, int, hipStream_t,
+                                               const double*);
 template hipError_t launch_div_tau<double>(double*, double*, double*, const double*,
                                            const double*, const double*, const double*, int, int,
                                            Scalar<double>, double, hipStream_t);
@@ -425,7 +433,7 @@ axs_status axs_solid_sigma_f64(double* sigma_s_11, double* sigma_s_12, double* s
   AXS_CHECK_ARG(nr >= 3 && nz >= 3, "need nr >= 3 and nz >= 3");
   solid_sigma_kernel<double><<<dim3(ceil_div(nz, 256), nr), 256, 0, as_stream(stream)>>>(
       sigma_s_11, sigma_s_12, sigma_s_22, G, dx, eta_1, eta_2, eta_1z, eta_1r, eta_2z, eta_2r,
-      nr, nz);
+      nr, nz, nullptr);
   AXS_LAUNCH_CHECK("solid_sigma");
   return AXS_OK;
 }
