@@ -1,5 +1,7 @@
 // DCT-II / DCT-III of rows with nz = 2048 (half-length complex FFT of
-// M = 1024 points): one wavefront per row, the FFT in registers.
+// M = 1024 points): one wavefront per row, the FFT in registers; nz = 4096
+// (M = 2048): two wavefronts per row, one 1024-point FFT each and a
+// radix-2 join.
 //
 // Four-step split M = 16 x 64 (n = 64 n1 + n2, k = k1 + 16 k2):
 //   pass 1  lane n2 holds x[64 n1 + n2], n1 < 16: DFT-16 over n1 in
@@ -110,15 +112,17 @@ __device__ __forceinline__ Cplx<double> twid(const Cplx<double>* __restrict__ tw
 }
 
 // FFT of the M = 1024 complex values in buf (natural order, lane-shared
-// LDS); result in natural order in buf.  blockDim.x must be 64.
-template <typename T, int SGN>
+// LDS); result in natural order in buf.  One wavefront (lane = threadIdx.x
+// mod 64) per buffer; every wave of the block calls it (block barriers).
+// tw[TS t] = e^{-2 pi i t / 1024} (TS = 2: the table of a 2048-point plan).
+template <typename T, int SGN, int TS = 1>
 __device__ __forceinline__ void fft1024_wave(Cplx<T>* buf, const Cplx<double>* __restrict__ tw) {
-  const int l = threadIdx.x;
+  const int l = threadIdx.x & 63;
   Cplx<T> v[16];
   // ---- pass 1: lane n2 = l
   Cplx<double> t1[15];
 #pragma unroll
-  for (int k1 = 1; k1 < 16; ++k1) t1[k1 - 1] = twid<SGN>(tw, l * k1);
+  for (int k1 = 1; k1 < 16; ++k1) t1[k1 - 1] = twid<SGN>(tw, TS * l * k1);
 #pragma unroll
   for (int n1 = 0; n1 < 16; ++n1) v[n1] = buf[64 * n1 + l];
   dft16<T, SGN>(v);
@@ -135,7 +139,7 @@ __device__ __forceinline__ void fft1024_wave(Cplx<T>* buf, const Cplx<double>* _
     const int k1 = l >> 2, m = l & 3;
     Cplx<double> t2[15];
 #pragma unroll
-    for (int k2a = 1; k2a < 16; ++k2a) t2[k2a - 1] = twid<SGN>(tw, 16 * m * k2a);
+    for (int k2a = 1; k2a < 16; ++k2a) t2[k2a - 1] = twid<SGN>(tw, TS * 16 * m * k2a);
 #pragma unroll
     for (int c = 0; c < 16; ++c) v[c] = buf[k1 * kYStride + m + 4 * c];
     dft16<T, SGN>(v);
@@ -212,6 +216,96 @@ __device__ __forceinline__ void dct2_post_pairs(const Cplx<T>* Z, const Cplx<dou
   if (l == 0) {
     constexpr int k = M / 2;
     const Cplx<T> zk = Z[k];
+    const Cplx<T> sk = cadd(zk, cconj(zk)), dk = csub(zk, cconj(zk));
+    const Cplx<T> wdk = cmul(Cplx<T>{T(ah.x), T(ah.y)}, dk);
+    const Cplx<T> Vk = {T(0.5) * (sk.x + wdk.y), T(0.5) * (sk.y - wdk.x)};
+    const Cplx<T> wk = cmul(Cplx<T>{T(bh.x), T(bh.y)}, Vk);
+    out[k] = wk.x;
+    out[N - k] = -wk.y;
+  }
+}
+
+}  // namespace axs
+
+namespace axs {
+
+// nz = 4096 (M = 2048 = 2 x 1024): the packed sequence z splits into its even
+// and odd samples, each transformed by one wavefront with fft1024_wave (the
+// first buffer holds z_{2p}, the second z_{2p+1}), joined by one radix-2 step
+//   Z_k = E_k + W_2048^k O_k,   Z_{k+1024} = E_k - W_2048^k O_k   (k < 1024)
+// written over E_k and O_k: Z_k for k < 1024 is at k of the first buffer,
+// for k >= 1024 at k - 1024 of the second.  The inverse joins with the
+// conjugate twiddle.
+constexpr int kF2048M = 2048;
+
+// Buffer entry of packed complex sample m (before the FFTs).
+__device__ __forceinline__ int f2048_in_slot(int m) { return ((m & 1) ? kF1024Lds : 0) + (m >> 1); }
+// Buffer entry of transform value k (after the join).
+__device__ __forceinline__ int f2048_out_slot(int k) {
+  return k < kF1024M ? k : kF1024Lds + (k - kF1024M);
+}
+
+// The join over bufs (two buffers of kF1024Lds entries), 128 threads, block
+// barrier after.
+template <typename T, int SGN>
+__device__ __forceinline__ void f2048_join(Cplx<T>* bufs, const Cplx<double>* __restrict__ tw) {
+  const int t = threadIdx.x;
+  Cplx<double> w[kF1024M / 128];
+#pragma unroll
+  for (int q = 0; q < kF1024M / 128; ++q) w[q] = twid<SGN>(tw, t + 128 * q);
+#pragma unroll
+  for (int q = 0; q < kF1024M / 128; ++q) {
+    const int k = t + 128 * q;
+    const Cplx<T> e = bufs[k];
+    const Cplx<T> o = cmul(bufs[kF1024Lds + k], Cplx<T>{T(w[q].x), T(w[q].y)});
+    bufs[k] = cadd(e, o);
+    bufs[kF1024Lds + k] = csub(e, o);
+  }
+  __syncthreads();
+}
+
+// DCT-II post step for M = 2048 (as dct2_post_pairs), 128 threads: thread t
+// takes k = t + 128 q (q < 8) with its partner M - k; thread 0 also takes
+// k = M / 2.
+template <typename T>
+__device__ __forceinline__ void dct2_post_pairs2048(const Cplx<T>* bufs,
+                                                    const Cplx<double>* __restrict__ e1,
+                                                    const Cplx<double>* __restrict__ e2,
+                                                    T* __restrict__ out) {
+  constexpr int M = kF2048M, N = 2 * kF2048M, KP = M / 2 / 128;
+  const int t = threadIdx.x;
+  Cplx<double> a1[KP], b1[KP], a2[KP], b2[KP];
+#pragma unroll
+  for (int q = 0; q < KP; ++q) {
+    const int k = t + 128 * q;
+    a1[q] = e1[k];
+    b1[q] = e1[M - k];
+    a2[q] = e2[k];
+    b2[q] = e2[M - k];
+  }
+  const Cplx<double> ah = e1[M / 2], bh = e2[M / 2];
+#pragma unroll
+  for (int q = 0; q < KP; ++q) {
+    const int k = t + 128 * q, kp = M - k;
+    const Cplx<T> zk = bufs[f2048_out_slot(k)], zp = bufs[f2048_out_slot(kp & (M - 1))];
+    const Cplx<T> sk = cadd(zk, cconj(zp)), dk = csub(zk, cconj(zp));
+    const Cplx<T> wdk = cmul(Cplx<T>{T(a1[q].x), T(a1[q].y)}, dk);
+    const Cplx<T> Vk = {T(0.5) * (sk.x + wdk.y), T(0.5) * (sk.y - wdk.x)};
+    const Cplx<T> wk = cmul(Cplx<T>{T(a2[q].x), T(a2[q].y)}, Vk);
+    const Cplx<T> sp = cadd(zp, cconj(zk)), dp = csub(zp, cconj(zk));
+    const Cplx<T> wdp = cmul(Cplx<T>{T(b1[q].x), T(b1[q].y)}, dp);
+    const Cplx<T> Vp = {T(0.5) * (sp.x + wdp.y), T(0.5) * (sp.y - wdp.x)};
+    const Cplx<T> wp = cmul(Cplx<T>{T(b2[q].x), T(b2[q].y)}, Vp);
+    out[k] = wk.x;
+    out[kp] = wp.x;
+    if (k > 0) {
+      out[N - k] = -wk.y;
+      out[N - kp] = -wp.y;
+    }
+  }
+  if (t == 0) {
+    constexpr int k = M / 2;
+    const Cplx<T> zk = bufs[f2048_out_slot(k)];
     const Cplx<T> sk = cadd(zk, cconj(zk)), dk = csub(zk, cconj(zk));
     const Cplx<T> wdk = cmul(Cplx<T>{T(ah.x), T(ah.y)}, dk);
     const Cplx<T> Vk = {T(0.5) * (sk.x + wdk.y), T(0.5) * (sk.y - wdk.x)};

@@ -577,6 +577,146 @@ __global__ void __launch_bounds__(64) dct3_rows_m1024_kernel(
   for (int q = 0; q < N / 64; ++q) out[l + 64 * q] = zr[d0 + dq * q] * invN;
 }
 
+// nz = 4096 row passes: two wavefronts per row (axs_fft1024.hpp, M = 2048).
+// Same inputs, outputs and taper as the kernels above.
+template <typename T, bool TAPER>
+__global__ void __launch_bounds__(128) dct2_rows_m2048_kernel(
+    const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
+    const double* __restrict__ rscale, const double* __restrict__ wz,
+    const double* __restrict__ wr, int nr, int w, const Cplx<double>* __restrict__ tw,
+    const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2, StepFinalize fin) {
+  constexpr int N = 2 * kF2048M;
+  if (fin.partials && blockIdx.x == nr) {  // completes the previous step
+    __shared__ double fin_red[256];
+    const int cb = blockIdx.y;
+    const RigidConfig& cfg = fin.cfgs[cb];
+    finalize_pending(fin.partials + (int64_t)cb * fin.count, fin.count,
+                     fin.slots + (int64_t)cb * kSlotsPerCase, fin.sc + (int64_t)cb * AXS_RIGID_SCALARS,
+                     fin.drag ? fin.drag + (int64_t)cb * cfg.history : nullptr, cfg, fin_red);
+    return;
+  }
+  __shared__ Cplx<T> bufs[2 * kF1024Lds];
+  const int t = threadIdx.x;
+  const int i = blockIdx.x;
+  const int64_t cs = (int64_t)blockIdx.y * nr * N;  // case of a batch
+  src += cs;
+  dst += cs;
+  const int srow = (TAPER && i >= nr - w) ? nr - w : i;
+  const T* row = src + (int64_t)srow * N;
+  T v[N / 128];
+#pragma unroll
+  for (int q = 0; q < N / 128; ++q) v[q] = row[t + 128 * q];
+  T left = T(0), right = T(0), rowf = T(1);
+  if (TAPER) {
+    left = row[w - 1];
+    right = row[N - w];
+    if (i >= nr - w) rowf = T(wr[i - (nr - w)]);
+  }
+  const T rs = rscale ? T(rscale[i]) : T(1);
+  T* out_t = (TAPER && tapered) ? tapered + cs + (int64_t)i * N : nullptr;
+  // Makhoul's reorder into the packed sequence, its even and odd complex
+  // samples into the two buffers
+  T* zr = reinterpret_cast<T*>(bufs);
+#pragma unroll
+  for (int q = 0; q < N / 128; ++q) {
+    const int j = t + 128 * q;
+    T x = v[q];
+    if (TAPER) {
+      if (j < w)
+        x = T(wz[j]) * left;
+      else if (j >= N - w)
+        x = T(wz[w + (j - (N - w))]) * right;
+      if (i >= nr - w) x = rowf * x;
+      if (i == 0) x = T(0);
+      if (out_t) out_t[j] = x;
+    }
+    const int d = dct_dst_index(j, N);  // real slot of the packed sequence
+    zr[2 * f2048_in_slot(d >> 1) + (d & 1)] = rs * x;
+  }
+  __syncthreads();
+  fft1024_wave<T, -1, 2>(bufs + (t >> 6) * kF1024Lds, tw);
+  f2048_join<T, -1>(bufs, tw);
+  dct2_post_pairs2048<T>(bufs, e1, e2, dst + (int64_t)i * N);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(128) dct3_rows_m2048_kernel(
+    const T* __restrict__ src, T* __restrict__ dst, const Cplx<double>* __restrict__ tw,
+    const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2) {
+  constexpr int M = kF2048M, N = 2 * kF2048M;
+  constexpr int KP = M / 2 / 128;  // index pairs (k, M - k) per thread
+  typedef T T2 __attribute__((ext_vector_type(2)));
+  __shared__ Cplx<T> bufs[2 * kF1024Lds];
+  static_assert(2 * 2 * kF1024Lds >= N, "the staged row fits the two buffers");
+  T* raw = reinterpret_cast<T*>(bufs);  // the input row, staged before the pre step
+  const int t = threadIdx.x;
+  const int i = blockIdx.x;
+  const int64_t cs = (int64_t)blockIdx.y * gridDim.x * N;  // case of a batch
+  const T2* row = reinterpret_cast<const T2*>(src + cs + (int64_t)i * N);
+  T2 v[N / 256];
+#pragma unroll
+  for (int q = 0; q < N / 256; ++q) v[q] = row[t + 128 * q];
+  Cplx<double> q2k[KP], q2m[KP], q1k[KP], q1m[KP];
+#pragma unroll
+  for (int q = 0; q < KP; ++q) {
+    const int k = t + 128 * q;
+    q2k[q] = e2[k];
+    q2m[q] = e2[M - k];
+    q1k[q] = e1[k];
+    q1m[q] = e1[M - k];
+  }
+  const Cplx<double> q2h = e2[M / 2], q1h = e1[M / 2];
+#pragma unroll
+  for (int q = 0; q < N / 256; ++q) reinterpret_cast<T2*>(raw)[t + 128 * q] = v[q];
+  __syncthreads();
+  T xk[KP], xnk[KP], xm[KP], xnm[KP];
+#pragma unroll
+  for (int q = 0; q < KP; ++q) {
+    const int k = t + 128 * q;
+    xk[q] = raw[k];
+    xnk[q] = (k == 0) ? T(0) : raw[N - k];
+    xm[q] = raw[M - k];
+    xnm[q] = raw[M + k];
+  }
+  const T xh = raw[M / 2], xnh = raw[N - M / 2];
+  __syncthreads();  // raw is read above and overwritten by Z below
+  // DCT-III pre step (as dct3_rows_m1024_kernel), Z_k to its split slot
+#pragma unroll
+  for (int q = 0; q < KP; ++q) {
+    const int k = t + 128 * q;
+    const Cplx<T> Vk = cmul(Cplx<T>{T(q2k[q].x), T(-q2k[q].y)}, Cplx<T>{xk[q], -xnk[q]});
+    const Cplx<T> Vm = cmul(Cplx<T>{T(q2m[q].x), T(-q2m[q].y)}, Cplx<T>{xm[q], -xnm[q]});
+    {
+      const Cplx<T> A = cadd(Vk, cconj(Vm)), D = csub(Vk, cconj(Vm));
+      const Cplx<T> B = cmul(D, Cplx<T>{T(q1k[q].x), T(-q1k[q].y)});
+      bufs[f2048_in_slot(k)] = Cplx<T>{A.x - B.y, A.y + B.x};
+    }
+    if (k != 0) {
+      const Cplx<T> A = cadd(Vm, cconj(Vk)), D = csub(Vm, cconj(Vk));
+      const Cplx<T> B = cmul(D, Cplx<T>{T(q1m[q].x), T(-q1m[q].y)});
+      bufs[f2048_in_slot(M - k)] = Cplx<T>{A.x - B.y, A.y + B.x};
+    }
+  }
+  if (t == 0) {  // k = M / 2 is its own partner
+    const Cplx<T> Vk = cmul(Cplx<T>{T(q2h.x), T(-q2h.y)}, Cplx<T>{xh, -xnh});
+    const Cplx<T> A = cadd(Vk, cconj(Vk)), D = csub(Vk, cconj(Vk));
+    const Cplx<T> B = cmul(D, Cplx<T>{T(q1h.x), T(-q1h.y)});
+    bufs[f2048_in_slot(M / 2)] = Cplx<T>{A.x - B.y, A.y + B.x};
+  }
+  __syncthreads();
+  fft1024_wave<T, +1, 2>(bufs + (t >> 6) * kF1024Lds, tw);
+  f2048_join<T, +1>(bufs, tw);
+  const T invN = T(1) / T(N);
+  T* out = dst + cs + (int64_t)i * N;
+  const T* zr = reinterpret_cast<const T*>(bufs);
+#pragma unroll
+  for (int q = 0; q < N / 128; ++q) {
+    const int j = t + 128 * q;
+    const int d = dct_dst_index(j, N);
+    out[j] = zr[2 * f2048_out_slot(d >> 1) + (d & 1)] * invN;
+  }
+}
+
 // Column pass: for each z-mode k solve the tridiagonal system
 //   a_i x_{i-1} + (b_i + lam_k) x_i + c_i x_{i+1} = d_i,   i in [0, nr)
 // in place on d (row-major (nr, nz), k contiguous).  Workgroup = K modes x
@@ -858,6 +998,12 @@ hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const do
                                                           StepFinalize{});
     return hipGetLastError();
   }
+  if (M == kF2048M) {
+    dct2_rows_m2048_kernel<T, false><<<rows, 128, 0, s>>>(src, nullptr, dst, rscale, nullptr,
+                                                           nullptr, rows, 0, pl.tw, pl.e1, pl.e2,
+                                                           StepFinalize{});
+    return hipGetLastError();
+  }
   if (per <= 1)
     dct2_rows_kernel<T, 1><<<rows, bs, shm, s>>>(src, dst, rscale, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
   else if (per <= 2)
@@ -881,6 +1027,11 @@ hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered
   const size_t shm = sizeof(Cplx<T>) * M;
   if (M == kF1024M) {
     dct2_rows_m1024_kernel<T, true><<<dim3(gx, batch), 64, 0, s>>>(
+        src, tapered, dst, pl.rvec, wz, wr, pl.nr, w, pl.tw, pl.e1, pl.e2, f);
+    return hipGetLastError();
+  }
+  if (M == kF2048M) {
+    dct2_rows_m2048_kernel<T, true><<<dim3(gx, batch), 128, 0, s>>>(
         src, tapered, dst, pl.rvec, wz, wr, pl.nr, w, pl.tw, pl.e1, pl.e2, f);
     return hipGetLastError();
   }
@@ -909,6 +1060,10 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
   const dim3 grid(rows, batch);
   if (M == kF1024M) {
     dct3_rows_m1024_kernel<T><<<grid, 64, 0, s>>>(src, dst, pl.tw, pl.e1, pl.e2);
+    return hipGetLastError();
+  }
+  if (M == kF2048M) {
+    dct3_rows_m2048_kernel<T><<<grid, 128, 0, s>>>(src, dst, pl.tw, pl.e1, pl.e2);
     return hipGetLastError();
   }
   if (per <= 1)

@@ -241,17 +241,15 @@ def test_stokes_mixed_radix(nz, nr):
     assert rel_l2(psi, exact_stokes_solve(w, dx)) < 1e-12
 
 
-@pytest.mark.parametrize("nr", [64, 131])
-def test_stokes_nz2048_wave_per_row(nr):
-    """nz = 2048 runs the one-wave-per-row register FFT (axs_fft1024.hpp):
-    against the exact discrete solution, plus an inverse check: the DCT-II
-    of r omega followed by the DCT-III returns r omega (via a solve of a
-    field whose spectrum is the identity on a single mode)."""
+@pytest.mark.parametrize("nz,nr", [(2048, 64), (2048, 131), (4096, 64), (4096, 37)])
+def test_stokes_register_fft_rows(nz, nr):
+    """nz = 2048 runs the one-wave-per-row register FFT (axs_fft1024.hpp),
+    nz = 4096 two waves per row (one 1024-point FFT each and a radix-2
+    join): against the exact discrete solution and its residual."""
     from pyaxisymflow_amd.kernels.FastDiagonalisationStokesSolver import (
         FastDiagonalisationStokesSolver,
     )
 
-    nz = 2048
     dx = 1.0 / nz
     z = np.linspace(dx / 2, 1 - dx / 2, nz)
     r = np.linspace(dx / 2, nr * dx - dx / 2, nr)
@@ -261,7 +259,7 @@ def test_stokes_nz2048_wave_per_row(nr):
     psi = np.zeros_like(w)
     FastDiagonalisationStokesSolver(nr, nz, dx).solve(solution_field=psi, rhs_field=w)
     exact = exact_stokes_solve(w, dx)
-    print(f"nz=2048 nr={nr}: rel L2 vs exact {rel_l2(psi, exact):.2e}", flush=True)
+    print(f"nz={nz} nr={nr}: rel L2 vs exact {rel_l2(psi, exact):.2e}", flush=True)
     assert rel_l2(psi, exact) < 1e-12
     assert stokes_residual(psi, w, dx) <= max(2 * stokes_residual(exact, w, dx), 1e-13)
 
