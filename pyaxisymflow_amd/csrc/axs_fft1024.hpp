@@ -26,6 +26,14 @@ constexpr int kYStride = 68;                          // Y row (k1) stride, comp
 constexpr int kEStrideM = 20, kEStrideK = 80;         // E[k2a][m][k1] strides
 constexpr int kF1024Lds = 16 * kEStrideK;             // complex entries (>= M, >= 16*68)
 static_assert(kF1024Lds >= kF1024M && kF1024Lds >= 16 * kYStride, "LDS buffer");
+// M = 512 = 8 x 8 x 8 (fft512_wave): Y[k1][n2] rows of 64 + 4, E[k2a][m][k1]
+constexpr int kF512M = 512;
+constexpr int kY512 = 68, kE5M = 8, kE5K = 72;
+constexpr int kF512Lds = 576;
+static_assert(kF512Lds >= 8 * kY512 && kF512Lds >= 7 * kE5K + 7 * kE5M + 8, "LDS buffer");
+// LDS entries of the one-wave transform of M points
+template <int M>
+constexpr int kWaveLds = M == kF1024M ? kF1024Lds : kF512Lds;
 
 template <typename T, int SGN>
 __device__ __forceinline__ void dft4n(Cplx<T>& a0, Cplx<T>& a1, Cplx<T>& a2, Cplx<T>& a3) {
@@ -105,6 +113,25 @@ __device__ __forceinline__ void dft16(Cplx<T> (&x)[16]) {
   for (int ka = 0; ka < 4; ++ka) dft4n<T, SGN>(x[4 * ka], x[4 * ka + 1], x[4 * ka + 2], x[4 * ka + 3]);
 }
 
+// DFT-8 in place, natural order: the DFT-4s of the even and odd samples,
+// joined by W_8^k = W_16^(2k).
+template <typename T, int SGN>
+__device__ __forceinline__ void dft8(Cplx<T> (&x)[8]) {
+  dft4n<T, SGN>(x[0], x[2], x[4], x[6]);
+  dft4n<T, SGN>(x[1], x[3], x[5], x[7]);
+  const Cplx<T> o0 = x[1], o1 = w16<T, SGN, 2>(x[3]), o2 = w16<T, SGN, 4>(x[5]),
+                o3 = w16<T, SGN, 6>(x[7]);
+  const Cplx<T> e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+  x[0] = cadd(e0, o0);
+  x[4] = csub(e0, o0);
+  x[1] = cadd(e1, o1);
+  x[5] = csub(e1, o1);
+  x[2] = cadd(e2, o2);
+  x[6] = csub(e2, o2);
+  x[3] = cadd(e3, o3);
+  x[7] = csub(e3, o3);
+}
+
 template <int SGN>
 __device__ __forceinline__ Cplx<double> twid(const Cplx<double>* __restrict__ tw, int t) {
   const Cplx<double> w = tw[t];
@@ -170,19 +197,83 @@ __device__ __forceinline__ void fft1024_wave(Cplx<T>* buf, const Cplx<double>* _
   __syncthreads();
 }
 
+// FFT of the M = 512 complex values in buf, one wavefront, three radix-8
+// passes (n = n2 + 64 n1, k = k1 + 8 k', n2 = m + 8 c, k' = k2a + 8 k2b):
+//   pass 1  lane n2: DFT-8 over n1, times W_512^(n2 k1)       -> Y[k1][n2]
+//   pass 2  lane (k1, m): DFT-8 over c, times W_64^(m k2a)     -> E[k2a][m][k1]
+//   pass 3  lane (k2a, k1): DFT-8 over m                       -> X[k1 + 8 k2a + 64 k2b]
+// tw[t] = e^{-2 pi i t / 512}.  Natural order in and out of buf.
+template <typename T, int SGN>
+__device__ __forceinline__ void fft512_wave(Cplx<T>* buf, const Cplx<double>* __restrict__ tw) {
+  const int l = threadIdx.x & 63;
+  Cplx<T> v[8];
+  {
+    Cplx<double> t1[7];
+#pragma unroll
+    for (int k1 = 1; k1 < 8; ++k1) t1[k1 - 1] = twid<SGN>(tw, l * k1);
+#pragma unroll
+    for (int n1 = 0; n1 < 8; ++n1) v[n1] = buf[64 * n1 + l];
+    dft8<T, SGN>(v);
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) {
+      Cplx<T> b = v[k1];
+      if (k1 > 0) b = cmul(b, Cplx<T>{T(t1[k1 - 1].x), T(t1[k1 - 1].y)});
+      buf[k1 * kY512 + l] = b;
+    }
+  }
+  __syncthreads();
+  {
+    const int k1 = l >> 3, m = l & 7;
+    Cplx<double> t2[7];
+#pragma unroll
+    for (int k2a = 1; k2a < 8; ++k2a) t2[k2a - 1] = twid<SGN>(tw, 8 * m * k2a);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = buf[k1 * kY512 + m + 8 * c];
+    dft8<T, SGN>(v);
+    __syncthreads();
+#pragma unroll
+    for (int k2a = 0; k2a < 8; ++k2a) {
+      Cplx<T> e = v[k2a];
+      if (k2a > 0) e = cmul(e, Cplx<T>{T(t2[k2a - 1].x), T(t2[k2a - 1].y)});
+      buf[k2a * kE5K + m * kE5M + k1] = e;
+    }
+  }
+  __syncthreads();
+  {
+    const int k2a = l >> 3, k1 = l & 7;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = buf[k2a * kE5K + m * kE5M + k1];
+    dft8<T, SGN>(v);
+    __syncthreads();
+#pragma unroll
+    for (int k2b = 0; k2b < 8; ++k2b) buf[k1 + 8 * k2a + 64 * k2b] = v[k2b];
+  }
+  __syncthreads();
+}
+
+// The one-wave transform of M = 512 or 1024 points.
+template <typename T, int SGN, int M>
+__device__ __forceinline__ void fft_wave(Cplx<T>* buf, const Cplx<double>* __restrict__ tw) {
+  if constexpr (M == kF1024M)
+    fft1024_wave<T, SGN>(buf, tw);
+  else
+    fft512_wave<T, SGN>(buf, tw);
+}
+
 }  // namespace axs
 
 namespace axs {
 
-// DCT-II post step (as dct2_post): lane l takes k = l + 64 q (q < 8) with
+// DCT-II post step (as dct2_post): lane l takes k = l + 64 q (q < M / 128) with
 // its partner M - k, writing outputs k, N - k, M - k and N - (M - k); lane 0
 // also takes k = M / 2, its own partner.  The twiddles are loaded before
 // any is used.
-template <typename T>
+template <typename T, int M>
 __device__ __forceinline__ void dct2_post_pairs(const Cplx<T>* Z, const Cplx<double>* __restrict__ e1,
                                                 const Cplx<double>* __restrict__ e2,
                                                 T* __restrict__ out) {
-  constexpr int M = kF1024M, N = 2 * kF1024M, KP = M / 2 / 64;
+  constexpr int N = 2 * M, KP = M / 2 / 64;
   const int l = threadIdx.x;
   Cplx<double> a1[KP], b1[KP], a2[KP], b2[KP];
 #pragma unroll

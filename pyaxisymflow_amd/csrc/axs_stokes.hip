@@ -430,15 +430,16 @@ __global__ void __launch_bounds__(256) dct3_rows_kernel(
                           0, 0);
 }
 
-// nz = 2048 row passes: one wavefront per row, FFT in registers
-// (axs_fft1024.hpp).  Same inputs, outputs and taper as the kernels above.
-template <typename T, bool TAPER>
-__global__ void __launch_bounds__(64) dct2_rows_m1024_kernel(
+// nz = 2M = 1024 and 2048 row passes: one wavefront per row, FFT in
+// registers (axs_fft1024.hpp).  Same inputs, outputs and taper as the
+// kernels above.
+template <typename T, bool TAPER, int M>
+__global__ void __launch_bounds__(64) dct2_rows_wave_kernel(
     const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
     const double* __restrict__ rscale, const double* __restrict__ wz,
     const double* __restrict__ wr, int nr, int w, const Cplx<double>* __restrict__ tw,
     const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2, StepFinalize fin) {
-  constexpr int N = 2 * kF1024M;
+  constexpr int N = 2 * M;
   if (fin.partials && blockIdx.x == nr) {  // completes the previous step
     __shared__ double fin_red[256];
     const int cb = blockIdx.y;
@@ -448,7 +449,7 @@ __global__ void __launch_bounds__(64) dct2_rows_m1024_kernel(
                      fin.drag ? fin.drag + (int64_t)cb * cfg.history : nullptr, cfg, fin_red);
     return;
   }
-  __shared__ Cplx<T> buf[kF1024Lds];
+  __shared__ Cplx<T> buf[kWaveLds<M>];
   const int l = threadIdx.x;
   const int i = blockIdx.x;
   const int64_t cs = (int64_t)blockIdx.y * nr * N;  // case of a batch
@@ -492,19 +493,20 @@ __global__ void __launch_bounds__(64) dct2_rows_m1024_kernel(
     zr[d0 + dq * q] = rs * t;
   }
   __syncthreads();
-  fft1024_wave<T, -1>(buf, tw);
+  fft_wave<T, -1, M>(buf, tw);
   T* out = dst + (int64_t)i * N;
-  dct2_post_pairs<T>(buf, e1, e2, out);
+  dct2_post_pairs<T, M>(buf, e1, e2, out);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(64) dct3_rows_m1024_kernel(
+template <typename T, int M>
+__global__ void __launch_bounds__(64) dct3_rows_wave_kernel(
     const T* __restrict__ src, T* __restrict__ dst, const Cplx<double>* __restrict__ tw,
     const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2) {
-  constexpr int M = kF1024M, N = 2 * kF1024M;
+  constexpr int N = 2 * M;
   constexpr int KP = M / 2 / 64;  // index pairs (k, M - k) per lane
   typedef T T2 __attribute__((ext_vector_type(2)));
-  __shared__ Cplx<T> buf[kF1024Lds];
+  __shared__ Cplx<T> buf[kWaveLds<M>];
+  static_assert(2 * kWaveLds<M> >= N, "the staged row fits the buffer");
   T* raw = reinterpret_cast<T*>(buf);  // the input row, staged before the pre step
   const int l = threadIdx.x;
   const int i = blockIdx.x;
@@ -565,7 +567,7 @@ __global__ void __launch_bounds__(64) dct3_rows_m1024_kernel(
     buf[M / 2] = Cplx<T>{A.x - B.y, A.y + B.x};
   }
   __syncthreads();
-  fft1024_wave<T, +1>(buf, tw);
+  fft_wave<T, +1, M>(buf, tw);
   const T invN = T(1) / T(N);
   T* out = dst + cs + (int64_t)i * N;
   // output j = l + 64 q reads packed slot j / 2 or N - 1 - (j - 1) / 2 (the
@@ -680,7 +682,7 @@ __global__ void __launch_bounds__(128) dct3_rows_m2048_kernel(
   }
   const T xh = raw[M / 2], xnh = raw[N - M / 2];
   __syncthreads();  // raw is read above and overwritten by Z below
-  // DCT-III pre step (as dct3_rows_m1024_kernel), Z_k to its split slot
+  // DCT-III pre step (as dct3_rows_wave_kernel), Z_k to its split slot
 #pragma unroll
   for (int q = 0; q < KP; ++q) {
     const int k = t + 128 * q;
@@ -992,10 +994,13 @@ hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const do
   // worst stage is radix 2 (M/2 butterflies); 2*per*bs >= M covers the row loads
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
-  if (M == kF1024M) {
-    dct2_rows_m1024_kernel<T, false><<<rows, 64, 0, s>>>(src, nullptr, dst, rscale, nullptr,
-                                                          nullptr, rows, 0, pl.tw, pl.e1, pl.e2,
-                                                          StepFinalize{});
+  if (M == kF1024M || M == kF512M) {
+    if (M == kF1024M)
+      dct2_rows_wave_kernel<T, false, kF1024M><<<rows, 64, 0, s>>>(
+          src, nullptr, dst, rscale, nullptr, nullptr, rows, 0, pl.tw, pl.e1, pl.e2, StepFinalize{});
+    else
+      dct2_rows_wave_kernel<T, false, kF512M><<<rows, 64, 0, s>>>(
+          src, nullptr, dst, rscale, nullptr, nullptr, rows, 0, pl.tw, pl.e1, pl.e2, StepFinalize{});
     return hipGetLastError();
   }
   if (M == kF2048M) {
@@ -1025,9 +1030,13 @@ hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered
   const int bs = fft_block(M);
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
-  if (M == kF1024M) {
-    dct2_rows_m1024_kernel<T, true><<<dim3(gx, batch), 64, 0, s>>>(
-        src, tapered, dst, pl.rvec, wz, wr, pl.nr, w, pl.tw, pl.e1, pl.e2, f);
+  if (M == kF1024M || M == kF512M) {
+    if (M == kF1024M)
+      dct2_rows_wave_kernel<T, true, kF1024M><<<dim3(gx, batch), 64, 0, s>>>(
+          src, tapered, dst, pl.rvec, wz, wr, pl.nr, w, pl.tw, pl.e1, pl.e2, f);
+    else
+      dct2_rows_wave_kernel<T, true, kF512M><<<dim3(gx, batch), 64, 0, s>>>(
+          src, tapered, dst, pl.rvec, wz, wr, pl.nr, w, pl.tw, pl.e1, pl.e2, f);
     return hipGetLastError();
   }
   if (M == kF2048M) {
@@ -1058,8 +1067,11 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
   const dim3 grid(rows, batch);
-  if (M == kF1024M) {
-    dct3_rows_m1024_kernel<T><<<grid, 64, 0, s>>>(src, dst, pl.tw, pl.e1, pl.e2);
+  if (M == kF1024M || M == kF512M) {
+    if (M == kF1024M)
+      dct3_rows_wave_kernel<T, kF1024M><<<grid, 64, 0, s>>>(src, dst, pl.tw, pl.e1, pl.e2);
+    else
+      dct3_rows_wave_kernel<T, kF512M><<<grid, 64, 0, s>>>(src, dst, pl.tw, pl.e1, pl.e2);
     return hipGetLastError();
   }
   if (M == kF2048M) {
