@@ -25,7 +25,8 @@ struct RaggedGroup {
   int vt0, vz, vtiles;       // velocity_max: first tile, tiles along z, tiles per case
   int ft0, fz, ftiles;       // step_fused: first tile, tiles along z, tiles per case
   int part0;                 // first force partial of the group
-  int pad;
+  int wave;                  // 1: the row passes run the one-grid one-wave kernels
+                             // (nz = 1024) and the group has no row blocks here
   int64_t cell0;             // first cell of the group's first case
   double dx;
   FftPlanArgs fp;

@@ -360,6 +360,9 @@ static hipError_t ragged_groups(const RigidConfig* cfgs, int nbatch, const Rigid
       G.bs *= 2;
     }
     G.rb0 = row;
+    // nz = 1024: the 512-point register FFT, one wavefront per row, run by
+    // the one-grid kernels over the group's cases
+    G.wave = nz == 1024 ? 1 : 0;
     G.K = pl->tri_K;
     G.P = pl->tri_P;
     G.tchunks = (int)ceil_div(nz, G.K);
@@ -386,7 +389,7 @@ static hipError_t ragged_groups(const RigidConfig* cfgs, int nbatch, const Rigid
     G.r = b.r + r_off;  // the group's first case's tables
     G.kill_wz = b.kill_wz + 2 * w_off;
     G.kill_wr = b.kill_wr + w_off;
-    row += nc * nr / G.rpb;
+    if (!G.wave) row += nc * nr / G.rpb;
     tri += nc * G.tchunks;
     vt += nc * G.vtiles;
     ft += nc * G.ftiles;
@@ -493,9 +496,10 @@ hipError_t RaggedStepper<T>::init_ragged(const RigidConfig* cfgs, int nbatch,
   if (groups.empty()) return hipErrorInvalidValue;
   rows = vt_tiles = ft_tiles = maxM = 0;
   for (const RaggedGroup& G : groups) {
-    rows += G.ncase * G.nr / G.rpb;  // row-pass workgroups
     vt_tiles += G.ncase * G.vtiles;
     ft_tiles += G.ncase * G.ftiles;
+    if (G.wave) continue;  // row passes of their own
+    rows += G.ncase * G.nr / G.rpb;  // row-pass workgroups
     // LDS complex entries per workgroup: rpb rows and the twiddle table
     maxM = std::max(maxM, (G.rpb + 1) * (G.nz / 2));
   }
@@ -596,12 +600,28 @@ hipError_t RaggedStepper<T>::enqueue_stage(int stage, hipStream_t s) {
       fin.sc = sc;
       fin.drag = buf.drag;
       fin.cfgs = dcfg;
-      return launch_taper_dct2_ragged<T>(cur, sp, dgroups, ng, batch, rows, w, maxM, &fin, s);
+      hipError_t e = launch_taper_dct2_ragged<T>(cur, sp, dgroups, ng, batch, rows, w, maxM, &fin,
+                                                 s);
+      for (int g = 0; g < ng && e == hipSuccess; ++g) {
+        const RaggedGroup& G = groups[g];
+        if (G.wave)
+          e = launch_taper_dct2_rows<T>(*plans[g], cur + G.cell0, nullptr, sp + G.cell0,
+                                        G.kill_wz, G.kill_wr, w, s, G.ncase, nullptr);
+      }
+      return e;
     }
     case 1:
       return launch_tridiag_ragged<T>(sp, dgroups, ng, tri_blocks, tri_shm, tri_L, s);
-    case 2:
-      return launch_dct3_ragged<T>(sp, psi, dgroups, ng, rows, maxM, s);
+    case 2: {
+      hipError_t e = rows > 0 ? launch_dct3_ragged<T>(sp, psi, dgroups, ng, rows, maxM, s)
+                              : hipSuccess;
+      for (int g = 0; g < ng && e == hipSuccess; ++g) {
+        const RaggedGroup& G = groups[g];
+        if (G.wave)
+          e = launch_dct3_rows<T>(*plans[g], sp + G.cell0, psi + G.cell0, G.nr, s, G.ncase);
+      }
+      return e;
+    }
     case 3:
       return launch_velocity_max_ragged<T>(psi, sc, slots, dcfg, dgroups, ng, vt_tiles, vmax_tr,
                                            s);

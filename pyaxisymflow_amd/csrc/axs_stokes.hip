@@ -1137,6 +1137,7 @@ hipError_t launch_taper_dct2_ragged(const T* src, T* dst, const RaggedGroup* grp
                                     const StepFinalize* fin, hipStream_t s) {
   const StepFinalize f = fin ? *fin : StepFinalize{};
   const int blocks = row_blocks + (f.partials ? ncases : 0);
+  if (blocks == 0) return hipSuccess;  // every group runs its rows elsewhere
   taper_dct2_ragged_kernel<T><<<blocks, 256, sizeof(Cplx<T>) * lds_cplx, s>>>(src, dst, grp, ngrp,
                                                                               ncases, w, f);
   return hipGetLastError();

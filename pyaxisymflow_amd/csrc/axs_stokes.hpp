@@ -78,7 +78,7 @@ hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch =
 
 // Uneven batches (axs_ragged.hpp): one launch per pass over every group.
 // Row passes run 256-thread workgroups of rpb rows with bs threads each
-// (nz <= 1024), lds_cplx complex LDS entries per workgroup (the largest
+// (nz <= 1024; groups marked wave have no row blocks here), lds_cplx complex LDS entries per workgroup (the largest
 // (rpb + 1) M: the rows, then the group's twiddle table); the forward pass puts one completion workgroup per case in front
 // of the rows when fin is given (force partials of case q at
 // part0 + (q - case0) * ftiles of its group).

@@ -21,7 +21,8 @@ from oracle.drivers import RigidBodyCase
 
 pytestmark = pytest.mark.gpu
 
-GRIDS = [(128, 64), (192, 96), (96, 48)]  # (nz, nr), z in [0, 1], r in [0, 0.5]
+# (nz, nr), z in [0, 1], r in [0, 0.5]; nz = 1024 runs the one-wave row passes
+GRIDS = [(128, 64), (192, 96), (96, 48), (1024, 512)]
 
 
 def rel_l2(a, b):
@@ -51,8 +52,8 @@ def test_uneven_equals_one_grid_batches(monkeypatch):
     from pyaxisymflow_amd.stepper import RigidBatchStepper, UnevenBatchStepper
 
     # interleaved on purpose: the stepper groups the cases by grid
-    cases = uneven_cases([[0, 3], [1, 4, 6], [2]])
-    order = [0, 2, 1, 5, 3, 4]
+    cases = uneven_cases([[0, 3], [1, 4, 6], [2], [5, 7]])
+    order = [0, 2, 6, 1, 5, 3, 7, 4]
     mixed = [cases[q] for q in order]
     monkeypatch.setenv("AXS_TRIDIAG_LK", "8,1")  # segments of 8 rows (K is per group)
     uneven = UnevenBatchStepper(mixed)
