@@ -26,7 +26,8 @@ constexpr int kYStride = 68;                          // Y row (k1) stride, comp
 constexpr int kEStrideM = 20, kEStrideK = 80;         // E[k2a][m][k1] strides
 constexpr int kF1024Lds = 16 * kEStrideK;             // complex entries (>= M, >= 16*68)
 static_assert(kF1024Lds >= kF1024M && kF1024Lds >= 16 * kYStride, "LDS buffer");
-// M = 512 = 8 x 8 x 8 (fft512_wave): Y[k1][n2] rows of 64 + 4, E[k2a][m][k1]
+// M = R1 x 64 = R1 x 8 x 8, R1 = 4, 6, 8 (fft_r64_wave): Y[k1][n2] rows of
+// 64 + 4, E[k2a][m][k1]
 constexpr int kF512M = 512;
 constexpr int kY512 = 68, kE5M = 8, kE5K = 72;
 constexpr int kF512Lds = 576;
@@ -132,6 +133,35 @@ __device__ __forceinline__ void dft8(Cplx<T> (&x)[8]) {
   x[7] = csub(e3, o3);
 }
 
+// DFT-6 in place, natural order: the DFT-3s of the even and odd samples,
+// joined by W_6^k.
+template <typename T, int SGN>
+__device__ __forceinline__ void dft6(Cplx<T> (&x)[6]) {
+  Cplx<T> e[3] = {x[0], x[2], x[4]}, o[3] = {x[1], x[3], x[5]};
+  dft_small<T, 3>(e, SGN);
+  dft_small<T, 3>(o, SGN);
+  constexpr double h = 0.86602540378443864676;  // sin(pi / 3)
+  const Cplx<T> o1 = cmul(o[1], Cplx<T>{T(0.5), T(SGN * h)});
+  const Cplx<T> o2 = cmul(o[2], Cplx<T>{T(-0.5), T(SGN * h)});
+  x[0] = cadd(e[0], o[0]);
+  x[3] = csub(e[0], o[0]);
+  x[1] = cadd(e[1], o1);
+  x[4] = csub(e[1], o1);
+  x[2] = cadd(e[2], o2);
+  x[5] = csub(e[2], o2);
+}
+
+// DFT-R in place, natural order (R = 4, 6, 8).
+template <typename T, int SGN, int R>
+__device__ __forceinline__ void dft_r(Cplx<T> (&x)[R]) {
+  if constexpr (R == 8)
+    dft8<T, SGN>(x);
+  else if constexpr (R == 6)
+    dft6<T, SGN>(x);
+  else
+    dft4n<T, SGN>(x[0], x[1], x[2], x[3]);
+}
+
 template <int SGN>
 __device__ __forceinline__ Cplx<double> twid(const Cplx<double>* __restrict__ tw, int t) {
   const Cplx<double> w = tw[t];
@@ -197,26 +227,30 @@ __device__ __forceinline__ void fft1024_wave(Cplx<T>* buf, const Cplx<double>* _
   __syncthreads();
 }
 
-// FFT of the M = 512 complex values in buf, one wavefront, three radix-8
-// passes (n = n2 + 64 n1, k = k1 + 8 k', n2 = m + 8 c, k' = k2a + 8 k2b):
-//   pass 1  lane n2: DFT-8 over n1, times W_512^(n2 k1)       -> Y[k1][n2]
-//   pass 2  lane (k1, m): DFT-8 over c, times W_64^(m k2a)     -> E[k2a][m][k1]
-//   pass 3  lane (k2a, k1): DFT-8 over m                       -> X[k1 + 8 k2a + 64 k2b]
-// tw[t] = e^{-2 pi i t / 512}.  Natural order in and out of buf.
-template <typename T, int SGN>
-__device__ __forceinline__ void fft512_wave(Cplx<T>* buf, const Cplx<double>* __restrict__ tw) {
+// FFT of the M = R1 x 64 complex values in buf (R1 = 4, 6, 8), one
+// wavefront, three passes (n = n2 + 64 n1, k = k1 + R1 k', n2 = m + 8 c,
+// k' = k2a + 8 k2b):
+//   pass 1  lane n2: DFT-R1 over n1, times W_M^(n2 k1)          -> Y[k1][n2]
+//   pass 2  lane (k1, m), k1 < R1: DFT-8 over c, times W_64^(m k2a)
+//                                                               -> E[k2a][m][k1]
+//   pass 3  lane (k2a, k1), k1 < R1: DFT-8 over m     -> X[k1 + R1 (k2a + 8 k2b)]
+// (lanes with k1 >= R1 idle in passes 2 and 3).  tw[t] = e^{-2 pi i t / M}.
+// Natural order in and out of buf.
+template <typename T, int SGN, int R1>
+__device__ __forceinline__ void fft_r64_wave(Cplx<T>* buf, const Cplx<double>* __restrict__ tw) {
+  static_assert(R1 == 4 || R1 == 6 || R1 == 8, "first radix");
   const int l = threadIdx.x & 63;
-  Cplx<T> v[8];
   {
-    Cplx<double> t1[7];
+    Cplx<T> v[R1];
+    Cplx<double> t1[R1 - 1];
 #pragma unroll
-    for (int k1 = 1; k1 < 8; ++k1) t1[k1 - 1] = twid<SGN>(tw, l * k1);
+    for (int k1 = 1; k1 < R1; ++k1) t1[k1 - 1] = twid<SGN>(tw, l * k1);
 #pragma unroll
-    for (int n1 = 0; n1 < 8; ++n1) v[n1] = buf[64 * n1 + l];
-    dft8<T, SGN>(v);
+    for (int n1 = 0; n1 < R1; ++n1) v[n1] = buf[64 * n1 + l];
+    dft_r<T, SGN, R1>(v);
     __syncthreads();
 #pragma unroll
-    for (int k1 = 0; k1 < 8; ++k1) {
+    for (int k1 = 0; k1 < R1; ++k1) {
       Cplx<T> b = v[k1];
       if (k1 > 0) b = cmul(b, Cplx<T>{T(t1[k1 - 1].x), T(t1[k1 - 1].y)});
       buf[k1 * kY512 + l] = b;
@@ -225,40 +259,49 @@ __device__ __forceinline__ void fft512_wave(Cplx<T>* buf, const Cplx<double>* __
   __syncthreads();
   {
     const int k1 = l >> 3, m = l & 7;
-    Cplx<double> t2[7];
+    Cplx<T> v[8];
+    if (k1 < R1) {
+      Cplx<double> t2[7];
 #pragma unroll
-    for (int k2a = 1; k2a < 8; ++k2a) t2[k2a - 1] = twid<SGN>(tw, 8 * m * k2a);
+      for (int k2a = 1; k2a < 8; ++k2a) t2[k2a - 1] = twid<SGN>(tw, R1 * m * k2a);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) v[c] = buf[k1 * kY512 + m + 8 * c];
-    dft8<T, SGN>(v);
+      for (int c = 0; c < 8; ++c) v[c] = buf[k1 * kY512 + m + 8 * c];
+      dft8<T, SGN>(v);
+#pragma unroll
+      for (int k2a = 1; k2a < 8; ++k2a)
+        v[k2a] = cmul(v[k2a], Cplx<T>{T(t2[k2a - 1].x), T(t2[k2a - 1].y)});
+    }
     __syncthreads();
+    if (k1 < R1) {
 #pragma unroll
-    for (int k2a = 0; k2a < 8; ++k2a) {
-      Cplx<T> e = v[k2a];
-      if (k2a > 0) e = cmul(e, Cplx<T>{T(t2[k2a - 1].x), T(t2[k2a - 1].y)});
-      buf[k2a * kE5K + m * kE5M + k1] = e;
+      for (int k2a = 0; k2a < 8; ++k2a) buf[k2a * kE5K + m * kE5M + k1] = v[k2a];
     }
   }
   __syncthreads();
   {
     const int k2a = l >> 3, k1 = l & 7;
+    Cplx<T> v[8];
+    if (k1 < R1) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = buf[k2a * kE5K + m * kE5M + k1];
-    dft8<T, SGN>(v);
+      for (int m = 0; m < 8; ++m) v[m] = buf[k2a * kE5K + m * kE5M + k1];
+      dft8<T, SGN>(v);
+    }
     __syncthreads();
+    if (k1 < R1) {
 #pragma unroll
-    for (int k2b = 0; k2b < 8; ++k2b) buf[k1 + 8 * k2a + 64 * k2b] = v[k2b];
+      for (int k2b = 0; k2b < 8; ++k2b) buf[k1 + R1 * (k2a + 8 * k2b)] = v[k2b];
+    }
   }
   __syncthreads();
 }
 
-// The one-wave transform of M = 512 or 1024 points.
+// The one-wave transform of M = 256, 384, 512 or 1024 points.
 template <typename T, int SGN, int M>
 __device__ __forceinline__ void fft_wave(Cplx<T>* buf, const Cplx<double>* __restrict__ tw) {
   if constexpr (M == kF1024M)
     fft1024_wave<T, SGN>(buf, tw);
   else
-    fft512_wave<T, SGN>(buf, tw);
+    fft_r64_wave<T, SGN, M / 64>(buf, tw);
 }
 
 }  // namespace axs
@@ -288,7 +331,7 @@ __device__ __forceinline__ void dct2_post_pairs(const Cplx<T>* Z, const Cplx<dou
 #pragma unroll
   for (int q = 0; q < KP; ++q) {
     const int k = l + 64 * q, kp = M - k;
-    const Cplx<T> zk = Z[k], zp = Z[kp & (M - 1)];
+    const Cplx<T> zk = Z[k], zp = Z[kp == M ? 0 : kp];  // Z is periodic in M
     const Cplx<T> sk = cadd(zk, cconj(zp)), dk = csub(zk, cconj(zp));
     const Cplx<T> wdk = cmul(Cplx<T>{T(a1[q].x), T(a1[q].y)}, dk);
     const Cplx<T> Vk = {T(0.5) * (sk.x + wdk.y), T(0.5) * (sk.y - wdk.x)};

@@ -26,7 +26,7 @@ struct RaggedGroup {
   int ft0, fz, ftiles;       // step_fused: first tile, tiles along z, tiles per case
   int part0;                 // first force partial of the group
   int wave;                  // 1: the row passes run the one-grid one-wave kernels
-                             // (nz = 1024) and the group has no row blocks here
+                             // (nz = 512, 768, 1024) and the group has no row blocks here
   int64_t cell0;             // first cell of the group's first case
   double dx;
   FftPlanArgs fp;

@@ -360,9 +360,9 @@ static hipError_t ragged_groups(const RigidConfig* cfgs, int nbatch, const Rigid
       G.bs *= 2;
     }
     G.rb0 = row;
-    // nz = 1024: the 512-point register FFT, one wavefront per row, run by
-    // the one-grid kernels over the group's cases
-    G.wave = nz == 1024 ? 1 : 0;
+    // half-lengths with a one-wave register FFT: the one-grid kernels run
+    // the group's row passes over its cases
+    G.wave = wave_rows(nz / 2) ? 1 : 0;
     G.K = pl->tri_K;
     G.P = pl->tri_P;
     G.tchunks = (int)ceil_div(nz, G.K);

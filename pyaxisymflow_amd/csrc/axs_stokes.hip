@@ -981,6 +981,9 @@ __global__ void __launch_bounds__(1024) tridiag_ragged_kernel(T* __restrict__ d,
 
 // ------------------------------------------------------------------ launchers
 
+// Half-lengths with a one-wave register transform (axs_fft1024.hpp).
+bool wave_rows(int M) { return M == 256 || M == 384 || M == 512 || M == 1024; }
+
 static int fft_block(int M) {
   // threads so that every stage needs at most PER butterflies per thread
   return std::min(256, std::max(64, M / 2));
@@ -994,13 +997,15 @@ hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const do
   // worst stage is radix 2 (M/2 butterflies); 2*per*bs >= M covers the row loads
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
-  if (M == kF1024M || M == kF512M) {
-    if (M == kF1024M)
-      dct2_rows_wave_kernel<T, false, kF1024M><<<rows, 64, 0, s>>>(
-          src, nullptr, dst, rscale, nullptr, nullptr, rows, 0, pl.tw, pl.e1, pl.e2, StepFinalize{});
-    else
-      dct2_rows_wave_kernel<T, false, kF512M><<<rows, 64, 0, s>>>(
-          src, nullptr, dst, rscale, nullptr, nullptr, rows, 0, pl.tw, pl.e1, pl.e2, StepFinalize{});
+  if (wave_rows(M)) {
+#define AXS_DCT2_WAVE(MM)                                                                     \
+  dct2_rows_wave_kernel<T, false, MM><<<rows, 64, 0, s>>>(                                     \
+      src, nullptr, dst, rscale, nullptr, nullptr, rows, 0, pl.tw, pl.e1, pl.e2, StepFinalize{})
+    if (M == 1024) AXS_DCT2_WAVE(1024);
+    else if (M == 512) AXS_DCT2_WAVE(512);
+    else if (M == 384) AXS_DCT2_WAVE(384);
+    else AXS_DCT2_WAVE(256);
+#undef AXS_DCT2_WAVE
     return hipGetLastError();
   }
   if (M == kF2048M) {
@@ -1030,13 +1035,15 @@ hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered
   const int bs = fft_block(M);
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
-  if (M == kF1024M || M == kF512M) {
-    if (M == kF1024M)
-      dct2_rows_wave_kernel<T, true, kF1024M><<<dim3(gx, batch), 64, 0, s>>>(
-          src, tapered, dst, pl.rvec, wz, wr, pl.nr, w, pl.tw, pl.e1, pl.e2, f);
-    else
-      dct2_rows_wave_kernel<T, true, kF512M><<<dim3(gx, batch), 64, 0, s>>>(
-          src, tapered, dst, pl.rvec, wz, wr, pl.nr, w, pl.tw, pl.e1, pl.e2, f);
+  if (wave_rows(M)) {
+#define AXS_TAPER_WAVE(MM)                                                                    \
+  dct2_rows_wave_kernel<T, true, MM><<<dim3(gx, batch), 64, 0, s>>>(                           \
+      src, tapered, dst, pl.rvec, wz, wr, pl.nr, w, pl.tw, pl.e1, pl.e2, f)
+    if (M == 1024) AXS_TAPER_WAVE(1024);
+    else if (M == 512) AXS_TAPER_WAVE(512);
+    else if (M == 384) AXS_TAPER_WAVE(384);
+    else AXS_TAPER_WAVE(256);
+#undef AXS_TAPER_WAVE
     return hipGetLastError();
   }
   if (M == kF2048M) {
@@ -1067,11 +1074,14 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
   const size_t shm = sizeof(Cplx<T>) * M;
   const dim3 grid(rows, batch);
-  if (M == kF1024M || M == kF512M) {
-    if (M == kF1024M)
-      dct3_rows_wave_kernel<T, kF1024M><<<grid, 64, 0, s>>>(src, dst, pl.tw, pl.e1, pl.e2);
-    else
-      dct3_rows_wave_kernel<T, kF512M><<<grid, 64, 0, s>>>(src, dst, pl.tw, pl.e1, pl.e2);
+  if (wave_rows(M)) {
+#define AXS_DCT3_WAVE(MM) \
+  dct3_rows_wave_kernel<T, MM><<<grid, 64, 0, s>>>(src, dst, pl.tw, pl.e1, pl.e2)
+    if (M == 1024) AXS_DCT3_WAVE(1024);
+    else if (M == 512) AXS_DCT3_WAVE(512);
+    else if (M == 384) AXS_DCT3_WAVE(384);
+    else AXS_DCT3_WAVE(256);
+#undef AXS_DCT3_WAVE
     return hipGetLastError();
   }
   if (M == kF2048M) {

@@ -75,6 +75,9 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
                             int batch = 1);
 template <typename T>
 hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch = 1);
+// True when rows of half-length M run a one-wave register transform
+// (M = 256, 384, 512, 1024; M = 2048 runs two waves per row).
+bool wave_rows(int M);
 
 // Uneven batches (axs_ragged.hpp): one launch per pass over every group.
 // Row passes run 256-thread workgroups of rpb rows with bs threads each
