@@ -627,13 +627,14 @@ __device__ __forceinline__ void step_fused_tile(
 // VGPRs and under 40 KiB of LDS, four blocks per CU.  FP32 one-grid: the
 // same bound (82 VGPRs, five blocks per CU; a one-grid launch of the
 // precision form has two or three blocks per CU, so more occupancy buys
-// nothing and spills cost 4%).  FP32 uneven batches: at most 64 VGPRs and
-// 20 KiB, eight blocks per CU (the spills this costs are cheaper than the
-// lost occupancy: 367 -> 362 us per batched step).
+// nothing and spills cost 4%).  FP32 uneven batches: at most 72 VGPRs,
+// seven blocks per CU (the spills this costs are cheaper than the lost
+// occupancy; per batched step 367 us at five blocks, 371 at six, 335 at
+// eight, 327 at seven).
 template <typename T>
 constexpr int kFusedWaves = 4;
 template <typename T>
-constexpr int kRaggedFusedWaves = sizeof(T) == 8 ? 4 : 8;
+constexpr int kRaggedFusedWaves = sizeof(T) == 8 ? 4 : 7;
 
 // 256 threads, waves and LDS as kFusedWaves says.
 template <typename T>
