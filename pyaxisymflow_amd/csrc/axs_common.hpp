@@ -65,19 +65,25 @@ struct RRow {
   __device__ __forceinline__ T at(int i, int) const { return static_cast<T>(r[i]); }
 };
 
-// x / d rounded to nearest, given y = RN(1 / d): q = RN(x y) is within one
-// ulp of x / d, the residual x - q d is exact under fma, and q + RN(r y) is
-// then the correctly rounded quotient (Markstein's correction step,
-// P. Markstein, "Computation of elementary functions on the IBM RISC
-// System/6000 processor", IBM J. Res. Dev. 34(1), 1990).  Holds while
-// neither the quotient nor the residual overflows or underflows; a zero
-// quotient comes back as +0.  Three FP ops instead of the IEEE division
-// sequence when the divisor is constant per launch or per row.
+// x / d for a divisor constant per launch or per row, given y = RN(1 / d).
+// FP64 (the reference's arithmetic, matched bit for bit): rounded to
+// nearest.  q = RN(x y) is within one ulp of x / d, the residual x - q d is
+// exact under fma, and q + RN(r y) is then the correctly rounded quotient
+// (Markstein's correction step, P. Markstein, "Computation of elementary
+// functions on the IBM RISC System/6000 processor", IBM J. Res. Dev.
+// 34(1), 1990).  Holds while neither the quotient nor the residual
+// overflows or underflows; a zero quotient comes back as +0.  Three FP ops
+// instead of the IEEE division sequence.  FP32 (the precision and batched
+// forms, held to 1e-4 of FP64, not to bits): x y, within an ulp.
 template <typename T>
-__device__ __forceinline__ T div_rn(T x, T d, T y) {
-  const T q = x * y;
-  const T r = fma(-q, d, x);
-  return fma(r, y, q);
+__device__ __forceinline__ T div_by(T x, T d, T y) {
+  if constexpr (sizeof(T) == 4) {
+    return x * y;
+  } else {
+    const T q = x * y;
+    const T r = fma(-q, d, x);
+    return fma(r, y, q);
+  }
 }
 
 }  // namespace axs

@@ -27,8 +27,8 @@
 // such a fence costs an L2 write-back; kernel boundaries order the passes.
 //
 // Per-cell arithmetic and its order are those of the single-purpose kernels.
-// Divisions by dx^2, 2 dx and r_i use div_rn (correctly rounded, so the
-// results are unchanged); ENO3 faces shared by two cells are evaluated once
+// Divisions by dx^2, 2 dx and r_i use div_by (correctly rounded in FP64, so
+// the results are unchanged; the reciprocal product in FP32); ENO3 faces shared by two cells are evaluated once
 // (the backward face of a cell is the forward face of its lower neighbour,
 // bit for bit: same comparison, same products, same sums).
 #include "axs_fused.hpp"
@@ -69,8 +69,8 @@ __device__ __forceinline__ void velocity_rn(const T* p, int ld, int i, int j, in
     nrv = -(p[-2] - T(4) * p[-1] + T(3) * p[0]);
   else
     nrv = -(p[1] - p[-1]);
-  uz = div_rn(div_rn(nzv, two_dx, inv2dx), ri, inv_ri) + ufree;
-  ur = div_rn(div_rn(nrv, two_dx, inv2dx), ri, inv_ri);
+  uz = div_by(div_by(nzv, two_dx, inv2dx), ri, inv_ri) + ufree;
+  ur = div_by(div_by(nrv, two_dx, inv2dx), ri, inv_ri);
 }
 
 // (u + ldc U) / (1 + ldc); outside the body (ldc == 0) the quotient is the
@@ -178,8 +178,8 @@ __device__ __forceinline__ void velocity_max_tile(const T* __restrict__ psi,
       const int e = tid + q * 256;
       const int a = e / kVTZ, b = e % kVTZ;
       const T* p = &sp[a + 2][b + 2];
-      const T uz = div_rn(div_rn(p[kVPZ] - p[-kVPZ], two_dx, inv2dx), sri[a], sinv[a]) + ufree;
-      const T ur = div_rn(div_rn(-(p[1] - p[-1]), two_dx, inv2dx), sri[a], sinv[a]);
+      const T uz = div_by(div_by(p[kVPZ] - p[-kVPZ], two_dx, inv2dx), sri[a], sinv[a]) + ufree;
+      const T ur = div_by(div_by(-(p[1] - p[-1]), two_dx, inv2dx), sri[a], sinv[a]);
       m = fmax(m, double(fabs(uz) + fabs(ur)));
     }
   } else {
@@ -386,8 +386,8 @@ __device__ __forceinline__ void step_fused_tile(
         const int q5 = kVS * ps + k, a4 = q5 - 1;
         const T* p = &sp[q5 + 1][pb + 1];
         const T uz =
-            div_rn(div_rn(p[kR6Z] - p[-kR6Z], two_dx, inv2dx), sri[q5 + 1], sinv[q5 + 1]) + ufree;
-        const T ur = div_rn(div_rn(-(p[1] - p[-1]), two_dx, inv2dx), sri[q5 + 1], sinv[q5 + 1]);
+            div_by(div_by(p[kR6Z] - p[-kR6Z], two_dx, inv2dx), sri[q5 + 1], sinv[q5 + 1]) + ufree;
+        const T ur = div_by(div_by(-(p[1] - p[-1]), two_dx, inv2dx), sri[q5 + 1], sinv[q5 + 1]);
         const T ldc = lam_dt * vc[k];
         const T pzv = brinkman(uz, ldc, U);
         const T prv = brinkman(ur, ldc, T(0));
@@ -420,8 +420,8 @@ __device__ __forceinline__ void step_fused_tile(
       T uz, ur;
       if constexpr (IN) {
         // compute_velocity_from_psi.py:10-17, central differences
-        uz = div_rn(div_rn(p[kR6Z] - p[-kR6Z], two_dx, inv2dx), sri[q5 + 1], sinv[q5 + 1]) + ufree;
-        ur = div_rn(div_rn(-(p[1] - p[-1]), two_dx, inv2dx), sri[q5 + 1], sinv[q5 + 1]);
+        uz = div_by(div_by(p[kR6Z] - p[-kR6Z], two_dx, inv2dx), sri[q5 + 1], sinv[q5 + 1]) + ufree;
+        ur = div_by(div_by(-(p[1] - p[-1]), two_dx, inv2dx), sri[q5 + 1], sinv[q5 + 1]);
       } else {
         velocity_rn<T>(p, kR6Z, i, pj, nr, nz, two_dx, inv2dx, sri[q5 + 1], sinv[q5 + 1], ufree,
                        uz, ur);
@@ -469,15 +469,15 @@ __device__ __forceinline__ void step_fused_tile(
       const int i = i0 - 4 + a4;
       T fv = T(0);
       if constexpr (IN) {
-        const T pv = div_rn(drp[k - 1] - drm[k - 1], two_dx, inv2dx) -
-                     div_rn(dz[k + 1] - dz[k - 1], two_dx, inv2dx);
+        const T pv = div_by(drp[k - 1] - drm[k - 1], two_dx, inv2dx) -
+                     div_by(dz[k + 1] - dz[k - 1], two_dx, inv2dx);
         fv = vw[k - 1] + pv;
       } else {
         if (pjin && i >= 0 && i < nr) {
           fv = taper_apply(vw[k - 1], i, pj, nr, nz, kw, wz, wr);
           if (i >= 1 && i <= nr - 2 && pj >= 1 && pj <= nz - 2) {
-            const T pv = div_rn(drp[k - 1] - drm[k - 1], two_dx, inv2dx) -
-                         div_rn(dz[k + 1] - dz[k - 1], two_dx, inv2dx);
+            const T pv = div_by(drp[k - 1] - drm[k - 1], two_dx, inv2dx) -
+                         div_by(dz[k + 1] - dz[k - 1], two_dx, inv2dx);
             fv = fv + pv;
           }
         }
@@ -576,8 +576,8 @@ __device__ __forceinline__ void step_fused_tile(
       T v = ws[k + 1];
       if (IN || (jok && i >= 1 && i <= nr - 2)) {
         const T up = ws[k + 2], dn = ws[k];
-        const T Lp = div_rn(up + dn + sa[ea][eb + 1] + sa[ea][eb - 1] - T(4) * v, dx2, inv_dx2) +
-                     div_rn(div_rn(up - dn, two_dx, inv2dx), sri[a6], sinv[a6]) - v * sinv2[a6];
+        const T Lp = div_by(up + dn + sa[ea][eb + 1] + sa[ea][eb - 1] - T(4) * v, dx2, inv_dx2) +
+                     div_by(div_by(up - dn, two_dx, inv2dx), sri[a6], sinv[a6]) - v * sinv2[a6];
         v = v + half_nu_dt * Lp;
       }
       st[a][b] = v;  // bufC (u_pen_z) was last read before the previous barrier
@@ -602,8 +602,8 @@ __device__ __forceinline__ void step_fused_tile(
       T v = sa[a + 2][b + 2];
       if (IN || (jok && i >= 1 && i <= nr - 2)) {
         const T tc = wt[k + 1], up = wt[k + 2], dn = wt[k];
-        const T Lp = div_rn(up + dn + st[ta][tb + 1] + st[ta][tb - 1] - T(4) * tc, dx2, inv_dx2) +
-                     div_rn(div_rn(up - dn, two_dx, inv2dx), sri[a6], sinv[a6]) - tc * sinv2[a6];
+        const T Lp = div_by(up + dn + st[ta][tb + 1] + st[ta][tb - 1] - T(4) * tc, dx2, inv_dx2) +
+                     div_by(div_by(up - dn, two_dx, inv2dx), sri[a6], sinv[a6]) - tc * sinv2[a6];
         v = v + nu_dt * Lp;
       }
       if constexpr (IN)
