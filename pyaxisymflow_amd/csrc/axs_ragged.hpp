@@ -25,8 +25,9 @@ struct RaggedGroup {
   int vt0, vz, vtiles;       // velocity_max: first tile, tiles along z, tiles per case
   int ft0, fz, ftiles;       // step_fused: first tile, tiles along z, tiles per case
   int part0;                 // first force partial of the group
-  int wave;                  // 1: the row passes run the one-grid one-wave kernels
-                             // (nz = 512, 768, 1024) and the group has no row blocks here
+  int wave;                  // 1: the row passes run one-wave register transforms
+                             // (nz = 512, 768, 1024): no Stockham row blocks here
+  int wr0;                   // first one-wave row block of the group (wave groups)
   int64_t cell0;             // first cell of the group's first case
   double dx;
   FftPlanArgs fp;

@@ -330,7 +330,7 @@ static hipError_t ragged_groups(const RigidConfig* cfgs, int nbatch, const Rigid
                                 int& tri_blocks, size_t& tri_shm) {
   const int kw = cfgs[0].kill_width;
   int64_t cell = 0, r_off = 0, w_off = 0;
-  int row = 0, tri = 0, vt = 0, ft = 0;
+  int row = 0, wrow = 0, tri = 0, vt = 0, ft = 0;
   tri_shm = 0;
   for (int q = 0; q < nbatch;) {
     int e = q;
@@ -363,6 +363,7 @@ static hipError_t ragged_groups(const RigidConfig* cfgs, int nbatch, const Rigid
     // half-lengths with a one-wave register FFT: the one-grid kernels run
     // the group's row passes over its cases
     G.wave = wave_rows(nz / 2) ? 1 : 0;
+    G.wr0 = wrow;
     G.K = pl->tri_K;
     G.P = pl->tri_P;
     G.tchunks = (int)ceil_div(nz, G.K);
@@ -389,7 +390,10 @@ static hipError_t ragged_groups(const RigidConfig* cfgs, int nbatch, const Rigid
     G.r = b.r + r_off;  // the group's first case's tables
     G.kill_wz = b.kill_wz + 2 * w_off;
     G.kill_wr = b.kill_wr + w_off;
-    if (!G.wave) row += nc * nr / G.rpb;
+    if (G.wave)
+      wrow += nc * nr;
+    else
+      row += nc * nr / G.rpb;
     tri += nc * G.tchunks;
     vt += nc * G.vtiles;
     ft += nc * G.ftiles;
@@ -494,11 +498,14 @@ hipError_t RaggedStepper<T>::init_ragged(const RigidConfig* cfgs, int nbatch,
   if (t1) (void)hipEventDestroy(t1);
   if (e != hipSuccess && groups.empty()) return e;
   if (groups.empty()) return hipErrorInvalidValue;
-  rows = vt_tiles = ft_tiles = maxM = 0;
+  rows = wave_blocks = vt_tiles = ft_tiles = maxM = 0;
   for (const RaggedGroup& G : groups) {
     vt_tiles += G.ncase * G.vtiles;
     ft_tiles += G.ncase * G.ftiles;
-    if (G.wave) continue;  // row passes of their own
+    if (G.wave) {  // one-wave register rows, one launch over all such groups
+      wave_blocks += G.ncase * G.nr;
+      continue;
+    }
     rows += G.ncase * G.nr / G.rpb;  // row-pass workgroups
     // LDS complex entries per workgroup: rpb rows and the twiddle table
     maxM = std::max(maxM, (G.rpb + 1) * (G.nz / 2));
@@ -602,25 +609,16 @@ hipError_t RaggedStepper<T>::enqueue_stage(int stage, hipStream_t s) {
       fin.cfgs = dcfg;
       hipError_t e = launch_taper_dct2_ragged<T>(cur, sp, dgroups, ng, batch, rows, w, maxM, &fin,
                                                  s);
-      for (int g = 0; g < ng && e == hipSuccess; ++g) {
-        const RaggedGroup& G = groups[g];
-        if (G.wave)
-          e = launch_taper_dct2_rows<T>(*plans[g], cur + G.cell0, nullptr, sp + G.cell0,
-                                        G.kill_wz, G.kill_wr, w, s, G.ncase, nullptr);
-      }
-      return e;
+      if (e != hipSuccess) return e;
+      return launch_wave_rows_ragged<T>(cur, sp, dgroups, ng, wave_blocks, w, true, s);
     }
     case 1:
       return launch_tridiag_ragged<T>(sp, dgroups, ng, tri_blocks, tri_shm, tri_L, s);
     case 2: {
       hipError_t e = rows > 0 ? launch_dct3_ragged<T>(sp, psi, dgroups, ng, rows, maxM, s)
                               : hipSuccess;
-      for (int g = 0; g < ng && e == hipSuccess; ++g) {
-        const RaggedGroup& G = groups[g];
-        if (G.wave)
-          e = launch_dct3_rows<T>(*plans[g], sp + G.cell0, psi + G.cell0, G.nr, s, G.ncase);
-      }
-      return e;
+      if (e != hipSuccess) return e;
+      return launch_wave_rows_ragged<T>(sp, psi, dgroups, ng, wave_blocks, w, false, s);
     }
     case 3:
       return launch_velocity_max_ragged<T>(psi, sc, slots, dcfg, dgroups, ng, vt_tiles, vmax_tr,

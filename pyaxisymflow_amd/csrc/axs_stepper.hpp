@@ -108,7 +108,7 @@ struct RaggedStepper : RigidStepperBase {
   std::vector<StokesPlan*> plans;
   std::vector<RaggedGroup> groups;
   RaggedGroup* dgroups = nullptr;
-  int rows = 0, tri_blocks = 0, vt_tiles = 0, ft_tiles = 0, maxM = 0;
+  int rows = 0, wave_blocks = 0, tri_blocks = 0, vt_tiles = 0, ft_tiles = 0, maxM = 0;
   int tri_L = 8;    // radial segment length (timed search over 4, 8, 16 at set-up)
   int vmax_tr = 16;  // velocity_max tile rows (timed search over 16, 8, 32 at set-up)
   size_t tri_shm = 0;

@@ -433,28 +433,17 @@ __global__ void __launch_bounds__(256) dct3_rows_kernel(
 // nz = 2M = 1024 and 2048 row passes: one wavefront per row, FFT in
 // registers (axs_fft1024.hpp).  Same inputs, outputs and taper as the
 // kernels above.
+// Row i of one case (src, tapered, dst at the case), one wavefront, buf of
+// kWaveLds<M> entries.
 template <typename T, bool TAPER, int M>
-__global__ void __launch_bounds__(64) dct2_rows_wave_kernel(
+__device__ __forceinline__ void dct2_row_wave(
     const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
     const double* __restrict__ rscale, const double* __restrict__ wz,
     const double* __restrict__ wr, int nr, int w, const Cplx<double>* __restrict__ tw,
-    const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2, StepFinalize fin) {
+    const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2, int i,
+    Cplx<T>* buf) {
   constexpr int N = 2 * M;
-  if (fin.partials && blockIdx.x == nr) {  // completes the previous step
-    __shared__ double fin_red[256];
-    const int cb = blockIdx.y;
-    const RigidConfig& cfg = fin.cfgs[cb];
-    finalize_pending(fin.partials + (int64_t)cb * fin.count, fin.count,
-                     fin.slots + (int64_t)cb * kSlotsPerCase, fin.sc + (int64_t)cb * AXS_RIGID_SCALARS,
-                     fin.drag ? fin.drag + (int64_t)cb * cfg.history : nullptr, cfg, fin_red);
-    return;
-  }
-  __shared__ Cplx<T> buf[kWaveLds<M>];
   const int l = threadIdx.x;
-  const int i = blockIdx.x;
-  const int64_t cs = (int64_t)blockIdx.y * nr * N;  // case of a batch
-  src += cs;
-  dst += cs;
   const int srow = (TAPER && i >= nr - w) ? nr - w : i;
   const T* row = src + (int64_t)srow * N;
   T v[N / 64];
@@ -467,7 +456,7 @@ __global__ void __launch_bounds__(64) dct2_rows_wave_kernel(
     if (i >= nr - w) rowf = T(wr[i - (nr - w)]);
   }
   const T rs = rscale ? T(rscale[i]) : T(1);
-  T* out_t = (TAPER && tapered) ? tapered + cs + (int64_t)i * N : nullptr;
+  T* out_t = (TAPER && tapered) ? tapered + (int64_t)i * N : nullptr;
   // Makhoul's reorder: element j goes to real slot j / 2 (j even) or
   // N - 1 - (j - 1) / 2 (j odd) of the packed sequence; j = l + 64 q has the
   // parity of l, so the slot is a per-lane base plus or minus 32 q.
@@ -498,20 +487,42 @@ __global__ void __launch_bounds__(64) dct2_rows_wave_kernel(
   dct2_post_pairs<T, M>(buf, e1, e2, out);
 }
 
+template <typename T, bool TAPER, int M>
+__global__ void __launch_bounds__(64) dct2_rows_wave_kernel(
+    const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
+    const double* __restrict__ rscale, const double* __restrict__ wz,
+    const double* __restrict__ wr, int nr, int w, const Cplx<double>* __restrict__ tw,
+    const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2, StepFinalize fin) {
+  if (fin.partials && blockIdx.x == nr) {  // completes the previous step
+    __shared__ double fin_red[256];
+    const int cb = blockIdx.y;
+    const RigidConfig& cfg = fin.cfgs[cb];
+    finalize_pending(fin.partials + (int64_t)cb * fin.count, fin.count,
+                     fin.slots + (int64_t)cb * kSlotsPerCase, fin.sc + (int64_t)cb * AXS_RIGID_SCALARS,
+                     fin.drag ? fin.drag + (int64_t)cb * cfg.history : nullptr, cfg, fin_red);
+    return;
+  }
+  __shared__ Cplx<T> buf[kWaveLds<M>];
+  const int64_t cs = (int64_t)blockIdx.y * nr * (2 * M);  // case of a batch
+  dct2_row_wave<T, TAPER, M>(src + cs, tapered ? tapered + cs : nullptr, dst + cs, rscale, wz, wr,
+                             nr, w, tw, e1, e2, blockIdx.x, buf);
+}
+
+// Row i of one case (src, dst at the case), one wavefront, buf of
+// kWaveLds<M> entries.
 template <typename T, int M>
-__global__ void __launch_bounds__(64) dct3_rows_wave_kernel(
-    const T* __restrict__ src, T* __restrict__ dst, const Cplx<double>* __restrict__ tw,
-    const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2) {
+__device__ __forceinline__ void dct3_row_wave(const T* __restrict__ src, T* __restrict__ dst,
+                                              const Cplx<double>* __restrict__ tw,
+                                              const Cplx<double>* __restrict__ e1,
+                                              const Cplx<double>* __restrict__ e2, int i,
+                                              Cplx<T>* buf) {
   constexpr int N = 2 * M;
   constexpr int KP = M / 2 / 64;  // index pairs (k, M - k) per lane
   typedef T T2 __attribute__((ext_vector_type(2)));
-  __shared__ Cplx<T> buf[kWaveLds<M>];
   static_assert(2 * kWaveLds<M> >= N, "the staged row fits the buffer");
   T* raw = reinterpret_cast<T*>(buf);  // the input row, staged before the pre step
   const int l = threadIdx.x;
-  const int i = blockIdx.x;
-  const int64_t cs = (int64_t)blockIdx.y * gridDim.x * N;  // case of a batch
-  const T2* row = reinterpret_cast<const T2*>(src + cs + (int64_t)i * N);
+  const T2* row = reinterpret_cast<const T2*>(src + (int64_t)i * N);
   // Every global load of the pre step is issued before the first wait: the
   // row once (two values per lane and load) and the twiddles of the pairs.
   T2 v[N / 128];
@@ -569,7 +580,7 @@ __global__ void __launch_bounds__(64) dct3_rows_wave_kernel(
   __syncthreads();
   fft_wave<T, +1, M>(buf, tw);
   const T invN = T(1) / T(N);
-  T* out = dst + cs + (int64_t)i * N;
+  T* out = dst + (int64_t)i * N;
   // output j = l + 64 q reads packed slot j / 2 or N - 1 - (j - 1) / 2 (the
   // parity of j is that of l): a per-lane base plus or minus 32 q
   const T* zr = reinterpret_cast<const T*>(buf);
@@ -577,6 +588,55 @@ __global__ void __launch_bounds__(64) dct3_rows_wave_kernel(
   const int dq = (l & 1) ? -32 : 32;
 #pragma unroll
   for (int q = 0; q < N / 64; ++q) out[l + 64 * q] = zr[d0 + dq * q] * invN;
+}
+
+template <typename T, int M>
+__global__ void __launch_bounds__(64) dct3_rows_wave_kernel(
+    const T* __restrict__ src, T* __restrict__ dst, const Cplx<double>* __restrict__ tw,
+    const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2) {
+  __shared__ Cplx<T> buf[kWaveLds<M>];
+  const int64_t cs = (int64_t)blockIdx.y * gridDim.x * (2 * M);  // case of a batch
+  dct3_row_wave<T, M>(src + cs, dst + cs, tw, e1, e2, blockIdx.x, buf);
+}
+
+// Uneven batches: the rows of every group with a one-wave register transform
+// (G.wave: nz = 512, 768, 1024) in one launch, one wavefront per row; block
+// b is row b - G.wr0 of its group's cases.
+template <typename T>
+__global__ void __launch_bounds__(64) taper_dct2_wave_ragged_kernel(
+    const T* __restrict__ src, T* __restrict__ dst, const RaggedGroup* __restrict__ grp, int ngrp,
+    int w) {
+  __shared__ Cplx<T> buf[kF512Lds];
+  const int b = blockIdx.x;
+  const RaggedGroup& G = grp[ragged_group<&RaggedGroup::wr0>(grp, ngrp, b)];
+  const int row = b - G.wr0, cl = row / G.nr, i = row % G.nr;
+  const int64_t cs = G.cell0 + (int64_t)cl * G.nr * G.nz;
+#define AXS_WAVE_ROW(MM)                                                                     \
+  dct2_row_wave<T, true, MM>(src + cs, nullptr, dst + cs, G.rvec, G.kill_wz, G.kill_wr, G.nr, \
+                             w, G.tw, G.e1, G.e2, i, buf)
+  if (G.nz == 1024)
+    AXS_WAVE_ROW(512);
+  else if (G.nz == 768)
+    AXS_WAVE_ROW(384);
+  else
+    AXS_WAVE_ROW(256);
+#undef AXS_WAVE_ROW
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) dct3_wave_ragged_kernel(
+    const T* __restrict__ src, T* __restrict__ dst, const RaggedGroup* __restrict__ grp, int ngrp) {
+  __shared__ Cplx<T> buf[kF512Lds];
+  const int b = blockIdx.x;
+  const RaggedGroup& G = grp[ragged_group<&RaggedGroup::wr0>(grp, ngrp, b)];
+  const int row = b - G.wr0, cl = row / G.nr, i = row % G.nr;
+  const int64_t cs = G.cell0 + (int64_t)cl * G.nr * G.nz;
+  if (G.nz == 1024)
+    dct3_row_wave<T, 512>(src + cs, dst + cs, G.tw, G.e1, G.e2, i, buf);
+  else if (G.nz == 768)
+    dct3_row_wave<T, 384>(src + cs, dst + cs, G.tw, G.e1, G.e2, i, buf);
+  else
+    dct3_row_wave<T, 256>(src + cs, dst + cs, G.tw, G.e1, G.e2, i, buf);
 }
 
 // nz = 4096 row passes: two wavefronts per row (axs_fft1024.hpp, M = 2048).
@@ -1161,6 +1221,17 @@ hipError_t launch_dct3_ragged(const T* src, T* dst, const RaggedGroup* grp, int 
 }
 
 template <typename T>
+hipError_t launch_wave_rows_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp,
+                                   int wave_rows, int w, bool forward, hipStream_t s) {
+  if (wave_rows == 0) return hipSuccess;
+  if (forward)
+    taper_dct2_wave_ragged_kernel<T><<<wave_rows, 64, 0, s>>>(src, dst, grp, ngrp, w);
+  else
+    dct3_wave_ragged_kernel<T><<<wave_rows, 64, 0, s>>>(src, dst, grp, ngrp);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_tridiag_ragged(T* d, const RaggedGroup* grp, int ngrp, int blocks, size_t shm,
                                  int L, hipStream_t s) {
   if (L == 4)
@@ -1179,6 +1250,8 @@ hipError_t launch_tridiag_ragged(T* d, const RaggedGroup* grp, int ngrp, int blo
                                                   int, int, const StepFinalize*, hipStream_t);    \
   template hipError_t launch_dct3_ragged<T>(const T*, T*, const RaggedGroup*, int, int, int,      \
                                             hipStream_t);                                         \
+  template hipError_t launch_wave_rows_ragged<T>(const T*, T*, const RaggedGroup*, int, int, int, \
+                                                 bool, hipStream_t);                              \
   template hipError_t launch_tridiag_ragged<T>(T*, const RaggedGroup*, int, int, size_t, int,     \
                                                hipStream_t);
 AXS_RAGGED_INST(double)

@@ -92,6 +92,12 @@ hipError_t launch_taper_dct2_ragged(const T* src, T* dst, const RaggedGroup* grp
 template <typename T>
 hipError_t launch_dct3_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp,
                               int row_blocks, int lds_cplx, hipStream_t s);
+// The row passes of the groups marked wave (nz = 512, 768, 1024) in one launch
+// of wave_rows one-wavefront blocks (block b: row b - wr0 of its group):
+// forward = tapered DCT-II (src field -> dst spectrum), else DCT-III.
+template <typename T>
+hipError_t launch_wave_rows_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp,
+                                   int wave_rows, int w, bool forward, hipStream_t s);
 // Radial solves with segments of L rows (4, 8 or 16; the groups' plans
 // must have been made for that L), K and P per group, one 1024-thread
 // workgroup per K modes of a case.
