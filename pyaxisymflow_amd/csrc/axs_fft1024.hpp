@@ -151,15 +151,17 @@ __device__ __forceinline__ void dft6(Cplx<T> (&x)[6]) {
   x[5] = csub(e[2], o2);
 }
 
-// DFT-R in place, natural order (R = 4, 6, 8).
+// DFT-R in place, natural order (R = 2, 3, 4, 6, 8).
 template <typename T, int SGN, int R>
 __device__ __forceinline__ void dft_r(Cplx<T> (&x)[R]) {
   if constexpr (R == 8)
     dft8<T, SGN>(x);
   else if constexpr (R == 6)
     dft6<T, SGN>(x);
-  else
+  else if constexpr (R == 4)
     dft4n<T, SGN>(x[0], x[1], x[2], x[3]);
+  else
+    dft_small<T, R>(x, SGN);
 }
 
 template <int SGN>
@@ -227,7 +229,7 @@ __device__ __forceinline__ void fft1024_wave(Cplx<T>* buf, const Cplx<double>* _
   __syncthreads();
 }
 
-// FFT of the M = R1 x 64 complex values in buf (R1 = 4, 6, 8), one
+// FFT of the M = R1 x 64 complex values in buf (R1 = 2, 3, 4, 6, 8), one
 // wavefront, three passes (n = n2 + 64 n1, k = k1 + R1 k', n2 = m + 8 c,
 // k' = k2a + 8 k2b):
 //   pass 1  lane n2: DFT-R1 over n1, times W_M^(n2 k1)          -> Y[k1][n2]
@@ -238,7 +240,7 @@ __device__ __forceinline__ void fft1024_wave(Cplx<T>* buf, const Cplx<double>* _
 // Natural order in and out of buf.
 template <typename T, int SGN, int R1>
 __device__ __forceinline__ void fft_r64_wave(Cplx<T>* buf, const Cplx<double>* __restrict__ tw) {
-  static_assert(R1 == 4 || R1 == 6 || R1 == 8, "first radix");
+  static_assert(R1 == 2 || R1 == 3 || R1 == 4 || R1 == 6 || R1 == 8, "first radix");
   const int l = threadIdx.x & 63;
   {
     Cplx<T> v[R1];
@@ -295,7 +297,7 @@ __device__ __forceinline__ void fft_r64_wave(Cplx<T>* buf, const Cplx<double>* _
   __syncthreads();
 }
 
-// The one-wave transform of M = 256, 384, 512 or 1024 points.
+// The one-wave transform of M = 128, 192, 256, 384, 512 or 1024 points.
 template <typename T, int SGN, int M>
 __device__ __forceinline__ void fft_wave(Cplx<T>* buf, const Cplx<double>* __restrict__ tw) {
   if constexpr (M == kF1024M)
@@ -316,21 +318,27 @@ template <typename T, int M>
 __device__ __forceinline__ void dct2_post_pairs(const Cplx<T>* Z, const Cplx<double>* __restrict__ e1,
                                                 const Cplx<double>* __restrict__ e2,
                                                 T* __restrict__ out) {
-  constexpr int N = 2 * M, KP = M / 2 / 64;
+  // pairs k < M / 2 over 64 lanes (the last round partial when 64 does
+  // not divide M / 2)
+  constexpr int N = 2 * M, KP = (M / 2 + 63) / 64;
+  constexpr bool kFull = (M / 2) % 64 == 0;
   const int l = threadIdx.x;
   Cplx<double> a1[KP], b1[KP], a2[KP], b2[KP];
 #pragma unroll
   for (int q = 0; q < KP; ++q) {
     const int k = l + 64 * q;
-    a1[q] = e1[k];
-    b1[q] = e1[M - k];
-    a2[q] = e2[k];
-    b2[q] = e2[M - k];
+    if (kFull || k < M / 2) {
+      a1[q] = e1[k];
+      b1[q] = e1[M - k];
+      a2[q] = e2[k];
+      b2[q] = e2[M - k];
+    }
   }
   const Cplx<double> ah = e1[M / 2], bh = e2[M / 2];
 #pragma unroll
   for (int q = 0; q < KP; ++q) {
     const int k = l + 64 * q, kp = M - k;
+    if (!kFull && k >= M / 2) break;
     const Cplx<T> zk = Z[k], zp = Z[kp == M ? 0 : kp];  // Z is periodic in M
     const Cplx<T> sk = cadd(zk, cconj(zp)), dk = csub(zk, cconj(zp));
     const Cplx<T> wdk = cmul(Cplx<T>{T(a1[q].x), T(a1[q].y)}, dk);

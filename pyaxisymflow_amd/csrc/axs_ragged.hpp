@@ -26,7 +26,7 @@ struct RaggedGroup {
   int ft0, fz, ftiles;       // step_fused: first tile, tiles along z, tiles per case
   int part0;                 // first force partial of the group
   int wave;                  // 1: the row passes run one-wave register transforms
-                             // (nz = 512, 768, 1024): no Stockham row blocks here
+                             // (nz = 256, 384, 512, 768, 1024): no Stockham row blocks here
   int wr0;                   // first one-wave row block of the group (wave groups)
   int64_t cell0;             // first cell of the group's first case
   double dx;

@@ -517,7 +517,10 @@ __device__ __forceinline__ void dct3_row_wave(const T* __restrict__ src, T* __re
                                               const Cplx<double>* __restrict__ e2, int i,
                                               Cplx<T>* buf) {
   constexpr int N = 2 * M;
-  constexpr int KP = M / 2 / 64;  // index pairs (k, M - k) per lane
+  // index pairs (k, M - k), k < M / 2, per lane (the last round partial when
+  // 64 does not divide M / 2)
+  constexpr int KP = (M / 2 + 63) / 64;
+  constexpr bool kFull = (M / 2) % 64 == 0;
   typedef T T2 __attribute__((ext_vector_type(2)));
   static_assert(2 * kWaveLds<M> >= N, "the staged row fits the buffer");
   T* raw = reinterpret_cast<T*>(buf);  // the input row, staged before the pre step
@@ -532,10 +535,12 @@ __device__ __forceinline__ void dct3_row_wave(const T* __restrict__ src, T* __re
 #pragma unroll
   for (int q = 0; q < KP; ++q) {
     const int k = l + 64 * q;
-    q2k[q] = e2[k];
-    q2m[q] = e2[M - k];
-    q1k[q] = e1[k];
-    q1m[q] = e1[M - k];
+    if (kFull || k < M / 2) {
+      q2k[q] = e2[k];
+      q2m[q] = e2[M - k];
+      q1k[q] = e1[k];
+      q1m[q] = e1[M - k];
+    }
   }
   const Cplx<double> q2h = e2[M / 2], q1h = e1[M / 2];
 #pragma unroll
@@ -545,10 +550,12 @@ __device__ __forceinline__ void dct3_row_wave(const T* __restrict__ src, T* __re
 #pragma unroll
   for (int q = 0; q < KP; ++q) {
     const int k = l + 64 * q;
-    xk[q] = raw[k];
-    xnk[q] = (k == 0) ? T(0) : raw[N - k];
-    xm[q] = raw[M - k];
-    xnm[q] = raw[M + k];
+    if (kFull || k < M / 2) {
+      xk[q] = raw[k];
+      xnk[q] = (k == 0) ? T(0) : raw[N - k];
+      xm[q] = raw[M - k];
+      xnm[q] = raw[M + k];
+    }
   }
   const T xh = raw[M / 2], xnh = raw[N - M / 2];
   __syncthreads();  // raw is read above and overwritten by Z below
@@ -558,6 +565,7 @@ __device__ __forceinline__ void dct3_row_wave(const T* __restrict__ src, T* __re
 #pragma unroll
   for (int q = 0; q < KP; ++q) {
     const int k = l + 64 * q;
+    if (!kFull && k >= M / 2) break;
     const Cplx<T> Vk = cmul(Cplx<T>{T(q2k[q].x), T(-q2k[q].y)}, Cplx<T>{xk[q], -xnk[q]});
     const Cplx<T> Vm = cmul(Cplx<T>{T(q2m[q].x), T(-q2m[q].y)}, Cplx<T>{xm[q], -xnm[q]});
     {
@@ -600,7 +608,7 @@ __global__ void __launch_bounds__(64) dct3_rows_wave_kernel(
 }
 
 // Uneven batches: the rows of every group with a one-wave register transform
-// (G.wave: nz = 512, 768, 1024) in one launch, one wavefront per row; block
+// (G.wave: nz = 256, 384, 512, 768, 1024) in one launch, one wavefront per row; block
 // b is row b - G.wr0 of its group's cases.
 template <typename T>
 __global__ void __launch_bounds__(64) taper_dct2_wave_ragged_kernel(
@@ -618,8 +626,12 @@ __global__ void __launch_bounds__(64) taper_dct2_wave_ragged_kernel(
     AXS_WAVE_ROW(512);
   else if (G.nz == 768)
     AXS_WAVE_ROW(384);
-  else
+  else if (G.nz == 512)
     AXS_WAVE_ROW(256);
+  else if (G.nz == 384)
+    AXS_WAVE_ROW(192);
+  else
+    AXS_WAVE_ROW(128);
 #undef AXS_WAVE_ROW
 }
 
@@ -635,8 +647,12 @@ __global__ void __launch_bounds__(64) dct3_wave_ragged_kernel(
     dct3_row_wave<T, 512>(src + cs, dst + cs, G.tw, G.e1, G.e2, i, buf);
   else if (G.nz == 768)
     dct3_row_wave<T, 384>(src + cs, dst + cs, G.tw, G.e1, G.e2, i, buf);
-  else
+  else if (G.nz == 512)
     dct3_row_wave<T, 256>(src + cs, dst + cs, G.tw, G.e1, G.e2, i, buf);
+  else if (G.nz == 384)
+    dct3_row_wave<T, 192>(src + cs, dst + cs, G.tw, G.e1, G.e2, i, buf);
+  else
+    dct3_row_wave<T, 128>(src + cs, dst + cs, G.tw, G.e1, G.e2, i, buf);
 }
 
 // nz = 4096 row passes: two wavefronts per row (axs_fft1024.hpp, M = 2048).
@@ -1042,7 +1058,9 @@ __global__ void __launch_bounds__(1024) tridiag_ragged_kernel(T* __restrict__ d,
 // ------------------------------------------------------------------ launchers
 
 // Half-lengths with a one-wave register transform (axs_fft1024.hpp).
-bool wave_rows(int M) { return M == 256 || M == 384 || M == 512 || M == 1024; }
+bool wave_rows(int M) {
+  return M == 128 || M == 192 || M == 256 || M == 384 || M == 512 || M == 1024;
+}
 
 static int fft_block(int M) {
   // threads so that every stage needs at most PER butterflies per thread
@@ -1064,7 +1082,9 @@ hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const do
     if (M == 1024) AXS_DCT2_WAVE(1024);
     else if (M == 512) AXS_DCT2_WAVE(512);
     else if (M == 384) AXS_DCT2_WAVE(384);
-    else AXS_DCT2_WAVE(256);
+    else if (M == 256) AXS_DCT2_WAVE(256);
+    else if (M == 192) AXS_DCT2_WAVE(192);
+    else AXS_DCT2_WAVE(128);
 #undef AXS_DCT2_WAVE
     return hipGetLastError();
   }
@@ -1102,7 +1122,9 @@ hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered
     if (M == 1024) AXS_TAPER_WAVE(1024);
     else if (M == 512) AXS_TAPER_WAVE(512);
     else if (M == 384) AXS_TAPER_WAVE(384);
-    else AXS_TAPER_WAVE(256);
+    else if (M == 256) AXS_TAPER_WAVE(256);
+    else if (M == 192) AXS_TAPER_WAVE(192);
+    else AXS_TAPER_WAVE(128);
 #undef AXS_TAPER_WAVE
     return hipGetLastError();
   }
@@ -1140,7 +1162,9 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
     if (M == 1024) AXS_DCT3_WAVE(1024);
     else if (M == 512) AXS_DCT3_WAVE(512);
     else if (M == 384) AXS_DCT3_WAVE(384);
-    else AXS_DCT3_WAVE(256);
+    else if (M == 256) AXS_DCT3_WAVE(256);
+    else if (M == 192) AXS_DCT3_WAVE(192);
+    else AXS_DCT3_WAVE(128);
 #undef AXS_DCT3_WAVE
     return hipGetLastError();
   }

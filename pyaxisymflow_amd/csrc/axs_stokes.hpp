@@ -76,7 +76,7 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
 template <typename T>
 hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch = 1);
 // True when rows of half-length M run a one-wave register transform
-// (M = 256, 384, 512, 1024; M = 2048 runs two waves per row).
+// (M = 128, 192, 256, 384, 512, 1024; M = 2048 runs two waves per row).
 bool wave_rows(int M);
 
 // Uneven batches (axs_ragged.hpp): one launch per pass over every group.
@@ -92,7 +92,7 @@ hipError_t launch_taper_dct2_ragged(const T* src, T* dst, const RaggedGroup* grp
 template <typename T>
 hipError_t launch_dct3_ragged(const T* src, T* dst, const RaggedGroup* grp, int ngrp,
                               int row_blocks, int lds_cplx, hipStream_t s);
-// The row passes of the groups marked wave (nz = 512, 768, 1024) in one launch
+// The row passes of the groups marked wave (nz = 256 to 1024) in one launch
 // of wave_rows one-wavefront blocks (block b: row b - wr0 of its group):
 // forward = tapered DCT-II (src field -> dst spectrum), else DCT-III.
 template <typename T>

@@ -241,11 +241,11 @@ def test_stokes_mixed_radix(nz, nr):
     assert rel_l2(psi, exact_stokes_solve(w, dx)) < 1e-12
 
 
-@pytest.mark.parametrize("nz,nr", [(512, 50), (768, 40), (1024, 64), (1024, 45), (2048, 64),
-                                   (2048, 131), (4096, 64), (4096, 37)])
+@pytest.mark.parametrize("nz,nr", [(256, 40), (384, 30), (512, 50), (768, 40), (1024, 64),
+                                   (1024, 45), (2048, 64), (2048, 131), (4096, 64), (4096, 37)])
 def test_stokes_register_fft_rows(nz, nr):
-    """nz = 512, 768, 1024 and 2048 run the one-wave-per-row register FFTs
-    (axs_fft1024.hpp: R1 x 8 x 8 points with R1 = 4, 6, 8, and 16 x 64),
+    """nz = 256 to 2048 run the one-wave-per-row register FFTs
+    (axs_fft1024.hpp: R1 x 8 x 8 points with R1 = 2, 3, 4, 6, 8, and 16 x 64),
     nz = 4096 two waves per row (one 1024-point FFT each and a radix-2
     join): against the exact discrete solution and its residual."""
     from pyaxisymflow_amd.kernels.FastDiagonalisationStokesSolver import (

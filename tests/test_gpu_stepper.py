@@ -87,10 +87,11 @@ def test_flow_past_sphere_dx_not_power_of_two():
     np.testing.assert_allclose(gpu.drag_history(), np.array(cpu.drag), rtol=1e-10, atol=1e-12)
 
 
-@pytest.mark.parametrize("nz,nr", [(512, 40), (768, 40), (1024, 40), (2048, 40), (4096, 40)])
+@pytest.mark.parametrize("nz,nr", [(256, 40), (384, 40), (512, 40), (768, 40), (1024, 40),
+                                   (2048, 40), (4096, 40)])
 def test_register_fft_row_passes_vs_oracle(nz, nr):
     """The tapered forward row pass and the inverse row pass of the register
-    FFTs (nz = 512 to 2048: one wavefront per row; nz = 4096: two, joined
+    FFTs (nz = 256 to 2048: one wavefront per row; nz = 4096: two, joined
     by a radix-2 step) inside the step: 5 steps against the oracle."""
     case = sphere_case(nz=nz, nr=nr, seed=3)
     cpu, gpu = run_pair(case, 5)
