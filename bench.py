@@ -264,6 +264,43 @@ def time_precision(grid_size_z=1024, no_cycles=20):
     )
 
 
+def time_implicit(sizes=((256, 128), (1024, 512), (2048, 1024)), reps=20):
+    """Implicit Euler diffusion (kernels/implicit_diffusion_solver.py, the
+    drop-in ImplicitEulerDiffusionStepper) on device-resident fields: one
+    step = DST-I rows, radial tridiagonal solves, DST-I rows.  256 x 128 is
+    the torus-streaming grid that uses it; dt = dx^2 / 4 / nu
+    (flow_past_sphere.py:90).  The dense DST-I rows are 2 nz^2 flops per row
+    each (the parity tests check the result against the oracle)."""
+    import torch
+
+    from pyaxisymflow_amd.kernels.implicit_diffusion_solver import ImplicitEulerDiffusionStepper
+
+    res = {}
+    for nz, nr in sizes:
+        dx, nu = 1.0 / nz, 1.5e-3
+        dt = dx**2 / 4 / nu
+        w0 = np.random.default_rng(0).standard_normal((nr, nz))
+        st = ImplicitEulerDiffusionStepper(dt, nu, nr, nz, dx)
+        w = torch.from_numpy(w0).cuda()
+        st.step(w, dt)
+        torch.cuda.synchronize()
+        s = torch.cuda.current_stream()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            st.step(w, dt)
+        e1.record(s)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        flops = 2 * (2.0 * nr * nz * nz)
+        res[f"{nz}x{nr}"] = dict(us_per_step=round(us, 2),
+                                 mcell_steps_per_s=round(nr * nz / us, 2),
+                                 dst_tflops=round(flops / (us * 1e-6) / 1e12, 2))
+    return dict(unit="us per implicit step (drop-in, fields resident)", sizes=res,
+                note="DST-I rows as a dense product (fast sine transforms need nz+1-point FFTs)")
+
+
 def time_serving(n_requests, steps, max_active=16, seed=0):
     """The serving form: soft-sphere requests (pyaxisymflow_amd.serve:
     nr lognormal around 256 in [128, 1024], 5-smooth; G in [0.1, 10]).  A
@@ -354,6 +391,7 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=2000)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-implicit", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every case is independent and this build runs on "
@@ -418,6 +456,8 @@ def main():
         forms["precision_fp32"] = time_precision()
     if not args.no_batched:
         forms["batched_fp32"] = time_batched(args.batched_steps, 2)
+    if not args.no_implicit:
+        forms["implicit_diffusion"] = time_implicit()
     if not args.no_serve:
         forms["serving_soft"] = time_serving(args.serve_requests, args.serve_steps)
     if forms:

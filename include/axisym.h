@@ -139,6 +139,22 @@ axs_status axs_stokes_plan_destroy(axs_stokes_plan* plan);
 axs_status axs_stokes_solve_f64(axs_stokes_plan* plan, double* psi, const double* rhs,
                                 void* stream);
 
+/* ---- Implicit Euler diffusion: kernels/implicit_diffusion_solver.py:6-143
+ * (ImplicitEulerDiffusionStepper; called by flow_past_sphere.py:180-181,
+ * spheroid_streaming.py:236-237, torus_streaming.py:182 when
+ * implicit_diffusion is set).  Solves (I - nu_dt L) w' = w in place, with
+ * L w = A_r w + w P_z^T built as :40-72 (zero ghost values at every edge:
+ * Dirichlet P_z, A_r = P_r + D_r / r - I / r^2), by a DST-I along z, one
+ * radial tridiagonal solve per z-mode and the DST-I back.  nu_dt is
+ * kinematic_viscosity * time_step, fixed per plan (the reference rejects any
+ * other dt, :115-120).  The plan owns its device tables and scratch. */
+typedef struct axs_implicit_plan axs_implicit_plan;
+axs_status axs_implicit_diffusion_plan_create(int nr, int nz, double dx, double nu_dt,
+                                              axs_implicit_plan** plan_out);
+axs_status axs_implicit_diffusion_plan_destroy(axs_implicit_plan* plan);
+axs_status axs_implicit_diffusion_step_f64(axs_implicit_plan* plan, double* omega,
+                                           void* stream);
+
 /* ---- A14: core/src/extrapolate_using_least_squares.hpp:450-467
  * (extrapolate_using_least_squares_till_first_order<short, double>, bound
  * by core/extrapolate_using_least_squares_bind.cpp:11-41).  Flags are int16

@@ -94,6 +94,9 @@ SIGNATURES = {
     "axs_stokes_plan_create": [_I, _I, _D, _I, ctypes.POINTER(_P)],
     "axs_stokes_plan_destroy": [_P],
     "axs_stokes_solve_f64": [_P, _P, _P, _P],
+    "axs_implicit_diffusion_plan_create": [_I, _I, _D, _D, ctypes.POINTER(_P)],
+    "axs_implicit_diffusion_plan_destroy": [_P],
+    "axs_implicit_diffusion_step_f64": [_P, _P, _P],
     "axs_extrapolate_lsq_first_order_f64": [_P, _P, _P, _P, _P, _P, _I, _I, _I,
                                             ctypes.POINTER(_I), _P],
     "axs_rigid_stepper_create": [ctypes.POINTER(RigidConfig), ctypes.POINTER(RigidBuffers),

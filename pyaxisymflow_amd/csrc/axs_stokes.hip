@@ -21,6 +21,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 namespace axs {
@@ -85,7 +86,7 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
     int K, L;
     bool G;
   };
-  static std::map<std::pair<int, int>, Choice> cache;
+  static std::map<std::tuple<int, int, int>, Choice> cache;
   Choice best{tri_K, tri_L, tri_G};
   bool cached = false;
   // AXS_TRIDIAG_LK="L,K" pins the choice (the search times the candidates,
@@ -103,7 +104,7 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
   }
   if (!cached) {
     std::lock_guard<std::mutex> g(mu);
-    auto it = cache.find({nr, nz});
+    auto it = cache.find({nr, nz, tune_key});
     if (it != cache.end()) {
       best = it->second;
       cached = true;
@@ -165,7 +166,7 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
     (void)hipEventDestroy(t1);
     if (e != hipSuccess) return e;
     std::lock_guard<std::mutex> g(mu);
-    cache[{nr, nz}] = best;
+    cache[{nr, nz, tune_key}] = best;
   }
   tri_G = best.G;
   if (best.K == tri_K && best.L == tri_L) return hipSuccess;
@@ -222,6 +223,17 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
     const double s = std::sin(M_PI * (double)k / (2.0 * nz));
     h_lam[k] = (4.0 / (dx * dx)) * s * s;
   }
+  hipError_t e;
+  if ((e = upload(&tw, h_tw)) != hipSuccess) return e;
+  if ((e = upload(&e1, h_e1)) != hipSuccess) return e;
+  if ((e = upload(&e2, h_e2)) != hipSuccess) return e;
+  if ((e = upload(&rvec, h_r)) != hipSuccess) return e;
+  return set_tridiag(h_a, h_b, h_c, h_lam);
+}
+
+hipError_t StokesPlan::set_tridiag(const std::vector<double>& h_a, const std::vector<double>& h_b,
+                                   const std::vector<double>& h_c,
+                                   const std::vector<double>& h_lam) {
   // Segment layout of the column pass and c'_{s-1} at every segment start.
   // Segments of L = 8 rows (16 for very tall grids) keep the per-thread
   // registers small; K modes per workgroup so that K * P <= 1024 and, when
@@ -247,14 +259,10 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
   if (tri_lds_bytes(nr, tri_L, tri_K, tri_P, tri_G) > 65536) return hipErrorInvalidValue;
   std::vector<double> h_cseg = pivot_table(h_a, h_b, h_c, h_lam, nr, nz, tri_L);
   hipError_t e;
-  if ((e = upload(&tw, h_tw)) != hipSuccess) return e;
-  if ((e = upload(&e1, h_e1)) != hipSuccess) return e;
-  if ((e = upload(&e2, h_e2)) != hipSuccess) return e;
   if ((e = upload(&a, h_a)) != hipSuccess) return e;
   if ((e = upload(&b, h_b)) != hipSuccess) return e;
   if ((e = upload(&c, h_c)) != hipSuccess) return e;
   if ((e = upload(&lam, h_lam)) != hipSuccess) return e;
-  if ((e = upload(&rvec, h_r)) != hipSuccess) return e;
   if ((e = upload(&cseg, h_cseg)) != hipSuccess) return e;
   e = hipMalloc(reinterpret_cast<void**>(&spec), sizeof(double) * (size_t)nr * nz);
   if (e != hipSuccess) return e;

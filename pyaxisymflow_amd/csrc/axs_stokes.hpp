@@ -38,8 +38,16 @@ struct StokesPlan {
   // > 0: segments of force_L rows and as many modes per workgroup as fit
   // 1024 threads, no timed search (the layout an uneven batch launches)
   int force_L = 0;
+  // separates the cached column-pass searches of plans whose matrices
+  // differ on the same grid (0: Stokes, 1: implicit diffusion)
+  int tune_key = 0;
 
   hipError_t init(int nr, int nz, double dx);
+  // Column-pass tables (a, b, c, lam, pivots at segment starts), the
+  // spectral buffer and the segment layout for the given matrices; nr and
+  // nz must be set.
+  hipError_t set_tridiag(const std::vector<double>& a, const std::vector<double>& b,
+                         const std::vector<double>& c, const std::vector<double>& lam);
   hipError_t tune_tridiag(const std::vector<double>& a, const std::vector<double>& b,
                           const std::vector<double>& c, const std::vector<double>& lam);
   int tri_threads() const;
