@@ -36,6 +36,9 @@ STAGE_BYTES_PER_CELL = {
     "velocity_max": 8,       # read psi
     "body_mask": 8,          # write chi (moving body only)
     "step_fused": 32,        # read psi, chi, omega; write omega
+    # implicit-diffusion drivers only: DST-I rows, radial solve, DST-I rows
+    # (16 B each; the DST-I matrix stays in L2 for nz <= 1024)
+    "implicit_diffusion": 48,
 }
 
 
@@ -427,7 +430,8 @@ def main():
             "nz": nz, "nr": nr, "domain": "z in [0,2], r in [0,1]", "lambda": 1e4, "nu": 1.5e-3,
             "U": 1.0, "graph": True, "last_Cd": cd,
             "stage_us": {k: round(v, 2) for k, v in stages.items()},
-            "step_bytes_per_cell": sum(v for k, v in STAGE_BYTES_PER_CELL.items() if k != "body_mask"),
+            "step_bytes_per_cell": sum(v for k, v in STAGE_BYTES_PER_CELL.items()
+                                       if k not in ("body_mask", "implicit_diffusion")),
             "graded_bytes_per_cell": 88,
             "copy_kernel_GBs": round(copy_gbs, 1),
         },

@@ -213,12 +213,14 @@ typedef struct {
   int precision;       /* AXS_F64 or AXS_F32 */
   int moving_body;     /* 1: chi = H(phi) of the oscillating spheroid, rebuilt each step */
   int average_psi;     /* 1: avg_psi += psi dt, restarted at each period boundary */
-  int pad;
+  int implicit_diffusion; /* 1: implicit Euler diffusion (implicit_diffusion_solver.py)
+                          * replaces the RK2 stages, dt = dt_diffusive every step */
   double dx, nu, brink_lam;
   double U0;           /* free stream added to u_z each step */
   double T_ramp;       /* ramp sin(pi t / (2 T_ramp)) while t < T_ramp; <= 0: none */
   double CFL, eps;     /* dt = min(dt_diffusive, CFL dx / (max|u| + eps), dt_cap) */
-  double dt_diffusive; /* 0.9 dx^2 / 4 / nu as the driver forms it */
+  double dt_diffusive; /* 0.9 dx^2 / 4 / nu as the driver forms it; with
+                        * implicit_diffusion the fixed dt (dx^2 / 4 / nu) */
   double osc_amp, osc_omega; /* penalisation velocity U_z(t) = osc_amp cos(osc_omega t) */
   double drag_scale, drag_div; /* force = drag_scale * sum(r chi (u_z - U_z(t))) / drag_div */
   double dt_cap;       /* third term of the dt min (HUGE_VAL: none) */

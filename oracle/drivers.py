@@ -10,6 +10,7 @@ oracle kernels.  TEST ORACLE ONLY.
 import numpy as np
 
 from oracle import kernels as K
+from oracle.implicit_diffusion import ImplicitEulerDiffusionStepper, exact_implicit_diffusion
 from oracle.stokes import FastDiagonalisationStokesSolver, exact_stokes_solve
 
 
@@ -56,7 +57,7 @@ class RigidBodyCase:
 
     def __init__(self, nz, nr, z_len, nu, brink_lam, char_func, U_0=1.0,
                  T_ramp=None, r_sph=None, CFL=0.1, vorticity=None,
-                 solver="eig", oscillation=None):
+                 solver="eig", oscillation=None, implicit_diffusion=False):
         self.dx, self.z, self.r, self.Z, self.R = grid(nz, nr, z_len)
         self.nz, self.nr = nz, nr
         self.nu = nu
@@ -81,6 +82,15 @@ class RigidBodyCase:
         self.solver_kind = solver
         if solver == "eig":
             self.solver = FastDiagonalisationStokesSolver(nr, nz, self.dx)
+        # flow_past_sphere.py:90-98: implicit Euler diffusion at the fixed
+        # step dx^2 / 4 / nu (eig route, or the exact route with solver="exact")
+        self.implicit_diffusion = implicit_diffusion
+        if implicit_diffusion:
+            self.diffusion_dt_limit = self.dx**2 / 4 / nu
+            if solver == "eig":
+                self.implicit_stepper = ImplicitEulerDiffusionStepper(
+                    time_step=self.diffusion_dt_limit, kinematic_viscosity=nu,
+                    grid_size_r=nr, grid_size_z=nz, dx=self.dx)
 
     def solve(self):
         if self.solver_kind == "eig":
@@ -104,10 +114,13 @@ class RigidBodyCase:
         else:
             U_amp, omega = self.oscillation
             U_pen = U_amp * np.cos(omega * self.t)
-        dt = min(
-            0.9 * dx**2 / 4 / self.nu,
-            self.CFL * dx / (np.amax(np.fabs(self.u_z) + np.fabs(self.u_r)) + self.eps),
-        )
+        if self.implicit_diffusion:  # :146-148
+            dt = self.diffusion_dt_limit
+        else:
+            dt = min(
+                0.9 * dx**2 / 4 / self.nu,
+                self.CFL * dx / (np.amax(np.fabs(self.u_z) + np.fabs(self.u_r)) + self.eps),
+            )
         u_z_upen = self.u_z.copy()
         u_r_upen = self.u_r.copy()
         K.brinkmann_penalize(self.brink_lam, dt, self.char_func, U_pen, 0.0,
@@ -120,7 +133,12 @@ class RigidBodyCase:
             self.drag.append(K.drag_coefficient(self.R, self.char_func, self.u_z, dx,
                                                 self.brink_lam, self.r_sph))
         K.advect_vorticity_via_eno3(self.vorticity, self.u_z, self.u_r, dt, dx)
-        K.diffusion_RK2_unb(self.vorticity, self.temp_vorticity, self.R, self.nu, dt, dx)
+        if not self.implicit_diffusion:
+            K.diffusion_RK2_unb(self.vorticity, self.temp_vorticity, self.R, self.nu, dt, dx)
+        elif self.solver_kind == "eig":  # :179-181
+            self.implicit_stepper.step(vorticity_field=self.vorticity, dt=dt)
+        else:
+            self.vorticity[...] = exact_implicit_diffusion(self.vorticity, self.nu * dt, dx)
         self.t += dt
         self.it += 1
         self.dts.append(dt)

@@ -34,7 +34,8 @@ class RigidConfig(ctypes.Structure):
     """axs_rigid_config (include/axisym.h)."""
 
     _fields_ = [(n, ctypes.c_int) for n in (
-        "nr", "nz", "kill_width", "history", "precision", "moving_body", "average_psi", "pad")] + [
+        "nr", "nz", "kill_width", "history", "precision", "moving_body", "average_psi",
+        "implicit_diffusion")] + [
         (n, ctypes.c_double) for n in (
             "dx", "nu", "brink_lam", "U0", "T_ramp", "CFL", "eps", "dt_diffusive", "osc_amp",
             "osc_omega", "drag_scale", "drag_div", "dt_cap", "period", "t_end", "body_z0",

@@ -274,8 +274,11 @@ struct FusedLds {
 // One output tile.  IN: the tile and its 6-cell halo lie at least kill_width
 // cells inside every domain edge (checked by the caller), so no cell needs a
 // bounds test, a one-sided velocity formula, the kill taper, the axis mirror
-// or a border exemption; the other tiles take every test.
-template <typename T, bool IN>
+// or a border exemption; the other tiles take every test.  RK2 false: the
+// implicit-diffusion form of the drivers (flow_past_sphere.py:178-181), whose
+// tile output is the ENO3 result (the diffusion solve follows as its own
+// passes).
+template <typename T, bool IN, bool RK2 = true>
 __device__ __forceinline__ void step_fused_tile(
     const T* __restrict__ psi, const T* __restrict__ chi, const T* __restrict__ wsrc,
     T* __restrict__ wdst, T* __restrict__ avg, const double* __restrict__ r,
@@ -558,6 +561,19 @@ __device__ __forceinline__ void step_fused_tile(
   }
   __syncthreads();
 
+  if constexpr (!RK2) {
+    // the ENO3 output of the tile's own cells is the step's result
+    if (tid < (kFR / kOH) * kFZ) {
+      const int b = tid % kFZ, a0 = (tid / kFZ) * kOH;
+      const int j = j0 + b;
+#pragma unroll
+      for (int k = 0; k < kOH; ++k) {
+        const int i = i0 + a0 + k;
+        if (!IN && (j >= nz || i >= nr)) continue;
+        wdst[(int64_t)i * nz + j] = sa[a0 + k + 2][b + 2];
+      }
+    }
+  } else {
   // ---- 5. RK2 stage 1 on R1 (diffusion_RK2.py:9-26)
   const T dx2 = dx * dx, inv_dx2 = T(1) / dx2;
   const T half_nu_dt = T(0.5 * cfg.nu * ds.dt), nu_dt = T(cfg.nu * ds.dt);
@@ -612,6 +628,7 @@ __device__ __forceinline__ void step_fused_tile(
         wdst[(int64_t)i * nz + j] = v;
     }
   }
+  }  // RK2
 
   // ---- force partial of this tile; block (0, 0) publishes the dt scalars
   //      (read back from LDS, not held in registers through the phases)
@@ -637,7 +654,7 @@ template <typename T>
 constexpr int kRaggedFusedWaves = sizeof(T) == 8 ? 4 : 7;
 
 // 256 threads, waves and LDS as kFusedWaves says.
-template <typename T>
+template <typename T, bool RK2>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFusedWaves<T>))) step_fused_kernel(
     const T* __restrict__ psi, const T* __restrict__ chi, const T* __restrict__ wsrc,
     T* __restrict__ wdst, T* __restrict__ avg, const double* __restrict__ r,
@@ -674,10 +691,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFused
   const bool interior = i0 - 6 >= m && i0 + kFR + 6 <= nr - m && j0 - 6 >= m &&
                         j0 + kFZ + 6 <= nz - m;
   if (interior)
-    step_fused_tile<T, true>(psi, chi, wsrc, wdst, avg, r, wz, wr, kw, nr, nz, dx, sc, slots,
+    step_fused_tile<T, true, RK2>(psi, chi, wsrc, wdst, avg, r, wz, wr, kw, nr, nz, dx, sc, slots,
                              partials, cfg, L, i0, j0, tile);
   else
-    step_fused_tile<T, false>(psi, chi, wsrc, wdst, avg, r, wz, wr, kw, nr, nz, dx, sc, slots,
+    step_fused_tile<T, false, RK2>(psi, chi, wsrc, wdst, avg, r, wz, wr, kw, nr, nz, dx, sc, slots,
                               partials, cfg, L, i0, j0, tile);
 }
 
@@ -797,9 +814,14 @@ template <typename T>
 hipError_t launch_step_fused(const T* psi, const T* chi, const T* wsrc, T* wdst, T* avg,
                              const double* r, const double* wz, const double* wr, int kw, int nr,
                              int nz, T dx, double* sc, const double* slots, double* partials,
-                             const RigidConfig* cfgs, int batch, hipStream_t s) {
-  step_fused_kernel<T><<<dim3(ceil_div(nz, kFZ), ceil_div(nr, kFR), batch), 256, 0, s>>>(
-      psi, chi, wsrc, wdst, avg, r, wz, wr, kw, nr, nz, dx, sc, slots, partials, cfgs);
+                             const RigidConfig* cfgs, int batch, hipStream_t s, bool rk2) {
+  const dim3 grid(ceil_div(nz, kFZ), ceil_div(nr, kFR), batch);
+  if (rk2)
+    step_fused_kernel<T, true><<<grid, 256, 0, s>>>(psi, chi, wsrc, wdst, avg, r, wz, wr, kw, nr,
+                                                    nz, dx, sc, slots, partials, cfgs);
+  else
+    step_fused_kernel<T, false><<<grid, 256, 0, s>>>(psi, chi, wsrc, wdst, avg, r, wz, wr, kw, nr,
+                                                     nz, dx, sc, slots, partials, cfgs);
   return hipGetLastError();
 }
 
@@ -861,7 +883,7 @@ hipError_t launch_finalize_ragged(const double* partials, double* slots, double*
   template hipError_t launch_step_fused<T>(const T*, const T*, const T*, T*, T*,            \
                                            const double*, const double*, const double*, int, \
                                            int, int, T, double*, const double*, double*,    \
-                                           const RigidConfig*, int, hipStream_t);            \
+                                           const RigidConfig*, int, hipStream_t, bool);      \
   template hipError_t launch_velocity_out<T>(const T*, const T*, T*, T*, const double*, int, \
                                              int, T, const double*, const RigidConfig*, int, \
                                              hipStream_t);

@@ -42,6 +42,9 @@ __device__ __forceinline__ DtScalars compute_dt(const double* sc, double m,
     fr = 0.0;
     reset = 1.0;
   }
+  // implicit diffusion: the fixed step the diffusion plan was built for
+  // (flow_past_sphere.py:146-148, spheroid_streaming.py:190-191)
+  if (cfg.implicit_diffusion) return DtScalars{cfg.dt_diffusive, fr, reset};
   double dt = cfg.dt_diffusive;
   const double cfl_dt = cfg.CFL * cfg.dx / (m + cfg.eps);
   if (cfl_dt < dt) dt = cfl_dt;
@@ -142,7 +145,7 @@ template <typename T>
 hipError_t launch_step_fused(const T* psi, const T* chi, const T* wsrc, T* wdst, T* avg,
                              const double* r, const double* wz, const double* wr, int kw, int nr,
                              int nz, T dx, double* sc, const double* slots, double* partials,
-                             const RigidConfig* cfgs, int batch, hipStream_t s);
+                             const RigidConfig* cfgs, int batch, hipStream_t s, bool rk2 = true);
 template <typename T>
 hipError_t launch_velocity_out(const T* psi, const T* chi, T* pz, T* pr, const double* r, int nr,
                                int nz, T dx, const double* sc, const RigidConfig* cfgs, int batch,

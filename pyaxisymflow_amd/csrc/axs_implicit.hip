@@ -18,7 +18,7 @@
 #pragma clang fp contract(fast)
 
 #include "axs_common.hpp"
-#include "axs_stokes.hpp"
+#include "axs_implicit.hpp"
 
 #include <cmath>
 #include <new>
@@ -100,38 +100,16 @@ static hipError_t launch_dst1_rows(const double* in, double* out, const double* 
   return hipGetLastError();
 }
 
-}  // namespace axs
+ImplicitPlan::~ImplicitPlan() {
+  if (S) (void)hipFree(S);
+}
 
-using namespace axs;
-
-struct axs_implicit_plan {
-  StokesPlan tri;         // column-pass tables and the spectral buffer
-  double* S = nullptr;    // orthonormal DST-I matrix, nz x nz
-  ~axs_implicit_plan() {
-    if (S) (void)hipFree(S);
-  }
-};
-
-extern "C" {
-
-axs_status axs_implicit_diffusion_plan_create(int nr, int nz, double dx, double nu_dt,
-                                              axs_implicit_plan** plan_out) {
-  AXS_CHECK_ARG(plan_out, "null plan_out");
-  *plan_out = nullptr;
-  AXS_CHECK_ARG(nr >= 3 && nz >= 2, "need nr >= 3 and nz >= 2");
-  AXS_CHECK_ARG(nz <= 4096 && nr <= 4096, "grid too large for the row/column passes");
-  AXS_CHECK_ARG(dx > 0.0, "dx must be positive");
-  AXS_CHECK_ARG(nu_dt >= 0.0 && std::isfinite(nu_dt), "nu * dt must be finite and >= 0");
-  axs_implicit_plan* p = new (std::nothrow) axs_implicit_plan();
-  if (!p) {
-    set_error("out of host memory");
-    return AXS_ERR_ALLOC;
-  }
-  StokesPlan& pl = p->tri;
-  pl.nr = nr;
-  pl.nz = nz;
-  pl.dx = dx;
-  pl.tune_key = 1;
+hipError_t ImplicitPlan::init(int nr, int nz, double dx, double nu_dt_, int batch) {
+  nu_dt = nu_dt_;
+  tri.nr = nr;
+  tri.nz = nz;
+  tri.dx = dx;
+  tri.tune_key = 1;
   // radial coordinate as implicit_diffusion_solver.py:22-26 (numpy.linspace)
   std::vector<double> r(nr);
   const double start = dx / 2, stop = nr * dx - dx / 2;
@@ -156,18 +134,57 @@ axs_status axs_implicit_diffusion_plan_create(int nr, int nz, double dx, double 
   }
   // S[j, k] = sqrt(2 / (nz + 1)) sin(pi (j+1)(k+1) / (nz + 1)), the angle
   // reduced modulo 2 pi in integers first
-  std::vector<double> S((size_t)nz * nz);
+  std::vector<double> h((size_t)nz * nz);
   const double norm = std::sqrt(2.0 / (nz + 1));
   const int64_t period = 2 * (int64_t)(nz + 1);
   for (int j = 0; j < nz; ++j)
     for (int k = 0; k < nz; ++k) {
       const int64_t m = ((int64_t)(j + 1) * (k + 1)) % period;
-      S[(size_t)j * nz + k] = norm * std::sin(M_PI * (double)m / (double)(nz + 1));
+      h[(size_t)j * nz + k] = norm * std::sin(M_PI * (double)m / (double)(nz + 1));
     }
-  hipError_t e = pl.set_tridiag(a, b, c, lam);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->S), sizeof(double) * S.size());
-  if (e == hipSuccess)
-    e = hipMemcpy(p->S, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice);
+  hipError_t e = tri.set_tridiag(a, b, c, lam);
+  if (e == hipSuccess && batch > 1) {  // the spectral buffer of a whole batch
+    (void)hipFree(tri.spec);
+    tri.spec = nullptr;
+    e = hipMalloc(reinterpret_cast<void**>(&tri.spec), sizeof(double) * (size_t)nr * nz * batch);
+  }
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&S), sizeof(double) * h.size());
+  if (e == hipSuccess) e = hipMemcpy(S, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice);
+  return e;
+}
+
+hipError_t ImplicitPlan::launch(double* omega, int batch, hipStream_t s) {
+  const int rows = tri.nr * batch;
+  hipError_t e = launch_dst1_rows(omega, tri.spec, S, rows, tri.nz, s);
+  if (e == hipSuccess) e = launch_tridiag<double>(tri, tri.spec, s, batch);
+  if (e == hipSuccess) e = launch_dst1_rows(tri.spec, omega, S, rows, tri.nz, s);
+  return e;
+}
+
+}  // namespace axs
+
+using namespace axs;
+
+struct axs_implicit_plan {
+  ImplicitPlan impl;
+};
+
+extern "C" {
+
+axs_status axs_implicit_diffusion_plan_create(int nr, int nz, double dx, double nu_dt,
+                                              axs_implicit_plan** plan_out) {
+  AXS_CHECK_ARG(plan_out, "null plan_out");
+  *plan_out = nullptr;
+  AXS_CHECK_ARG(nr >= 3 && nz >= 2, "need nr >= 3 and nz >= 2");
+  AXS_CHECK_ARG(nz <= 4096 && nr <= 4096, "grid too large for the row/column passes");
+  AXS_CHECK_ARG(dx > 0.0, "dx must be positive");
+  AXS_CHECK_ARG(nu_dt >= 0.0 && std::isfinite(nu_dt), "nu * dt must be finite and >= 0");
+  axs_implicit_plan* p = new (std::nothrow) axs_implicit_plan();
+  if (!p) {
+    set_error("out of host memory");
+    return AXS_ERR_ALLOC;
+  }
+  hipError_t e = p->impl.init(nr, nz, dx, nu_dt);
   if (e != hipSuccess) {
     set_error(std::string("implicit diffusion plan: ") + hipGetErrorString(e));
     delete p;
@@ -185,11 +202,7 @@ axs_status axs_implicit_diffusion_plan_destroy(axs_implicit_plan* plan) {
 axs_status axs_implicit_diffusion_step_f64(axs_implicit_plan* plan, double* omega,
                                            void* stream) {
   AXS_CHECK_ARG(plan && omega, "null pointer");
-  StokesPlan& pl = plan->tri;
-  hipStream_t s = as_stream(stream);
-  AXS_HIP(launch_dst1_rows(omega, pl.spec, plan->S, pl.nr, pl.nz, s));
-  AXS_HIP(launch_tridiag<double>(pl, pl.spec, s));
-  AXS_HIP(launch_dst1_rows(pl.spec, omega, plan->S, pl.nr, pl.nz, s));
+  AXS_HIP(plan->impl.launch(omega, 1, as_stream(stream)));
   return AXS_OK;
 }
 

@@ -31,6 +31,7 @@
 // radial tridiagonal elimination, the mask level set, and all reductions
 // and step scalars stay FP64.
 #include "axs_fused.hpp"
+#include "axs_implicit.hpp"
 #include "axs_stepper.hpp"
 
 #include <algorithm>
@@ -84,8 +85,8 @@ __global__ void init_scalars_kernel(double* __restrict__ sc, double* __restrict_
 // ------------------------------------------------------------------ stepper
 
 const char* const kRigidStageNames[kRigidStages] = {
-    "taper_dct2_rows", "tridiag_columns", "dct3_rows",
-    "velocity_max",    "body_mask",       "step_fused"};
+    "taper_dct2_rows", "tridiag_columns", "dct3_rows",         "velocity_max",
+    "body_mask",       "step_fused",      "implicit_diffusion"};
 
 template <typename T>
 __global__ void copy_field_kernel(T* __restrict__ dst, const T* __restrict__ src, int64_t n) {
@@ -105,6 +106,7 @@ RigidStepperBase::~RigidStepperBase() {
   if (spec) (void)hipFree(spec);
   if (dcfg) (void)hipFree(dcfg);
   delete plan;
+  delete implicit;
 }
 
 hipError_t RigidStepperBase::init(const RigidConfig* cfgs, int nbatch, const RigidBuffers& b) {
@@ -126,6 +128,13 @@ hipError_t RigidStepperBase::init(const RigidConfig* cfgs, int nbatch, const Rig
   plan->fft = fp;
   hipError_t e = plan->init(cfg.nr, cfg.nz, cfg.dx);
   if (e != hipSuccess) return e;
+  if (cfg.implicit_diffusion) {
+    // implicit_diffusion_solver.py:16: nu_times_dt = time_step * kinematic_viscosity
+    implicit = new (std::nothrow) ImplicitPlan();
+    if (!implicit) return hipErrorOutOfMemory;
+    e = implicit->init(cfg.nr, cfg.nz, cfg.dx, cfg.dt_diffusive * cfg.nu, nbatch);
+    if (e != hipSuccess) return e;
+  }
   blocks_x = ceil_div(cfg.nz, 256);
   npartials = fused_blocks(cfg.nr, cfg.nz);
   const size_t cells = (size_t)cfg.nr * cfg.nz * batch;
@@ -147,6 +156,7 @@ hipError_t RigidStepperBase::init(const RigidConfig* cfgs, int nbatch, const Rig
 
 bool RigidStepperBase::stage_enabled(int stage) const {
   if (stage == 4) return cfg.moving_body != 0;
+  if (stage == 6) return cfg.implicit_diffusion != 0;
   return stage >= 0 && stage < kRigidStages;
 }
 
@@ -190,7 +200,14 @@ hipError_t RigidStepper<T>::enqueue_stage(int stage, hipStream_t s) {
       return launch_step_fused<T>(psi, chi, cur, nxt,
                                   cfg.average_psi ? static_cast<T*>(buf.avg_psi) : nullptr,
                                   buf.r, buf.kill_wz, buf.kill_wr, w, nr, nz, T(cfg.dx), sc, slots,
-                                  partials, dcfg, B, s);
+                                  partials, dcfg, B, s, cfg.implicit_diffusion == 0);
+    case 6:
+      // implicit Euler diffusion of the step's new vorticity, in place
+      // (flow_past_sphere.py:179-181); FP64 only (checked at creation)
+      if constexpr (sizeof(T) == 8)
+        return implicit->launch(reinterpret_cast<double*>(nxt), B, s);
+      else
+        return hipErrorInvalidValue;
     default:
       return hipErrorInvalidValue;
   }
@@ -685,6 +702,7 @@ static RigidConfig to_internal(const axs_rigid_config& x) {
   c.precision = x.precision;
   c.moving_body = x.moving_body;
   c.average_psi = x.average_psi;
+  c.implicit_diffusion = x.implicit_diffusion;
   c.dx = x.dx;
   c.nu = x.nu;
   c.brink_lam = x.brink_lam;
@@ -745,6 +763,15 @@ axs_status axs_rigid_batch_create(const axs_rigid_config* cfgs, int batch,
     for (int q = 0; q < batch; ++q)
       AXS_CHECK_ARG(cfgs[q].nz <= 1024 && cfgs[q].nr <= 4096,
                     "a batch of uneven grids takes nz <= 1024 and nr <= 4096");
+  }
+  if (c0.implicit_diffusion) {
+    AXS_CHECK_ARG(!uneven && c0.precision == AXS_F64,
+                  "implicit diffusion runs in the FP64 one-grid stepper");
+    for (int q = 0; q < batch; ++q)
+      AXS_CHECK_ARG(cfgs[q].implicit_diffusion &&
+                        cfgs[q].dt_diffusive * cfgs[q].nu == c0.dt_diffusive * c0.nu &&
+                        cfgs[q].dt_diffusive > 0.0,
+                    "implicit diffusion: every case of a batch needs the same nu * dt > 0");
   }
   AXS_CHECK_ARG(bufs->vorticity && bufs->psi && bufs->u_pen_z && bufs->u_pen_r &&
                     bufs->char_func && bufs->work0 && bufs->scalars && bufs->r && bufs->kill_wz &&

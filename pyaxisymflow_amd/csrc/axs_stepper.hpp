@@ -34,7 +34,7 @@ static_assert(kScCount <= AXS_RIGID_SCALARS, "scalar slots exceed the documented
 
 struct RigidConfig {
   int nr, nz, kill_width, history;
-  int precision, moving_body, average_psi, pad;
+  int precision, moving_body, average_psi, implicit_diffusion;
   double dx, nu, brink_lam, U0, T_ramp, CFL, eps, dt_diffusive;
   double osc_amp, osc_omega, drag_scale, drag_div;
   double dt_cap, period, t_end;
@@ -51,18 +51,21 @@ struct RigidBuffers {
   const double* kill_wr;
 };
 
-constexpr int kRigidStages = 6;
+constexpr int kRigidStages = 7;
 extern const char* const kRigidStageNames[kRigidStages];
 
 // `batch` cases of one (nr, nz) grid stepped together; every field buffer
 // holds the cases back to back, and each case has its own configuration
 // (viscosity, body, ramp, dt rule, force scale) and device scalars.
+struct ImplicitPlan;
+
 struct RigidStepperBase {
   RigidConfig cfg{};                // case 0 (grid-level fields are shared)
   RigidConfig* dcfg = nullptr;      // per-case configurations, device
   int batch = 1;
   RigidBuffers buf{};
   StokesPlan* plan = nullptr;
+  ImplicitPlan* implicit = nullptr;  // implicit_diffusion: the diffusion solve after step_fused
   void* spec = nullptr;        // spectral work buffer, batch * nr * nz doubles
   double* partials = nullptr;  // force partial sums of the fused blocks
   double* slots = nullptr;     // velocity maximum slots

@@ -50,15 +50,20 @@ def kill_weights(z, r, dx, width=3):
 
 _CASE_KEYS = ("nu", "brink_lam", "char_func", "U_0", "T_ramp", "r_sph", "CFL", "vorticity",
               "oscillation", "moving_body", "average_psi", "dt_cap", "period", "t_end",
-              "force_scale")
+              "force_scale", "implicit_diffusion")
 
 
 def _case_config(nr, nz, dx, kill_width, history, precision, nu, brink_lam, char_func=None,
                  U_0=1.0, T_ramp=None, r_sph=None, CFL=0.1, vorticity=None, oscillation=None,
                  moving_body=None, average_psi=False, dt_cap=None, period=None, t_end=None,
-                 force_scale=None):
+                 force_scale=None, implicit_diffusion=None):
     """axs_rigid_config of one case (flow_past_sphere.py:39-56,
-    spheroid_streaming.py:40-112 for the streaming terms)."""
+    spheroid_streaming.py:40-112 for the streaming terms).
+
+    ``implicit_diffusion``: None or False for the RK2 diffusion; True for
+    the implicit Euler diffusion at flow_past_sphere.py's fixed step
+    dx^2 / 4 / nu (:90, 146-148); a number for that fixed step instead
+    (spheroid_streaming.py's dt_limit, :104-112, 190-191)."""
     if char_func is None and moving_body is None:
         raise ValueError("need char_func or moving_body")
     cfg = _Config()
@@ -68,6 +73,13 @@ def _case_config(nr, nz, dx, kill_width, history, precision, nu, brink_lam, char
     cfg.CFL = CFL
     cfg.eps = np.finfo(float).eps
     cfg.dt_diffusive = 0.9 * dx**2 / 4 / nu  # flow_past_sphere.py:151
+    if implicit_diffusion is not None and implicit_diffusion is not False:
+        if precision != "f64":
+            raise NotImplementedError("implicit diffusion runs in the FP64 stepper")
+        cfg.implicit_diffusion = 1
+        # flow_past_sphere.py:90 (diffusion_dt_limit) or the caller's fixed step
+        cfg.dt_diffusive = (dx**2 / 4 / nu if implicit_diffusion is True
+                            else float(implicit_diffusion))
     cfg.dt_cap = np.inf if dt_cap is None else dt_cap
     cfg.period = 0.0 if period is None else period
     cfg.t_end = 0.0 if t_end is None else t_end
@@ -440,16 +452,23 @@ class RigidBodyStepper(RigidBatchStepper):
 
     ``precision="f32"`` is the precision form: fields, stencils and
     z-transforms in FP32; radial solve, mask level set and reductions FP64.
+
+    ``implicit_diffusion=True`` replaces the RK2 diffusion by the implicit
+    Euler step of kernels/implicit_diffusion_solver.py at the fixed
+    dt = dx^2 / 4 / nu (flow_past_sphere.py:90-98, 146-148, 179-181); a
+    number sets that fixed dt instead.  FP64 only.
     """
 
     def __init__(self, nz, nr, z_len, nu, brink_lam, char_func=None, U_0=1.0, T_ramp=None,
                  r_sph=None, CFL=0.1, vorticity=None, oscillation=None, history=1 << 16,
                  kill_width=3, r_len=None, use_graph=True, precision="f64", moving_body=None,
-                 average_psi=False, dt_cap=None, period=None, t_end=None, force_scale=None):
+                 average_psi=False, dt_cap=None, period=None, t_end=None, force_scale=None,
+                 implicit_diffusion=None):
         case = dict(nu=nu, brink_lam=brink_lam, char_func=char_func, U_0=U_0, T_ramp=T_ramp,
                     r_sph=r_sph, CFL=CFL, vorticity=vorticity, oscillation=oscillation,
                     moving_body=moving_body, average_psi=average_psi, dt_cap=dt_cap,
-                    period=period, t_end=t_end, force_scale=force_scale)
+                    period=period, t_end=t_end, force_scale=force_scale,
+                    implicit_diffusion=implicit_diffusion)
         super().__init__(nz, nr, z_len, [case], kill_width=kill_width, r_len=r_len,
                          use_graph=use_graph, precision=precision, history=history)
 

@@ -80,3 +80,52 @@ def test_implicit_diffusion_torch_in_place_large():
     res = O.implicit_diffusion_residual(out, w, nu * dt, dx)
     print(f"implicit 2048x1024 residual {res:.2e}")
     assert res < 1e-12
+
+
+def implicit_sphere_case(nz=256, nr=128, z_len=2.0, nu=0.02):
+    """Flow past a sphere with the implicit-diffusion branch of
+    flow_past_sphere.py (:90-98, 146-148, 179-181): fixed dt = dx^2 / 4 / nu;
+    nu = 0.02 (Re = 7.5) keeps that dt under the advective limit."""
+    from oracle.drivers import gaussian_vorticity, grid
+    from oracle.kernels import smooth_Heaviside
+
+    dx, z, r, Z, R = grid(nz, nr, z_len)
+    phi = 0.075 - np.sqrt((Z - 0.5) ** 2 + R**2)
+    chi = np.zeros_like(phi)
+    smooth_Heaviside(chi, phi, dx * 2**0.5)
+    w0 = gaussian_vorticity(Z, R, 16, 0, 0.02, 0.1, z_len, nr * dx)
+    return (nz, nr, z_len, nu, 1e4, chi), dict(U_0=1.0, T_ramp=1.5, r_sph=0.075, vorticity=w0)
+
+
+@pytest.mark.parametrize("solver", ["exact", "eig"])
+def test_stepper_implicit_diffusion_matches_oracle_driver(solver):
+    from oracle.drivers import RigidBodyCase
+    from pyaxisymflow_amd.stepper import RigidBodyStepper
+
+    args, kw = implicit_sphere_case()
+    steps = 30
+    cpu = RigidBodyCase(*args, solver=solver, implicit_diffusion=True, **kw)
+    for _ in range(steps):
+        cpu.step()
+    gpu = RigidBodyStepper(*args, implicit_diffusion=True, **kw)
+    gpu.step(steps)
+    e_w = rel_l2(gpu.vorticity(), cpu.vorticity)
+    cd = gpu.drag_history()
+    e_cd = np.max(np.abs(cd[:steps] - np.array(cpu.drag))) / np.max(np.abs(cpu.drag))
+    print(f"implicit stepper ({solver}): omega {e_w:.2e}, Cd {e_cd:.2e}, dt {cpu.dts[0]:.3e}")
+    assert abs(gpu.time() - cpu.t) <= 1e-13 * cpu.t
+    assert e_w < 1e-10
+    assert e_cd < 1e-10
+
+
+def test_stepper_implicit_graph_replay_matches_eager():
+    from pyaxisymflow_amd.stepper import RigidBodyStepper
+
+    args, kw = implicit_sphere_case(nz=128, nr=64)
+    a = RigidBodyStepper(*args, implicit_diffusion=True, use_graph=True, **kw)
+    b = RigidBodyStepper(*args, implicit_diffusion=True, use_graph=False, **kw)
+    for n in (16, 3, 16):
+        a.step(n)
+        b.step(n)
+        assert np.array_equal(a.vorticity(), b.vorticity())
+    assert np.array_equal(a.drag_history()[:35], b.drag_history()[:35])
