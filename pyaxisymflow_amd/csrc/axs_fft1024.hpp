@@ -236,9 +236,10 @@ __device__ __forceinline__ void fft1024_wave(Cplx<T>* buf, const Cplx<double>* _
 //   pass 2  lane (k1, m), k1 < R1: DFT-8 over c, times W_64^(m k2a)
 //                                                               -> E[k2a][m][k1]
 //   pass 3  lane (k2a, k1), k1 < R1: DFT-8 over m     -> X[k1 + R1 (k2a + 8 k2b)]
-// (lanes with k1 >= R1 idle in passes 2 and 3).  tw[t] = e^{-2 pi i t / M}.
-// Natural order in and out of buf.
-template <typename T, int SGN, int R1>
+// (lanes with k1 >= R1 idle in passes 2 and 3).  tw[TS t] = e^{-2 pi i t / M}
+// (TS = 2: the table of a plan of twice the length).  Natural order in and
+// out of buf.
+template <typename T, int SGN, int R1, int TS = 1>
 __device__ __forceinline__ void fft_r64_wave(Cplx<T>* buf, const Cplx<double>* __restrict__ tw) {
   static_assert(R1 == 2 || R1 == 3 || R1 == 4 || R1 == 6 || R1 == 8, "first radix");
   const int l = threadIdx.x & 63;
@@ -246,7 +247,7 @@ __device__ __forceinline__ void fft_r64_wave(Cplx<T>* buf, const Cplx<double>* _
     Cplx<T> v[R1];
     Cplx<double> t1[R1 - 1];
 #pragma unroll
-    for (int k1 = 1; k1 < R1; ++k1) t1[k1 - 1] = twid<SGN>(tw, l * k1);
+    for (int k1 = 1; k1 < R1; ++k1) t1[k1 - 1] = twid<SGN>(tw, TS * l * k1);
 #pragma unroll
     for (int n1 = 0; n1 < R1; ++n1) v[n1] = buf[64 * n1 + l];
     dft_r<T, SGN, R1>(v);
@@ -265,7 +266,7 @@ __device__ __forceinline__ void fft_r64_wave(Cplx<T>* buf, const Cplx<double>* _
     if (k1 < R1) {
       Cplx<double> t2[7];
 #pragma unroll
-      for (int k2a = 1; k2a < 8; ++k2a) t2[k2a - 1] = twid<SGN>(tw, R1 * m * k2a);
+      for (int k2a = 1; k2a < 8; ++k2a) t2[k2a - 1] = twid<SGN>(tw, TS * R1 * m * k2a);
 #pragma unroll
       for (int c = 0; c < 8; ++c) v[c] = buf[k1 * kY512 + m + 8 * c];
       dft8<T, SGN>(v);
@@ -297,13 +298,14 @@ __device__ __forceinline__ void fft_r64_wave(Cplx<T>* buf, const Cplx<double>* _
   __syncthreads();
 }
 
-// The one-wave transform of M = 128, 192, 256, 384, 512 or 1024 points.
-template <typename T, int SGN, int M>
+// The one-wave transform of M = 128, 192, 256, 384, 512 or 1024 points
+// (twiddle table stride TS as above).
+template <typename T, int SGN, int M, int TS = 1>
 __device__ __forceinline__ void fft_wave(Cplx<T>* buf, const Cplx<double>* __restrict__ tw) {
   if constexpr (M == kF1024M)
-    fft1024_wave<T, SGN>(buf, tw);
+    fft1024_wave<T, SGN, TS>(buf, tw);
   else
-    fft_r64_wave<T, SGN, M / 64>(buf, tw);
+    fft_r64_wave<T, SGN, M / 64, TS>(buf, tw);
 }
 
 }  // namespace axs
@@ -371,50 +373,55 @@ __device__ __forceinline__ void dct2_post_pairs(const Cplx<T>* Z, const Cplx<dou
 
 namespace axs {
 
-// nz = 4096 (M = 2048 = 2 x 1024): the packed sequence z splits into its even
-// and odd samples, each transformed by one wavefront with fft1024_wave (the
+// Two wavefronts per row, M = 2 HM (HM = 1024: nz = 4096; HM = 512:
+// nz = 2048): the packed sequence z splits into its even and odd samples,
+// each transformed by one wavefront with the one-wave HM-point FFT (the
 // first buffer holds z_{2p}, the second z_{2p+1}), joined by one radix-2 step
-//   Z_k = E_k + W_2048^k O_k,   Z_{k+1024} = E_k - W_2048^k O_k   (k < 1024)
-// written over E_k and O_k: Z_k for k < 1024 is at k of the first buffer,
-// for k >= 1024 at k - 1024 of the second.  The inverse joins with the
-// conjugate twiddle.
+//   Z_k = E_k + W_M^k O_k,   Z_{k+HM} = E_k - W_M^k O_k   (k < HM)
+// written over E_k and O_k: Z_k for k < HM is at k of the first buffer, for
+// k >= HM at k - HM of the second.  The inverse joins with the conjugate
+// twiddle.
 constexpr int kF2048M = 2048;
 
 // Buffer entry of packed complex sample m (before the FFTs).
-__device__ __forceinline__ int f2048_in_slot(int m) { return ((m & 1) ? kF1024Lds : 0) + (m >> 1); }
+template <int HM>
+__device__ __forceinline__ int split_in_slot(int m) {
+  return ((m & 1) ? kWaveLds<HM> : 0) + (m >> 1);
+}
 // Buffer entry of transform value k (after the join).
-__device__ __forceinline__ int f2048_out_slot(int k) {
-  return k < kF1024M ? k : kF1024Lds + (k - kF1024M);
+template <int HM>
+__device__ __forceinline__ int split_out_slot(int k) {
+  return k < HM ? k : kWaveLds<HM> + (k - HM);
 }
 
-// The join over bufs (two buffers of kF1024Lds entries), 128 threads, block
-// barrier after.
-template <typename T, int SGN>
-__device__ __forceinline__ void f2048_join(Cplx<T>* bufs, const Cplx<double>* __restrict__ tw) {
+// The join over bufs (two buffers of kWaveLds<HM> entries), 128 threads,
+// block barrier after.  tw[k] = e^{-2 pi i k / (2 HM)}.
+template <typename T, int SGN, int HM>
+__device__ __forceinline__ void split_join(Cplx<T>* bufs, const Cplx<double>* __restrict__ tw) {
   const int t = threadIdx.x;
-  Cplx<double> w[kF1024M / 128];
+  Cplx<double> w[HM / 128];
 #pragma unroll
-  for (int q = 0; q < kF1024M / 128; ++q) w[q] = twid<SGN>(tw, t + 128 * q);
+  for (int q = 0; q < HM / 128; ++q) w[q] = twid<SGN>(tw, t + 128 * q);
 #pragma unroll
-  for (int q = 0; q < kF1024M / 128; ++q) {
+  for (int q = 0; q < HM / 128; ++q) {
     const int k = t + 128 * q;
     const Cplx<T> e = bufs[k];
-    const Cplx<T> o = cmul(bufs[kF1024Lds + k], Cplx<T>{T(w[q].x), T(w[q].y)});
+    const Cplx<T> o = cmul(bufs[kWaveLds<HM> + k], Cplx<T>{T(w[q].x), T(w[q].y)});
     bufs[k] = cadd(e, o);
-    bufs[kF1024Lds + k] = csub(e, o);
+    bufs[kWaveLds<HM> + k] = csub(e, o);
   }
   __syncthreads();
 }
 
-// DCT-II post step for M = 2048 (as dct2_post_pairs), 128 threads: thread t
-// takes k = t + 128 q (q < 8) with its partner M - k; thread 0 also takes
-// k = M / 2.
-template <typename T>
-__device__ __forceinline__ void dct2_post_pairs2048(const Cplx<T>* bufs,
-                                                    const Cplx<double>* __restrict__ e1,
-                                                    const Cplx<double>* __restrict__ e2,
-                                                    T* __restrict__ out) {
-  constexpr int M = kF2048M, N = 2 * kF2048M, KP = M / 2 / 128;
+// DCT-II post step for M = 2 HM (as dct2_post_pairs), 128 threads: thread t
+// takes k = t + 128 q (q < HM / 128) with its partner M - k; thread 0 also
+// takes k = M / 2.
+template <typename T, int HM>
+__device__ __forceinline__ void dct2_post_pairs_split(const Cplx<T>* bufs,
+                                                      const Cplx<double>* __restrict__ e1,
+                                                      const Cplx<double>* __restrict__ e2,
+                                                      T* __restrict__ out) {
+  constexpr int M = 2 * HM, N = 2 * M, KP = M / 2 / 128;
   const int t = threadIdx.x;
   Cplx<double> a1[KP], b1[KP], a2[KP], b2[KP];
 #pragma unroll
@@ -429,7 +436,7 @@ __device__ __forceinline__ void dct2_post_pairs2048(const Cplx<T>* bufs,
 #pragma unroll
   for (int q = 0; q < KP; ++q) {
     const int k = t + 128 * q, kp = M - k;
-    const Cplx<T> zk = bufs[f2048_out_slot(k)], zp = bufs[f2048_out_slot(kp & (M - 1))];
+    const Cplx<T> zk = bufs[split_out_slot<HM>(k)], zp = bufs[split_out_slot<HM>(kp & (M - 1))];
     const Cplx<T> sk = cadd(zk, cconj(zp)), dk = csub(zk, cconj(zp));
     const Cplx<T> wdk = cmul(Cplx<T>{T(a1[q].x), T(a1[q].y)}, dk);
     const Cplx<T> Vk = {T(0.5) * (sk.x + wdk.y), T(0.5) * (sk.y - wdk.x)};
@@ -447,7 +454,7 @@ __device__ __forceinline__ void dct2_post_pairs2048(const Cplx<T>* bufs,
   }
   if (t == 0) {
     constexpr int k = M / 2;
-    const Cplx<T> zk = bufs[f2048_out_slot(k)];
+    const Cplx<T> zk = bufs[split_out_slot<HM>(k)];
     const Cplx<T> sk = cadd(zk, cconj(zk)), dk = csub(zk, cconj(zk));
     const Cplx<T> wdk = cmul(Cplx<T>{T(ah.x), T(ah.y)}, dk);
     const Cplx<T> Vk = {T(0.5) * (sk.x + wdk.y), T(0.5) * (sk.y - wdk.x)};

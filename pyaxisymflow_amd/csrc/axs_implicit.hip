@@ -21,68 +21,94 @@
 #include "axs_implicit.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <new>
 #include <vector>
 
 namespace axs {
 
 // out[i, k] = sum_j in[i, j] S[j, k] for rows i < rows (S symmetric, nz x
-// nz).  Workgroup: a 32-row x 64-column output tile, 256 threads, each
-// thread 2 rows x 4 columns (columns tx + 16 c, rows ty and ty + 16);
-// j advances in chunks of 16 staged in LDS.  Tiles that reach past the
-// grid read zeros there and skip those stores.
-constexpr int kDstBM = 32, kDstBN = 64, kDstBK = 16;
+// nz).  Workgroup: a BM-row x 64-column output tile, 256 threads, each
+// thread BM / 16 rows x 4 columns (rows ty + 16 h, columns tx + 16 c); j
+// advances in chunks of 16 staged in LDS, two LDS buffers so the global
+// loads of the next chunk are in flight while this one is multiplied (one
+// barrier per chunk).  Tiles that reach past the grid read zeros there and
+// skip those stores.
+constexpr int kDstBN = 64, kDstBK = 16;
 
-__global__ void __launch_bounds__(256) dst1_rows_kernel(const double* __restrict__ in,
+template <int BM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) dst1_rows_kernel(const double* __restrict__ in,
                                                          double* __restrict__ out,
                                                          const double* __restrict__ S, int rows,
                                                          int nz) {
-  __shared__ double sA[kDstBK][kDstBM + 1];
-  __shared__ double sB[kDstBK][kDstBN];
+  constexpr int RH = BM / 16;                 // rows per thread
+  constexpr int NA = BM * kDstBK / 256;       // A elements per thread per chunk
+  constexpr int NB = kDstBK * kDstBN / 256;   // B elements per thread per chunk
+  __shared__ double sA[2][kDstBK][BM + 1];
+  __shared__ double sB[2][kDstBK][kDstBN];
   const int t = threadIdx.x;
   const int tx = t & 15, ty = t >> 4;
   const int col0 = blockIdx.x * kDstBN;
-  const int row0 = blockIdx.y * kDstBM;
-  double acc[2][4] = {};
-  for (int k0 = 0; k0 < nz; k0 += kDstBK) {
-    double va[2], vb[4];
+  const int row0 = blockIdx.y * BM;
+  double acc[RH][4] = {};
+  double va[NA], vb[NB];
+  auto load = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {  // A: 32 x 16, element e = t + 256 q
+    for (int q = 0; q < NA; ++q) {  // A: BM x 16, element e = t + 256 q
       const int e = t + 256 * q, r = e >> 4, kk = e & 15;
       const int gi = row0 + r, gj = k0 + kk;
       va[q] = (gi < rows && gj < nz) ? in[(int64_t)gi * nz + gj] : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {  // B: 16 x 64, element e = t + 256 q
+    for (int q = 0; q < NB; ++q) {  // B: 16 x 64, element e = t + 256 q
       const int e = t + 256 * q, kk = e >> 6, cc = e & 63;
       const int gj = k0 + kk, gk = col0 + cc;
       vb[q] = (gj < nz && gk < nz) ? S[(int64_t)gj * nz + gk] : 0.0;
     }
-    __syncthreads();  // the previous chunk's reads are done
+  };
+  auto stage = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < NA; ++q) {
       const int e = t + 256 * q;
-      sA[e & 15][e >> 4] = va[q];
+      sA[buf][e & 15][e >> 4] = va[q];
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NB; ++q) {
       const int e = t + 256 * q;
-      sB[e >> 6][e & 63] = vb[q];
+      sB[buf][e >> 6][e & 63] = vb[q];
     }
-    __syncthreads();
-#pragma unroll
+  };
+  load(0);
+  stage(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < nz; k0 += kDstBK) {
+    const bool more = k0 + kDstBK < nz;
+    if (more) load(k0 + kDstBK);
+    const double(*pa)[BM + 1] = sA[buf];
+    const double(*pb)[kDstBN] = sB[buf];
+#pragma unroll 4
     for (int kk = 0; kk < kDstBK; ++kk) {
-      const double a0 = sA[kk][ty], a1 = sA[kk][ty + 16];
+      double a[RH], bv[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const double bv = sB[kk][tx + 16 * c];
-        acc[0][c] = fma(a0, bv, acc[0][c]);
-        acc[1][c] = fma(a1, bv, acc[1][c]);
-      }
+      for (int h = 0; h < RH; ++h) a[h] = pa[kk][ty + 16 * h];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bv[c] = pb[kk][tx + 16 * c];
+#pragma unroll
+      for (int h = 0; h < RH; ++h)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[h][c] = fma(a[h], bv[c], acc[h][c]);
+    }
+    if (more) {
+      // the other buffer was last read in the previous chunk, before the
+      // barrier that ended it
+      stage(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
     }
   }
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < RH; ++h) {
     const int gi = row0 + ty + 16 * h;
     if (gi >= rows) continue;
 #pragma unroll
@@ -93,10 +119,25 @@ __global__ void __launch_bounds__(256) dst1_rows_kernel(const double* __restrict
   }
 }
 
+// Tile height: 64 rows when that still gives every CU a workgroup, else 32
+// (AXS_DST_BM = 32 or 64 pins it).
+static int dst_tile_rows(int rows, int nz) {
+  static const int pin = [] {
+    const char* v = std::getenv("AXS_DST_BM");
+    return v ? std::atoi(v) : 0;
+  }();
+  if (pin == 32 || pin == 64) return pin;
+  return (int64_t)ceil_div(nz, kDstBN) * ceil_div(rows, 64) >= 256 ? 64 : 32;
+}
+
 static hipError_t launch_dst1_rows(const double* in, double* out, const double* S, int rows,
                                    int nz, hipStream_t s) {
-  const dim3 grid(ceil_div(nz, kDstBN), ceil_div(rows, kDstBM));
-  dst1_rows_kernel<<<grid, 256, 0, s>>>(in, out, S, rows, nz);
+  if (dst_tile_rows(rows, nz) == 64)
+    dst1_rows_kernel<64><<<dim3(ceil_div(nz, kDstBN), ceil_div(rows, 64)), 256, 0, s>>>(
+        in, out, S, rows, nz);
+  else
+    dst1_rows_kernel<32><<<dim3(ceil_div(nz, kDstBN), ceil_div(rows, 32)), 256, 0, s>>>(
+        in, out, S, rows, nz);
   return hipGetLastError();
 }
 

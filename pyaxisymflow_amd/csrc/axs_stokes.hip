@@ -663,15 +663,17 @@ __global__ void __launch_bounds__(64) dct3_wave_ragged_kernel(
     dct3_row_wave<T, 128>(src + cs, dst + cs, G.tw, G.e1, G.e2, i, buf);
 }
 
-// nz = 4096 row passes: two wavefronts per row (axs_fft1024.hpp, M = 2048).
-// Same inputs, outputs and taper as the kernels above.
-template <typename T, bool TAPER>
-__global__ void __launch_bounds__(128) dct2_rows_m2048_kernel(
+// Two wavefronts per row (axs_fft1024.hpp, M = 2 HM): nz = 4096 (HM = 1024).
+// Same inputs, outputs and taper as the kernels above.  (HM = 512 would
+// serve nz = 2048 with two waves per row; it measured slower there than the
+// one-wave kernels: 12.3 vs 11.3 us forward, 10.5 vs 9.9 us inverse.)
+template <typename T, bool TAPER, int HM>
+__global__ void __launch_bounds__(128) dct2_rows_split_kernel(
     const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
     const double* __restrict__ rscale, const double* __restrict__ wz,
     const double* __restrict__ wr, int nr, int w, const Cplx<double>* __restrict__ tw,
     const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2, StepFinalize fin) {
-  constexpr int N = 2 * kF2048M;
+  constexpr int N = 4 * HM;
   if (fin.partials && blockIdx.x == nr) {  // completes the previous step
     __shared__ double fin_red[256];
     const int cb = blockIdx.y;
@@ -681,7 +683,7 @@ __global__ void __launch_bounds__(128) dct2_rows_m2048_kernel(
                      fin.drag ? fin.drag + (int64_t)cb * cfg.history : nullptr, cfg, fin_red);
     return;
   }
-  __shared__ Cplx<T> bufs[2 * kF1024Lds];
+  __shared__ Cplx<T> bufs[2 * kWaveLds<HM>];
   const int t = threadIdx.x;
   const int i = blockIdx.x;
   const int64_t cs = (int64_t)blockIdx.y * nr * N;  // case of a batch
@@ -717,23 +719,23 @@ __global__ void __launch_bounds__(128) dct2_rows_m2048_kernel(
       if (out_t) out_t[j] = x;
     }
     const int d = dct_dst_index(j, N);  // real slot of the packed sequence
-    zr[2 * f2048_in_slot(d >> 1) + (d & 1)] = rs * x;
+    zr[2 * split_in_slot<HM>(d >> 1) + (d & 1)] = rs * x;
   }
   __syncthreads();
-  fft1024_wave<T, -1, 2>(bufs + (t >> 6) * kF1024Lds, tw);
-  f2048_join<T, -1>(bufs, tw);
-  dct2_post_pairs2048<T>(bufs, e1, e2, dst + (int64_t)i * N);
+  fft_wave<T, -1, HM, 2>(bufs + (t >> 6) * kWaveLds<HM>, tw);
+  split_join<T, -1, HM>(bufs, tw);
+  dct2_post_pairs_split<T, HM>(bufs, e1, e2, dst + (int64_t)i * N);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(128) dct3_rows_m2048_kernel(
+template <typename T, int HM>
+__global__ void __launch_bounds__(128) dct3_rows_split_kernel(
     const T* __restrict__ src, T* __restrict__ dst, const Cplx<double>* __restrict__ tw,
     const Cplx<double>* __restrict__ e1, const Cplx<double>* __restrict__ e2) {
-  constexpr int M = kF2048M, N = 2 * kF2048M;
+  constexpr int M = 2 * HM, N = 4 * HM;
   constexpr int KP = M / 2 / 128;  // index pairs (k, M - k) per thread
   typedef T T2 __attribute__((ext_vector_type(2)));
-  __shared__ Cplx<T> bufs[2 * kF1024Lds];
-  static_assert(2 * 2 * kF1024Lds >= N, "the staged row fits the two buffers");
+  __shared__ Cplx<T> bufs[2 * kWaveLds<HM>];
+  static_assert(2 * 2 * kWaveLds<HM> >= N, "the staged row fits the two buffers");
   T* raw = reinterpret_cast<T*>(bufs);  // the input row, staged before the pre step
   const int t = threadIdx.x;
   const int i = blockIdx.x;
@@ -775,23 +777,23 @@ __global__ void __launch_bounds__(128) dct3_rows_m2048_kernel(
     {
       const Cplx<T> A = cadd(Vk, cconj(Vm)), D = csub(Vk, cconj(Vm));
       const Cplx<T> B = cmul(D, Cplx<T>{T(q1k[q].x), T(-q1k[q].y)});
-      bufs[f2048_in_slot(k)] = Cplx<T>{A.x - B.y, A.y + B.x};
+      bufs[split_in_slot<HM>(k)] = Cplx<T>{A.x - B.y, A.y + B.x};
     }
     if (k != 0) {
       const Cplx<T> A = cadd(Vm, cconj(Vk)), D = csub(Vm, cconj(Vk));
       const Cplx<T> B = cmul(D, Cplx<T>{T(q1m[q].x), T(-q1m[q].y)});
-      bufs[f2048_in_slot(M - k)] = Cplx<T>{A.x - B.y, A.y + B.x};
+      bufs[split_in_slot<HM>(M - k)] = Cplx<T>{A.x - B.y, A.y + B.x};
     }
   }
   if (t == 0) {  // k = M / 2 is its own partner
     const Cplx<T> Vk = cmul(Cplx<T>{T(q2h.x), T(-q2h.y)}, Cplx<T>{xh, -xnh});
     const Cplx<T> A = cadd(Vk, cconj(Vk)), D = csub(Vk, cconj(Vk));
     const Cplx<T> B = cmul(D, Cplx<T>{T(q1h.x), T(-q1h.y)});
-    bufs[f2048_in_slot(M / 2)] = Cplx<T>{A.x - B.y, A.y + B.x};
+    bufs[split_in_slot<HM>(M / 2)] = Cplx<T>{A.x - B.y, A.y + B.x};
   }
   __syncthreads();
-  fft1024_wave<T, +1, 2>(bufs + (t >> 6) * kF1024Lds, tw);
-  f2048_join<T, +1>(bufs, tw);
+  fft_wave<T, +1, HM, 2>(bufs + (t >> 6) * kWaveLds<HM>, tw);
+  split_join<T, +1, HM>(bufs, tw);
   const T invN = T(1) / T(N);
   T* out = dst + cs + (int64_t)i * N;
   const T* zr = reinterpret_cast<const T*>(bufs);
@@ -799,7 +801,7 @@ __global__ void __launch_bounds__(128) dct3_rows_m2048_kernel(
   for (int q = 0; q < N / 128; ++q) {
     const int j = t + 128 * q;
     const int d = dct_dst_index(j, N);
-    out[j] = zr[2 * f2048_out_slot(d >> 1) + (d & 1)] * invN;
+    out[j] = zr[2 * split_out_slot<HM>(d >> 1) + (d & 1)] * invN;
   }
 }
 
@@ -1097,9 +1099,8 @@ hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const do
     return hipGetLastError();
   }
   if (M == kF2048M) {
-    dct2_rows_m2048_kernel<T, false><<<rows, 128, 0, s>>>(src, nullptr, dst, rscale, nullptr,
-                                                           nullptr, rows, 0, pl.tw, pl.e1, pl.e2,
-                                                           StepFinalize{});
+    dct2_rows_split_kernel<T, false, 1024><<<rows, 128, 0, s>>>(
+        src, nullptr, dst, rscale, nullptr, nullptr, rows, 0, pl.tw, pl.e1, pl.e2, StepFinalize{});
     return hipGetLastError();
   }
   if (per <= 1)
@@ -1137,7 +1138,7 @@ hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered
     return hipGetLastError();
   }
   if (M == kF2048M) {
-    dct2_rows_m2048_kernel<T, true><<<dim3(gx, batch), 128, 0, s>>>(
+    dct2_rows_split_kernel<T, true, 1024><<<dim3(gx, batch), 128, 0, s>>>(
         src, tapered, dst, pl.rvec, wz, wr, pl.nr, w, pl.tw, pl.e1, pl.e2, f);
     return hipGetLastError();
   }
@@ -1177,7 +1178,7 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
     return hipGetLastError();
   }
   if (M == kF2048M) {
-    dct3_rows_m2048_kernel<T><<<grid, 128, 0, s>>>(src, dst, pl.tw, pl.e1, pl.e2);
+    dct3_rows_split_kernel<T, 1024><<<grid, 128, 0, s>>>(src, dst, pl.tw, pl.e1, pl.e2);
     return hipGetLastError();
   }
   if (per <= 1)
