@@ -304,6 +304,42 @@ def time_implicit(sizes=((256, 128), (1024, 512), (2048, 1024)), reps=20):
                 note="DST-I rows as a dense product (fast sine transforms need nz+1-point FFTs)")
 
 
+def time_potential(sizes=((256, 128), (2048, 1024)), reps=20):
+    """Velocity potential (kernels/FastDiagonalisationPotentialSolver.py, the
+    drop-in solver) plus compute_velocity_from_phi on device-resident fields:
+    one solve = DCT-II rows, the mode-0 gauge solve, the radial solves,
+    DCT-III rows; 256 x 128 is the torus-streaming grid that uses it."""
+    import torch
+
+    from pyaxisymflow_amd.kernels.compute_velocity_from_phi import compute_velocity_from_phi_unb
+    from pyaxisymflow_amd.kernels.FastDiagonalisationPotentialSolver import (
+        FastDiagonalisationPotentialSolver,
+    )
+
+    res = {}
+    for nz, nr in sizes:
+        dx = 1.0 / nz
+        st = FastDiagonalisationPotentialSolver(nr, nz, dx)
+        rhs = torch.from_numpy(np.random.default_rng(0).standard_normal((nr, nz))).cuda()
+        phi, u, v = (torch.empty_like(rhs) for _ in range(3))
+        st.solve(phi, rhs)
+        compute_velocity_from_phi_unb(u, v, phi, dx)
+        torch.cuda.synchronize()
+        s = torch.cuda.current_stream()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            st.solve(phi, rhs)
+            compute_velocity_from_phi_unb(u, v, phi, dx)
+        e1.record(s)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        res[f"{nz}x{nr}"] = dict(us_per_solve_and_velocity=round(us, 2),
+                                 mcells_per_s=round(nr * nz / us, 2))
+    return dict(unit="us per potential solve + velocity (drop-in, fields resident)", sizes=res)
+
+
 def time_serving(n_requests, steps, max_active=16, seed=0):
     """The serving form: soft-sphere requests (pyaxisymflow_amd.serve:
     nr lognormal around 256 in [128, 1024], 5-smooth; G in [0.1, 10]).  A
@@ -462,6 +498,7 @@ def main():
         forms["batched_fp32"] = time_batched(args.batched_steps, 2)
     if not args.no_implicit:
         forms["implicit_diffusion"] = time_implicit()
+        forms["potential"] = time_potential()
     if not args.no_serve:
         forms["serving_soft"] = time_serving(args.serve_requests, args.serve_steps)
     if forms:

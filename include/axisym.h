@@ -139,6 +139,26 @@ axs_status axs_stokes_plan_destroy(axs_stokes_plan* plan);
 axs_status axs_stokes_solve_f64(axs_stokes_plan* plan, double* psi, const double* rhs,
                                 void* stream);
 
+/* ---- Velocity potential: kernels/FastDiagonalisationPotentialSolver.py:6-145
+ * (called by torus_streaming.py:176).  Solves A_r phi + phi P_z^T = rhs with
+ * P_z, P_r, D_r as built at :41-86 (Neumann z and r), by a DCT-II along z
+ * and one radial tridiagonal solve per z-mode.  Mode 0's radial system is
+ * singular (constants are in the null space); it is solved in the gauge
+ * phi_hat_0[nr - 1] = 0, where the reference divides by a round-off
+ * eigenvalue (:114-117).  phi therefore matches the reference up to an
+ * additive constant when rhs is compatible, and its gradient does not see
+ * the constant.  phi and rhs may alias. */
+typedef struct axs_potential_plan axs_potential_plan;
+axs_status axs_potential_plan_create(int nr, int nz, double dx, int bc_type,
+                                     axs_potential_plan** plan_out);
+axs_status axs_potential_plan_destroy(axs_potential_plan* plan);
+axs_status axs_potential_solve_f64(axs_potential_plan* plan, double* phi, const double* rhs,
+                                   void* stream);
+/* kernels/compute_velocity_from_phi.py:4-17 (compute_velocity_from_phi_unb):
+ * u_z = d phi / dz, u_r = d phi / dr, second-order one-sided at the edges. */
+axs_status axs_velocity_from_phi_f64(double* u_z, double* u_r, const double* phi, int nr, int nz,
+                                     double dx, void* stream);
+
 /* ---- Implicit Euler diffusion: kernels/implicit_diffusion_solver.py:6-143
  * (ImplicitEulerDiffusionStepper; called by flow_past_sphere.py:180-181,
  * spheroid_streaming.py:236-237, torus_streaming.py:182 when

@@ -95,6 +95,10 @@ SIGNATURES = {
     "axs_stokes_plan_create": [_I, _I, _D, _I, ctypes.POINTER(_P)],
     "axs_stokes_plan_destroy": [_P],
     "axs_stokes_solve_f64": [_P, _P, _P, _P],
+    "axs_potential_plan_create": [_I, _I, _D, _I, ctypes.POINTER(_P)],
+    "axs_potential_plan_destroy": [_P],
+    "axs_potential_solve_f64": [_P, _P, _P, _P],
+    "axs_velocity_from_phi_f64": [_P, _P, _P, _I, _I, _D, _P],
     "axs_implicit_diffusion_plan_create": [_I, _I, _D, _D, ctypes.POINTER(_P)],
     "axs_implicit_diffusion_plan_destroy": [_P],
     "axs_implicit_diffusion_step_f64": [_P, _P, _P],

@@ -59,7 +59,7 @@ int StokesPlan::tri_threads() const { return tri_K * tri_P; }
 
 // c'_{pL-1} of the forward elimination at every segment start p L, per mode
 // (the pivot recurrence depends only on the matrix, not on the data).
-static std::vector<double> pivot_table(const std::vector<double>& a, const std::vector<double>& b,
+std::vector<double> pivot_table(const std::vector<double>& a, const std::vector<double>& b,
                                        const std::vector<double>& c,
                                        const std::vector<double>& lam, int nr, int nz, int L) {
   const int P = (nr + L - 1) / L;
@@ -182,6 +182,31 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
 }
 
 hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
+  hipError_t e = init_transforms(nr_, nz_, dx_);
+  if (e != hipSuccess) return e;
+  const std::vector<double>& h_r = host_r;
+  // A_r = P_r - D_r (:45-65, Neumann BCs :79-86), entries as the reference forms them.
+  const double inv_dx2 = 1.0 / dx / dx;
+  const double inv_2dx = 1.0 / 2.0 / dx;
+  std::vector<double> h_a(nr, 0.0), h_b(nr), h_c(nr, 0.0);
+  for (int i = 0; i < nr; ++i) {
+    const double P_sub = inv_dx2 * -1.0, P_sup = inv_dx2 * -1.0;
+    const double D_sub = (inv_2dx * 1.0) / h_r[i], D_sup = (inv_2dx * -1.0) / h_r[i];
+    h_b[i] = inv_dx2 * 2.0;
+    if (i > 0) h_a[i] = P_sub - D_sub;
+    if (i < nr - 1) h_c[i] = P_sup - D_sup;
+  }
+  h_b[nr - 1] = inv_dx2;                 // P_r[-1,-1] = inv_dx2
+  if (nr > 1) h_a[nr - 1] = inv_dx2 * -1.0 - 0.0;  // D_r[-1,-2] = 0
+  std::vector<double> h_lam(nz);
+  for (int k = 0; k < nz; ++k) {
+    const double s = std::sin(M_PI * (double)k / (2.0 * nz));
+    h_lam[k] = (4.0 / (dx * dx)) * s * s;
+  }
+  return set_tridiag(h_a, h_b, h_c, h_lam);
+}
+
+hipError_t StokesPlan::init_transforms(int nr_, int nz_, double dx_) {
   nr = nr_;
   nz = nz_;
   dx = dx_;
@@ -205,30 +230,12 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
   const double step = (nr > 1) ? (stop - start) / (nr - 1) : 0.0;
   for (int i = 0; i < nr; ++i) h_r[i] = i * step + start;
   if (nr > 1) h_r[nr - 1] = stop;
-  // A_r = P_r - D_r (:45-65, Neumann BCs :79-86), entries as the reference forms them.
-  const double inv_dx2 = 1.0 / dx / dx;
-  const double inv_2dx = 1.0 / 2.0 / dx;
-  std::vector<double> h_a(nr, 0.0), h_b(nr), h_c(nr, 0.0);
-  for (int i = 0; i < nr; ++i) {
-    const double P_sub = inv_dx2 * -1.0, P_sup = inv_dx2 * -1.0;
-    const double D_sub = (inv_2dx * 1.0) / h_r[i], D_sup = (inv_2dx * -1.0) / h_r[i];
-    h_b[i] = inv_dx2 * 2.0;
-    if (i > 0) h_a[i] = P_sub - D_sub;
-    if (i < nr - 1) h_c[i] = P_sup - D_sup;
-  }
-  h_b[nr - 1] = inv_dx2;                 // P_r[-1,-1] = inv_dx2
-  if (nr > 1) h_a[nr - 1] = inv_dx2 * -1.0 - 0.0;  // D_r[-1,-2] = 0
-  std::vector<double> h_lam(nz);
-  for (int k = 0; k < nz; ++k) {
-    const double s = std::sin(M_PI * (double)k / (2.0 * nz));
-    h_lam[k] = (4.0 / (dx * dx)) * s * s;
-  }
+  host_r = h_r;
   hipError_t e;
   if ((e = upload(&tw, h_tw)) != hipSuccess) return e;
   if ((e = upload(&e1, h_e1)) != hipSuccess) return e;
   if ((e = upload(&e2, h_e2)) != hipSuccess) return e;
-  if ((e = upload(&rvec, h_r)) != hipSuccess) return e;
-  return set_tridiag(h_a, h_b, h_c, h_lam);
+  return upload(&rvec, h_r);
 }
 
 hipError_t StokesPlan::set_tridiag(const std::vector<double>& h_a, const std::vector<double>& h_b,
@@ -1189,6 +1196,24 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
     dct3_rows_kernel<T, 4><<<grid, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
   else
     dct3_rows_kernel<T, 8><<<grid, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1, pl.e2);
+  return hipGetLastError();
+}
+
+// One tridiagonal system of nr rows held contiguously in d (segments of 16
+// rows, one thread each, coefficients read from global memory).
+__global__ void __launch_bounds__(256) tridiag_one_kernel(
+    double* __restrict__ d, const double* __restrict__ a, const double* __restrict__ b,
+    const double* __restrict__ c, const double* __restrict__ lam,
+    const double* __restrict__ cseg, int nr, int P) {
+  tridiag_columns_body<double, 16, false, true>(d, a, b, c, lam, cseg, nr, 1, 1, P, 0);
+}
+
+hipError_t launch_tridiag_one(double* d, const double* a, const double* b, const double* c,
+                              const double* lam, const double* cseg, int nr, hipStream_t s) {
+  const int P = (nr + 15) / 16;
+  if (P > 256) return hipErrorInvalidValue;
+  tridiag_one_kernel<<<1, P, tri_lds_bytes(nr, 16, 1, P, true), s>>>(d, a, b, c, lam, cseg, nr,
+                                                                      P);
   return hipGetLastError();
 }
 

@@ -42,7 +42,12 @@ struct StokesPlan {
   // differ on the same grid (0: Stokes, 1: implicit diffusion)
   int tune_key = 0;
 
+  std::vector<double> host_r;  // r_i on the host (init_transforms)
+
+  // Transform tables and r, then the Stokes column-pass tables.
   hipError_t init(int nr, int nz, double dx);
+  // The DCT tables (tw, e1, e2) and r only (fft must be set).
+  hipError_t init_transforms(int nr, int nz, double dx);
   // Column-pass tables (a, b, c, lam, pivots at segment starts), the
   // spectral buffer and the segment layout for the given matrices; nr and
   // nz must be set.
@@ -83,6 +88,15 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
                             int batch = 1);
 template <typename T>
 hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch = 1);
+// c'_{pL-1} of the forward elimination at every segment start p L, per mode,
+// laid out (P, nz).
+std::vector<double> pivot_table(const std::vector<double>& a, const std::vector<double>& b,
+                                const std::vector<double>& c, const std::vector<double>& lam,
+                                int nr, int nz, int L);
+// One tridiagonal system a_i x_{i-1} + (b_i + lam[0]) x_i + c_i x_{i+1} = d_i
+// of nr <= 4096 contiguous rows, in place; cseg = pivot_table(..., nz = 1, L = 16).
+hipError_t launch_tridiag_one(double* d, const double* a, const double* b, const double* c,
+                              const double* lam, const double* cseg, int nr, hipStream_t s);
 // True when rows of half-length M run a one-wave register transform
 // (M = 128, 192, 256, 384, 512, 1024; M = 2048 runs two waves per row).
 bool wave_rows(int M);
