@@ -119,6 +119,25 @@ axs_status axs_update_vorticity_from_solid_stress_f64(double* vorticity, double*
                                                       int nr, int nz, double dt, double dx,
                                                       void* stream);
 
+/* ---- Periodic z: kernels/periodic_boundary_ghost_comm.py:4-32.  Ghost
+ * columns of a z-periodic field: field[:, :g] = field[:, -2g:-g],
+ * field[:, -g:] = field[:, g:2g]; eta_shift != 0 is the reference map's
+ * version (gen_periodic_boundary_ghost_comm_eta: the left ghosts get
+ * - z_max + 2 g dx, the right ones + z_max - 2 g dx).  Needs nz >= 4 g. */
+axs_status axs_periodic_ghost_comm_f64(double* field, int nr, int nz, int ghost, int eta_shift,
+                                       double z_max, double dx, void* stream);
+/* kernels/diffusion_RK2.py:47-90 (diffusion_RK2_periodic): the ghost
+ * exchange of `ghost` columns before each RK2 stage. */
+axs_status axs_diffusion_rk2_periodic_f64(double* vorticity, double* temp_vorticity,
+                                          const double* R, int nr, int nz, double nu, double dt,
+                                          double dx, int ghost, void* stream);
+/* elasto_kernels/div_tau.py:37-75 (update_vorticity_from_solid_stress_periodic):
+ * ghost exchange of tau11, tau12, tau22 (in place), tau_z and tau_r, then
+ * of tau_r and tau_z before the vorticity update. */
+axs_status axs_update_vorticity_from_solid_stress_periodic_f64(
+    double* vorticity, double* tau_z, double* tau_r, double* tau11, double* tau12, double* tau22,
+    const double* R, int nr, int nz, double dt, double dx, int ghost, void* stream);
+
 /* ---- A13: elasto_kernels/advect_refmap_via_eno3.py:8-54 with the
  * non-conservative flux of pyst_kernels/advection_flux.py:166-332.
  * eta1 is even and eta2 odd about the axis; `scratch` holds 2 (nr, nz) buffers. */
@@ -128,10 +147,15 @@ axs_status axs_advect_refmap_eno3_f64(double* eta1, double* eta2, const double* 
 
 /* ---- A2: kernels/FastDiagonalisationStokesSolver.py:6-156.
  * Solves A_r psi + psi P_z^T = r * rhs with P_z, P_r, D_r as built at
- * :41-97 (Neumann z and r), by a DCT-II along z and one radial tridiagonal
- * solve per z-mode.  The plan owns its device tables and scratch. */
+ * :41-97 for each bc_type (Neumann z and r; Neumann or Dirichlet r with
+ * periodic z), by a DCT-II (periodic: a real DFT) along z and one radial
+ * tridiagonal solve per z-mode.  The plan owns its device tables and
+ * scratch. */
 typedef struct axs_stokes_plan axs_stokes_plan;
 #define AXS_BC_NEUMANN_Z_AND_R 0
+/* :88-97: periodic P_z (real DFT along z); Neumann or Dirichlet at r_max */
+#define AXS_BC_NEUMANN_R_PERIODIC_Z 1
+#define AXS_BC_DIRICHLET_R_PERIODIC_Z 2
 axs_status axs_stokes_plan_create(int nr, int nz, double dx, int bc_type,
                                   axs_stokes_plan** plan_out);
 axs_status axs_stokes_plan_destroy(axs_stokes_plan* plan);

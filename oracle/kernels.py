@@ -256,6 +256,12 @@ def advect_refmap_via_eno3(eta1, eta2, u_z, u_r, dt, dx):
 
 def diffusion_RK2_unb(vorticity, temp_vorticity, R, nu, dt, dx):
     """diffusion_RK2.py:4-44"""
+    _rk2_stage1(vorticity, temp_vorticity, R, nu, dt, dx)
+    _rk2_stage2(vorticity, temp_vorticity, R, nu, dt, dx)
+
+
+def _rk2_stage1(vorticity, temp_vorticity, R, nu, dt, dx):
+    """diffusion_RK2.py:9-26 (and :52-70 of the periodic form)"""
     temp_vorticity[...] = vorticity
     temp_vorticity[1:-1, 1:-1] += (
         0.5
@@ -274,6 +280,10 @@ def diffusion_RK2_unb(vorticity, temp_vorticity, R, nu, dt, dx):
             - vorticity[1:-1, 1:-1] * (R[1:-1, 1:-1] ** -2)
         )
     )
+
+
+def _rk2_stage2(vorticity, temp_vorticity, R, nu, dt, dx):
+    """diffusion_RK2.py:27-44 (and :72-90 of the periodic form)"""
     vorticity[1:-1, 1:-1] += (
         nu
         * dt
@@ -366,6 +376,12 @@ def update_vorticity_from_solid_stress(
     vorticity, tau_z, tau_r, tau11, tau12, tau22, R, dt, dx
 ):
     """div_tau.py:4-34"""
+    _div_tau(tau_z, tau_r, tau11, tau12, tau22, R, dx)
+    _curl_tau(vorticity, tau_z, tau_r, dt, dx)
+
+
+def _div_tau(tau_z, tau_r, tau11, tau12, tau22, R, dx):
+    """div_tau.py:7-28 (and :45-67 of the periodic form)"""
     tau_z[1:-1, 1:-1] = (
         tau11[1:-1, 2:] - tau11[1:-1, :-2] + tau12[2:, 1:-1] - tau12[:-2, 1:-1]
     ) / (2 * dx) + tau12[1:-1, 1:-1] / R[1:-1, 1:-1]
@@ -386,6 +402,10 @@ def update_vorticity_from_solid_stress(
         + 4 * tau22[1, 1:-1]
         - 3 * tau22[0, 1:-1]
     ) / (2 * dx) + tau22[0, 1:-1] / R[0, 1:-1]
+
+
+def _curl_tau(vorticity, tau_z, tau_r, dt, dx):
+    """div_tau.py:29-34 (and :71-75 of the periodic form)"""
     vorticity[1:-1, 1:-1] += (
         dt
         * (tau_r[1:-1, 2:] - tau_r[1:-1, :-2] - tau_z[2:, 1:-1] + tau_z[:-2, 1:-1])

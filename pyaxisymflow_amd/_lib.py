@@ -99,6 +99,10 @@ SIGNATURES = {
     "axs_potential_plan_destroy": [_P],
     "axs_potential_solve_f64": [_P, _P, _P, _P],
     "axs_velocity_from_phi_f64": [_P, _P, _P, _I, _I, _D, _P],
+    "axs_periodic_ghost_comm_f64": [_P, _I, _I, _I, _I, _D, _D, _P],
+    "axs_diffusion_rk2_periodic_f64": [_P, _P, _P, _I, _I, _D, _D, _D, _I, _P],
+    "axs_update_vorticity_from_solid_stress_periodic_f64": [_P, _P, _P, _P, _P, _P, _P, _I, _I,
+                                                            _D, _D, _I, _P],
     "axs_implicit_diffusion_plan_create": [_I, _I, _D, _D, ctypes.POINTER(_P)],
     "axs_implicit_diffusion_plan_destroy": [_P],
     "axs_implicit_diffusion_step_f64": [_P, _P, _P],

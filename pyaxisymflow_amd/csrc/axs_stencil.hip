@@ -327,6 +327,36 @@ template hipError_t launch_div_tau<double>(double*, double*, double*, const doub
 
 using namespace axs;
 
+// periodic_boundary_ghost_comm.py:4-32: the ghost columns of a field
+// periodic along z, field[:, :g] = field[:, -2g:-g] and
+// field[:, -g:] = field[:, g:2g]; with eta_shift the reference map's
+// version subtracts / adds the period, (x - Z_max) + 2 g dx on the left and
+// (x + Z_max) - 2 g dx on the right (c = 2 g dx as the reference forms it).
+// Sources and destinations are disjoint for nz >= 4 g.
+__global__ void periodic_ghost_comm_kernel(double* __restrict__ f, int nz, int g, int eta_shift,
+                                           double z_max, double c) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 2 * g) return;
+  double* row = f + (int64_t)blockIdx.y * nz;
+  if (q < g) {
+    double v = row[nz - 2 * g + q];
+    if (eta_shift) v = (v - z_max) + c;
+    row[q] = v;
+  } else {
+    double v = row[q];  // column g + (q - g)
+    if (eta_shift) v = (v + z_max) - c;
+    row[nz - g + (q - g)] = v;
+  }
+}
+
+static hipError_t launch_ghost_comm(double* f, int nr, int nz, int g, int eta_shift, double z_max,
+                                    double dx, hipStream_t s) {
+  const double c = (double)(2 * g) * dx;
+  periodic_ghost_comm_kernel<<<dim3(ceil_div(2 * g, 64), nr), 64, 0, s>>>(f, nz, g, eta_shift,
+                                                                          z_max, c);
+  return hipGetLastError();
+}
+
 extern "C" {
 
 const char* axs_last_error(void) { return g_last_error.c_str(); }
@@ -450,6 +480,52 @@ axs_status axs_update_vorticity_from_solid_stress_f64(double* vorticity, double*
   div_tau_stage1_kernel<double, RFull<double>><<<dim3(ceil_div(nz - 2, 256), nr - 1), 256, 0, s>>>(
       tau_z, tau_r, tau11, tau12, tau22, RFull<double>{R, nz}, nr, nz, 2.0 * dx);
   AXS_LAUNCH_CHECK("div_tau_stage1");
+  div_tau_stage2_kernel<double><<<dim3(ceil_div(nz - 2, 256), nr - 2), 256, 0, s>>>(
+      vorticity, tau_z, tau_r, nr, nz, Scalar<double>{dt, nullptr}, 2.0 * dx);
+  AXS_LAUNCH_CHECK("div_tau_stage2");
+  return AXS_OK;
+}
+
+axs_status axs_periodic_ghost_comm_f64(double* field, int nr, int nz, int ghost, int eta_shift,
+                                       double z_max, double dx, void* stream) {
+  AXS_CHECK_ARG(field, "null pointer");
+  AXS_CHECK_ARG(ghost > 0 && nr >= 1 && nz >= 4 * ghost, "need ghost > 0 and nz >= 4 ghost");
+  AXS_HIP(launch_ghost_comm(field, nr, nz, ghost, eta_shift, z_max, dx, as_stream(stream)));
+  return AXS_OK;
+}
+
+axs_status axs_diffusion_rk2_periodic_f64(double* vorticity, double* temp_vorticity,
+                                          const double* R, int nr, int nz, double nu, double dt,
+                                          double dx, int ghost, void* stream) {
+  AXS_CHECK_ARG(vorticity && temp_vorticity && R, "null pointer");
+  AXS_CHECK_ARG(nr >= 3 && ghost > 0 && nz >= 4 * ghost, "need nr >= 3, nz >= 4 ghost");
+  AXS_CHECK_ARG(vorticity != temp_vorticity, "vorticity and temp_vorticity must differ");
+  hipStream_t s = as_stream(stream);
+  RFull<double> RA{R, nz};
+  const double dx2 = dx * dx;
+  AXS_HIP(launch_ghost_comm(vorticity, nr, nz, ghost, 0, 0.0, dx, s));
+  rk2_stage1_kernel<double, RFull<double>><<<dim3(ceil_div(nz, 256), nr), 256, 0, s>>>(
+      vorticity, temp_vorticity, RA, nr, nz, Scalar<double>{0.5 * nu * dt, nullptr}, dx2, 2.0 * dx);
+  AXS_LAUNCH_CHECK("rk2_stage1");
+  AXS_HIP(launch_ghost_comm(temp_vorticity, nr, nz, ghost, 0, 0.0, dx, s));
+  rk2_stage2_kernel<double, RFull<double>><<<dim3(ceil_div(nz - 2, 256), nr - 2), 256, 0, s>>>(
+      vorticity, temp_vorticity, RA, nr, nz, Scalar<double>{nu * dt, nullptr}, dx2, 2.0 * dx);
+  AXS_LAUNCH_CHECK("rk2_stage2");
+  return AXS_OK;
+}
+
+axs_status axs_update_vorticity_from_solid_stress_periodic_f64(
+    double* vorticity, double* tau_z, double* tau_r, double* tau11, double* tau12, double* tau22,
+    const double* R, int nr, int nz, double dt, double dx, int ghost, void* stream) {
+  AXS_CHECK_ARG(vorticity && tau_z && tau_r && tau11 && tau12 && tau22 && R, "null pointer");
+  AXS_CHECK_ARG(nr >= 3 && ghost > 0 && nz >= 4 * ghost, "need nr >= 3, nz >= 4 ghost");
+  hipStream_t s = as_stream(stream);
+  for (double* t : {tau11, tau12, tau22}) AXS_HIP(launch_ghost_comm(t, nr, nz, ghost, 0, 0.0, dx, s));
+  div_tau_stage1_kernel<double, RFull<double>><<<dim3(ceil_div(nz - 2, 256), nr - 1), 256, 0, s>>>(
+      tau_z, tau_r, tau11, tau12, tau22, RFull<double>{R, nz}, nr, nz, 2.0 * dx);
+  AXS_LAUNCH_CHECK("div_tau_stage1");
+  AXS_HIP(launch_ghost_comm(tau_r, nr, nz, ghost, 0, 0.0, dx, s));
+  AXS_HIP(launch_ghost_comm(tau_z, nr, nz, ghost, 0, 0.0, dx, s));
   div_tau_stage2_kernel<double><<<dim3(ceil_div(nz - 2, 256), nr - 2), 256, 0, s>>>(
       vorticity, tau_z, tau_r, nr, nz, Scalar<double>{dt, nullptr}, 2.0 * dx);
   AXS_LAUNCH_CHECK("div_tau_stage2");

@@ -181,7 +181,8 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
   return hipSuccess;
 }
 
-hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
+hipError_t StokesPlan::init(int nr_, int nz_, double dx_, int bc_) {
+  bc = bc_;
   hipError_t e = init_transforms(nr_, nz_, dx_);
   if (e != hipSuccess) return e;
   const std::vector<double>& h_r = host_r;
@@ -196,12 +197,22 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_) {
     if (i > 0) h_a[i] = P_sub - D_sub;
     if (i < nr - 1) h_c[i] = P_sup - D_sup;
   }
-  h_b[nr - 1] = inv_dx2;                 // P_r[-1,-1] = inv_dx2
-  if (nr > 1) h_a[nr - 1] = inv_dx2 * -1.0 - 0.0;  // D_r[-1,-2] = 0
+  if (bc != 2) {                           // Neumann at r_max (:79-86)
+    h_b[nr - 1] = inv_dx2;                 // P_r[-1,-1] = inv_dx2
+    if (nr > 1) h_a[nr - 1] = inv_dx2 * -1.0 - 0.0;  // D_r[-1,-2] = 0
+  }
   std::vector<double> h_lam(nz);
-  for (int k = 0; k < nz; ++k) {
-    const double s = std::sin(M_PI * (double)k / (2.0 * nz));
-    h_lam[k] = (4.0 / (dx * dx)) * s * s;
+  if (bc == 0) {  // DCT-II modes of the Neumann P_z
+    for (int k = 0; k < nz; ++k) {
+      const double s = std::sin(M_PI * (double)k / (2.0 * nz));
+      h_lam[k] = (4.0 / (dx * dx)) * s * s;
+    }
+  } else {  // DFT modes of the periodic P_z in the packed real layout
+    for (int p = 0; p < nz; ++p) {
+      const int k = (p == 0) ? 0 : (p == nz - 1) ? nz / 2 : (p + 1) / 2;
+      const double s = std::sin(M_PI * (double)k / (double)nz);
+      h_lam[p] = (4.0 / (dx * dx)) * s * s;
+    }
   }
   return set_tridiag(h_a, h_b, h_c, h_lam);
 }
@@ -281,6 +292,73 @@ hipError_t StokesPlan::set_tridiag(const std::vector<double>& h_a, const std::ve
 }
 
 // ------------------------------------------------------------------ kernels
+
+// Periodic z (bc_type *_periodic_along_z): real DFT of each row through the
+// complex FFT of its M = nz / 2 packed pairs z_m = x_{2m} + i x_{2m+1}:
+//   X_k = (Z_k + conj Z_{M-k}) / 2 - i W_N^k (Z_k - conj Z_{M-k}) / 2,  k = 0..M,
+// stored as the packed real row [X_0, Re X_1, Im X_1, ..., Re X_{M-1},
+// Im X_{M-1}, X_M] (X_0 and X_M are real).  Every entry is one real
+// unknown of its mode's radial system.  W_N^k = e1[k].
+template <int PER>
+__global__ void __launch_bounds__(256) rfft_rows_kernel(
+    const double* __restrict__ src, double* __restrict__ dst, const double* __restrict__ rscale,
+    int nz, FftPlanArgs fp, const Cplx<double>* __restrict__ tw,
+    const Cplx<double>* __restrict__ e1) {
+  extern __shared__ unsigned char smem_raw[];
+  const int M = fp.M;
+  Cplx<double>* buf = reinterpret_cast<Cplx<double>*>(smem_raw);
+  const int i = blockIdx.x;
+  const double* row = src + (int64_t)i * nz;
+  const double sc = rscale ? rscale[i] : 1.0;
+  for (int j = threadIdx.x; j < nz; j += blockDim.x) packed_component(buf, j) = sc * row[j];
+  __syncthreads();
+  fft_lds<double, PER>(buf, fp, tw, -1, threadIdx.x, blockDim.x);
+  double* out = dst + (int64_t)i * nz;
+  for (int k = threadIdx.x; k <= M; k += blockDim.x) {
+    const Cplx<double> zk = buf[k == M ? 0 : k], zm = buf[k == 0 ? 0 : M - k];
+    const Cplx<double> a = cadd(zk, cconj(zm)), b = csub(zk, cconj(zm));
+    const Cplx<double> c = cmul(e1[k], b);
+    const double xr = 0.5 * (a.x + c.y), xi = 0.5 * (a.y - c.x);
+    if (k == 0)
+      out[0] = xr;
+    else if (k == M)
+      out[nz - 1] = xr;
+    else {
+      out[2 * k - 1] = xr;
+      out[2 * k] = xi;
+    }
+  }
+}
+
+// Inverse: E_k = (X_k + conj X_{M-k}) / 2, O_k = conj(W_N^k) (X_k - conj X_{M-k}) / 2,
+// Z_k = E_k + i O_k, inverse FFT, x_{2m} + i x_{2m+1} = z_m / M.
+template <int PER>
+__global__ void __launch_bounds__(256) irfft_rows_kernel(
+    const double* __restrict__ src, double* __restrict__ dst, int nz, FftPlanArgs fp,
+    const Cplx<double>* __restrict__ tw, const Cplx<double>* __restrict__ e1) {
+  extern __shared__ unsigned char smem_raw[];
+  const int M = fp.M;
+  Cplx<double>* buf = reinterpret_cast<Cplx<double>*>(smem_raw);
+  const int i = blockIdx.x;
+  const double* row = src + (int64_t)i * nz;
+  auto X = [&](int k) -> Cplx<double> {
+    if (k == 0) return {row[0], 0.0};
+    if (k == M) return {row[nz - 1], 0.0};
+    return {row[2 * k - 1], row[2 * k]};
+  };
+  for (int k = threadIdx.x; k < M; k += blockDim.x) {
+    const Cplx<double> xk = X(k), xm = X(M - k);
+    const Cplx<double> e = {0.5 * (xk.x + xm.x), 0.5 * (xk.y - xm.y)};
+    const Cplx<double> d = csub(xk, cconj(xm));
+    const Cplx<double> o = cmul(cconj(e1[k]), Cplx<double>{0.5 * d.x, 0.5 * d.y});
+    buf[k] = {e.x - o.y, e.y + o.x};
+  }
+  __syncthreads();
+  fft_lds<double, PER>(buf, fp, tw, +1, threadIdx.x, blockDim.x);
+  double* out = dst + (int64_t)i * nz;
+  const double invM = 1.0 / (double)M;
+  for (int j = threadIdx.x; j < nz; j += blockDim.x) out[j] = packed_component(buf, j) * invM;
+}
 
 // Row pass: x_j = scale_i * src[i, j]  ->  X = DCT-II(x)  ->  dst[i, :].
 // The row is read coalesced and scattered into the packed LDS sequence; the
@@ -1199,6 +1277,31 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
   return hipGetLastError();
 }
 
+hipError_t launch_rfft_rows(const StokesPlan& pl, const double* src, double* dst,
+                            const double* rscale, int rows, hipStream_t s, bool inverse) {
+  const int M = pl.nz / 2;
+  const int bs = fft_block(M);
+  const int per = std::max(1, (M / 2 + bs - 1) / bs);
+  const size_t shm = sizeof(Cplx<double>) * M;
+#define AXS_RFFT(P)                                                                           \
+  do {                                                                                        \
+    if (inverse)                                                                              \
+      irfft_rows_kernel<P><<<rows, bs, shm, s>>>(src, dst, pl.nz, pl.fft, pl.tw, pl.e1);     \
+    else                                                                                      \
+      rfft_rows_kernel<P><<<rows, bs, shm, s>>>(src, dst, rscale, pl.nz, pl.fft, pl.tw, pl.e1); \
+  } while (0)
+  if (per <= 1)
+    AXS_RFFT(1);
+  else if (per <= 2)
+    AXS_RFFT(2);
+  else if (per <= 4)
+    AXS_RFFT(4);
+  else
+    AXS_RFFT(8);
+#undef AXS_RFFT
+  return hipGetLastError();
+}
+
 // One tridiagonal system of nr rows held contiguously in d (segments of 16
 // rows, one thread each, coefficients read from global memory).
 __global__ void __launch_bounds__(256) tridiag_one_kernel(
@@ -1342,8 +1445,9 @@ axs_status axs_stokes_plan_create(int nr, int nz, double dx, int bc_type,
                                   axs_stokes_plan** plan_out) {
   AXS_CHECK_ARG(plan_out, "null plan_out");
   *plan_out = nullptr;
-  AXS_CHECK_ARG(bc_type == AXS_BC_NEUMANN_Z_AND_R,
-                "only homogenous_neumann_along_z_and_r is supported");
+  AXS_CHECK_ARG(bc_type == AXS_BC_NEUMANN_Z_AND_R || bc_type == AXS_BC_NEUMANN_R_PERIODIC_Z ||
+                    bc_type == AXS_BC_DIRICHLET_R_PERIODIC_Z,
+                "unknown bc_type");
   AXS_CHECK_ARG(nr >= 3 && nz >= 4 && nz % 2 == 0, "need nr >= 3 and even nz >= 4");
   AXS_CHECK_ARG(nz <= 4096 && nr <= 4096, "grid too large for the row/column passes");
   AXS_CHECK_ARG(dx > 0.0, "dx must be positive");
@@ -1358,7 +1462,7 @@ axs_status axs_stokes_plan_create(int nr, int nz, double dx, int bc_type,
     return AXS_ERR_ALLOC;
   }
   p->impl.fft = fp;
-  hipError_t e = p->impl.init(nr, nz, dx);
+  hipError_t e = p->impl.init(nr, nz, dx, bc_type);
   if (e != hipSuccess) {
     set_error(std::string("stokes plan: ") + hipGetErrorString(e));
     delete p;
@@ -1378,6 +1482,12 @@ axs_status axs_stokes_solve_f64(axs_stokes_plan* plan, double* psi, const double
   AXS_CHECK_ARG(plan && psi && rhs, "null pointer");
   StokesPlan& pl = plan->impl;
   hipStream_t s = as_stream(stream);
+  if (pl.bc != AXS_BC_NEUMANN_Z_AND_R) {  // periodic z: real DFT rows
+    AXS_HIP(launch_rfft_rows(pl, rhs, pl.spec, pl.rvec, pl.nr, s, false));
+    AXS_HIP(launch_tridiag<double>(pl, pl.spec, s));
+    AXS_HIP(launch_rfft_rows(pl, pl.spec, psi, nullptr, pl.nr, s, true));
+    return AXS_OK;
+  }
   AXS_HIP(launch_dct2_rows<double>(pl, rhs, pl.spec, pl.rvec, pl.nr, s));
   AXS_HIP(launch_tridiag<double>(pl, pl.spec, s));
   AXS_HIP(launch_dct3_rows<double>(pl, pl.spec, psi, pl.nr, s));

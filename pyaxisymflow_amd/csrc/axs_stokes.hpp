@@ -44,8 +44,12 @@ struct StokesPlan {
 
   std::vector<double> host_r;  // r_i on the host (init_transforms)
 
+  // boundary conditions (AXS_BC_*): 0 Neumann z and r (DCT rows), 1 Neumann
+  // r and periodic z, 2 Dirichlet r and periodic z (real-DFT rows)
+  int bc = 0;
+
   // Transform tables and r, then the Stokes column-pass tables.
-  hipError_t init(int nr, int nz, double dx);
+  hipError_t init(int nr, int nz, double dx, int bc = 0);
   // The DCT tables (tw, e1, e2) and r only (fft must be set).
   hipError_t init_transforms(int nr, int nz, double dx);
   // Column-pass tables (a, b, c, lam, pivots at segment starts), the
@@ -93,6 +97,10 @@ hipError_t launch_tridiag(const StokesPlan& pl, T* d, hipStream_t s, int batch =
 std::vector<double> pivot_table(const std::vector<double>& a, const std::vector<double>& b,
                                 const std::vector<double>& c, const std::vector<double>& lam,
                                 int nr, int nz, int L);
+// Real DFT rows of a periodic-z plan (packed real spectrum, see
+// rfft_rows_kernel), or their inverse.
+hipError_t launch_rfft_rows(const StokesPlan& pl, const double* src, double* dst,
+                            const double* rscale, int rows, hipStream_t s, bool inverse);
 // One tridiagonal system a_i x_{i-1} + (b_i + lam[0]) x_i + c_i x_{i+1} = d_i
 // of nr <= 4096 contiguous rows, in place; cseg = pivot_table(..., nz = 1, L = 16).
 hipError_t launch_tridiag_one(double* d, const double* a, const double* b, const double* c,

@@ -34,3 +34,22 @@ def gen_advect_refmap_via_eno3(
         st.finish()
 
     return advect_refmap_via_eno3
+
+
+def gen_advect_refmap_via_eno3_periodic(
+    dx, grid_size_r, grid_size_z, per_communicator1, per_communicator2, real_t=np.float64,
+    num_threads=False,
+):
+    """advect_refmap_via_eno3.py:57-116: the ghost exchange of u_z, u_r
+    (per_communicator1), eta1 (per_communicator2, the shifted exchange) and
+    eta2 (per_communicator1), then the same ENO3 steps."""
+    step = gen_advect_refmap_via_eno3(dx, grid_size_r, grid_size_z, real_t, num_threads)
+
+    def advect_refmap_via_eno3_periodic(eta1, eta2, u_z, u_r, dt):
+        per_communicator1(u_z)
+        per_communicator1(u_r)
+        per_communicator2(eta1)
+        per_communicator1(eta2)
+        step(eta1, eta2, u_z, u_r, dt)
+
+    return advect_refmap_via_eno3_periodic

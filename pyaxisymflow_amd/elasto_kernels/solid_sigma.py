@@ -23,3 +23,15 @@ def solid_sigma(
         nr, nz, _lib.stream_handle(),
     )
     st.finish()
+
+
+def solid_sigma_periodic(
+    sigma_s_11, sigma_s_12, sigma_s_22, G, dx, eta_1, eta_2, eta_1z, eta_1r, eta_2z, eta_2r,
+    per_communicator1, per_communicator2,
+):
+    """solid_sigma.py:32-60: the ghost exchange of eta_1 (per_communicator2)
+    and eta_2 (per_communicator1), then the same stress."""
+    per_communicator2(eta_1)
+    per_communicator1(eta_2)
+    solid_sigma(sigma_s_11, sigma_s_12, sigma_s_22, G, dx, eta_1, eta_2, eta_1z, eta_1r, eta_2z,
+                eta_2r)

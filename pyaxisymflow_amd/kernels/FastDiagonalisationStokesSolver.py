@@ -5,7 +5,11 @@ import numpy as np
 
 from pyaxisymflow_amd import _lib
 
-_BC = {"homogenous_neumann_along_z_and_r": 0}
+_BC = {
+    "homogenous_neumann_along_z_and_r": 0,
+    "homogenous_neumann_along_r_and_periodic_along_z": 1,
+    "homogenous_dirichlet_along_r_and_periodic_along_z": 2,
+}
 
 
 class FastDiagonalisationStokesSolver:
@@ -14,10 +18,11 @@ class FastDiagonalisationStokesSolver:
     (FastDiagonalisationStokesSolver.py:41-97).
 
     Instead of the reference's dense eigen-decomposition (four dense GEMMs
-    per solve), the z-operator is diagonalised by a DCT-II (its exact
-    eigenbasis) and each z-mode's radial operator, which is tridiagonal, is
-    solved directly: same linear system, O(nr nz log nz) work, and no
-    O(n^3) set-up.  Tables and the spectral buffer live on the device.
+    per solve), the z-operator is diagonalised by its exact eigenbasis (a
+    DCT-II for Neumann z, a real DFT for the periodic bc_types of :88-97) and
+    each z-mode's radial operator, which is tridiagonal, is solved directly:
+    same linear system, O(nr nz log nz) work, and no O(n^3) set-up.  Tables
+    and the spectral buffer live on the device.
     """
 
     def __init__(

@@ -35,3 +35,19 @@ def gen_advect_vorticity_via_eno3(
         st.finish()
 
     return advect_vorticity_via_eno3
+
+
+def gen_advect_vorticity_via_eno3_periodic(
+    dx, grid_size_r, grid_size_z, per_communicator, real_t=np.float64, num_threads=False
+):
+    """advect_vorticity_via_eno3.py:47-91: the ghost exchange of u_z, u_r and
+    the vorticity, then the same ENO3 step."""
+    step = gen_advect_vorticity_via_eno3(dx, grid_size_r, grid_size_z, real_t, num_threads)
+
+    def advect_vorticity_via_eno3_periodic(vorticity, u_z, u_r, dt):
+        per_communicator(u_z)
+        per_communicator(u_r)
+        per_communicator(vorticity)
+        step(vorticity, u_z, u_r, dt)
+
+    return advect_vorticity_via_eno3_periodic

@@ -16,3 +16,10 @@ def compute_velocity_from_psi_unb(u_z, u_r, psi, R, dx):
         nr, nz, float(dx), _lib.stream_handle(),
     )
     st.finish()
+
+
+def compute_velocity_from_psi_periodic(u_z, u_r, psi, R, dx, per_communicator):
+    """compute_velocity_from_psi.py:20-35: the ghost exchange of psi, then the
+    unbounded stencil."""
+    per_communicator(psi)
+    compute_velocity_from_psi_unb(u_z, u_r, psi, R, dx)

@@ -15,3 +15,11 @@ def compute_vorticity_from_velocity_unb(vort, u_z, u_r, dx):
         nr, nz, float(dx), _lib.stream_handle(),
     )
     st.finish()
+
+
+def compute_vorticity_from_velocity_periodic(vort, u_z, u_r, dx, per_communicator):
+    """compute_vorticity_from_velocity.py:16-28: the ghost exchange of u_r and
+    u_z, then the interior stencil."""
+    per_communicator(u_r)
+    per_communicator(u_z)
+    compute_vorticity_from_velocity_unb(vort, u_z, u_r, dx)
