@@ -273,8 +273,10 @@ def test_stokes_rejects_unsupported():
 
     with pytest.raises(NotImplementedError):
         FastDiagonalisationStokesSolver(8, 14, 0.1)  # 7 has no radix
+    # the three bc_types of FastDiagonalisationStokesSolver.py:78-97 are
+    # supported (periodic z: test_gpu_periodic.py); any other is refused
     with pytest.raises(NotImplementedError):
-        FastDiagonalisationStokesSolver(8, 16, 0.1, bc_type="homogenous_neumann_along_r_and_periodic_along_z")
+        FastDiagonalisationStokesSolver(8, 16, 0.1, bc_type="homogenous_dirichlet_along_z_and_r")
 
 
 @pytest.mark.parametrize("nz,seed", [(64, 1), (128, 2), (256, 3)])
