@@ -131,7 +131,7 @@ constexpr int kVPZ = kVTZ + 4;       // psi tile columns (halo 2)
 
 // Tile (bx, by), TR rows by kVTZ columns (TR = 8, 16 or 32), of one case
 // (psi, sc, slots at the case); the maximum goes to slot `slot`.
-template <typename T, int TR>
+template <typename T, int TR, bool STREAM = false>
 __device__ __forceinline__ void velocity_max_tile(const T* __restrict__ psi,
                                                   const double* __restrict__ r, int nr, int nz,
                                                   T dx, const double* __restrict__ sc,
@@ -145,6 +145,36 @@ __device__ __forceinline__ void velocity_max_tile(const T* __restrict__ psi,
   __shared__ double wm[4];
   const int tid = threadIdx.x;
   const int j0 = bx * kVTZ, i0 = by * TR;
+  const T two_dx = T(2) * dx, inv2dx = T(1) / two_dx;
+  const T ufree = T(cfg.U0 * sc[kScPrefac]);
+  double m = 0.0;
+  const bool interior = i0 >= 1 && i0 + TR <= nr - 1 && j0 >= 1 && j0 + kVTZ <= nz - 1;
+  if (STREAM && interior) {
+    // no cell of the tile lies on a domain edge: each thread walks TR / 4
+    // rows of one column, reading psi and its four neighbours directly (the
+    // neighbour rows and columns come from L1/L2), no LDS staging; central
+    // differences only, the same arithmetic as the staged path below
+    constexpr int RS = TR / 4;
+    const int b = tid & 63, g = tid >> 6;
+    const int ia = i0 + g * RS;
+    const T* col = psi + (int64_t)ia * nz + (j0 + b);
+    T c[RS + 2], lf[RS], rt[RS];
+#pragma unroll
+    for (int k = 0; k < RS + 2; ++k) c[k] = col[(int64_t)(k - 1) * nz];
+#pragma unroll
+    for (int k = 0; k < RS; ++k) {
+      lf[k] = col[(int64_t)k * nz - 1];
+      rt[k] = col[(int64_t)k * nz + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < RS; ++k) {
+      const T ri = T(r[ia + k]);
+      const T rinv = T(1) / ri;
+      const T uz = div_by(div_by(c[k + 2] - c[k], two_dx, inv2dx), ri, rinv) + ufree;
+      const T ur = div_by(div_by(-(rt[k] - lf[k]), two_dx, inv2dx), ri, rinv);
+      m = fmax(m, double(fabs(uz) + fabs(ur)));
+    }
+  } else {
   constexpr int NP = (kVPR * kVPZ + 255) / 256;
   T v[NP];
 #pragma unroll
@@ -167,10 +197,7 @@ __device__ __forceinline__ void velocity_max_tile(const T* __restrict__ psi,
     if (e < kVPR * kVPZ) sp[e / kVPZ][e % kVPZ] = v[q];
   }
   __syncthreads();
-  const T two_dx = T(2) * dx, inv2dx = T(1) / two_dx;
-  const T ufree = T(cfg.U0 * sc[kScPrefac]);
-  double m = 0.0;
-  if (i0 >= 1 && i0 + TR <= nr - 1 && j0 >= 1 && j0 + kVTZ <= nz - 1) {
+  if (interior) {
     // no cell of the tile lies on a domain edge: central differences only
     // (the formulas velocity_rn takes away from the edges)
 #pragma unroll
@@ -196,6 +223,7 @@ __device__ __forceinline__ void velocity_max_tile(const T* __restrict__ psi,
       }
     }
   }
+  }  // staged path
   m = wave_max(m);
   if ((tid & 63) == 0) wm[tid >> 6] = m;
   __syncthreads();
@@ -214,7 +242,7 @@ __global__ void __launch_bounds__(256) velocity_max_kernel(const T* __restrict__
                                                            double* __restrict__ slots,
                                                            const RigidConfig* __restrict__ cfgs) {
   const int cb = blockIdx.z;  // case of a batch
-  velocity_max_tile<T, kVTR>(psi + (int64_t)cb * nr * nz, r, nr, nz, dx,
+  velocity_max_tile<T, kVTR, true>(psi + (int64_t)cb * nr * nz, r, nr, nz, dx,
                        sc + (int64_t)cb * AXS_RIGID_SCALARS, slots + (int64_t)cb * kSlotsPerCase,
                        cfgs[cb], blockIdx.x, blockIdx.y,
                        (blockIdx.y * gridDim.x + blockIdx.x) % kMaxSlots);
