@@ -61,30 +61,9 @@ def profile_traffic(stage, nz, nr, dtype="double"):
 
 
 def workload(nz, nr, seed=0):
-    from pyaxisymflow_amd.stepper import rigid_grid
+    from pyaxisymflow_amd.workloads import kernel_scale_workload
 
-    z_len = 2.0
-    dx, z, r = rigid_grid(nz, nr, z_len)
-    Z, R = np.meshgrid(z, r)
-    rng = np.random.default_rng(seed)
-    n = 64
-    a = rng.standard_normal(n)
-    s = rng.uniform(0.01, 0.05, n)
-    cz = rng.uniform(0, z_len, n)
-    cr = rng.uniform(0, nr * dx, n)
-    w = np.zeros_like(Z)
-    for i in range(n):
-        w += a[i] * np.exp(-((Z - cz[i]) ** 2 + (R - cr[i]) ** 2) / (2 * s[i] ** 2))
-    rng2 = np.random.default_rng(seed + 1)
-    rad = rng2.uniform(0.02, 0.06, 8)
-    zc = rng2.uniform(0.2, 1.8, 8)
-    phi = np.full_like(Z, -np.inf)
-    for i in range(8):
-        phi = np.maximum(phi, rad[i] - np.sqrt((Z - zc[i]) ** 2 + R**2))
-    wdt = np.sqrt(2) * dx
-    chi = (phi >= wdt) + (np.fabs(phi) < wdt) * 0.5 * (1 + phi / wdt + np.sin(np.pi * phi / wdt) / np.pi)
-    return dict(z_len=z_len, dx=dx, vorticity=w, char_func=chi.astype(np.float64),
-                r_sph=float(rad.mean()), nu=1.5e-3, brink_lam=1e4, U_0=1.0)
+    return kernel_scale_workload(nz, nr, seed)
 
 
 def time_gpu(nz, nr, steps, warmup, stage_reps):

@@ -311,6 +311,12 @@ axs_status axs_rigid_stepper_destroy(axs_rigid_stepper* st);
  * graph per ping-pong direction) on first use and replays it (stream must
  * not be the null stream). */
 axs_status axs_rigid_stepper_run(axs_rigid_stepper* st, int nsteps, int use_graph, void* stream);
+/* Graph launches queued so far by axs_rigid_stepper_run with use_graph:
+ * chunks of 16 steps and single steps (a run takes as many chunks as fit,
+ * from either ping-pong parity, then single steps).  Diagnostics; no
+ * counterpart in the reference. */
+axs_status axs_rigid_stepper_graph_launches(const axs_rigid_stepper* st, int64_t* chunks,
+                                            int64_t* singles);
 /* Profiling hook: the step is a fixed sequence of stages (taper + dct2 rows,
  * tridiagonal columns, dct3 rows, velocity maximum, body mask, fused
  * penalisation / advection / diffusion, step scalars; disabled stages are

@@ -121,6 +121,7 @@ SIGNATURES = {
     "axs_rigid_stepper_run": [_P, _I, _I, _P],
     "axs_rigid_stepper_reset_scalars": [_P, _P],
     "axs_rigid_stepper_velocity": [_P, _P],
+    "axs_rigid_stepper_graph_launches": [_P, ctypes.POINTER(_L), ctypes.POINTER(_L)],
     "axs_rigid_stepper_stage_count": [],
     "axs_rigid_stepper_stage_name": [_I],
     "axs_rigid_stepper_run_stage": [_P, _I, _I, _P],

@@ -98,9 +98,9 @@ RigidStepperBase::~RigidStepperBase() {
   for (int q = 0; q < 2; ++q) {
     if (graph_exec[q]) (void)hipGraphExecDestroy(graph_exec[q]);
     if (graph[q]) (void)hipGraphDestroy(graph[q]);
+    if (chunk_exec[q]) (void)hipGraphExecDestroy(chunk_exec[q]);
+    if (chunk_graph[q]) (void)hipGraphDestroy(chunk_graph[q]);
   }
-  if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
-  if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
   if (partials) (void)hipFree(partials);
   if (slots) (void)hipFree(slots);
   if (spec) (void)hipFree(spec);
@@ -256,16 +256,17 @@ hipError_t RigidStepperBase::run(int nsteps, bool use_graph, hipStream_t s) {
     for (int q = 0; q < 2; ++q) {
       if (graph_exec[q]) (void)hipGraphExecDestroy(graph_exec[q]);
       if (graph[q]) (void)hipGraphDestroy(graph[q]);
+      if (chunk_exec[q]) (void)hipGraphExecDestroy(chunk_exec[q]);
+      if (chunk_graph[q]) (void)hipGraphDestroy(chunk_graph[q]);
       graph_exec[q] = nullptr;
       graph[q] = nullptr;
+      chunk_exec[q] = nullptr;
+      chunk_graph[q] = nullptr;
     }
-    if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
-    if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
-    chunk_exec = nullptr;
-    chunk_graph = nullptr;
     const int saved = parity;
-    // graphs: one step from each parity, and kChunkSteps steps from parity 0
-    for (int q = 0; q < 3; ++q) {
+    // graphs: one step from each parity (q = 0, 1), and kChunkSteps steps
+    // from each parity (q = 2, 3)
+    for (int q = 0; q < 4; ++q) {
       hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
       if (e != hipSuccess) {
         parity = saved;
@@ -276,14 +277,14 @@ hipError_t RigidStepperBase::run(int nsteps, bool use_graph, hipStream_t s) {
         parity = q;
         le = enqueue_step(s);
       } else {
-        parity = 0;
+        parity = q - 2;
         for (int k = 0; k < kChunkSteps && le == hipSuccess; ++k) {
           le = enqueue_step(s);
           parity ^= 1;
         }
       }
-      hipGraph_t* g = (q < 2) ? &graph[q] : &chunk_graph;
-      hipGraphExec_t* x = (q < 2) ? &graph_exec[q] : &chunk_exec;
+      hipGraph_t* g = (q < 2) ? &graph[q] : &chunk_graph[q - 2];
+      hipGraphExec_t* x = (q < 2) ? &graph_exec[q] : &chunk_exec[q - 2];
       e = hipStreamEndCapture(s, g);
       if (le == hipSuccess && e == hipSuccess) e = hipGraphInstantiate(x, *g, nullptr, nullptr, 0);
       if (le != hipSuccess || e != hipSuccess) {
@@ -295,15 +296,16 @@ hipError_t RigidStepperBase::run(int nsteps, bool use_graph, hipStream_t s) {
     graph_stream = s;
   }
   int q = 0;
-  if (parity == 0) {
-    for (; q + kChunkSteps <= nsteps; q += kChunkSteps) {
-      hipError_t e = hipGraphLaunch(chunk_exec, s);
-      if (e != hipSuccess) return e;
-    }
+  // a chunk leaves the parity as it found it (kChunkSteps is even)
+  for (; q + kChunkSteps <= nsteps; q += kChunkSteps) {
+    hipError_t e = hipGraphLaunch(chunk_exec[parity], s);
+    if (e != hipSuccess) return e;
+    ++chunk_launches;
   }
   for (; q < nsteps; ++q) {
     hipError_t e = hipGraphLaunch(graph_exec[parity], s);
     if (e != hipSuccess) return e;
+    ++step_launches;
     parity ^= 1;
   }
   return settle(s);
@@ -835,6 +837,14 @@ axs_status axs_rigid_stepper_run(axs_rigid_stepper* st, int nsteps, int use_grap
   AXS_CHECK_ARG(nsteps >= 0, "negative step count");
   AXS_CHECK_ARG(!use_graph || stream, "graph capture needs a non-null stream");
   AXS_HIP(st->impl->run(nsteps, use_graph != 0, as_stream(stream)));
+  return AXS_OK;
+}
+
+axs_status axs_rigid_stepper_graph_launches(const axs_rigid_stepper* st, int64_t* chunks,
+                                            int64_t* singles) {
+  AXS_CHECK_ARG(st && chunks && singles, "null argument");
+  *chunks = st->impl->chunk_launches;
+  *singles = st->impl->step_launches;
   return AXS_OK;
 }
 

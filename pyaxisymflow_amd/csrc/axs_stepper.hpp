@@ -76,12 +76,16 @@ struct RigidStepperBase {
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   // kChunkSteps consecutive steps (an even count, so the vorticity pair
-  // returns to the parity it started from) in one graph: a graph launch
-  // costs microseconds, as much as a small kernel.
+  // returns to the parity it started from) in one graph, one graph per
+  // starting parity: a graph launch costs microseconds, as much as a small
+  // kernel.
   static constexpr int kChunkSteps = 16;
-  hipGraph_t chunk_graph = nullptr;
-  hipGraphExec_t chunk_exec = nullptr;
+  static_assert(kChunkSteps % 2 == 0, "a chunk must preserve the ping-pong parity");
+  hipGraph_t chunk_graph[2] = {nullptr, nullptr};
+  hipGraphExec_t chunk_exec[2] = {nullptr, nullptr};
   hipStream_t graph_stream = nullptr;
+  // graph launches so far: kChunkSteps-step chunks and single steps
+  int64_t chunk_launches = 0, step_launches = 0;
 
   hipError_t init(const RigidConfig* cfgs, int nbatch, const RigidBuffers& b);
   virtual hipError_t enqueue_stage(int stage, hipStream_t s) = 0;

@@ -134,7 +134,9 @@ class SoftServer:
                     progressed = True
                     continue
                 k = min(self.chunk, r["steps"] - a["done"])
-                a["st"].step(k)
+                # join=False: queued on the request's own stream only, so the
+                # chunks of all active requests overlap on the device
+                a["st"].step(k, join=False)
                 ev = torch.cuda.Event()
                 ev.record(a["st"].stream)
                 a["ev"] = ev

@@ -204,15 +204,27 @@ class RigidBatchStepper:
         self._h = handle
 
     # -------------------------------------------------------------- stepping
-    def step(self, n=1):
+    def step(self, n=1, join=True):
         """Queues n timesteps of every case on this stepper's stream
-        (asynchronous)."""
+        (asynchronous).  join=True orders the steps after the work already
+        queued on torch's current stream and that stream after the steps;
+        join=False queues them on the stepper's stream alone (the state
+        accessors below still wait for them), so several steppers driven
+        from one thread run concurrently."""
         torch = _lib.torch()
-        cur = torch.cuda.current_stream()
-        self.stream.wait_stream(cur)
+        cur = torch.cuda.current_stream() if join else None
+        if join:
+            self.stream.wait_stream(cur)
         _lib.call("axs_rigid_stepper_run", self._h, int(n), 1 if self.use_graph else 0,
                   ctypes.c_void_p(self.stream.cuda_stream))
-        cur.wait_stream(self.stream)
+        if join:
+            cur.wait_stream(self.stream)
+
+    def graph_launches(self):
+        """(chunk graph launches, single-step graph launches) queued so far."""
+        c, s = ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("axs_rigid_stepper_graph_launches", self._h, ctypes.byref(c), ctypes.byref(s))
+        return c.value, s.value
 
     def synchronize(self):
         self.stream.synchronize()
@@ -395,6 +407,7 @@ class UnevenBatchStepper:
         return t[o:o + nr * nz].view(nr, nz)
 
     step = RigidBatchStepper.step
+    graph_launches = RigidBatchStepper.graph_launches
     synchronize = RigidBatchStepper.synchronize
     _host = RigidBatchStepper._host
     __del__ = RigidBatchStepper.__del__
@@ -597,13 +610,18 @@ class SoftSphereStepper:
                   ctypes.byref(handle))
         self._h = handle
 
-    def step(self, n=1):
+    def step(self, n=1, join=True):
+        """Queues n steps on this stepper's stream; join as in
+        RigidBatchStepper.step (the server passes join=False so that the
+        admitted requests' steps overlap on the device)."""
         torch = _lib.torch()
-        cur = torch.cuda.current_stream()
-        self.stream.wait_stream(cur)
+        cur = torch.cuda.current_stream() if join else None
+        if join:
+            self.stream.wait_stream(cur)
         _lib.call("axs_soft_stepper_run", self._h, int(n), 1 if self.use_graph else 0,
                   ctypes.c_void_p(self.stream.cuda_stream))
-        cur.wait_stream(self.stream)
+        if join:
+            cur.wait_stream(self.stream)
 
     def synchronize(self):
         self.stream.synchronize()

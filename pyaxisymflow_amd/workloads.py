@@ -1,4 +1,14 @@
-"""Synthetic case sets for the batched rigid-body form.
+"""Synthetic workloads: the kernel-scale plain FP64 field and the case sets
+of the batched rigid-body form.
+
+Kernel-scale field (kernel_scale_workload): nz x nr on z in [0, 2], dx =
+2 / nz, r in [0, nr dx]; vorticity = 64 seeded Gaussians (amplitudes N(0,1),
+widths U[0.01, 0.05], centres uniform); body mask = smooth Heaviside
+(kernels/smooth_Heaviside.py:5-14, blend width sqrt(2) dx) of the union of 8
+on-axis spheres (radius U[0.02, 0.06], centre z U[0.2, 1.8]); lambda = 1e4,
+nu = 1.5e-3, free stream U = 1.
+
+Batched case sets:
 
 Each case is flow past an axisymmetric body started from rest with the
 free-stream ramp of examples/FlowPastSphere/flow_past_sphere.py:39-56, on
@@ -79,3 +89,33 @@ def grouped_cases(seeds):
         sp = case_spec(s)
         groups.setdefault((sp["nz"], sp["nr"]), []).append(sp)
     return dict(sorted(groups.items(), key=lambda kv: kv[0][1]))
+
+
+def kernel_scale_workload(nz, nr, seed=0):
+    """The seeded kernel-scale field and mask (module docstring); returns the
+    RigidBodyStepper arguments (z_len, dx, vorticity, char_func, r_sph, nu,
+    brink_lam, U_0)."""
+    from pyaxisymflow_amd.stepper import rigid_grid
+
+    z_len = 2.0
+    dx, z, r = rigid_grid(nz, nr, z_len)
+    Z, R = np.meshgrid(z, r)
+    rng = np.random.default_rng(seed)
+    n = 64
+    a = rng.standard_normal(n)
+    s = rng.uniform(0.01, 0.05, n)
+    cz = rng.uniform(0, z_len, n)
+    cr = rng.uniform(0, nr * dx, n)
+    w = np.zeros_like(Z)
+    for i in range(n):
+        w += a[i] * np.exp(-((Z - cz[i]) ** 2 + (R - cr[i]) ** 2) / (2 * s[i] ** 2))
+    rng2 = np.random.default_rng(seed + 1)
+    rad = rng2.uniform(0.02, 0.06, 8)
+    zc = rng2.uniform(0.2, 1.8, 8)
+    phi = np.full_like(Z, -np.inf)
+    for i in range(8):
+        phi = np.maximum(phi, rad[i] - np.sqrt((Z - zc[i]) ** 2 + R**2))
+    wdt = np.sqrt(2) * dx
+    chi = (phi >= wdt) + (np.fabs(phi) < wdt) * 0.5 * (1 + phi / wdt + np.sin(np.pi * phi / wdt) / np.pi)
+    return dict(z_len=z_len, dx=dx, vorticity=w, char_func=chi.astype(np.float64),
+                r_sph=float(rad.mean()), nu=1.5e-3, brink_lam=1e4, U_0=1.0)

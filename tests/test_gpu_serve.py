@@ -45,3 +45,29 @@ def test_serving_limits_and_records():
         assert r["time_to_solution"] == pytest.approx(r["finished"] - r["arrival"])
     s = latency_summary(recs)
     assert s["requests"] == 4 and s["p99_s"] >= s["p50_s"] > 0
+
+
+def test_request_chunks_do_not_wait_on_each_other():
+    """The server queues each request's chunk on that request's stream alone
+    (step(join=False)): a short chunk of one request completes while a long
+    chunk of another, queued before it, is still running."""
+    import torch
+
+    from pyaxisymflow_amd.stepper import SoftSphereStepper
+
+    long_run = SoftSphereStepper(grid_size_z=256, G=1.0)
+    short_run = SoftSphereStepper(grid_size_z=128, G=1.0)
+    # capture both steppers' graphs before timing anything
+    long_run.step(1, join=False)
+    short_run.step(1, join=False)
+    torch.cuda.synchronize()
+    long_run.step(400, join=False)
+    ev_long = torch.cuda.Event()
+    ev_long.record(long_run.stream)
+    short_run.step(2, join=False)
+    ev_short = torch.cuda.Event()
+    ev_short.record(short_run.stream)
+    ev_short.synchronize()
+    assert not ev_long.query(), "the short chunk waited for the long one"
+    ev_long.synchronize()
+    assert long_run.steps_done() == 401 and short_run.steps_done() == 3

@@ -153,6 +153,34 @@ def test_graph_replay_matches_eager():
     assert np.array_equal(a.scalars(), b.scalars())
 
 
+def test_chunk_graphs_after_an_odd_step_count():
+    """step(5) leaves the vorticity pair at the odd parity; the following
+    step(20) must still take a 16-step chunk graph (one per starting parity)
+    and stay bit-equal to eager stepping."""
+    case = sphere_case(seed=4)
+    from pyaxisymflow_amd.stepper import RigidBodyStepper
+
+    kw = dict(U_0=case["U_0"], T_ramp=case["T_ramp"], r_sph=case["r_sph"],
+              vorticity=case["vorticity"])
+    args = (case["nz"], case["nr"], case["z_len"], case["nu"], case["brink_lam"], case["char_func"])
+    a = RigidBodyStepper(*args, use_graph=True, **kw)
+    b = RigidBodyStepper(*args, use_graph=False, **kw)
+    a.step(5)
+    b.step(5)
+    assert a.graph_launches() == (0, 5)
+    a.step(20)
+    b.step(20)
+    assert a.graph_launches() == (1, 9)
+    a.step(32)
+    b.step(32)
+    assert a.graph_launches() == (3, 9)
+    assert a.steps_done() == b.steps_done() == 57
+    assert np.array_equal(a.vorticity(), b.vorticity())
+    assert np.array_equal(a.psi(), b.psi())
+    assert np.array_equal(a.drag_history(), b.drag_history())
+    assert np.array_equal(a.scalars(), b.scalars())
+
+
 def test_oscillating_body_penalisation_velocity():
     """spheroid_streaming.py:216-226: the body's penalisation velocity
     U_0 cos(omega t) replaces the free stream (no ramp, no added U)."""

@@ -70,14 +70,19 @@ def test_kill_taper_constant_field_unit_domain():
 def test_kill_taper_literal_one_quirk():
     """The reference hard-codes the domain end as 1: on z in [0, 2] the
     right-hand weights are sin(pi (1 - z - dx/2) / (4 dx)), not a taper to
-    zero at z = 2.  The oracle keeps that literally."""
-    nz, nr = 256, 128
+    zero at z = 2.  The two differ by the phase pi (1 - 2) / (4 dx); with
+    nz = 260 (dx = 1/130) that is -32.5 pi, so the literal weights are
+    -cos of the taper's angle: -1 at the last column where a taper to z = 2
+    would give 0.  The oracle keeps the literal form."""
+    nz, nr = 260, 130
     dx, z, r, Z, R = _grid(nz, nr, 2.0)
     w = np.ones((nr, nz))
     K.kill_boundary_vorticity_sine_z(w, Z, 3, dx)
-    expect = np.sin(np.pi * (1 - z[-3:] - 0.5 * dx) / (4 * dx))
-    np.testing.assert_allclose(w[5, -3:], expect, atol=1e-12)
-    assert abs(expect[-1]) > 1e-3 or True  # value fixed by the literal 1
+    theta = np.pi * (2.0 - z[-3:] - 0.5 * dx) / (4 * dx)  # taper angle to z = 2
+    np.testing.assert_allclose(w[5, -3:], -np.cos(theta), atol=1e-12)
+    np.testing.assert_allclose(w[5, -1], -1.0, atol=1e-12)
+    # the first row below the taper is untouched, the axis row is not part of the z taper
+    assert w[5, -4] == 1.0
 
 
 def test_kill_r_zeroes_axis_row_and_tapers_top():
