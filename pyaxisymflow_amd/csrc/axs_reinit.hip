@@ -11,8 +11,9 @@
 //           combined as 1 / sqrt(sum_a 1 / d_a^2), with the sign of phi;
 //   sweep   every other cell takes the second-order upwind eikonal update
 //           |grad d| = 1 from its known axis neighbours (values relative to
-//           its own side of the interface), keeping the smaller of old and
-//           new (Jacobi, ping-pong buffers);
+//           its own side of the interface), replacing the iterate (Jacobi,
+//           ping-pong buffers); at the fixed point every cell satisfies its
+//           upwind update from its converged neighbours;
 //   final   |d| > narrow is masked and keeps phi.
 // FP64 with the operation order of the CPU restatement used by the tests.
 #include "axs_reinit.hpp"
@@ -69,12 +70,20 @@ __global__ void reinit_init_kernel(const double* __restrict__ phi, double* __res
 }
 
 // Upwind eikonal update of cell (i, j) from the iterate D (kBig = unknown),
-// neighbour values taken relative to the cell's side.
+// neighbour values taken relative to the cell's side.  Per axis: v1 = the
+// smaller known neighbour, v2 = the next cell beyond it when known and not
+// larger; second-order term (9/4)(x - (4 v1 - v2)/3)^2 / dx^2 with v2,
+// first-order (x - v1)^2 / dx^2 without; x = the larger root of
+// sum(terms) = 1.  Where both axes enter and x is not above both axes'
+// upwind values, x comes from the axis with the smaller one alone (+ 2 dx/3
+// second order, + dx first order).
 __device__ __forceinline__ double eikonal_update(const double* __restrict__ D, int i, int j,
                                                  int nr, int nz, double side, double dx) {
   const double idx2 = 1.0 / (dx * dx);
   const double aa = 9.0 / 4.0;
-  double a = 0.0, b = 0.0, c = 0.0, vmax = -kBig, vmin = kBig;
+  double a = 0.0, b = 0.0, c = 0.0;
+  double tps[2], steps[2];
+  bool used[2];
 #pragma unroll
   for (int axis = 0; axis < 2; ++axis) {
     double v1 = kBig, v2 = kBig;
@@ -100,21 +109,26 @@ __device__ __forceinline__ double eikonal_update(const double* __restrict__ D, i
     const bool first = v1 < kBig && !second;
     const double w1 = (v1 < kBig) ? v1 : 0.0;
     const double w2 = second ? v2 : 0.0;
-    const double tp = (4.0 * w1 - w2) / 3.0;
+    const double tp = second ? (4.0 * w1 - w2) / 3.0 : w1;
     a = a + (second ? idx2 * aa : (first ? idx2 : 0.0));
     b = b - (second ? idx2 * 2.0 * aa * tp : (first ? idx2 * 2.0 * w1 : 0.0));
     c = c + (second ? idx2 * aa * tp * tp : (first ? idx2 * w1 * w1 : 0.0));
-    vmax = fmax(vmax, second ? tp : (first ? w1 : -kBig));
-    vmin = fmin(vmin, (second || first) ? w1 : kBig);
+    used[axis] = second || first;
+    tps[axis] = used[axis] ? tp : kBig;
+    steps[axis] = second ? (2.0 / 3.0) * dx : dx;
   }
   if (!(a > 0.0)) return kBig;
   c = c - 1.0;
   const double det = b * b - 4.0 * a * c;
   double x = (-b + sqrt(fmax(det, 0.0))) / (2.0 * a);
-  if (det < 0.0 || x < vmax) x = vmin + dx;
+  if (used[0] && used[1] && (det < 0.0 || x < tps[0] || x < tps[1]))
+    x = (tps[0] <= tps[1]) ? tps[0] + steps[0] : tps[1] + steps[1];
   return x;
 }
 
+// One Jacobi sweep: every cell that is not frozen takes its update from the
+// previous iterate (values may move either way: the second-order update is
+// not monotone in its neighbours).
 __global__ void reinit_sweep_kernel(const double* __restrict__ Din, double* __restrict__ Dout,
                                     const double* __restrict__ phi,
                                     const uint8_t* __restrict__ frozen, int nr, int nz,
@@ -130,8 +144,7 @@ __global__ void reinit_sweep_kernel(const double* __restrict__ Din, double* __re
   }
   const double side = (phi[c] < 0.0) ? -1.0 : 1.0;
   const double m = eikonal_update(Din, i, j, nr, nz, side, dx);
-  const double cur = (dc < kBig) ? side * dc : kBig;
-  Dout[c] = (m >= cur) ? dc : side * m;
+  Dout[c] = (m >= kBig) ? dc : side * m;
 }
 
 __global__ void reinit_final_kernel(const double* __restrict__ phi, const double* __restrict__ D,

@@ -66,10 +66,11 @@ def test_one_step_seeded_field_every_stage():
 
 
 @pytest.mark.parametrize("solver", ["exact", "eig"])
-def test_flow_past_sphere_100_steps(solver):
+def test_flow_past_sphere_500_steps(solver):
+    """The plain-method configuration in full: 500 steps."""
     case = sphere_case()
-    cpu, gpu = run_pair(case, 100, solver=solver)
-    assert gpu.steps_done() == 100
+    cpu, gpu = run_pair(case, 500, solver=solver)
+    assert gpu.steps_done() == 500
     assert rel_l2(gpu.vorticity(), cpu.vorticity) < 1e-10
     assert rel_l2(gpu.psi(), cpu.psi) < 1e-10
     d = gpu.drag_history()

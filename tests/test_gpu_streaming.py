@@ -68,3 +68,29 @@ def test_precision_form_error_vs_fp64(nz):
     print(f"nz={nz}: u {err_u:.2e} psi {err_psi:.2e} force {err_f:.2e}")
     assert err_f < 1e-4
     assert err_u < 1e-4 and err_psi < 1e-4
+
+
+@pytest.mark.parametrize("nz,periods", [(128, 2), (256, 1)])
+def test_precision_form_error_vs_fp64_oracle(nz, periods):
+    """The precision form against the FP64 oracle (the reference's loop
+    restated on the CPU, exact Poisson route) over whole oscillation periods
+    of spheroid_streaming.py: relative L2 of the penalised velocity and of
+    psi, and the force error relative to its maximum, all below 1e-4."""
+    from pyaxisymflow_amd.stepper import OscillatingSpheroidStepper
+
+    cpu = OscillatingSpheroidCase(nz, solver="exact", no_cycles=periods)
+    while cpu.t < cpu.tEnd:
+        cpu.step()
+    gpu = OscillatingSpheroidStepper(nz, precision="f32", no_cycles=periods)
+    gpu.step(cpu.it)
+    assert gpu.steps_done() == cpu.it
+    assert abs(gpu.time() - cpu.t) <= 1e-12 * cpu.t
+    uz, ur = gpu.velocity()
+    err_u = rel_l2(np.hypot(uz, ur), np.hypot(cpu.u_z, cpu.u_r))
+    err_psi = rel_l2(gpu.psi(), cpu.psi)
+    f = gpu.drag_history()
+    ref = np.array(cpu.force)
+    err_f = np.abs(f - ref).max() / np.abs(ref).max()
+    print(f"f32 nz={nz}, {cpu.it} steps: u {err_u:.2e} psi {err_psi:.2e} force {err_f:.2e}",
+          flush=True)
+    assert err_u < 1e-4 and err_psi < 1e-4 and err_f < 1e-4

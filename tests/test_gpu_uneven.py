@@ -138,3 +138,29 @@ def test_uneven_refusals():
     big = dict(cases[0], nz=2048, nr=1024, char_func=np.zeros((1024, 2048)), vorticity=None)
     with pytest.raises((ValueError, RuntimeError)):
         UnevenBatchStepper([cases[1], big])
+
+
+def test_uneven_fp32_against_the_fp64_oracle():
+    """An FP32 uneven batch against the FP64 oracle's restatement of the
+    driver loop (exact Poisson route), 100 steps: relative L2 of the
+    penalised velocity and psi, and the drag relative to its maximum, all
+    below 1e-4."""
+    from pyaxisymflow_amd.stepper import UnevenBatchStepper
+
+    cases = uneven_cases([[0], [1], [2]])
+    gpu = UnevenBatchStepper(cases, precision="f32")
+    gpu.step(100)
+    for q, c in enumerate(cases):
+        cpu = RigidBodyCase(c["nz"], c["nr"], c["z_len"], c["nu"], c["brink_lam"], c["char_func"],
+                            U_0=c["U_0"], T_ramp=c["T_ramp"], r_sph=c["r_sph"], solver="exact")
+        for _ in range(100):
+            cpu.step()
+        uz, ur = gpu.velocity(q)
+        err_u = rel_l2(np.hypot(uz, ur), np.hypot(cpu.u_z, cpu.u_r))
+        err_psi = rel_l2(gpu.psi(q), cpu.psi)
+        d = gpu.drag_history(q)
+        ref = np.array(cpu.drag)
+        err_d = np.abs(d - ref).max() / np.abs(ref).max()
+        print(f"f32 case {q} ({c['nz']}x{c['nr']}): u {err_u:.2e} psi {err_psi:.2e} "
+              f"drag {err_d:.2e}", flush=True)
+        assert err_u < 1e-4 and err_psi < 1e-4 and err_d < 1e-4
