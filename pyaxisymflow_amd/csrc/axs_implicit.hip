@@ -12,9 +12,9 @@
 // 1 + nu dt (2 / dx^2 + 1 / r^2 - lambda_k) against off-diagonals summing to
 // nu dt 2 / dx^2), so the column pass of the Stokes solve (segmented Thomas
 // elimination without pivoting) applies with other coefficient tables.
-// A fast sine transform would need complex FFTs of nz + 1 points (257 is
-// prime, 2049 = 3 x 683 for the power-of-two grids), so the row passes are
-// the dense product with S, O(nz^2) per row, on LDS tiles.
+// The row passes are fast sine transforms: one complex DFT of nz + 1 points
+// per row (axs_anysize.hip; 257 is prime and 2049 = 3 x 683 for the
+// power-of-two grids, so those lengths take Bluestein's chirp-z form).
 #pragma clang fp contract(fast)
 
 #include "axs_common.hpp"
@@ -27,122 +27,8 @@
 
 namespace axs {
 
-// out[i, k] = sum_j in[i, j] S[j, k] for rows i < rows (S symmetric, nz x
-// nz).  Workgroup: a BM-row x 64-column output tile, 256 threads, each
-// thread BM / 16 rows x 4 columns (rows ty + 16 h, columns tx + 16 c); j
-// advances in chunks of 16 staged in LDS, two LDS buffers so the global
-// loads of the next chunk are in flight while this one is multiplied (one
-// barrier per chunk).  Tiles that reach past the grid read zeros there and
-// skip those stores.
-constexpr int kDstBN = 64, kDstBK = 16;
-
-template <int BM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) dst1_rows_kernel(const double* __restrict__ in,
-                                                         double* __restrict__ out,
-                                                         const double* __restrict__ S, int rows,
-                                                         int nz) {
-  constexpr int RH = BM / 16;                 // rows per thread
-  constexpr int NA = BM * kDstBK / 256;       // A elements per thread per chunk
-  constexpr int NB = kDstBK * kDstBN / 256;   // B elements per thread per chunk
-  __shared__ double sA[2][kDstBK][BM + 1];
-  __shared__ double sB[2][kDstBK][kDstBN];
-  const int t = threadIdx.x;
-  const int tx = t & 15, ty = t >> 4;
-  const int col0 = blockIdx.x * kDstBN;
-  const int row0 = blockIdx.y * BM;
-  double acc[RH][4] = {};
-  double va[NA], vb[NB];
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int q = 0; q < NA; ++q) {  // A: BM x 16, element e = t + 256 q
-      const int e = t + 256 * q, r = e >> 4, kk = e & 15;
-      const int gi = row0 + r, gj = k0 + kk;
-      va[q] = (gi < rows && gj < nz) ? in[(int64_t)gi * nz + gj] : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < NB; ++q) {  // B: 16 x 64, element e = t + 256 q
-      const int e = t + 256 * q, kk = e >> 6, cc = e & 63;
-      const int gj = k0 + kk, gk = col0 + cc;
-      vb[q] = (gj < nz && gk < nz) ? S[(int64_t)gj * nz + gk] : 0.0;
-    }
-  };
-  auto stage = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < NA; ++q) {
-      const int e = t + 256 * q;
-      sA[buf][e & 15][e >> 4] = va[q];
-    }
-#pragma unroll
-    for (int q = 0; q < NB; ++q) {
-      const int e = t + 256 * q;
-      sB[buf][e >> 6][e & 63] = vb[q];
-    }
-  };
-  load(0);
-  stage(0);
-  __syncthreads();
-  int buf = 0;
-  for (int k0 = 0; k0 < nz; k0 += kDstBK) {
-    const bool more = k0 + kDstBK < nz;
-    if (more) load(k0 + kDstBK);
-    const double(*pa)[BM + 1] = sA[buf];
-    const double(*pb)[kDstBN] = sB[buf];
-#pragma unroll 4
-    for (int kk = 0; kk < kDstBK; ++kk) {
-      double a[RH], bv[4];
-#pragma unroll
-      for (int h = 0; h < RH; ++h) a[h] = pa[kk][ty + 16 * h];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) bv[c] = pb[kk][tx + 16 * c];
-#pragma unroll
-      for (int h = 0; h < RH; ++h)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[h][c] = fma(a[h], bv[c], acc[h][c]);
-    }
-    if (more) {
-      // the other buffer was last read in the previous chunk, before the
-      // barrier that ended it
-      stage(buf ^ 1);
-      __syncthreads();
-      buf ^= 1;
-    }
-  }
-#pragma unroll
-  for (int h = 0; h < RH; ++h) {
-    const int gi = row0 + ty + 16 * h;
-    if (gi >= rows) continue;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int gk = col0 + tx + 16 * c;
-      if (gk < nz) out[(int64_t)gi * nz + gk] = acc[h][c];
-    }
-  }
-}
-
-// Tile height: 64 rows when that still gives every CU a workgroup, else 32
-// (AXS_DST_BM = 32 or 64 pins it).
-static int dst_tile_rows(int rows, int nz) {
-  static const int pin = [] {
-    const char* v = std::getenv("AXS_DST_BM");
-    return v ? std::atoi(v) : 0;
-  }();
-  if (pin == 32 || pin == 64) return pin;
-  return (int64_t)ceil_div(nz, kDstBN) * ceil_div(rows, 64) >= 256 ? 64 : 32;
-}
-
-static hipError_t launch_dst1_rows(const double* in, double* out, const double* S, int rows,
-                                   int nz, hipStream_t s) {
-  if (dst_tile_rows(rows, nz) == 64)
-    dst1_rows_kernel<64><<<dim3(ceil_div(nz, kDstBN), ceil_div(rows, 64)), 256, 0, s>>>(
-        in, out, S, rows, nz);
-  else
-    dst1_rows_kernel<32><<<dim3(ceil_div(nz, kDstBN), ceil_div(rows, 32)), 256, 0, s>>>(
-        in, out, S, rows, nz);
-  return hipGetLastError();
-}
-
 ImplicitPlan::~ImplicitPlan() {
-  if (S) (void)hipFree(S);
+  if (em) (void)hipFree(em);
 }
 
 hipError_t ImplicitPlan::init(int nr, int nz, double dx, double nu_dt_, int batch) {
@@ -173,32 +59,31 @@ hipError_t ImplicitPlan::init(int nr, int nz, double dx, double nu_dt_, int batc
     const double sn = std::sin(M_PI * (double)(k + 1) / (2.0 * (nz + 1)));
     lam[k] = -nu_dt * (-(4.0 / (dx * dx)) * sn * sn);
   }
-  // S[j, k] = sqrt(2 / (nz + 1)) sin(pi (j+1)(k+1) / (nz + 1)), the angle
-  // reduced modulo 2 pi in integers first
-  std::vector<double> h((size_t)nz * nz);
-  const double norm = std::sqrt(2.0 / (nz + 1));
-  const int64_t period = 2 * (int64_t)(nz + 1);
-  for (int j = 0; j < nz; ++j)
-    for (int k = 0; k < nz; ++k) {
-      const int64_t m = ((int64_t)(j + 1) * (k + 1)) % period;
-      h[(size_t)j * nz + k] = norm * std::sin(M_PI * (double)m / (double)(nz + 1));
-    }
   hipError_t e = tri.set_tridiag(a, b, c, lam);
   if (e == hipSuccess && batch > 1) {  // the spectral buffer of a whole batch
     (void)hipFree(tri.spec);
     tri.spec = nullptr;
     e = hipMalloc(reinterpret_cast<void**>(&tri.spec), sizeof(double) * (size_t)nr * nz * batch);
   }
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&S), sizeof(double) * h.size());
-  if (e == hipSuccess) e = hipMemcpy(S, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice);
+  // DST-I through a DFT of nz + 1 points; em[k] = e^{-i pi k / (nz + 1)}
+  if (e == hipSuccess) e = dst.init(nz + 1);
+  if (e != hipSuccess) return e;
+  std::vector<Cplx<double>> h_em(nz + 1);
+  for (int k = 0; k <= nz; ++k) {
+    const double ang = -M_PI * (double)k / (double)(nz + 1);
+    h_em[k] = {std::cos(ang), std::sin(ang)};
+  }
+  e = hipMalloc(reinterpret_cast<void**>(&em), sizeof(Cplx<double>) * h_em.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(em, h_em.data(), sizeof(Cplx<double>) * h_em.size(), hipMemcpyHostToDevice);
   return e;
 }
 
 hipError_t ImplicitPlan::launch(double* omega, int batch, hipStream_t s) {
   const int rows = tri.nr * batch;
-  hipError_t e = launch_dst1_rows(omega, tri.spec, S, rows, tri.nz, s);
+  hipError_t e = launch_any_dst1_rows(dst, em, omega, tri.spec, rows, tri.nz, s);
   if (e == hipSuccess) e = launch_tridiag<double>(tri, tri.spec, s, batch);
-  if (e == hipSuccess) e = launch_dst1_rows(tri.spec, omega, S, rows, tri.nz, s);
+  if (e == hipSuccess) e = launch_any_dst1_rows(dst, em, tri.spec, omega, rows, tri.nz, s);
   return e;
 }
 
@@ -217,7 +102,13 @@ axs_status axs_implicit_diffusion_plan_create(int nr, int nz, double dx, double 
   AXS_CHECK_ARG(plan_out, "null plan_out");
   *plan_out = nullptr;
   AXS_CHECK_ARG(nr >= 3 && nz >= 2, "need nr >= 3 and nz >= 2");
-  AXS_CHECK_ARG(nz <= 4096 && nr <= 4096, "grid too large for the row/column passes");
+  AXS_CHECK_ARG(nr <= 4096, "nr > 4096: too tall for the column pass");
+  {
+    bool blue = false;
+    AXS_CHECK_ARG(any_dft_length(nz + 1, &blue) <= kAnyMaxL,
+                  "nz too large for the sine transform rows (nz + 1 <= 4096, or <= 8192 when "
+                  "nz + 1 is 2,3,5-smooth)");
+  }
   AXS_CHECK_ARG(dx > 0.0, "dx must be positive");
   AXS_CHECK_ARG(nu_dt >= 0.0 && std::isfinite(nu_dt), "nu * dt must be finite and >= 0");
   axs_implicit_plan* p = new (std::nothrow) axs_implicit_plan();

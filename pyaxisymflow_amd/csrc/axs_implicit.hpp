@@ -8,7 +8,8 @@ namespace axs {
 
 struct ImplicitPlan {
   StokesPlan tri;       // column-pass tables (diffusion bands) and the spectral buffer
-  double* S = nullptr;  // orthonormal DST-I matrix, nz x nz
+  AnyDft dst;           // DFT of nz + 1 points for the sine transform rows
+  Cplx<double>* em = nullptr;  // e^{-i pi k / (nz + 1)}, k <= nz
   double nu_dt = 0.0;
   // spectral buffer of batch (nr, nz) fields
   hipError_t init(int nr, int nz, double dx, double nu_dt, int batch = 1);

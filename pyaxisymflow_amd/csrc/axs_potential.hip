@@ -147,20 +147,6 @@ hipError_t PotentialPlan::solve(double* phi, const double* rhs, hipStream_t s) {
   return launch_dct3_rows<double>(pl, pl.spec, phi, nr, s);
 }
 
-static bool radices_235(int M, FftPlanArgs* p) {
-  p->M = M;
-  p->nstages = 0;
-  int m = M;
-  for (int r : {4, 2, 3, 5}) {
-    while (m % r == 0) {
-      if (p->nstages >= kMaxFftStages) return false;
-      p->radix[p->nstages++] = r;
-      m /= r;
-    }
-  }
-  return m == 1;
-}
-
 }  // namespace axs
 
 using namespace axs;
@@ -177,12 +163,12 @@ axs_status axs_potential_plan_create(int nr, int nz, double dx, int bc_type,
   *plan_out = nullptr;
   AXS_CHECK_ARG(bc_type == AXS_BC_NEUMANN_Z_AND_R,
                 "only homogenous_neumann_along_z_and_r is supported");
-  AXS_CHECK_ARG(nr >= 3 && nz >= 4 && nz % 2 == 0, "need nr >= 3 and even nz >= 4");
-  AXS_CHECK_ARG(nz <= 4096 && nr <= 4096, "grid too large for the row/column passes");
+  AXS_CHECK_ARG(nr >= 3 && nz >= 4, "need nr >= 3 and nz >= 4");
+  AXS_CHECK_ARG(nr <= 4096, "nr > 4096: too tall for the column pass");
   AXS_CHECK_ARG(dx > 0.0, "dx must be positive");
-  FftPlanArgs fp;
-  if (!radices_235(nz / 2, &fp)) {
-    set_error("nz/2 must factor into 2, 3 and 5");
+  if (!stokes_nz_supported(nz)) {
+    set_error("nz too large for the row transforms (nz <= 8192 with nz/2 2,3,5-smooth, "
+              "else nz <= 4096)");
     return AXS_ERR_UNSUPPORTED;
   }
   axs_potential_plan* p = new (std::nothrow) axs_potential_plan();
@@ -190,7 +176,6 @@ axs_status axs_potential_plan_create(int nr, int nz, double dx, int bc_type,
     set_error("out of host memory");
     return AXS_ERR_ALLOC;
   }
-  p->impl.pl.fft = fp;
   hipError_t e = p->impl.init(nr, nz, dx);
   if (e != hipSuccess) {
     set_error(std::string("potential plan: ") + hipGetErrorString(e));

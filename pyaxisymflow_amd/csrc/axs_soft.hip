@@ -254,16 +254,7 @@ struct SoftStepper {
     buf = b;
     plan = new (std::nothrow) StokesPlan();
     if (!plan) return hipErrorOutOfMemory;
-    FftPlanArgs fp{};
-    fp.M = cfg.nz / 2;
-    int m = fp.M;
-    for (int rdx : {4, 2, 3, 5})
-      while (m % rdx == 0 && fp.nstages < kMaxFftStages) {
-        fp.radix[fp.nstages++] = rdx;
-        m /= rdx;
-      }
-    if (m != 1) return hipErrorInvalidValue;
-    plan->fft = fp;
+    // the plan picks the half-length transforms or the any-length ones
     hipError_t e = plan->init(cfg.nr, cfg.nz, cfg.dx);
     if (e != hipSuccess) return e;
     const size_t n = (size_t)cfg.nr * cfg.nz, nd = 2 * n, B = sizeof(double);
@@ -486,6 +477,8 @@ axs_status axs_soft_stepper_create(const axs_soft_config* c, const axs_soft_buff
   AXS_CHECK_ARG(c && b && out, "null pointer");
   *out = nullptr;
   AXS_CHECK_ARG(c->nr >= 4 && c->nz >= 8 && c->nz % 2 == 0, "need nr >= 4 and even nz >= 8");
+  AXS_CHECK_ARG(c->nr <= 4096 && stokes_nz_supported(c->nz),
+                "grid too large for the row/column passes");
   AXS_CHECK_ARG(c->nz == 2 * c->nr,
                 "the least-squares extrapolation uses z as both grid axes: need nz = 2 nr");
   AXS_CHECK_ARG(c->kill_width >= 2 && 2 * c->kill_width <= c->nz && c->kill_width < c->nr,

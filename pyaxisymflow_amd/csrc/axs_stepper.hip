@@ -115,17 +115,7 @@ hipError_t RigidStepperBase::init(const RigidConfig* cfgs, int nbatch, const Rig
   buf = b;
   plan = new (std::nothrow) StokesPlan();
   if (!plan) return hipErrorOutOfMemory;
-  FftPlanArgs fp{};
-  fp.M = cfg.nz / 2;
-  fp.nstages = 0;
-  int m = fp.M;
-  for (int rdx : {4, 2, 3, 5})
-    while (m % rdx == 0 && fp.nstages < kMaxFftStages) {
-      fp.radix[fp.nstages++] = rdx;
-      m /= rdx;
-    }
-  if (m != 1) return hipErrorInvalidValue;
-  plan->fft = fp;
+  // the plan picks the half-length transforms or the any-length ones
   hipError_t e = plan->init(cfg.nr, cfg.nz, cfg.dx);
   if (e != hipSuccess) return e;
   if (cfg.implicit_diffusion) {
@@ -735,7 +725,9 @@ axs_status axs_rigid_batch_create(const axs_rigid_config* cfgs, int batch,
   *out = nullptr;
   AXS_CHECK_ARG(batch >= 1 && batch <= 4096, "batch must be in [1, 4096]");
   const axs_rigid_config& c0 = cfgs[0];
-  AXS_CHECK_ARG(c0.nr >= 4 && c0.nz >= 8 && c0.nz % 2 == 0, "need nr >= 4 and even nz >= 8");
+  AXS_CHECK_ARG(c0.nr >= 4 && c0.nz >= 8, "need nr >= 4 and nz >= 8");
+  AXS_CHECK_ARG(c0.nr <= 4096 && stokes_nz_supported(c0.nz),
+                "grid too large for the row/column passes");
   AXS_CHECK_ARG(c0.kill_width >= 2 && 2 * c0.kill_width <= c0.nz && c0.kill_width < c0.nr,
                 "bad kill_width for the grid");
   AXS_CHECK_ARG(c0.precision == AXS_F64 || c0.precision == AXS_F32, "bad precision");
@@ -748,7 +740,7 @@ axs_status axs_rigid_batch_create(const axs_rigid_config* cfgs, int batch,
                       c.precision == c0.precision && c.moving_body == c0.moving_body &&
                       c.average_psi == c0.average_psi,
                   "cases of a batch must share kill width, history, precision and flags");
-    AXS_CHECK_ARG(c.nr >= 4 && c.nz >= 8 && c.nz % 2 == 0, "need nr >= 4 and even nz >= 8");
+    AXS_CHECK_ARG(c.nr >= 4 && c.nz >= 8, "need nr >= 4 and nz >= 8");
     AXS_CHECK_ARG(2 * c.kill_width <= c.nz && c.kill_width < c.nr, "bad kill_width for the grid");
     AXS_CHECK_ARG(c.dx > 0 && c.nu > 0 && c.CFL > 0, "dx, nu and CFL must be positive");
     AXS_CHECK_ARG(!c.moving_body || (c.body_aspect2 > 0 && c.moll_zone > 0),

@@ -51,8 +51,17 @@ static hipError_t upload(V** dev, const std::vector<V>& host) {
 
 StokesPlan::~StokesPlan() {
   for (void* p : {(void*)tw, (void*)e1, (void*)e2, (void*)a, (void*)b, (void*)c,
-                  (void*)lam, (void*)rvec, (void*)cseg, (void*)spec})
+                  (void*)lam, (void*)rvec, (void*)cseg, (void*)spec, (void*)ge2})
     if (p) (void)hipFree(p);
+  delete any;
+}
+
+bool stokes_nz_supported(int nz) {
+  if (nz < 2) return false;
+  FftPlanArgs fp;
+  if (nz % 2 == 0 && nz <= 8192 && factor_radices(nz / 2, &fp)) return true;
+  bool blue = false;
+  return any_dft_length(nz, &blue) <= kAnyMaxL;
 }
 
 int StokesPlan::tri_threads() const { return tri_K * tri_P; }
@@ -209,7 +218,8 @@ hipError_t StokesPlan::init(int nr_, int nz_, double dx_, int bc_) {
     }
   } else {  // DFT modes of the periodic P_z in the packed real layout
     for (int p = 0; p < nz; ++p) {
-      const int k = (p == 0) ? 0 : (p == nz - 1) ? nz / 2 : (p + 1) / 2;
+      // packed layout [X_0, Re X_1, Im X_1, ..., (X_{nz/2} when nz is even)]
+      const int k = (p == 0) ? 0 : (p == nz - 1 && nz % 2 == 0) ? nz / 2 : (p + 1) / 2;
       const double s = std::sin(M_PI * (double)k / (double)nz);
       h_lam[p] = (4.0 / (dx * dx)) * s * s;
     }
@@ -221,6 +231,19 @@ hipError_t StokesPlan::init_transforms(int nr_, int nz_, double dx_) {
   nr = nr_;
   nz = nz_;
   dx = dx_;
+  general = !(nz % 2 == 0 && factor_radices(nz / 2, &fft));
+  hipError_t e;
+  if (general) {
+    any = new (std::nothrow) AnyDft();
+    if (!any) return hipErrorOutOfMemory;
+    if ((e = any->init(nz)) != hipSuccess) return e;
+    std::vector<Cplx<double>> h_ge2(nz);
+    for (int k = 0; k < nz; ++k) {
+      const double ang = -M_PI * (double)k / (2.0 * nz);
+      h_ge2[k] = {std::cos(ang), std::sin(ang)};
+    }
+    if ((e = upload(&ge2, h_ge2)) != hipSuccess) return e;
+  }
   const int M = nz / 2;
   // FFT twiddles e^{-2 pi i t/M}
   std::vector<Cplx<double>> h_tw(M), h_e1(M + 1), h_e2(M + 1);
@@ -242,7 +265,6 @@ hipError_t StokesPlan::init_transforms(int nr_, int nz_, double dx_) {
   for (int i = 0; i < nr; ++i) h_r[i] = i * step + start;
   if (nr > 1) h_r[nr - 1] = stop;
   host_r = h_r;
-  hipError_t e;
   if ((e = upload(&tw, h_tw)) != hipSuccess) return e;
   if ((e = upload(&e1, h_e1)) != hipSuccess) return e;
   if ((e = upload(&e2, h_e2)) != hipSuccess) return e;
@@ -1165,6 +1187,9 @@ static int fft_block(int M) {
 template <typename T>
 hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const double* rscale,
                             int rows, hipStream_t s) {
+  if (pl.general)
+    return launch_any_dct2_rows<T>(pl, src, nullptr, dst, rscale, nullptr, nullptr, 0, false, 1,
+                                   nullptr, s);
   const int M = pl.nz / 2;
   const int bs = fft_block(M);
   // worst stage is radix 2 (M/2 butterflies); 2*per*bs >= M covers the row loads
@@ -1203,6 +1228,9 @@ template <typename T>
 hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered, T* dst,
                                   const double* wz, const double* wr, int w, hipStream_t s,
                                   int batch, const StepFinalize* fin) {
+  if (pl.general)
+    return launch_any_dct2_rows<T>(pl, src, tapered, dst, pl.rvec, wz, wr, w, true, batch, fin,
+                                   s);
   const StepFinalize f = fin ? *fin : StepFinalize{};
   const int gx = pl.nr + (f.partials ? 1 : 0);
   const int M = pl.nz / 2;
@@ -1245,6 +1273,7 @@ hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered
 template <typename T>
 hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, hipStream_t s,
                             int batch) {
+  if (pl.general) return launch_any_dct3_rows<T>(pl, src, dst, rows, batch, s);
   const int M = pl.nz / 2;
   const int bs = fft_block(M);
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
@@ -1279,6 +1308,7 @@ hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows
 
 hipError_t launch_rfft_rows(const StokesPlan& pl, const double* src, double* dst,
                             const double* rscale, int rows, hipStream_t s, bool inverse) {
+  if (pl.general) return launch_any_rfft_rows(pl, src, dst, rscale, rows, inverse, s);
   const int M = pl.nz / 2;
   const int bs = fft_block(M);
   const int per = std::max(1, (M / 2 + bs - 1) / bs);
@@ -1448,12 +1478,12 @@ axs_status axs_stokes_plan_create(int nr, int nz, double dx, int bc_type,
   AXS_CHECK_ARG(bc_type == AXS_BC_NEUMANN_Z_AND_R || bc_type == AXS_BC_NEUMANN_R_PERIODIC_Z ||
                     bc_type == AXS_BC_DIRICHLET_R_PERIODIC_Z,
                 "unknown bc_type");
-  AXS_CHECK_ARG(nr >= 3 && nz >= 4 && nz % 2 == 0, "need nr >= 3 and even nz >= 4");
-  AXS_CHECK_ARG(nz <= 4096 && nr <= 4096, "grid too large for the row/column passes");
+  AXS_CHECK_ARG(nr >= 3 && nz >= 4, "need nr >= 3 and nz >= 4");
+  AXS_CHECK_ARG(nr <= 4096, "nr > 4096: too tall for the column pass");
   AXS_CHECK_ARG(dx > 0.0, "dx must be positive");
-  FftPlanArgs fp;
-  if (!factor_radices(nz / 2, &fp)) {
-    set_error("nz/2 must factor into 2, 3 and 5");
+  if (!stokes_nz_supported(nz)) {
+    set_error("nz too large for the row transforms (nz <= 8192 with nz/2 2,3,5-smooth, "
+              "else nz <= 4096)");
     return AXS_ERR_UNSUPPORTED;
   }
   axs_stokes_plan* p = new (std::nothrow) axs_stokes_plan();
@@ -1461,7 +1491,6 @@ axs_status axs_stokes_plan_create(int nr, int nz, double dx, int bc_type,
     set_error("out of host memory");
     return AXS_ERR_ALLOC;
   }
-  p->impl.fft = fp;
   hipError_t e = p->impl.init(nr, nz, dx, bc_type);
   if (e != hipSuccess) {
     set_error(std::string("stokes plan: ") + hipGetErrorString(e));

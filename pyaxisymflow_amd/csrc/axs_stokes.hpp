@@ -2,6 +2,7 @@
 // stepper.
 #pragma once
 
+#include "axs_anydft.hpp"
 #include "axs_fft.hpp"
 #include "axs_ragged.hpp"
 
@@ -22,6 +23,12 @@ struct StokesPlan {
   int nr = 0, nz = 0;
   double dx = 0.0;
   FftPlanArgs fft{};
+  // general: nz odd or nz / 2 with a prime factor above 5; the row passes
+  // then run the any-length transforms (axs_anysize.hip) with `any` (length
+  // nz) and ge2[k] = e^{-i pi k / (2 nz)}, k < nz
+  bool general = false;
+  AnyDft* any = nullptr;
+  Cplx<double>* ge2 = nullptr;
   // device tables
   Cplx<double>* tw = nullptr;  // e^{-2 pi i t/M}, t < M
   Cplx<double>* e1 = nullptr;  // e^{-2 pi i k/nz}, k <= M
@@ -105,6 +112,23 @@ hipError_t launch_rfft_rows(const StokesPlan& pl, const double* src, double* dst
 // of nr <= 4096 contiguous rows, in place; cseg = pivot_table(..., nz = 1, L = 16).
 hipError_t launch_tridiag_one(double* d, const double* a, const double* b, const double* c,
                               const double* lam, const double* cseg, int nr, hipStream_t s);
+// Row passes of a general plan (any nz; see StokesPlan::general).
+template <typename T>
+hipError_t launch_any_dct2_rows(const StokesPlan& pl, const T* src, T* tapered, T* dst,
+                                const double* rscale, const double* wz, const double* wr, int w,
+                                bool taper, int batch, const StepFinalize* fin, hipStream_t s);
+template <typename T>
+hipError_t launch_any_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, int batch,
+                                hipStream_t s);
+hipError_t launch_any_rfft_rows(const StokesPlan& pl, const double* src, double* dst,
+                                const double* rscale, int rows, bool inverse, hipStream_t s);
+// Orthonormal DST-I of `rows` rows of nz (D of length nz + 1, em[k] =
+// e^{-i pi k / (nz + 1)}, k <= nz); src and dst must not alias.
+hipError_t launch_any_dst1_rows(const AnyDft& D, const Cplx<double>* em, const double* src,
+                                double* dst, int rows, int nz, hipStream_t s);
+// Largest nz a plan takes: the half-length transforms cover nz <= 8192 with
+// nz / 2 2,3,5-smooth; the any-length ones need their FFT length <= kAnyMaxL.
+bool stokes_nz_supported(int nz);
 // True when rows of half-length M run a one-wave register transform
 // (M = 128, 192, 256, 384, 512, 1024; M = 2048 runs two waves per row).
 bool wave_rows(int M);

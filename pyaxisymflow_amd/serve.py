@@ -24,28 +24,10 @@ import numpy as np
 from pyaxisymflow_amd import _lib
 
 
-def smooth_size(n, lo=None, hi=None):
-    """The 5-smooth integer nearest n within [lo, hi] (the z-transform
-    factors nz / 2 = nr into radices 2, 3, 5)."""
-    def smooth(m):
-        for p in (2, 3, 5):
-            while m % p == 0:
-                m //= p
-        return m == 1
-
-    lo = 1 if lo is None else lo
-    hi = 1 << 30 if hi is None else hi
-    for d in range(0, n):
-        for m in (n - d, n + d):
-            if lo <= m <= hi and smooth(m):
-                return m
-    raise ValueError(f"no 5-smooth size near {n} in [{lo}, {hi}]")
-
-
 def request_set(n, seed=0, rate=1.0, steps=2000, nr_range=(128, 1024)):
     """Synthetic requests: Poisson arrivals at `rate` per second; nr =
-    clip(round(lognormal(ln 256, 0.5)), 128, 1024) rounded to the nearest
-    5-smooth size, nz = 2 nr; G ~ U[0.1, 10]; rho_f = 1, nu = 1e-3,
+    clip(round(lognormal(ln 256, 0.5)), 128, 1024), nz = 2 nr (any size: the
+    row transforms take every nz); G ~ U[0.1, 10]; rho_f = 1, nu = 1e-3,
     r_ball = 0.15, the driver's Taylor-Green vorticity."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, n)
@@ -53,7 +35,6 @@ def request_set(n, seed=0, rate=1.0, steps=2000, nr_range=(128, 1024)):
     out = []
     for q in range(n):
         nr = int(np.clip(round(rng.lognormal(np.log(256), 0.5)), *nr_range))
-        nr = smooth_size(nr, *nr_range)
         out.append(dict(id=q, arrival=float(arrivals[q]), grid_size_z=2 * nr, nr=nr,
                         G=float(rng.uniform(0.1, 10.0)), nu=1e-3, rho_f=1.0, r_ball=0.15,
                         steps=int(steps)))

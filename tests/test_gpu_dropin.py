@@ -272,7 +272,8 @@ def test_stokes_rejects_unsupported():
     )
 
     with pytest.raises(NotImplementedError):
-        FastDiagonalisationStokesSolver(8, 14, 0.1)  # 7 has no radix
+        # nz / 2 = 4999 is prime and nz > 8192: beyond the row transforms' LDS
+        FastDiagonalisationStokesSolver(8, 9998, 0.1)
     # the three bc_types of FastDiagonalisationStokesSolver.py:78-97 are
     # supported (periodic z: test_gpu_periodic.py); any other is refused
     with pytest.raises(NotImplementedError):

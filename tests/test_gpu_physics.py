@@ -1,13 +1,11 @@
 """Physics-level check of the composed rigid-body step (row A16): flow past a
-sphere run to a converged drag coefficient with the reference driver's own
+sphere run to a steady drag coefficient with the reference driver's own
 set-up (examples/FlowPastSphere/flow_past_sphere.py:27-209 defaults:
 grid_size_z = 512, domain_AR = 0.5, r_sph = 0.1 at z = 0.25, U_0 = 1,
 brink_lam = 1e12, sine ramp over T_ramp = 20 r_sph / U_0, explicit RK2
-diffusion, CFL 0.1) and its convergence rule (converge_drag: means over
-samples of 30 steps, stop when two consecutive means differ by less than
-1e-5, the first sample ignored; tEnd = 300 r_sph / U_0).
+diffusion, CFL 0.1), to t = 100 r_sph / U_0, a third of the driver's tEnd.
 
-The converged Cd is compared with what the reference's post-processing
+The steady Cd is compared with what the reference's post-processing
 compares it with (examples/FlowPastSphere/post_processing.py:59-84): the
 empirical law Cd = 24/Re (1 + 0.15 Re^0.687) and the Roos & Willmarth
 measurements (experimental_data_Roos_Willmarth.txt; the pairs nearest the
@@ -37,13 +35,13 @@ def empirical_cd(re):
     return 24.0 / re * (1.0 + 0.15 * re**0.687)
 
 
-def converged_drag(re, grid_size_z=512, chunk=3000):
-    """The driver's loop on the GPU stepper, with its convergence rule
-    applied on the host to the device-recorded Cd history.  The rule is
-    applied from t = 2 T_ramp on: while the free stream ramps up the drag
-    passes through stationary points where two 30-step means can agree to
-    1e-5 (at Re = 100 the rule as written stops at t = 1.97, inside the
-    ramp, with Cd = 1.25)."""
+def steady_drag(re, grid_size_z=512, t_mid=12.0, t_end=20.0, chunk=4000):
+    """The driver's loop on the GPU stepper to t_end (60 and 100 r_sph / U_0):
+    the Cd means over the last 300 steps before t_mid and before t_end.
+    (The reference's own stopping rule, two consecutive 30-step means within
+    1e-5, also fires at stationary points of the start-up transient: at
+    Re = 100 inside the ramp, t = 1.97, and again just after it, t = 4.6, with
+    the wake still growing; a fixed horizon avoids both.)"""
     from oracle.kernels import smooth_Heaviside
     from pyaxisymflow_amd.stepper import RigidBodyStepper, rigid_grid
 
@@ -54,41 +52,26 @@ def converged_drag(re, grid_size_z=512, chunk=3000):
     chi = np.zeros_like(Z)
     smooth_Heaviside(chi, -np.sqrt((Z - 0.25) ** 2 + R**2) + r_sph, dx * 2**0.5)
     nu = U_0 * 2 * r_sph / re
-    t_end = 300 * r_sph / U_0
-    T_ramp = 20 * r_sph / U_0
     history = 600_000
-    st = RigidBodyStepper(nz, nr, 1.0, nu, 1e12, chi, U_0=U_0, T_ramp=T_ramp, r_sph=r_sph,
-                          r_len=domain_AR, history=history)
-    while st.time() < 2 * T_ramp:
-        st.step(chunk)
-    start = st.steps_done()
-    sample, drag_diff = 30, 1e-5
-    prev_mean, done = 0.0, start
-    while True:
-        st.step(chunk)
+    st = RigidBodyStepper(nz, nr, 1.0, nu, 1e12, chi, U_0=U_0, T_ramp=20 * r_sph / U_0,
+                          r_sph=r_sph, r_len=domain_AR, history=history)
+    out = []
+    for t_stop in (t_mid, t_end):
+        while st.time() < t_stop:
+            st.step(chunk)
         cd = st.drag_history()
         assert len(cd) == st.steps_done() <= history
-        while done + sample <= len(cd):
-            mean = float(np.sum(cd[done:done + sample])) / sample
-            done += sample
-            if prev_mean == 0.0:
-                prev_mean = mean
-                continue
-            diff = abs(mean - prev_mean)
-            prev_mean = mean
-            if diff < drag_diff:
-                return prev_mean, done, st.time(), cd[start:done]
-        if st.time() >= t_end:
-            return prev_mean, done, st.time(), cd[start:done]
+        out.append(float(np.mean(cd[-300:])))
+    return out[0], out[1], st.steps_done()
 
 
 @pytest.mark.parametrize("re", [50, 100, 150])
-def test_converged_drag_against_the_empirical_law(re):
-    cd, steps, t, tail = converged_drag(re)
+def test_steady_drag_against_the_empirical_law(re):
+    cd_mid, cd, steps = steady_drag(re)
     law = empirical_cd(re)
     exp = ROOS_WILLMARTH[re]
-    print(f"Re {re}: Cd {cd:.4f} after {steps} steps (t = {t:.2f}); law {law:.4f} "
-          f"({(cd - law) / law:+.1%}); Roos & Willmarth {exp}; Cd at 2 T_ramp "
-          f"{tail[0]:.4f}", flush=True)
+    print(f"Re {re}: Cd {cd:.4f} at t = 20 ({steps} steps), {cd_mid:.4f} at t = 12; law "
+          f"{law:.4f} ({(cd - law) / law:+.1%}); Roos & Willmarth {exp}", flush=True)
     assert np.isfinite(cd)
-    assert abs(cd - law) / law < 0.15
+    assert abs(cd - cd_mid) / cd < 0.01  # steady
+    assert abs(cd - law) / law < 0.10
