@@ -92,6 +92,20 @@ axs_status axs_penalization_force_sum_f64(const double* R, const double* chi,
                                           double* partials, double* out_sum,
                                           void* stream);
 
+/* Allocation-free variant of axs_penalization_force_sum_f64: no counterpart
+ * in the reference; for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same result, bit for bit; the partial sums live in a
+ * library-owned device array of 1024 doubles (8 KiB per GPU, held until
+ * the process ends) instead of a caller workspace, and the call allocates
+ * nothing.  n <= AXS_NOALLOC_MAX_CELLS.  Calls on one GPU share that
+ * array: calls that overlap on one GPU (different streams, or threads)
+ * race and must be serialised by the caller. */
+#define AXS_NOALLOC_MAX_CELLS ((int64_t)1 << 40)
+axs_status axs_penalization_force_sum_noalloc_f64(const double* R, const double* chi,
+                                                  const double* u_z, double U, int64_t n,
+                                                  double* out_sum, void* stream);
+
 /* ---- A10: the dt rule's reduction np.amax(np.fabs(u_z) + np.fabs(u_r))
  * (examples/FlowPastSphere/flow_past_sphere.py:150-153).  Writes to
  * out_max[0] (device). */

@@ -87,6 +87,7 @@ SIGNATURES = {
     "axs_diffusion_rk2_unb_f64": [_P, _P, _P, _I, _I, _D, _D, _D, _P],
     "axs_reduce_partials_size": [_L],
     "axs_penalization_force_sum_f64": [_P, _P, _P, _D, _L, _P, _P, _P],
+    "axs_penalization_force_sum_noalloc_f64": [_P, _P, _P, _D, _L, _P, _P],
     "axs_max_abs_velocity_sum_f64": [_P, _P, _L, _P, _P, _P],
     "axs_device_copy_f64": [_P, _P, _L, _P],
     "axs_solid_sigma_f64": [_P, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P, _I, _I, _P],
