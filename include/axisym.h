@@ -240,6 +240,45 @@ axs_status axs_reinit_distance_f64(const double* phi, double* out, uint8_t* mask
                                    int nz, double dx, double narrow, int iterations,
                                    void* stream);
 
+/* ---- Particle remeshing with the M'4 kernel (next to the path: the
+ * alternative advection of kernels/advect_particle.py:5-68, used by the
+ * pipe-flow and particle-in-bubble drivers).
+ * P2M: core/src/particles_to_mesh.hpp:71-91 (periodic) and :165-185
+ * (unbounded), bound as particles_to_mesh_2D_mp4 / _unbounded_mp4
+ * (core/particles_to_mesh_bind.cpp:35-90): q at particles (px, py) onto the
+ * (n0, n1) mesh `out` (cleared first), cell centres at (k + 1/2) d.  The
+ * particle arrays have the mesh's shape (n0, n1): each cell gathers from the
+ * particles within D + 2 of it, D the largest distance (in cells) of a
+ * particle's nearest mesh index from its array index, measured first
+ * (synchronizes `stream`; max_displacement_host, may be NULL, receives D).
+ * Results equal the reference's scatter bit for bit.  D > 4 (or a periodic
+ * mesh below max(2 D + 4, 8) cells along an axis) returns
+ * AXS_ERR_UNSUPPORTED. */
+axs_status axs_particles_to_mesh_2d_mp4_f64(const double* px, const double* py, const double* q,
+                                            double* out, int n0, int n1, double dx, double dy,
+                                            int periodic, int* max_displacement_host,
+                                            void* stream);
+/* M2P: core/src/mesh_to_particles.hpp:111-131 (periodic) and :209-229
+ * (unbounded), bound as mesh_to_particles_2D_mp4 / _unbounded_mp4: the
+ * (m0, m1) mesh fields fx, fy at np particles (px, py) into ox, oy.  Any
+ * particle count and layout.  A particle too far out for a mesh index
+ * (|x / dx| or |y / dy| >= 1e9, where the reference is undefined) gets NaN. */
+axs_status axs_mesh_to_particles_2d_mp4_f64(const double* fx, const double* fy, const double* px,
+                                            const double* py, double* ox, double* oy, int m0,
+                                            int m1, int64_t np, double dx, double dy,
+                                            int periodic, void* stream);
+/* kernels/advect_particle.py:5-35 (unbounded) and :38-68 (periodic): the
+ * particles of the doubled (2 nr, nz) grid pushed by the mirrored velocity
+ * (u_z even, u_r odd about the axis), the mirrored vorticity (odd) remeshed
+ * by P2M into vort_double (a (2 nr, nz) work buffer), the particles reset to
+ * (Z_double, R_double) and vorticity = the upper half.  Synchronizes `stream`. */
+axs_status axs_advect_vorticity_via_particles_f64(double* z_particles, double* r_particles,
+                                                  double* vort_particles, double* vorticity,
+                                                  const double* Z_double, const double* R_double,
+                                                  const double* u_z, const double* u_r,
+                                                  double* vort_double, int nr, int nz, double dx,
+                                                  double dt, int periodic, void* stream);
+
 /* ---- A16: the rigid-body driver step, device resident.
  * examples/FlowPastSphere/flow_past_sphere.py:107-188 (static body, free
  * stream with the sine ramp, drag coefficient Cd) and

@@ -109,6 +109,11 @@ SIGNATURES = {
     "axs_implicit_diffusion_step_f64": [_P, _P, _P],
     "axs_extrapolate_lsq_first_order_f64": [_P, _P, _P, _P, _P, _P, _I, _I, _I,
                                             ctypes.POINTER(_I), _P],
+    "axs_particles_to_mesh_2d_mp4_f64": [_P, _P, _P, _P, _I, _I, _D, _D, _I, ctypes.POINTER(_I),
+                                         _P],
+    "axs_mesh_to_particles_2d_mp4_f64": [_P, _P, _P, _P, _P, _P, _I, _I, _L, _D, _D, _I, _P],
+    "axs_advect_vorticity_via_particles_f64": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _D, _D,
+                                               _I, _P],
     "axs_rigid_stepper_create": [ctypes.POINTER(RigidConfig), ctypes.POINTER(RigidBuffers),
                                  ctypes.POINTER(_P)],
     "axs_reinit_distance_f64": [_P, _P, _P, _I, _I, _D, _D, _I, _P],

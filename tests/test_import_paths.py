@@ -40,6 +40,14 @@ REFERENCE_NAMES = [
      "extrapolate_eta_using_least_squares"),  # :8
     ("core.extrapolate_using_least_squares",
      "extrapolate_using_least_squares_till_first_order"),  # _bind.cpp:11-41
+    ("kernels.advect_particle", "advect_vorticity_via_particles"),  # :5
+    ("kernels.advect_particle", "advect_vorticity_via_particles_periodic"),  # :38
+    ("core.particles_to_mesh", "particles_to_mesh_2D_mp4"),  # particles_to_mesh_bind.cpp:111
+    ("core.particles_to_mesh", "particles_to_mesh_2D_unbounded_mp4"),  # :116
+    ("core.mesh_to_particles", "mesh_to_particles_2D_mp4"),  # mesh_to_particles_bind.cpp:188
+    ("core.mesh_to_particles", "mesh_to_particles_2D_unbounded_mp4"),  # :193
+    ("utils.dump_vtk", "vtk_init"),  # :5
+    ("utils.dump_vtk", "vtk_write"),  # :29
 ]
 
 
