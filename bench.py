@@ -243,7 +243,30 @@ def time_precision(grid_size_z=1024, no_cycles=20):
         psi_rel_l2_vs_f64=rel(b.psi(), a.psi()),
         psi_avg_rel_l2_vs_f64=rel(b.avg_psi(), a.avg_psi()),
         drag_max_rel_err_vs_f64=float(np.abs(fb - fa).max() / np.abs(fa).max()),
+        vs_oracle=precision_vs_oracle(),
     )
+
+
+def precision_vs_oracle(grid_size_z=256, no_cycles=1):
+    """The precision form against the FP64 oracle (the reference's loop
+    restated on the CPU, exact Poisson route) over one whole period at
+    256 x 128 (the oracle needs about an hour for 20 periods at 1024 x 512)."""
+    from oracle.drivers import OscillatingSpheroidCase
+    from pyaxisymflow_amd.stepper import OscillatingSpheroidStepper
+
+    cpu = OscillatingSpheroidCase(grid_size_z, solver="exact", no_cycles=no_cycles)
+    while cpu.t < cpu.tEnd:
+        cpu.step()
+    gpu = OscillatingSpheroidStepper(grid_size_z, precision="f32", no_cycles=no_cycles)
+    gpu.step(cpu.it)
+    rel = lambda x, y: float(np.linalg.norm(x - y) / np.linalg.norm(y))
+    uz, ur = gpu.velocity()
+    f, ref = gpu.drag_history(), np.array(cpu.force)
+    return dict(workload=f"same driver at {grid_size_z}x{grid_size_z // 2}, {no_cycles} period, "
+                         f"{cpu.it} steps",
+                u_rel_l2=rel(np.hypot(uz, ur), np.hypot(cpu.u_z, cpu.u_r)),
+                psi_rel_l2=rel(gpu.psi(), cpu.psi),
+                drag_max_rel_err=float(np.abs(f - ref).max() / np.abs(ref).max()))
 
 
 def time_implicit(sizes=((256, 128), (1024, 512), (2048, 1024)), reps=20):
@@ -319,11 +342,12 @@ def time_potential(sizes=((256, 128), (2048, 1024)), reps=20):
     return dict(unit="us per potential solve + velocity (drop-in, fields resident)", sizes=res)
 
 
-def time_serving(n_requests, steps, max_active=16, seed=0):
+def time_serving(n_requests, steps, max_active=32, seed=0):
     """The serving form: soft-sphere requests (pyaxisymflow_amd.serve:
-    nr lognormal around 256 in [128, 1024], 5-smooth; G in [0.1, 10]).  A
-    closed-loop pass (every request queued at t = 0) measures the capacity;
-    the open-loop pass offers Poisson arrivals at 80% of it and reports the
+    nr lognormal around 256 in [128, 1024]; G in [0.1, 10]), continuously
+    batched (one launch per pass over the admitted requests).  A closed-loop
+    pass (every request queued at t = 0) measures the capacity; the
+    open-loop pass offers Poisson arrivals at 80% of it and reports the
     time-to-solution percentiles."""
     from pyaxisymflow_amd.serve import SoftServer, latency_summary, request_set
 
@@ -352,6 +376,8 @@ def time_serving(n_requests, steps, max_active=16, seed=0):
         # steps whose extrapolation loop outran the queued sweeps and was
         # finished by the one-workgroup tail (results unchanged)
         lsq_tail_steps=int(sum(r["lsq_unconverged"] for r in recs + recs2)),
+        queued_lsq_sweeps=srv.last_batch_info[1] if srv.last_batch_info else None,
+        precision="f64",
     )
 
 
@@ -380,6 +406,14 @@ def cpu_baseline(nz, nr, steps):
         blas = []
     cores = len(os.sched_getaffinity(0))
     threads = max([b[2] for b in blas if b[2]] or [cores])
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return dict(
         value=nr * nz * steps / dt / 1e6,
         unit="Mcell-steps/s",
@@ -387,7 +421,7 @@ def cpu_baseline(nz, nr, steps):
         kind="port",
         sample=(f"oracle (numpy, eig-route dense Poisson) at {nz}x{nr}: {steps} steps after 1 "
                 f"warm-up, {dt:.2f} s; eig set-up {setup:.2f} s not included; "
-                f"affinity {cores} cores; blas {blas}"),
+                f"affinity {cores} cores ({model}); blas {blas}"),
     )
 
 

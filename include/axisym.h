@@ -428,6 +428,32 @@ axs_status axs_soft_stepper_destroy(axs_soft_stepper* st);
 /* Queues nsteps steps on `stream` (use_graph: captured once, replayed). */
 axs_status axs_soft_stepper_run(axs_soft_stepper* st, int nsteps, int use_graph, void* stream);
 
+/* Soft batch: many soft-sphere runs (the loop of
+ * examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:101-213,
+ * each with its own grid and solid) stepped together, every pass of the
+ * step one launch over all of them.  The serving form of the soft step; the
+ * reference runs one such loop per process.  A case's arithmetic is the
+ * single-case step's with the any-length row transforms (FP64), so a case
+ * gives the same bits alone and in any mix.  Cases join (add) and leave
+ * (remove) between runs; both wait for the device to finish queued work.
+ * The configuration and buffers are those of axs_soft_stepper_create
+ * (lsq_sweeps and lsq_grow are the batch's: the queue starts at 24 sweeps
+ * and grows as in lsq_grow); the buffers must stay valid until the case is
+ * removed.  nz up to 4096 (nr up to 2048). */
+typedef struct axs_soft_batch axs_soft_batch;
+axs_status axs_soft_batch_create(int max_cases, axs_soft_batch** out);
+axs_status axs_soft_batch_add(axs_soft_batch* b, const axs_soft_config* cfg,
+                              const axs_soft_buffers* bufs, int* case_id);
+axs_status axs_soft_batch_remove(axs_soft_batch* b, int case_id);
+/* Queues nsteps steps of every case on `stream` (use_graph: captured after
+ * each change of the case set, replayed). */
+axs_status axs_soft_batch_run(axs_soft_batch* b, int nsteps, int use_graph, void* stream);
+/* Cases, extrapolation sweeps queued per step, graph launches and captures
+ * so far (any pointer may be null). */
+axs_status axs_soft_batch_info(const axs_soft_batch* b, int* ncase, int* queued_sweeps,
+                               int64_t* graph_launches, int64_t* captures);
+axs_status axs_soft_batch_destroy(axs_soft_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

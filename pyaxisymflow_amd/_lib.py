@@ -121,6 +121,12 @@ SIGNATURES = {
                                 ctypes.POINTER(_P)],
     "axs_soft_stepper_destroy": [_P],
     "axs_soft_stepper_run": [_P, _I, _I, _P],
+    "axs_soft_batch_create": [_I, _P],
+    "axs_soft_batch_add": [_P, ctypes.POINTER(SoftConfig), ctypes.POINTER(SoftBuffers), _P],
+    "axs_soft_batch_remove": [_P, _I],
+    "axs_soft_batch_run": [_P, _I, _I, _P],
+    "axs_soft_batch_info": [_P, _P, _P, _P, _P],
+    "axs_soft_batch_destroy": [_P],
     "axs_rigid_batch_create": [ctypes.POINTER(RigidConfig), _I, ctypes.POINTER(RigidBuffers),
                                ctypes.POINTER(_P)],
     "axs_rigid_stepper_destroy": [_P],

@@ -23,6 +23,7 @@
 
 #include "axs_anydft.hpp"
 #include "axs_fused.hpp"
+#include "axs_softbatch.hpp"
 #include "axs_stokes.hpp"
 
 #include <algorithm>
@@ -51,10 +52,10 @@ static bool radices_of(int M, FftPlanArgs* p) {
 int any_dft_length(int N, bool* bluestein) {
   FftPlanArgs p;
   if (radices_of(N, &p)) {
-    *bluestein = false;
+    if (bluestein) *bluestein = false;
     return N;
   }
-  *bluestein = true;
+  if (bluestein) *bluestein = true;
   int L = 2 * N - 1;
   while (!radices_of(L, &p)) ++L;
   return L;
@@ -148,10 +149,73 @@ AnyDft::~AnyDft() {
 
 // ------------------------------------------------------------------ kernels
 
-// Forward DCT-II rows of any length: x = rscale_i * src row (with the kill
-// taper of kill_boundary_vorticity_sine.py:4-27 applied on the fly when
-// TAPER, as taper_dct2_row does), dst row = X.  Case blockIdx.y of a batch;
-// workgroup nr completes the previous step when fin is given.
+// Forward DCT-II of row i of one field (src, tapered, dst at the case):
+// x = rscale_i * src row (with the kill taper of
+// kill_boundary_vorticity_sine.py:4-27 applied on the fly when TAPER, as
+// taper_dct2_row does; the tapered row also goes to `tapered` when not
+// null), dst row = X.  Threads tid of nth, work of a.L entries in LDS; every
+// thread of the block calls it (block barriers).
+template <typename T, int PER, bool TAPER>
+__device__ void any_dct2_row(const T* __restrict__ src, T* __restrict__ tapered,
+                             T* __restrict__ dst, const double* __restrict__ rscale,
+                             const double* __restrict__ wz, const double* __restrict__ wr, int nr,
+                             int nz, int w, const AnyDftArgs& a,
+                             const Cplx<double>* __restrict__ e2, int i, Cplx<T>* work, int tid,
+                             int nth) {
+  const int srow = (TAPER && i >= nr - w) ? nr - w : i;
+  const T* row = src + (int64_t)srow * nz;
+  const T rs = rscale ? T(rscale[i]) : T(1);
+  T left = T(0), right = T(0), rowf = T(1);
+  if constexpr (TAPER) {
+    left = row[w - 1];
+    right = row[nz - w];
+    rowf = (i == 0) ? T(0) : ((i >= nr - w) ? T(wr[i - (nr - w)]) : T(1));
+  }
+  T* out_t = (TAPER && tapered) ? tapered + (int64_t)i * nz : nullptr;
+  for (int j = tid; j < nz; j += nth) {
+    T t = row[j];
+    if constexpr (TAPER) {
+      if (j < w)
+        t = T(wz[j]) * left;
+      else if (j >= nz - w)
+        t = T(wz[w + (j - (nz - w))]) * right;
+      if (i >= nr - w) t = rowf * t;
+      if (i == 0) t = T(0);
+      if (out_t) out_t[j] = t;
+    }
+    work[dct_dst_index(j, nz)] = Cplx<T>{rs * t, T(0)};
+  }
+  __syncthreads();
+  any_dft<T, PER>(work, a, -1, tid, nth);
+  T* out = dst + (int64_t)i * nz;
+  for (int k = tid; k < nz; k += nth) {
+    const Cplx<double> e = e2[k];
+    const Cplx<T> v = work[k];
+    out[k] = T(e.x) * v.x - T(e.y) * v.y;
+  }
+}
+
+// Inverse (DCT-III scaled by 1 / N) of row i of one field.
+template <typename T, int PER>
+__device__ void any_dct3_row(const T* __restrict__ src, T* __restrict__ dst, int nz,
+                             const AnyDftArgs& a, const Cplx<double>* __restrict__ e2, int i,
+                             Cplx<T>* work, int tid, int nth) {
+  const T* row = src + (int64_t)i * nz;
+  for (int k = tid; k < nz; k += nth) {
+    const T xk = row[k];
+    const T xn = (k == 0) ? T(0) : row[nz - k];
+    const Cplx<double> e = e2[k];
+    work[k] = cmul(Cplx<T>{T(e.x), T(-e.y)}, Cplx<T>{xk, -xn});
+  }
+  __syncthreads();
+  any_dft<T, PER>(work, a, +1, tid, nth);
+  const T invN = T(1) / T(nz);
+  T* out = dst + (int64_t)i * nz;
+  for (int j = tid; j < nz; j += nth) out[j] = work[dct_dst_index(j, nz)].x * invN;
+}
+
+// Forward DCT-II rows; case blockIdx.y of a batch; workgroup nr completes the
+// previous step when fin is given.
 template <typename T, int PER, bool TAPER>
 __global__ void __launch_bounds__(1024) any_dct2_rows_kernel(
     const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
@@ -168,63 +232,42 @@ __global__ void __launch_bounds__(1024) any_dct2_rows_kernel(
     return;
   }
   extern __shared__ unsigned char smem_raw[];
-  Cplx<T>* work = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int64_t cs = (int64_t)blockIdx.y * nr * nz;
-  const int i = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
-  const int srow = (TAPER && i >= nr - w) ? nr - w : i;
-  const T* row = src + cs + (int64_t)srow * nz;
-  const T rs = rscale ? T(rscale[i]) : T(1);
-  T left = T(0), right = T(0), rowf = T(1);
-  if constexpr (TAPER) {
-    left = row[w - 1];
-    right = row[nz - w];
-    rowf = (i == 0) ? T(0) : ((i >= nr - w) ? T(wr[i - (nr - w)]) : T(1));
-  }
-  T* out_t = (TAPER && tapered) ? tapered + cs + (int64_t)i * nz : nullptr;
-  for (int j = tid; j < nz; j += nth) {
-    T t = row[j];
-    if constexpr (TAPER) {
-      if (j < w)
-        t = T(wz[j]) * left;
-      else if (j >= nz - w)
-        t = T(wz[w + (j - (nz - w))]) * right;
-      if (i >= nr - w) t = rowf * t;
-      if (i == 0) t = T(0);
-      if (out_t) out_t[j] = t;
-    }
-    work[dct_dst_index(j, nz)] = Cplx<T>{rs * t, T(0)};
-  }
-  __syncthreads();
-  any_dft<T, PER>(work, a, -1, tid, nth);
-  T* out = dst + cs + (int64_t)i * nz;
-  for (int k = tid; k < nz; k += nth) {
-    const Cplx<double> e = e2[k];
-    const Cplx<T> v = work[k];
-    out[k] = T(e.x) * v.x - T(e.y) * v.y;
-  }
+  any_dct2_row<T, PER, TAPER>(src + cs, (TAPER && tapered) ? tapered + cs : nullptr, dst + cs,
+                              rscale, wz, wr, nr, nz, w, a, e2, blockIdx.x,
+                              reinterpret_cast<Cplx<T>*>(smem_raw), threadIdx.x, blockDim.x);
 }
 
-// Inverse (DCT-III scaled by 1 / N) rows of any length.
 template <typename T, int PER>
 __global__ void __launch_bounds__(1024) any_dct3_rows_kernel(
     const T* __restrict__ src, T* __restrict__ dst, int nr, int nz, AnyDftArgs a,
     const Cplx<double>* __restrict__ e2) {
   extern __shared__ unsigned char smem_raw[];
-  Cplx<T>* work = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int64_t cs = (int64_t)blockIdx.y * nr * nz;
-  const int i = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
-  const T* row = src + cs + (int64_t)i * nz;
-  for (int k = tid; k < nz; k += nth) {
-    const T xk = row[k];
-    const T xn = (k == 0) ? T(0) : row[nz - k];
-    const Cplx<double> e = e2[k];
-    work[k] = cmul(Cplx<T>{T(e.x), T(-e.y)}, Cplx<T>{xk, -xn});
-  }
-  __syncthreads();
-  any_dft<T, PER>(work, a, +1, tid, nth);
-  const T invN = T(1) / T(nz);
-  T* out = dst + cs + (int64_t)i * nz;
-  for (int j = tid; j < nz; j += nth) out[j] = work[dct_dst_index(j, nz)].x * invN;
+  any_dct3_row<T, PER>(src + cs, dst + cs, nz, a, e2, blockIdx.x,
+                       reinterpret_cast<Cplx<T>*>(smem_raw), threadIdx.x, blockDim.x);
+}
+
+// Soft batches: row blockIdx.x of case blockIdx.y (tapered omega -> w0,
+// r omega -> spectrum; the spectrum -> psi).
+template <int PER>
+__global__ void __launch_bounds__(1024) soft_batch_dct2_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.y];
+  if ((int)blockIdx.x >= C.nr) return;
+  extern __shared__ unsigned char smem_raw[];
+  any_dct2_row<double, PER, true>(C.vorticity, C.w0, C.spec, C.rvec, C.kill_wz, C.kill_wr, C.nr,
+                                  C.nz, C.kill_width, C.any, C.ge2, blockIdx.x,
+                                  reinterpret_cast<Cplx<double>*>(smem_raw), threadIdx.x,
+                                  blockDim.x);
+}
+
+template <int PER>
+__global__ void __launch_bounds__(1024) soft_batch_dct3_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.y];
+  if ((int)blockIdx.x >= C.nr) return;
+  extern __shared__ unsigned char smem_raw[];
+  any_dct3_row<double, PER>(C.spec, C.psi, C.nz, C.any, C.ge2, blockIdx.x,
+                            reinterpret_cast<Cplx<double>*>(smem_raw), threadIdx.x, blockDim.x);
 }
 
 // Real DFT rows of any length into the packed real layout of the periodic
@@ -360,7 +403,10 @@ static void allow_large_lds() {
         (const void*)any_rfft_rows_kernel<4>, (const void*)any_irfft_rows_kernel<1>,
         (const void*)any_irfft_rows_kernel<2>, (const void*)any_irfft_rows_kernel<4>,
         (const void*)any_dst1_rows_kernel<1>, (const void*)any_dst1_rows_kernel<2>,
-        (const void*)any_dst1_rows_kernel<4>};
+        (const void*)any_dst1_rows_kernel<4>, (const void*)soft_batch_dct2_kernel<1>,
+        (const void*)soft_batch_dct2_kernel<2>, (const void*)soft_batch_dct2_kernel<4>,
+        (const void*)soft_batch_dct3_kernel<1>, (const void*)soft_batch_dct3_kernel<2>,
+        (const void*)soft_batch_dct3_kernel<4>};
     int limit = optin;
     for (const void* k : ks) {
       hipFuncAttributes fa{};
@@ -380,12 +426,12 @@ static void allow_large_lds() {
 
 // ------------------------------------------------------------------ launchers
 
-#define AXS_ANY_PER(D, CALL) \
+#define AXS_ANY_PER(PERV, CALL) \
   do {                       \
-    if ((D).per == 1) {      \
+    if ((PERV) == 1) {       \
       constexpr int P = 1;   \
       CALL;                  \
-    } else if ((D).per == 2) { \
+    } else if ((PERV) == 2) { \
       constexpr int P = 2;   \
       CALL;                  \
     } else {                 \
@@ -404,10 +450,10 @@ hipError_t launch_any_dct2_rows(const StokesPlan& pl, const T* src, T* tapered, 
   const size_t shm = D.lds_bytes<T>();
   const int nth = D.nth;
   if (taper)
-    AXS_ANY_PER(D, (any_dct2_rows_kernel<T, P, true><<<grid, nth, shm, s>>>(
+    AXS_ANY_PER(D.per, (any_dct2_rows_kernel<T, P, true><<<grid, nth, shm, s>>>(
                        src, tapered, dst, rscale, wz, wr, pl.nr, pl.nz, w, D.args, pl.ge2, f)));
   else
-    AXS_ANY_PER(D, (any_dct2_rows_kernel<T, P, false><<<grid, nth, shm, s>>>(
+    AXS_ANY_PER(D.per, (any_dct2_rows_kernel<T, P, false><<<grid, nth, shm, s>>>(
                        src, nullptr, dst, rscale, nullptr, nullptr, pl.nr, pl.nz, 0, D.args,
                        pl.ge2, f)));
   return hipGetLastError();
@@ -417,7 +463,7 @@ template <typename T>
 hipError_t launch_any_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, int batch,
                                 hipStream_t s) {
   const AnyDft& D = *pl.any;
-  AXS_ANY_PER(D, (any_dct3_rows_kernel<T, P><<<dim3(rows, batch), D.nth, D.lds_bytes<T>(), s>>>(
+  AXS_ANY_PER(D.per, (any_dct3_rows_kernel<T, P><<<dim3(rows, batch), D.nth, D.lds_bytes<T>(), s>>>(
                      src, dst, rows, pl.nz, D.args, pl.ge2)));
   return hipGetLastError();
 }
@@ -427,9 +473,9 @@ hipError_t launch_any_rfft_rows(const StokesPlan& pl, const double* src, double*
   const AnyDft& D = *pl.any;
   const size_t shm = D.lds_bytes<double>();
   if (inverse)
-    AXS_ANY_PER(D, (any_irfft_rows_kernel<P><<<rows, D.nth, shm, s>>>(src, dst, pl.nz, D.args)));
+    AXS_ANY_PER(D.per, (any_irfft_rows_kernel<P><<<rows, D.nth, shm, s>>>(src, dst, pl.nz, D.args)));
   else
-    AXS_ANY_PER(D, (any_rfft_rows_kernel<P><<<rows, D.nth, shm, s>>>(src, dst, rscale, pl.nz,
+    AXS_ANY_PER(D.per, (any_rfft_rows_kernel<P><<<rows, D.nth, shm, s>>>(src, dst, rscale, pl.nz,
                                                                       D.args)));
   return hipGetLastError();
 }
@@ -437,8 +483,20 @@ hipError_t launch_any_rfft_rows(const StokesPlan& pl, const double* src, double*
 hipError_t launch_any_dst1_rows(const AnyDft& D, const Cplx<double>* em, const double* src,
                                 double* dst, int rows, int nz, hipStream_t s) {
   const double norm = std::sqrt(2.0 / (double)(nz + 1)) / 4.0;
-  AXS_ANY_PER(D, (any_dst1_rows_kernel<P><<<rows, D.nth, D.lds_bytes<double>(), s>>>(
+  AXS_ANY_PER(D.per, (any_dst1_rows_kernel<P><<<rows, D.nth, D.lds_bytes<double>(), s>>>(
                      src, dst, nz, D.args, em, norm)));
+  return hipGetLastError();
+}
+
+hipError_t launch_soft_batch_dct2(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
+  const dim3 grid(d.max_nr, d.ncase);
+  AXS_ANY_PER(d.any_per, (soft_batch_dct2_kernel<P><<<grid, d.any_nth, d.any_lds, s>>>(cases)));
+  return hipGetLastError();
+}
+
+hipError_t launch_soft_batch_dct3(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
+  const dim3 grid(d.max_nr, d.ncase);
+  AXS_ANY_PER(d.any_per, (soft_batch_dct3_kernel<P><<<grid, d.any_nth, d.any_lds, s>>>(cases)));
   return hipGetLastError();
 }
 

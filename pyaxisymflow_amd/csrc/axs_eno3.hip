@@ -14,6 +14,7 @@
 // (non-conservative), accumulated in the order x-front, x-back, y-front,
 // y-back with inv_dx = -dt/dx (advection_timestep.py:45-54).
 #include "axs_eno3.hpp"
+#include "axs_softbatch.hpp"
 
 namespace axs {
 
@@ -75,20 +76,20 @@ __device__ __forceinline__ void load_row5(const T* __restrict__ a, int i, int j,
   for (int o = -2; o <= 2; ++o) out[o + 2] = row[o];
 }
 
-// One conservative ENO3 step: out = f + flux(f, u), f odd about the axis.
+// One conservative ENO3 step at cell (i, j): out = f + flux(f, u), f with
+// axis parity f_parity.
 template <typename T>
-__global__ void eno3_conservative_kernel(const T* __restrict__ f, const T* __restrict__ uz,
-                                         const T* __restrict__ ur, T* __restrict__ out, int nr,
-                                         int nz, Scalar<T> dt_by_dx, T f_parity) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  if (j >= nz) return;
+__device__ __forceinline__ void eno3_conservative_at(const T* __restrict__ f,
+                                                     const T* __restrict__ uz,
+                                                     const T* __restrict__ ur,
+                                                     T* __restrict__ out, int nr, int nz,
+                                                     T dt_by_dx, T f_parity, int i, int j) {
   const int64_t c = (int64_t)i * nz + j;
   if (i >= nr - 2 || j < 2 || j >= nz - 2) {
     out[c] = f[c];
     return;
   }
-  const T s = -dt_by_dx.get();
+  const T s = -dt_by_dx;
   T fx[5], ux[5], fy[5], uy[5];
   load_row5(f, i, j, nz, fx);
   load_row5(uz, i, j, nz, ux);
@@ -101,23 +102,30 @@ __global__ void eno3_conservative_kernel(const T* __restrict__ f, const T* __res
   out[c] = f[c] + flux;
 }
 
-// One non-conservative ENO3 step for two fields sharing the velocity:
-// eta1 (even) and eta2 (odd).
 template <typename T>
-__global__ void eno3_nonconservative2_kernel(const T* __restrict__ e1, const T* __restrict__ e2,
-                                             const T* __restrict__ uz, const T* __restrict__ ur,
-                                             T* __restrict__ o1, T* __restrict__ o2, int nr,
-                                             int nz, Scalar<T> dt_by_dx) {
+__global__ void eno3_conservative_kernel(const T* __restrict__ f, const T* __restrict__ uz,
+                                         const T* __restrict__ ur, T* __restrict__ out, int nr,
+                                         int nz, Scalar<T> dt_by_dx, T f_parity) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= nz) return;
+  eno3_conservative_at<T>(f, uz, ur, out, nr, nz, dt_by_dx.get(), f_parity, i, j);
+}
+
+// One non-conservative ENO3 step for two fields sharing the velocity at cell
+// (i, j): eta1 (even) and eta2 (odd).
+template <typename T>
+__device__ __forceinline__ void eno3_nonconservative2_at(
+    const T* __restrict__ e1, const T* __restrict__ e2, const T* __restrict__ uz,
+    const T* __restrict__ ur, T* __restrict__ o1, T* __restrict__ o2, int nr, int nz,
+    T dt_by_dx, int i, int j) {
   const int64_t c = (int64_t)i * nz + j;
   if (i >= nr - 2 || j < 2 || j >= nz - 2) {
     o1[c] = e1[c];
     o2[c] = e2[c];
     return;
   }
-  const T s = -dt_by_dx.get();
+  const T s = -dt_by_dx;
   T ux[5], uy[5];
   load_row5(uz, i, j, nz, ux);
   load_column5(ur, i, j, nz, T(-1), uy);
@@ -134,6 +142,46 @@ __global__ void eno3_nonconservative2_kernel(const T* __restrict__ e1, const T* 
     flux = flux - s * eno3_back_nc(fy, uy) * uyc;
     (q == 0 ? o1 : o2)[c] = f[c] + flux;
   }
+}
+
+template <typename T>
+__global__ void eno3_nonconservative2_kernel(const T* __restrict__ e1, const T* __restrict__ e2,
+                                             const T* __restrict__ uz, const T* __restrict__ ur,
+                                             T* __restrict__ o1, T* __restrict__ o2, int nr,
+                                             int nz, Scalar<T> dt_by_dx) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= nz) return;
+  eno3_nonconservative2_at<T>(e1, e2, uz, ur, o1, o2, nr, nz, dt_by_dx.get(), i, j);
+}
+
+// Soft batches: cell (blockIdx.y, blockIdx.x * 256 + tid) of case blockIdx.z.
+// The reference map: (eta1, eta2) -> (e1n, e2n).
+__global__ void soft_batch_refmap_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.nr || j >= C.nz) return;
+  eno3_nonconservative2_at<double>(C.eta1, C.eta2, C.u_z, C.u_r, C.e1n, C.e2n, C.nr, C.nz,
+                                   C.scalars[kSsDtByDx], i, j);
+}
+
+// The vorticity: w0 (after the stress update) -> w1.
+__global__ void soft_batch_advect_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.nr || j >= C.nz) return;
+  eno3_conservative_at<double>(C.w0, C.u_z, C.u_r, C.w1, C.nr, C.nz, C.scalars[kSsDtByDx], -1.0,
+                               i, j);
+}
+
+hipError_t launch_soft_batch_refmap(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
+  soft_batch_refmap_kernel<<<dim3(ceil_div(d.max_nz, 256), d.max_nr, d.ncase), 256, 0, s>>>(cases);
+  return hipGetLastError();
+}
+
+hipError_t launch_soft_batch_advect(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
+  soft_batch_advect_kernel<<<dim3(ceil_div(d.max_nz, 256), d.max_nr, d.ncase), 256, 0, s>>>(cases);
+  return hipGetLastError();
 }
 
 template <typename T>

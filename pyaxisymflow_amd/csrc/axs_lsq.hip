@@ -12,6 +12,7 @@
 // bit.  Out-of-range cells read as flag 0 (the reference has no bounds
 // checks there, :205-219, 276-280).
 #include "axs_lsq.hpp"
+#include "axs_softbatch.hpp"
 
 #include <vector>
 
@@ -244,7 +245,7 @@ __global__ void lsq_sweep_pingpong_kernel(
   }
 }
 
-__global__ void __launch_bounds__(512) lsq_sweep_tail_kernel(
+__device__ void lsq_sweep_tail_body(
     int16_t* __restrict__ cur_a, double* __restrict__ ex_a, double* __restrict__ ey_a,
     int16_t* __restrict__ cur_b, double* __restrict__ ex_b, double* __restrict__ ey_b,
     const int16_t* __restrict__ tgt, const double* __restrict__ gx, const double* __restrict__ gy,
@@ -339,6 +340,89 @@ __global__ void __launch_bounds__(512) lsq_sweep_tail_kernel(
   // the last sweep fitted nothing and copied the previous sweep's fits
   // across: both sets now hold every fit, the home set included
   if (threadIdx.x == 0) counters[sweeps] = ran;
+}
+
+__global__ void __launch_bounds__(512) lsq_sweep_tail_kernel(
+    int16_t* __restrict__ cur_a, double* __restrict__ ex_a, double* __restrict__ ey_a,
+    int16_t* __restrict__ cur_b, double* __restrict__ ex_b, double* __restrict__ ey_b,
+    const int16_t* __restrict__ tgt, const double* __restrict__ gx, const double* __restrict__ gy,
+    int n0, int n1, const int* __restrict__ list, const unsigned* __restrict__ count,
+    unsigned* __restrict__ counters, int sweeps, int* __restrict__ tail, int cap) {
+  lsq_sweep_tail_body(cur_a, ex_a, ey_a, cur_b, ex_b, ey_b, tgt, gx, gy, n0, n1, list, count,
+                      counters, sweeps, tail, cap);
+}
+
+// ------------------------------------------------------------ soft batches
+// The doubled (2 nr, nz) grid of every case, z serving as both axes'
+// coordinates as in the single-case step; each case has its own list,
+// counters and tail lists.
+
+__global__ void soft_batch_lsq_list_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int I = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= 2 * C.nr || j >= C.nz) return;
+  const int64_t c = (int64_t)I * C.nz + j;
+  if (C.cur[c] ^ C.tgt[c]) C.lw.list[atomicAdd(C.lw.count, 1u)] = (int)c;
+}
+
+// Ping-pong sweep q over each case's list (blockIdx.y = case): sweep q reads
+// the first set (cur, e1d, e2d) when q is even.
+__global__ void soft_batch_lsq_sweep_kernel(const SoftCase* __restrict__ cs, int q) {
+  const SoftCase& C = cs[blockIdx.y];
+  const LsqWork& w = C.lw;
+  if (q > 0 && w.counters[q - 1] == 0u) return;  // the reference loop has ended
+  const bool odd = q & 1;
+  const int16_t* cur_in = odd ? w.cur2 : C.cur;
+  const double* ex_in = odd ? w.ex2 : C.e1d;
+  const double* ey_in = odd ? w.ey2 : C.e2d;
+  int16_t* cur_out = odd ? C.cur : w.cur2;
+  double* ex_out = odd ? C.e1d : w.ex2;
+  double* ey_out = odd ? C.e2d : w.ey2;
+  const int n0 = 2 * C.nr, n1 = C.nz;
+  const int nl = (int)*w.count;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nl; k += gridDim.x * blockDim.x) {
+    const int c = w.list[k];
+    double vx, vy;
+    if ((cur_in[c] ^ C.tgt[c]) &&
+        lsq_fit_cell(cur_in, ex_in, ey_in, C.z, C.z, n0, n1, c / n1, c % n1, vx, vy)) {
+      ex_out[c] = vx;
+      ey_out[c] = vy;
+      cur_out[c] = 1;
+      atomicAdd(w.counters + q, 1u);
+    } else {
+      ex_out[c] = ex_in[c];
+      ey_out[c] = ey_in[c];
+      cur_out[c] = cur_in[c];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(512) soft_batch_lsq_tail_kernel(const SoftCase* __restrict__ cs,
+                                                                  int sweeps) {
+  const SoftCase& C = cs[blockIdx.x];
+  const LsqWork& w = C.lw;
+  lsq_sweep_tail_body(C.cur, C.e1d, C.e2d, w.cur2, w.ex2, w.ey2, C.tgt, C.z, C.z, 2 * C.nr, C.nz,
+                      w.list, w.count, w.counters, sweeps, w.tail, w.cap);
+}
+
+hipError_t launch_soft_batch_lsq_list(const SoftCase* cases, const SoftBatchDims& d,
+                                      hipStream_t s) {
+  soft_batch_lsq_list_kernel<<<dim3(ceil_div(d.max_nz, 256), 2 * d.max_nr, d.ncase), 256, 0, s>>>(
+      cases);
+  return hipGetLastError();
+}
+
+hipError_t launch_soft_batch_lsq_sweep(const SoftCase* cases, const SoftBatchDims& d, int sweep,
+                                       hipStream_t s) {
+  constexpr int kBlocksPerCase = 32;  // a case's band: a few thousand cells
+  soft_batch_lsq_sweep_kernel<<<dim3(kBlocksPerCase, d.ncase), 256, 0, s>>>(cases, sweep);
+  return hipGetLastError();
+}
+
+hipError_t launch_soft_batch_lsq_tail(const SoftCase* cases, const SoftBatchDims& d, int sweeps,
+                                      hipStream_t s) {
+  soft_batch_lsq_tail_kernel<<<d.ncase, 512, 0, s>>>(cases, sweeps);
+  return hipGetLastError();
 }
 
 __global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {

@@ -17,6 +17,7 @@
 //   final   |d| > narrow is masked and keeps phi.
 // FP64 with the operation order of the CPU restatement used by the tests.
 #include "axs_reinit.hpp"
+#include "axs_softbatch.hpp"
 
 #include <cmath>
 
@@ -28,12 +29,10 @@ int reinit_iterations(double narrow, double dx) {
   return 2 * static_cast<int>(std::ceil(narrow / dx)) + 8;
 }
 
-__global__ void reinit_init_kernel(const double* __restrict__ phi, double* __restrict__ D,
-                                   double* __restrict__ dist, uint8_t* __restrict__ frozen,
-                                   int nr, int nz, double dx) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  if (j >= nz) return;
+__device__ __forceinline__ void reinit_init_at(const double* __restrict__ phi,
+                                               double* __restrict__ D, double* __restrict__ dist,
+                                               uint8_t* __restrict__ frozen, int nr, int nz,
+                                               double dx, int i, int j) {
   const int64_t c = (int64_t)i * nz + j;
   const double p = phi[c];
   double inv = 0.0;
@@ -67,6 +66,15 @@ __global__ void reinit_init_kernel(const double* __restrict__ phi, double* __res
   dist[c] = dv;
   frozen[c] = fr ? 1 : 0;
   D[c] = fr ? dv : kBig;
+}
+
+__global__ void reinit_init_kernel(const double* __restrict__ phi, double* __restrict__ D,
+                                   double* __restrict__ dist, uint8_t* __restrict__ frozen,
+                                   int nr, int nz, double dx) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= nz) return;
+  reinit_init_at(phi, D, dist, frozen, nr, nz, dx, i, j);
 }
 
 // Upwind eikonal update of cell (i, j) from the iterate D (kBig = unknown),
@@ -129,13 +137,11 @@ __device__ __forceinline__ double eikonal_update(const double* __restrict__ D, i
 // One Jacobi sweep: every cell that is not frozen takes its update from the
 // previous iterate (values may move either way: the second-order update is
 // not monotone in its neighbours).
-__global__ void reinit_sweep_kernel(const double* __restrict__ Din, double* __restrict__ Dout,
-                                    const double* __restrict__ phi,
-                                    const uint8_t* __restrict__ frozen, int nr, int nz,
-                                    double dx) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  if (j >= nz) return;
+__device__ __forceinline__ void reinit_sweep_at(const double* __restrict__ Din,
+                                                double* __restrict__ Dout,
+                                                const double* __restrict__ phi,
+                                                const uint8_t* __restrict__ frozen, int nr,
+                                                int nz, double dx, int i, int j) {
   const int64_t c = (int64_t)i * nz + j;
   const double dc = Din[c];
   if (frozen[c]) {
@@ -147,16 +153,79 @@ __global__ void reinit_sweep_kernel(const double* __restrict__ Din, double* __re
   Dout[c] = (m >= kBig) ? dc : side * m;
 }
 
+__global__ void reinit_sweep_kernel(const double* __restrict__ Din, double* __restrict__ Dout,
+                                    const double* __restrict__ phi,
+                                    const uint8_t* __restrict__ frozen, int nr, int nz,
+                                    double dx) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= nz) return;
+  reinit_sweep_at(Din, Dout, phi, frozen, nr, nz, dx, i, j);
+}
+
+__device__ __forceinline__ void reinit_final_at(const double* __restrict__ phi,
+                                                const double* __restrict__ D,
+                                                const double* __restrict__ dist,
+                                                const uint8_t* __restrict__ frozen,
+                                                double* __restrict__ out,
+                                                uint8_t* __restrict__ masked, int64_t c,
+                                                double narrow) {
+  const double d = frozen[c] ? dist[c] : D[c];
+  const bool m = !(fabs(d) <= narrow);
+  out[c] = m ? phi[c] : d;
+  if (masked) masked[c] = m ? 1 : 0;
+}
+
 __global__ void reinit_final_kernel(const double* __restrict__ phi, const double* __restrict__ D,
                                     const double* __restrict__ dist,
                                     const uint8_t* __restrict__ frozen, double* __restrict__ out,
                                     uint8_t* __restrict__ masked, int64_t n, double narrow) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  const double d = frozen[c] ? dist[c] : D[c];
-  const bool m = !(fabs(d) <= narrow);
-  out[c] = m ? phi[c] : d;
-  if (masked) masked[c] = m ? 1 : 0;
+  reinit_final_at(phi, D, dist, frozen, out, masked, c, narrow);
+}
+
+// Soft batches: cell (blockIdx.y, blockIdx.x * 256 + tid) of case
+// blockIdx.z; phi is reinitialised in place.  A case sweeps its own count
+// of iterations: sweeps beyond it exit, and its final pass reads the buffer
+// its own last sweep wrote.
+__global__ void soft_batch_reinit_init_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.nr || j >= C.nz) return;
+  reinit_init_at(C.phi, C.rw.d0, C.rw.dist, C.rw.frozen, C.nr, C.nz, C.dx, i, j);
+}
+
+__global__ void soft_batch_reinit_sweep_kernel(const SoftCase* __restrict__ cs, int q) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= C.reinit_iterations || i >= C.nr || j >= C.nz) return;
+  const bool odd = q & 1;
+  reinit_sweep_at(odd ? C.rw.d1 : C.rw.d0, odd ? C.rw.d0 : C.rw.d1, C.phi, C.rw.frozen, C.nr,
+                  C.nz, C.dx, i, j);
+}
+
+__global__ void soft_batch_reinit_final_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.nr || j >= C.nz) return;
+  const double* D = (C.reinit_iterations & 1) ? C.rw.d1 : C.rw.d0;
+  reinit_final_at(C.phi, D, C.rw.dist, C.rw.frozen, C.phi, nullptr, (int64_t)i * C.nz + j,
+                  C.reinit_band);
+}
+
+hipError_t launch_soft_batch_reinit(const SoftCase* cases, const SoftBatchDims& d, int iterations,
+                                    hipStream_t s) {
+  const dim3 g(ceil_div(d.max_nz, 256), d.max_nr, d.ncase);
+  soft_batch_reinit_init_kernel<<<g, 256, 0, s>>>(cases);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  for (int q = 0; q < iterations; ++q) {
+    soft_batch_reinit_sweep_kernel<<<g, 256, 0, s>>>(cases, q);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  soft_batch_reinit_final_kernel<<<g, 256, 0, s>>>(cases);
+  return hipGetLastError();
 }
 
 hipError_t launch_reinit(const double* phi, double* out, uint8_t* masked, int nr, int nz,

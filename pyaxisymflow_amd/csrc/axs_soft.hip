@@ -43,24 +43,12 @@
 #include "axs_fused.hpp"
 #include "axs_lsq.hpp"
 #include "axs_reinit.hpp"
+#include "axs_softbatch.hpp"
 #include "axs_stencil.hpp"
 #include "axs_stokes.hpp"
 
 namespace axs {
 
-// scalar slots of a soft step (AXS_RIGID_SCALARS doubles)
-enum : int {
-  kSsT = 0,         // time
-  kSsDt,            // dt of the current step
-  kSsDtByDx,        // dt / dx
-  kSsHalfNuDt,      // 0.5 nu dt
-  kSsNuDt,          // nu dt
-  kSsMaxU,          // max(|u_z| + |u_r|)
-  kSsIt,            // step counter
-  kSsLsqSweeps,     // sweeps that fitted cells in the last step
-  kSsLsqUnconverged,  // steps whose last allowed sweep still fitted cells
-  kSsCount
-};
 static_assert(kSsCount <= AXS_RIGID_SCALARS, "soft scalar slots exceed the array");
 
 struct SoftConfig {
@@ -89,38 +77,52 @@ __global__ void soft_velocity_max_kernel(const double* __restrict__ uz,
 }
 
 // One block of 64: the dt rule, derived scalars, cleared slots.
-__global__ void soft_dt_kernel(double* __restrict__ sc, double* __restrict__ slots,
-                               SoftConfig cfg) {
+__device__ __forceinline__ void soft_dt_body(double* __restrict__ sc, double* __restrict__ slots,
+                                             double CFL, double dx, double eps, double G,
+                                             double rho_f, double nu) {
   double m = __longlong_as_double(static_cast<long long>(
       __hip_atomic_load(reinterpret_cast<unsigned long long*>(slots + threadIdx.x * kSlotStride),
                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
   slots[threadIdx.x * kSlotStride] = 0.0;
   if (threadIdx.x != 0) return;
-  double dt = cfg.CFL * cfg.dx / (m + cfg.eps);
-  const double d2 = cfg.CFL * cfg.dx / sqrt(cfg.G / cfg.rho_f);
-  const double d3 = 0.9 * (cfg.dx * cfg.dx) / 4.0 / cfg.nu;  // 0.9 * dx**2 / 4 / nu
+  double dt = CFL * dx / (m + eps);
+  const double d2 = CFL * dx / sqrt(G / rho_f);
+  const double d3 = 0.9 * (dx * dx) / 4.0 / nu;  // 0.9 * dx**2 / 4 / nu
   if (d2 < dt) dt = d2;
   if (d3 < dt) dt = d3;
   sc[kSsMaxU] = m;
   sc[kSsDt] = dt;
-  sc[kSsDtByDx] = dt / cfg.dx;
-  sc[kSsHalfNuDt] = 0.5 * cfg.nu * dt;
-  sc[kSsNuDt] = cfg.nu * dt;
+  sc[kSsDtByDx] = dt / dx;
+  sc[kSsHalfNuDt] = 0.5 * nu * dt;
+  sc[kSsNuDt] = nu * dt;
+}
+
+__global__ void soft_dt_kernel(double* __restrict__ sc, double* __restrict__ slots,
+                               SoftConfig cfg) {
+  soft_dt_body(sc, slots, cfg.CFL, cfg.dx, cfg.eps, cfg.G, cfg.rho_f, cfg.nu);
 }
 
 // eta <- advected eta; phi pinned to the sphere of eta in phi > -3 dx.
+__device__ __forceinline__ void soft_pin_at(double* __restrict__ e1, double* __restrict__ e2,
+                                            const double* __restrict__ e1n,
+                                            const double* __restrict__ e2n,
+                                            double* __restrict__ phi, int64_t c, double z_cm,
+                                            double r_cm, double r_ball, double dx) {
+  const double a = e1n[c], b = e2n[c];
+  e1[c] = a;
+  e2[c] = b;
+  const double da = a - z_cm, db = b - r_cm;
+  const double orig = -sqrt(da * da + db * db) + r_ball;
+  if (phi[c] > -3 * dx) phi[c] = orig;
+}
+
 __global__ void soft_pin_kernel(double* __restrict__ e1, double* __restrict__ e2,
                                 const double* __restrict__ e1n, const double* __restrict__ e2n,
                                 double* __restrict__ phi, int64_t n, SoftConfig cfg) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  const double a = e1n[c], b = e2n[c];
-  e1[c] = a;
-  e2[c] = b;
-  const double da = a - cfg.z_cm, db = b - cfg.r_cm;
-  const double orig = -sqrt(da * da + db * db) + cfg.r_ball;
-  if (phi[c] > -3 * cfg.dx) phi[c] = orig;
+  soft_pin_at(e1, e2, e1n, e2n, phi, c, cfg.z_cm, cfg.r_cm, cfg.r_ball, cfg.dx);
 }
 
 // chi = smooth Heaviside(phi) (:165-166, written from the upper half of the
@@ -128,20 +130,12 @@ __global__ void soft_pin_kernel(double* __restrict__ e1, double* __restrict__ e2
 // extrapolate_eta_using_least_squares_unb.py:19-24 and the flags of
 // extrapolate_using_least_squares.py:33-35 (lower bound 0); block (0, 0)
 // also clears the extrapolation sweeps' counters.
-__global__ void soft_lsq_mirror_kernel(const double* __restrict__ phi,
-                                       double* __restrict__ chi, double moll_zone,
-                                       const double* __restrict__ e1,
-                                       const double* __restrict__ e2, double* __restrict__ e1d,
-                                       double* __restrict__ e2d, int16_t* __restrict__ cur,
-                                       int16_t* __restrict__ tgt, double* __restrict__ e1d2,
-                                       double* __restrict__ e2d2, int16_t* __restrict__ cur2,
-                                       int nr, int nz, double extrap_zone,
-                                       unsigned* __restrict__ counters, int ncounters) {
-  if (blockIdx.x == 0 && blockIdx.y == 0)
-    for (int k = threadIdx.x; k < ncounters; k += blockDim.x) counters[k] = 0u;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int I = blockIdx.y;  // doubled row, [0, 2 nr)
-  if (j >= nz) return;
+__device__ __forceinline__ void soft_lsq_mirror_at(
+    const double* __restrict__ phi, double* __restrict__ chi, double moll_zone,
+    const double* __restrict__ e1, const double* __restrict__ e2, double* __restrict__ e1d,
+    double* __restrict__ e2d, int16_t* __restrict__ cur, int16_t* __restrict__ tgt,
+    double* __restrict__ e1d2, double* __restrict__ e2d2, int16_t* __restrict__ cur2, int nr,
+    int nz, double extrap_zone, int I, int j) {
   const bool upper = I >= nr;
   const int i = upper ? I - nr : nr - 1 - I;
   const int64_t c = (int64_t)i * nz + j;
@@ -165,6 +159,24 @@ __global__ void soft_lsq_mirror_kernel(const double* __restrict__ phi,
   cur2[d] = known;
 }
 
+__global__ void soft_lsq_mirror_kernel(const double* __restrict__ phi,
+                                       double* __restrict__ chi, double moll_zone,
+                                       const double* __restrict__ e1,
+                                       const double* __restrict__ e2, double* __restrict__ e1d,
+                                       double* __restrict__ e2d, int16_t* __restrict__ cur,
+                                       int16_t* __restrict__ tgt, double* __restrict__ e1d2,
+                                       double* __restrict__ e2d2, int16_t* __restrict__ cur2,
+                                       int nr, int nz, double extrap_zone,
+                                       unsigned* __restrict__ counters, int ncounters) {
+  if (blockIdx.x == 0 && blockIdx.y == 0)
+    for (int k = threadIdx.x; k < ncounters; k += blockDim.x) counters[k] = 0u;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int I = blockIdx.y;  // doubled row, [0, 2 nr)
+  if (j >= nz) return;
+  soft_lsq_mirror_at(phi, chi, moll_zone, e1, e2, e1d, e2d, cur, tgt, e1d2, e2d2, cur2, nr, nz,
+                     extrap_zone, I, j);
+}
+
 // Extrapolated eta back into the fields from whichever ping-pong buffer set
 // holds the sweeps' result.
 __global__ void soft_lsq_back_kernel(double* __restrict__ e1, double* __restrict__ e2,
@@ -181,11 +193,10 @@ __global__ void soft_lsq_back_kernel(double* __restrict__ e1, double* __restrict
   e2[c] = (second ? e2d2 : e2d)[off + c];
 }
 
-__global__ void soft_finalize_kernel(double* __restrict__ sc,
-                                     const unsigned* __restrict__ counters, int sweeps,
-                                     unsigned* __restrict__ list_count,
-                                     unsigned* __restrict__ need) {
-  if (threadIdx.x != 0) return;
+__device__ __forceinline__ void soft_finalize_body(double* __restrict__ sc,
+                                                   const unsigned* __restrict__ counters,
+                                                   int sweeps, unsigned* __restrict__ list_count,
+                                                   unsigned* __restrict__ need) {
   *list_count = 0u;  // for the next step's list of cells to extrapolate
   // sweeps that fitted cells: the queued ones, then the tail's (its last
   // sweep fits nothing)
@@ -196,9 +207,17 @@ __global__ void soft_finalize_kernel(double* __restrict__ sc,
     sc[kSsLsqUnconverged] = sc[kSsLsqUnconverged] + 1.0;  // steps the tail finished
   }
   sc[kSsLsqSweeps] = double(used);
-  if ((unsigned)used + 1u > *need) *need = (unsigned)used + 1u;  // sweeps this loop took
+  atomicMax(need, (unsigned)used + 1u);  // sweeps this loop took
   sc[kSsT] = sc[kSsT] + sc[kSsDt];
   sc[kSsIt] = sc[kSsIt] + 1.0;
+}
+
+__global__ void soft_finalize_kernel(double* __restrict__ sc,
+                                     const unsigned* __restrict__ counters, int sweeps,
+                                     unsigned* __restrict__ list_count,
+                                     unsigned* __restrict__ need) {
+  if (threadIdx.x != 0) return;
+  soft_finalize_body(sc, counters, sweeps, list_count, need);
 }
 
 __global__ void soft_zero_kernel(double* __restrict__ p, int64_t n, int64_t stride) {
@@ -249,13 +268,21 @@ struct SoftStepper {
     return hipSuccess;
   }
 
-  hipError_t init(const SoftConfig& c, const SoftBuffers& b) {
+  // batch_case: the state of one case of a soft batch (any-length row
+  // transforms and radial segments of kSoftBatchTriL rows, which every case
+  // of a batch launch shares)
+  hipError_t init(const SoftConfig& c, const SoftBuffers& b, bool batch_case = false) {
     cfg = c;
     buf = b;
     plan = new (std::nothrow) StokesPlan();
     if (!plan) return hipErrorOutOfMemory;
+    if (batch_case) {
+      plan->force_general = true;
+      plan->force_L = kSoftBatchTriL;
+    }
     // the plan picks the half-length transforms or the any-length ones
     hipError_t e = plan->init(cfg.nr, cfg.nz, cfg.dx);
+    if (e == hipSuccess && batch_case && plan->tri_G) e = hipErrorInvalidValue;
     if (e != hipSuccess) return e;
     const size_t n = (size_t)cfg.nr * cfg.nz, nd = 2 * n, B = sizeof(double);
     double** fields[] = {&w0, &w1, &tmp, &spec, &e1n, &e2n, &s11, &s12, &s22,
@@ -462,20 +489,377 @@ struct SoftStepper {
   }
 };
 
+// ------------------------------------------------------------------ batches
+// Kernels over every case of a soft batch: per cell, case blockIdx.z (rows
+// and columns beyond the case's grid exit), or one block per case.
+
+__global__ void soft_batch_dt_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.x];
+  soft_dt_body(C.scalars, C.slots, C.CFL, C.dx, C.eps, C.G, C.rho_f, C.nu);
+}
+
+__global__ void soft_batch_pin_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.nr || j >= C.nz) return;
+  soft_pin_at(C.eta1, C.eta2, C.e1n, C.e2n, C.phi, (int64_t)i * C.nz + j, C.z_cm, C.r_cm,
+              C.r_ball, C.dx);
+}
+
+__global__ void soft_batch_mirror_kernel(const SoftCase* __restrict__ cs, int ncounters) {
+  const SoftCase& C = cs[blockIdx.z];
+  if (blockIdx.x == 0 && blockIdx.y == 0)
+    for (int k = threadIdx.x; k < ncounters; k += blockDim.x) C.lw.counters[k] = 0u;
+  const int I = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= 2 * C.nr || j >= C.nz) return;
+  soft_lsq_mirror_at(C.phi, C.chi, C.moll_zone, C.eta1, C.eta2, C.e1d, C.e2d, C.cur, C.tgt,
+                     C.lw.ex2, C.lw.ey2, C.lw.cur2, C.nr, C.nz, C.extrap_zone, I, j);
+}
+
+__global__ void soft_batch_back_kernel(const SoftCase* __restrict__ cs, int sweeps) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.nr || j >= C.nz) return;
+  const int64_t n = (int64_t)C.nr * C.nz, c = (int64_t)i * C.nz + j;
+  const bool second = lsq_result_is_second(C.lw.counters, sweeps);
+  C.eta1[c] = (second ? C.lw.ex2 : C.e1d)[n + c];
+  C.eta2[c] = (second ? C.lw.ey2 : C.e2d)[n + c];
+}
+
+__global__ void soft_batch_finalize_kernel(const SoftCase* __restrict__ cs, int sweeps) {
+  if (threadIdx.x != 0) return;
+  const SoftCase& C = cs[blockIdx.x];
+  soft_finalize_body(C.scalars, C.lw.counters, sweeps, C.lw.count, C.need);
+}
+
+// Many soft cases stepped together: each pass of the step is one launch over
+// all cases (SoftCase table on the device), in the single-case step's
+// order, so a case's results are the same bits alone and in any mix.  Cases
+// join and leave between runs; the step graphs are captured again after a
+// change.  The extrapolation queue (sweeps launched per step) is the
+// batch's, grown from the most sweeps any case has taken as in the single
+// stepper; the result never depends on it (a longer loop is finished by each
+// case's tail workgroup).
+struct SoftBatch {
+  struct Entry {
+    int id;
+    SoftStepper* st;
+  };
+  std::vector<Entry> cases;
+  int next_id = 1;
+  int max_cases = 0;
+  SoftCase* d_cases = nullptr;
+  std::vector<SoftCase> h_cases;
+  SoftBatchDims dims{};
+  int sweeps = 24;
+  unsigned* need = nullptr;
+  unsigned* need_host = nullptr;
+  hipEvent_t need_ev = nullptr;
+  bool need_pending = false;
+  int since_request = 0;
+  static constexpr int kChunkSteps = 8;
+  hipGraph_t graph = nullptr, chunk_graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr, chunk_exec = nullptr;
+  hipStream_t graph_stream = nullptr;
+  bool table_dirty = true;
+  int64_t chunk_launches = 0, step_launches = 0, captures = 0;
+
+  hipError_t init(int cap) {
+    max_cases = cap;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_cases), sizeof(SoftCase) * cap);
+    if (e != hipSuccess) return e;
+    if ((e = hipMalloc(reinterpret_cast<void**>(&need), sizeof(unsigned))) != hipSuccess) return e;
+    if ((e = hipMemset(need, 0, sizeof(unsigned))) != hipSuccess) return e;
+    if ((e = hipHostMalloc(reinterpret_cast<void**>(&need_host), sizeof(unsigned),
+                           hipHostMallocDefault)) != hipSuccess)
+      return e;
+    return hipEventCreateWithFlags(&need_ev, hipEventDisableTiming);
+  }
+
+  void drop_graphs() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
+    if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    chunk_exec = nullptr;
+    chunk_graph = nullptr;
+  }
+
+  ~SoftBatch() {
+    (void)hipDeviceSynchronize();
+    drop_graphs();
+    for (Entry& c : cases) delete c.st;
+    if (need_ev) (void)hipEventDestroy(need_ev);
+    if (need_host) (void)hipHostFree(need_host);
+    if (need) (void)hipFree(need);
+    if (d_cases) (void)hipFree(d_cases);
+  }
+
+  static SoftCase describe(const SoftStepper& S, unsigned* need) {
+    SoftCase C{};
+    const SoftConfig& g = S.cfg;
+    C.nr = g.nr;
+    C.nz = g.nz;
+    C.kill_width = g.kill_width;
+    C.reinit_iterations = g.reinit_iterations;
+    C.dx = g.dx;
+    C.nu = g.nu;
+    C.G = g.G;
+    C.rho_f = g.rho_f;
+    C.CFL = g.CFL;
+    C.eps = g.eps;
+    C.r_ball = g.r_ball;
+    C.z_cm = g.z_cm;
+    C.r_cm = g.r_cm;
+    C.moll_zone = g.moll_zone;
+    C.extrap_zone = g.extrap_zone;
+    C.reinit_band = g.reinit_band;
+    const SoftBuffers& b = S.buf;
+    C.vorticity = b.vorticity;
+    C.psi = b.psi;
+    C.u_z = b.u_z;
+    C.u_r = b.u_r;
+    C.eta1 = b.eta1;
+    C.eta2 = b.eta2;
+    C.phi = b.phi;
+    C.chi = b.char_func;
+    C.scalars = b.scalars;
+    C.r = b.r;
+    C.z = b.z;
+    C.kill_wz = b.kill_wz;
+    C.kill_wr = b.kill_wr;
+    C.w0 = S.w0;
+    C.w1 = S.w1;
+    C.tmp = S.tmp;
+    C.spec = S.spec;
+    C.e1n = S.e1n;
+    C.e2n = S.e2n;
+    C.slots = S.slots;
+    C.e1d = S.e1d;
+    C.e2d = S.e2d;
+    C.cur = S.cur;
+    C.tgt = S.tgt;
+    C.s11 = S.s11;
+    C.s12 = S.s12;
+    C.s22 = S.s22;
+    C.g1z = S.g1z;
+    C.g1r = S.g1r;
+    C.g2z = S.g2z;
+    C.g2r = S.g2r;
+    C.tz = S.tz;
+    C.tr = S.tr;
+    C.rw = S.rw;
+    C.lw = S.lw;
+    C.need = need;
+    const StokesPlan& P = *S.plan;
+    C.any = P.any->args;
+    C.ge2 = P.ge2;
+    C.ta = P.a;
+    C.tb = P.b;
+    C.tc = P.c;
+    C.tlam = P.lam;
+    C.tcseg = P.cseg;
+    C.rvec = P.rvec;
+    C.tK = P.tri_K;
+    C.tP = P.tri_P;
+    return C;
+  }
+
+  // Device table and launch shapes of the current cases.
+  hipError_t refresh() {
+    h_cases.clear();
+    dims = SoftBatchDims{};
+    for (const Entry& c : cases) {
+      const SoftStepper& S = *c.st;
+      h_cases.push_back(describe(S, need));
+      const StokesPlan& P = *S.plan;
+      dims.max_nr = std::max(dims.max_nr, P.nr);
+      dims.max_nz = std::max(dims.max_nz, P.nz);
+      dims.max_chunks = std::max(dims.max_chunks, (int)ceil_div(P.nz, P.tri_K));
+      dims.any_nth = std::max(dims.any_nth, P.any->nth);
+      dims.any_per = std::max(dims.any_per, P.any->per);
+      dims.any_lds = std::max(dims.any_lds, P.any->lds_bytes<double>());
+      dims.tri_lds =
+          std::max(dims.tri_lds, tri_lds_bytes(P.nr, kSoftBatchTriL, P.tri_K, P.tri_P, false));
+    }
+    dims.ncase = (int)cases.size();
+    drop_graphs();
+    table_dirty = false;
+    if (cases.empty()) return hipSuccess;
+    return hipMemcpy(d_cases, h_cases.data(), sizeof(SoftCase) * cases.size(),
+                     hipMemcpyHostToDevice);
+  }
+
+  hipError_t add(const SoftConfig& c, const SoftBuffers& b, int* id) {
+    if ((int)cases.size() >= max_cases) return hipErrorInvalidValue;
+    SoftStepper* st = new (std::nothrow) SoftStepper();
+    if (!st) return hipErrorOutOfMemory;
+    SoftConfig cc = c;
+    cc.lsq_sweeps = kMaxQueued;  // counters sized for the largest queue
+    cc.lsq_grow = 0;
+    hipError_t e = st->init(cc, b, true);
+    if (e != hipSuccess) {
+      delete st;
+      return e;
+    }
+    // a queued step of the old set may still read the table
+    if ((e = hipDeviceSynchronize()) != hipSuccess) {
+      delete st;
+      return e;
+    }
+    cases.push_back(Entry{next_id, st});
+    *id = next_id++;
+    table_dirty = true;
+    return hipSuccess;
+  }
+
+  hipError_t remove(int id) {
+    for (size_t k = 0; k < cases.size(); ++k) {
+      if (cases[k].id != id) continue;
+      // the case's scratch may still be in use by queued steps
+      hipError_t e = hipDeviceSynchronize();
+      if (e != hipSuccess) return e;
+      delete cases[k].st;
+      cases.erase(cases.begin() + k);
+      table_dirty = true;
+      return hipSuccess;
+    }
+    return hipErrorInvalidValue;
+  }
+
+  static constexpr int kMaxQueued = SoftStepper::kMaxQueued;
+
+  hipError_t enqueue_step(hipStream_t s) {
+    const SoftCase* T = d_cases;
+    const SoftBatchDims& d = dims;
+    hipError_t e;
+#define AXS_TRY_STEP(...)                            \
+  do {                                               \
+    if ((e = (__VA_ARGS__)) != hipSuccess) return e; \
+  } while (0)
+#define AXS_TRY_LAUNCH()                                 \
+  do {                                                   \
+    if ((e = hipGetLastError()) != hipSuccess) return e; \
+  } while (0)
+    const dim3 g(ceil_div(d.max_nz, 256), d.max_nr, d.ncase);
+    const dim3 gd(ceil_div(d.max_nz, 256), 2 * d.max_nr, d.ncase);
+    int iters = 0;
+    for (const SoftCase& c : h_cases) iters = std::max(iters, c.reinit_iterations);
+    AXS_TRY_STEP(launch_soft_batch_dct2(T, d, s));
+    AXS_TRY_STEP(launch_soft_batch_tridiag(T, d, s));
+    AXS_TRY_STEP(launch_soft_batch_dct3(T, d, s));
+    AXS_TRY_STEP(launch_soft_batch_velocity(T, d, s));
+    soft_batch_dt_kernel<<<d.ncase, 64, 0, s>>>(T);
+    AXS_TRY_LAUNCH();
+    AXS_TRY_STEP(launch_soft_batch_refmap(T, d, s));
+    soft_batch_pin_kernel<<<g, 256, 0, s>>>(T);
+    AXS_TRY_LAUNCH();
+    AXS_TRY_STEP(launch_soft_batch_reinit(T, d, iters, s));
+    soft_batch_mirror_kernel<<<gd, 256, 0, s>>>(T, sweeps + 1);
+    AXS_TRY_LAUNCH();
+    AXS_TRY_STEP(launch_soft_batch_lsq_list(T, d, s));
+    for (int q = 0; q < sweeps; ++q) AXS_TRY_STEP(launch_soft_batch_lsq_sweep(T, d, q, s));
+    AXS_TRY_STEP(launch_soft_batch_lsq_tail(T, d, sweeps, s));
+    soft_batch_back_kernel<<<g, 256, 0, s>>>(T, sweeps);
+    AXS_TRY_LAUNCH();
+    AXS_TRY_STEP(launch_soft_batch_stress(T, d, s));
+    AXS_TRY_STEP(launch_soft_batch_advect(T, d, s));
+    AXS_TRY_STEP(launch_soft_batch_diffuse(T, d, s));
+    soft_batch_finalize_kernel<<<d.ncase, 64, 0, s>>>(T, sweeps);
+    AXS_TRY_LAUNCH();
+#undef AXS_TRY_STEP
+#undef AXS_TRY_LAUNCH
+    return hipSuccess;
+  }
+
+  hipError_t capture(hipStream_t s) {
+    drop_graphs();
+    ++captures;
+    for (int q = 0; q < 2; ++q) {
+      hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) return e;
+      hipError_t le = hipSuccess;
+      for (int k = 0; k < (q == 0 ? 1 : kChunkSteps) && le == hipSuccess; ++k)
+        le = enqueue_step(s);
+      hipGraph_t* gr = q == 0 ? &graph : &chunk_graph;
+      hipGraphExec_t* x = q == 0 ? &graph_exec : &chunk_exec;
+      e = hipStreamEndCapture(s, gr);
+      if (le != hipSuccess) return le;
+      if (e != hipSuccess) return e;
+      if ((e = hipGraphInstantiate(x, *gr, nullptr, nullptr, 0)) != hipSuccess) return e;
+    }
+    graph_stream = s;
+    return hipSuccess;
+  }
+
+  // Queue growth as in SoftStepper::poll_need.
+  hipError_t poll_need() {
+    if (!need_pending) return hipSuccess;
+    hipError_t e = since_request >= 16 ? hipEventSynchronize(need_ev) : hipEventQuery(need_ev);
+    if (e == hipErrorNotReady) return hipSuccess;
+    if (e != hipSuccess) return e;
+    need_pending = false;
+    const int want = std::min(kMaxQueued, (int)((*need_host + 8u + 7u) / 8u * 8u));
+    if (want > sweeps) {
+      sweeps = want;
+      drop_graphs();
+    }
+    return hipSuccess;
+  }
+  hipError_t request_need(hipStream_t s) {
+    if (need_pending) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(need_host, need, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipEventRecord(need_ev, s);
+    if (e == hipSuccess) {
+      need_pending = true;
+      since_request = 0;
+    }
+    return e;
+  }
+
+  hipError_t run(int nsteps, bool use_graph, hipStream_t s) {
+    hipError_t e;
+    if (table_dirty && (e = refresh()) != hipSuccess) return e;
+    if (cases.empty() || nsteps == 0) return hipSuccess;
+    if (!use_graph) {
+      for (int q = 0; q < nsteps; ++q)
+        if ((e = enqueue_step(s)) != hipSuccess) return e;
+      return hipSuccess;
+    }
+    if (graph_stream != s) drop_graphs();
+    int q = 0;
+    while (q < nsteps) {
+      if ((e = poll_need()) != hipSuccess) return e;
+      if (!graph_exec && (e = capture(s)) != hipSuccess) return e;
+      const bool chunk = q + kChunkSteps <= nsteps;
+      if ((e = hipGraphLaunch(chunk ? chunk_exec : graph_exec, s)) != hipSuccess) return e;
+      ++(chunk ? chunk_launches : step_launches);
+      q += chunk ? kChunkSteps : 1;
+      ++since_request;
+      if ((e = request_need(s)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+};
+
 }  // namespace axs
 
 using namespace axs;
+
+struct axs_soft_batch {
+  SoftBatch impl;
+};
 
 struct axs_soft_stepper {
   SoftStepper impl;
 };
 
-extern "C" {
-
-axs_status axs_soft_stepper_create(const axs_soft_config* c, const axs_soft_buffers* b,
-                                   axs_soft_stepper** out) {
-  AXS_CHECK_ARG(c && b && out, "null pointer");
-  *out = nullptr;
+// Checked copy of a soft configuration and its buffers.
+static axs_status soft_config_from(const axs_soft_config* c, const axs_soft_buffers* b,
+                                   SoftConfig& cfg, SoftBuffers& bb) {
+  AXS_CHECK_ARG(c && b, "null pointer");
   AXS_CHECK_ARG(c->nr >= 4 && c->nz >= 8 && c->nz % 2 == 0, "need nr >= 4 and even nz >= 8");
   AXS_CHECK_ARG(c->nr <= 4096 && stokes_nz_supported(c->nz),
                 "grid too large for the row/column passes");
@@ -491,12 +875,7 @@ axs_status axs_soft_stepper_create(const axs_soft_config* c, const axs_soft_buff
   AXS_CHECK_ARG(b->vorticity && b->psi && b->u_z && b->u_r && b->eta1 && b->eta2 && b->phi &&
                     b->char_func && b->scalars && b->r && b->z && b->kill_wz && b->kill_wr,
                 "null buffer");
-  axs_soft_stepper* st = new (std::nothrow) axs_soft_stepper();
-  if (!st) {
-    set_error("out of host memory");
-    return AXS_ERR_ALLOC;
-  }
-  SoftConfig cfg{};
+  cfg = SoftConfig{};
   cfg.nr = c->nr;
   cfg.nz = c->nz;
   cfg.kill_width = c->kill_width;
@@ -516,8 +895,27 @@ axs_status axs_soft_stepper_create(const axs_soft_config* c, const axs_soft_buff
   cfg.extrap_zone = c->extrap_zone;
   cfg.reinit_band = c->reinit_band;
   if (cfg.reinit_iterations < 0) cfg.reinit_iterations = reinit_iterations(cfg.reinit_band, cfg.dx);
-  SoftBuffers bb{b->vorticity, b->psi,     b->u_z, b->u_r, b->eta1,    b->eta2,   b->phi,
-                 b->char_func, b->scalars, b->r,   b->z,   b->kill_wz, b->kill_wr};
+  AXS_CHECK_ARG(cfg.reinit_iterations <= 1 << 16, "too many reinit iterations");
+  bb = SoftBuffers{b->vorticity, b->psi,     b->u_z, b->u_r, b->eta1,    b->eta2,   b->phi,
+                   b->char_func, b->scalars, b->r,   b->z,   b->kill_wz, b->kill_wr};
+  return AXS_OK;
+}
+
+extern "C" {
+
+axs_status axs_soft_stepper_create(const axs_soft_config* c, const axs_soft_buffers* b,
+                                   axs_soft_stepper** out) {
+  AXS_CHECK_ARG(out, "null pointer");
+  *out = nullptr;
+  SoftConfig cfg;
+  SoftBuffers bb;
+  const axs_status st0 = soft_config_from(c, b, cfg, bb);
+  if (st0 != AXS_OK) return st0;
+  axs_soft_stepper* st = new (std::nothrow) axs_soft_stepper();
+  if (!st) {
+    set_error("out of host memory");
+    return AXS_ERR_ALLOC;
+  }
   hipError_t e = st->impl.init(cfg, bb);
   if (e != hipSuccess) {
     set_error(std::string("soft stepper: ") + hipGetErrorString(e));
@@ -525,6 +923,75 @@ axs_status axs_soft_stepper_create(const axs_soft_config* c, const axs_soft_buff
     return AXS_ERR_HIP;
   }
   *out = st;
+  return AXS_OK;
+}
+
+axs_status axs_soft_batch_create(int max_cases, axs_soft_batch** out) {
+  AXS_CHECK_ARG(out, "null pointer");
+  *out = nullptr;
+  AXS_CHECK_ARG(max_cases >= 1 && max_cases <= 4096, "max_cases must be in [1, 4096]");
+  axs_soft_batch* bt = new (std::nothrow) axs_soft_batch();
+  if (!bt) {
+    set_error("out of host memory");
+    return AXS_ERR_ALLOC;
+  }
+  hipError_t e = bt->impl.init(max_cases);
+  if (e != hipSuccess) {
+    set_error(std::string("soft batch: ") + hipGetErrorString(e));
+    delete bt;
+    return AXS_ERR_HIP;
+  }
+  *out = bt;
+  return AXS_OK;
+}
+
+axs_status axs_soft_batch_add(axs_soft_batch* bt, const axs_soft_config* c,
+                              const axs_soft_buffers* b, int* case_id) {
+  AXS_CHECK_ARG(bt && case_id, "null pointer");
+  AXS_CHECK_ARG((int)bt->impl.cases.size() < bt->impl.max_cases, "the batch is full");
+  SoftConfig cfg;
+  SoftBuffers bb;
+  const axs_status st0 = soft_config_from(c, b, cfg, bb);
+  if (st0 != AXS_OK) return st0;
+  AXS_CHECK_ARG(any_dft_length(cfg.nz, nullptr) <= kAnyMaxL && cfg.nr <= 1024 * kSoftBatchTriL,
+                "grid too large for a soft batch");
+  hipError_t e = bt->impl.add(cfg, bb, case_id);
+  if (e != hipSuccess) {
+    set_error(std::string("soft batch add: ") + hipGetErrorString(e));
+    return AXS_ERR_HIP;
+  }
+  return AXS_OK;
+}
+
+axs_status axs_soft_batch_remove(axs_soft_batch* bt, int case_id) {
+  AXS_CHECK_ARG(bt, "null batch");
+  bool known = false;
+  for (const SoftBatch::Entry& c : bt->impl.cases) known = known || c.id == case_id;
+  AXS_CHECK_ARG(known, "no such case in the batch");
+  AXS_HIP(bt->impl.remove(case_id));
+  return AXS_OK;
+}
+
+axs_status axs_soft_batch_run(axs_soft_batch* bt, int nsteps, int use_graph, void* stream) {
+  AXS_CHECK_ARG(bt, "null batch");
+  AXS_CHECK_ARG(nsteps >= 0, "negative step count");
+  AXS_CHECK_ARG(!use_graph || stream, "graph capture needs a non-null stream");
+  AXS_HIP(bt->impl.run(nsteps, use_graph != 0, as_stream(stream)));
+  return AXS_OK;
+}
+
+axs_status axs_soft_batch_info(const axs_soft_batch* bt, int* ncase, int* queued_sweeps,
+                               int64_t* graph_launches, int64_t* captures) {
+  AXS_CHECK_ARG(bt, "null batch");
+  if (ncase) *ncase = (int)bt->impl.cases.size();
+  if (queued_sweeps) *queued_sweeps = bt->impl.sweeps;
+  if (graph_launches) *graph_launches = bt->impl.chunk_launches + bt->impl.step_launches;
+  if (captures) *captures = bt->impl.captures;
+  return AXS_OK;
+}
+
+axs_status axs_soft_batch_destroy(axs_soft_batch* bt) {
+  delete bt;
   return AXS_OK;
 }
 

@@ -1,6 +1,8 @@
 // Pointwise and small-stencil kernels of the timestep (drop-in forms).
 // Each kernel restates one reference function; the expression order follows
 // the reference so FP64 results agree to round-off.
+#include "axs_fused.hpp"
+#include "axs_softbatch.hpp"
 #include "axs_stencil.hpp"
 
 #include <mutex>
@@ -48,12 +50,10 @@ __global__ void kill_r_kernel(T* __restrict__ w, const T* __restrict__ R, int nr
 // ---------------------------------------------------------------- A3
 // compute_velocity_from_psi.py:4-17
 template <typename T, class RA>
-__global__ void velocity_from_psi_kernel(T* __restrict__ u_z, T* __restrict__ u_r,
-                                         const T* __restrict__ psi, RA R, int nr, int nz,
-                                         T dx) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  int i = blockIdx.y;
-  if (j >= nz) return;
+__device__ __forceinline__ void velocity_from_psi_at(T* __restrict__ u_z, T* __restrict__ u_r,
+                                                     const T* __restrict__ psi, RA R, int nr,
+                                                     int nz, T dx, int i, int j, T& uz_out,
+                                                     T& ur_out) {
   const int64_t c = (int64_t)i * nz + j;
   const T two_dx = T(2) * dx;
   const T r = R.at(i, j);
@@ -75,6 +75,19 @@ __global__ void velocity_from_psi_kernel(T* __restrict__ u_z, T* __restrict__ u_
   }
   u_z[c] = uz;
   u_r[c] = ur;
+  uz_out = uz;
+  ur_out = ur;
+}
+
+template <typename T, class RA>
+__global__ void velocity_from_psi_kernel(T* __restrict__ u_z, T* __restrict__ u_r,
+                                         const T* __restrict__ psi, RA R, int nr, int nz,
+                                         T dx) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = blockIdx.y;
+  if (j >= nz) return;
+  T uz, ur;
+  velocity_from_psi_at<T, RA>(u_z, u_r, psi, R, nr, nz, dx, i, j, uz, ur);
 }
 
 // ---------------------------------------------------------------- A4
@@ -130,17 +143,24 @@ __device__ __forceinline__ T axisym_laplacian(const T* __restrict__ f, int64_t c
 }
 
 template <typename T, class RA>
-__global__ void rk2_stage1_kernel(const T* __restrict__ w, T* __restrict__ tmp, RA R, int nr,
-                                  int nz, Scalar<T> half_nu_dt_s, T dx2, T two_dx) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  int i = blockIdx.y;
-  if (j >= nz) return;
+__device__ __forceinline__ void rk2_stage1_at(const T* __restrict__ w, T* __restrict__ tmp, RA R,
+                                              int nr, int nz, T half_nu_dt, T dx2, T two_dx,
+                                              int i, int j) {
   const int64_t c = (int64_t)i * nz + j;
   if (i == 0 || i == nr - 1 || j == 0 || j == nz - 1) {
     tmp[c] = w[c];
     return;
   }
-  tmp[c] = w[c] + half_nu_dt_s.get() * axisym_laplacian<T>(w, c, nz, R.at(i, j), dx2, two_dx);
+  tmp[c] = w[c] + half_nu_dt * axisym_laplacian<T>(w, c, nz, R.at(i, j), dx2, two_dx);
+}
+
+template <typename T, class RA>
+__global__ void rk2_stage1_kernel(const T* __restrict__ w, T* __restrict__ tmp, RA R, int nr,
+                                  int nz, Scalar<T> half_nu_dt_s, T dx2, T two_dx) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = blockIdx.y;
+  if (j >= nz) return;
+  rk2_stage1_at<T, RA>(w, tmp, R, nr, nz, half_nu_dt_s.get(), dx2, two_dx, i, j);
 }
 
 template <typename T, class RA>
@@ -156,18 +176,26 @@ __global__ void rk2_stage2_kernel(T* __restrict__ w, const T* __restrict__ tmp, 
 // Stage 2 writing a separate output: interior = src + nu dt L(tmp), border =
 // src (so the field can return to its home buffer after an out-of-place step).
 template <typename T, class RA>
+__device__ __forceinline__ void rk2_stage2_out_at(const T* __restrict__ src,
+                                                  const T* __restrict__ tmp, T* __restrict__ out,
+                                                  RA R, int nr, int nz, T nu_dt, T dx2, T two_dx,
+                                                  int i, int j) {
+  const int64_t c = (int64_t)i * nz + j;
+  if (i == 0 || i == nr - 1 || j == 0 || j == nz - 1) {
+    out[c] = src[c];
+    return;
+  }
+  out[c] = src[c] + nu_dt * axisym_laplacian<T>(tmp, c, nz, R.at(i, j), dx2, two_dx);
+}
+
+template <typename T, class RA>
 __global__ void rk2_stage2_out_kernel(const T* __restrict__ src, const T* __restrict__ tmp,
                                       T* __restrict__ out, RA R, int nr, int nz,
                                       Scalar<T> nu_dt_s, T dx2, T two_dx) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   int i = blockIdx.y;
   if (j >= nz) return;
-  const int64_t c = (int64_t)i * nz + j;
-  if (i == 0 || i == nr - 1 || j == 0 || j == nz - 1) {
-    out[c] = src[c];
-    return;
-  }
-  out[c] = src[c] + nu_dt_s.get() * axisym_laplacian<T>(tmp, c, nz, R.at(i, j), dx2, two_dx);
+  rk2_stage2_out_at<T, RA>(src, tmp, out, R, nr, nz, nu_dt_s.get(), dx2, two_dx, i, j);
 }
 
 template <typename T, class RA>
@@ -193,15 +221,12 @@ template hipError_t launch_rk2<float, RRow<float>>(const float*, float*, float*,
 // solid_sigma.py:4-29.  Gradient scratch entries the reference never writes
 // keep their previous contents and still enter sigma.
 template <typename T>
-__global__ void solid_sigma_kernel(T* __restrict__ s11, T* __restrict__ s12,
-                                   T* __restrict__ s22, T G, T dx,
-                                   const T* __restrict__ e1, const T* __restrict__ e2,
-                                   T* __restrict__ e1z, T* __restrict__ e1r,
-                                   T* __restrict__ e2z, T* __restrict__ e2r, int nr, int nz,
-                                   const T* __restrict__ chi) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  int i = blockIdx.y;
-  if (j >= nz) return;
+__device__ __forceinline__ void solid_sigma_at(T* __restrict__ s11, T* __restrict__ s12,
+                                               T* __restrict__ s22, T G, T dx,
+                                               const T* __restrict__ e1, const T* __restrict__ e2,
+                                               T* __restrict__ e1z, T* __restrict__ e1r,
+                                               T* __restrict__ e2z, T* __restrict__ e2r, int nr,
+                                               int nz, const T* __restrict__ chi, int i, int j) {
   const int64_t c = (int64_t)i * nz + j;
   const T two_dx = T(2) * dx;
   T a1z = e1z[c], a2z = e2z[c], a1r = e1r[c], a2r = e2r[c];
@@ -236,16 +261,27 @@ __global__ void solid_sigma_kernel(T* __restrict__ s11, T* __restrict__ s12,
   }
 }
 
+template <typename T>
+__global__ void solid_sigma_kernel(T* __restrict__ s11, T* __restrict__ s12,
+                                   T* __restrict__ s22, T G, T dx,
+                                   const T* __restrict__ e1, const T* __restrict__ e2,
+                                   T* __restrict__ e1z, T* __restrict__ e1r,
+                                   T* __restrict__ e2z, T* __restrict__ e2r, int nr, int nz,
+                                   const T* __restrict__ chi) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = blockIdx.y;
+  if (j >= nz) return;
+  solid_sigma_at<T>(s11, s12, s22, G, dx, e1, e2, e1z, e1r, e2z, e2r, nr, nz, chi, i, j);
+}
+
 // ---------------------------------------------------------------- A12
 // div_tau.py:4-34: tau on the interior and on row 0 (one-sided in r) ...
 template <typename T, class RA>
-__global__ void div_tau_stage1_kernel(T* __restrict__ tz, T* __restrict__ tr,
-                                      const T* __restrict__ t11, const T* __restrict__ t12,
-                                      const T* __restrict__ t22, RA R, int nr, int nz,
-                                      T two_dx) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int i = blockIdx.y;
-  if (j >= nz - 1 || i >= nr - 1) return;
+__device__ __forceinline__ void div_tau_stage1_at(T* __restrict__ tz, T* __restrict__ tr,
+                                                  const T* __restrict__ t11,
+                                                  const T* __restrict__ t12,
+                                                  const T* __restrict__ t22, RA R, int nz,
+                                                  T two_dx, int i, int j) {
   const int64_t c = (int64_t)i * nz + j;
   const T r = R.at(i, j);
   if (i == 0) {
@@ -259,7 +295,26 @@ __global__ void div_tau_stage1_kernel(T* __restrict__ tz, T* __restrict__ tr,
   }
 }
 
+template <typename T, class RA>
+__global__ void div_tau_stage1_kernel(T* __restrict__ tz, T* __restrict__ tr,
+                                      const T* __restrict__ t11, const T* __restrict__ t12,
+                                      const T* __restrict__ t22, RA R, int nr, int nz,
+                                      T two_dx) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int i = blockIdx.y;
+  if (j >= nz - 1 || i >= nr - 1) return;
+  div_tau_stage1_at<T, RA>(tz, tr, t11, t12, t22, R, nz, two_dx, i, j);
+}
+
 // ... then omega's interior gains dt * curl(tau).
+template <typename T>
+__device__ __forceinline__ void div_tau_stage2_at(T* __restrict__ w, const T* __restrict__ tz,
+                                                  const T* __restrict__ tr, int nz, T dt,
+                                                  T two_dx, int i, int j) {
+  const int64_t c = (int64_t)i * nz + j;
+  w[c] = w[c] + dt * (tr[c + 1] - tr[c - 1] - tz[c + nz] + tz[c - nz]) / two_dx;
+}
+
 template <typename T>
 __global__ void div_tau_stage2_kernel(T* __restrict__ w, const T* __restrict__ tz,
                                       const T* __restrict__ tr, int nr, int nz, Scalar<T> dt_s,
@@ -267,9 +322,103 @@ __global__ void div_tau_stage2_kernel(T* __restrict__ w, const T* __restrict__ t
   int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
   int i = blockIdx.y + 1;
   if (j >= nz - 1) return;
-  const int64_t c = (int64_t)i * nz + j;
-  const T dt = dt_s.get();
-  w[c] = w[c] + dt * (tr[c + 1] - tr[c - 1] - tz[c + nz] + tz[c - nz]) / two_dx;
+  div_tau_stage2_at<T>(w, tz, tr, nz, dt_s.get(), two_dx, i, j);
+}
+
+// ---------------------------------------------------------------- soft batches
+// Cell (blockIdx.y, blockIdx.x * 256 + tid) of case blockIdx.z (rows and
+// columns beyond the case's own grid exit).
+
+// u from psi, and the step's max(|u_z| + |u_r|) folded per workgroup into the
+// case's spread atomicMax slots (the maximum does not depend on the order).
+__global__ void __launch_bounds__(256) soft_batch_velocity_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int i = blockIdx.y;
+  if (i >= C.nr) return;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  double m = 0.0;
+  if (j < C.nz) {
+    double uz, ur;
+    velocity_from_psi_at<double, RRow<double>>(C.u_z, C.u_r, C.psi, RRow<double>{C.r}, C.nr,
+                                               C.nz, C.dx, i, j, uz, ur);
+    m = fabs(uz) + fabs(ur);
+  }
+  __shared__ double wm[4];
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+    const int slot = (blockIdx.y * gridDim.x + blockIdx.x) % kMaxSlots;
+    atomicMax(reinterpret_cast<unsigned long long*>(C.slots + slot * kSlotStride),
+              static_cast<unsigned long long>(__double_as_longlong(m)));
+  }
+}
+
+__global__ void soft_batch_sigma_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.nr || j >= C.nz) return;
+  solid_sigma_at<double>(C.s11, C.s12, C.s22, C.G, C.dx, C.eta1, C.eta2, C.g1z, C.g1r, C.g2z,
+                         C.g2r, C.nr, C.nz, C.chi, i, j);
+}
+
+__global__ void soft_batch_div_tau1_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (i >= C.nr - 1 || j >= C.nz - 1) return;
+  div_tau_stage1_at<double, RRow<double>>(C.tz, C.tr, C.s11, C.s12, C.s22, RRow<double>{C.r},
+                                          C.nz, 2.0 * C.dx, i, j);
+}
+
+__global__ void soft_batch_div_tau2_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int i = blockIdx.y + 1, j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (i >= C.nr - 1 || j >= C.nz - 1) return;
+  div_tau_stage2_at<double>(C.w0, C.tz, C.tr, C.nz, C.scalars[kSsDt], 2.0 * C.dx, i, j);
+}
+
+__global__ void soft_batch_rk2_stage1_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.nr || j >= C.nz) return;
+  rk2_stage1_at<double, RRow<double>>(C.w1, C.tmp, RRow<double>{C.r}, C.nr, C.nz,
+                                      C.scalars[kSsHalfNuDt], C.dx * C.dx, 2.0 * C.dx, i, j);
+}
+
+__global__ void soft_batch_rk2_stage2_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.z];
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.nr || j >= C.nz) return;
+  rk2_stage2_out_at<double, RRow<double>>(C.w1, C.tmp, C.vorticity, RRow<double>{C.r}, C.nr, C.nz,
+                                          C.scalars[kSsNuDt], C.dx * C.dx, 2.0 * C.dx, i, j);
+}
+
+hipError_t launch_soft_batch_velocity(const SoftCase* cases, const SoftBatchDims& d,
+                                      hipStream_t s) {
+  soft_batch_velocity_kernel<<<dim3(ceil_div(d.max_nz, 256), d.max_nr, d.ncase), 256, 0, s>>>(
+      cases);
+  return hipGetLastError();
+}
+
+hipError_t launch_soft_batch_stress(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
+  const dim3 g(ceil_div(d.max_nz, 256), d.max_nr, d.ncase);
+  soft_batch_sigma_kernel<<<g, 256, 0, s>>>(cases);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  soft_batch_div_tau1_kernel<<<g, 256, 0, s>>>(cases);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  soft_batch_div_tau2_kernel<<<g, 256, 0, s>>>(cases);
+  return hipGetLastError();
+}
+
+hipError_t launch_soft_batch_diffuse(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
+  const dim3 g(ceil_div(d.max_nz, 256), d.max_nr, d.ncase);
+  soft_batch_rk2_stage1_kernel<<<g, 256, 0, s>>>(cases);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  soft_batch_rk2_stage2_kernel<<<g, 256, 0, s>>>(cases);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- launchers

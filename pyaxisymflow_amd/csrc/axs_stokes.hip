@@ -13,6 +13,7 @@
 
 #include "axs_fft1024.hpp"
 #include "axs_fused.hpp"
+#include "axs_softbatch.hpp"
 #include "axs_stokes.hpp"
 
 #include <algorithm>
@@ -231,7 +232,7 @@ hipError_t StokesPlan::init_transforms(int nr_, int nz_, double dx_) {
   nr = nr_;
   nz = nz_;
   dx = dx_;
-  general = !(nz % 2 == 0 && factor_radices(nz / 2, &fft));
+  general = force_general || !(nz % 2 == 0 && factor_radices(nz / 2, &fft));
   hipError_t e;
   if (general) {
     any = new (std::nothrow) AnyDft();
@@ -1170,6 +1171,22 @@ __global__ void __launch_bounds__(1024) tridiag_ragged_kernel(T* __restrict__ d,
   const int q = b - G.tri0, cl = q / G.tchunks, chunk = q % G.tchunks;
   tridiag_columns_body<T, L, false, false>(d + G.cell0 + (int64_t)cl * G.nr * G.nz, G.a, G.b,
                                            G.c, G.lam, G.cseg, G.nr, G.nz, G.K, G.P, chunk);
+}
+
+// Soft batches: mode chunk blockIdx.x of case blockIdx.y, segments of
+// kSoftBatchTriL rows (every case's plan is made for that L).
+__global__ void __launch_bounds__(1024) soft_batch_tridiag_kernel(const SoftCase* __restrict__ cs) {
+  const SoftCase& C = cs[blockIdx.y];
+  if ((int)blockIdx.x * C.tK >= C.nz) return;
+  tridiag_columns_body<double, kSoftBatchTriL, false, false>(C.spec, C.ta, C.tb, C.tc, C.tlam,
+                                                             C.tcseg, C.nr, C.nz, C.tK, C.tP,
+                                                             blockIdx.x);
+}
+
+hipError_t launch_soft_batch_tridiag(const SoftCase* cases, const SoftBatchDims& d,
+                                     hipStream_t s) {
+  soft_batch_tridiag_kernel<<<dim3(d.max_chunks, d.ncase), 1024, d.tri_lds, s>>>(cases);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ launchers

@@ -45,6 +45,9 @@ struct StokesPlan {
   // > 0: segments of force_L rows and as many modes per workgroup as fit
   // 1024 threads, no timed search (the layout an uneven batch launches)
   int force_L = 0;
+  // true: the any-length row transforms for every nz (the form a soft batch
+  // launches over cases of different lengths)
+  bool force_general = false;
   // separates the cached column-pass searches of plans whose matrices
   // differ on the same grid (0: Stokes, 1: implicit diffusion)
   int tune_key = 0;

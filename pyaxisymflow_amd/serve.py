@@ -5,17 +5,17 @@ A request is one run of the soft-sphere loop
 (examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:101-213)
 for a fixed number of steps, with its own grid and shear modulus.  The
 server admits queued requests while a slot is free and their device
-footprint fits the HBM budget, gives each admitted request its own
-device-resident SoftSphereStepper on its own stream, and keeps every active
-request supplied with work in chunks of graph-replayed steps; a request
-that finishes frees its slot and memory at once, so new arrivals join
-between chunks of the running ones (continuous batching).  Requests share
-nothing but the device, so a request's results do not depend on what else
-runs beside it.
+footprint fits the HBM budget, as cases of one SoftSphereBatch: every pass
+of a step is one launch over all admitted requests.  The batch runs in
+chunks of graph-replayed steps; between chunks finished requests leave and
+arrivals join (continuous batching).  A chunk is as long as the shortest
+remaining run (at most `chunk` steps), so no request takes a step beyond its
+own count.  A case's arithmetic does not depend on the other cases, so a
+request's results are the same bits alone and in any mix.
 
 Time-to-solution of a request = completion time - arrival time (queueing,
-set-up, and stepping), measured on the host clock, completion observed via
-an event recorded after the request's last chunk.
+set-up, and stepping), measured on the host clock at the end of the chunk
+that completes it.
 """
 import time
 
@@ -53,10 +53,10 @@ class SoftServer:
     """Continuous batching of soft-sphere requests on one GPU.
 
     max_active: slots (concurrent requests); hbm_budget: bytes the active
-    requests may hold (default: 80% of free device memory); chunk: steps
-    queued per request between scheduling decisions."""
+    requests may hold (default: 80% of free device memory); chunk: most
+    steps run between scheduling decisions."""
 
-    def __init__(self, max_active=16, hbm_budget=None, chunk=100):
+    def __init__(self, max_active=64, hbm_budget=None, chunk=100):
         torch = _lib.torch()
         dev = _lib.device()
         if hbm_budget is None:
@@ -66,19 +66,21 @@ class SoftServer:
         self.hbm_budget = int(hbm_budget)
         self.chunk = int(chunk)
 
-    def run(self, requests, keep_state=False, poll_s=2e-4):
+    def run(self, requests, keep_state=False):
         """Serves `requests` (dicts from request_set) with arrivals relative
         to the start of the call; returns one record per request (arrival,
         admitted, finished, time_to_solution, steps, final time) and, with
         keep_state, each request's final fields."""
         torch = _lib.torch()
-        from pyaxisymflow_amd.stepper import SoftSphereStepper
+        from pyaxisymflow_amd.stepper import SoftSphereBatch
 
         pending = sorted(requests, key=lambda r: r["arrival"])
         for r in pending:
             if footprint_bytes(r) > self.hbm_budget:
                 raise ValueError(f"request {r['id']} needs {footprint_bytes(r)} bytes, more "
                                  f"than the budget {self.hbm_budget}")
+        batch = SoftSphereBatch(max_cases=self.max_active)
+        self.last_batch_info = None
         active, records, used = [], {}, 0
         t0 = time.perf_counter()
         while pending or active:
@@ -87,49 +89,42 @@ class SoftServer:
             while (pending and pending[0]["arrival"] <= now and len(active) < self.max_active
                    and used + footprint_bytes(pending[0]) <= self.hbm_budget):
                 r = pending.pop(0)
-                st = SoftSphereStepper(grid_size_z=r["grid_size_z"], G=r["G"], nu=r["nu"],
-                                       rho_f=r["rho_f"], r_ball=r["r_ball"])
+                case = batch.add(grid_size_z=r["grid_size_z"], G=r["G"], nu=r["nu"],
+                                 rho_f=r["rho_f"], r_ball=r["r_ball"])
                 used += footprint_bytes(r)
-                active.append(dict(req=r, st=st, done=0, ev=None,
+                active.append(dict(req=r, case=case, done=0,
                                    admitted=time.perf_counter() - t0))
-            progressed = False
+            if not active:
+                wait = pending[0]["arrival"] - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.01))
+                continue
+            k = min(self.chunk, min(a["req"]["steps"] - a["done"] for a in active))
+            batch.step(k, join=False)
+            ev = torch.cuda.Event()
+            ev.record(batch.stream)
+            ev.synchronize()
+            finished = time.perf_counter() - t0
             for a in list(active):
-                if a["ev"] is not None and not a["ev"].query():
-                    continue
-                r = a["req"]
-                if a["done"] >= r["steps"]:
-                    st = a["st"]
-                    rec = dict(id=r["id"], nr=r["nr"], arrival=r["arrival"],
-                               admitted=a["admitted"], finished=time.perf_counter() - t0,
-                               steps=a["done"], sim_time=st.time(),
-                               lsq_unconverged=st.lsq_unconverged())
-                    rec["time_to_solution"] = rec["finished"] - rec["arrival"]
-                    if keep_state:
-                        e1, e2 = st.eta()
-                        rec["state"] = dict(vorticity=st.vorticity(), psi=st.psi(), eta1=e1,
-                                            eta2=e2, phi=st.phi(), scalars=st.scalars())
-                    records[r["id"]] = rec
-                    used -= footprint_bytes(r)
-                    active.remove(a)
-                    del st
-                    progressed = True
-                    continue
-                k = min(self.chunk, r["steps"] - a["done"])
-                # join=False: queued on the request's own stream only, so the
-                # chunks of all active requests overlap on the device
-                a["st"].step(k, join=False)
-                ev = torch.cuda.Event()
-                ev.record(a["st"].stream)
-                a["ev"] = ev
                 a["done"] += k
-                progressed = True
-            if not progressed:
-                if pending and not active:
-                    wait = pending[0]["arrival"] - (time.perf_counter() - t0)
-                    if wait > 0:
-                        time.sleep(min(wait, 0.01))
-                else:
-                    time.sleep(poll_s)
+                r = a["req"]
+                if a["done"] < r["steps"]:
+                    continue
+                c = a["case"]
+                rec = dict(id=r["id"], nr=r["nr"], arrival=r["arrival"], admitted=a["admitted"],
+                           finished=finished, steps=a["done"], sim_time=c.time(),
+                           lsq_unconverged=c.lsq_unconverged())
+                rec["time_to_solution"] = rec["finished"] - rec["arrival"]
+                if keep_state:
+                    e1, e2 = c.eta()
+                    rec["state"] = dict(vorticity=c.vorticity(), psi=c.psi(), eta1=e1, eta2=e2,
+                                        phi=c.phi(), scalars=c.scalars())
+                records[r["id"]] = rec
+                used -= footprint_bytes(r)
+                batch.remove(c)
+                active.remove(a)
+        self.last_batch_info = batch.info()
+        del batch
         return [records[k] for k in sorted(records)]
 
 

@@ -526,6 +526,60 @@ SS_T, SS_DT, SS_DT_BY_DX, SS_HALF_NU_DT, SS_NU_DT, SS_MAXU, SS_IT, SS_LSQ_SWEEPS
     SS_LSQ_UNCONVERGED = range(9)
 
 
+def _soft_case_state(obj, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_width):
+    """Device state, configuration and buffer table of one soft-sphere run
+    (examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:27-98),
+    set as attributes of obj."""
+    torch = _lib.torch()
+    dev = _lib.device()
+    dx = 1.0 / grid_size_z
+    nz, nr = int(grid_size_z), int(domain_AR * grid_size_z)
+    obj.nz, obj.nr, obj.dx = nz, nr, dx
+    moll_zone = dx * 4
+    extrap_zone = moll_zone + 4 * dx
+    z = np.linspace(0 + dx / 2, 1 - dx / 2, nz)
+    r = np.linspace(0 + dx / 2, domain_AR - dx / 2, nr)
+    obj.z, obj.r = z, r
+    Z, R = np.meshgrid(z, r)
+    phi = -np.sqrt((Z - 0.5) ** 2 + (R - 0.0) ** 2) + r_ball
+    kz = kr = 2 * np.pi
+    w = C * (kz**2 + kr**2) * np.sin(kz * Z) * np.sin(kr * R)
+
+    def t(a):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+    obj.t_vorticity = t(w)
+    obj.t_psi = t(0 * Z)
+    obj.t_u_z = t(0 * Z)
+    obj.t_u_r = t(0 * Z)
+    obj.t_eta1 = t(Z)
+    obj.t_eta2 = t(R)
+    obj.t_phi = t(phi)
+    obj.t_char_func = t(0 * Z)
+    from pyaxisymflow_amd.kernels.smooth_Heaviside import smooth_Heaviside
+
+    smooth_Heaviside(obj.t_char_func, obj.t_phi, moll_zone)
+    obj.t_scalars = torch.zeros(N_SCALARS, dtype=torch.float64, device=dev)
+    obj.t_r, obj.t_z = t(r), t(z)
+    wz, wr = kill_weights(z, r, dx, kill_width)
+    obj.t_wz, obj.t_wr = t(wz), t(wr)
+    cfg = _lib.SoftConfig()
+    cfg.nr, cfg.nz, cfg.kill_width = nr, nz, kill_width
+    cfg.lsq_sweeps, cfg.lsq_grow = 24, 1
+    cfg.reinit_iterations = -1
+    cfg.dx, cfg.nu, cfg.G, cfg.rho_f, cfg.CFL = dx, nu, G, rho_f, 0.1
+    cfg.eps = np.finfo(float).eps
+    cfg.r_ball, cfg.z_cm, cfg.r_cm = r_ball, 0.5, 0.0
+    cfg.moll_zone, cfg.extrap_zone, cfg.reinit_band = moll_zone, extrap_zone, extrap_zone
+    obj.cfg = cfg
+    bufs = _lib.SoftBuffers()
+    for name in ("vorticity", "psi", "u_z", "u_r", "eta1", "eta2", "phi", "char_func", "scalars",
+                 "r", "z"):
+        setattr(bufs, name, getattr(obj, "t_" + name).data_ptr())
+    bufs.kill_wz, bufs.kill_wr = obj.t_wz.data_ptr(), obj.t_wr.data_ptr()
+    obj._bufs = bufs
+
+
 class SoftSphereStepper:
     """examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:27-215
     on the GPU: an elastic sphere (reference map eta, shear modulus G) in an
@@ -549,64 +603,18 @@ class SoftSphereStepper:
                  C=5e-2, lsq_sweeps=None, use_graph=True, kill_width=3):
         torch = _lib.torch()
         dev = _lib.device()
-        dx = 1.0 / grid_size_z
-        nz, nr = int(grid_size_z), int(domain_AR * grid_size_z)
-        self.nz, self.nr, self.dx = nz, nr, dx
         self.use_graph = use_graph
         self.stream = torch.cuda.Stream(device=dev)
-        moll_zone = dx * 4
-        extrap_zone = moll_zone + 4 * dx
-        z = np.linspace(0 + dx / 2, 1 - dx / 2, nz)
-        r = np.linspace(0 + dx / 2, domain_AR - dx / 2, nr)
-        self.z, self.r = z, r
-        Z, R = np.meshgrid(z, r)
-        phi = -np.sqrt((Z - 0.5) ** 2 + (R - 0.0) ** 2) + r_ball
-        kz = kr = 2 * np.pi
-        w = C * (kz**2 + kr**2) * np.sin(kz * Z) * np.sin(kr * R)
-
-        def t(a):
-            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
-
-        self.t_vorticity = t(w)
-        self.t_psi = t(0 * Z)
-        self.t_u_z = t(0 * Z)
-        self.t_u_r = t(0 * Z)
-        self.t_eta1 = t(Z)
-        self.t_eta2 = t(R)
-        self.t_phi = t(phi)
-        self.t_char_func = t(0 * Z)
-        from pyaxisymflow_amd.kernels.smooth_Heaviside import smooth_Heaviside
-
-        smooth_Heaviside(self.t_char_func, self.t_phi, moll_zone)
-        self.t_scalars = torch.zeros(N_SCALARS, dtype=torch.float64, device=dev)
-        self.t_r, self.t_z = t(r), t(z)
-        wz, wr = kill_weights(z, r, dx, kill_width)
-        self.t_wz, self.t_wr = t(wz), t(wr)
-        cfg = _lib.SoftConfig()
-        cfg.nr, cfg.nz, cfg.kill_width = nr, nz, kill_width
+        _soft_case_state(self, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_width)
         # the reference's loop needs ~ band / dx sweeps at first and more as
         # the solid deforms (15-21 for G >= 1 at nz = 512, ~60 after 1000
         # steps at G = 0.1); every queued sweep costs a graph node (~5 us)
         # even after the loop has ended, a tail sweep ~50 us
         if lsq_sweeps:
-            cfg.lsq_sweeps, cfg.lsq_grow = int(lsq_sweeps), 0
-        else:
-            cfg.lsq_sweeps, cfg.lsq_grow = 24, 1
-        cfg.reinit_iterations = -1
-        cfg.dx, cfg.nu, cfg.G, cfg.rho_f, cfg.CFL = dx, nu, G, rho_f, 0.1
-        cfg.eps = np.finfo(float).eps
-        cfg.r_ball, cfg.z_cm, cfg.r_cm = r_ball, 0.5, 0.0
-        cfg.moll_zone, cfg.extrap_zone, cfg.reinit_band = moll_zone, extrap_zone, extrap_zone
-        self.cfg = cfg
-        bufs = _lib.SoftBuffers()
-        for name in ("vorticity", "psi", "u_z", "u_r", "eta1", "eta2", "phi", "char_func",
-                     "scalars", "r", "z"):
-            setattr(bufs, name, getattr(self, "t_" + name).data_ptr())
-        bufs.kill_wz, bufs.kill_wr = self.t_wz.data_ptr(), self.t_wr.data_ptr()
-        self._bufs = bufs
+            self.cfg.lsq_sweeps, self.cfg.lsq_grow = int(lsq_sweeps), 0
         torch.cuda.synchronize(dev)
         handle = ctypes.c_void_p()
-        _lib.call("axs_soft_stepper_create", ctypes.byref(cfg), ctypes.byref(bufs),
+        _lib.call("axs_soft_stepper_create", ctypes.byref(self.cfg), ctypes.byref(self._bufs),
                   ctypes.byref(handle))
         self._h = handle
 
@@ -672,6 +680,117 @@ class SoftSphereStepper:
             try:
                 self.synchronize()
                 _lib.load().axs_soft_stepper_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class _SoftCaseView:
+    """Host views of one case of a SoftSphereBatch (its fields live in
+    tensors the batch steps in place)."""
+
+    def __init__(self, batch):
+        self._batch = batch
+
+    def _host(self, t):
+        self._batch.synchronize()
+        return t.cpu().numpy()
+
+    def vorticity(self):
+        return self._host(self.t_vorticity)
+
+    def psi(self):
+        return self._host(self.t_psi)
+
+    def velocity(self):
+        return self._host(self.t_u_z), self._host(self.t_u_r)
+
+    def eta(self):
+        return self._host(self.t_eta1), self._host(self.t_eta2)
+
+    def phi(self):
+        return self._host(self.t_phi)
+
+    def char_func(self):
+        return self._host(self.t_char_func)
+
+    def scalars(self):
+        return self._host(self.t_scalars)
+
+    def time(self):
+        return float(self.scalars()[SS_T])
+
+    def steps_done(self):
+        return int(self.scalars()[SS_IT])
+
+    def lsq_unconverged(self):
+        return int(self.scalars()[SS_LSQ_UNCONVERGED])
+
+
+class SoftSphereBatch:
+    """Many soft-sphere runs stepped together on one stream, each pass of the
+    step one launch over all of them (axs_soft_batch_*).  ``add`` takes the
+    SoftSphereStepper parameters and returns a case whose fields can be read
+    like the stepper's; ``remove`` drops it.  A case's results are the same
+    bits alone and in any mix; cases join and leave between ``step`` calls."""
+
+    def __init__(self, max_cases=64, use_graph=True):
+        torch = _lib.torch()
+        dev = _lib.device()
+        self.use_graph = use_graph
+        self.stream = torch.cuda.Stream(device=dev)
+        handle = ctypes.c_void_p()
+        _lib.call("axs_soft_batch_create", int(max_cases), ctypes.byref(handle))
+        self._h = handle
+        self.cases = {}
+
+    def add(self, grid_size_z=256, G=1.0, nu=1e-3, rho_f=1.0, r_ball=0.15, domain_AR=0.5, C=5e-2,
+            kill_width=3):
+        torch = _lib.torch()
+        case = _SoftCaseView(self)
+        _soft_case_state(case, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_width)
+        torch.cuda.synchronize(_lib.device())
+        cid = ctypes.c_int()
+        _lib.call("axs_soft_batch_add", self._h, ctypes.byref(case.cfg), ctypes.byref(case._bufs),
+                  ctypes.byref(cid))
+        case.id = cid.value
+        self.cases[case.id] = case
+        return case
+
+    def remove(self, case):
+        _lib.call("axs_soft_batch_remove", self._h, int(case.id))
+        del self.cases[case.id]
+
+    def step(self, n=1, join=True):
+        """Queues n steps of every case on the batch's stream (join: ordered
+        after, and before, the current stream's work)."""
+        torch = _lib.torch()
+        cur = torch.cuda.current_stream() if join else None
+        if join:
+            self.stream.wait_stream(cur)
+        _lib.call("axs_soft_batch_run", self._h, int(n), 1 if self.use_graph else 0,
+                  ctypes.c_void_p(self.stream.cuda_stream))
+        if join:
+            cur.wait_stream(self.stream)
+
+    def synchronize(self):
+        self.stream.synchronize()
+
+    def info(self):
+        """(cases, extrapolation sweeps queued per step, graph launches,
+        graph captures)."""
+        n, q = ctypes.c_int(), ctypes.c_int()
+        g, c = ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("axs_soft_batch_info", self._h, ctypes.byref(n), ctypes.byref(q),
+                  ctypes.byref(g), ctypes.byref(c))
+        return n.value, q.value, g.value, c.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self.synchronize()
+                _lib.load().axs_soft_batch_destroy(h)
             except Exception:
                 pass
             self._h = None

@@ -1,8 +1,10 @@
 """Serving form (continuous batching of soft-sphere requests).
 
-* A request's results are the same alone and in the mix: requests share only
-  the device, so the final fields of a request run beside others equal, bit
-  for bit, those of the same request run alone.
+* A request's results are the same alone and in the mix: the admitted
+  requests are cases of one soft batch (one launch per pass over all of
+  them) whose arithmetic per case does not depend on the others, so the
+  final fields of a request run beside others, joining and leaving at
+  different chunks, equal, bit for bit, those of the same request run alone.
 * Every request completes its steps; time-to-solution >= its stepping; the
   slot and memory limits hold (admission waits instead of overcommitting).
 """
@@ -48,9 +50,9 @@ def test_serving_limits_and_records():
 
 
 def test_request_chunks_do_not_wait_on_each_other():
-    """The server queues each request's chunk on that request's stream alone
-    (step(join=False)): a short chunk of one request completes while a long
-    chunk of another, queued before it, is still running."""
+    """Single-case steppers stepped with join=False queue on their own
+    streams only: a short chunk of one completes while a long chunk of
+    another, queued before it, is still running."""
     import torch
 
     from pyaxisymflow_amd.stepper import SoftSphereStepper

@@ -1,0 +1,111 @@
+"""Soft batch (axs_soft_batch_*): many soft-sphere runs
+(soft_sphere_in_taylor_green_vortex.py:101-213) stepped together, one
+launch per pass over all of them.
+
+* One step of a case in a mixed batch against the oracle's exact route
+  (oracle/drivers.py SoftSphereCase): every field within 1e-12 relative L2,
+  dt within 1e-12 (the single-case stepper's bar).
+* A case gives the same bits alone and in a mix of other grids and solids,
+  also when cases join and leave between runs, and graph replay equals
+  eager stepping.
+Grids 48 x 24 to 160 x 80 (nz = 2 nr, as the extrapolation requires).
+"""
+import numpy as np
+import pytest
+
+from oracle.drivers import SoftSphereCase
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _fields(c):
+    if isinstance(c, SoftSphereCase):
+        return dict(w=c.vorticity, psi=c.psi, e1=c.eta1, e2=c.eta2, phi=c.ball_phi,
+                    chi=c.ball_char_func)
+    e1, e2 = c.eta()
+    return dict(w=c.vorticity(), psi=c.psi(), e1=e1, e2=e2, phi=c.phi(), chi=c.char_func())
+
+
+def _assert_same_bits(a, b, what):
+    fa, fb = _fields(a), _fields(b)
+    for k in fa:
+        assert np.array_equal(fa[k], fb[k]), f"{what}: {k}"
+    assert np.array_equal(a.scalars()[:9], b.scalars()[:9]), what
+
+
+def test_batch_case_one_step_matches_oracle():
+    from pyaxisymflow_amd.stepper import SoftSphereBatch
+
+    print("soft batch: 1 step vs oracle, nz 128 among 96 and 160", flush=True)
+    cpu = SoftSphereCase(grid_size_z=128, solver="exact")
+    b = SoftSphereBatch(max_cases=4)
+    b.add(grid_size_z=96, G=0.5)
+    case = b.add(grid_size_z=128)
+    b.add(grid_size_z=160, G=5.0)
+    cpu.step()
+    b.step(1)
+    g, c = _fields(case), _fields(cpu)
+    errs = {k: rel_l2(g[k], c[k]) for k in c}
+    print("soft batch 1 step: " + " ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    sc = case.scalars()
+    assert case.steps_done() == 1
+    assert abs(sc[1] - cpu.dts[-1]) <= 1e-12 * cpu.dts[-1]
+    assert case.lsq_unconverged() == 0
+    for k, v in errs.items():
+        assert v < 1e-12, k
+
+
+def test_batch_case_alone_equals_in_mix():
+    from pyaxisymflow_amd.stepper import SoftSphereBatch
+
+    print("soft batch: alone vs mix, nz 64, 12 steps", flush=True)
+    alone = SoftSphereBatch(max_cases=1)
+    a = alone.add(grid_size_z=64, G=2.0)
+    mix = SoftSphereBatch(max_cases=4)
+    mix.add(grid_size_z=100, G=0.3)
+    m = mix.add(grid_size_z=64, G=2.0)
+    mix.add(grid_size_z=48, G=8.0)
+    alone.step(12)
+    mix.step(12)
+    _assert_same_bits(a, m, "alone vs mix")
+    assert a.steps_done() == 12
+
+
+def test_batch_join_leave_between_runs():
+    from pyaxisymflow_amd.stepper import SoftSphereBatch
+
+    print("soft batch: join/leave, nz 64, 5 + 9 steps", flush=True)
+    alone = SoftSphereBatch(max_cases=1)
+    a = alone.add(grid_size_z=64)
+    alone.step(14)
+    b = SoftSphereBatch(max_cases=3)
+    other = b.add(grid_size_z=80, G=0.2)
+    x = b.add(grid_size_z=64)
+    b.step(5)
+    b.remove(other)
+    late = b.add(grid_size_z=120, G=3.0)
+    b.step(9)
+    _assert_same_bits(a, x, "join/leave")
+    assert late.steps_done() == 9 and x.steps_done() == 14
+    n, queued, launches, captures = b.info()
+    print(f"soft batch info: cases {n} queued sweeps {queued} graph launches {launches} "
+          f"captures {captures}", flush=True)
+    assert n == 2 and captures >= 2
+
+
+def test_batch_graph_equals_eager():
+    from pyaxisymflow_amd.stepper import SoftSphereBatch
+
+    print("soft batch: graph vs eager, nz 64 + 96, 19 steps", flush=True)
+    g = SoftSphereBatch(max_cases=2, use_graph=True)
+    e = SoftSphereBatch(max_cases=2, use_graph=False)
+    ga, gb = g.add(grid_size_z=64), g.add(grid_size_z=96, G=0.5)
+    ea, eb = e.add(grid_size_z=64), e.add(grid_size_z=96, G=0.5)
+    g.step(19)  # eight-step graphs plus single-step remainders
+    e.step(19)
+    _assert_same_bits(ga, ea, "graph vs eager (64)")
+    _assert_same_bits(gb, eb, "graph vs eager (96)")
