@@ -274,8 +274,9 @@ def time_implicit(sizes=((256, 128), (1024, 512), (2048, 1024)), reps=20):
     drop-in ImplicitEulerDiffusionStepper) on device-resident fields: one
     step = DST-I rows, radial tridiagonal solves, DST-I rows.  256 x 128 is
     the torus-streaming grid that uses it; dt = dx^2 / 4 / nu
-    (flow_past_sphere.py:90).  The dense DST-I rows are 2 nz^2 flops per row
-    each (the parity tests check the result against the oracle)."""
+    (flow_past_sphere.py:90).  The DST-I rows run as (nz + 1)-point DFTs
+    (Stockham when nz + 1 factors into 2, 3 and 5, else chirp-z); the parity
+    tests check the result against the oracle."""
     import torch
 
     from pyaxisymflow_amd.kernels.implicit_diffusion_solver import ImplicitEulerDiffusionStepper
@@ -298,12 +299,11 @@ def time_implicit(sizes=((256, 128), (1024, 512), (2048, 1024)), reps=20):
         e1.record(s)
         torch.cuda.synchronize()
         us = e0.elapsed_time(e1) * 1e3 / reps
-        flops = 2 * (2.0 * nr * nz * nz)
         res[f"{nz}x{nr}"] = dict(us_per_step=round(us, 2),
-                                 mcell_steps_per_s=round(nr * nz / us, 2),
-                                 dst_tflops=round(flops / (us * 1e-6) / 1e12, 2))
+                                 mcell_steps_per_s=round(nr * nz / us, 2))
     return dict(unit="us per implicit step (drop-in, fields resident)", sizes=res,
-                note="DST-I rows as a dense product (fast sine transforms need nz+1-point FFTs)")
+                note="DST-I rows as (nz + 1)-point DFTs in LDS (chirp-z when nz + 1 is not "
+                     "2,3,5-smooth)")
 
 
 def time_potential(sizes=((256, 128), (2048, 1024)), reps=20):

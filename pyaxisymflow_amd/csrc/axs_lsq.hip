@@ -211,12 +211,27 @@ __global__ void lsq_commit_list_kernel(int16_t* __restrict__ cur, double* __rest
   }
 }
 
+// Appends value to list for the lanes with take set: one atomic per wave
+// for all of them (a counter that every lane of a large grid bumps alone
+// serialises the whole pass), lanes in lane order after the wave's base.
+__device__ __forceinline__ void list_append(bool take, int value, int* __restrict__ list,
+                                            unsigned* __restrict__ count) {
+  const unsigned long long mask = __ballot(take);
+  if (mask == 0ull) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+  unsigned base = 0u;
+  if (lane == leader) base = atomicAdd(count, static_cast<unsigned>(__popcll(mask)));
+  base = __shfl(base, leader);
+  if (take) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
 // Cells with current ^ target set, in any order (the set is what matters:
 // a sweep's fits read only the previous sweep's state); *count must be zero.
 __global__ void lsq_list_kernel(const int16_t* __restrict__ cur, const int16_t* __restrict__ tgt,
                                 int64_t n, int* __restrict__ list, unsigned* __restrict__ count) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < n && (cur[c] ^ tgt[c])) list[atomicAdd(count, 1u)] = (int)c;
+  list_append(c < n && (cur[c] ^ tgt[c]), (int)c, list, count);
 }
 
 __global__ void lsq_sweep_pingpong_kernel(
@@ -360,9 +375,9 @@ __global__ void __launch_bounds__(512) lsq_sweep_tail_kernel(
 __global__ void soft_batch_lsq_list_kernel(const SoftCase* __restrict__ cs) {
   const SoftCase& C = cs[blockIdx.z];
   const int I = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= 2 * C.nr || j >= C.nz) return;
+  if (I >= 2 * C.nr) return;
   const int64_t c = (int64_t)I * C.nz + j;
-  if (C.cur[c] ^ C.tgt[c]) C.lw.list[atomicAdd(C.lw.count, 1u)] = (int)c;
+  list_append(j < C.nz && (C.cur[c] ^ C.tgt[c]), (int)c, C.lw.list, C.lw.count);
 }
 
 // Ping-pong sweep q over each case's list (blockIdx.y = case): sweep q reads

@@ -22,6 +22,8 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <string>
+#include <sys/stat.h>
 #include <tuple>
 #include <vector>
 
@@ -88,6 +90,68 @@ std::vector<double> pivot_table(const std::vector<double>& a, const std::vector<
 // Autotuner of the column pass: times every admissible (modes per
 // workgroup K, rows per segment L) on this device with the plan's own
 // matrices and keeps the fastest; results are cached per (nr, nz).
+// On-disk record of the column-pass layouts the timed search has chosen, so
+// that later processes on the same kind of device take the same layout (the
+// radial solve's round-off depends on it) without searching again.  One line
+// per choice: "<arch> <nr> <nz> <key> <L> <K> <G>"; the first line for a
+// grid wins.  AXS_TUNE_CACHE names the file ("off": none); by default
+// $XDG_CACHE_HOME/axisym-hip/tridiag.txt or ~/.cache/axisym-hip/tridiag.txt.
+static std::string tune_cache_path() {
+  if (const char* p = std::getenv("AXS_TUNE_CACHE")) return std::strcmp(p, "off") ? p : "";
+  std::string dir;
+  if (const char* x = std::getenv("XDG_CACHE_HOME"))
+    dir = x;
+  else if (const char* h = std::getenv("HOME"))
+    dir = std::string(h) + "/.cache";
+  else
+    return "";
+  dir += "/axisym-hip";
+  (void)::mkdir(dir.substr(0, dir.rfind('/')).c_str(), 0755);
+  (void)::mkdir(dir.c_str(), 0755);
+  return dir + "/tridiag.txt";
+}
+
+static std::string device_arch() {
+  int dev = 0;
+  hipDeviceProp_t prop{};
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return "unknown";
+  }
+  std::string a = prop.gcnArchName;
+  for (char& c : a)
+    if (c == ' ') c = '_';
+  return a + "_cu" + std::to_string(prop.multiProcessorCount);
+}
+
+static bool tune_cache_read(const std::string& path, const std::string& arch, int nr, int nz,
+                            int key, int* L, int* K, int* G) {
+  if (path.empty()) return false;
+  std::FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) return false;
+  char a[128];
+  int r, z, k, l, kk, g;
+  bool found = false;
+  while (!found && std::fscanf(f, "%127s %d %d %d %d %d %d", a, &r, &z, &k, &l, &kk, &g) == 7)
+    if (arch == a && r == nr && z == nz && k == key) {
+      *L = l;
+      *K = kk;
+      *G = g;
+      found = true;
+    }
+  std::fclose(f);
+  return found;
+}
+
+static void tune_cache_write(const std::string& path, const std::string& arch, int nr, int nz,
+                             int key, int L, int K, int G) {
+  if (path.empty()) return;
+  if (std::FILE* f = std::fopen(path.c_str(), "a")) {
+    std::fprintf(f, "%s %d %d %d %d %d %d\n", arch.c_str(), nr, nz, key, L, K, G);
+    std::fclose(f);
+  }
+}
+
 hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::vector<double>& hb,
                                     const std::vector<double>& hc,
                                     const std::vector<double>& hlam) {
@@ -99,25 +163,37 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
   static std::map<std::tuple<int, int, int>, Choice> cache;
   Choice best{tri_K, tri_L, tri_G};
   bool cached = false;
+  // a layout is usable when its workgroup and LDS fit
+  auto usable = [&](int L, int K, int G) {
+    if ((L != 4 && L != 8 && L != 16) || K < 1 || G < 0 || G > 1) return false;
+    const int P = (nr + L - 1) / L;
+    return K * P <= 1024 && tri_lds_bytes(nr, L, K, P, G == 1) <= 65536;
+  };
   // AXS_TRIDIAG_LK="L,K" pins the choice (the search times the candidates,
   // so two processes may pick different ones when timings tie, and the
   // radial solve's round-off depends on the choice)
   if (const char* pin = std::getenv("AXS_TRIDIAG_LK")) {  // "L,K" or "L,K,G" (G = 1: global)
     int L = 0, K = 0, G = 0;
     const int got = std::sscanf(pin, "%d,%d,%d", &L, &K, &G);
-    if (got < 2 || (L != 4 && L != 8 && L != 16) || K < 1 || G < 0 || G > 1)
-      return hipErrorInvalidValue;
-    const int P = (nr + L - 1) / L;
-    if (K * P > 1024 || tri_lds_bytes(nr, L, K, P, G == 1) > 65536) return hipErrorInvalidValue;
+    if (got < 2 || !usable(L, K, G)) return hipErrorInvalidValue;
     best = {K, L, G == 1};
     cached = true;
   }
+  const std::string path = cached ? std::string() : tune_cache_path();
+  const std::string arch = cached ? std::string() : device_arch();
   if (!cached) {
     std::lock_guard<std::mutex> g(mu);
     auto it = cache.find({nr, nz, tune_key});
     if (it != cache.end()) {
       best = it->second;
       cached = true;
+    } else {
+      int L = 0, K = 0, G = 0;
+      if (tune_cache_read(path, arch, nr, nz, tune_key, &L, &K, &G) && usable(L, K, G)) {
+        best = {K, L, G == 1};
+        cache[{nr, nz, tune_key}] = best;
+        cached = true;
+      }
     }
   }
   if (!cached) {
@@ -176,6 +252,12 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
     (void)hipEventDestroy(t1);
     if (e != hipSuccess) return e;
     std::lock_guard<std::mutex> g(mu);
+    // another process may have recorded this grid meanwhile: its choice wins
+    int L = 0, K = 0, G = 0;
+    if (tune_cache_read(path, arch, nr, nz, tune_key, &L, &K, &G) && usable(L, K, G))
+      best = {K, L, G == 1};
+    else
+      tune_cache_write(path, arch, nr, nz, tune_key, best.L, best.K, best.G ? 1 : 0);
     cache[{nr, nz, tune_key}] = best;
   }
   tri_G = best.G;
