@@ -439,7 +439,7 @@ axs_status axs_soft_stepper_run(axs_soft_stepper* st, int nsteps, int use_graph,
  * The configuration and buffers are those of axs_soft_stepper_create
  * (lsq_sweeps and lsq_grow are the batch's: the queue starts at 24 sweeps
  * and grows as in lsq_grow); the buffers must stay valid until the case is
- * removed.  nz up to 4096 (nr up to 2048). */
+ * removed.  At most 1024 cases; nz up to 4096 (nr up to 2048). */
 typedef struct axs_soft_batch axs_soft_batch;
 axs_status axs_soft_batch_create(int max_cases, axs_soft_batch** out);
 axs_status axs_soft_batch_add(axs_soft_batch* b, const axs_soft_config* cfg,

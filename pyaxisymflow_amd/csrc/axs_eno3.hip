@@ -155,32 +155,34 @@ __global__ void eno3_nonconservative2_kernel(const T* __restrict__ e1, const T* 
   eno3_nonconservative2_at<T>(e1, e2, uz, ur, o1, o2, nr, nz, dt_by_dx.get(), i, j);
 }
 
-// Soft batches: cell (blockIdx.y, blockIdx.x * 256 + tid) of case blockIdx.z.
+// Soft batches: one row segment of 256 cells of one case per workgroup.
 // The reference map: (eta1, eta2) -> (e1n, e2n).
-__global__ void soft_batch_refmap_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.nr || j >= C.nz) return;
+__global__ void soft_batch_refmap_kernel(const SoftCase* __restrict__ cs,
+                                         const uint32_t* __restrict__ blocks) {
+  const SoftBlock b = soft_block(blocks);
+  const SoftCase& C = cs[b.c];
+  if (b.j >= C.nz) return;
   eno3_nonconservative2_at<double>(C.eta1, C.eta2, C.u_z, C.u_r, C.e1n, C.e2n, C.nr, C.nz,
-                                   C.scalars[kSsDtByDx], i, j);
+                                   C.scalars[kSsDtByDx], b.i, b.j);
 }
 
 // The vorticity: w0 (after the stress update) -> w1.
-__global__ void soft_batch_advect_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.nr || j >= C.nz) return;
+__global__ void soft_batch_advect_kernel(const SoftCase* __restrict__ cs,
+                                         const uint32_t* __restrict__ blocks) {
+  const SoftBlock b = soft_block(blocks);
+  const SoftCase& C = cs[b.c];
+  if (b.j >= C.nz) return;
   eno3_conservative_at<double>(C.w0, C.u_z, C.u_r, C.w1, C.nr, C.nz, C.scalars[kSsDtByDx], -1.0,
-                               i, j);
+                               b.i, b.j);
 }
 
 hipError_t launch_soft_batch_refmap(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
-  soft_batch_refmap_kernel<<<dim3(ceil_div(d.max_nz, 256), d.max_nr, d.ncase), 256, 0, s>>>(cases);
+  soft_batch_refmap_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   return hipGetLastError();
 }
 
 hipError_t launch_soft_batch_advect(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
-  soft_batch_advect_kernel<<<dim3(ceil_div(d.max_nz, 256), d.max_nr, d.ncase), 256, 0, s>>>(cases);
+  soft_batch_advect_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   return hipGetLastError();
 }
 

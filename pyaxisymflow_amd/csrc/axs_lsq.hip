@@ -372,11 +372,12 @@ __global__ void __launch_bounds__(512) lsq_sweep_tail_kernel(
 // coordinates as in the single-case step; each case has its own list,
 // counters and tail lists.
 
-__global__ void soft_batch_lsq_list_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int I = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= 2 * C.nr) return;
-  const int64_t c = (int64_t)I * C.nz + j;
+__global__ void soft_batch_lsq_list_kernel(const SoftCase* __restrict__ cs,
+                                           const uint32_t* __restrict__ blocks) {
+  const SoftBlock b = soft_block(blocks);  // a row of the doubled grid
+  const SoftCase& C = cs[b.c];
+  const int j = b.j;
+  const int64_t c = (int64_t)b.i * C.nz + j;
   list_append(j < C.nz && (C.cur[c] ^ C.tgt[c]), (int)c, C.lw.list, C.lw.count);
 }
 
@@ -422,8 +423,7 @@ __global__ void __launch_bounds__(512) soft_batch_lsq_tail_kernel(const SoftCase
 
 hipError_t launch_soft_batch_lsq_list(const SoftCase* cases, const SoftBatchDims& d,
                                       hipStream_t s) {
-  soft_batch_lsq_list_kernel<<<dim3(ceil_div(d.max_nz, 256), 2 * d.max_nr, d.ncase), 256, 0, s>>>(
-      cases);
+  soft_batch_lsq_list_kernel<<<d.ndbl_blocks, 256, 0, s>>>(cases, d.dbl_blocks);
   return hipGetLastError();
 }
 

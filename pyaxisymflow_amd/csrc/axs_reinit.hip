@@ -185,46 +185,49 @@ __global__ void reinit_final_kernel(const double* __restrict__ phi, const double
   reinit_final_at(phi, D, dist, frozen, out, masked, c, narrow);
 }
 
-// Soft batches: cell (blockIdx.y, blockIdx.x * 256 + tid) of case
-// blockIdx.z; phi is reinitialised in place.  A case sweeps its own count
-// of iterations: sweeps beyond it exit, and its final pass reads the buffer
+// Soft batches: one row segment of 256 cells of one case per workgroup;
+// phi is reinitialised in place.  A case sweeps its own count of
+// iterations: sweeps beyond it exit, and its final pass reads the buffer
 // its own last sweep wrote.
-__global__ void soft_batch_reinit_init_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.nr || j >= C.nz) return;
-  reinit_init_at(C.phi, C.rw.d0, C.rw.dist, C.rw.frozen, C.nr, C.nz, C.dx, i, j);
+__global__ void soft_batch_reinit_init_kernel(const SoftCase* __restrict__ cs,
+                                              const uint32_t* __restrict__ blocks) {
+  const SoftBlock b = soft_block(blocks);
+  const SoftCase& C = cs[b.c];
+  if (b.j >= C.nz) return;
+  reinit_init_at(C.phi, C.rw.d0, C.rw.dist, C.rw.frozen, C.nr, C.nz, C.dx, b.i, b.j);
 }
 
-__global__ void soft_batch_reinit_sweep_kernel(const SoftCase* __restrict__ cs, int q) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= C.reinit_iterations || i >= C.nr || j >= C.nz) return;
+__global__ void soft_batch_reinit_sweep_kernel(const SoftCase* __restrict__ cs,
+                                               const uint32_t* __restrict__ blocks, int q) {
+  const SoftBlock b = soft_block(blocks);
+  const SoftCase& C = cs[b.c];
+  if (q >= C.reinit_iterations || b.j >= C.nz) return;
   const bool odd = q & 1;
   reinit_sweep_at(odd ? C.rw.d1 : C.rw.d0, odd ? C.rw.d0 : C.rw.d1, C.phi, C.rw.frozen, C.nr,
-                  C.nz, C.dx, i, j);
+                  C.nz, C.dx, b.i, b.j);
 }
 
-__global__ void soft_batch_reinit_final_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.nr || j >= C.nz) return;
+__global__ void soft_batch_reinit_final_kernel(const SoftCase* __restrict__ cs,
+                                               const uint32_t* __restrict__ blocks) {
+  const SoftBlock b = soft_block(blocks);
+  const SoftCase& C = cs[b.c];
+  if (b.j >= C.nz) return;
   const double* D = (C.reinit_iterations & 1) ? C.rw.d1 : C.rw.d0;
-  reinit_final_at(C.phi, D, C.rw.dist, C.rw.frozen, C.phi, nullptr, (int64_t)i * C.nz + j,
+  reinit_final_at(C.phi, D, C.rw.dist, C.rw.frozen, C.phi, nullptr, (int64_t)b.i * C.nz + b.j,
                   C.reinit_band);
 }
 
 hipError_t launch_soft_batch_reinit(const SoftCase* cases, const SoftBatchDims& d, int iterations,
                                     hipStream_t s) {
-  const dim3 g(ceil_div(d.max_nz, 256), d.max_nr, d.ncase);
-  soft_batch_reinit_init_kernel<<<g, 256, 0, s>>>(cases);
+  const int g = d.ncell_blocks;
+  soft_batch_reinit_init_kernel<<<g, 256, 0, s>>>(cases, d.cell_blocks);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   for (int q = 0; q < iterations; ++q) {
-    soft_batch_reinit_sweep_kernel<<<g, 256, 0, s>>>(cases, q);
+    soft_batch_reinit_sweep_kernel<<<g, 256, 0, s>>>(cases, d.cell_blocks, q);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  soft_batch_reinit_final_kernel<<<g, 256, 0, s>>>(cases);
+  soft_batch_reinit_final_kernel<<<g, 256, 0, s>>>(cases, d.cell_blocks);
   return hipGetLastError();
 }
 

@@ -498,29 +498,32 @@ __global__ void soft_batch_dt_kernel(const SoftCase* __restrict__ cs) {
   soft_dt_body(C.scalars, C.slots, C.CFL, C.dx, C.eps, C.G, C.rho_f, C.nu);
 }
 
-__global__ void soft_batch_pin_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.nr || j >= C.nz) return;
-  soft_pin_at(C.eta1, C.eta2, C.e1n, C.e2n, C.phi, (int64_t)i * C.nz + j, C.z_cm, C.r_cm,
+__global__ void soft_batch_pin_kernel(const SoftCase* __restrict__ cs,
+                                      const uint32_t* __restrict__ blocks) {
+  const SoftBlock b = soft_block(blocks);
+  const SoftCase& C = cs[b.c];
+  if (b.j >= C.nz) return;
+  soft_pin_at(C.eta1, C.eta2, C.e1n, C.e2n, C.phi, (int64_t)b.i * C.nz + b.j, C.z_cm, C.r_cm,
               C.r_ball, C.dx);
 }
 
-__global__ void soft_batch_mirror_kernel(const SoftCase* __restrict__ cs, int ncounters) {
-  const SoftCase& C = cs[blockIdx.z];
-  if (blockIdx.x == 0 && blockIdx.y == 0)
+__global__ void soft_batch_mirror_kernel(const SoftCase* __restrict__ cs,
+                                         const uint32_t* __restrict__ blocks, int ncounters) {
+  const SoftBlock b = soft_block(blocks);  // a row of the doubled grid
+  const SoftCase& C = cs[b.c];
+  if (b.i == 0 && b.j < 256)  // the case's first workgroup clears its sweep counters
     for (int k = threadIdx.x; k < ncounters; k += blockDim.x) C.lw.counters[k] = 0u;
-  const int I = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= 2 * C.nr || j >= C.nz) return;
+  if (b.j >= C.nz) return;
   soft_lsq_mirror_at(C.phi, C.chi, C.moll_zone, C.eta1, C.eta2, C.e1d, C.e2d, C.cur, C.tgt,
-                     C.lw.ex2, C.lw.ey2, C.lw.cur2, C.nr, C.nz, C.extrap_zone, I, j);
+                     C.lw.ex2, C.lw.ey2, C.lw.cur2, C.nr, C.nz, C.extrap_zone, b.i, b.j);
 }
 
-__global__ void soft_batch_back_kernel(const SoftCase* __restrict__ cs, int sweeps) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.nr || j >= C.nz) return;
-  const int64_t n = (int64_t)C.nr * C.nz, c = (int64_t)i * C.nz + j;
+__global__ void soft_batch_back_kernel(const SoftCase* __restrict__ cs,
+                                       const uint32_t* __restrict__ blocks, int sweeps) {
+  const SoftBlock b = soft_block(blocks);
+  const SoftCase& C = cs[b.c];
+  if (b.j >= C.nz) return;
+  const int64_t n = (int64_t)C.nr * C.nz, c = (int64_t)b.i * C.nz + b.j;
   const bool second = lsq_result_is_second(C.lw.counters, sweeps);
   C.eta1[c] = (second ? C.lw.ex2 : C.e1d)[n + c];
   C.eta2[c] = (second ? C.lw.ey2 : C.e2d)[n + c];
@@ -563,6 +566,8 @@ struct SoftBatch {
   hipStream_t graph_stream = nullptr;
   bool table_dirty = true;
   int64_t chunk_launches = 0, step_launches = 0, captures = 0;
+  uint32_t* d_blocks = nullptr;  // cell then doubled-grid block tables
+  size_t blocks_cap = 0;
 
   hipError_t init(int cap) {
     max_cases = cap;
@@ -595,6 +600,7 @@ struct SoftBatch {
     if (need_host) (void)hipHostFree(need_host);
     if (need) (void)hipFree(need);
     if (d_cases) (void)hipFree(d_cases);
+    if (d_blocks) (void)hipFree(d_blocks);
   }
 
   static SoftCase describe(const SoftStepper& S, unsigned* need) {
@@ -685,9 +691,39 @@ struct SoftBatch {
           std::max(dims.tri_lds, tri_lds_bytes(P.nr, kSoftBatchTriL, P.tri_K, P.tri_P, false));
     }
     dims.ncase = (int)cases.size();
+    // block tables: every 256-cell row segment of every case, over its grid
+    // and over its doubled grid
+    std::vector<uint32_t> tab;
+    for (int c = 0; c < dims.ncase; ++c) {
+      const SoftCase& C = h_cases[c];
+      for (int i = 0; i < C.nr; ++i)
+        for (int b = 0; b < (int)ceil_div(C.nz, 256); ++b) tab.push_back(soft_block_entry(c, i, b));
+    }
+    dims.ncell_blocks = (int)tab.size();
+    for (int c = 0; c < dims.ncase; ++c) {
+      const SoftCase& C = h_cases[c];
+      for (int i = 0; i < 2 * C.nr; ++i)
+        for (int b = 0; b < (int)ceil_div(C.nz, 256); ++b) tab.push_back(soft_block_entry(c, i, b));
+    }
+    dims.ndbl_blocks = (int)tab.size() - dims.ncell_blocks;
     drop_graphs();
     table_dirty = false;
     if (cases.empty()) return hipSuccess;
+    hipError_t e;
+    if (tab.size() > blocks_cap) {
+      if (d_blocks) (void)hipFree(d_blocks);
+      d_blocks = nullptr;
+      blocks_cap = 0;
+      if ((e = hipMalloc(reinterpret_cast<void**>(&d_blocks), sizeof(uint32_t) * tab.size())) !=
+          hipSuccess)
+        return e;
+      blocks_cap = tab.size();
+    }
+    if ((e = hipMemcpy(d_blocks, tab.data(), sizeof(uint32_t) * tab.size(),
+                       hipMemcpyHostToDevice)) != hipSuccess)
+      return e;
+    dims.cell_blocks = d_blocks;
+    dims.dbl_blocks = d_blocks + dims.ncell_blocks;
     return hipMemcpy(d_cases, h_cases.data(), sizeof(SoftCase) * cases.size(),
                      hipMemcpyHostToDevice);
   }
@@ -743,8 +779,6 @@ struct SoftBatch {
   do {                                                   \
     if ((e = hipGetLastError()) != hipSuccess) return e; \
   } while (0)
-    const dim3 g(ceil_div(d.max_nz, 256), d.max_nr, d.ncase);
-    const dim3 gd(ceil_div(d.max_nz, 256), 2 * d.max_nr, d.ncase);
     int iters = 0;
     for (const SoftCase& c : h_cases) iters = std::max(iters, c.reinit_iterations);
     AXS_TRY_STEP(launch_soft_batch_dct2(T, d, s));
@@ -754,15 +788,15 @@ struct SoftBatch {
     soft_batch_dt_kernel<<<d.ncase, 64, 0, s>>>(T);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_soft_batch_refmap(T, d, s));
-    soft_batch_pin_kernel<<<g, 256, 0, s>>>(T);
+    soft_batch_pin_kernel<<<d.ncell_blocks, 256, 0, s>>>(T, d.cell_blocks);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_soft_batch_reinit(T, d, iters, s));
-    soft_batch_mirror_kernel<<<gd, 256, 0, s>>>(T, sweeps + 1);
+    soft_batch_mirror_kernel<<<d.ndbl_blocks, 256, 0, s>>>(T, d.dbl_blocks, sweeps + 1);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_soft_batch_lsq_list(T, d, s));
     for (int q = 0; q < sweeps; ++q) AXS_TRY_STEP(launch_soft_batch_lsq_sweep(T, d, q, s));
     AXS_TRY_STEP(launch_soft_batch_lsq_tail(T, d, sweeps, s));
-    soft_batch_back_kernel<<<g, 256, 0, s>>>(T, sweeps);
+    soft_batch_back_kernel<<<d.ncell_blocks, 256, 0, s>>>(T, d.cell_blocks, sweeps);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_soft_batch_stress(T, d, s));
     AXS_TRY_STEP(launch_soft_batch_advect(T, d, s));
@@ -929,7 +963,8 @@ axs_status axs_soft_stepper_create(const axs_soft_config* c, const axs_soft_buff
 axs_status axs_soft_batch_create(int max_cases, axs_soft_batch** out) {
   AXS_CHECK_ARG(out, "null pointer");
   *out = nullptr;
-  AXS_CHECK_ARG(max_cases >= 1 && max_cases <= 4096, "max_cases must be in [1, 4096]");
+  AXS_CHECK_ARG(max_cases >= 1 && max_cases <= kSoftBatchMaxCases,
+                "max_cases must be in [1, 1024]");
   axs_soft_batch* bt = new (std::nothrow) axs_soft_batch();
   if (!bt) {
     set_error("out of host memory");

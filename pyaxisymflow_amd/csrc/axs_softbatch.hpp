@@ -2,11 +2,11 @@
 // advanced by one launch per pass over all of them (the serving form of
 // examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:101-213).
 //
-// Every pass takes the device table of SoftCase entries; the case of a
-// workgroup is blockIdx.z for the row-parallel passes (grid sized for the
-// largest case; a workgroup beyond its own case's rows or columns exits at
-// once) and blockIdx.y for the passes with one workgroup per case or per
-// row.  A case's arithmetic is the single-case step's, so a case gives the
+// Every pass takes the device table of SoftCase entries; the per-cell
+// passes take a block table (256 cells of one row of one case per
+// workgroup), the row transforms and the radial solve a grid sized for the
+// largest case (blockIdx.y = case; workgroups beyond a case's rows or mode
+// chunks exit at once), and the per-case passes one workgroup per case.  A case's arithmetic is the single-case step's, so a case gives the
 // same bits alone and beside others.
 #pragma once
 
@@ -50,13 +50,36 @@ struct SoftCase {
   int tK, tP;
 };
 
-// Launch shapes of a batch (host side): the largest case's dimensions and
-// the any-length transform's block shape over all cases.
+// Launch shapes of a batch (host side): the largest case's dimensions, the
+// any-length transform's block shape over all cases, and the block tables
+// of the per-cell passes (device arrays): one entry per 256-cell row
+// segment of every case, over its (nr, nz) grid and over its doubled
+// (2 nr, nz) grid, so that a per-cell pass launches exactly the workgroups
+// the cases have (a grid sized for the largest case would launch mostly
+// empty ones).
 struct SoftBatchDims {
   int ncase = 0, max_nr = 0, max_nz = 0, max_chunks = 0;
   int any_nth = 64, any_per = 1;
   size_t any_lds = 0, tri_lds = 0;
+  const uint32_t* cell_blocks = nullptr;
+  int ncell_blocks = 0;
+  const uint32_t* dbl_blocks = nullptr;
+  int ndbl_blocks = 0;
 };
+
+// Block-table entry: case (10 bits), row (13 bits), 256-column block (9 bits).
+constexpr int kSoftBatchMaxCases = 1024;
+__host__ __device__ constexpr uint32_t soft_block_entry(int c, int row, int colblk) {
+  return uint32_t(c) | (uint32_t(row) << 10) | (uint32_t(colblk) << 23);
+}
+struct SoftBlock {
+  int c, i, j;  // case, row, this thread's column
+};
+__device__ __forceinline__ SoftBlock soft_block(const uint32_t* __restrict__ t) {
+  const uint32_t v = t[blockIdx.x];
+  return SoftBlock{int(v & 1023u), int((v >> 10) & 8191u),
+                   int(v >> 23) * 256 + int(threadIdx.x)};
+}
 
 // Radial segment length of every case of a batch (the tridiagonal launch
 // is compiled for one L).

@@ -326,16 +326,16 @@ __global__ void div_tau_stage2_kernel(T* __restrict__ w, const T* __restrict__ t
 }
 
 // ---------------------------------------------------------------- soft batches
-// Cell (blockIdx.y, blockIdx.x * 256 + tid) of case blockIdx.z (rows and
-// columns beyond the case's own grid exit).
+// One row segment of 256 cells of one case per workgroup (the batch's
+// block table); columns beyond the case's own nz exit.
 
 // u from psi, and the step's max(|u_z| + |u_r|) folded per workgroup into the
 // case's spread atomicMax slots (the maximum does not depend on the order).
-__global__ void __launch_bounds__(256) soft_batch_velocity_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int i = blockIdx.y;
-  if (i >= C.nr) return;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) soft_batch_velocity_kernel(
+    const SoftCase* __restrict__ cs, const uint32_t* __restrict__ blocks) {
+  const SoftBlock b = soft_block(blocks);
+  const SoftCase& C = cs[b.c];
+  const int i = b.i, j = b.j;
   double m = 0.0;
   if (j < C.nz) {
     double uz, ur;
@@ -349,75 +349,82 @@ __global__ void __launch_bounds__(256) soft_batch_velocity_kernel(const SoftCase
   __syncthreads();
   if (threadIdx.x == 0) {
     m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
-    const int slot = (blockIdx.y * gridDim.x + blockIdx.x) % kMaxSlots;
+    const int slot = blockIdx.x % kMaxSlots;
     atomicMax(reinterpret_cast<unsigned long long*>(C.slots + slot * kSlotStride),
               static_cast<unsigned long long>(__double_as_longlong(m)));
   }
 }
 
-__global__ void soft_batch_sigma_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.nr || j >= C.nz) return;
+__global__ void soft_batch_sigma_kernel(const SoftCase* __restrict__ cs,
+                                        const uint32_t* __restrict__ blocks) {
+  const SoftBlock b = soft_block(blocks);
+  const SoftCase& C = cs[b.c];
+  const int i = b.i, j = b.j;
+  if (j >= C.nz) return;
   solid_sigma_at<double>(C.s11, C.s12, C.s22, C.G, C.dx, C.eta1, C.eta2, C.g1z, C.g1r, C.g2z,
                          C.g2r, C.nr, C.nz, C.chi, i, j);
 }
 
-__global__ void soft_batch_div_tau1_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+__global__ void soft_batch_div_tau1_kernel(const SoftCase* __restrict__ cs,
+                                           const uint32_t* __restrict__ blocks) {
+  const SoftBlock b = soft_block(blocks);
+  const SoftCase& C = cs[b.c];
+  const int i = b.i, j = b.j + 1;
   if (i >= C.nr - 1 || j >= C.nz - 1) return;
   div_tau_stage1_at<double, RRow<double>>(C.tz, C.tr, C.s11, C.s12, C.s22, RRow<double>{C.r},
                                           C.nz, 2.0 * C.dx, i, j);
 }
 
-__global__ void soft_batch_div_tau2_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int i = blockIdx.y + 1, j = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  if (i >= C.nr - 1 || j >= C.nz - 1) return;
+__global__ void soft_batch_div_tau2_kernel(const SoftCase* __restrict__ cs,
+                                           const uint32_t* __restrict__ blocks) {
+  const SoftBlock b = soft_block(blocks);
+  const SoftCase& C = cs[b.c];
+  const int i = b.i, j = b.j + 1;
+  if (i < 1 || i >= C.nr - 1 || j >= C.nz - 1) return;
   div_tau_stage2_at<double>(C.w0, C.tz, C.tr, C.nz, C.scalars[kSsDt], 2.0 * C.dx, i, j);
 }
 
-__global__ void soft_batch_rk2_stage1_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.nr || j >= C.nz) return;
+__global__ void soft_batch_rk2_stage1_kernel(const SoftCase* __restrict__ cs,
+                                             const uint32_t* __restrict__ blocks) {
+  const SoftBlock b = soft_block(blocks);
+  const SoftCase& C = cs[b.c];
+  const int i = b.i, j = b.j;
+  if (j >= C.nz) return;
   rk2_stage1_at<double, RRow<double>>(C.w1, C.tmp, RRow<double>{C.r}, C.nr, C.nz,
                                       C.scalars[kSsHalfNuDt], C.dx * C.dx, 2.0 * C.dx, i, j);
 }
 
-__global__ void soft_batch_rk2_stage2_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.z];
-  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.nr || j >= C.nz) return;
+__global__ void soft_batch_rk2_stage2_kernel(const SoftCase* __restrict__ cs,
+                                             const uint32_t* __restrict__ blocks) {
+  const SoftBlock b = soft_block(blocks);
+  const SoftCase& C = cs[b.c];
+  const int i = b.i, j = b.j;
+  if (j >= C.nz) return;
   rk2_stage2_out_at<double, RRow<double>>(C.w1, C.tmp, C.vorticity, RRow<double>{C.r}, C.nr, C.nz,
                                           C.scalars[kSsNuDt], C.dx * C.dx, 2.0 * C.dx, i, j);
 }
 
 hipError_t launch_soft_batch_velocity(const SoftCase* cases, const SoftBatchDims& d,
                                       hipStream_t s) {
-  soft_batch_velocity_kernel<<<dim3(ceil_div(d.max_nz, 256), d.max_nr, d.ncase), 256, 0, s>>>(
-      cases);
+  soft_batch_velocity_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   return hipGetLastError();
 }
 
 hipError_t launch_soft_batch_stress(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
-  const dim3 g(ceil_div(d.max_nz, 256), d.max_nr, d.ncase);
-  soft_batch_sigma_kernel<<<g, 256, 0, s>>>(cases);
+  soft_batch_sigma_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  soft_batch_div_tau1_kernel<<<g, 256, 0, s>>>(cases);
+  soft_batch_div_tau1_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  soft_batch_div_tau2_kernel<<<g, 256, 0, s>>>(cases);
+  soft_batch_div_tau2_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   return hipGetLastError();
 }
 
 hipError_t launch_soft_batch_diffuse(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
-  const dim3 g(ceil_div(d.max_nz, 256), d.max_nr, d.ncase);
-  soft_batch_rk2_stage1_kernel<<<g, 256, 0, s>>>(cases);
+  soft_batch_rk2_stage1_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  soft_batch_rk2_stage2_kernel<<<g, 256, 0, s>>>(cases);
+  soft_batch_rk2_stage2_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   return hipGetLastError();
 }
 
