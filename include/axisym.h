@@ -454,6 +454,35 @@ axs_status axs_soft_batch_info(const axs_soft_batch* b, int* ncase, int* queued_
                                int64_t* graph_launches, int64_t* captures);
 axs_status axs_soft_batch_destroy(axs_soft_batch* b);
 
+/* ---- Device-resident step of examples/PeriodicRigidSlab/periodic_rigid_slab.py:93-158:
+ * a rigid annular wall (char_func) oscillating along z at U0 cos(omega t),
+ * Brinkman penalised, in a z-periodic domain of nz columns whose first and
+ * last `ghost` columns are ghost copies; the periodic Stokes solve
+ * (Neumann r) runs on the nz - 2 ghost interior columns.  Step order:
+ * kill_boundary_vorticity_sine_r -> psi -> compute_velocity_from_psi_periodic
+ * -> dt = min(0.9 dx^2 / 4 / nu, CFL dx / (max|u_z| + |u_r| + eps)) ->
+ * brinkmann_penalize (u becomes the penalised velocity) ->
+ * compute_vorticity_from_velocity_periodic of the correction, added to omega
+ * -> advect_vorticity_via_eno3_periodic -> diffusion_RK2_periodic.  FP64.
+ * Buffers: (nr, nz) vorticity, psi, u_z, u_r, char_func; scalars
+ * AXS_RIGID_SCALARS doubles (t, dt, dt/dx, nu dt/2, nu dt, U0 cos(omega t),
+ * max|u|, step counter); r (nr); kill_wr (kill_width weights of the top
+ * rows, sin(pi (1 - r - dx/2) / 2 / (kill_width - 1) / dx) as the reference
+ * forms them). */
+typedef struct axs_slab_stepper axs_slab_stepper;
+typedef struct {
+  int nr, nz, ghost, kill_width;
+  double dx, nu, lam, U0, omega, CFL, eps;
+} axs_slab_config;
+typedef struct {
+  double *vorticity, *psi, *u_z, *u_r, *scalars;
+  const double *char_func, *r, *kill_wr;
+} axs_slab_buffers;
+axs_status axs_slab_stepper_create(const axs_slab_config* cfg, const axs_slab_buffers* bufs,
+                                   axs_slab_stepper** out);
+axs_status axs_slab_stepper_destroy(axs_slab_stepper* st);
+axs_status axs_slab_stepper_run(axs_slab_stepper* st, int nsteps, int use_graph, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,6 +4,10 @@ oracle kernels.  TEST ORACLE ONLY.
 * ``RigidBodyCase`` follows examples/FlowPastSphere/flow_past_sphere.py:39-188
   (static body, free-stream ramp, drag coefficient) and, with an oscillating
   body, examples/SpheroidStreaming/spheroid_streaming.py:129-246.
+* ``PeriodicRigidSlabCase`` follows
+  examples/PeriodicRigidSlab/periodic_rigid_slab.py:33-158 (z-periodic,
+  oscillating annular wall), and ``womersley_wall_driven`` restates its
+  closed-form theory (theory_rigid_slab.py:9-23).
 * Synthetic inputs (seeded Gaussian vorticity, sphere masks) follow the
   workload definitions used by the tests and the benchmark.
 """
@@ -363,3 +367,99 @@ class SoftSphereCase:
         self.it += 1
         self.dts.append(dt)
         return dt
+
+
+class PeriodicRigidSlabCase:
+    """examples/PeriodicRigidSlab/periodic_rigid_slab.py:33-158: a rigid
+    annular wall (|r - R_wall_center| < wall_thickness, smoothed over
+    dx sqrt(2)) oscillating along z at U_0 cos(omega t) in a z-periodic
+    domain of grid_size_z columns, the first and last ghost_size of them
+    ghost copies; the periodic Stokes solve (Neumann r) runs on the
+    interior columns.  ``solver`` "exact" (DFT + banded radial solves) or
+    "eig" (the reference's dense route)."""
+
+    def __init__(self, grid_size_z=16, max_z=0.05, max_r=0.5, brink_lam=1e4,
+                 wall_thickness=0.05, R_wall_center=0.3, Re=25, U_0=1.0, freq=8.0,
+                 ghost_size=2, CFL=0.1, solver="exact"):
+        from oracle import periodic as P
+        from oracle.stokes import NEUMANN_R_PERIODIC_Z
+
+        self.P = P
+        self.bc = NEUMANN_R_PERIODIC_Z
+        domain_AR = max_r / max_z
+        nz, nr = grid_size_z, int(domain_AR * grid_size_z)
+        dx = max_r / nr
+        self.nz, self.nr, self.dx = nz, nr, dx
+        self.z = np.linspace(0 + dx / 2, max_z - dx / 2, nz)
+        self.r = np.linspace(0 + dx / 2, max_r - dx / 2, nr)
+        self.Z, self.R = np.meshgrid(self.z, self.r)
+        self.brink_lam = brink_lam
+        self.moll_zone = dx * 2**0.5
+        self.R_tube = R_wall_center - wall_thickness
+        self.U_0 = U_0
+        self.nu = U_0 * 2 * self.R_tube / Re
+        self.omega = 2 * np.pi * freq
+        self.g = ghost_size
+        self.CFL = CFL
+        self.eps = np.finfo(float).eps
+        self.comm = P.gen_periodic_boundary_ghost_comm(ghost_size)
+        z0 = 0 * self.Z
+        self.vorticity, self.penal_vorticity, self.temp_vorticity = z0.copy(), z0.copy(), z0.copy()
+        self.psi, self.u_z, self.u_r = z0.copy(), z0.copy(), z0.copy()
+        self.u_z_upen, self.u_r_upen = z0.copy(), z0.copy()
+        phi0 = -np.sqrt((self.R - R_wall_center) ** 2) + wall_thickness
+        self.char_func = 0 * self.Z
+        K.smooth_Heaviside(self.char_func, phi0, self.moll_zone)
+        self.psi_inner = self.psi[..., ghost_size:-ghost_size].copy()
+        self.solver_kind = solver
+        if solver == "eig":
+            self.solver = FastDiagonalisationStokesSolver(nr, nz - 2 * ghost_size, dx,
+                                                          bc_type=self.bc)
+        self.t = 0.0
+        self.it = 0
+        self.dts = []
+
+    def step(self):
+        """periodic_rigid_slab.py:93-158, one iteration."""
+        P, g, dx = self.P, self.g, self.dx
+        K.kill_boundary_vorticity_sine_r(self.vorticity, self.R, 3, dx)
+        self.psi_inner[...] = self.psi[..., g:-g]
+        if self.solver_kind == "eig":
+            self.solver.solve(solution_field=self.psi_inner,
+                              rhs_field=self.vorticity[:, g:-g])
+        else:
+            self.psi_inner[...] = exact_stokes_solve(self.vorticity[:, g:-g], dx, self.bc)
+        self.psi[..., g:-g] = self.psi_inner
+        P.compute_velocity_from_psi_periodic(self.u_z, self.u_r, self.psi, self.R, dx, self.comm)
+        dt = min(
+            0.9 * dx**2 / 4 / self.nu,
+            self.CFL * dx / (np.amax(np.fabs(self.u_z) + np.fabs(self.u_r)) + self.eps),
+        )
+        self.u_z_upen[...] = self.u_z.copy()
+        self.u_r_upen[...] = self.u_r.copy()
+        K.brinkmann_penalize(self.brink_lam, dt, self.char_func,
+                             self.U_0 * np.cos(self.omega * self.t), 0.0, self.u_z_upen,
+                             self.u_r_upen, self.u_z, self.u_r)
+        P.compute_vorticity_from_velocity_periodic(self.penal_vorticity,
+                                                   self.u_z - self.u_z_upen,
+                                                   self.u_r - self.u_r_upen, dx, self.comm)
+        self.vorticity[...] += self.penal_vorticity
+        P.advect_vorticity_via_eno3_periodic(self.vorticity, self.u_z, self.u_r, dt, dx,
+                                             self.comm)
+        P.diffusion_RK2_periodic(self.vorticity, self.temp_vorticity, self.R, self.nu, dt, dx,
+                                 self.comm)
+        self.t += dt
+        self.it += 1
+        self.dts.append(dt)
+        return dt
+
+
+def womersley_wall_driven(r, t, U_0, omega, nu, R_tube):
+    """theory_rigid_slab.py:9-23: axial velocity in a tube of radius R_tube
+    whose wall moves at U_0 cos(omega t), u = U_0 Re(2 A J0(lam r) e^{-i omega t})
+    with lam = sqrt(i omega / nu) and A = 1 / (2 J0(lam R_tube))."""
+    from scipy.special import jv
+
+    lam = np.sqrt(1j * (omega / nu))
+    A = 0.5 / jv(0, lam * R_tube)
+    return U_0 * np.real(2 * (A * jv(0, lam * r)) * np.exp(-1j * omega * t))

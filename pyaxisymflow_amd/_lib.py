@@ -68,6 +68,20 @@ class SoftBuffers(ctypes.Structure):
         "kill_wz", "kill_wr")]
 
 
+class SlabConfig(ctypes.Structure):
+    """axs_slab_config (include/axisym.h)."""
+
+    _fields_ = [(n, ctypes.c_int) for n in ("nr", "nz", "ghost", "kill_width")] + [
+        (n, ctypes.c_double) for n in ("dx", "nu", "lam", "U0", "omega", "CFL", "eps")]
+
+
+class SlabBuffers(ctypes.Structure):
+    """axs_slab_buffers (include/axisym.h)."""
+
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "vorticity", "psi", "u_z", "u_r", "scalars", "char_func", "r", "kill_wr")]
+
+
 AXS_F64 = 0
 AXS_F32 = 1
 AXS_RIGID_SCALARS = 24
@@ -127,6 +141,10 @@ SIGNATURES = {
     "axs_soft_batch_run": [_P, _I, _I, _P],
     "axs_soft_batch_info": [_P, _P, _P, _P, _P],
     "axs_soft_batch_destroy": [_P],
+    "axs_slab_stepper_create": [ctypes.POINTER(SlabConfig), ctypes.POINTER(SlabBuffers),
+                                ctypes.POINTER(_P)],
+    "axs_slab_stepper_destroy": [_P],
+    "axs_slab_stepper_run": [_P, _I, _I, _P],
     "axs_rigid_batch_create": [ctypes.POINTER(RigidConfig), _I, ctypes.POINTER(RigidBuffers),
                                ctypes.POINTER(_P)],
     "axs_rigid_stepper_destroy": [_P],

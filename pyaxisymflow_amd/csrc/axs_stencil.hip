@@ -210,6 +210,28 @@ hipError_t launch_rk2(const T* w_in, T* tmp, T* w_out, RA R, int nr, int nz, Sca
       w_in, tmp, w_out, R, nr, nz, nu_dt, dx2, T(2) * dx);
   return hipGetLastError();
 }
+template <typename T, class RA>
+hipError_t launch_rk2_stage1(const T* w_in, T* tmp, RA R, int nr, int nz, Scalar<T> half_nu_dt,
+                             T dx, hipStream_t s) {
+  rk2_stage1_kernel<T, RA><<<dim3(ceil_div(nz, 256), nr), 256, 0, s>>>(w_in, tmp, R, nr, nz,
+                                                                       half_nu_dt, dx * dx,
+                                                                       T(2) * dx);
+  return hipGetLastError();
+}
+
+template <typename T, class RA>
+hipError_t launch_rk2_stage2(const T* w_in, const T* tmp, T* w_out, RA R, int nr, int nz,
+                             Scalar<T> nu_dt, T dx, hipStream_t s) {
+  rk2_stage2_out_kernel<T, RA><<<dim3(ceil_div(nz, 256), nr), 256, 0, s>>>(
+      w_in, tmp, w_out, R, nr, nz, nu_dt, dx * dx, T(2) * dx);
+  return hipGetLastError();
+}
+template hipError_t launch_rk2_stage1<double, RRow<double>>(const double*, double*, RRow<double>,
+                                                            int, int, Scalar<double>, double,
+                                                            hipStream_t);
+template hipError_t launch_rk2_stage2<double, RRow<double>>(const double*, const double*, double*,
+                                                            RRow<double>, int, int,
+                                                            Scalar<double>, double, hipStream_t);
 template hipError_t launch_rk2<double, RRow<double>>(const double*, double*, double*, RRow<double>,
                                                      int, int, Scalar<double>, Scalar<double>,
                                                      double, hipStream_t);

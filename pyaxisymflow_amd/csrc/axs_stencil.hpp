@@ -25,6 +25,14 @@ namespace axs {
 template <typename T, class RA>
 hipError_t launch_rk2(const T* w_in, T* tmp, T* w_out, RA R, int nr, int nz, Scalar<T> half_nu_dt,
                       Scalar<T> nu_dt, T dx, hipStream_t s);
+// The two stages of launch_rk2 as separate launches (the periodic form
+// exchanges the ghost columns of tmp between them, diffusion_RK2.py:47-90).
+template <typename T, class RA>
+hipError_t launch_rk2_stage1(const T* w_in, T* tmp, RA R, int nr, int nz, Scalar<T> half_nu_dt,
+                             T dx, hipStream_t s);
+template <typename T, class RA>
+hipError_t launch_rk2_stage2(const T* w_in, const T* tmp, T* w_out, RA R, int nr, int nz,
+                             Scalar<T> nu_dt, T dx, hipStream_t s);
 // compute_velocity_from_psi.py:4-17 with r per row.
 template <typename T>
 hipError_t launch_velocity_from_psi(T* u_z, T* u_r, const T* psi, const double* r, int nr, int nz,

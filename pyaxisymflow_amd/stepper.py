@@ -794,3 +794,121 @@ class SoftSphereBatch:
             except Exception:
                 pass
             self._h = None
+
+
+class PeriodicRigidSlabStepper:
+    """examples/PeriodicRigidSlab/periodic_rigid_slab.py:33-158 on the GPU: a
+    rigid annular wall (|r - R_wall_center| < wall_thickness, smoothed over
+    dx sqrt(2)) oscillating along z at U_0 cos(omega t) in a z-periodic
+    domain (ghost_size ghost columns at each end), every stage of the loop
+    body (:93-158) device resident and replayed from hipGraphs
+    (axs_slab_stepper_*).  Parameters and derived quantities follow the
+    driver: nr = (max_r / max_z) grid_size_z, dx = max_r / nr,
+    nu = 2 U_0 R_tube / Re with R_tube = R_wall_center - wall_thickness,
+    omega = 2 pi freq."""
+
+    def __init__(self, grid_size_z=16, max_z=0.05, max_r=0.5, brink_lam=1e4, wall_thickness=0.05,
+                 R_wall_center=0.3, Re=25, U_0=1.0, freq=8.0, ghost_size=2, CFL=0.1,
+                 use_graph=True, kill_width=3):
+        torch = _lib.torch()
+        dev = _lib.device()
+        domain_AR = max_r / max_z
+        nz, nr = int(grid_size_z), int(domain_AR * grid_size_z)
+        dx = max_r / nr
+        self.nz, self.nr, self.dx = nz, nr, dx
+        self.use_graph = use_graph
+        self.stream = torch.cuda.Stream(device=dev)
+        self.z = np.linspace(0 + dx / 2, max_z - dx / 2, nz)
+        self.r = np.linspace(0 + dx / 2, max_r - dx / 2, nr)
+        Z, R = np.meshgrid(self.z, self.r)
+        self.R_tube = R_wall_center - wall_thickness
+        self.U_0 = U_0
+        self.nu = U_0 * 2 * self.R_tube / Re
+        self.omega = 2 * np.pi * freq
+        phi0 = -np.sqrt((R - R_wall_center) ** 2) + wall_thickness
+
+        def t(a):
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+        self.t_vorticity = t(0 * Z)
+        self.t_psi = t(0 * Z)
+        self.t_u_z = t(0 * Z)
+        self.t_u_r = t(0 * Z)
+        self.t_char_func = t(0 * Z)
+        self.t_phi0 = t(phi0)
+        from pyaxisymflow_amd.kernels.smooth_Heaviside import smooth_Heaviside
+
+        smooth_Heaviside(self.t_char_func, self.t_phi0, dx * 2**0.5)
+        self.t_scalars = torch.zeros(N_SCALARS, dtype=torch.float64, device=dev)
+        self.t_r = t(self.r)
+        # kill_boundary_vorticity_sine.py:17-27 weights of the top rows, as
+        # the reference forms them (its literal 1 as the r end)
+        w = kill_width
+        self.t_wr = t(np.sin(np.pi * (1 - self.r[-w:] - 0.5 * dx) / 2 / (w - 1) / dx))
+        cfg = _lib.SlabConfig()
+        cfg.nr, cfg.nz, cfg.ghost, cfg.kill_width = nr, nz, int(ghost_size), w
+        cfg.dx, cfg.nu, cfg.lam, cfg.U0, cfg.omega = dx, self.nu, brink_lam, U_0, self.omega
+        cfg.CFL, cfg.eps = CFL, np.finfo(float).eps
+        self.cfg = cfg
+        bufs = _lib.SlabBuffers()
+        for name in ("vorticity", "psi", "u_z", "u_r", "scalars", "char_func", "r"):
+            setattr(bufs, name, getattr(self, "t_" + name).data_ptr())
+        bufs.kill_wr = self.t_wr.data_ptr()
+        self._bufs = bufs
+        torch.cuda.synchronize(dev)
+        handle = ctypes.c_void_p()
+        _lib.call("axs_slab_stepper_create", ctypes.byref(cfg), ctypes.byref(bufs),
+                  ctypes.byref(handle))
+        self._h = handle
+
+    def step(self, n=1, join=True):
+        torch = _lib.torch()
+        cur = torch.cuda.current_stream() if join else None
+        if join:
+            self.stream.wait_stream(cur)
+        _lib.call("axs_slab_stepper_run", self._h, int(n), 1 if self.use_graph else 0,
+                  ctypes.c_void_p(self.stream.cuda_stream))
+        if join:
+            cur.wait_stream(self.stream)
+
+    def synchronize(self):
+        self.stream.synchronize()
+
+    def _host(self, t):
+        self.synchronize()
+        return t.cpu().numpy()
+
+    def vorticity(self):
+        return self._host(self.t_vorticity)
+
+    def psi(self):
+        return self._host(self.t_psi)
+
+    def velocity(self):
+        """The penalised velocity of the last step (what it advected with)."""
+        return self._host(self.t_u_z), self._host(self.t_u_r)
+
+    def char_func(self):
+        return self._host(self.t_char_func)
+
+    def scalars(self):
+        return self._host(self.t_scalars)
+
+    def time(self):
+        return float(self.scalars()[0])
+
+    def dt(self):
+        return float(self.scalars()[1])
+
+    def steps_done(self):
+        return int(self.scalars()[7])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self.synchronize()
+                _lib.load().axs_slab_stepper_destroy(h)
+            except Exception:
+                pass
+            self._h = None
