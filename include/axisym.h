@@ -483,6 +483,35 @@ axs_status axs_slab_stepper_create(const axs_slab_config* cfg, const axs_slab_bu
 axs_status axs_slab_stepper_destroy(axs_slab_stepper* st);
 axs_status axs_slab_stepper_run(axs_slab_stepper* st, int nsteps, int use_graph, void* stream);
 
+/* ---- Device-resident step of examples/TorusStreaming/torus_streaming.py:129-233
+ * with implicit diffusion (the driver's setting): a torus (tube radius a,
+ * cross-section aspect AR, centre (Z_cm, R_cm(t)), R_cm(t) = R_cm + e a
+ * sum_k amp_k sin(omega_k t)) dilating in r.  Step order: kill taper (z
+ * then r) -> psi -> u from psi -> the torus mask at R_cm(t) -> potential of
+ * char_func * rate / r, rate = e a sum_k amp_k omega_k cos(omega_k t), u +=
+ * grad phi -> brinkmann_penalize towards (0, rate) into u_z_pen, u_r_pen ->
+ * omega += curl(u_pen - u) -> ENO3 with u -> implicit Euler diffusion at the
+ * fixed dt.  FP64.  Buffers: (nr, nz) vorticity, psi, u_z, u_r, u_z_pen,
+ * u_r_pen, char_func, vel_phi; scalars AXS_RIGID_SCALARS doubles (t, dt,
+ * dt/dx, R_cm(t), rate, step counter); r (nr), z (nz), kill_wz
+ * (2 kill_width), kill_wr (kill_width) as for the rigid stepper.  The
+ * potential's singular mode is solved in the gauge of
+ * axs_potential_solve_f64 (the velocity does not see it). */
+typedef struct axs_torus_stepper axs_torus_stepper;
+typedef struct {
+  int nr, nz, kill_width, nmodes; /* nmodes <= 4 */
+  double dx, nu, lam, moll_zone, a, AR, e, Z_cm, R_cm, dt;
+  double amp[4], omega[4];
+} axs_torus_config;
+typedef struct {
+  double *vorticity, *psi, *u_z, *u_r, *u_z_pen, *u_r_pen, *char_func, *vel_phi, *scalars;
+  const double *r, *z, *kill_wz, *kill_wr;
+} axs_torus_buffers;
+axs_status axs_torus_stepper_create(const axs_torus_config* cfg, const axs_torus_buffers* bufs,
+                                    axs_torus_stepper** out);
+axs_status axs_torus_stepper_destroy(axs_torus_stepper* st);
+axs_status axs_torus_stepper_run(axs_torus_stepper* st, int nsteps, int use_graph, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -463,3 +463,112 @@ def womersley_wall_driven(r, t, U_0, omega, nu, R_tube):
     lam = np.sqrt(1j * (omega / nu))
     A = 0.5 / jv(0, lam * R_tube)
     return U_0 * np.real(2 * (A * jv(0, lam * r)) * np.exp(-1j * omega * t))
+
+
+class TorusStreamingCase:
+    """examples/TorusStreaming/torus_streaming.py:22-233 with implicit
+    diffusion (the driver's setting): a torus whose cross-section centre
+    oscillates in r as a sum of modes; the dilation's source flow from the
+    velocity potential, the surface velocity by Brinkman penalisation.
+    ``solver`` "exact" (DCT + banded radial solves for psi and phi, the
+    potential's singular mode in the gauge phi_hat_0[nr-1] = 0, DST-I +
+    banded for the implicit step) or "eig" (the reference's dense routes)."""
+
+    def __init__(self, grid_size_z=256, domain_AR=0.5, solver="exact"):
+        from oracle.potential import FastDiagonalisationPotentialSolver, exact_potential_solve
+        from oracle.potential import compute_velocity_from_phi_unb
+
+        self._exact_potential = exact_potential_solve
+        self._vel_phi = compute_velocity_from_phi_unb
+        dx = 1.0 / grid_size_z
+        nz, nr = grid_size_z, int(domain_AR * grid_size_z)
+        self.nz, self.nr, self.dx = nz, nr, dx
+        self.CFL = 0.1
+        self.brink_lam = 1e4
+        self.moll_zone = dx * 2**0.5
+        r_core = 0.125
+        self.a = 0.34146341463414637 * r_core
+        self.AR = 0.4285714285714286
+        self.e = 0.0675
+        self.freq_modes = np.array([16, 32])
+        self.amplitude_modes = np.array([1.0, 0.3])
+        freq = self.freq_modes[np.argmax(self.amplitude_modes)]
+        omega = 2 * np.pi * freq
+        self.omega_modes = 2 * np.pi * self.freq_modes
+        self.nu = omega * (0.11235582096628798 * r_core) ** 2
+        U_osc = self.e * self.a * np.sum(self.amplitude_modes * self.omega_modes)
+        self.z = np.linspace(0 + dx / 2, 1 - dx / 2, nz)
+        self.r = np.linspace(0 + dx / 2, domain_AR - dx / 2, nr)
+        self.Z, self.R = np.meshgrid(self.z, self.r)
+        z0 = 0 * self.Z
+        self.vorticity, self.penal_vorticity = z0.copy(), z0.copy()
+        self.psi, self.phi, self.u_z, self.u_r = z0.copy(), z0.copy(), z0.copy(), z0.copy()
+        self.u_z_pen, self.u_r_pen = z0.copy(), z0.copy()
+        self.vel_phi, self.u_z_divg, self.u_r_divg, self.vel_divg = (z0.copy(), z0.copy(),
+                                                                     z0.copy(), z0.copy())
+        self.Z_cm, self.R_cm = 0.5, r_core
+        phi0 = -np.sqrt((self.Z - self.Z_cm) ** 2 + (self.R - self.R_cm) ** 2 / self.AR**2) + \
+            self.a
+        self.char_func = 0 * self.Z
+        K.smooth_Heaviside(self.char_func, phi0, self.moll_zone)
+        # :106-112, timestep aligned to the cycle
+        freqTimer_limit = 1 / freq
+        data_per_cycle = 100
+        dataTimer_limit = 1 / freq / data_per_cycle
+        diffusion_dt_limit = self.CFL / U_osc
+        num_diffusion_substeps = max(int(freqTimer_limit / diffusion_dt_limit), data_per_cycle)
+        diffusion_dt_limit = freqTimer_limit / num_diffusion_substeps
+        self.dt_limit = min(diffusion_dt_limit, freqTimer_limit, dataTimer_limit)
+        self.solver_kind = solver
+        if solver == "eig":
+            self.stokes = FastDiagonalisationStokesSolver(nr, nz, dx)
+            self.potential = FastDiagonalisationPotentialSolver(nr, nz, dx)
+            self.implicit = ImplicitEulerDiffusionStepper(
+                time_step=self.dt_limit, kinematic_viscosity=self.nu, grid_size_r=nr,
+                grid_size_z=nz, dx=dx)
+        self.t = 0.0
+        self.it = 0
+
+    def step(self):
+        """torus_streaming.py:129-233, one iteration (no plots, averages or
+        data dumps)."""
+        dx = self.dx
+        K.kill_boundary_vorticity_sine_z(self.vorticity, self.Z, 3, dx)
+        K.kill_boundary_vorticity_sine_r(self.vorticity, self.R, 3, dx)
+        if self.solver_kind == "eig":
+            self.stokes.solve(solution_field=self.psi, rhs_field=self.vorticity)
+        else:
+            self.psi[...] = exact_stokes_solve(self.vorticity, dx)
+        K.compute_velocity_from_psi_unb(self.u_z, self.u_r, self.psi, self.R, dx)
+        e, a, t = self.e, self.a, self.t
+        R_cm_t = self.R_cm + e * a * np.sum(self.amplitude_modes * np.sin(self.omega_modes * t))
+        self.phi[...] = a - np.sqrt((self.Z - self.Z_cm) ** 2 + (self.R - R_cm_t) ** 2 /
+                                    self.AR**2)
+        K.smooth_Heaviside(self.char_func, self.phi, self.moll_zone)
+        self.vel_divg[...] = (
+            e * a * np.sum(self.amplitude_modes * self.omega_modes *
+                           np.cos(self.omega_modes * t)) / self.R)
+        rhs = self.char_func * self.vel_divg
+        if self.solver_kind == "eig":
+            self.potential.solve(solution_field=self.vel_phi, rhs_field=rhs)
+        else:
+            self.vel_phi[...] = self._exact_potential(rhs, dx)
+        self._vel_phi(self.u_z_divg, self.u_r_divg, self.vel_phi, dx)
+        self.u_z[...] += self.u_z_divg
+        self.u_r[...] += self.u_r_divg
+        dt = self.dt_limit
+        K.brinkmann_penalize(
+            self.brink_lam, dt, self.char_func, 0.0,
+            e * a * np.sum(self.amplitude_modes * self.omega_modes * np.cos(self.omega_modes * t)),
+            self.u_z, self.u_r, self.u_z_pen, self.u_r_pen)
+        K.compute_vorticity_from_velocity_unb(self.penal_vorticity, self.u_z_pen - self.u_z,
+                                              self.u_r_pen - self.u_r, dx)
+        self.vorticity[...] += self.penal_vorticity
+        K.advect_vorticity_via_eno3(self.vorticity, self.u_z, self.u_r, dt, dx)
+        if self.solver_kind == "eig":
+            self.implicit.step(vorticity_field=self.vorticity, dt=dt)
+        else:
+            self.vorticity[...] = exact_implicit_diffusion(self.vorticity, self.nu * dt, dx)
+        self.t += dt
+        self.it += 1
+        return dt

@@ -82,6 +82,23 @@ class SlabBuffers(ctypes.Structure):
         "vorticity", "psi", "u_z", "u_r", "scalars", "char_func", "r", "kill_wr")]
 
 
+class TorusConfig(ctypes.Structure):
+    """axs_torus_config (include/axisym.h)."""
+
+    _fields_ = [(n, ctypes.c_int) for n in ("nr", "nz", "kill_width", "nmodes")] + [
+        (n, ctypes.c_double) for n in (
+            "dx", "nu", "lam", "moll_zone", "a", "AR", "e", "Z_cm", "R_cm", "dt")] + [
+        ("amp", ctypes.c_double * 4), ("omega", ctypes.c_double * 4)]
+
+
+class TorusBuffers(ctypes.Structure):
+    """axs_torus_buffers (include/axisym.h)."""
+
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "vorticity", "psi", "u_z", "u_r", "u_z_pen", "u_r_pen", "char_func", "vel_phi", "scalars",
+        "r", "z", "kill_wz", "kill_wr")]
+
+
 AXS_F64 = 0
 AXS_F32 = 1
 AXS_RIGID_SCALARS = 24
@@ -145,6 +162,10 @@ SIGNATURES = {
                                 ctypes.POINTER(_P)],
     "axs_slab_stepper_destroy": [_P],
     "axs_slab_stepper_run": [_P, _I, _I, _P],
+    "axs_torus_stepper_create": [ctypes.POINTER(TorusConfig), ctypes.POINTER(TorusBuffers),
+                                 ctypes.POINTER(_P)],
+    "axs_torus_stepper_destroy": [_P],
+    "axs_torus_stepper_run": [_P, _I, _I, _P],
     "axs_rigid_batch_create": [ctypes.POINTER(RigidConfig), _I, ctypes.POINTER(RigidBuffers),
                                ctypes.POINTER(_P)],
     "axs_rigid_stepper_destroy": [_P],

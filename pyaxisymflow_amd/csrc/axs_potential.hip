@@ -21,8 +21,7 @@
 // file keeps the library's default of no FMA contraction, so the velocity
 // stencil below rounds as the reference does.)
 
-#include "axs_common.hpp"
-#include "axs_stokes.hpp"
+#include "axs_potential.hpp"
 
 #include <new>
 #include <vector>
@@ -82,20 +81,6 @@ __global__ void velocity_from_phi_kernel(double* __restrict__ u_z, double* __res
   u_r[c] = ur;
 }
 
-struct PotentialPlan {
-  StokesPlan pl;  // DCT tables, the potential's column-pass tables, spectral buffer
-  // mode 0 in the gauge phi_hat_0[nr - 1] = 0: bands with the last row
-  // replaced, pivots at segment starts (L = 16) and the column buffer
-  double *a0 = nullptr, *b0 = nullptr, *c0 = nullptr, *lam0 = nullptr, *cseg0 = nullptr;
-  double* col0 = nullptr;
-  hipError_t init(int nr, int nz, double dx);
-  hipError_t solve(double* phi, const double* rhs, hipStream_t s);
-  ~PotentialPlan() {
-    for (double* p : {a0, b0, c0, lam0, cseg0, col0})
-      if (p) (void)hipFree(p);
-  }
-};
-
 static hipError_t upload_vec(double** dev, const std::vector<double>& h) {
   hipError_t e = hipMalloc(reinterpret_cast<void**>(dev), sizeof(double) * h.size());
   if (e != hipSuccess) return e;
@@ -131,6 +116,17 @@ hipError_t PotentialPlan::init(int nr, int nz, double dx) {
   if ((e = upload_vec(&lam0, gl)) != hipSuccess) return e;
   if ((e = upload_vec(&cseg0, pivot_table(ga, gb, gc, gl, nr, 1, 16))) != hipSuccess) return e;
   return hipMalloc(reinterpret_cast<void**>(&col0), sizeof(double) * nr);
+}
+
+hipError_t launch_velocity_from_phi(double* u_z, double* u_r, const double* phi, int nr, int nz,
+                                    double dx, hipStream_t s) {
+  velocity_from_phi_kernel<<<dim3(ceil_div(nz, 256), nr), 256, 0, s>>>(u_z, u_r, phi, nr, nz, dx);
+  return hipGetLastError();
+}
+
+PotentialPlan::~PotentialPlan() {
+  for (double* p : {a0, b0, c0, lam0, cseg0, col0})
+    if (p) (void)hipFree(p);
 }
 
 hipError_t PotentialPlan::solve(double* phi, const double* rhs, hipStream_t s) {

@@ -912,3 +912,133 @@ class PeriodicRigidSlabStepper:
             except Exception:
                 pass
             self._h = None
+
+
+class TorusStreamingStepper:
+    """examples/TorusStreaming/torus_streaming.py:22-233 on the GPU with
+    implicit diffusion (the driver's setting): a torus (tube radius a =
+    0.3415 r_core, cross-section aspect AR) whose centre oscillates in r as
+    R_cm + e a sum_k A_k sin(w_k t), its dilation's source flow from the
+    velocity potential and its surface velocity by Brinkman penalisation,
+    every stage device resident and replayed from hipGraphs
+    (axs_torus_stepper_*).  Parameters follow the driver (:22-60): grid
+    256 x 128, e = 0.0675, modes 16 and 32 Hz with amplitudes 1 and 0.3,
+    nu = omega (0.1124 r_core)^2, and the cycle-aligned dt (:106-112)."""
+
+    def __init__(self, grid_size_z=256, domain_AR=0.5, use_graph=True, kill_width=3):
+        torch = _lib.torch()
+        dev = _lib.device()
+        dx = 1.0 / grid_size_z
+        nz, nr = int(grid_size_z), int(domain_AR * grid_size_z)
+        self.nz, self.nr, self.dx = nz, nr, dx
+        self.use_graph = use_graph
+        self.stream = torch.cuda.Stream(device=dev)
+        moll_zone = dx * 2**0.5
+        r_core = 0.125
+        a = 0.34146341463414637 * r_core
+        AR = 0.4285714285714286
+        e = 0.0675
+        freq_modes = np.array([16, 32])
+        amplitude_modes = np.array([1.0, 0.3])
+        freq = freq_modes[np.argmax(amplitude_modes)]
+        omega = 2 * np.pi * freq
+        omega_modes = 2 * np.pi * freq_modes
+        nu = omega * (0.11235582096628798 * r_core) ** 2
+        U_osc = e * a * np.sum(amplitude_modes * omega_modes)
+        freqTimer_limit = 1 / freq
+        data_per_cycle = 100
+        dataTimer_limit = 1 / freq / data_per_cycle
+        diffusion_dt_limit = 0.1 / U_osc
+        num_diffusion_substeps = max(int(freqTimer_limit / diffusion_dt_limit), data_per_cycle)
+        diffusion_dt_limit = freqTimer_limit / num_diffusion_substeps
+        self.dt_limit = min(diffusion_dt_limit, freqTimer_limit, dataTimer_limit)
+        self.z = np.linspace(0 + dx / 2, 1 - dx / 2, nz)
+        self.r = np.linspace(0 + dx / 2, domain_AR - dx / 2, nr)
+        Z, R = np.meshgrid(self.z, self.r)
+        Z_cm, R_cm = 0.5, r_core
+        phi0 = -np.sqrt((Z - Z_cm) ** 2 + (R - R_cm) ** 2 / AR**2) + a
+
+        def t(x):
+            return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(dev)
+
+        for name in ("vorticity", "psi", "u_z", "u_r", "u_z_pen", "u_r_pen", "char_func",
+                     "vel_phi"):
+            setattr(self, "t_" + name, t(0 * Z))
+        self.t_phi0 = t(phi0)
+        from pyaxisymflow_amd.kernels.smooth_Heaviside import smooth_Heaviside
+
+        smooth_Heaviside(self.t_char_func, self.t_phi0, moll_zone)
+        self.t_scalars = torch.zeros(N_SCALARS, dtype=torch.float64, device=dev)
+        self.t_r, self.t_z = t(self.r), t(self.z)
+        wz, wr = kill_weights(self.z, self.r, dx, kill_width)
+        self.t_wz, self.t_wr = t(wz), t(wr)
+        cfg = _lib.TorusConfig()
+        cfg.nr, cfg.nz, cfg.kill_width, cfg.nmodes = nr, nz, kill_width, len(freq_modes)
+        cfg.dx, cfg.nu, cfg.lam, cfg.moll_zone = dx, nu, 1e4, moll_zone
+        cfg.a, cfg.AR, cfg.e, cfg.Z_cm, cfg.R_cm, cfg.dt = a, AR, e, Z_cm, R_cm, self.dt_limit
+        for k in range(len(freq_modes)):
+            cfg.amp[k], cfg.omega[k] = amplitude_modes[k], omega_modes[k]
+        self.cfg = cfg
+        bufs = _lib.TorusBuffers()
+        for name in ("vorticity", "psi", "u_z", "u_r", "u_z_pen", "u_r_pen", "char_func",
+                     "vel_phi", "scalars", "r", "z"):
+            setattr(bufs, name, getattr(self, "t_" + name).data_ptr())
+        bufs.kill_wz, bufs.kill_wr = self.t_wz.data_ptr(), self.t_wr.data_ptr()
+        self._bufs = bufs
+        torch.cuda.synchronize(dev)
+        handle = ctypes.c_void_p()
+        _lib.call("axs_torus_stepper_create", ctypes.byref(cfg), ctypes.byref(bufs),
+                  ctypes.byref(handle))
+        self._h = handle
+
+    def step(self, n=1, join=True):
+        torch = _lib.torch()
+        cur = torch.cuda.current_stream() if join else None
+        if join:
+            self.stream.wait_stream(cur)
+        _lib.call("axs_torus_stepper_run", self._h, int(n), 1 if self.use_graph else 0,
+                  ctypes.c_void_p(self.stream.cuda_stream))
+        if join:
+            cur.wait_stream(self.stream)
+
+    def synchronize(self):
+        self.stream.synchronize()
+
+    def _host(self, t):
+        self.synchronize()
+        return t.cpu().numpy()
+
+    def vorticity(self):
+        return self._host(self.t_vorticity)
+
+    def psi(self):
+        return self._host(self.t_psi)
+
+    def velocity(self):
+        """u from psi plus the potential flow (what the step advected with)."""
+        return self._host(self.t_u_z), self._host(self.t_u_r)
+
+    def penalised_velocity(self):
+        return self._host(self.t_u_z_pen), self._host(self.t_u_r_pen)
+
+    def char_func(self):
+        return self._host(self.t_char_func)
+
+    def scalars(self):
+        return self._host(self.t_scalars)
+
+    def time(self):
+        return float(self.scalars()[0])
+
+    def steps_done(self):
+        return int(self.scalars()[5])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self.synchronize()
+                _lib.load().axs_torus_stepper_destroy(h)
+            except Exception:
+                pass
+            self._h = None
