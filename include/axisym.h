@@ -417,6 +417,17 @@ typedef struct {
   double dx, nu, G, rho_f, CFL, eps;
   double r_ball, z_cm, r_cm;               /* phi pin: r_ball - |eta - (z_cm, r_cm)| */
   double moll_zone, extrap_zone, reinit_band;
+  /* streaming != 0: examples/SoftSphereStreaming/soft_sphere_streaming.py:128-264
+   * instead of the Taylor-Green loop: omega is advected right after the level
+   * set's reinit (before the solid stress acts); after the stress a rigid
+   * core (radius tether_rad, centre (z_cm + e_amp r_ball sin(omega t), r_cm),
+   * mask smoothed over moll_zone) is penalised (brink_lam) towards
+   * U0 cos(omega t), u becoming the penalised velocity and omega gaining the
+   * curl of the correction; dt is clipped to the end of the cycle
+   * (freq_limit; scalars slot 9 holds the cycle timer, reset when it
+   * reaches freq_limit) and to t_end.  The soft batch takes streaming = 0. */
+  int streaming;
+  double brink_lam, U0, omega, e_amp, tether_rad, freq_limit, t_end;
 } axs_soft_config;
 typedef struct {
   double *vorticity, *psi, *u_z, *u_r, *eta1, *eta2, *phi, *char_func, *scalars;

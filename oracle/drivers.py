@@ -4,6 +4,10 @@ oracle kernels.  TEST ORACLE ONLY.
 * ``RigidBodyCase`` follows examples/FlowPastSphere/flow_past_sphere.py:39-188
   (static body, free-stream ramp, drag coefficient) and, with an oscillating
   body, examples/SpheroidStreaming/spheroid_streaming.py:129-246.
+* ``SoftSphereCase`` follows
+  examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:27-215
+  and ``SoftSphereStreamingCase``
+  examples/SoftSphereStreaming/soft_sphere_streaming.py:28-264.
 * ``PeriodicRigidSlabCase`` follows
   examples/PeriodicRigidSlab/periodic_rigid_slab.py:33-158 (z-periodic,
   oscillating annular wall), and ``womersley_wall_driven`` restates its
@@ -364,6 +368,113 @@ class SoftSphereCase:
         K.advect_vorticity_via_eno3(self.vorticity, self.u_z, self.u_r, dt, dx)
         K.diffusion_RK2_unb(self.vorticity, self.temp_vorticity, R, self.nu, dt, dx)
         self.t += dt
+        self.it += 1
+        self.dts.append(dt)
+        return dt
+
+
+class SoftSphereStreamingCase(SoftSphereCase):
+    """examples/SoftSphereStreaming/soft_sphere_streaming.py:28-264: an
+    elastic sphere around a rigid core (radius zeta r_ball) tethered to
+    oscillate along z at U_0 cos(omega t), U_0 = e r_ball omega, in fluid at
+    rest.  Against the Taylor-Green loop the vorticity is advected right
+    after the reinit, the core is penalised after the stress, and dt is
+    clipped to the end of each cycle and to tEnd.  Period averages and
+    plots are left out."""
+
+    def __init__(self, grid_size_z=256, domain_AR=0.5, r_ball=0.15, freq=16, nond_AC=0.125,
+                 e=0.1, Cauchy=0.1, zeta=0.25, no_cycles=30, rho_f=1.0, brink_lam=1e8,
+                 solver="eig"):
+        omega = 2 * np.pi * freq
+        U_0 = e * r_ball * omega
+        Rs = (e / nond_AC) ** 2
+        nu = e * U_0 * r_ball / Rs
+        G = e * rho_f * (r_ball * omega) ** 2 / Cauchy
+        super().__init__(grid_size_z=grid_size_z, G=G, nu=nu, rho_f=rho_f, r_ball=r_ball,
+                         domain_AR=domain_AR, solver=solver, C=0.0)
+        dx = self.dx
+        # :40-46 (moll_zone 2 dx here)
+        self.moll_zone = dx * 2
+        self.extrap_zone = self.moll_zone + 4 * dx
+        self.reinit_band = self.extrap_zone
+        Z, R = self.Z, self.R
+        self.ball_phi = -np.sqrt((Z - self.Z_cm) ** 2 + (R - self.R_cm) ** 2) + r_ball
+        self.ball_char_func = 0 * Z
+        K.smooth_Heaviside(self.ball_char_func, self.ball_phi, self.moll_zone)
+        self.inside_solid = self.ball_char_func > 0.5
+        self.vorticity = 0 * Z
+        self.brink_lam = brink_lam
+        self.U_0, self.omega, self.e = U_0, omega, e
+        self.fixed_rad = zeta * r_ball
+        self.freqTimer_limit = 1 / freq
+        self.tEnd = no_cycles / freq
+        self.freqTimer = 0.0
+        self.tether_phi = 0 * Z
+        self.tether_char_func = 0 * Z
+        self.penal_vorticity = 0 * Z
+
+    def step(self):
+        """soft_sphere_streaming.py:128-264, one iteration."""
+        from oracle.reinit import distance
+
+        dx, Z, R = self.dx, self.Z, self.R
+        K.kill_boundary_vorticity_sine_z(self.vorticity, Z, 3, dx)
+        K.kill_boundary_vorticity_sine_r(self.vorticity, R, 3, dx)
+        if self.solver_kind == "eig":
+            self.solver.solve(solution_field=self.psi, rhs_field=self.vorticity)
+        else:
+            self.psi[...] = exact_stokes_solve(self.vorticity, dx)
+        K.compute_velocity_from_psi_unb(self.u_z, self.u_r, self.psi, R, dx)
+        if self.freqTimer >= self.freqTimer_limit:
+            self.freqTimer = 0.0
+        dt = min(
+            self.CFL * dx / np.sqrt(self.G / self.rho_f),
+            self.CFL * dx / (np.amax(np.fabs(self.u_z) + np.fabs(self.u_r)) + self.eps),
+            0.9 * dx**2 / 4 / self.nu,
+        )
+        if self.freqTimer + dt > self.freqTimer_limit:
+            dt = self.freqTimer_limit - self.freqTimer
+        if self.t + dt > self.tEnd:
+            dt = self.tEnd - self.t
+        K.advect_refmap_via_eno3(self.eta1, self.eta2, self.u_z, self.u_r, dt, dx)
+        self.phi_orig[...] = (
+            -np.sqrt((self.eta1 - self.Z_cm) ** 2 + (self.eta2 - self.R_cm) ** 2) + self.r_ball
+        )
+        band = np.where(self.ball_phi > -3 * dx)
+        self.ball_phi[band] = self.phi_orig[band]
+        self.bad_phi[...] = self.ball_phi
+        d, masked = distance(self.ball_phi, dx, self.reinit_band)
+        self.ball_phi = np.where(masked, self.bad_phi, d)
+        K.advect_vorticity_via_eno3(self.vorticity, self.u_z, self.u_r, dt, dx)
+        K.smooth_Heaviside(self.ball_char_func, self.ball_phi, self.moll_zone)
+        self.inside_solid[...] = self.ball_char_func > 0.5
+        K.extrapolate_eta_with_least_squares(
+            self.inside_solid, self.ball_phi, self.eta1, self.eta2, self.ball_phi_double,
+            self.eta1_double, self.eta2_double, self.extrap_zone, self.nr, self.z,
+        )
+        s11, s12, s22 = self.sigma
+        K.solid_sigma(s11, s12, s22, self.G, dx, self.eta1, self.eta2, *self.eta_grad)
+        s11[...] = self.ball_char_func * s11
+        s12[...] = self.ball_char_func * s12
+        s22[...] = self.ball_char_func * s22
+        K.update_vorticity_from_solid_stress(
+            self.vorticity, self.tau_z, self.tau_r, s11, s12, s22, R, dt, dx
+        )
+        Z_cm_t = self.Z_cm + self.e * self.r_ball * np.sin(self.omega * self.t)
+        self.tether_phi[...] = -np.sqrt((Z - Z_cm_t) ** 2 + (R - self.R_cm) ** 2) + \
+            self.fixed_rad
+        K.smooth_Heaviside(self.tether_char_func, self.tether_phi, self.moll_zone)
+        u_z_upen = self.u_z.copy()
+        u_r_upen = self.u_r.copy()
+        K.brinkmann_penalize(self.brink_lam, dt, self.tether_char_func,
+                             self.U_0 * np.cos(self.omega * self.t), 0.0, u_z_upen, u_r_upen,
+                             self.u_z, self.u_r)
+        K.compute_vorticity_from_velocity_unb(self.penal_vorticity, self.u_z - u_z_upen,
+                                              self.u_r - u_r_upen, dx)
+        self.vorticity[...] += self.penal_vorticity
+        K.diffusion_RK2_unb(self.vorticity, self.temp_vorticity, R, self.nu, dt, dx)
+        self.t += dt
+        self.freqTimer += dt
         self.it += 1
         self.dts.append(dt)
         return dt

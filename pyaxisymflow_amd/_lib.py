@@ -57,7 +57,9 @@ class SoftConfig(ctypes.Structure):
         "nr", "nz", "kill_width", "lsq_sweeps", "reinit_iterations", "lsq_grow")] + [
         (n, ctypes.c_double) for n in (
             "dx", "nu", "G", "rho_f", "CFL", "eps", "r_ball", "z_cm", "r_cm", "moll_zone",
-            "extrap_zone", "reinit_band")]
+            "extrap_zone", "reinit_band")] + [("streaming", ctypes.c_int)] + [
+        (n, ctypes.c_double) for n in (
+            "brink_lam", "U0", "omega", "e_amp", "tether_rad", "freq_limit", "t_end")]
 
 
 class SoftBuffers(ctypes.Structure):

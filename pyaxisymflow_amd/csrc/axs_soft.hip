@@ -54,6 +54,13 @@ static_assert(kSsCount <= AXS_RIGID_SCALARS, "soft scalar slots exceed the array
 struct SoftConfig {
   int nr, nz, kill_width, lsq_sweeps, reinit_iterations, lsq_grow;
   double dx, nu, G, rho_f, CFL, eps, r_ball, z_cm, r_cm, moll_zone, extrap_zone, reinit_band;
+  // streaming (examples/SoftSphereStreaming/soft_sphere_streaming.py:128-264):
+  // the vorticity is advected before the solid stress acts, a rigid core
+  // of radius tether_rad at (z_cm + e_amp r_ball sin(omega t), r_cm) is
+  // penalised (brink_lam) towards U0 cos(omega t) after it, and dt is
+  // clipped to the end of the cycle (freq_limit) and to t_end
+  int streaming;
+  double brink_lam, U0, omega, e_amp, tether_rad, freq_limit, t_end;
 };
 
 // ------------------------------------------------------------------ kernels
@@ -100,7 +107,62 @@ __device__ __forceinline__ void soft_dt_body(double* __restrict__ sc, double* __
 
 __global__ void soft_dt_kernel(double* __restrict__ sc, double* __restrict__ slots,
                                SoftConfig cfg) {
+  if (cfg.streaming && threadIdx.x == 0 && sc[kSsFreqTimer] >= cfg.freq_limit)
+    sc[kSsFreqTimer] = 0.0;  // :128-161 (the cycle's plot resets the timer)
   soft_dt_body(sc, slots, cfg.CFL, cfg.dx, cfg.eps, cfg.G, cfg.rho_f, cfg.nu);
+  if (!cfg.streaming || threadIdx.x != 0) return;
+  // :168-171
+  double dt = sc[kSsDt];
+  if (sc[kSsFreqTimer] + dt > cfg.freq_limit) dt = cfg.freq_limit - sc[kSsFreqTimer];
+  if (sc[kSsT] + dt > cfg.t_end) dt = cfg.t_end - sc[kSsT];
+  sc[kSsDt] = dt;
+  sc[kSsDtByDx] = dt / cfg.dx;
+  sc[kSsHalfNuDt] = 0.5 * cfg.nu * dt;
+  sc[kSsNuDt] = cfg.nu * dt;
+  // :243 and :250-260, at the step's start time
+  const double t = sc[kSsT];
+  sc[kSsZtether] = cfg.z_cm + cfg.e_amp * cfg.r_ball * sin(cfg.omega * t);
+  sc[kSsUwall] = cfg.U0 * cos(cfg.omega * t);
+}
+
+// The tethered core (:243-264): its mask smooth_Heaviside(tether_rad -
+// |(z, r) - (Z_tether, r_cm)|, moll_zone); u_upen = u, u = Brinkman(u_upen)
+// towards (U_wall, 0), du = u - u_upen.
+__global__ void soft_tether_kernel(double* __restrict__ uz, double* __restrict__ ur,
+                                   double* __restrict__ duz, double* __restrict__ dur,
+                                   const double* __restrict__ r, const double* __restrict__ z,
+                                   const double* __restrict__ sc, SoftConfig cfg) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= cfg.nz) return;
+  const int64_t c = (int64_t)i * cfg.nz + j;
+  const double a = z[j] - sc[kSsZtether], b = r[i] - cfg.r_cm;
+  const double phi = -sqrt(a * a + b * b) + cfg.tether_rad;
+  const double chi = smooth_heaviside(phi, cfg.moll_zone);
+  const double ldc = (cfg.brink_lam * sc[kSsDt]) * chi;
+  const double den = 1.0 + ldc;
+  const double gz = uz[c], gr = ur[c];
+  const double pz = (gz + ldc * sc[kSsUwall]) / den;
+  const double pr = (gr + ldc * 0.0) / den;
+  uz[c] = pz;
+  ur[c] = pr;
+  duz[c] = pz - gz;
+  dur[c] = pr - gr;
+}
+
+// omega += p, p the interior curl of (du_z, du_r) and 0 on the border.
+__global__ void soft_add_curl_kernel(double* __restrict__ w, const double* __restrict__ duz,
+                                     const double* __restrict__ dur, int nr, int nz, double dx) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= nz) return;
+  const int64_t c = (int64_t)i * nz + j;
+  double p = 0.0;
+  if (i >= 1 && i <= nr - 2 && j >= 1 && j <= nz - 2) {
+    const double two_dx = 2.0 * dx;
+    p = (dur[c + 1] - dur[c - 1]) / two_dx - (duz[c + nz] - duz[c - nz]) / two_dx;
+  }
+  w[c] = w[c] + p;
 }
 
 // eta <- advected eta; phi pinned to the sphere of eta in phi > -3 dx.
@@ -215,8 +277,9 @@ __device__ __forceinline__ void soft_finalize_body(double* __restrict__ sc,
 __global__ void soft_finalize_kernel(double* __restrict__ sc,
                                      const unsigned* __restrict__ counters, int sweeps,
                                      unsigned* __restrict__ list_count,
-                                     unsigned* __restrict__ need) {
+                                     unsigned* __restrict__ need, int streaming) {
   if (threadIdx.x != 0) return;
+  if (streaming) sc[kSsFreqTimer] = sc[kSsFreqTimer] + sc[kSsDt];
   soft_finalize_body(sc, counters, sweeps, list_count, need);
 }
 
@@ -363,6 +426,10 @@ struct SoftStepper {
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_reinit(buf.phi, buf.phi, nullptr, nr, nz, cfg.dx, cfg.reinit_band,
                                cfg.reinit_iterations, rw, s));
+    if (cfg.streaming)  // soft_sphere_streaming.py:197-198
+      AXS_TRY_STEP(launch_eno3_conservative<double>(w0, buf.u_z, buf.u_r, w1, nr, nz,
+                                                    Scalar<double>{0.0, sc + kSsDtByDx}, -1.0,
+                                                    s));
     soft_lsq_mirror_kernel<<<dim3(ceil_div(nz, 256), 2 * nr), 256, 0, s>>>(
         buf.phi, buf.char_func, cfg.moll_zone, buf.eta1, buf.eta2, e1d, e2d, cur, tgt, lw.ex2,
         lw.ey2, lw.cur2, nr, nz, cfg.extrap_zone, lw.counters, cfg.lsq_sweeps + 1);
@@ -383,16 +450,31 @@ struct SoftStepper {
     soft_lsq_back_kernel<<<ceil_div(n, 256), 256, 0, s>>>(
         buf.eta1, buf.eta2, e1d, e2d, lw.ex2, lw.ey2, lw.counters, cfg.lsq_sweeps, n, n);
     AXS_TRY_LAUNCH();
+    // the TG driver (:186-213) adds the stress to the tapered vorticity and
+    // then advects it; the streaming driver (:198-264) advects it first
+    // (ENO3 below, w0 -> w1 before the stress) and penalises the core after
+    double* wstress = cfg.streaming ? w1 : w0;
     AXS_TRY_STEP(launch_solid_sigma<double>(s11, s12, s22, cfg.G, cfg.dx, buf.eta1, buf.eta2, g1z,
                                             g1r, g2z, g2r, nr, nz, s, buf.char_func));
-    AXS_TRY_STEP(launch_div_tau<double>(w0, tz, tr, s11, s12, s22, buf.r, nr, nz,
+    AXS_TRY_STEP(launch_div_tau<double>(wstress, tz, tr, s11, s12, s22, buf.r, nr, nz,
                                         Scalar<double>{0.0, sc + kSsDt}, cfg.dx, s));
-    AXS_TRY_STEP(launch_eno3_conservative<double>(w0, buf.u_z, buf.u_r, w1, nr, nz,
-                                                  Scalar<double>{0.0, sc + kSsDtByDx}, -1.0, s));
+    if (!cfg.streaming) {
+      AXS_TRY_STEP(launch_eno3_conservative<double>(w0, buf.u_z, buf.u_r, w1, nr, nz,
+                                                    Scalar<double>{0.0, sc + kSsDtByDx}, -1.0,
+                                                    s));
+    } else {
+      // e1n / e2n (the advected eta, consumed by the pin) hold the
+      // correction; the stress scratch keeps its never-written entries
+      soft_tether_kernel<<<g2, 256, 0, s>>>(buf.u_z, buf.u_r, e1n, e2n, buf.r, buf.z, sc, cfg);
+      AXS_TRY_LAUNCH();
+      soft_add_curl_kernel<<<g2, 256, 0, s>>>(w1, e1n, e2n, nr, nz, cfg.dx);
+      AXS_TRY_LAUNCH();
+    }
     AXS_TRY_STEP(launch_rk2<double, RRow<double>>(
         w1, tmp, buf.vorticity, RRow<double>{buf.r}, nr, nz, Scalar<double>{0.0, sc + kSsHalfNuDt},
         Scalar<double>{0.0, sc + kSsNuDt}, cfg.dx, s));
-    soft_finalize_kernel<<<1, 64, 0, s>>>(sc, lw.counters, cfg.lsq_sweeps, lw.count, need);
+    soft_finalize_kernel<<<1, 64, 0, s>>>(sc, lw.counters, cfg.lsq_sweeps, lw.count, need,
+                                          cfg.streaming);
     AXS_TRY_LAUNCH();
 #undef AXS_TRY_STEP
 #undef AXS_TRY_LAUNCH
@@ -928,6 +1010,18 @@ static axs_status soft_config_from(const axs_soft_config* c, const axs_soft_buff
   cfg.moll_zone = c->moll_zone;
   cfg.extrap_zone = c->extrap_zone;
   cfg.reinit_band = c->reinit_band;
+  cfg.streaming = c->streaming != 0;
+  if (cfg.streaming) {
+    AXS_CHECK_ARG(c->brink_lam >= 0 && c->tether_rad > 0 && c->freq_limit > 0 && c->t_end > 0,
+                  "streaming needs brink_lam >= 0 and positive tether_rad, freq_limit, t_end");
+    cfg.brink_lam = c->brink_lam;
+    cfg.U0 = c->U0;
+    cfg.omega = c->omega;
+    cfg.e_amp = c->e_amp;
+    cfg.tether_rad = c->tether_rad;
+    cfg.freq_limit = c->freq_limit;
+    cfg.t_end = c->t_end;
+  }
   if (cfg.reinit_iterations < 0) cfg.reinit_iterations = reinit_iterations(cfg.reinit_band, cfg.dx);
   AXS_CHECK_ARG(cfg.reinit_iterations <= 1 << 16, "too many reinit iterations");
   bb = SoftBuffers{b->vorticity, b->psi,     b->u_z, b->u_r, b->eta1,    b->eta2,   b->phi,
@@ -988,6 +1082,7 @@ axs_status axs_soft_batch_add(axs_soft_batch* bt, const axs_soft_config* c,
   SoftBuffers bb;
   const axs_status st0 = soft_config_from(c, b, cfg, bb);
   if (st0 != AXS_OK) return st0;
+  AXS_CHECK_ARG(!cfg.streaming, "the soft batch steps the Taylor-Green loop (streaming = 0)");
   AXS_CHECK_ARG(any_dft_length(cfg.nz, nullptr) <= kAnyMaxL && cfg.nr <= 1024 * kSoftBatchTriL,
                 "grid too large for a soft batch");
   hipError_t e = bt->impl.add(cfg, bb, case_id);

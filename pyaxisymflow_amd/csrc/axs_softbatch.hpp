@@ -27,6 +27,9 @@ enum : int {
   kSsIt,              // step counter
   kSsLsqSweeps,       // sweeps that fitted cells in the last step
   kSsLsqUnconverged,  // steps whose last allowed sweep still fitted cells
+  kSsFreqTimer,       // streaming: time since the cycle timer last reset
+  kSsUwall,           // streaming: tethered core's velocity U_0 cos(omega t)
+  kSsZtether,         // streaming: tethered core's centre Z_cm + e r_ball sin(omega t)
   kSsCount
 };
 

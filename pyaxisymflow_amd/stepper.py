@@ -526,7 +526,8 @@ SS_T, SS_DT, SS_DT_BY_DX, SS_HALF_NU_DT, SS_NU_DT, SS_MAXU, SS_IT, SS_LSQ_SWEEPS
     SS_LSQ_UNCONVERGED = range(9)
 
 
-def _soft_case_state(obj, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_width):
+def _soft_case_state(obj, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_width,
+                     moll_cells=4):
     """Device state, configuration and buffer table of one soft-sphere run
     (examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:27-98),
     set as attributes of obj."""
@@ -535,7 +536,7 @@ def _soft_case_state(obj, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_
     dx = 1.0 / grid_size_z
     nz, nr = int(grid_size_z), int(domain_AR * grid_size_z)
     obj.nz, obj.nr, obj.dx = nz, nr, dx
-    moll_zone = dx * 4
+    moll_zone = dx * moll_cells
     extrap_zone = moll_zone + 4 * dx
     z = np.linspace(0 + dx / 2, 1 - dx / 2, nz)
     r = np.linspace(0 + dx / 2, domain_AR - dx / 2, nr)
@@ -1042,3 +1043,47 @@ class TorusStreamingStepper:
             except Exception:
                 pass
             self._h = None
+
+
+class SoftSphereStreamingStepper(SoftSphereStepper):
+    """examples/SoftSphereStreaming/soft_sphere_streaming.py:28-264 on the
+    GPU: an elastic sphere (radius r_ball, shear modulus G) around a rigid
+    core of radius zeta r_ball tethered to oscillate along z, U_0 cos(omega t)
+    with U_0 = e r_ball omega, in fluid initially at rest (streaming flow).
+    The loop body runs on the device as the soft stepper's streaming mode:
+    the vorticity is advected before the solid stress acts, the core is
+    Brinkman-penalised after it, and dt is clipped to the end of each cycle
+    and to t_end.  Parameters and derived quantities follow the driver
+    (:28-90): moll_zone = 2 dx, extrap_zone = moll_zone + 4 dx,
+    nu = e U_0 r_ball / Rs with Rs = (e / nond_AC)^2, G = e rho_f
+    (r_ball omega)^2 / Cauchy, brink_lam = 1e8.  The period averages and
+    plots of the driver are not computed."""
+
+    def __init__(self, grid_size_z=256, domain_AR=0.5, r_ball=0.15, freq=16, nond_AC=0.125,
+                 e=0.1, Cauchy=0.1, zeta=0.25, no_cycles=30, rho_f=1.0, brink_lam=1e8,
+                 lsq_sweeps=None, use_graph=True, kill_width=3):
+        torch = _lib.torch()
+        dev = _lib.device()
+        self.use_graph = use_graph
+        self.stream = torch.cuda.Stream(device=dev)
+        omega = 2 * np.pi * freq
+        U_0 = e * r_ball * omega
+        Rs = (e / nond_AC) ** 2
+        nu = e * U_0 * r_ball / Rs
+        G = e * rho_f * (r_ball * omega) ** 2 / Cauchy
+        self.omega, self.U_0, self.nu, self.G = omega, U_0, nu, G
+        _soft_case_state(self, grid_size_z, G, nu, rho_f, r_ball, domain_AR, 0.0, kill_width,
+                         moll_cells=2)
+        cfg = self.cfg
+        if lsq_sweeps:
+            cfg.lsq_sweeps, cfg.lsq_grow = int(lsq_sweeps), 0
+        cfg.streaming = 1
+        cfg.brink_lam, cfg.U0, cfg.omega, cfg.e_amp = brink_lam, U_0, omega, e
+        cfg.tether_rad = zeta * r_ball
+        cfg.freq_limit = 1 / freq
+        cfg.t_end = no_cycles / freq
+        torch.cuda.synchronize(dev)
+        handle = ctypes.c_void_p()
+        _lib.call("axs_soft_stepper_create", ctypes.byref(cfg), ctypes.byref(self._bufs),
+                  ctypes.byref(handle))
+        self._h = handle
