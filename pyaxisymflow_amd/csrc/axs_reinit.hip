@@ -19,6 +19,7 @@
 #include "axs_reinit.hpp"
 #include "axs_softbatch.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace axs {
@@ -85,8 +86,27 @@ __global__ void reinit_init_kernel(const double* __restrict__ phi, double* __res
 // sum(terms) = 1.  Where both axes enter and x is not above both axes'
 // upwind values, x comes from the axis with the smaller one alone (+ 2 dx/3
 // second order, + dx first order).
-__device__ __forceinline__ double eikonal_update(const double* __restrict__ D, int i, int j,
-                                                 int nr, int nz, double side, double dx) {
+// The iterate as the update reads it: kBig outside the domain (a neighbour
+// beyond the edge is not known).  From a global array, or from an LDS tile
+// of width w whose origin is global cell (i0, j0).
+struct GlobalIterate {
+  const double* __restrict__ D;
+  int nr, nz;
+  __device__ __forceinline__ double at(int i, int j) const {
+    return (i >= 0 && i < nr && j >= 0 && j < nz) ? D[(int64_t)i * nz + j] : kBig;
+  }
+};
+struct TileIterate {
+  const double* T;
+  int i0, j0, w, nr, nz;
+  __device__ __forceinline__ double at(int i, int j) const {
+    return (i >= 0 && i < nr && j >= 0 && j < nz) ? T[(i - i0) * w + (j - j0)] : kBig;
+  }
+};
+
+template <class IT>
+__device__ __forceinline__ double eikonal_update(const IT& D, int i, int j, double side,
+                                                 double dx) {
   const double idx2 = 1.0 / (dx * dx);
   const double aa = 9.0 / 4.0;
   double a = 0.0, b = 0.0, c = 0.0;
@@ -100,14 +120,10 @@ __device__ __forceinline__ double eikonal_update(const double* __restrict__ D, i
       const int i1 = axis == 0 ? i + s : i, j1 = axis == 0 ? j : j + s;
       const int i2 = axis == 0 ? i + 2 * s : i, j2 = axis == 0 ? j : j + 2 * s;
       double n1 = kBig, n2 = kBig;
-      if (i1 >= 0 && i1 < nr && j1 >= 0 && j1 < nz) {
-        const double v = D[(int64_t)i1 * nz + j1];
-        if (v < kBig) n1 = side * v;
-      }
-      if (i2 >= 0 && i2 < nr && j2 >= 0 && j2 < nz) {
-        const double v = D[(int64_t)i2 * nz + j2];
-        if (v < kBig) n2 = side * v;
-      }
+      const double u1 = D.at(i1, j1);
+      if (u1 < kBig) n1 = side * u1;
+      const double u2 = D.at(i2, j2);
+      if (u2 < kBig) n2 = side * u2;
       if (n1 < v1) {
         v2 = (n2 <= n1) ? n2 : kBig;
         v1 = n1;
@@ -149,9 +165,91 @@ __device__ __forceinline__ void reinit_sweep_at(const double* __restrict__ Din,
     return;
   }
   const double side = (phi[c] < 0.0) ? -1.0 : 1.0;
-  const double m = eikonal_update(Din, i, j, nr, nz, side, dx);
+  const double m = eikonal_update(GlobalIterate{Din, nr, nz}, i, j, side, dx);
   Dout[c] = (m >= kBig) ? dc : side * m;
 }
+
+// S Jacobi sweeps of one 8 x 32 tile in LDS (temporal blocking): the tile
+// and a halo of 2 S cells are loaded once; sweep s updates the tile with a
+// halo of 2 (S - s) cells (every neighbour it reads is within two cells and
+// was updated by the sweep before), so after nsw <= S sweeps the tile itself
+// holds what nsw sweeps over the whole grid give, bit for bit.  Flags per
+// cell: 1 frozen, 2 negative side, 4 outside the domain (its value stays
+// kBig, as the global sweep treats a neighbour beyond the edge).
+constexpr int kRtR = kReinitTileRows, kRtZ = kReinitTileCols;
+template <int S>
+struct ReinitTile {
+  static constexpr int H = 2 * S, RR = kRtR + 2 * H, RZ = kRtZ + 2 * H, NC = RR * RZ;
+  double a[NC], b[NC];
+  uint8_t f[NC];
+};
+
+template <int S>
+__device__ __forceinline__ void reinit_tile_sweeps(const double* __restrict__ Din,
+                                                   double* __restrict__ Dout,
+                                                   const double* __restrict__ phi,
+                                                   const uint8_t* __restrict__ frozen, int nr,
+                                                   int nz, double dx, int ti0, int tj0, int nsw,
+                                                   ReinitTile<S>& L) {
+  using Tl = ReinitTile<S>;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  for (int e = tid; e < Tl::NC; e += nth) {
+    const int i = ti0 - Tl::H + e / Tl::RZ, j = tj0 - Tl::H + e % Tl::RZ;
+    if (i >= 0 && i < nr && j >= 0 && j < nz) {
+      const int64_t c = (int64_t)i * nz + j;
+      L.a[e] = Din[c];
+      L.f[e] = uint8_t((frozen[c] ? 1 : 0) | (phi[c] < 0.0 ? 2 : 0));
+    } else {
+      L.a[e] = kBig;
+      L.f[e] = 4;
+    }
+  }
+  __syncthreads();
+  double* A = L.a;
+  double* B = L.b;
+  for (int s = 0; s < nsw; ++s) {
+    const int h = Tl::H - 2 * (s + 1);  // halo of the cells this sweep updates
+    const int w = kRtZ + 2 * h, n = (kRtR + 2 * h) * w;
+    const TileIterate it{A, ti0 - Tl::H, tj0 - Tl::H, Tl::RZ, nr, nz};
+    for (int e = tid; e < n; e += nth) {
+      const int a = e / w - h, b = e % w - h;  // tile-relative
+      const int li = (a + Tl::H) * Tl::RZ + (b + Tl::H);
+      const uint8_t fl = L.f[li];
+      const double dc = A[li];
+      double v = dc;
+      if (!(fl & 5)) {
+        const double side = (fl & 2) ? -1.0 : 1.0;
+        const double m = eikonal_update(it, ti0 + a, tj0 + b, side, dx);
+        v = (m >= kBig) ? dc : side * m;
+      }
+      B[li] = v;
+    }
+    __syncthreads();
+    double* t = A;
+    A = B;
+    B = t;
+  }
+  for (int e = tid; e < kRtR * kRtZ; e += nth) {
+    const int a = e / kRtZ, b = e % kRtZ;
+    const int i = ti0 + a, j = tj0 + b;
+    if (i < nr && j < nz) Dout[(int64_t)i * nz + j] = A[(a + Tl::H) * Tl::RZ + (b + Tl::H)];
+  }
+}
+
+template <int S>
+__global__ void __launch_bounds__(256) reinit_tiles_kernel(const double* __restrict__ Din,
+                                                           double* __restrict__ Dout,
+                                                           const double* __restrict__ phi,
+                                                           const uint8_t* __restrict__ frozen,
+                                                           int nr, int nz, double dx, int nsw) {
+  __shared__ ReinitTile<S> L;
+  reinit_tile_sweeps<S>(Din, Dout, phi, frozen, nr, nz, dx, blockIdx.y * kRtR, blockIdx.x * kRtZ,
+                        nsw, L);
+}
+
+// Sweeps per launch of the tiled form (16 x 40 cells in LDS, 11 KB; the
+// halo costs 1.34x the tile's own updates at S = 2).
+constexpr int kReinitSweepsPerLaunch = 2;
 
 __global__ void reinit_sweep_kernel(const double* __restrict__ Din, double* __restrict__ Dout,
                                     const double* __restrict__ phi,
@@ -197,14 +295,21 @@ __global__ void soft_batch_reinit_init_kernel(const SoftCase* __restrict__ cs,
   reinit_init_at(C.phi, C.rw.d0, C.rw.dist, C.rw.frozen, C.nr, C.nz, C.dx, b.i, b.j);
 }
 
-__global__ void soft_batch_reinit_sweep_kernel(const SoftCase* __restrict__ cs,
-                                               const uint32_t* __restrict__ blocks, int q) {
-  const SoftBlock b = soft_block(blocks);
-  const SoftCase& C = cs[b.c];
-  if (q >= C.reinit_iterations || b.j >= C.nz) return;
-  const bool odd = q & 1;
-  reinit_sweep_at(odd ? C.rw.d1 : C.rw.d0, odd ? C.rw.d0 : C.rw.d1, C.phi, C.rw.frozen, C.nr,
-                  C.nz, C.dx, b.i, b.j);
+// kReinitSweepsPerLaunch sweeps (fewer when the case has fewer left) of one
+// 8 x 32 tile of one case (the batch's reinit tile table); launch k reads
+// the first buffer when k is even.
+__global__ void __launch_bounds__(256) soft_batch_reinit_tiles_kernel(
+    const SoftCase* __restrict__ cs, const uint32_t* __restrict__ tiles, int q0) {
+  constexpr int S = kReinitSweepsPerLaunch;
+  __shared__ ReinitTile<S> L;
+  const uint32_t v = tiles[blockIdx.x];
+  const SoftCase& C = cs[v & 1023u];
+  const int nsw = std::min(S, C.reinit_iterations - q0);
+  if (nsw <= 0) return;
+  const bool odd = (q0 / S) & 1;
+  reinit_tile_sweeps<S>(odd ? C.rw.d1 : C.rw.d0, odd ? C.rw.d0 : C.rw.d1, C.phi, C.rw.frozen,
+                        C.nr, C.nz, C.dx, int((v >> 10) & 8191u) * kRtR, int(v >> 23) * kRtZ, nsw,
+                        L);
 }
 
 __global__ void soft_batch_reinit_final_kernel(const SoftCase* __restrict__ cs,
@@ -212,7 +317,9 @@ __global__ void soft_batch_reinit_final_kernel(const SoftCase* __restrict__ cs,
   const SoftBlock b = soft_block(blocks);
   const SoftCase& C = cs[b.c];
   if (b.j >= C.nz) return;
-  const double* D = (C.reinit_iterations & 1) ? C.rw.d1 : C.rw.d0;
+  constexpr int S = kReinitSweepsPerLaunch;
+  const int launches = (C.reinit_iterations + S - 1) / S;
+  const double* D = (launches & 1) ? C.rw.d1 : C.rw.d0;
   reinit_final_at(C.phi, D, C.rw.dist, C.rw.frozen, C.phi, nullptr, (int64_t)b.i * C.nz + b.j,
                   C.reinit_band);
 }
@@ -223,8 +330,8 @@ hipError_t launch_soft_batch_reinit(const SoftCase* cases, const SoftBatchDims& 
   soft_batch_reinit_init_kernel<<<g, 256, 0, s>>>(cases, d.cell_blocks);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  for (int q = 0; q < iterations; ++q) {
-    soft_batch_reinit_sweep_kernel<<<g, 256, 0, s>>>(cases, d.cell_blocks, q);
+  for (int q = 0; q < iterations; q += kReinitSweepsPerLaunch) {
+    soft_batch_reinit_tiles_kernel<<<d.nreinit_tiles, 256, 0, s>>>(cases, d.reinit_tiles, q);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   soft_batch_reinit_final_kernel<<<g, 256, 0, s>>>(cases, d.cell_blocks);
@@ -240,8 +347,11 @@ hipError_t launch_reinit(const double* phi, double* out, uint8_t* masked, int nr
   if (e != hipSuccess) return e;
   double* a = w.d0;
   double* b = w.d1;
-  for (int q = 0; q < iterations; ++q) {
-    reinit_sweep_kernel<<<grid, 256, 0, s>>>(a, b, phi, w.frozen, nr, nz, dx);
+  constexpr int S = kReinitSweepsPerLaunch;
+  const dim3 tiles(ceil_div(nz, kRtZ), ceil_div(nr, kRtR));
+  for (int q = 0; q < iterations; q += S) {
+    reinit_tiles_kernel<S><<<tiles, 256, 0, s>>>(a, b, phi, w.frozen, nr, nz, dx,
+                                                 std::min(S, iterations - q));
     if ((e = hipGetLastError()) != hipSuccess) return e;
     double* t = a;
     a = b;

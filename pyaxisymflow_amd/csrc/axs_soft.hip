@@ -788,6 +788,14 @@ struct SoftBatch {
         for (int b = 0; b < (int)ceil_div(C.nz, 256); ++b) tab.push_back(soft_block_entry(c, i, b));
     }
     dims.ndbl_blocks = (int)tab.size() - dims.ncell_blocks;
+    const size_t tiles0 = tab.size();
+    for (int c = 0; c < dims.ncase; ++c) {
+      const SoftCase& C = h_cases[c];
+      for (int ti = 0; ti < (int)ceil_div(C.nr, kReinitTileRows); ++ti)
+        for (int tj = 0; tj < (int)ceil_div(C.nz, kReinitTileCols); ++tj)
+          tab.push_back(soft_block_entry(c, ti, tj));
+    }
+    dims.nreinit_tiles = (int)(tab.size() - tiles0);
     drop_graphs();
     table_dirty = false;
     if (cases.empty()) return hipSuccess;
@@ -806,6 +814,7 @@ struct SoftBatch {
       return e;
     dims.cell_blocks = d_blocks;
     dims.dbl_blocks = d_blocks + dims.ncell_blocks;
+    dims.reinit_tiles = d_blocks + tiles0;
     return hipMemcpy(d_cases, h_cases.data(), sizeof(SoftCase) * cases.size(),
                      hipMemcpyHostToDevice);
   }

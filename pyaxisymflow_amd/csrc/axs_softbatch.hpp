@@ -68,7 +68,12 @@ struct SoftBatchDims {
   int ncell_blocks = 0;
   const uint32_t* dbl_blocks = nullptr;
   int ndbl_blocks = 0;
+  const uint32_t* reinit_tiles = nullptr;  // reinit tiles of every case's grid
+  int nreinit_tiles = 0;
 };
+
+// Tile of the level-set reinit's tiled sweeps (rows x columns).
+constexpr int kReinitTileRows = 8, kReinitTileCols = 32;
 
 // Block-table entry: case (10 bits), row (13 bits), 256-column block (9 bits).
 constexpr int kSoftBatchMaxCases = 1024;
