@@ -32,8 +32,10 @@ int reinit_iterations(double narrow, double dx) {
 
 __device__ __forceinline__ void reinit_init_at(const double* __restrict__ phi,
                                                double* __restrict__ D, double* __restrict__ dist,
-                                               uint8_t* __restrict__ frozen, int nr, int nz,
+                                               uint8_t* __restrict__ frozen,
+                                               uint8_t* __restrict__ changed, int nr, int nz,
                                                double dx, int i, int j) {
+  changed[(int64_t)i * nz + j] = 1;  // the first sweep updates every cell
   const int64_t c = (int64_t)i * nz + j;
   const double p = phi[c];
   double inv = 0.0;
@@ -71,11 +73,11 @@ __device__ __forceinline__ void reinit_init_at(const double* __restrict__ phi,
 
 __global__ void reinit_init_kernel(const double* __restrict__ phi, double* __restrict__ D,
                                    double* __restrict__ dist, uint8_t* __restrict__ frozen,
-                                   int nr, int nz, double dx) {
+                                   uint8_t* __restrict__ changed, int nr, int nz, double dx) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= nz) return;
-  reinit_init_at(phi, D, dist, frozen, nr, nz, dx, i, j);
+  reinit_init_at(phi, D, dist, frozen, changed, nr, nz, dx, i, j);
 }
 
 // Upwind eikonal update of cell (i, j) from the iterate D (kBig = unknown),
@@ -182,13 +184,22 @@ struct ReinitTile {
   static constexpr int H = 2 * S, RR = kRtR + 2 * H, RZ = kRtZ + 2 * H, NC = RR * RZ;
   double a[NC], b[NC];
   uint8_t f[NC];
+  uint8_t ca[NC], cb[NC];  // changed by the sweep before (ping-pong)
 };
 
+// A cell's update reads only its neighbours one and two cells away along
+// each axis: where none of them changed in the sweep before, the update
+// gives what it gave then, so the cell keeps its value without recomputing
+// it (the result is the same bits; the first sweep updates every cell).
+// Changes are compared bit for bit.  ch_in / ch_out carry them between
+// launches.
 template <int S>
 __device__ __forceinline__ void reinit_tile_sweeps(const double* __restrict__ Din,
                                                    double* __restrict__ Dout,
                                                    const double* __restrict__ phi,
-                                                   const uint8_t* __restrict__ frozen, int nr,
+                                                   const uint8_t* __restrict__ frozen,
+                                                   const uint8_t* __restrict__ ch_in,
+                                                   uint8_t* __restrict__ ch_out, int nr,
                                                    int nz, double dx, int ti0, int tj0, int nsw,
                                                    ReinitTile<S>& L) {
   using Tl = ReinitTile<S>;
@@ -199,14 +210,18 @@ __device__ __forceinline__ void reinit_tile_sweeps(const double* __restrict__ Di
       const int64_t c = (int64_t)i * nz + j;
       L.a[e] = Din[c];
       L.f[e] = uint8_t((frozen[c] ? 1 : 0) | (phi[c] < 0.0 ? 2 : 0));
+      L.ca[e] = ch_in[c];
     } else {
       L.a[e] = kBig;
       L.f[e] = 4;
+      L.ca[e] = 0;
     }
   }
   __syncthreads();
   double* A = L.a;
   double* B = L.b;
+  uint8_t* CA = L.ca;
+  uint8_t* CB = L.cb;
   for (int s = 0; s < nsw; ++s) {
     const int h = Tl::H - 2 * (s + 1);  // halo of the cells this sweep updates
     const int w = kRtZ + 2 * h, n = (kRtR + 2 * h) * w;
@@ -217,22 +232,33 @@ __device__ __forceinline__ void reinit_tile_sweeps(const double* __restrict__ Di
       const uint8_t fl = L.f[li];
       const double dc = A[li];
       double v = dc;
-      if (!(fl & 5)) {
+      const int RZ = Tl::RZ;
+      const bool inputs_changed = CA[li - 2 * RZ] | CA[li - RZ] | CA[li + RZ] | CA[li + 2 * RZ] |
+                                  CA[li - 2] | CA[li - 1] | CA[li + 1] | CA[li + 2];
+      if (!(fl & 5) && inputs_changed) {
         const double side = (fl & 2) ? -1.0 : 1.0;
         const double m = eikonal_update(it, ti0 + a, tj0 + b, side, dx);
         v = (m >= kBig) ? dc : side * m;
       }
       B[li] = v;
+      CB[li] = uint8_t(__double_as_longlong(v) != __double_as_longlong(dc));
     }
     __syncthreads();
     double* t = A;
     A = B;
     B = t;
+    uint8_t* tc = CA;
+    CA = CB;
+    CB = tc;
   }
   for (int e = tid; e < kRtR * kRtZ; e += nth) {
     const int a = e / kRtZ, b = e % kRtZ;
     const int i = ti0 + a, j = tj0 + b;
-    if (i < nr && j < nz) Dout[(int64_t)i * nz + j] = A[(a + Tl::H) * Tl::RZ + (b + Tl::H)];
+    if (i < nr && j < nz) {
+      const int li = (a + Tl::H) * Tl::RZ + (b + Tl::H);
+      Dout[(int64_t)i * nz + j] = A[li];
+      ch_out[(int64_t)i * nz + j] = CA[li];
+    }
   }
 }
 
@@ -241,10 +267,12 @@ __global__ void __launch_bounds__(256) reinit_tiles_kernel(const double* __restr
                                                            double* __restrict__ Dout,
                                                            const double* __restrict__ phi,
                                                            const uint8_t* __restrict__ frozen,
-                                                           int nr, int nz, double dx, int nsw) {
+                                                           const uint8_t* __restrict__ ch_in,
+                                                           uint8_t* __restrict__ ch_out, int nr,
+                                                           int nz, double dx, int nsw) {
   __shared__ ReinitTile<S> L;
-  reinit_tile_sweeps<S>(Din, Dout, phi, frozen, nr, nz, dx, blockIdx.y * kRtR, blockIdx.x * kRtZ,
-                        nsw, L);
+  reinit_tile_sweeps<S>(Din, Dout, phi, frozen, ch_in, ch_out, nr, nz, dx, blockIdx.y * kRtR,
+                        blockIdx.x * kRtZ, nsw, L);
 }
 
 // Sweeps per launch of the tiled form (16 x 40 cells in LDS, 11 KB; the
@@ -292,7 +320,7 @@ __global__ void soft_batch_reinit_init_kernel(const SoftCase* __restrict__ cs,
   const SoftBlock b = soft_block(blocks);
   const SoftCase& C = cs[b.c];
   if (b.j >= C.nz) return;
-  reinit_init_at(C.phi, C.rw.d0, C.rw.dist, C.rw.frozen, C.nr, C.nz, C.dx, b.i, b.j);
+  reinit_init_at(C.phi, C.rw.d0, C.rw.dist, C.rw.frozen, C.rw.ch0, C.nr, C.nz, C.dx, b.i, b.j);
 }
 
 // kReinitSweepsPerLaunch sweeps (fewer when the case has fewer left) of one
@@ -308,8 +336,8 @@ __global__ void __launch_bounds__(256) soft_batch_reinit_tiles_kernel(
   if (nsw <= 0) return;
   const bool odd = (q0 / S) & 1;
   reinit_tile_sweeps<S>(odd ? C.rw.d1 : C.rw.d0, odd ? C.rw.d0 : C.rw.d1, C.phi, C.rw.frozen,
-                        C.nr, C.nz, C.dx, int((v >> 10) & 8191u) * kRtR, int(v >> 23) * kRtZ, nsw,
-                        L);
+                        odd ? C.rw.ch1 : C.rw.ch0, odd ? C.rw.ch0 : C.rw.ch1, C.nr, C.nz, C.dx,
+                        int((v >> 10) & 8191u) * kRtR, int(v >> 23) * kRtZ, nsw, L);
 }
 
 __global__ void soft_batch_reinit_final_kernel(const SoftCase* __restrict__ cs,
@@ -342,20 +370,25 @@ hipError_t launch_reinit(const double* phi, double* out, uint8_t* masked, int nr
                          double dx, double narrow, int iterations, const ReinitWork& w,
                          hipStream_t s) {
   const dim3 grid(ceil_div(nz, 256), nr);
-  reinit_init_kernel<<<grid, 256, 0, s>>>(phi, w.d0, w.dist, w.frozen, nr, nz, dx);
+  reinit_init_kernel<<<grid, 256, 0, s>>>(phi, w.d0, w.dist, w.frozen, w.ch0, nr, nz, dx);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   double* a = w.d0;
   double* b = w.d1;
   constexpr int S = kReinitSweepsPerLaunch;
   const dim3 tiles(ceil_div(nz, kRtZ), ceil_div(nr, kRtR));
+  uint8_t* ca = w.ch0;
+  uint8_t* cb = w.ch1;
   for (int q = 0; q < iterations; q += S) {
-    reinit_tiles_kernel<S><<<tiles, 256, 0, s>>>(a, b, phi, w.frozen, nr, nz, dx,
+    reinit_tiles_kernel<S><<<tiles, 256, 0, s>>>(a, b, phi, w.frozen, ca, cb, nr, nz, dx,
                                                  std::min(S, iterations - q));
     if ((e = hipGetLastError()) != hipSuccess) return e;
     double* t = a;
     a = b;
     b = t;
+    uint8_t* tc = ca;
+    ca = cb;
+    cb = tc;
   }
   const int64_t n = (int64_t)nr * nz;
   reinit_final_kernel<<<ceil_div(n, 256), 256, 0, s>>>(phi, a, w.dist, w.frozen, out, masked, n,
@@ -383,6 +416,8 @@ axs_status axs_reinit_distance_f64(const double* phi, double* out, uint8_t* mask
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.d1), sizeof(double) * n);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.dist), sizeof(double) * n);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.frozen), n);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.ch0), n);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.ch1), n);
   hipStream_t s = as_stream(stream);
   if (e == hipSuccess) e = launch_reinit(phi, out, masked, nr, nz, dx, narrow, iterations, w, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -390,6 +425,8 @@ axs_status axs_reinit_distance_f64(const double* phi, double* out, uint8_t* mask
   if (w.d1) (void)hipFree(w.d1);
   if (w.dist) (void)hipFree(w.dist);
   if (w.frozen) (void)hipFree(w.frozen);
+  if (w.ch0) (void)hipFree(w.ch0);
+  if (w.ch1) (void)hipFree(w.ch1);
   if (e != hipSuccess) {
     set_error(std::string("reinit: ") + hipGetErrorString(e));
     return AXS_ERR_HIP;

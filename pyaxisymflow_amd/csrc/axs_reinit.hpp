@@ -12,6 +12,8 @@ struct ReinitWork {
   double* d1;       // distance iterate (pong)
   double* dist;     // interface distances of the frozen cells
   uint8_t* frozen;  // 1 on interface cells
+  uint8_t* ch0;     // 1 where the last sweep changed the cell (ping)
+  uint8_t* ch1;     // (pong)
 };
 
 // Sweeps the reference count of Jacobi iterations for a band.

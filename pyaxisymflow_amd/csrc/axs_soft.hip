@@ -353,6 +353,8 @@ struct SoftStepper {
     for (double** f : fields)
       if ((e = alloc(f, n * B)) != hipSuccess) return e;
     if ((e = alloc(&rw.frozen, n)) != hipSuccess) return e;
+    if ((e = alloc(&rw.ch0, n)) != hipSuccess) return e;
+    if ((e = alloc(&rw.ch1, n)) != hipSuccess) return e;
     if ((e = alloc(&e1d, nd * B)) != hipSuccess) return e;
     if ((e = alloc(&e2d, nd * B)) != hipSuccess) return e;
     if ((e = alloc(&cur, nd * sizeof(int16_t))) != hipSuccess) return e;
