@@ -30,12 +30,34 @@ int reinit_iterations(double narrow, double dx) {
   return 2 * static_cast<int>(std::ceil(narrow / dx)) + 8;
 }
 
+// The sweeps run on 8 x 32 tiles (kReinitTileRows x kReinitTileCols).
+__host__ __device__ __forceinline__ int reinit_tile_cols(int nz) {
+  return (nz + kReinitTileCols - 1) / kReinitTileCols;
+}
+__host__ __device__ __forceinline__ int reinit_tile_count(int nr, int nz) {
+  return ((nr + kReinitTileRows - 1) / kReinitTileRows) * reinit_tile_cols(nz);
+}
+
+size_t reinit_tile_flag_bytes(int nr, int nz) { return 4 * (size_t)reinit_tile_count(nr, nz); }
+
+// Tile flags (ReinitWork::tiles, four rows of one byte per tile): row p
+// (p = launch parity) = some cell of the tile changed during launch k,
+// row 2 + p = some cell changed in launch k's last sweep.  Before the first
+// launch they read as launch -1 (row 1: nothing, row 3: the tile holds an
+// interface cell) and launch -2 (row 2: nothing).
+//
+// Both iterate buffers start as the initial iterate (kBig off the
+// interface), with the interface cells marked changed in the first and
+// nothing in the second: a cell none of whose neighbours is an interface
+// cell has only kBig neighbours, which leave it kBig, so marking the
+// interface cells alone starts the sweeps as marking every cell would.
 __device__ __forceinline__ void reinit_init_at(const double* __restrict__ phi,
-                                               double* __restrict__ D, double* __restrict__ dist,
+                                               double* __restrict__ D, double* __restrict__ D1,
+                                               double* __restrict__ dist,
                                                uint8_t* __restrict__ frozen,
-                                               uint8_t* __restrict__ changed, int nr, int nz,
+                                               uint8_t* __restrict__ ch0, uint8_t* __restrict__ ch1,
+                                               uint8_t* __restrict__ tiles, int nr, int nz,
                                                double dx, int i, int j) {
-  changed[(int64_t)i * nz + j] = 1;  // the first sweep updates every cell
   const int64_t c = (int64_t)i * nz + j;
   const double p = phi[c];
   double inv = 0.0;
@@ -69,15 +91,20 @@ __device__ __forceinline__ void reinit_init_at(const double* __restrict__ phi,
   dist[c] = dv;
   frozen[c] = fr ? 1 : 0;
   D[c] = fr ? dv : kBig;
+  D1[c] = fr ? dv : kBig;
+  ch0[c] = fr ? 1 : 0;
+  ch1[c] = 0;
+  if (fr)
+    tiles[3 * (size_t)reinit_tile_count(nr, nz) + (i / kReinitTileRows) * reinit_tile_cols(nz) +
+          j / kReinitTileCols] = 1;
 }
 
-__global__ void reinit_init_kernel(const double* __restrict__ phi, double* __restrict__ D,
-                                   double* __restrict__ dist, uint8_t* __restrict__ frozen,
-                                   uint8_t* __restrict__ changed, int nr, int nz, double dx) {
+__global__ void reinit_init_kernel(const double* __restrict__ phi, ReinitWork w, int nr, int nz,
+                                   double dx) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= nz) return;
-  reinit_init_at(phi, D, dist, frozen, changed, nr, nz, dx, i, j);
+  reinit_init_at(phi, w.d0, w.d1, w.dist, w.frozen, w.ch0, w.ch1, w.tiles, nr, nz, dx, i, j);
 }
 
 // Upwind eikonal update of cell (i, j) from the iterate D (kBig = unknown),
@@ -193,17 +220,57 @@ struct ReinitTile {
 // it (the result is the same bits; the first sweep updates every cell).
 // Changes are compared bit for bit.  ch_in / ch_out carry them between
 // launches.
+//
+// Launch k of a tile whose 3 x 3 tile neighbourhood (which holds its halo)
+// changed nothing in launch k-1's last sweep changes nothing either: it
+// only brings its output buffer up to date, which holds what launch k-2
+// left (copy the iterate when launch k-1 changed the tile, clear the marks
+// when launch k-2's last sweep left some).
 template <int S>
 __device__ __forceinline__ void reinit_tile_sweeps(const double* __restrict__ Din,
                                                    double* __restrict__ Dout,
                                                    const double* __restrict__ phi,
                                                    const uint8_t* __restrict__ frozen,
                                                    const uint8_t* __restrict__ ch_in,
-                                                   uint8_t* __restrict__ ch_out, int nr,
-                                                   int nz, double dx, int ti0, int tj0, int nsw,
+                                                   uint8_t* __restrict__ ch_out,
+                                                   uint8_t* __restrict__ tiles, int nr, int nz,
+                                                   double dx, int tr, int tc, int k, int nsw,
                                                    ReinitTile<S>& L) {
   using Tl = ReinitTile<S>;
+  static_assert(Tl::H <= kRtR && Tl::H <= kRtZ, "the halo must lie in the neighbour tiles");
   const int tid = threadIdx.x, nth = blockDim.x;
+  const int ti0 = tr * kRtR, tj0 = tc * kRtZ;
+  const int ntc = reinit_tile_cols(nz), nt = reinit_tile_count(nr, nz);
+  const int ntr = nt / ntc, t = tr * ntc + tc;
+  const int p = k & 1;
+  uint8_t* any_w = tiles + (size_t)p * nt;
+  const uint8_t* any_p = tiles + (size_t)(p ^ 1) * nt;
+  uint8_t* last_w = tiles + (size_t)(2 + p) * nt;
+  const uint8_t* last_p = tiles + (size_t)(2 + (p ^ 1)) * nt;
+  int near = 0;
+  if (tid < 9) {
+    const int r = tr + tid / 3 - 1, q = tc + tid % 3 - 1;
+    if (r >= 0 && r < ntr && q >= 0 && q < ntc) near = last_p[r * ntc + q];
+  }
+  if (!__syncthreads_or(near)) {
+    const bool copy = any_p[t] != 0, clear = last_w[t] != 0;
+    __syncthreads();  // every thread has read last_w[t]
+    if (copy || clear)
+      for (int e = tid; e < kRtR * kRtZ; e += nth) {
+        const int i = ti0 + e / kRtZ, j = tj0 + e % kRtZ;
+        if (i >= nr || j >= nz) continue;
+        const int64_t c = (int64_t)i * nz + j;
+        if (copy) Dout[c] = Din[c];
+        if (clear) ch_out[c] = 0;
+      }
+    if (tid == 0) {
+      any_w[t] = 0;
+      last_w[t] = 0;
+    }
+    return;
+  }
+  __shared__ int any_change;
+  if (tid == 0) any_change = 0;
   for (int e = tid; e < Tl::NC; e += nth) {
     const int i = ti0 - Tl::H + e / Tl::RZ, j = tj0 - Tl::H + e % Tl::RZ;
     if (i >= 0 && i < nr && j >= 0 && j < nz) {
@@ -241,7 +308,9 @@ __device__ __forceinline__ void reinit_tile_sweeps(const double* __restrict__ Di
         v = (m >= kBig) ? dc : side * m;
       }
       B[li] = v;
-      CB[li] = uint8_t(__double_as_longlong(v) != __double_as_longlong(dc));
+      const bool ch = __double_as_longlong(v) != __double_as_longlong(dc);
+      CB[li] = uint8_t(ch);
+      if (ch && a >= 0 && a < kRtR && b >= 0 && b < kRtZ) any_change = 1;
     }
     __syncthreads();
     double* t = A;
@@ -251,6 +320,7 @@ __device__ __forceinline__ void reinit_tile_sweeps(const double* __restrict__ Di
     CA = CB;
     CB = tc;
   }
+  int last = 0;
   for (int e = tid; e < kRtR * kRtZ; e += nth) {
     const int a = e / kRtZ, b = e % kRtZ;
     const int i = ti0 + a, j = tj0 + b;
@@ -258,7 +328,13 @@ __device__ __forceinline__ void reinit_tile_sweeps(const double* __restrict__ Di
       const int li = (a + Tl::H) * Tl::RZ + (b + Tl::H);
       Dout[(int64_t)i * nz + j] = A[li];
       ch_out[(int64_t)i * nz + j] = CA[li];
+      last |= CA[li];
     }
+  }
+  last = __syncthreads_or(last);
+  if (tid == 0) {
+    any_w[t] = uint8_t(any_change != 0);
+    last_w[t] = uint8_t(last != 0);
   }
 }
 
@@ -268,11 +344,12 @@ __global__ void __launch_bounds__(256) reinit_tiles_kernel(const double* __restr
                                                            const double* __restrict__ phi,
                                                            const uint8_t* __restrict__ frozen,
                                                            const uint8_t* __restrict__ ch_in,
-                                                           uint8_t* __restrict__ ch_out, int nr,
-                                                           int nz, double dx, int nsw) {
+                                                           uint8_t* __restrict__ ch_out,
+                                                           uint8_t* __restrict__ tiles, int nr,
+                                                           int nz, double dx, int k, int nsw) {
   __shared__ ReinitTile<S> L;
-  reinit_tile_sweeps<S>(Din, Dout, phi, frozen, ch_in, ch_out, nr, nz, dx, blockIdx.y * kRtR,
-                        blockIdx.x * kRtZ, nsw, L);
+  reinit_tile_sweeps<S>(Din, Dout, phi, frozen, ch_in, ch_out, tiles, nr, nz, dx, blockIdx.y,
+                        blockIdx.x, k, nsw, L);
 }
 
 // Sweeps per launch of the tiled form (16 x 40 cells in LDS, 11 KB; the
@@ -289,13 +366,21 @@ __global__ void reinit_sweep_kernel(const double* __restrict__ Din, double* __re
   reinit_sweep_at(Din, Dout, phi, frozen, nr, nz, dx, i, j);
 }
 
+// Also leaves the tile flags zero for the next reinit.
 __device__ __forceinline__ void reinit_final_at(const double* __restrict__ phi,
                                                 const double* __restrict__ D,
                                                 const double* __restrict__ dist,
                                                 const uint8_t* __restrict__ frozen,
+                                                uint8_t* __restrict__ tiles,
                                                 double* __restrict__ out,
-                                                uint8_t* __restrict__ masked, int64_t c,
-                                                double narrow) {
+                                                uint8_t* __restrict__ masked, int nr, int nz,
+                                                int i, int j, double narrow) {
+  if (i % kReinitTileRows == 0 && j % kReinitTileCols == 0) {
+    const int nt = reinit_tile_count(nr, nz);
+    const int t = (i / kReinitTileRows) * reinit_tile_cols(nz) + j / kReinitTileCols;
+    for (int r = 0; r < 4; ++r) tiles[(size_t)r * nt + t] = 0;
+  }
+  const int64_t c = (int64_t)i * nz + j;
   const double d = frozen[c] ? dist[c] : D[c];
   const bool m = !(fabs(d) <= narrow);
   out[c] = m ? phi[c] : d;
@@ -303,12 +388,12 @@ __device__ __forceinline__ void reinit_final_at(const double* __restrict__ phi,
 }
 
 __global__ void reinit_final_kernel(const double* __restrict__ phi, const double* __restrict__ D,
-                                    const double* __restrict__ dist,
-                                    const uint8_t* __restrict__ frozen, double* __restrict__ out,
-                                    uint8_t* __restrict__ masked, int64_t n, double narrow) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  reinit_final_at(phi, D, dist, frozen, out, masked, c, narrow);
+                                    ReinitWork w, double* __restrict__ out,
+                                    uint8_t* __restrict__ masked, int nr, int nz, double narrow) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= nz) return;
+  reinit_final_at(phi, D, w.dist, w.frozen, w.tiles, out, masked, nr, nz, i, j, narrow);
 }
 
 // Soft batches: one row segment of 256 cells of one case per workgroup;
@@ -320,7 +405,8 @@ __global__ void soft_batch_reinit_init_kernel(const SoftCase* __restrict__ cs,
   const SoftBlock b = soft_block(blocks);
   const SoftCase& C = cs[b.c];
   if (b.j >= C.nz) return;
-  reinit_init_at(C.phi, C.rw.d0, C.rw.dist, C.rw.frozen, C.rw.ch0, C.nr, C.nz, C.dx, b.i, b.j);
+  reinit_init_at(C.phi, C.rw.d0, C.rw.d1, C.rw.dist, C.rw.frozen, C.rw.ch0, C.rw.ch1, C.rw.tiles,
+                 C.nr, C.nz, C.dx, b.i, b.j);
 }
 
 // kReinitSweepsPerLaunch sweeps (fewer when the case has fewer left) of one
@@ -334,10 +420,11 @@ __global__ void __launch_bounds__(256) soft_batch_reinit_tiles_kernel(
   const SoftCase& C = cs[v & 1023u];
   const int nsw = std::min(S, C.reinit_iterations - q0);
   if (nsw <= 0) return;
-  const bool odd = (q0 / S) & 1;
+  const int k = q0 / S;
+  const bool odd = k & 1;
   reinit_tile_sweeps<S>(odd ? C.rw.d1 : C.rw.d0, odd ? C.rw.d0 : C.rw.d1, C.phi, C.rw.frozen,
-                        odd ? C.rw.ch1 : C.rw.ch0, odd ? C.rw.ch0 : C.rw.ch1, C.nr, C.nz, C.dx,
-                        int((v >> 10) & 8191u) * kRtR, int(v >> 23) * kRtZ, nsw, L);
+                        odd ? C.rw.ch1 : C.rw.ch0, odd ? C.rw.ch0 : C.rw.ch1, C.rw.tiles, C.nr,
+                        C.nz, C.dx, int((v >> 10) & 8191u), int(v >> 23), k, nsw, L);
 }
 
 __global__ void soft_batch_reinit_final_kernel(const SoftCase* __restrict__ cs,
@@ -348,8 +435,8 @@ __global__ void soft_batch_reinit_final_kernel(const SoftCase* __restrict__ cs,
   constexpr int S = kReinitSweepsPerLaunch;
   const int launches = (C.reinit_iterations + S - 1) / S;
   const double* D = (launches & 1) ? C.rw.d1 : C.rw.d0;
-  reinit_final_at(C.phi, D, C.rw.dist, C.rw.frozen, C.phi, nullptr, (int64_t)b.i * C.nz + b.j,
-                  C.reinit_band);
+  reinit_final_at(C.phi, D, C.rw.dist, C.rw.frozen, C.rw.tiles, C.phi, nullptr, C.nr, C.nz, b.i,
+                  b.j, C.reinit_band);
 }
 
 hipError_t launch_soft_batch_reinit(const SoftCase* cases, const SoftBatchDims& d, int iterations,
@@ -370,7 +457,7 @@ hipError_t launch_reinit(const double* phi, double* out, uint8_t* masked, int nr
                          double dx, double narrow, int iterations, const ReinitWork& w,
                          hipStream_t s) {
   const dim3 grid(ceil_div(nz, 256), nr);
-  reinit_init_kernel<<<grid, 256, 0, s>>>(phi, w.d0, w.dist, w.frozen, w.ch0, nr, nz, dx);
+  reinit_init_kernel<<<grid, 256, 0, s>>>(phi, w, nr, nz, dx);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   double* a = w.d0;
@@ -380,8 +467,8 @@ hipError_t launch_reinit(const double* phi, double* out, uint8_t* masked, int nr
   uint8_t* ca = w.ch0;
   uint8_t* cb = w.ch1;
   for (int q = 0; q < iterations; q += S) {
-    reinit_tiles_kernel<S><<<tiles, 256, 0, s>>>(a, b, phi, w.frozen, ca, cb, nr, nz, dx,
-                                                 std::min(S, iterations - q));
+    reinit_tiles_kernel<S><<<tiles, 256, 0, s>>>(a, b, phi, w.frozen, ca, cb, w.tiles, nr, nz, dx,
+                                                 q / S, std::min(S, iterations - q));
     if ((e = hipGetLastError()) != hipSuccess) return e;
     double* t = a;
     a = b;
@@ -390,9 +477,7 @@ hipError_t launch_reinit(const double* phi, double* out, uint8_t* masked, int nr
     ca = cb;
     cb = tc;
   }
-  const int64_t n = (int64_t)nr * nz;
-  reinit_final_kernel<<<ceil_div(n, 256), 256, 0, s>>>(phi, a, w.dist, w.frozen, out, masked, n,
-                                                       narrow);
+  reinit_final_kernel<<<grid, 256, 0, s>>>(phi, a, w, out, masked, nr, nz, narrow);
   return hipGetLastError();
 }
 
@@ -418,7 +503,10 @@ axs_status axs_reinit_distance_f64(const double* phi, double* out, uint8_t* mask
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.frozen), n);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.ch0), n);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.ch1), n);
+  const size_t tb = reinit_tile_flag_bytes(nr, nz);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.tiles), tb);
   hipStream_t s = as_stream(stream);
+  if (e == hipSuccess) e = hipMemsetAsync(w.tiles, 0, tb, s);
   if (e == hipSuccess) e = launch_reinit(phi, out, masked, nr, nz, dx, narrow, iterations, w, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (w.d0) (void)hipFree(w.d0);
@@ -427,6 +515,7 @@ axs_status axs_reinit_distance_f64(const double* phi, double* out, uint8_t* mask
   if (w.frozen) (void)hipFree(w.frozen);
   if (w.ch0) (void)hipFree(w.ch0);
   if (w.ch1) (void)hipFree(w.ch1);
+  if (w.tiles) (void)hipFree(w.tiles);
   if (e != hipSuccess) {
     set_error(std::string("reinit: ") + hipGetErrorString(e));
     return AXS_ERR_HIP;

@@ -355,6 +355,9 @@ struct SoftStepper {
     if ((e = alloc(&rw.frozen, n)) != hipSuccess) return e;
     if ((e = alloc(&rw.ch0, n)) != hipSuccess) return e;
     if ((e = alloc(&rw.ch1, n)) != hipSuccess) return e;
+    const size_t tb = reinit_tile_flag_bytes(cfg.nr, cfg.nz);
+    if ((e = alloc(&rw.tiles, tb)) != hipSuccess) return e;
+    if ((e = hipMemset(rw.tiles, 0, tb)) != hipSuccess) return e;
     if ((e = alloc(&e1d, nd * B)) != hipSuccess) return e;
     if ((e = alloc(&e2d, nd * B)) != hipSuccess) return e;
     if ((e = alloc(&cur, nd * sizeof(int16_t))) != hipSuccess) return e;
