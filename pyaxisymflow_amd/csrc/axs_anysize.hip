@@ -13,7 +13,11 @@
 //   v_n = x_{2n}, v_{N-1-n} = x_{2n+1},  V = DFT_N(v),
 //   X_k = Re(e^{-i pi k / (2N)} V_k)      (X_k = sum_n x_n cos(pi k (2n+1) / (2N)))
 // and its inverse V_k = e^{i pi k / (2N)} (X_k - i X_{N-k}) (X_N = 0),
-// v = DFT_N^{-1}(V) / N, x unshuffled; valid for odd N too.
+// v = DFT_N^{-1}(V) / N, x unshuffled; valid for odd N too.  For even N the
+// real sequence v is packed as z_m = v_{2m} + i v_{2m+1} and V comes from
+// one length-N/2 DFT of z (the split of dct2_post_at / dct3_pre_at, as the
+// half-length transforms of the smooth lengths do): half the DFT length,
+// half the LDS.
 // DST-I of x[0..N) (S_k = sum_j x_j sin(pi (j+1)(k+1) / (N+1))) through the
 // odd extension y of length 2M, M = N + 1 (y_0 = y_M = 0, y_m = x_{m-1},
 // y_{2M-m} = -x_{m-1}), packed as z_p = y_{2p} + i y_{2p+1}:
@@ -155,12 +159,23 @@ AnyDft::~AnyDft() {
 // taper_dct2_row does; the tapered row also goes to `tapered` when not
 // null), dst row = X.  Threads tid of nth, work of a.L entries in LDS; every
 // thread of the block calls it (block barriers).
+// The tables of one plan's DCT rows: a = the DFT (length nz, or nz / 2
+// when half), ge2[k] = e^{-i pi k / (2 nz)} for k < nz (full form), e1[k] =
+// e^{-2 pi i k / nz} and e2[k] = e^{-i pi k / (2 nz)} for k <= nz / 2 (half
+// form).
+struct AnyDctTables {
+  AnyDftArgs a;
+  const Cplx<double>* ge2;
+  const Cplx<double>* e1;
+  const Cplx<double>* e2;
+  int half;
+};
+
 template <typename T, int PER, bool TAPER>
 __device__ void any_dct2_row(const T* __restrict__ src, T* __restrict__ tapered,
                              T* __restrict__ dst, const double* __restrict__ rscale,
                              const double* __restrict__ wz, const double* __restrict__ wr, int nr,
-                             int nz, int w, const AnyDftArgs& a,
-                             const Cplx<double>* __restrict__ e2, int i, Cplx<T>* work, int tid,
+                             int nz, int w, const AnyDctTables& tb, int i, Cplx<T>* work, int tid,
                              int nth) {
   const int srow = (TAPER && i >= nr - w) ? nr - w : i;
   const T* row = src + (int64_t)srow * nz;
@@ -183,13 +198,23 @@ __device__ void any_dct2_row(const T* __restrict__ src, T* __restrict__ tapered,
       if (i == 0) t = T(0);
       if (out_t) out_t[j] = t;
     }
-    work[dct_dst_index(j, nz)] = Cplx<T>{rs * t, T(0)};
+    if (tb.half)
+      packed_component(work, dct_dst_index(j, nz)) = rs * t;
+    else
+      work[dct_dst_index(j, nz)] = Cplx<T>{rs * t, T(0)};
   }
   __syncthreads();
-  any_dft<T, PER>(work, a, -1, tid, nth);
+  any_dft<T, PER>(work, tb.a, -1, tid, nth);
   T* out = dst + (int64_t)i * nz;
+  if (tb.half) {
+    const int M = nz / 2;
+    for (int k = tid; k <= M / 2; k += nth)
+      dct2_post_at<T>(work, M, k, tb.e1[k], tb.e1[M - k], tb.e2[k], tb.e2[M - k],
+                      [&](int q, T val) { out[q] = val; });
+    return;
+  }
   for (int k = tid; k < nz; k += nth) {
-    const Cplx<double> e = e2[k];
+    const Cplx<double> e = tb.ge2[k];
     const Cplx<T> v = work[k];
     out[k] = T(e.x) * v.x - T(e.y) * v.y;
   }
@@ -198,9 +223,25 @@ __device__ void any_dct2_row(const T* __restrict__ src, T* __restrict__ tapered,
 // Inverse (DCT-III scaled by 1 / N) of row i of one field.
 template <typename T, int PER>
 __device__ void any_dct3_row(const T* __restrict__ src, T* __restrict__ dst, int nz,
-                             const AnyDftArgs& a, const Cplx<double>* __restrict__ e2, int i,
-                             Cplx<T>* work, int tid, int nth) {
+                             const AnyDctTables& tb, int i, Cplx<T>* work, int tid, int nth) {
   const T* row = src + (int64_t)i * nz;
+  const T invN = T(1) / T(nz);
+  T* out = dst + (int64_t)i * nz;
+  if (tb.half) {
+    const int M = nz / 2;
+    for (int k = tid; k < M; k += nth) {
+      const int mk = M - k;  // in [1, M]
+      const T xk = row[k], xnk = (k == 0) ? T(0) : row[nz - k];
+      const T xm = row[mk], xnm = row[nz - mk];
+      work[k] = dct3_pre_at<T>(xk, xnk, xm, xnm, tb.e2[k], tb.e2[mk], tb.e1[k]);
+    }
+    __syncthreads();
+    any_dft<T, PER>(work, tb.a, +1, tid, nth);
+    for (int j = tid; j < nz; j += nth)
+      out[j] = packed_component(work, dct_dst_index(j, nz)) * invN;
+    return;
+  }
+  const Cplx<double>* __restrict__ e2 = tb.ge2;
   for (int k = tid; k < nz; k += nth) {
     const T xk = row[k];
     const T xn = (k == 0) ? T(0) : row[nz - k];
@@ -208,9 +249,7 @@ __device__ void any_dct3_row(const T* __restrict__ src, T* __restrict__ dst, int
     work[k] = cmul(Cplx<T>{T(e.x), T(-e.y)}, Cplx<T>{xk, -xn});
   }
   __syncthreads();
-  any_dft<T, PER>(work, a, +1, tid, nth);
-  const T invN = T(1) / T(nz);
-  T* out = dst + (int64_t)i * nz;
+  any_dft<T, PER>(work, tb.a, +1, tid, nth);
   for (int j = tid; j < nz; j += nth) out[j] = work[dct_dst_index(j, nz)].x * invN;
 }
 
@@ -220,8 +259,7 @@ template <typename T, int PER, bool TAPER>
 __global__ void __launch_bounds__(1024) any_dct2_rows_kernel(
     const T* __restrict__ src, T* __restrict__ tapered, T* __restrict__ dst,
     const double* __restrict__ rscale, const double* __restrict__ wz,
-    const double* __restrict__ wr, int nr, int nz, int w, AnyDftArgs a,
-    const Cplx<double>* __restrict__ e2, StepFinalize fin) {
+    const double* __restrict__ wr, int nr, int nz, int w, AnyDctTables tb, StepFinalize fin) {
   if (fin.partials && blockIdx.x == nr) {  // completes the previous step
     __shared__ double fin_red[1024];  // one entry per thread (finalize_pending)
     const int cb = blockIdx.y;
@@ -234,17 +272,16 @@ __global__ void __launch_bounds__(1024) any_dct2_rows_kernel(
   extern __shared__ unsigned char smem_raw[];
   const int64_t cs = (int64_t)blockIdx.y * nr * nz;
   any_dct2_row<T, PER, TAPER>(src + cs, (TAPER && tapered) ? tapered + cs : nullptr, dst + cs,
-                              rscale, wz, wr, nr, nz, w, a, e2, blockIdx.x,
+                              rscale, wz, wr, nr, nz, w, tb, blockIdx.x,
                               reinterpret_cast<Cplx<T>*>(smem_raw), threadIdx.x, blockDim.x);
 }
 
 template <typename T, int PER>
 __global__ void __launch_bounds__(1024) any_dct3_rows_kernel(
-    const T* __restrict__ src, T* __restrict__ dst, int nr, int nz, AnyDftArgs a,
-    const Cplx<double>* __restrict__ e2) {
+    const T* __restrict__ src, T* __restrict__ dst, int nr, int nz, AnyDctTables tb) {
   extern __shared__ unsigned char smem_raw[];
   const int64_t cs = (int64_t)blockIdx.y * nr * nz;
-  any_dct3_row<T, PER>(src + cs, dst + cs, nz, a, e2, blockIdx.x,
+  any_dct3_row<T, PER>(src + cs, dst + cs, nz, tb, blockIdx.x,
                        reinterpret_cast<Cplx<T>*>(smem_raw), threadIdx.x, blockDim.x);
 }
 
@@ -255,8 +292,9 @@ __global__ void __launch_bounds__(1024) soft_batch_dct2_kernel(const SoftCase* _
   const SoftCase& C = cs[blockIdx.y];
   if ((int)blockIdx.x >= C.nr) return;
   extern __shared__ unsigned char smem_raw[];
+  const AnyDctTables tb{C.any, C.ge2, C.te1, C.te2, C.any_half};
   any_dct2_row<double, PER, true>(C.vorticity, C.w0, C.spec, C.rvec, C.kill_wz, C.kill_wr, C.nr,
-                                  C.nz, C.kill_width, C.any, C.ge2, blockIdx.x,
+                                  C.nz, C.kill_width, tb, blockIdx.x,
                                   reinterpret_cast<Cplx<double>*>(smem_raw), threadIdx.x,
                                   blockDim.x);
 }
@@ -266,7 +304,8 @@ __global__ void __launch_bounds__(1024) soft_batch_dct3_kernel(const SoftCase* _
   const SoftCase& C = cs[blockIdx.y];
   if ((int)blockIdx.x >= C.nr) return;
   extern __shared__ unsigned char smem_raw[];
-  any_dct3_row<double, PER>(C.spec, C.psi, C.nz, C.any, C.ge2, blockIdx.x,
+  const AnyDctTables tb{C.any, C.ge2, C.te1, C.te2, C.any_half};
+  any_dct3_row<double, PER>(C.spec, C.psi, C.nz, tb, blockIdx.x,
                             reinterpret_cast<Cplx<double>*>(smem_raw), threadIdx.x, blockDim.x);
 }
 
@@ -445,17 +484,17 @@ hipError_t launch_any_dct2_rows(const StokesPlan& pl, const T* src, T* tapered, 
                                 const double* rscale, const double* wz, const double* wr, int w,
                                 bool taper, int batch, const StepFinalize* fin, hipStream_t s) {
   const AnyDft& D = *pl.any;
+  const AnyDctTables tb{D.args, pl.ge2, pl.e1, pl.e2, pl.any_half ? 1 : 0};
   const StepFinalize f = fin ? *fin : StepFinalize{};
   const dim3 grid(pl.nr + (f.partials ? 1 : 0), batch);
   const size_t shm = D.lds_bytes<T>();
   const int nth = D.nth;
   if (taper)
     AXS_ANY_PER(D.per, (any_dct2_rows_kernel<T, P, true><<<grid, nth, shm, s>>>(
-                       src, tapered, dst, rscale, wz, wr, pl.nr, pl.nz, w, D.args, pl.ge2, f)));
+                       src, tapered, dst, rscale, wz, wr, pl.nr, pl.nz, w, tb, f)));
   else
     AXS_ANY_PER(D.per, (any_dct2_rows_kernel<T, P, false><<<grid, nth, shm, s>>>(
-                       src, nullptr, dst, rscale, nullptr, nullptr, pl.nr, pl.nz, 0, D.args,
-                       pl.ge2, f)));
+                       src, nullptr, dst, rscale, nullptr, nullptr, pl.nr, pl.nz, 0, tb, f)));
   return hipGetLastError();
 }
 
@@ -463,8 +502,9 @@ template <typename T>
 hipError_t launch_any_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, int batch,
                                 hipStream_t s) {
   const AnyDft& D = *pl.any;
+  const AnyDctTables tb{D.args, pl.ge2, pl.e1, pl.e2, pl.any_half ? 1 : 0};
   AXS_ANY_PER(D.per, (any_dct3_rows_kernel<T, P><<<dim3(rows, batch), D.nth, D.lds_bytes<T>(), s>>>(
-                     src, dst, rows, pl.nz, D.args, pl.ge2)));
+                     src, dst, rows, pl.nz, tb)));
   return hipGetLastError();
 }
 

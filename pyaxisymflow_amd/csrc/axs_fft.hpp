@@ -231,33 +231,55 @@ __device__ __forceinline__ void dct2_post_twiddles(PostTwiddles<KQ>& t, int M,
   }
 }
 
+// One k <= M/2 of the post-processing and its partner M - k; e1k = e1[k],
+// e1p = e1[M - k], e2k = e2[k], e2p = e2[M - k].
+template <typename T, typename Out>
+__device__ __forceinline__ void dct2_post_at(const Cplx<T>* Z, int M, int k, Cplx<double> e1k,
+                                             Cplx<double> e1p, Cplx<double> e2k,
+                                             Cplx<double> e2p, Out out) {
+  const int N = 2 * M;
+  const int kp = M - k;  // partner, in [M/2, M]; Z is periodic in M
+  const Cplx<T> zk = Z[k], zp = Z[kp == M ? 0 : kp];
+  // V_k = s/2 - i w d / 2 with s = Z_k + conj(Z_kp), d = Z_k - conj(Z_kp)
+  const Cplx<T> sk = cadd(zk, cconj(zp)), dk = csub(zk, cconj(zp));
+  const Cplx<T> wdk = cmul(Cplx<T>{T(e1k.x), T(e1k.y)}, dk);
+  const Cplx<T> Vk = {T(0.5) * (sk.x + wdk.y), T(0.5) * (sk.y - wdk.x)};
+  const Cplx<T> wk = cmul(Cplx<T>{T(e2k.x), T(e2k.y)}, Vk);
+  out(k, wk.x);
+  if (k > 0) out(N - k, -wk.y);
+  if (kp != k) {
+    const Cplx<T> sp = cadd(zp, cconj(zk)), dp = csub(zp, cconj(zk));
+    const Cplx<T> wdp = cmul(Cplx<T>{T(e1p.x), T(e1p.y)}, dp);
+    const Cplx<T> Vp = {T(0.5) * (sp.x + wdp.y), T(0.5) * (sp.y - wdp.x)};
+    const Cplx<T> wp = cmul(Cplx<T>{T(e2p.x), T(e2p.y)}, Vp);
+    out(kp, wp.x);
+    if (kp != M) out(N - kp, -wp.y);  // for kp == M, N-kp == M is already written
+  }
+}
+
 template <typename T, int KQ, typename Out>
 __device__ __forceinline__ void dct2_post(const Cplx<T>* Z, int M, const PostTwiddles<KQ>& t,
                                           Out out, int tid, int nth) {
-  const int N = 2 * M;
 #pragma unroll
   for (int q = 0; q < KQ; ++q) {
     const int k = tid + q * nth;
-    if (k <= M / 2) {
-      const int kp = M - k;  // partner, in [M/2, M]; Z is periodic in M
-      const Cplx<T> zk = Z[k], zp = Z[kp == M ? 0 : kp];
-      // V_k = s/2 - i w d / 2 with s = Z_k + conj(Z_kp), d = Z_k - conj(Z_kp)
-      const Cplx<T> sk = cadd(zk, cconj(zp)), dk = csub(zk, cconj(zp));
-      const Cplx<T> wdk = cmul(Cplx<T>{T(t.e1k[q].x), T(t.e1k[q].y)}, dk);
-      const Cplx<T> Vk = {T(0.5) * (sk.x + wdk.y), T(0.5) * (sk.y - wdk.x)};
-      const Cplx<T> wk = cmul(Cplx<T>{T(t.e2k[q].x), T(t.e2k[q].y)}, Vk);
-      out(k, wk.x);
-      if (k > 0) out(N - k, -wk.y);
-      if (kp != k) {
-        const Cplx<T> sp = cadd(zp, cconj(zk)), dp = csub(zp, cconj(zk));
-        const Cplx<T> wdp = cmul(Cplx<T>{T(t.e1p[q].x), T(t.e1p[q].y)}, dp);
-        const Cplx<T> Vp = {T(0.5) * (sp.x + wdp.y), T(0.5) * (sp.y - wdp.x)};
-        const Cplx<T> wp = cmul(Cplx<T>{T(t.e2p[q].x), T(t.e2p[q].y)}, Vp);
-        out(kp, wp.x);
-        if (kp != M) out(N - kp, -wp.y);  // for kp == M, N-kp == M is already written
-      }
-    }
+    if (k <= M / 2) dct2_post_at<T>(Z, M, k, t.e1k[q], t.e1p[q], t.e2k[q], t.e2p[q], out);
   }
+}
+
+// Z_k of the inverse pre-processing below from xk = X_k, xnk = X_{N-k}
+// (0 for k = 0), xm = X_{M-k}, xnm = X_{N-M+k}, q2k = e2[k], q2m = e2[M-k],
+// q1k = e1[k].
+template <typename T>
+__device__ __forceinline__ Cplx<T> dct3_pre_at(T xk, T xnk, T xm, T xnm, Cplx<double> q2k,
+                                               Cplx<double> q2m, Cplx<double> q1k) {
+  const Cplx<T> Vk = cmul(Cplx<T>{T(q2k.x), T(-q2k.y)}, Cplx<T>{xk, -xnk});
+  const Cplx<T> Vm = cmul(Cplx<T>{T(q2m.x), T(-q2m.y)}, Cplx<T>{xm, -xnm});
+  const Cplx<T> Vkm = cconj(Vm);  // V_{k+M}
+  const Cplx<T> A = cadd(Vk, Vkm);
+  const Cplx<T> D = csub(Vk, Vkm);
+  const Cplx<T> B = cmul(D, Cplx<T>{T(q1k.x), T(-q1k.y)});
+  return Cplx<T>{A.x - B.y, A.y + B.x};
 }
 
 // Inverse pre-processing: Z_k = A_k + i B_k from X (read via `in`), k in [0, M).
@@ -289,15 +311,7 @@ __device__ __forceinline__ void dct3_pre(Cplx<T>* Z, int M, const Cplx<double>* 
 #pragma unroll
   for (int q = 0; q < KQ; ++q) {
     const int k = tid + q * nth;
-    if (k < M) {
-      const Cplx<T> Vk = cmul(Cplx<T>{T(q2k[q].x), T(-q2k[q].y)}, Cplx<T>{xk[q], -xnk[q]});
-      const Cplx<T> Vm = cmul(Cplx<T>{T(q2m[q].x), T(-q2m[q].y)}, Cplx<T>{xm[q], -xnm[q]});
-      const Cplx<T> Vkm = cconj(Vm);  // V_{k+M}
-      const Cplx<T> A = cadd(Vk, Vkm);
-      const Cplx<T> D = csub(Vk, Vkm);
-      const Cplx<T> B = cmul(D, Cplx<T>{T(q1k[q].x), T(-q1k[q].y)});
-      Z[k] = Cplx<T>{A.x - B.y, A.y + B.x};
-    }
+    if (k < M) Z[k] = dct3_pre_at<T>(xk[q], xnk[q], xm[q], xnm[q], q2k[q], q2m[q], q1k[q]);
   }
 }
 

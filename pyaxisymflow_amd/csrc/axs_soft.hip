@@ -749,6 +749,9 @@ struct SoftBatch {
     const StokesPlan& P = *S.plan;
     C.any = P.any->args;
     C.ge2 = P.ge2;
+    C.te1 = P.e1;
+    C.te2 = P.e2;
+    C.any_half = P.any_half ? 1 : 0;
     C.ta = P.a;
     C.tb = P.b;
     C.tc = P.c;

@@ -49,6 +49,8 @@ struct SoftCase {
   // row transforms (any length) and the radial solve's tables
   AnyDftArgs any;
   const Cplx<double>* ge2;
+  const Cplx<double>*te1, *te2;  // the plan's e1, e2 (half-length cosine rows)
+  int any_half;
   const double *ta, *tb, *tc, *tlam, *tcseg, *rvec;
   int tK, tP;
 };

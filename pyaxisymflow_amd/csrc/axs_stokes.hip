@@ -317,9 +317,14 @@ hipError_t StokesPlan::init_transforms(int nr_, int nz_, double dx_) {
   general = force_general || !(nz % 2 == 0 && factor_radices(nz / 2, &fft));
   hipError_t e;
   if (general) {
+    // the cosine rows of an even nz take the half-length DFT; the periodic
+    // real-DFT rows take length nz
+    any_half = bc == 0 && nz % 2 == 0;
     any = new (std::nothrow) AnyDft();
     if (!any) return hipErrorOutOfMemory;
-    if ((e = any->init(nz)) != hipSuccess) return e;
+    if ((e = any->init(any_half ? nz / 2 : nz)) != hipSuccess) return e;
+  }
+  if (general && !any_half) {
     std::vector<Cplx<double>> h_ge2(nz);
     for (int k = 0; k < nz; ++k) {
       const double ang = -M_PI * (double)k / (2.0 * nz);

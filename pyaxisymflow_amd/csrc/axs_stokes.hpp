@@ -24,9 +24,11 @@ struct StokesPlan {
   double dx = 0.0;
   FftPlanArgs fft{};
   // general: nz odd or nz / 2 with a prime factor above 5; the row passes
-  // then run the any-length transforms (axs_anysize.hip) with `any` (length
-  // nz) and ge2[k] = e^{-i pi k / (2 nz)}, k < nz
+  // then run the any-length transforms (axs_anysize.hip) with `any`: length
+  // nz / 2 for the cosine rows of an even nz (any_half; tables e1, e2),
+  // else length nz with ge2[k] = e^{-i pi k / (2 nz)}, k < nz
   bool general = false;
+  bool any_half = false;
   AnyDft* any = nullptr;
   Cplx<double>* ge2 = nullptr;
   // device tables
