@@ -86,4 +86,19 @@ __device__ __forceinline__ T div_by(T x, T d, T y) {
   }
 }
 
+// Appends value to list for the lanes with take set: one atomic per wave
+// for all of them (a counter that every lane of a large grid bumps alone
+// serialises the whole pass), lanes in lane order after the wave's base.
+__device__ __forceinline__ void list_append(bool take, int value, int* __restrict__ list,
+                                            unsigned* __restrict__ count) {
+  const unsigned long long mask = __ballot(take);
+  if (mask == 0ull) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+  unsigned base = 0u;
+  if (lane == leader) base = atomicAdd(count, static_cast<unsigned>(__popcll(mask)));
+  base = __shfl(base, leader);
+  if (take) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
 }  // namespace axs

@@ -211,21 +211,6 @@ __global__ void lsq_commit_list_kernel(int16_t* __restrict__ cur, double* __rest
   }
 }
 
-// Appends value to list for the lanes with take set: one atomic per wave
-// for all of them (a counter that every lane of a large grid bumps alone
-// serialises the whole pass), lanes in lane order after the wave's base.
-__device__ __forceinline__ void list_append(bool take, int value, int* __restrict__ list,
-                                            unsigned* __restrict__ count) {
-  const unsigned long long mask = __ballot(take);
-  if (mask == 0ull) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-  unsigned base = 0u;
-  if (lane == leader) base = atomicAdd(count, static_cast<unsigned>(__popcll(mask)));
-  base = __shfl(base, leader);
-  if (take) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
-}
-
 // Cells with current ^ target set, in any order (the set is what matters:
 // a sweep's fits read only the previous sweep's state); *count must be zero.
 __global__ void lsq_list_kernel(const int16_t* __restrict__ cur, const int16_t* __restrict__ tgt,

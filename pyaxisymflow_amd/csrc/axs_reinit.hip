@@ -30,33 +30,51 @@ int reinit_iterations(double narrow, double dx) {
   return 2 * static_cast<int>(std::ceil(narrow / dx)) + 8;
 }
 
-// The sweeps run on 8 x 32 tiles (kReinitTileRows x kReinitTileCols).
+// The sweeps run on 8 x 32 tiles (kReinitTileRows x kReinitTileCols), S
+// sweeps per launch; a tile's sweeps read its cells and a halo of 2 S
+// (its read region).
+constexpr int kRtR = kReinitTileRows, kRtZ = kReinitTileCols;
+constexpr int kReinitSweepsPerLaunch = 2;
+constexpr int kReinitHalo = 2 * kReinitSweepsPerLaunch;
+static_assert(kReinitHalo <= kRtR && kReinitHalo <= kRtZ,
+              "a read region must lie in the 3 x 3 tiles around its tile");
 __host__ __device__ __forceinline__ int reinit_tile_cols(int nz) {
-  return (nz + kReinitTileCols - 1) / kReinitTileCols;
+  return (nz + kRtZ - 1) / kRtZ;
 }
 __host__ __device__ __forceinline__ int reinit_tile_count(int nr, int nz) {
-  return ((nr + kReinitTileRows - 1) / kReinitTileRows) * reinit_tile_cols(nz);
+  return ((nr + kRtR - 1) / kRtR) * reinit_tile_cols(nz);
 }
 
-size_t reinit_tile_flag_bytes(int nr, int nz) { return 4 * (size_t)reinit_tile_count(nr, nz); }
+int reinit_tile_total(int nr, int nz) { return reinit_tile_count(nr, nz); }
 
-// Tile flags (ReinitWork::tiles, four rows of one byte per tile): row p
-// (p = launch parity) = some cell of the tile changed during launch k,
-// row 2 + p = some cell changed in launch k's last sweep.  Before the first
-// launch they read as launch -1 (row 1: nothing, row 3: the tile holds an
-// interface cell) and launch -2 (row 2: nothing).
+// Work lists.  A launch takes only the tiles on its list: those with a
+// cell changed by the launch before's last sweep in their read region
+// (flag bit 0: run the sweeps), and those that ran in the launch before
+// (bit 1: one that does not run again copies its iterate into the other
+// buffer and clears its marks there, so that a tile off the lists has the
+// same iterate in both buffers and no marks).  Every other tile would
+// compute what it holds.  Flags (ReinitWork::tiles): one word per tile and
+// launch parity, set with atomicOr by the tiles that list it (the one that
+// finds it zero appends the tile) and cleared when the tile is taken.
+// Lists: two of `cap` entries (by parity); counters: three, launch k reads
+// k % 3, fills (k + 1) % 3 and clears (k + 2) % 3.
 //
 // Both iterate buffers start as the initial iterate (kBig off the
 // interface), with the interface cells marked changed in the first and
 // nothing in the second: a cell none of whose neighbours is an interface
 // cell has only kBig neighbours, which leave it kBig, so marking the
 // interface cells alone starts the sweeps as marking every cell would.
+// The first list holds the tiles with an interface cell in their read
+// region.
+
+// floor(x / d) for d > 0 and any x
+__device__ __forceinline__ int floor_div(int x, int d) { return (x >= 0) ? x / d : -((d - 1 - x) / d); }
 __device__ __forceinline__ void reinit_init_at(const double* __restrict__ phi,
                                                double* __restrict__ D, double* __restrict__ D1,
                                                double* __restrict__ dist,
                                                uint8_t* __restrict__ frozen,
                                                uint8_t* __restrict__ ch0, uint8_t* __restrict__ ch1,
-                                               uint8_t* __restrict__ tiles, int nr, int nz,
+                                               unsigned* __restrict__ tiles, int nr, int nz,
                                                double dx, int i, int j) {
   const int64_t c = (int64_t)i * nz + j;
   const double p = phi[c];
@@ -94,13 +112,20 @@ __device__ __forceinline__ void reinit_init_at(const double* __restrict__ phi,
   D1[c] = fr ? dv : kBig;
   ch0[c] = fr ? 1 : 0;
   ch1[c] = 0;
-  if (fr)
-    tiles[3 * (size_t)reinit_tile_count(nr, nz) + (i / kReinitTileRows) * reinit_tile_cols(nz) +
-          j / kReinitTileCols] = 1;
+  if (fr) {  // the tiles whose read region holds the cell run the first launch
+    const int ntc = reinit_tile_cols(nz), ntr = reinit_tile_count(nr, nz) / ntc;
+    const int r0 = max(0, floor_div(i - kReinitHalo, kRtR));
+    const int r1 = min(ntr - 1, floor_div(i + kReinitHalo, kRtR));
+    const int q0 = max(0, floor_div(j - kReinitHalo, kRtZ));
+    const int q1 = min(ntc - 1, floor_div(j + kReinitHalo, kRtZ));
+    for (int r = r0; r <= r1; ++r)
+      for (int q = q0; q <= q1; ++q) tiles[r * ntc + q] = 1u;
+  }
 }
 
 __global__ void reinit_init_kernel(const double* __restrict__ phi, ReinitWork w, int nr, int nz,
                                    double dx) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2) w.cnt[threadIdx.x] = 0u;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= nz) return;
@@ -205,7 +230,6 @@ __device__ __forceinline__ void reinit_sweep_at(const double* __restrict__ Din,
 // holds what nsw sweeps over the whole grid give, bit for bit.  Flags per
 // cell: 1 frozen, 2 negative side, 4 outside the domain (its value stays
 // kBig, as the global sweep treats a neighbour beyond the edge).
-constexpr int kRtR = kReinitTileRows, kRtZ = kReinitTileCols;
 template <int S>
 struct ReinitTile {
   static constexpr int H = 2 * S, RR = kRtR + 2 * H, RZ = kRtZ + 2 * H, NC = RR * RZ;
@@ -214,63 +238,57 @@ struct ReinitTile {
   uint8_t ca[NC], cb[NC];  // changed by the sweep before (ping-pong)
 };
 
-// A cell's update reads only its neighbours one and two cells away along
-// each axis: where none of them changed in the sweep before, the update
-// gives what it gave then, so the cell keeps its value without recomputing
-// it (the result is the same bits; the first sweep updates every cell).
-// Changes are compared bit for bit.  ch_in / ch_out carry them between
-// launches.
-//
-// Launch k of a tile whose 3 x 3 tile neighbourhood (which holds its halo)
-// changed nothing in launch k-1's last sweep changes nothing either: it
-// only brings its output buffer up to date, which holds what launch k-2
-// left (copy the iterate when launch k-1 changed the tile, clear the marks
-// when launch k-2's last sweep left some).
+// The tiles (neighbour index (dr + 1) * 3 + dc + 1, itself = 4) whose read
+// region holds cell (a, b) of a tile.
+__device__ __forceinline__ unsigned reinit_touch(int a, int b) {
+  const unsigned rows = 2u | (a < kReinitHalo ? 1u : 0u) | (a >= kRtR - kReinitHalo ? 4u : 0u);
+  const unsigned cols = 2u | (b < kReinitHalo ? 1u : 0u) | (b >= kRtZ - kReinitHalo ? 4u : 0u);
+  unsigned m = 0u;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+    if ((rows >> r) & 1u) m |= cols << (3 * r);
+  return m;
+}
+
+// Launch k of tile (tr, tc) taken from its list.  A cell's update reads
+// only its neighbours one and two cells away along each axis: where none
+// of them changed in the sweep before, the update gives what it gave then,
+// so the cell keeps its value without recomputing it (the same bits).
+// Changes are compared bit for bit; ch_in / ch_out carry them between
+// launches.  flags_k / flags_n: this launch's and the next one's tile
+// flags; list_n, cnt_n: the next launch's list (base + tile index) when
+// `more`.
 template <int S>
-__device__ __forceinline__ void reinit_tile_sweeps(const double* __restrict__ Din,
-                                                   double* __restrict__ Dout,
-                                                   const double* __restrict__ phi,
-                                                   const uint8_t* __restrict__ frozen,
-                                                   const uint8_t* __restrict__ ch_in,
-                                                   uint8_t* __restrict__ ch_out,
-                                                   uint8_t* __restrict__ tiles, int nr, int nz,
-                                                   double dx, int tr, int tc, int k, int nsw,
-                                                   ReinitTile<S>& L) {
+__device__ __forceinline__ void reinit_tile_step(
+    const double* __restrict__ Din, double* __restrict__ Dout, const double* __restrict__ phi,
+    const uint8_t* __restrict__ frozen, const uint8_t* __restrict__ ch_in,
+    uint8_t* __restrict__ ch_out, unsigned* __restrict__ flags_k, unsigned* __restrict__ flags_n,
+    unsigned* __restrict__ list_n, unsigned* __restrict__ cnt_n, unsigned base, bool more, int nr,
+    int nz, double dx, int tr, int tc, int nsw, ReinitTile<S>& L) {
   using Tl = ReinitTile<S>;
-  static_assert(Tl::H <= kRtR && Tl::H <= kRtZ, "the halo must lie in the neighbour tiles");
+  static_assert(Tl::H == kReinitHalo, "read region");
   const int tid = threadIdx.x, nth = blockDim.x;
   const int ti0 = tr * kRtR, tj0 = tc * kRtZ;
-  const int ntc = reinit_tile_cols(nz), nt = reinit_tile_count(nr, nz);
-  const int ntr = nt / ntc, t = tr * ntc + tc;
-  const int p = k & 1;
-  uint8_t* any_w = tiles + (size_t)p * nt;
-  const uint8_t* any_p = tiles + (size_t)(p ^ 1) * nt;
-  uint8_t* last_w = tiles + (size_t)(2 + p) * nt;
-  const uint8_t* last_p = tiles + (size_t)(2 + (p ^ 1)) * nt;
-  int near = 0;
-  if (tid < 9) {
-    const int r = tr + tid / 3 - 1, q = tc + tid % 3 - 1;
-    if (r >= 0 && r < ntr && q >= 0 && q < ntc) near = last_p[r * ntc + q];
+  const int ntc = reinit_tile_cols(nz), ntr = reinit_tile_count(nr, nz) / ntc;
+  const int t = tr * ntc + tc;
+  __shared__ unsigned s_flag, s_touch;
+  if (tid == 0) {
+    s_flag = flags_k[t];
+    flags_k[t] = 0u;
+    s_touch = 0u;
   }
-  if (!__syncthreads_or(near)) {
-    const bool copy = any_p[t] != 0, clear = last_w[t] != 0;
-    __syncthreads();  // every thread has read last_w[t]
-    if (copy || clear)
-      for (int e = tid; e < kRtR * kRtZ; e += nth) {
-        const int i = ti0 + e / kRtZ, j = tj0 + e % kRtZ;
-        if (i >= nr || j >= nz) continue;
-        const int64_t c = (int64_t)i * nz + j;
-        if (copy) Dout[c] = Din[c];
-        if (clear) ch_out[c] = 0;
-      }
-    if (tid == 0) {
-      any_w[t] = 0;
-      last_w[t] = 0;
+  __syncthreads();
+  if (!(s_flag & 1u)) {  // ran last launch, quiet since: bring the other buffer up to date
+    for (int e = tid; e < kRtR * kRtZ; e += nth) {
+      const int i = ti0 + e / kRtZ, j = tj0 + e % kRtZ;
+      if (i >= nr || j >= nz) continue;
+      const int64_t c = (int64_t)i * nz + j;
+      Dout[c] = Din[c];
+      ch_out[c] = 0;
     }
+    __syncthreads();  // the shared words are read before the next tile sets them
     return;
   }
-  __shared__ int any_change;
-  if (tid == 0) any_change = 0;
   for (int e = tid; e < Tl::NC; e += nth) {
     const int i = ti0 - Tl::H + e / Tl::RZ, j = tj0 - Tl::H + e % Tl::RZ;
     if (i >= 0 && i < nr && j >= 0 && j < nz) {
@@ -308,19 +326,17 @@ __device__ __forceinline__ void reinit_tile_sweeps(const double* __restrict__ Di
         v = (m >= kBig) ? dc : side * m;
       }
       B[li] = v;
-      const bool ch = __double_as_longlong(v) != __double_as_longlong(dc);
-      CB[li] = uint8_t(ch);
-      if (ch && a >= 0 && a < kRtR && b >= 0 && b < kRtZ) any_change = 1;
+      CB[li] = uint8_t(__double_as_longlong(v) != __double_as_longlong(dc));
     }
     __syncthreads();
-    double* t = A;
+    double* tp = A;
     A = B;
-    B = t;
-    uint8_t* tc = CA;
+    B = tp;
+    uint8_t* tcp = CA;
     CA = CB;
-    CB = tc;
+    CB = tcp;
   }
-  int last = 0;
+  unsigned touch = 0u;
   for (int e = tid; e < kRtR * kRtZ; e += nth) {
     const int a = e / kRtZ, b = e % kRtZ;
     const int i = ti0 + a, j = tj0 + b;
@@ -328,42 +344,48 @@ __device__ __forceinline__ void reinit_tile_sweeps(const double* __restrict__ Di
       const int li = (a + Tl::H) * Tl::RZ + (b + Tl::H);
       Dout[(int64_t)i * nz + j] = A[li];
       ch_out[(int64_t)i * nz + j] = CA[li];
-      last |= CA[li];
+      if (CA[li]) touch |= reinit_touch(a, b);
     }
   }
-  last = __syncthreads_or(last);
-  if (tid == 0) {
-    any_w[t] = uint8_t(any_change != 0);
-    last_w[t] = uint8_t(last != 0);
+  if (touch) atomicOr(&s_touch, touch);
+  __syncthreads();
+  if (more && tid < 9) {
+    const int r = tr + tid / 3 - 1, q = tc + tid % 3 - 1;
+    const unsigned bits = ((s_touch >> tid) & 1u) | (tid == 4 ? 2u : 0u);
+    if (bits && r >= 0 && r < ntr && q >= 0 && q < ntc) {
+      const int u = r * ntc + q;
+      if (atomicOr(&flags_n[u], bits) == 0u) list_n[atomicAdd(cnt_n, 1u)] = base + (unsigned)u;
+    }
   }
+  __syncthreads();  // the shared words and L are read before the next tile
 }
 
+// The first list: the tiles with a flag set by the init pass.
+__global__ void reinit_first_list_kernel(const unsigned* __restrict__ flags, int nt,
+                                         unsigned* __restrict__ list, unsigned* __restrict__ cnt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  list_append(t < nt && flags[t] != 0u, t, reinterpret_cast<int*>(list), cnt);
+}
+
+// Launch k over its list: every workgroup takes entries blockIdx.x,
+// blockIdx.x + gridDim.x, ...
 template <int S>
-__global__ void __launch_bounds__(256) reinit_tiles_kernel(const double* __restrict__ Din,
-                                                           double* __restrict__ Dout,
-                                                           const double* __restrict__ phi,
-                                                           const uint8_t* __restrict__ frozen,
-                                                           const uint8_t* __restrict__ ch_in,
-                                                           uint8_t* __restrict__ ch_out,
-                                                           uint8_t* __restrict__ tiles, int nr,
-                                                           int nz, double dx, int k, int nsw) {
+__global__ void __launch_bounds__(256) reinit_list_kernel(
+    const double* __restrict__ Din, double* __restrict__ Dout, const double* __restrict__ phi,
+    const uint8_t* __restrict__ frozen, const uint8_t* __restrict__ ch_in,
+    uint8_t* __restrict__ ch_out, unsigned* __restrict__ flags, unsigned* __restrict__ list,
+    unsigned* __restrict__ cnt, int nr, int nz, double dx, int k, int nsw, int more) {
   __shared__ ReinitTile<S> L;
-  reinit_tile_sweeps<S>(Din, Dout, phi, frozen, ch_in, ch_out, tiles, nr, nz, dx, blockIdx.y,
-                        blockIdx.x, k, nsw, L);
-}
-
-// Sweeps per launch of the tiled form (16 x 40 cells in LDS, 11 KB; the
-// halo costs 1.34x the tile's own updates at S = 2).
-constexpr int kReinitSweepsPerLaunch = 2;
-
-__global__ void reinit_sweep_kernel(const double* __restrict__ Din, double* __restrict__ Dout,
-                                    const double* __restrict__ phi,
-                                    const uint8_t* __restrict__ frozen, int nr, int nz,
-                                    double dx) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  if (j >= nz) return;
-  reinit_sweep_at(Din, Dout, phi, frozen, nr, nz, dx, i, j);
+  const int nt = reinit_tile_count(nr, nz), ntc = reinit_tile_cols(nz);
+  const int p = k & 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt[(k + 2) % 3] = 0u;
+  const int n = (int)cnt[k % 3];
+  for (int idx = blockIdx.x; idx < n; idx += gridDim.x) {
+    const int t = (int)list[(size_t)p * nt + idx];
+    reinit_tile_step<S>(Din, Dout, phi, frozen, ch_in, ch_out, flags + (size_t)p * nt,
+                        flags + (size_t)(p ^ 1) * nt, list + (size_t)(p ^ 1) * nt,
+                        cnt + (k + 1) % 3, 0u, more != 0, nr, nz, dx, t / ntc, t % ntc, nsw, L);
+  }
 }
 
 // Also leaves the tile flags zero for the next reinit.
@@ -371,14 +393,15 @@ __device__ __forceinline__ void reinit_final_at(const double* __restrict__ phi,
                                                 const double* __restrict__ D,
                                                 const double* __restrict__ dist,
                                                 const uint8_t* __restrict__ frozen,
-                                                uint8_t* __restrict__ tiles,
+                                                unsigned* __restrict__ tiles,
                                                 double* __restrict__ out,
                                                 uint8_t* __restrict__ masked, int nr, int nz,
                                                 int i, int j, double narrow) {
-  if (i % kReinitTileRows == 0 && j % kReinitTileCols == 0) {
+  if (i % kRtR == 0 && j % kRtZ == 0) {
     const int nt = reinit_tile_count(nr, nz);
-    const int t = (i / kReinitTileRows) * reinit_tile_cols(nz) + j / kReinitTileCols;
-    for (int r = 0; r < 4; ++r) tiles[(size_t)r * nt + t] = 0;
+    const int t = (i / kRtR) * reinit_tile_cols(nz) + j / kRtZ;
+    tiles[t] = 0u;
+    tiles[(size_t)nt + t] = 0u;
   }
   const int64_t c = (int64_t)i * nz + j;
   const double d = frozen[c] ? dist[c] : D[c];
@@ -398,10 +421,14 @@ __global__ void reinit_final_kernel(const double* __restrict__ phi, const double
 
 // Soft batches: one row segment of 256 cells of one case per workgroup;
 // phi is reinitialised in place.  A case sweeps its own count of
-// iterations: sweeps beyond it exit, and its final pass reads the buffer
-// its own last sweep wrote.
+// iterations: its tiles leave the lists after its last launch, and its
+// final pass reads the buffer its own last launch wrote.  The lists are
+// the batch's, over its reinit tile table (entry = the case's first table
+// index + the tile's index in the case).
 __global__ void soft_batch_reinit_init_kernel(const SoftCase* __restrict__ cs,
-                                              const uint32_t* __restrict__ blocks) {
+                                              const uint32_t* __restrict__ blocks,
+                                              unsigned* __restrict__ cnt) {
+  if (blockIdx.x == 0 && threadIdx.x < 2) cnt[threadIdx.x] = 0u;
   const SoftBlock b = soft_block(blocks);
   const SoftCase& C = cs[b.c];
   if (b.j >= C.nz) return;
@@ -409,22 +436,45 @@ __global__ void soft_batch_reinit_init_kernel(const SoftCase* __restrict__ cs,
                  C.nr, C.nz, C.dx, b.i, b.j);
 }
 
-// kReinitSweepsPerLaunch sweeps (fewer when the case has fewer left) of one
-// 8 x 32 tile of one case (the batch's reinit tile table); launch k reads
-// the first buffer when k is even.
-__global__ void __launch_bounds__(256) soft_batch_reinit_tiles_kernel(
-    const SoftCase* __restrict__ cs, const uint32_t* __restrict__ tiles, int q0) {
+__global__ void soft_batch_reinit_first_list_kernel(const SoftCase* __restrict__ cs,
+                                                    const uint32_t* __restrict__ tiles, int n,
+                                                    unsigned* __restrict__ list,
+                                                    unsigned* __restrict__ cnt) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  bool take = false;
+  if (idx < n) {
+    const uint32_t v = tiles[idx];
+    const SoftCase& C = cs[v & 1023u];
+    const int t = int((v >> 10) & 8191u) * reinit_tile_cols(C.nz) + int(v >> 23);
+    take = C.reinit_iterations > 0 && C.rw.tiles[t] != 0u;
+  }
+  list_append(take, idx, reinterpret_cast<int*>(list), cnt);
+}
+
+__global__ void __launch_bounds__(256) soft_batch_reinit_list_kernel(
+    const SoftCase* __restrict__ cs, const uint32_t* __restrict__ tiles, int ntab,
+    unsigned* __restrict__ list, unsigned* __restrict__ cnt, int k) {
   constexpr int S = kReinitSweepsPerLaunch;
   __shared__ ReinitTile<S> L;
-  const uint32_t v = tiles[blockIdx.x];
-  const SoftCase& C = cs[v & 1023u];
-  const int nsw = std::min(S, C.reinit_iterations - q0);
-  if (nsw <= 0) return;
-  const int k = q0 / S;
-  const bool odd = k & 1;
-  reinit_tile_sweeps<S>(odd ? C.rw.d1 : C.rw.d0, odd ? C.rw.d0 : C.rw.d1, C.phi, C.rw.frozen,
-                        odd ? C.rw.ch1 : C.rw.ch0, odd ? C.rw.ch0 : C.rw.ch1, C.rw.tiles, C.nr,
-                        C.nz, C.dx, int((v >> 10) & 8191u), int(v >> 23), k, nsw, L);
+  const int p = k & 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt[(k + 2) % 3] = 0u;
+  const int n = (int)cnt[k % 3];
+  for (int idx = blockIdx.x; idx < n; idx += gridDim.x) {
+    const unsigned e = list[(size_t)p * ntab + idx];
+    const uint32_t v = tiles[e];
+    const SoftCase& C = cs[v & 1023u];
+    const int nsw = std::min(S, C.reinit_iterations - k * S);
+    if (nsw <= 0) continue;  // block-uniform
+    const int tr = int((v >> 10) & 8191u), tc = int(v >> 23);
+    const int nt = reinit_tile_count(C.nr, C.nz);
+    const bool odd = p;
+    reinit_tile_step<S>(odd ? C.rw.d1 : C.rw.d0, odd ? C.rw.d0 : C.rw.d1, C.phi, C.rw.frozen,
+                        odd ? C.rw.ch1 : C.rw.ch0, odd ? C.rw.ch0 : C.rw.ch1,
+                        C.rw.tiles + (size_t)p * nt, C.rw.tiles + (size_t)(p ^ 1) * nt,
+                        list + (size_t)(p ^ 1) * ntab, cnt + (k + 1) % 3,
+                        (unsigned)C.reinit_tile_base, (k + 1) * S < C.reinit_iterations, C.nr,
+                        C.nz, C.dx, tr, tc, nsw, L);
+  }
 }
 
 __global__ void soft_batch_reinit_final_kernel(const SoftCase* __restrict__ cs,
@@ -439,14 +489,23 @@ __global__ void soft_batch_reinit_final_kernel(const SoftCase* __restrict__ cs,
                   b.j, C.reinit_band);
 }
 
+// Workgroups of a list launch: enough for every entry of a full list up to
+// eight per CU; more entries loop.
+static int reinit_list_grid(int entries) { return std::max(1, std::min(entries, 2048)); }
+
 hipError_t launch_soft_batch_reinit(const SoftCase* cases, const SoftBatchDims& d, int iterations,
                                     hipStream_t s) {
   const int g = d.ncell_blocks;
-  soft_batch_reinit_init_kernel<<<g, 256, 0, s>>>(cases, d.cell_blocks);
+  soft_batch_reinit_init_kernel<<<g, 256, 0, s>>>(cases, d.cell_blocks, d.reinit_cnt);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  for (int q = 0; q < iterations; q += kReinitSweepsPerLaunch) {
-    soft_batch_reinit_tiles_kernel<<<d.nreinit_tiles, 256, 0, s>>>(cases, d.reinit_tiles, q);
+  soft_batch_reinit_first_list_kernel<<<ceil_div(d.nreinit_tiles, 256), 256, 0, s>>>(
+      cases, d.reinit_tiles, d.nreinit_tiles, d.reinit_list, d.reinit_cnt);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const int grid = reinit_list_grid(d.nreinit_tiles);
+  for (int k = 0; k * kReinitSweepsPerLaunch < iterations; ++k) {
+    soft_batch_reinit_list_kernel<<<grid, 256, 0, s>>>(cases, d.reinit_tiles, d.nreinit_tiles,
+                                                       d.reinit_list, d.reinit_cnt, k);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   soft_batch_reinit_final_kernel<<<g, 256, 0, s>>>(cases, d.cell_blocks);
@@ -460,15 +519,19 @@ hipError_t launch_reinit(const double* phi, double* out, uint8_t* masked, int nr
   reinit_init_kernel<<<grid, 256, 0, s>>>(phi, w, nr, nz, dx);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  const int nt = reinit_tile_count(nr, nz);
+  reinit_first_list_kernel<<<ceil_div(nt, 256), 256, 0, s>>>(w.tiles, nt, w.list, w.cnt);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   double* a = w.d0;
   double* b = w.d1;
   constexpr int S = kReinitSweepsPerLaunch;
-  const dim3 tiles(ceil_div(nz, kRtZ), ceil_div(nr, kRtR));
   uint8_t* ca = w.ch0;
   uint8_t* cb = w.ch1;
-  for (int q = 0; q < iterations; q += S) {
-    reinit_tiles_kernel<S><<<tiles, 256, 0, s>>>(a, b, phi, w.frozen, ca, cb, w.tiles, nr, nz, dx,
-                                                 q / S, std::min(S, iterations - q));
+  const int lg = reinit_list_grid(nt);
+  for (int q = 0, k = 0; q < iterations; q += S, ++k) {
+    reinit_list_kernel<S><<<lg, 256, 0, s>>>(a, b, phi, w.frozen, ca, cb, w.tiles, w.list, w.cnt,
+                                             nr, nz, dx, k, std::min(S, iterations - q),
+                                             q + S < iterations ? 1 : 0);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     double* t = a;
     a = b;
@@ -503,10 +566,12 @@ axs_status axs_reinit_distance_f64(const double* phi, double* out, uint8_t* mask
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.frozen), n);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.ch0), n);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.ch1), n);
-  const size_t tb = reinit_tile_flag_bytes(nr, nz);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.tiles), tb);
+  const size_t nt = (size_t)reinit_tile_total(nr, nz);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.tiles), 2 * nt * sizeof(unsigned));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.list), 2 * nt * sizeof(unsigned));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w.cnt), 3 * sizeof(unsigned));
   hipStream_t s = as_stream(stream);
-  if (e == hipSuccess) e = hipMemsetAsync(w.tiles, 0, tb, s);
+  if (e == hipSuccess) e = hipMemsetAsync(w.tiles, 0, 2 * nt * sizeof(unsigned), s);
   if (e == hipSuccess) e = launch_reinit(phi, out, masked, nr, nz, dx, narrow, iterations, w, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (w.d0) (void)hipFree(w.d0);
@@ -516,6 +581,8 @@ axs_status axs_reinit_distance_f64(const double* phi, double* out, uint8_t* mask
   if (w.ch0) (void)hipFree(w.ch0);
   if (w.ch1) (void)hipFree(w.ch1);
   if (w.tiles) (void)hipFree(w.tiles);
+  if (w.list) (void)hipFree(w.list);
+  if (w.cnt) (void)hipFree(w.cnt);
   if (e != hipSuccess) {
     set_error(std::string("reinit: ") + hipGetErrorString(e));
     return AXS_ERR_HIP;

@@ -14,13 +14,15 @@ struct ReinitWork {
   uint8_t* frozen;  // 1 on interface cells
   uint8_t* ch0;     // 1 where the last sweep changed the cell (ping)
   uint8_t* ch1;     // (pong)
-  uint8_t* tiles;   // per 8 x 32 tile: changed in a launch / in its last
-                    // sweep, by launch parity (reinit_tile_flag_bytes, zero
-                    // before the first use; each reinit leaves it zero)
+  // sweep work lists over the 8 x 32 tiles (T = reinit_tile_total):
+  unsigned* tiles;  // 2 T flags, zero before the first use (each reinit
+                    // leaves them zero)
+  unsigned* list;   // 2 T entries
+  unsigned* cnt;    // 3 counters
 };
 
-// Bytes of ReinitWork::tiles for an nr x nz grid.
-size_t reinit_tile_flag_bytes(int nr, int nz);
+// Tiles of an nr x nz grid.
+int reinit_tile_total(int nr, int nz);
 
 // Sweeps the reference count of Jacobi iterations for a band.
 int reinit_iterations(double narrow, double dx);

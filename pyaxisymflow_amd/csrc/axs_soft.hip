@@ -355,9 +355,11 @@ struct SoftStepper {
     if ((e = alloc(&rw.frozen, n)) != hipSuccess) return e;
     if ((e = alloc(&rw.ch0, n)) != hipSuccess) return e;
     if ((e = alloc(&rw.ch1, n)) != hipSuccess) return e;
-    const size_t tb = reinit_tile_flag_bytes(cfg.nr, cfg.nz);
+    const size_t tb = 2 * sizeof(unsigned) * (size_t)reinit_tile_total(cfg.nr, cfg.nz);
     if ((e = alloc(&rw.tiles, tb)) != hipSuccess) return e;
     if ((e = hipMemset(rw.tiles, 0, tb)) != hipSuccess) return e;
+    if ((e = alloc(&rw.list, tb)) != hipSuccess) return e;
+    if ((e = alloc(&rw.cnt, 3 * sizeof(unsigned))) != hipSuccess) return e;
     if ((e = alloc(&e1d, nd * B)) != hipSuccess) return e;
     if ((e = alloc(&e2d, nd * B)) != hipSuccess) return e;
     if ((e = alloc(&cur, nd * sizeof(int16_t))) != hipSuccess) return e;
@@ -655,6 +657,9 @@ struct SoftBatch {
   int64_t chunk_launches = 0, step_launches = 0, captures = 0;
   uint32_t* d_blocks = nullptr;  // cell then doubled-grid block tables
   size_t blocks_cap = 0;
+  unsigned* d_rlist = nullptr;  // reinit work lists (two of the tile table's size)
+  unsigned* d_rcnt = nullptr;   // their three counters
+  size_t rlist_cap = 0;
 
   hipError_t init(int cap) {
     max_cases = cap;
@@ -662,6 +667,8 @@ struct SoftBatch {
     if (e != hipSuccess) return e;
     if ((e = hipMalloc(reinterpret_cast<void**>(&need), sizeof(unsigned))) != hipSuccess) return e;
     if ((e = hipMemset(need, 0, sizeof(unsigned))) != hipSuccess) return e;
+    if ((e = hipMalloc(reinterpret_cast<void**>(&d_rcnt), 3 * sizeof(unsigned))) != hipSuccess)
+      return e;
     if ((e = hipHostMalloc(reinterpret_cast<void**>(&need_host), sizeof(unsigned),
                            hipHostMallocDefault)) != hipSuccess)
       return e;
@@ -688,6 +695,8 @@ struct SoftBatch {
     if (need) (void)hipFree(need);
     if (d_cases) (void)hipFree(d_cases);
     if (d_blocks) (void)hipFree(d_blocks);
+    if (d_rlist) (void)hipFree(d_rlist);
+    if (d_rcnt) (void)hipFree(d_rcnt);
   }
 
   static SoftCase describe(const SoftStepper& S, unsigned* need) {
@@ -798,7 +807,8 @@ struct SoftBatch {
     dims.ndbl_blocks = (int)tab.size() - dims.ncell_blocks;
     const size_t tiles0 = tab.size();
     for (int c = 0; c < dims.ncase; ++c) {
-      const SoftCase& C = h_cases[c];
+      SoftCase& C = h_cases[c];
+      C.reinit_tile_base = (int)(tab.size() - tiles0);
       for (int ti = 0; ti < (int)ceil_div(C.nr, kReinitTileRows); ++ti)
         for (int tj = 0; tj < (int)ceil_div(C.nz, kReinitTileCols); ++tj)
           tab.push_back(soft_block_entry(c, ti, tj));
@@ -820,6 +830,17 @@ struct SoftBatch {
     if ((e = hipMemcpy(d_blocks, tab.data(), sizeof(uint32_t) * tab.size(),
                        hipMemcpyHostToDevice)) != hipSuccess)
       return e;
+    if ((size_t)dims.nreinit_tiles > rlist_cap) {
+      if (d_rlist) (void)hipFree(d_rlist);
+      d_rlist = nullptr;
+      rlist_cap = 0;
+      if ((e = hipMalloc(reinterpret_cast<void**>(&d_rlist),
+                         2 * sizeof(unsigned) * (size_t)dims.nreinit_tiles)) != hipSuccess)
+        return e;
+      rlist_cap = (size_t)dims.nreinit_tiles;
+    }
+    dims.reinit_list = d_rlist;
+    dims.reinit_cnt = d_rcnt;
     dims.cell_blocks = d_blocks;
     dims.dbl_blocks = d_blocks + dims.ncell_blocks;
     dims.reinit_tiles = d_blocks + tiles0;

@@ -44,6 +44,7 @@ struct SoftCase {
   int16_t *cur, *tgt;
   double *s11, *s12, *s22, *g1z, *g1r, *g2z, *g2r, *tz, *tr;
   ReinitWork rw;
+  int reinit_tile_base;  // the case's first entry in the batch's reinit tile table
   LsqWork lw;
   unsigned* need;
   // row transforms (any length) and the radial solve's tables
@@ -72,6 +73,8 @@ struct SoftBatchDims {
   int ndbl_blocks = 0;
   const uint32_t* reinit_tiles = nullptr;  // reinit tiles of every case's grid
   int nreinit_tiles = 0;
+  unsigned* reinit_list = nullptr;  // two work lists of nreinit_tiles entries
+  unsigned* reinit_cnt = nullptr;   // their three counters
 };
 
 // Tile of the level-set reinit's tiled sweeps (rows x columns).
