@@ -29,6 +29,9 @@ struct AnyDftArgs {
   const Cplx<double>* tw = nullptr;     // e^{-2 pi i t / L}, t < L
   const Cplx<double>* chirp = nullptr;  // w_n, n < N (Bluestein)
   const Cplx<double>* bhat = nullptr;   // FFT_L(b) / L (Bluestein)
+  // 1: the FFT reads tw from a copy in LDS after the L work entries
+  // (any_dft_stage), when both fit the LDS a workgroup may take
+  int stage = 0;
 };
 
 // Host owner of one length's device tables.
@@ -39,7 +42,7 @@ struct AnyDft {
   std::vector<void*> owned;  // device tables
   hipError_t init(int N);
   template <typename T>
-  size_t lds_bytes() const { return sizeof(Cplx<T>) * (size_t)args.L; }
+  size_t lds_bytes() const { return sizeof(Cplx<T>) * (size_t)args.L * (args.stage ? 2 : 1); }
   ~AnyDft();
 };
 
@@ -49,14 +52,32 @@ constexpr int kAnyMaxL = 8192;
 // FFT length the chirp-z form needs for N (N itself when N is 2,3,5-smooth).
 int any_dft_length(int N, bool* bluestein);
 
+// Copies the FFT twiddles into LDS after the work entries when the plan
+// stages them; call before the barrier that precedes any_dft.
+template <typename T>
+__device__ __forceinline__ void any_dft_stage(Cplx<T>* work, const AnyDftArgs& a, int tid,
+                                              int nth) {
+  if (a.stage) stage_twiddles(work + a.L, a.tw, a.L, tid, nth);
+}
+
+template <typename T, int PER>
+__device__ __forceinline__ void any_fft(Cplx<T>* work, const AnyDftArgs& a, int sgn, int tid,
+                                        int nth) {
+  if (a.stage)
+    fft_lds<T, PER>(work, a.fp, static_cast<const Cplx<T>*>(work + a.L), sgn, tid, nth);
+  else
+    fft_lds<T, PER>(work, a.fp, a.tw, sgn, tid, nth);
+}
+
 // In-place DFT of work[0..N) (exponent sign sgn), result in work[0..N),
 // by threads tid of nth (all of them call it: there are block barriers).
-// work holds L entries; the caller has written work[0..N) and issued a
-// barrier.  Ends with a barrier.
+// work holds L entries (then the staged twiddles); the caller has written
+// work[0..N), called any_dft_stage and issued a barrier.  Ends with a
+// barrier.
 template <typename T, int PER>
 __device__ void any_dft(Cplx<T>* work, const AnyDftArgs& a, int sgn, int tid, int nth) {
   if (!a.bluestein) {
-    fft_lds<T, PER>(work, a.fp, a.tw, sgn, tid, nth);
+    any_fft<T, PER>(work, a, sgn, tid, nth);
     return;
   }
   const int N = a.N, L = a.L;
@@ -69,7 +90,7 @@ __device__ void any_dft(Cplx<T>* work, const AnyDftArgs& a, int sgn, int tid, in
     }
   }
   __syncthreads();
-  fft_lds<T, PER>(work, a.fp, a.tw, -1, tid, nth);
+  any_fft<T, PER>(work, a, -1, tid, nth);
   for (int k = tid; k < L; k += nth) {
     Cplx<double> b;
     if (sgn < 0) {
@@ -81,7 +102,7 @@ __device__ void any_dft(Cplx<T>* work, const AnyDftArgs& a, int sgn, int tid, in
     work[k] = cmul(work[k], Cplx<T>{T(b.x), T(b.y)});
   }
   __syncthreads();
-  fft_lds<T, PER>(work, a.fp, a.tw, +1, tid, nth);
+  any_fft<T, PER>(work, a, +1, tid, nth);
   for (int k = tid; k < N; k += nth) {
     const Cplx<double> w = a.chirp[k];
     work[k] = cmul(work[k], Cplx<T>{T(w.x), T(sgn < 0 ? w.y : -w.y)});

@@ -79,8 +79,9 @@ __global__ void __launch_bounds__(1024) any_bhat_kernel(Cplx<double>* __restrict
   extern __shared__ unsigned char smem_raw[];
   Cplx<double>* work = reinterpret_cast<Cplx<double>*>(smem_raw);
   for (int k = threadIdx.x; k < a.L; k += blockDim.x) work[k] = b[k];
+  any_dft_stage(work, a, threadIdx.x, blockDim.x);
   __syncthreads();
-  fft_lds<double, PER>(work, a.fp, a.tw, -1, threadIdx.x, blockDim.x);
+  any_fft<double, PER>(work, a, -1, threadIdx.x, blockDim.x);
   const double inv = 1.0 / (double)a.L;
   for (int k = threadIdx.x; k < a.L; k += blockDim.x)
     b[k] = Cplx<double>{work[k].x * inv, work[k].y * inv};
@@ -99,6 +100,7 @@ hipError_t AnyDft::init(int N) {
   args.N = N;
   args.L = L;
   args.bluestein = blue ? 1 : 0;
+  args.stage = 2 * sizeof(Cplx<double>) * (size_t)L <= (size_t)g_any_lds_limit ? 1 : 0;
   radices_of(L, &args.fp);
   nth = std::min(1024, std::max(64, ((L / 2 + 63) / 64) * 64));
   per = (L / 2 + nth - 1) / nth;
@@ -187,6 +189,7 @@ __device__ void any_dct2_row(const T* __restrict__ src, T* __restrict__ tapered,
     rowf = (i == 0) ? T(0) : ((i >= nr - w) ? T(wr[i - (nr - w)]) : T(1));
   }
   T* out_t = (TAPER && tapered) ? tapered + (int64_t)i * nz : nullptr;
+  any_dft_stage(work, tb.a, tid, nth);
   for (int j = tid; j < nz; j += nth) {
     T t = row[j];
     if constexpr (TAPER) {
@@ -227,6 +230,7 @@ __device__ void any_dct3_row(const T* __restrict__ src, T* __restrict__ dst, int
   const T* row = src + (int64_t)i * nz;
   const T invN = T(1) / T(nz);
   T* out = dst + (int64_t)i * nz;
+  any_dft_stage(work, tb.a, tid, nth);
   if (tb.half) {
     const int M = nz / 2;
     for (int k = tid; k < M; k += nth) {
@@ -321,6 +325,7 @@ __global__ void __launch_bounds__(1024) any_rfft_rows_kernel(
   const int i = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
   const double* row = src + (int64_t)i * nz;
   const double sc = rscale ? rscale[i] : 1.0;
+  any_dft_stage(work, a, tid, nth);
   for (int j = tid; j < nz; j += nth) work[j] = Cplx<double>{sc * row[j], 0.0};
   __syncthreads();
   any_dft<double, PER>(work, a, -1, tid, nth);
@@ -345,6 +350,7 @@ __global__ void __launch_bounds__(1024) any_irfft_rows_kernel(const double* __re
   Cplx<double>* work = reinterpret_cast<Cplx<double>*>(smem_raw);
   const int i = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
   const double* row = src + (int64_t)i * nz;
+  any_dft_stage(work, a, tid, nth);
   for (int k = tid; k < nz; k += nth) {
     Cplx<double> v;
     if (k == 0) {
@@ -382,6 +388,7 @@ __global__ void __launch_bounds__(1024) any_dst1_rows_kernel(
   const int i = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
   const int M = nz + 1;
   const double* row = src + (int64_t)i * nz;
+  any_dft_stage(work, a, tid, nth);
   for (int j = tid; j < nz; j += nth) {
     const double x = row[j];
     packed_y(work, j + 1) = x;

@@ -414,7 +414,7 @@ hipError_t launch_soft_batch_lsq_list(const SoftCase* cases, const SoftBatchDims
 
 hipError_t launch_soft_batch_lsq_sweep(const SoftCase* cases, const SoftBatchDims& d, int sweep,
                                        hipStream_t s) {
-  constexpr int kBlocksPerCase = 32;  // a case's band: a few thousand cells
+  constexpr int kBlocksPerCase = 64;  // a case's band: up to ~16k cells, one per thread
   soft_batch_lsq_sweep_kernel<<<dim3(kBlocksPerCase, d.ncase), 256, 0, s>>>(cases, sweep);
   return hipGetLastError();
 }

@@ -566,7 +566,7 @@ def _soft_case_state(obj, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_
     obj.t_wz, obj.t_wr = t(wz), t(wr)
     cfg = _lib.SoftConfig()
     cfg.nr, cfg.nz, cfg.kill_width = nr, nz, kill_width
-    cfg.lsq_sweeps, cfg.lsq_grow = 24, 1
+    cfg.lsq_sweeps, cfg.lsq_grow = 12, 1
     cfg.reinit_iterations = -1
     cfg.dx, cfg.nu, cfg.G, cfg.rho_f, cfg.CFL = dx, nu, G, rho_f, 0.1
     cfg.eps = np.finfo(float).eps
@@ -592,8 +592,8 @@ class SoftSphereStepper:
     reinit_band = moll_zone + 4 dx, sphere of radius r_ball at (0.5, 0),
     vorticity C (kz^2 + kr^2) sin(kz z) sin(kr r) with kz = kr = 2 pi.
     ``lsq_sweeps`` fixes the number of extrapolation sweeps queued per step
-    as separate launches; by default the queue starts at 24 and grows with
-    the most sweeps a step has needed (+ 8, rounded up to 8) as graph runs
+    as separate launches; by default the queue starts at 12 and grows with
+    the most sweeps a step has needed (+ 2, rounded up to 4) as graph runs
     learn it. The reference loops until a sweep fits nothing, which takes
     8-21 sweeps for stiff solids and up to ~60 for the softest as they
     deform; a loop longer than the queue is finished by one workgroup, so
