@@ -494,6 +494,47 @@ axs_status axs_slab_stepper_create(const axs_slab_config* cfg, const axs_slab_bu
 axs_status axs_slab_stepper_destroy(axs_slab_stepper* st);
 axs_status axs_slab_stepper_run(axs_slab_stepper* st, int nsteps, int use_graph, void* stream);
 
+/* ---- Device-resident step of examples/PeriodicSoftSlab/periodic_soft_slab.py:214-396:
+ * an elastic layer (solid_phi > 0, blended by solid_char_func) driven through
+ * the fluid by an annular wall (wall_char_func) oscillating along z at
+ * V_wall cos(omega t), z-periodic as the rigid slab (ghost columns, the
+ * periodic Stokes solve on the interior columns).  Step order: dt = min(CFL
+ * dx / sqrt(G / rho), 0.9 dx^2 / 4 / nu, CFL dx / (max|u_z| + |u_r| + eps)),
+ * clipped to the sample timer (period freq_limit, starting at its limit, so
+ * the first dt is 0) and to t_end -> kill_boundary_vorticity_sine_r -> psi
+ * -> compute_velocity_from_psi_periodic (a sample, when the timer has run
+ * out: t and u_z of rows < sample_rows in column nz / 2) ->
+ * advect_refmap_via_eno3_periodic -> brinkmann_penalize towards the wall ->
+ * compute_vorticity_from_velocity_periodic of the correction, added to omega
+ * -> eta *= (solid_char_func > 0.5) -> the static PDE extrapolation of eta1
+ * and eta2 (bounded_static_PDE_extrapolation.py:58-123, Jacobi to
+ * extrap_tol, at most max_extrap_iterations per loop) ->
+ * solid_sigma_periodic, blended -> update_vorticity_from_solid_stress_periodic
+ * -> advect_vorticity_via_eno3_periodic -> diffusion_RK2_periodic.  FP64.
+ * Buffers: (nr, nz) vorticity, psi, u_z, u_r, eta1, eta2, solid_phi (its
+ * ghost columns are exchanged once), solid_char_func, wall_char_func; r (nr);
+ * kill_wr as for the rigid slab; scalars AXS_RIGID_SCALARS doubles (t, dt,
+ * dt/dx, nu dt/2, nu dt, V_wall cos(omega t), max|u|, step counter, sample
+ * timer, samples taken, Jacobi iterations of the last step, loops that hit
+ * the iteration bound); samples: sample_cap rows of 1 + sample_rows doubles
+ * (t, then the profile). */
+typedef struct axs_soft_slab_stepper axs_soft_slab_stepper;
+typedef struct {
+  int nr, nz, ghost, kill_width, sample_rows, sample_cap, max_extrap_iterations;
+  double dx, nu, G, rho, lam, V_wall, omega, CFL, eps, z_max, extrap_tol, extrap_band,
+      freq_limit, t_end;
+} axs_soft_slab_config;
+typedef struct {
+  double *vorticity, *psi, *u_z, *u_r, *eta1, *eta2, *solid_phi, *scalars, *samples;
+  const double *solid_char_func, *wall_char_func, *r, *kill_wr;
+} axs_soft_slab_buffers;
+axs_status axs_soft_slab_stepper_create(const axs_soft_slab_config* cfg,
+                                        const axs_soft_slab_buffers* bufs,
+                                        axs_soft_slab_stepper** out);
+axs_status axs_soft_slab_stepper_destroy(axs_soft_slab_stepper* st);
+axs_status axs_soft_slab_stepper_run(axs_soft_slab_stepper* st, int nsteps, int use_graph,
+                                     void* stream);
+
 /* ---- Device-resident step of examples/TorusStreaming/torus_streaming.py:129-233
  * with implicit diffusion (the driver's setting): a torus (tube radius a,
  * cross-section aspect AR, centre (Z_cm, R_cm(t)), R_cm(t) = R_cm + e a

@@ -12,6 +12,10 @@ oracle kernels.  TEST ORACLE ONLY.
   examples/PeriodicRigidSlab/periodic_rigid_slab.py:33-158 (z-periodic,
   oscillating annular wall), and ``womersley_wall_driven`` restates its
   closed-form theory (theory_rigid_slab.py:9-23).
+* ``PeriodicSoftSlabCase`` follows
+  examples/PeriodicSoftSlab/periodic_soft_slab.py:43-404 (an elastic layer
+  around the axis, the static PDE extrapolation of oracle.pde_extrapolation),
+  and ``soft_slab_theory`` restates theory_soft_slab.py:15-87.
 * Synthetic inputs (seeded Gaussian vorticity, sphere masks) follow the
   workload definitions used by the tests and the benchmark.
 """
@@ -682,4 +686,186 @@ class TorusStreamingCase:
             self.vorticity[...] = exact_implicit_diffusion(self.vorticity, self.nu * dt, dx)
         self.t += dt
         self.it += 1
+        return dt
+
+
+def soft_slab_theory(L_f, L_s, shear_rate, omega, G, V_wall, rho_f, nu_f, rho_ratio=1.0,
+                     nu_ratio=1.0, resolution=None):
+    """theory_soft_slab.py:15-87: the oscillatory axial velocity of a
+    Kelvin-Voigt solid (r < L_s, shear modulus G, viscosity mu_s) inside a
+    fluid layer (L_s < r < L_s + L_f) whose outer wall moves at
+    V_wall cos(omega t).  Fluid velocity 2 Re((A J0(lam1 y) + B Y0(lam1 y))
+    e^{-i omega t}), solid displacement Re-part of 2 C J0(lam2 y) e^{-i omega t}
+    (velocity -i omega times it), lam1 = sqrt(i omega / nu_f), lam2 = omega /
+    sqrt(-i omega nu_s + G / rho_s); A, B, C from the wall velocity, the
+    velocity and the shear stress matched at y = L_s.  The reference solves
+    the 3 x 3 system symbolically; here numerically (same equations).
+    Returns (velocity(t) at the positions Y, Y, (fluid(y, t), solid(y, t),
+    displacement(y, t))) with y = 1e-20 .. L_s + L_f (30 points) unless
+    ``resolution`` gives them."""
+    from scipy.special import jv, yv
+
+    rho_s = rho_ratio * rho_f
+    nu_s = nu_ratio * nu_f
+    mu_f = nu_f * rho_f
+    mu_s = nu_s * rho_s
+    lam1 = np.sqrt(1j * omega / nu_f)
+    lam2 = omega / np.sqrt(-1j * omega * nu_s + G / rho_s)
+    L = L_s + L_f
+    k1, k2 = jv(0, lam1 * L), yv(0, lam1 * L)
+    k3, k4 = jv(0, lam1 * L_s), yv(0, lam1 * L_s)
+    k5 = -1j * omega * jv(0, lam2 * L_s)
+    k6, k7 = jv(1, lam1 * L_s), yv(1, lam1 * L_s)
+    k8 = lam2 * jv(1, lam2 * L_s) / (mu_f * lam1) * (G - mu_s * 1j * omega)
+    M = np.array([[k1, k2, 0.0], [k3, k4, -k5], [k6, k7, -k8]], dtype=complex)
+    A, B, C = np.linalg.solve(M, np.array([V_wall / 2, 0.0, 0.0], dtype=complex))
+
+    def fluid(y, t):
+        return 2 * np.real((A * jv(0, lam1 * y) + B * yv(0, lam1 * y)) * np.exp(-1j * omega * t))
+
+    def displacement(y, t):
+        return 2 * np.real(C * jv(0, lam2 * y) * np.exp(-1j * omega * t))
+
+    def solid(y, t):
+        return 2 * np.real(-1j * omega * C * jv(0, lam2 * y) * np.exp(-1j * omega * t))
+
+    Y = np.linspace(1e-20, L, 30) if resolution is None else np.array(resolution, dtype=float)
+
+    def temporal(time):
+        return (Y < L_s) * solid(Y, time) + (Y >= L_s) * fluid(Y, time)
+
+    return temporal, Y, (fluid, solid, displacement)
+
+
+class PeriodicSoftSlabCase:
+    """examples/PeriodicSoftSlab/periodic_soft_slab.py:43-404: an elastic
+    layer r < L_s (eta extrapolated by the static PDE scheme) inside fluid,
+    driven by an annular wall at R_wall_center oscillating at
+    V_wall cos(omega t), z-periodic with ghost_size ghost columns on each
+    side; Re and Er fix nu_f and G (:80-102).  ``solver`` "exact" or "eig"
+    for the interior Stokes solve.  ``samples`` collects (t, u_z at column
+    nz / 2 for r < R_tube) where the driver compares with the theory
+    (:229-238)."""
+
+    def __init__(self, grid_size_r=256, Re=10.0, Er=0.25, domain_AR=32, zeta=1.0,
+                 solver="exact", t_end=None):
+        from oracle import periodic as P
+        from oracle.pde_extrapolation import StaticPDEExtrapolation
+        from oracle.stokes import NEUMANN_R_PERIODIC_Z
+
+        self.P = P
+        self.bc = NEUMANN_R_PERIODIC_Z
+        max_r = 0.5
+        max_z = max_r / domain_AR
+        nz = int(grid_size_r / domain_AR)
+        nr = grid_size_r
+        dx = max_r / grid_size_r
+        self.nr, self.nz, self.dx, self.max_z = nr, nz, dx, max_z
+        self.z = np.linspace(0 + dx / 2, max_z - dx / 2, nz)
+        self.r = np.linspace(0 + dx / 2, max_r - dx / 2, nr)
+        self.Z, self.R = np.meshgrid(self.z, self.r)
+        g = self.g = 2
+        self.comm = P.gen_periodic_boundary_ghost_comm(g)
+        self.comm_eta = P.gen_periodic_boundary_ghost_comm_eta(g, max_z, dx)
+        self.CFL = 0.1
+        self.eps = np.finfo(float).eps
+        self.brink_lam = 1e4
+        self.moll_zone = np.sqrt(2) * dx
+        self.extrap_zone = self.moll_zone + 3 * dx
+        self.extrap_tol = 1e-3
+        wall_half = 0.05
+        self.R_wall_center = 0.5 * max_r
+        self.R_tube = self.R_wall_center - wall_half
+        L = 2 * self.R_tube
+        self.L_f = self.R_tube * zeta / (1 + zeta)
+        self.L_s = self.R_tube / (1 + zeta)
+        self.freq = 1.0
+        self.freq_limit = 0.1 / self.freq
+        self.omega = 2 * np.pi * self.freq
+        self.freq_timer = self.freq_limit
+        self.V_wall = 1.0
+        self.rho_f = 1.0
+        self.shear_rate = 2 * self.V_wall / self.omega / L
+        self.nu = self.shear_rate * self.omega * self.L_f**2 / Re
+        self.G = self.rho_f * self.nu * self.shear_rate * self.omega / Er
+        self.t_end = 15 / self.freq if t_end is None else t_end
+        self.solid_phi = self.L_s - self.R
+        self.solid_char_func = 0 * self.Z
+        K.smooth_Heaviside(self.solid_char_func, self.solid_phi, self.moll_zone)
+        self.inside_solid = self.solid_char_func > 0.5
+        wall_phi = -np.sqrt((self.R - self.R_wall_center) ** 2) + wall_half
+        self.wall_char_func = 0 * self.Z
+        K.smooth_Heaviside(self.wall_char_func, wall_phi, self.moll_zone)
+        z0 = 0 * self.Z
+        self.vorticity, self.penal_vorticity, self.temp_vorticity = z0.copy(), z0.copy(), z0.copy()
+        self.psi, self.u_z, self.u_r = z0.copy(), z0.copy(), z0.copy()
+        self.u_z_upen, self.u_r_upen = z0.copy(), z0.copy()
+        self.eta1, self.eta2 = self.Z.copy(), self.R.copy()
+        self.s11, self.s12, self.s22 = z0.copy(), z0.copy(), z0.copy()
+        self.e1z, self.e2z, self.e1r, self.e2r = z0.copy(), z0.copy(), z0.copy(), z0.copy()
+        self.tau_z, self.tau_r = z0.copy(), z0.copy()
+        self.psi_inner = self.psi[..., g:-g].copy()
+        self.solver_kind = solver
+        if solver == "eig":
+            self.solver = FastDiagonalisationStokesSolver(nr, nz - 2 * g, dx, bc_type=self.bc)
+        self.extrapolator = StaticPDEExtrapolation(
+            dx=dx, grid_size_r=nr, grid_size_z=nz, extrap_tol=self.extrap_tol,
+            extrap_band=self.extrap_zone, periodic=True, per_communicator_gen=self.comm,
+            per_communicator_eta=self.comm_eta)
+        self.sample_rows = int(nr * self.R_tube / max_r)
+        self.samples = []
+        self.t = 0.0
+        self.it = 0
+        self.dts = []
+
+    def step(self):
+        """:214-396, one iteration."""
+        P, g, dx = self.P, self.g, self.dx
+        dt = min(self.CFL * dx / np.sqrt(self.G / self.rho_f), 0.9 * dx**2 / 4 / self.nu,
+                 self.CFL * dx / (np.amax(np.fabs(self.u_z) + np.fabs(self.u_r)) + self.eps))
+        if self.freq_timer + dt > self.freq_limit:
+            dt = self.freq_limit - self.freq_timer
+        if self.t + dt > self.t_end:
+            dt = self.t_end - self.t
+        K.kill_boundary_vorticity_sine_r(self.vorticity, self.R, 3, dx)
+        self.psi_inner[...] = self.psi[..., g:-g]
+        if self.solver_kind == "eig":
+            self.solver.solve(solution_field=self.psi_inner, rhs_field=self.vorticity[:, g:-g])
+        else:
+            self.psi_inner[...] = exact_stokes_solve(self.vorticity[:, g:-g], dx, self.bc)
+        self.psi[..., g:-g] = self.psi_inner
+        P.compute_velocity_from_psi_periodic(self.u_z, self.u_r, self.psi, self.R, dx, self.comm)
+        if self.freq_timer >= self.freq_limit:
+            self.freq_timer = 0.0
+            self.samples.append((self.t, self.u_z[:self.sample_rows, self.nz // 2].copy()))
+        P.advect_refmap_via_eno3_periodic(self.eta1, self.eta2, self.u_z, self.u_r, dt, dx,
+                                          self.comm, self.comm_eta)
+        self.u_z_upen[...] = self.u_z
+        self.u_r_upen[...] = self.u_r
+        K.brinkmann_penalize(self.brink_lam, dt, self.wall_char_func,
+                             self.V_wall * np.cos(self.omega * self.t), 0.0, self.u_z_upen,
+                             self.u_r_upen, self.u_z, self.u_r)
+        P.compute_vorticity_from_velocity_periodic(self.penal_vorticity, self.u_z - self.u_z_upen,
+                                                   self.u_r - self.u_r_upen, dx, self.comm)
+        self.vorticity[...] += self.penal_vorticity
+        self.eta1[...] = self.inside_solid * self.eta1
+        self.eta2[...] = self.inside_solid * self.eta2
+        self.extrapolator.extrapolate(self.eta1, self.solid_phi)
+        self.extrapolator.extrapolate(self.eta2, self.solid_phi)
+        P.solid_sigma_periodic(self.s11, self.s12, self.s22, self.G, dx, self.eta1, self.eta2,
+                               self.e1z, self.e1r, self.e2z, self.e2r, self.comm, self.comm_eta)
+        self.s11[...] = self.solid_char_func * self.s11
+        self.s12[...] = self.solid_char_func * self.s12
+        self.s22[...] = self.solid_char_func * self.s22
+        P.update_vorticity_from_solid_stress_periodic(self.vorticity, self.tau_z, self.tau_r,
+                                                      self.s11, self.s12, self.s22, self.R, dt,
+                                                      dx, self.comm)
+        P.advect_vorticity_via_eno3_periodic(self.vorticity, self.u_z, self.u_r, dt, dx,
+                                             self.comm)
+        P.diffusion_RK2_periodic(self.vorticity, self.temp_vorticity, self.R, self.nu, dt, dx,
+                                 self.comm)
+        self.t += dt
+        self.freq_timer += dt
+        self.it += 1
+        self.dts.append(dt)
         return dt

@@ -84,6 +84,24 @@ class SlabBuffers(ctypes.Structure):
         "vorticity", "psi", "u_z", "u_r", "scalars", "char_func", "r", "kill_wr")]
 
 
+class SoftSlabConfig(ctypes.Structure):
+    """axs_soft_slab_config (include/axisym.h)."""
+
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "nr", "nz", "ghost", "kill_width", "sample_rows", "sample_cap",
+        "max_extrap_iterations")] + [(n, ctypes.c_double) for n in (
+            "dx", "nu", "G", "rho", "lam", "V_wall", "omega", "CFL", "eps", "z_max", "extrap_tol",
+            "extrap_band", "freq_limit", "t_end")]
+
+
+class SoftSlabBuffers(ctypes.Structure):
+    """axs_soft_slab_buffers (include/axisym.h)."""
+
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "vorticity", "psi", "u_z", "u_r", "eta1", "eta2", "solid_phi", "scalars", "samples",
+        "solid_char_func", "wall_char_func", "r", "kill_wr")]
+
+
 class TorusConfig(ctypes.Structure):
     """axs_torus_config (include/axisym.h)."""
 
@@ -164,6 +182,10 @@ SIGNATURES = {
                                 ctypes.POINTER(_P)],
     "axs_slab_stepper_destroy": [_P],
     "axs_slab_stepper_run": [_P, _I, _I, _P],
+    "axs_soft_slab_stepper_create": [ctypes.POINTER(SoftSlabConfig),
+                                     ctypes.POINTER(SoftSlabBuffers), ctypes.POINTER(_P)],
+    "axs_soft_slab_stepper_destroy": [_P],
+    "axs_soft_slab_stepper_run": [_P, _I, _I, _P],
     "axs_torus_stepper_create": [ctypes.POINTER(TorusConfig), ctypes.POINTER(TorusBuffers),
                                  ctypes.POINTER(_P)],
     "axs_torus_stepper_destroy": [_P],

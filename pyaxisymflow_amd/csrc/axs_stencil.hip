@@ -477,15 +477,27 @@ hipError_t launch_solid_sigma(T* s11, T* s12, T* s22, T G, T dx, const T* e1, co
 }
 
 template <typename T>
-hipError_t launch_div_tau(T* w, T* tz, T* tr, const T* t11, const T* t12, const T* t22,
-                          const double* r, int nr, int nz, Scalar<T> dt, T dx, hipStream_t s) {
+hipError_t launch_div_tau_stage1(T* tz, T* tr, const T* t11, const T* t12, const T* t22,
+                                 const double* r, int nr, int nz, T dx, hipStream_t s) {
   div_tau_stage1_kernel<T, RRow<T>><<<dim3(ceil_div(nz - 2, 256), nr - 1), 256, 0, s>>>(
       tz, tr, t11, t12, t22, RRow<T>{r}, nr, nz, T(2) * dx);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_div_tau_stage2(T* w, const T* tz, const T* tr, int nr, int nz, Scalar<T> dt,
+                                 T dx, hipStream_t s) {
   div_tau_stage2_kernel<T><<<dim3(ceil_div(nz - 2, 256), nr - 2), 256, 0, s>>>(w, tz, tr, nr, nz,
                                                                               dt, T(2) * dx);
   return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_div_tau(T* w, T* tz, T* tr, const T* t11, const T* t12, const T* t22,
+                          const double* r, int nr, int nz, Scalar<T> dt, T dx, hipStream_t s) {
+  hipError_t e = launch_div_tau_stage1<T>(tz, tr, t11, t12, t22, r, nr, nz, dx, s);
+  if (e != hipSuccess) return e;
+  return launch_div_tau_stage2<T>(w, tz, tr, nr, nz, dt, dx, s);
 }
 
 template hipError_t launch_velocity_from_psi<double>(double*, double*, const double*,
@@ -500,6 +512,11 @@ template hipError_t launch_solid_sigma<double>(double*, double*, double*, double
 template hipError_t launch_div_tau<double>(double*, double*, double*, const double*,
                                            const double*, const double*, const double*, int, int,
                                            Scalar<double>, double, hipStream_t);
+template hipError_t launch_div_tau_stage1<double>(double*, double*, const double*, const double*,
+                                                  const double*, const double*, int, int, double,
+                                                  hipStream_t);
+template hipError_t launch_div_tau_stage2<double>(double*, const double*, const double*, int, int,
+                                                  Scalar<double>, double, hipStream_t);
 
 }  // namespace axs
 

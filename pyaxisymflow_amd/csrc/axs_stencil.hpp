@@ -51,5 +51,13 @@ hipError_t launch_solid_sigma(T* s11, T* s12, T* s22, T G, T dx, const T* e1, co
 template <typename T>
 hipError_t launch_div_tau(T* w, T* tz, T* tr, const T* t11, const T* t12, const T* t22,
                           const double* r, int nr, int nz, Scalar<T> dt, T dx, hipStream_t s);
+// Its two stages apart (the periodic form, div_tau.py:37-75, exchanges the
+// ghost columns of tau between them).
+template <typename T>
+hipError_t launch_div_tau_stage1(T* tz, T* tr, const T* t11, const T* t12, const T* t22,
+                                 const double* r, int nr, int nz, T dx, hipStream_t s);
+template <typename T>
+hipError_t launch_div_tau_stage2(T* w, const T* tz, const T* tr, int nr, int nz, Scalar<T> dt,
+                                 T dx, hipStream_t s);
 
 }  // namespace axs

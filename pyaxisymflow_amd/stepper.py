@@ -915,6 +915,165 @@ class PeriodicRigidSlabStepper:
             self._h = None
 
 
+class PeriodicSoftSlabStepper:
+    """examples/PeriodicSoftSlab/periodic_soft_slab.py:43-404 on the GPU: an
+    elastic layer r < L_s inside fluid, driven by an annular wall at
+    R_wall_center oscillating at V_wall cos(omega t), z-periodic with two
+    ghost columns on each side, every stage of the loop body (:214-396)
+    device resident and replayed from hipGraphs (axs_soft_slab_stepper_*).
+    Parameters and derived quantities follow the driver (:55-137): max_r =
+    0.5, max_z = max_r / domain_AR, nz = grid_size_r / domain_AR, dx = max_r /
+    grid_size_r, R_tube = 0.2, L_f = R_tube zeta / (1 + zeta), L_s = R_tube /
+    (1 + zeta), shear_rate = 2 V_wall / (omega L), nu = shear_rate omega L_f^2
+    / Re, G = rho nu shear_rate omega / Er, freq = 1, t_end = 15 / freq.
+    ``samples()`` returns what the driver compares with the theory
+    (:229-238): the times and u_z of rows r < R_tube in column nz / 2, taken
+    every 0.1 / freq."""
+
+    def __init__(self, grid_size_r=256, Re=10.0, Er=0.25, domain_AR=32, zeta=1.0, t_end=None,
+                 use_graph=True, kill_width=3, max_extrap_iterations=100000):
+        torch = _lib.torch()
+        dev = _lib.device()
+        max_r = 0.5
+        max_z = max_r / domain_AR
+        nz, nr = int(grid_size_r / domain_AR), int(grid_size_r)
+        dx = max_r / grid_size_r
+        self.nz, self.nr, self.dx, self.max_z = nz, nr, dx, max_z
+        self.use_graph = use_graph
+        self.stream = torch.cuda.Stream(device=dev)
+        self.z = np.linspace(0 + dx / 2, max_z - dx / 2, nz)
+        self.r = np.linspace(0 + dx / 2, max_r - dx / 2, nr)
+        Z, R = np.meshgrid(self.z, self.r)
+        moll_zone = np.sqrt(2) * dx
+        wall_half = 0.05
+        self.R_wall_center = 0.5 * max_r
+        self.R_tube = self.R_wall_center - wall_half
+        L = 2 * self.R_tube
+        self.L_f = self.R_tube * zeta / (1 + zeta)
+        self.L_s = self.R_tube / (1 + zeta)
+        self.freq = 1.0
+        self.freq_limit = 0.1 / self.freq
+        self.omega = 2 * np.pi * self.freq
+        self.V_wall, self.rho_f = 1.0, 1.0
+        self.shear_rate = 2 * self.V_wall / self.omega / L
+        self.nu = self.shear_rate * self.omega * self.L_f**2 / Re
+        self.G = self.rho_f * self.nu * self.shear_rate * self.omega / Er
+        self.t_end = 15 / self.freq if t_end is None else float(t_end)
+        self.sample_rows = int(nr * self.R_tube / max_r)
+        self.sample_cap = int(np.ceil(self.t_end / self.freq_limit)) + 2
+
+        def t(a):
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+        from pyaxisymflow_amd.kernels.smooth_Heaviside import smooth_Heaviside
+
+        self.t_vorticity, self.t_psi = t(0 * Z), t(0 * Z)
+        self.t_u_z, self.t_u_r = t(0 * Z), t(0 * Z)
+        self.t_eta1, self.t_eta2 = t(Z), t(R)
+        self.t_solid_phi = t(self.L_s - R)
+        self.t_solid_char_func = t(0 * Z)
+        smooth_Heaviside(self.t_solid_char_func, self.t_solid_phi, moll_zone)
+        self.t_wall_phi = t(-np.sqrt((R - self.R_wall_center) ** 2) + wall_half)
+        self.t_wall_char_func = t(0 * Z)
+        smooth_Heaviside(self.t_wall_char_func, self.t_wall_phi, moll_zone)
+        sc = np.zeros(N_SCALARS)
+        sc[8] = self.freq_limit  # the sample timer starts at its limit (:95)
+        self.t_scalars = t(sc)
+        self.t_samples = t(np.zeros((self.sample_cap, self.sample_rows + 1)))
+        self.t_r = t(self.r)
+        w = kill_width
+        # kill_boundary_vorticity_sine.py:17-27 weights of the top rows, as
+        # the reference forms them (its literal 1 as the r end)
+        self.t_wr = t(np.sin(np.pi * (1 - self.r[-w:] - 0.5 * dx) / 2 / (w - 1) / dx))
+        cfg = _lib.SoftSlabConfig()
+        cfg.nr, cfg.nz, cfg.ghost, cfg.kill_width = nr, nz, 2, w
+        cfg.sample_rows, cfg.sample_cap = self.sample_rows, self.sample_cap
+        cfg.max_extrap_iterations = int(max_extrap_iterations)
+        cfg.dx, cfg.nu, cfg.G, cfg.rho, cfg.lam = dx, self.nu, self.G, self.rho_f, 1e4
+        cfg.V_wall, cfg.omega, cfg.CFL, cfg.eps = self.V_wall, self.omega, 0.1, np.finfo(float).eps
+        cfg.z_max, cfg.extrap_tol = max_z, 1e-3
+        cfg.extrap_band = moll_zone + 3 * dx
+        cfg.freq_limit, cfg.t_end = self.freq_limit, self.t_end
+        self.cfg = cfg
+        bufs = _lib.SoftSlabBuffers()
+        for name in ("vorticity", "psi", "u_z", "u_r", "eta1", "eta2", "solid_phi", "scalars",
+                     "samples", "solid_char_func", "wall_char_func", "r"):
+            setattr(bufs, name, getattr(self, "t_" + name).data_ptr())
+        bufs.kill_wr = self.t_wr.data_ptr()
+        self._bufs = bufs
+        torch.cuda.synchronize(dev)
+        handle = ctypes.c_void_p()
+        _lib.call("axs_soft_slab_stepper_create", ctypes.byref(cfg), ctypes.byref(bufs),
+                  ctypes.byref(handle))
+        self._h = handle
+
+    def step(self, n=1, join=True):
+        torch = _lib.torch()
+        cur = torch.cuda.current_stream() if join else None
+        if join:
+            self.stream.wait_stream(cur)
+        _lib.call("axs_soft_slab_stepper_run", self._h, int(n), 1 if self.use_graph else 0,
+                  ctypes.c_void_p(self.stream.cuda_stream))
+        if join:
+            cur.wait_stream(self.stream)
+
+    def synchronize(self):
+        self.stream.synchronize()
+
+    def _host(self, t):
+        self.synchronize()
+        return t.cpu().numpy()
+
+    def vorticity(self):
+        return self._host(self.t_vorticity)
+
+    def psi(self):
+        return self._host(self.t_psi)
+
+    def velocity(self):
+        """The penalised velocity of the last step (what it advected with)."""
+        return self._host(self.t_u_z), self._host(self.t_u_r)
+
+    def eta(self):
+        return self._host(self.t_eta1), self._host(self.t_eta2)
+
+    def scalars(self):
+        return self._host(self.t_scalars)
+
+    def time(self):
+        return float(self.scalars()[0])
+
+    def dt(self):
+        return float(self.scalars()[1])
+
+    def steps_done(self):
+        return int(self.scalars()[7])
+
+    def jacobi_last(self):
+        """Jacobi iterations of the last step's four extrapolation loops."""
+        return int(self.scalars()[10])
+
+    def jacobi_capped(self):
+        """Extrapolation loops that stopped at the iteration bound."""
+        return int(self.scalars()[11])
+
+    def samples(self):
+        """(times, u_z profiles) of the samples taken so far."""
+        k = min(int(self.scalars()[9]), self.sample_cap)
+        s = self._host(self.t_samples)[:k]
+        return s[:, 0].copy(), s[:, 1:].copy()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self.synchronize()
+                _lib.load().axs_soft_slab_stepper_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
 class TorusStreamingStepper:
     """examples/TorusStreaming/torus_streaming.py:22-233 on the GPU with
     implicit diffusion (the driver's setting): a torus (tube radius a =
