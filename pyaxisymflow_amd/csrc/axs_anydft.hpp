@@ -30,7 +30,8 @@ struct AnyDftArgs {
   const Cplx<double>* chirp = nullptr;  // w_n, n < N (Bluestein)
   const Cplx<double>* bhat = nullptr;   // FFT_L(b) / L (Bluestein)
   // 1: the FFT reads tw from a copy in LDS after the L work entries
-  // (any_dft_stage), when both fit the LDS a workgroup may take
+  // (any_dft_stage), for rows short enough that two workgroups still share
+  // a CU
   int stage = 0;
 };
 

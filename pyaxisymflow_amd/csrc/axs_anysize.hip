@@ -100,7 +100,9 @@ hipError_t AnyDft::init(int N) {
   args.N = N;
   args.L = L;
   args.bluestein = blue ? 1 : 0;
-  args.stage = 2 * sizeof(Cplx<double>) * (size_t)L <= (size_t)g_any_lds_limit ? 1 : 0;
+  // staged only while the doubled footprint still lets two workgroups share
+  // a CU (longer rows lose more to occupancy than the LDS twiddles save)
+  args.stage = 2 * sizeof(Cplx<double>) * (size_t)L <= 65536 ? 1 : 0;
   radices_of(L, &args.fp);
   nth = std::min(1024, std::max(64, ((L / 2 + 63) / 64) * 64));
   per = (L / 2 + nth - 1) / nth;

@@ -527,7 +527,7 @@ struct SoftStepper {
   // Queued-sweep growth (cfg.lsq_grow): a graph launch is followed by an
   // asynchronous copy of the most sweeps a step has taken; once it has
   // landed (polled; waited for only when 16 graph launches are queued behind
-  // it, so the device keeps ~128 steps of work) the queue grows to that + 2,
+  // it, so the device keeps ~128 steps of work) the queue grows to that + 4,
   // rounded up to 4, and the graphs are dropped to be captured again.  Loops
   // that outrun the queue meanwhile are finished by the extrapolation tail.
   hipError_t poll_need() {
@@ -536,7 +536,7 @@ struct SoftStepper {
     if (e == hipErrorNotReady) return hipSuccess;
     if (e != hipSuccess) return e;
     need_pending = false;
-    const int want = std::min(kMaxQueued, (int)((*need_host + 2u + 3u) / 4u * 4u));
+    const int want = std::min(kMaxQueued, (int)((*need_host + 4u + 3u) / 4u * 4u));
     if (want > cfg.lsq_sweeps) {
       cfg.lsq_sweeps = want;
       drop_graphs();
@@ -955,7 +955,7 @@ struct SoftBatch {
     if (e == hipErrorNotReady) return hipSuccess;
     if (e != hipSuccess) return e;
     need_pending = false;
-    const int want = std::min(kMaxQueued, (int)((*need_host + 2u + 3u) / 4u * 4u));
+    const int want = std::min(kMaxQueued, (int)((*need_host + 4u + 3u) / 4u * 4u));
     if (want > sweeps) {
       sweeps = want;
       drop_graphs();

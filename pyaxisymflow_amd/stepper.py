@@ -593,7 +593,7 @@ class SoftSphereStepper:
     vorticity C (kz^2 + kr^2) sin(kz z) sin(kr r) with kz = kr = 2 pi.
     ``lsq_sweeps`` fixes the number of extrapolation sweeps queued per step
     as separate launches; by default the queue starts at 12 and grows with
-    the most sweeps a step has needed (+ 2, rounded up to 4) as graph runs
+    the most sweeps a step has needed (+ 4, rounded up to 4) as graph runs
     learn it. The reference loops until a sweep fits nothing, which takes
     8-21 sweeps for stiff solids and up to ~60 for the softest as they
     deform; a loop longer than the queue is finished by one workgroup, so
