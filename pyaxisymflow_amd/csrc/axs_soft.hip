@@ -248,9 +248,12 @@ __global__ void soft_lsq_back_kernel(double* __restrict__ e1, double* __restrict
                                      const double* __restrict__ e2d2,
                                      const unsigned* __restrict__ counters, int sweeps, int64_t n,
                                      int64_t off) {
+  __shared__ int s_second;  // the same for every cell: once per workgroup
+  if (threadIdx.x == 0) s_second = lsq_result_is_second(counters, sweeps) ? 1 : 0;
+  __syncthreads();
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  const bool second = lsq_result_is_second(counters, sweeps);
+  const bool second = s_second != 0;
   e1[c] = (second ? e1d2 : e1d)[off + c];
   e2[c] = (second ? e2d2 : e2d)[off + c];
 }
@@ -609,11 +612,14 @@ __global__ void soft_batch_mirror_kernel(const SoftCase* __restrict__ cs,
 
 __global__ void soft_batch_back_kernel(const SoftCase* __restrict__ cs,
                                        const uint32_t* __restrict__ blocks, int sweeps) {
+  __shared__ int s_second;
   const SoftBlock b = soft_block(blocks);
   const SoftCase& C = cs[b.c];
+  if (threadIdx.x == 0) s_second = lsq_result_is_second(C.lw.counters, sweeps) ? 1 : 0;
+  __syncthreads();
   if (b.j >= C.nz) return;
   const int64_t n = (int64_t)C.nr * C.nz, c = (int64_t)b.i * C.nz + b.j;
-  const bool second = lsq_result_is_second(C.lw.counters, sweeps);
+  const bool second = s_second != 0;
   C.eta1[c] = (second ? C.lw.ex2 : C.e1d)[n + c];
   C.eta2[c] = (second ? C.lw.ey2 : C.e2d)[n + c];
 }
