@@ -342,7 +342,32 @@ def time_potential(sizes=((256, 128), (2048, 1024)), reps=20):
     return dict(unit="us per potential solve + velocity (drop-in, fields resident)", sizes=res)
 
 
-def time_serving(n_requests, steps, max_active=32, seed=0):
+def serving_precision_error(steps, nr=256, G=1.0):
+    """One request of the serving set run to completion in the FP32-bulk
+    and the FP64 soft batch: relative L2 of the FP32 fields against the
+    FP64 ones (the FP64 batch matches the oracle to round-off)."""
+    from pyaxisymflow_amd.stepper import SoftSphereBatch
+
+    out = {}
+    views = {}
+    for prec in ("f32", "f64"):
+        b = SoftSphereBatch(max_cases=1, precision=prec)
+        c = b.add(grid_size_z=2 * nr, G=G)
+        b.step(steps)
+        e1, e2 = c.eta()
+        views[prec] = dict(vorticity=c.vorticity().astype(np.float64),
+                           psi=c.psi().astype(np.float64), eta1=e1, eta2=e2, phi=c.phi(),
+                           t=c.time())
+        del b
+    for k in ("vorticity", "psi", "eta1", "eta2", "phi"):
+        a, b = views["f32"][k], views["f64"][k]
+        out[f"{k}_rel_l2"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out["time_rel_err"] = abs(views["f32"]["t"] - views["f64"]["t"]) / views["f64"]["t"]
+    out["request"] = f"nr {nr}, nz {2 * nr}, G {G}, {steps} steps"
+    return out
+
+
+def time_serving(n_requests, steps, max_active=32, seed=0, precision="f32", open_loop=True):
     """The serving form: soft-sphere requests (pyaxisymflow_amd.serve:
     nr lognormal around 256 in [128, 1024]; G in [0.1, 10]), continuously
     batched (one launch per pass over the admitted requests).  A closed-loop
@@ -351,19 +376,25 @@ def time_serving(n_requests, steps, max_active=32, seed=0):
     time-to-solution percentiles."""
     from pyaxisymflow_amd.serve import SoftServer, latency_summary, request_set
 
-    srv = SoftServer(max_active=max_active, chunk=100)
+    srv = SoftServer(max_active=max_active, chunk=100, precision=precision)
     closed = request_set(n_requests, seed=seed, rate=1e9, steps=steps)
     t0 = time.perf_counter()
     recs = srv.run(closed)
     t_closed = time.perf_counter() - t0
     capacity = n_requests / t_closed
+    cells = sum(r["nr"] * r["grid_size_z"] for r in closed) * steps
+    if not open_loop:
+        return dict(closed_loop_wall_s=round(t_closed, 3),
+                    closed_loop_mcell_steps_per_s=round(cells / t_closed / 1e6, 2),
+                    capacity_requests_per_s=round(capacity, 3),
+                    lsq_tail_steps=int(sum(r["lsq_unconverged"] for r in recs)),
+                    precision=precision)
     rate = 0.8 * capacity
     open_set = request_set(n_requests, seed=seed, rate=rate, steps=steps)
     t0 = time.perf_counter()
     recs2 = srv.run(open_set)
     t_open = time.perf_counter() - t0
     s = latency_summary(recs2)
-    cells = sum(r["nr"] * r["grid_size_z"] for r in open_set) * steps
     return dict(
         p50_s=round(s["p50_s"], 4), p99_s=round(s["p99_s"], 4), mean_s=round(s["mean_s"], 4),
         unit="s (case time-to-solution: arrival to last step done)",
@@ -377,7 +408,7 @@ def time_serving(n_requests, steps, max_active=32, seed=0):
         # finished by the one-workgroup tail (results unchanged)
         lsq_tail_steps=int(sum(r["lsq_unconverged"] for r in recs + recs2)),
         queued_lsq_sweeps=srv.last_batch_info[1] if srv.last_batch_info else None,
-        precision="f64",
+        precision=precision,
     )
 
 
@@ -513,7 +544,13 @@ def main():
         forms["implicit_diffusion"] = time_implicit()
         forms["potential"] = time_potential()
     if not args.no_serve:
-        forms["serving_soft"] = time_serving(args.serve_requests, args.serve_steps)
+        sv = time_serving(args.serve_requests, args.serve_steps)
+        sv["dtype"] = ("f32 flow fields/transforms/stencils; f64 radial elimination, reference "
+                       "map, level set, reinit, extrapolation, scalars")
+        sv["f64"] = time_serving(args.serve_requests, args.serve_steps, precision="f64",
+                                 open_loop=False)
+        sv["f32_vs_f64"] = serving_precision_error(args.serve_steps)
+        forms["serving_soft"] = sv
     if forms:
         out["forms"] = forms
     if not args.no_cpu:

@@ -411,7 +411,7 @@ typedef struct {
   int lsq_sweeps;        /* extrapolation sweeps queued per step as launches */
   int reinit_iterations; /* < 0: 2 ceil(reinit_band / dx) + 8 */
   int lsq_grow;          /* nonzero: graph runs raise lsq_sweeps to the most sweeps a step
-                            has needed so far + 8, rounded up to 8 (at most 4096), as they
+                            has needed so far + 4, rounded up to 4 (at most 4096), as they
                             learn it (read back without blocking; the graphs are captured
                             again); eager runs keep it */
   double dx, nu, G, rho_f, CFL, eps;
@@ -448,13 +448,36 @@ axs_status axs_soft_stepper_run(axs_soft_stepper* st, int nsteps, int use_graph,
  * gives the same bits alone and in any mix.  Cases join (add) and leave
  * (remove) between runs; both wait for the device to finish queued work.
  * The configuration and buffers are those of axs_soft_stepper_create
- * (lsq_sweeps and lsq_grow are the batch's: the queue starts at 24 sweeps
+ * (lsq_sweeps and lsq_grow are the batch's: the queue starts at 12 sweeps
  * and grows as in lsq_grow); the buffers must stay valid until the case is
  * removed.  At most 1024 cases; nz up to 4096 (nr up to 2048). */
 typedef struct axs_soft_batch axs_soft_batch;
 axs_status axs_soft_batch_create(int max_cases, axs_soft_batch** out);
 axs_status axs_soft_batch_add(axs_soft_batch* b, const axs_soft_config* cfg,
                               const axs_soft_buffers* bufs, int* case_id);
+/* FP32-bulk soft batch (the serving form's precision; no single-process
+ * counterpart in the reference, whose loop is FP64 throughout): the flow
+ * fields vorticity, psi, u_z, u_r and char_func are floats, as are the
+ * batch's spectrum, stress and RK2 scratch; the row transforms, velocity,
+ * stress divergence, vorticity ENO3 and RK2 run in FP32.  FP64 stays for
+ * the radial elimination (FP32 spectrum in, FP64 recurrences), the
+ * reference map eta1/eta2 (its ENO3 step with the FP32 velocity
+ * converted), the stress arithmetic (stored as float), the level set phi
+ * and its reinit, the least-squares extrapolation and the scalars, so the
+ * mask and the extrapolation's cells come from FP64 values.  Cases of one
+ * batch share one storage type: an FP32 batch takes only
+ * axs_soft_batch_add_f32 and an FP64 one only axs_soft_batch_add; remove,
+ * run, info and destroy take either. */
+typedef struct {
+  float *vorticity, *psi, *u_z, *u_r;
+  double *eta1, *eta2, *phi;
+  float* char_func;
+  double* scalars;
+  const double *r, *z, *kill_wz, *kill_wr;
+} axs_soft_buffers_f32;
+axs_status axs_soft_batch_create_f32(int max_cases, axs_soft_batch** out);
+axs_status axs_soft_batch_add_f32(axs_soft_batch* b, const axs_soft_config* cfg,
+                                  const axs_soft_buffers_f32* bufs, int* case_id);
 axs_status axs_soft_batch_remove(axs_soft_batch* b, int case_id);
 /* Queues nsteps steps of every case on `stream` (use_graph: captured after
  * each change of the case set, replayed). */

@@ -70,6 +70,13 @@ class SoftBuffers(ctypes.Structure):
         "kill_wz", "kill_wr")]
 
 
+class SoftBuffersF32(ctypes.Structure):
+    """axs_soft_buffers_f32 (include/axisym.h): the same fields, the flow
+    fields (vorticity, psi, u_z, u_r, char_func) pointing at float32 data."""
+
+    _fields_ = SoftBuffers._fields_
+
+
 class SlabConfig(ctypes.Structure):
     """axs_slab_config (include/axisym.h)."""
 
@@ -174,6 +181,9 @@ SIGNATURES = {
     "axs_soft_stepper_run": [_P, _I, _I, _P],
     "axs_soft_batch_create": [_I, _P],
     "axs_soft_batch_add": [_P, ctypes.POINTER(SoftConfig), ctypes.POINTER(SoftBuffers), _P],
+    "axs_soft_batch_create_f32": [_I, _P],
+    "axs_soft_batch_add_f32": [_P, ctypes.POINTER(SoftConfig), ctypes.POINTER(SoftBuffersF32),
+                               _P],
     "axs_soft_batch_remove": [_P, _I],
     "axs_soft_batch_run": [_P, _I, _I, _P],
     "axs_soft_batch_info": [_P, _P, _P, _P, _P],

@@ -292,27 +292,28 @@ __global__ void __launch_bounds__(1024) any_dct3_rows_kernel(
 }
 
 // Soft batches: row blockIdx.x of case blockIdx.y (tapered omega -> w0,
-// r omega -> spectrum; the spectrum -> psi).
-template <int PER>
-__global__ void __launch_bounds__(1024) soft_batch_dct2_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.y];
+// r omega -> spectrum; the spectrum -> psi), in the batch's storage type.
+template <typename B, int PER>
+__global__ void __launch_bounds__(1024) soft_batch_dct2_kernel(
+    const SoftCaseT<B>* __restrict__ cs) {
+  const SoftCaseT<B>& C = cs[blockIdx.y];
   if ((int)blockIdx.x >= C.nr) return;
   extern __shared__ unsigned char smem_raw[];
   const AnyDctTables tb{C.any, C.ge2, C.te1, C.te2, C.any_half};
-  any_dct2_row<double, PER, true>(C.vorticity, C.w0, C.spec, C.rvec, C.kill_wz, C.kill_wr, C.nr,
-                                  C.nz, C.kill_width, tb, blockIdx.x,
-                                  reinterpret_cast<Cplx<double>*>(smem_raw), threadIdx.x,
-                                  blockDim.x);
+  any_dct2_row<B, PER, true>(C.vorticity, C.w0, C.spec, C.rvec, C.kill_wz, C.kill_wr, C.nr,
+                             C.nz, C.kill_width, tb, blockIdx.x,
+                             reinterpret_cast<Cplx<B>*>(smem_raw), threadIdx.x, blockDim.x);
 }
 
-template <int PER>
-__global__ void __launch_bounds__(1024) soft_batch_dct3_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.y];
+template <typename B, int PER>
+__global__ void __launch_bounds__(1024) soft_batch_dct3_kernel(
+    const SoftCaseT<B>* __restrict__ cs) {
+  const SoftCaseT<B>& C = cs[blockIdx.y];
   if ((int)blockIdx.x >= C.nr) return;
   extern __shared__ unsigned char smem_raw[];
   const AnyDctTables tb{C.any, C.ge2, C.te1, C.te2, C.any_half};
-  any_dct3_row<double, PER>(C.spec, C.psi, C.nz, tb, blockIdx.x,
-                            reinterpret_cast<Cplx<double>*>(smem_raw), threadIdx.x, blockDim.x);
+  any_dct3_row<B, PER>(C.spec, C.psi, C.nz, tb, blockIdx.x, reinterpret_cast<Cplx<B>*>(smem_raw),
+                       threadIdx.x, blockDim.x);
 }
 
 // Real DFT rows of any length into the packed real layout of the periodic
@@ -451,10 +452,19 @@ static void allow_large_lds() {
         (const void*)any_rfft_rows_kernel<4>, (const void*)any_irfft_rows_kernel<1>,
         (const void*)any_irfft_rows_kernel<2>, (const void*)any_irfft_rows_kernel<4>,
         (const void*)any_dst1_rows_kernel<1>, (const void*)any_dst1_rows_kernel<2>,
-        (const void*)any_dst1_rows_kernel<4>, (const void*)soft_batch_dct2_kernel<1>,
-        (const void*)soft_batch_dct2_kernel<2>, (const void*)soft_batch_dct2_kernel<4>,
-        (const void*)soft_batch_dct3_kernel<1>, (const void*)soft_batch_dct3_kernel<2>,
-        (const void*)soft_batch_dct3_kernel<4>};
+        (const void*)any_dst1_rows_kernel<4>,
+        (const void*)soft_batch_dct2_kernel<double, 1>,
+        (const void*)soft_batch_dct2_kernel<double, 2>,
+        (const void*)soft_batch_dct2_kernel<double, 4>,
+        (const void*)soft_batch_dct3_kernel<double, 1>,
+        (const void*)soft_batch_dct3_kernel<double, 2>,
+        (const void*)soft_batch_dct3_kernel<double, 4>,
+        (const void*)soft_batch_dct2_kernel<float, 1>,
+        (const void*)soft_batch_dct2_kernel<float, 2>,
+        (const void*)soft_batch_dct2_kernel<float, 4>,
+        (const void*)soft_batch_dct3_kernel<float, 1>,
+        (const void*)soft_batch_dct3_kernel<float, 2>,
+        (const void*)soft_batch_dct3_kernel<float, 4>};
     int limit = optin;
     for (const void* k : ks) {
       hipFuncAttributes fa{};
@@ -537,17 +547,32 @@ hipError_t launch_any_dst1_rows(const AnyDft& D, const Cplx<double>* em, const d
   return hipGetLastError();
 }
 
-hipError_t launch_soft_batch_dct2(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
+template <typename B>
+hipError_t launch_soft_batch_dct2(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                  hipStream_t s) {
   const dim3 grid(d.max_nr, d.ncase);
-  AXS_ANY_PER(d.any_per, (soft_batch_dct2_kernel<P><<<grid, d.any_nth, d.any_lds, s>>>(cases)));
+  AXS_ANY_PER(d.any_per,
+              (soft_batch_dct2_kernel<B, P><<<grid, d.any_nth, d.any_lds, s>>>(cases)));
   return hipGetLastError();
 }
 
-hipError_t launch_soft_batch_dct3(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
+template <typename B>
+hipError_t launch_soft_batch_dct3(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                  hipStream_t s) {
   const dim3 grid(d.max_nr, d.ncase);
-  AXS_ANY_PER(d.any_per, (soft_batch_dct3_kernel<P><<<grid, d.any_nth, d.any_lds, s>>>(cases)));
+  AXS_ANY_PER(d.any_per,
+              (soft_batch_dct3_kernel<B, P><<<grid, d.any_nth, d.any_lds, s>>>(cases)));
   return hipGetLastError();
 }
+
+template hipError_t launch_soft_batch_dct2<double>(const SoftCaseT<double>*, const SoftBatchDims&,
+                                                   hipStream_t);
+template hipError_t launch_soft_batch_dct2<float>(const SoftCaseT<float>*, const SoftBatchDims&,
+                                                  hipStream_t);
+template hipError_t launch_soft_batch_dct3<double>(const SoftCaseT<double>*, const SoftBatchDims&,
+                                                   hipStream_t);
+template hipError_t launch_soft_batch_dct3<float>(const SoftCaseT<float>*, const SoftBatchDims&,
+                                                  hipStream_t);
 
 #undef AXS_ANY_PER
 

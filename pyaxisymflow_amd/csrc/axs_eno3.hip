@@ -56,24 +56,26 @@ __device__ __forceinline__ int mirror_row(int m, int* flip) {
   return m;
 }
 
-template <typename T>
-__device__ __forceinline__ void load_column5(const T* __restrict__ a, int i, int j, int nz,
+// Stencil loads, converted to the arithmetic type T (S = T but for the
+// FP32 velocity of the FP32-bulk soft batch's FP64 reference-map step).
+template <typename S, typename T>
+__device__ __forceinline__ void load_column5(const S* __restrict__ a, int i, int j, int nz,
                                              T odd_sign, T* out) {
 #pragma unroll
   for (int o = -2; o <= 2; ++o) {
     int flip;
     const int m = mirror_row(i + o, &flip);
-    const T v = a[(int64_t)m * nz + j];
+    const T v = T(a[(int64_t)m * nz + j]);
     out[o + 2] = flip ? odd_sign * v : v;
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void load_row5(const T* __restrict__ a, int i, int j, int nz,
+template <typename S, typename T>
+__device__ __forceinline__ void load_row5(const S* __restrict__ a, int i, int j, int nz,
                                           T* out) {
-  const T* row = a + (int64_t)i * nz + j;
+  const S* row = a + (int64_t)i * nz + j;
 #pragma unroll
-  for (int o = -2; o <= 2; ++o) out[o + 2] = row[o];
+  for (int o = -2; o <= 2; ++o) out[o + 2] = T(row[o]);
 }
 
 // One conservative ENO3 step at cell (i, j): out = f + flux(f, u), f with
@@ -113,11 +115,11 @@ __global__ void eno3_conservative_kernel(const T* __restrict__ f, const T* __res
 }
 
 // One non-conservative ENO3 step for two fields sharing the velocity at cell
-// (i, j): eta1 (even) and eta2 (odd).
-template <typename T>
+// (i, j): eta1 (even) and eta2 (odd); the velocity stored in U.
+template <typename T, typename U = T>
 __device__ __forceinline__ void eno3_nonconservative2_at(
-    const T* __restrict__ e1, const T* __restrict__ e2, const T* __restrict__ uz,
-    const T* __restrict__ ur, T* __restrict__ o1, T* __restrict__ o2, int nr, int nz,
+    const T* __restrict__ e1, const T* __restrict__ e2, const U* __restrict__ uz,
+    const U* __restrict__ ur, T* __restrict__ o1, T* __restrict__ o2, int nr, int nz,
     T dt_by_dx, int i, int j) {
   const int64_t c = (int64_t)i * nz + j;
   if (i >= nr - 2 || j < 2 || j >= nz - 2) {
@@ -156,35 +158,51 @@ __global__ void eno3_nonconservative2_kernel(const T* __restrict__ e1, const T* 
 }
 
 // Soft batches: one row segment of 256 cells of one case per workgroup.
-// The reference map: (eta1, eta2) -> (e1n, e2n).
-__global__ void soft_batch_refmap_kernel(const SoftCase* __restrict__ cs,
+// The reference map (FP64, with the batch's velocity): (eta1, eta2) ->
+// (e1n, e2n).
+template <typename B>
+__global__ void soft_batch_refmap_kernel(const SoftCaseT<B>* __restrict__ cs,
                                          const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   if (b.j >= C.nz) return;
-  eno3_nonconservative2_at<double>(C.eta1, C.eta2, C.u_z, C.u_r, C.e1n, C.e2n, C.nr, C.nz,
-                                   C.scalars[kSsDtByDx], b.i, b.j);
+  eno3_nonconservative2_at<double, B>(C.eta1, C.eta2, C.u_z, C.u_r, C.e1n, C.e2n, C.nr, C.nz,
+                                      C.scalars[kSsDtByDx], b.i, b.j);
 }
 
 // The vorticity: w0 (after the stress update) -> w1.
-__global__ void soft_batch_advect_kernel(const SoftCase* __restrict__ cs,
+template <typename B>
+__global__ void soft_batch_advect_kernel(const SoftCaseT<B>* __restrict__ cs,
                                          const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   if (b.j >= C.nz) return;
-  eno3_conservative_at<double>(C.w0, C.u_z, C.u_r, C.w1, C.nr, C.nz, C.scalars[kSsDtByDx], -1.0,
-                               b.i, b.j);
+  eno3_conservative_at<B>(C.w0, C.u_z, C.u_r, C.w1, C.nr, C.nz, B(C.scalars[kSsDtByDx]), B(-1),
+                          b.i, b.j);
 }
 
-hipError_t launch_soft_batch_refmap(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
-  soft_batch_refmap_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
+template <typename B>
+hipError_t launch_soft_batch_refmap(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                    hipStream_t s) {
+  soft_batch_refmap_kernel<B><<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   return hipGetLastError();
 }
 
-hipError_t launch_soft_batch_advect(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
-  soft_batch_advect_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
+template <typename B>
+hipError_t launch_soft_batch_advect(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                    hipStream_t s) {
+  soft_batch_advect_kernel<B><<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   return hipGetLastError();
 }
+
+template hipError_t launch_soft_batch_refmap<double>(const SoftCaseT<double>*,
+                                                     const SoftBatchDims&, hipStream_t);
+template hipError_t launch_soft_batch_refmap<float>(const SoftCaseT<float>*, const SoftBatchDims&,
+                                                    hipStream_t);
+template hipError_t launch_soft_batch_advect<double>(const SoftCaseT<double>*,
+                                                     const SoftBatchDims&, hipStream_t);
+template hipError_t launch_soft_batch_advect<float>(const SoftCaseT<float>*, const SoftBatchDims&,
+                                                    hipStream_t);
 
 template <typename T>
 hipError_t launch_eno3_conservative(const T* f, const T* uz, const T* ur, T* out, int nr, int nz,

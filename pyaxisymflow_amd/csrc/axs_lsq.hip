@@ -357,10 +357,11 @@ __global__ void __launch_bounds__(512) lsq_sweep_tail_kernel(
 // coordinates as in the single-case step; each case has its own list,
 // counters and tail lists.
 
-__global__ void soft_batch_lsq_list_kernel(const SoftCase* __restrict__ cs,
+template <typename B>
+__global__ void soft_batch_lsq_list_kernel(const SoftCaseT<B>* __restrict__ cs,
                                            const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);  // a row of the doubled grid
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   const int j = b.j;
   const int64_t c = (int64_t)b.i * C.nz + j;
   list_append(j < C.nz && (C.cur[c] ^ C.tgt[c]), (int)c, C.lw.list, C.lw.count);
@@ -368,8 +369,9 @@ __global__ void soft_batch_lsq_list_kernel(const SoftCase* __restrict__ cs,
 
 // Ping-pong sweep q over each case's list (blockIdx.y = case): sweep q reads
 // the first set (cur, e1d, e2d) when q is even.
-__global__ void soft_batch_lsq_sweep_kernel(const SoftCase* __restrict__ cs, int q) {
-  const SoftCase& C = cs[blockIdx.y];
+template <typename B>
+__global__ void soft_batch_lsq_sweep_kernel(const SoftCaseT<B>* __restrict__ cs, int q) {
+  const SoftCaseT<B>& C = cs[blockIdx.y];
   const LsqWork& w = C.lw;
   if (q > 0 && w.counters[q - 1] == 0u) return;  // the reference loop has ended
   const bool odd = q & 1;
@@ -398,32 +400,49 @@ __global__ void soft_batch_lsq_sweep_kernel(const SoftCase* __restrict__ cs, int
   }
 }
 
-__global__ void __launch_bounds__(512) soft_batch_lsq_tail_kernel(const SoftCase* __restrict__ cs,
-                                                                  int sweeps) {
-  const SoftCase& C = cs[blockIdx.x];
+template <typename B>
+__global__ void __launch_bounds__(512) soft_batch_lsq_tail_kernel(
+    const SoftCaseT<B>* __restrict__ cs, int sweeps) {
+  const SoftCaseT<B>& C = cs[blockIdx.x];
   const LsqWork& w = C.lw;
   lsq_sweep_tail_body(C.cur, C.e1d, C.e2d, w.cur2, w.ex2, w.ey2, C.tgt, C.z, C.z, 2 * C.nr, C.nz,
                       w.list, w.count, w.counters, sweeps, w.tail, w.cap);
 }
 
-hipError_t launch_soft_batch_lsq_list(const SoftCase* cases, const SoftBatchDims& d,
+template <typename B>
+hipError_t launch_soft_batch_lsq_list(const SoftCaseT<B>* cases, const SoftBatchDims& d,
                                       hipStream_t s) {
-  soft_batch_lsq_list_kernel<<<d.ndbl_blocks, 256, 0, s>>>(cases, d.dbl_blocks);
+  soft_batch_lsq_list_kernel<B><<<d.ndbl_blocks, 256, 0, s>>>(cases, d.dbl_blocks);
   return hipGetLastError();
 }
 
-hipError_t launch_soft_batch_lsq_sweep(const SoftCase* cases, const SoftBatchDims& d, int sweep,
+template <typename B>
+hipError_t launch_soft_batch_lsq_sweep(const SoftCaseT<B>* cases, const SoftBatchDims& d, int sweep,
                                        hipStream_t s) {
   constexpr int kBlocksPerCase = 64;  // a case's band: up to ~16k cells, one per thread
-  soft_batch_lsq_sweep_kernel<<<dim3(kBlocksPerCase, d.ncase), 256, 0, s>>>(cases, sweep);
+  soft_batch_lsq_sweep_kernel<B><<<dim3(kBlocksPerCase, d.ncase), 256, 0, s>>>(cases, sweep);
   return hipGetLastError();
 }
 
-hipError_t launch_soft_batch_lsq_tail(const SoftCase* cases, const SoftBatchDims& d, int sweeps,
+template <typename B>
+hipError_t launch_soft_batch_lsq_tail(const SoftCaseT<B>* cases, const SoftBatchDims& d, int sweeps,
                                       hipStream_t s) {
-  soft_batch_lsq_tail_kernel<<<d.ncase, 512, 0, s>>>(cases, sweeps);
+  soft_batch_lsq_tail_kernel<B><<<d.ncase, 512, 0, s>>>(cases, sweeps);
   return hipGetLastError();
 }
+
+template hipError_t launch_soft_batch_lsq_list<double>(const SoftCaseT<double>*,
+                                                       const SoftBatchDims&, hipStream_t);
+template hipError_t launch_soft_batch_lsq_list<float>(const SoftCaseT<float>*,
+                                                      const SoftBatchDims&, hipStream_t);
+template hipError_t launch_soft_batch_lsq_sweep<double>(const SoftCaseT<double>*,
+                                                        const SoftBatchDims&, int, hipStream_t);
+template hipError_t launch_soft_batch_lsq_sweep<float>(const SoftCaseT<float>*,
+                                                       const SoftBatchDims&, int, hipStream_t);
+template hipError_t launch_soft_batch_lsq_tail<double>(const SoftCaseT<double>*,
+                                                       const SoftBatchDims&, int, hipStream_t);
+template hipError_t launch_soft_batch_lsq_tail<float>(const SoftCaseT<float>*,
+                                                      const SoftBatchDims&, int, hipStream_t);
 
 __global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;

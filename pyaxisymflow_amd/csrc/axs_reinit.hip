@@ -425,18 +425,20 @@ __global__ void reinit_final_kernel(const double* __restrict__ phi, const double
 // final pass reads the buffer its own last launch wrote.  The lists are
 // the batch's, over its reinit tile table (entry = the case's first table
 // index + the tile's index in the case).
-__global__ void soft_batch_reinit_init_kernel(const SoftCase* __restrict__ cs,
+template <typename B>
+__global__ void soft_batch_reinit_init_kernel(const SoftCaseT<B>* __restrict__ cs,
                                               const uint32_t* __restrict__ blocks,
                                               unsigned* __restrict__ cnt) {
   if (blockIdx.x == 0 && threadIdx.x < 2) cnt[threadIdx.x] = 0u;
   const SoftBlock b = soft_block(blocks);
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   if (b.j >= C.nz) return;
   reinit_init_at(C.phi, C.rw.d0, C.rw.d1, C.rw.dist, C.rw.frozen, C.rw.ch0, C.rw.ch1, C.rw.tiles,
                  C.nr, C.nz, C.dx, b.i, b.j);
 }
 
-__global__ void soft_batch_reinit_first_list_kernel(const SoftCase* __restrict__ cs,
+template <typename B>
+__global__ void soft_batch_reinit_first_list_kernel(const SoftCaseT<B>* __restrict__ cs,
                                                     const uint32_t* __restrict__ tiles, int n,
                                                     unsigned* __restrict__ list,
                                                     unsigned* __restrict__ cnt) {
@@ -444,15 +446,16 @@ __global__ void soft_batch_reinit_first_list_kernel(const SoftCase* __restrict__
   bool take = false;
   if (idx < n) {
     const uint32_t v = tiles[idx];
-    const SoftCase& C = cs[v & 1023u];
+    const SoftCaseT<B>& C = cs[v & 1023u];
     const int t = int((v >> 10) & 8191u) * reinit_tile_cols(C.nz) + int(v >> 23);
     take = C.reinit_iterations > 0 && C.rw.tiles[t] != 0u;
   }
   list_append(take, idx, reinterpret_cast<int*>(list), cnt);
 }
 
+template <typename B>
 __global__ void __launch_bounds__(256) soft_batch_reinit_list_kernel(
-    const SoftCase* __restrict__ cs, const uint32_t* __restrict__ tiles, int ntab,
+    const SoftCaseT<B>* __restrict__ cs, const uint32_t* __restrict__ tiles, int ntab,
     unsigned* __restrict__ list, unsigned* __restrict__ cnt, int k) {
   constexpr int S = kReinitSweepsPerLaunch;
   __shared__ ReinitTile<S> L;
@@ -462,7 +465,7 @@ __global__ void __launch_bounds__(256) soft_batch_reinit_list_kernel(
   for (int idx = blockIdx.x; idx < n; idx += gridDim.x) {
     const unsigned e = list[(size_t)p * ntab + idx];
     const uint32_t v = tiles[e];
-    const SoftCase& C = cs[v & 1023u];
+    const SoftCaseT<B>& C = cs[v & 1023u];
     const int nsw = std::min(S, C.reinit_iterations - k * S);
     if (nsw <= 0) continue;  // block-uniform
     const int tr = int((v >> 10) & 8191u), tc = int(v >> 23);
@@ -477,10 +480,11 @@ __global__ void __launch_bounds__(256) soft_batch_reinit_list_kernel(
   }
 }
 
-__global__ void soft_batch_reinit_final_kernel(const SoftCase* __restrict__ cs,
+template <typename B>
+__global__ void soft_batch_reinit_final_kernel(const SoftCaseT<B>* __restrict__ cs,
                                                const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   if (b.j >= C.nz) return;
   constexpr int S = kReinitSweepsPerLaunch;
   const int launches = (C.reinit_iterations + S - 1) / S;
@@ -493,24 +497,30 @@ __global__ void soft_batch_reinit_final_kernel(const SoftCase* __restrict__ cs,
 // eight per CU; more entries loop.
 static int reinit_list_grid(int entries) { return std::max(1, std::min(entries, 2048)); }
 
-hipError_t launch_soft_batch_reinit(const SoftCase* cases, const SoftBatchDims& d, int iterations,
-                                    hipStream_t s) {
+template <typename B>
+hipError_t launch_soft_batch_reinit(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                    int iterations, hipStream_t s) {
   const int g = d.ncell_blocks;
-  soft_batch_reinit_init_kernel<<<g, 256, 0, s>>>(cases, d.cell_blocks, d.reinit_cnt);
+  soft_batch_reinit_init_kernel<B><<<g, 256, 0, s>>>(cases, d.cell_blocks, d.reinit_cnt);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  soft_batch_reinit_first_list_kernel<<<ceil_div(d.nreinit_tiles, 256), 256, 0, s>>>(
+  soft_batch_reinit_first_list_kernel<B><<<ceil_div(d.nreinit_tiles, 256), 256, 0, s>>>(
       cases, d.reinit_tiles, d.nreinit_tiles, d.reinit_list, d.reinit_cnt);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int grid = reinit_list_grid(d.nreinit_tiles);
   for (int k = 0; k * kReinitSweepsPerLaunch < iterations; ++k) {
-    soft_batch_reinit_list_kernel<<<grid, 256, 0, s>>>(cases, d.reinit_tiles, d.nreinit_tiles,
+    soft_batch_reinit_list_kernel<B><<<grid, 256, 0, s>>>(cases, d.reinit_tiles, d.nreinit_tiles,
                                                        d.reinit_list, d.reinit_cnt, k);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  soft_batch_reinit_final_kernel<<<g, 256, 0, s>>>(cases, d.cell_blocks);
+  soft_batch_reinit_final_kernel<B><<<g, 256, 0, s>>>(cases, d.cell_blocks);
   return hipGetLastError();
 }
+
+template hipError_t launch_soft_batch_reinit<double>(const SoftCaseT<double>*,
+                                                     const SoftBatchDims&, int, hipStream_t);
+template hipError_t launch_soft_batch_reinit<float>(const SoftCaseT<float>*,
+                                                    const SoftBatchDims&, int, hipStream_t);
 
 hipError_t launch_reinit(const double* phi, double* out, uint8_t* masked, int nr, int nz,
                          double dx, double narrow, int iterations, const ReinitWork& w,

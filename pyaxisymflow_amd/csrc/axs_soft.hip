@@ -37,6 +37,7 @@
 // into a hipGraph (fills are kernels, no memset nodes).
 #include <algorithm>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "axs_eno3.hpp"
@@ -192,8 +193,9 @@ __global__ void soft_pin_kernel(double* __restrict__ e1, double* __restrict__ e2
 // extrapolate_eta_using_least_squares_unb.py:19-24 and the flags of
 // extrapolate_using_least_squares.py:33-35 (lower bound 0); block (0, 0)
 // also clears the extrapolation sweeps' counters.
+template <typename O>
 __device__ __forceinline__ void soft_lsq_mirror_at(
-    const double* __restrict__ phi, double* __restrict__ chi, double moll_zone,
+    const double* __restrict__ phi, O* __restrict__ chi, double moll_zone,
     const double* __restrict__ e1, const double* __restrict__ e2, double* __restrict__ e1d,
     double* __restrict__ e2d, int16_t* __restrict__ cur, int16_t* __restrict__ tgt,
     double* __restrict__ e1d2, double* __restrict__ e2d2, int16_t* __restrict__ cur2, int nr,
@@ -203,7 +205,7 @@ __device__ __forceinline__ void soft_lsq_mirror_at(
   const int64_t c = (int64_t)i * nz + j;
   const double p = phi[c];
   const double h = smooth_heaviside(p, moll_zone);
-  if (upper) chi[c] = h;
+  if (upper) chi[c] = O(h);
   const double in = (h > 0.5) ? 1.0 : 0.0;
   const double pd = -p;
   const double a = in * e1[c];
@@ -286,30 +288,39 @@ __global__ void soft_finalize_kernel(double* __restrict__ sc,
   soft_finalize_body(sc, counters, sweeps, list_count, need);
 }
 
-__global__ void soft_zero_kernel(double* __restrict__ p, int64_t n, int64_t stride) {
+template <typename T>
+__global__ void soft_zero_kernel(T* __restrict__ p, int64_t n, int64_t stride) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) p[k * stride] = 0.0;
+  if (k < n) p[k * stride] = T(0);
 }
 
 // ------------------------------------------------------------------ stepper
 
-struct SoftBuffers {
-  double *vorticity, *psi, *u_z, *u_r, *eta1, *eta2, *phi, *char_func, *scalars;
+// B: storage type of the flow fields (the soft batch's SoftCaseT); the
+// single-case stepper is FP64 (B = double).
+template <typename B>
+struct SoftBuffersT {
+  B *vorticity, *psi, *u_z, *u_r;
+  double *eta1, *eta2, *phi;
+  B* char_func;
+  double* scalars;
   const double *r, *z, *kill_wz, *kill_wr;
 };
+using SoftBuffers = SoftBuffersT<double>;
 
-struct SoftStepper {
+template <typename B>
+struct SoftStepperT {
   SoftConfig cfg{};
-  SoftBuffers buf{};
+  SoftBuffersT<B> buf{};
   StokesPlan* plan = nullptr;
   std::vector<void*> allocs;
-  double *w0 = nullptr, *w1 = nullptr, *tmp = nullptr, *spec = nullptr;
+  B *w0 = nullptr, *w1 = nullptr, *tmp = nullptr, *spec = nullptr;
   double *e1n = nullptr, *e2n = nullptr, *slots = nullptr;
   double *e1d = nullptr, *e2d = nullptr;
   int16_t *cur = nullptr, *tgt = nullptr;
-  double *s11 = nullptr, *s12 = nullptr, *s22 = nullptr;
-  double *g1z = nullptr, *g1r = nullptr, *g2z = nullptr, *g2r = nullptr;
-  double *tz = nullptr, *tr = nullptr;
+  B *s11 = nullptr, *s12 = nullptr, *s22 = nullptr;
+  B *g1z = nullptr, *g1r = nullptr, *g2z = nullptr, *g2r = nullptr;
+  B *tz = nullptr, *tr = nullptr;
   ReinitWork rw{};
   LsqWork lw{};
   hipGraph_t graph = nullptr;
@@ -337,7 +348,7 @@ struct SoftStepper {
   // batch_case: the state of one case of a soft batch (any-length row
   // transforms and radial segments of kSoftBatchTriL rows, which every case
   // of a batch launch shares)
-  hipError_t init(const SoftConfig& c, const SoftBuffers& b, bool batch_case = false) {
+  hipError_t init(const SoftConfig& c, const SoftBuffersT<B>& b, bool batch_case = false) {
     cfg = c;
     buf = b;
     plan = new (std::nothrow) StokesPlan();
@@ -350,11 +361,13 @@ struct SoftStepper {
     hipError_t e = plan->init(cfg.nr, cfg.nz, cfg.dx);
     if (e == hipSuccess && batch_case && plan->tri_G) e = hipErrorInvalidValue;
     if (e != hipSuccess) return e;
-    const size_t n = (size_t)cfg.nr * cfg.nz, nd = 2 * n, B = sizeof(double);
-    double** fields[] = {&w0, &w1, &tmp, &spec, &e1n, &e2n, &s11, &s12, &s22,
-                         &g1z, &g1r, &g2z, &g2r, &tz, &tr, &rw.d0, &rw.d1, &rw.dist};
-    for (double** f : fields)
-      if ((e = alloc(f, n * B)) != hipSuccess) return e;
+    const size_t n = (size_t)cfg.nr * cfg.nz, nd = 2 * n, D8 = sizeof(double);
+    B** flow[] = {&w0, &w1, &tmp, &spec, &s11, &s12, &s22, &g1z, &g1r, &g2z, &g2r, &tz, &tr};
+    for (B** f : flow)
+      if ((e = alloc(f, n * sizeof(B))) != hipSuccess) return e;
+    double** geo[] = {&e1n, &e2n, &rw.d0, &rw.d1, &rw.dist};
+    for (double** f : geo)
+      if ((e = alloc(f, n * D8)) != hipSuccess) return e;
     if ((e = alloc(&rw.frozen, n)) != hipSuccess) return e;
     if ((e = alloc(&rw.ch0, n)) != hipSuccess) return e;
     if ((e = alloc(&rw.ch1, n)) != hipSuccess) return e;
@@ -363,8 +376,8 @@ struct SoftStepper {
     if ((e = hipMemset(rw.tiles, 0, tb)) != hipSuccess) return e;
     if ((e = alloc(&rw.list, tb)) != hipSuccess) return e;
     if ((e = alloc(&rw.cnt, 3 * sizeof(unsigned))) != hipSuccess) return e;
-    if ((e = alloc(&e1d, nd * B)) != hipSuccess) return e;
-    if ((e = alloc(&e2d, nd * B)) != hipSuccess) return e;
+    if ((e = alloc(&e1d, nd * D8)) != hipSuccess) return e;
+    if ((e = alloc(&e2d, nd * D8)) != hipSuccess) return e;
     if ((e = alloc(&cur, nd * sizeof(int16_t))) != hipSuccess) return e;
     if ((e = alloc(&tgt, nd * sizeof(int16_t))) != hipSuccess) return e;
     if ((e = alloc(&lw.counters, sizeof(unsigned) * (kMaxQueued + 1))) != hipSuccess) return e;
@@ -380,23 +393,23 @@ struct SoftStepper {
     if ((e = alloc(&lw.tail, 4 * nd * sizeof(int))) != hipSuccess) return e;
     lw.cap = (int)nd;
     if ((e = alloc(&lw.cur2, nd * sizeof(int16_t))) != hipSuccess) return e;
-    if ((e = alloc(&lw.ex2, nd * B)) != hipSuccess) return e;
-    if ((e = alloc(&lw.ey2, nd * B)) != hipSuccess) return e;
+    if ((e = alloc(&lw.ex2, nd * D8)) != hipSuccess) return e;
+    if ((e = alloc(&lw.ey2, nd * D8)) != hipSuccess) return e;
     if ((e = alloc(&lw.count, sizeof(unsigned))) != hipSuccess) return e;
     if ((e = hipMemset(lw.count, 0, sizeof(unsigned))) != hipSuccess) return e;
-    if ((e = alloc(&slots, B * kSlotsPerCase)) != hipSuccess) return e;
+    if ((e = alloc(&slots, D8 * kSlotsPerCase)) != hipSuccess) return e;
     // scratch the reference allocates once and never fully rewrites
     // (gradient and tau scratch keep zeros where they are never written)
-    for (double* f : {g1z, g1r, g2z, g2r, tz, tr, s11, s12, s22}) {
-      soft_zero_kernel<<<ceil_div((int64_t)n, 256), 256>>>(f, (int64_t)n, 1);
+    for (B* f : {g1z, g1r, g2z, g2r, tz, tr, s11, s12, s22}) {
+      soft_zero_kernel<B><<<ceil_div((int64_t)n, 256), 256>>>(f, (int64_t)n, 1);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    soft_zero_kernel<<<1, 64>>>(slots, kMaxSlots, kSlotStride);
+    soft_zero_kernel<double><<<1, 64>>>(slots, kMaxSlots, kSlotStride);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipDeviceSynchronize();
   }
 
-  ~SoftStepper() {
+  ~SoftStepperT() {
     drop_graphs();
     if (need_ev) (void)hipEventDestroy(need_ev);
     if (need_host) (void)hipHostFree(need_host);
@@ -404,7 +417,9 @@ struct SoftStepper {
     delete plan;
   }
 
+  // The single-case FP64 step (B = double).
   hipError_t enqueue_step(hipStream_t s) {
+    static_assert(std::is_same<B, double>::value, "the single-case soft step is FP64");
     const int nr = cfg.nr, nz = cfg.nz;
     const int64_t n = (int64_t)nr * nz;
     const dim3 g2(ceil_div(nz, 256), nr);
@@ -580,29 +595,33 @@ struct SoftStepper {
     return hipSuccess;
   }
 };
+using SoftStepper = SoftStepperT<double>;
 
 // ------------------------------------------------------------------ batches
-// Kernels over every case of a soft batch: per cell, case blockIdx.z (rows
-// and columns beyond the case's grid exit), or one block per case.
+// Kernels over every case of a soft batch: one row segment of a case per
+// workgroup (the batch's block tables), or one block per case.
 
-__global__ void soft_batch_dt_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.x];
+template <typename B>
+__global__ void soft_batch_dt_kernel(const SoftCaseT<B>* __restrict__ cs) {
+  const SoftCaseT<B>& C = cs[blockIdx.x];
   soft_dt_body(C.scalars, C.slots, C.CFL, C.dx, C.eps, C.G, C.rho_f, C.nu);
 }
 
-__global__ void soft_batch_pin_kernel(const SoftCase* __restrict__ cs,
+template <typename B>
+__global__ void soft_batch_pin_kernel(const SoftCaseT<B>* __restrict__ cs,
                                       const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   if (b.j >= C.nz) return;
   soft_pin_at(C.eta1, C.eta2, C.e1n, C.e2n, C.phi, (int64_t)b.i * C.nz + b.j, C.z_cm, C.r_cm,
               C.r_ball, C.dx);
 }
 
-__global__ void soft_batch_mirror_kernel(const SoftCase* __restrict__ cs,
+template <typename B>
+__global__ void soft_batch_mirror_kernel(const SoftCaseT<B>* __restrict__ cs,
                                          const uint32_t* __restrict__ blocks, int ncounters) {
   const SoftBlock b = soft_block(blocks);  // a row of the doubled grid
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   if (b.i == 0 && b.j < 256)  // the case's first workgroup clears its sweep counters
     for (int k = threadIdx.x; k < ncounters; k += blockDim.x) C.lw.counters[k] = 0u;
   if (b.j >= C.nz) return;
@@ -610,11 +629,12 @@ __global__ void soft_batch_mirror_kernel(const SoftCase* __restrict__ cs,
                      C.lw.ex2, C.lw.ey2, C.lw.cur2, C.nr, C.nz, C.extrap_zone, b.i, b.j);
 }
 
-__global__ void soft_batch_back_kernel(const SoftCase* __restrict__ cs,
+template <typename B>
+__global__ void soft_batch_back_kernel(const SoftCaseT<B>* __restrict__ cs,
                                        const uint32_t* __restrict__ blocks, int sweeps) {
   __shared__ int s_second;
   const SoftBlock b = soft_block(blocks);
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   if (threadIdx.x == 0) s_second = lsq_result_is_second(C.lw.counters, sweeps) ? 1 : 0;
   __syncthreads();
   if (b.j >= C.nz) return;
@@ -624,30 +644,32 @@ __global__ void soft_batch_back_kernel(const SoftCase* __restrict__ cs,
   C.eta2[c] = (second ? C.lw.ey2 : C.e2d)[n + c];
 }
 
-__global__ void soft_batch_finalize_kernel(const SoftCase* __restrict__ cs, int sweeps) {
+template <typename B>
+__global__ void soft_batch_finalize_kernel(const SoftCaseT<B>* __restrict__ cs, int sweeps) {
   if (threadIdx.x != 0) return;
-  const SoftCase& C = cs[blockIdx.x];
+  const SoftCaseT<B>& C = cs[blockIdx.x];
   soft_finalize_body(C.scalars, C.lw.counters, sweeps, C.lw.count, C.need);
 }
 
 // Many soft cases stepped together: each pass of the step is one launch over
-// all cases (SoftCase table on the device), in the single-case step's
+// all cases (SoftCaseT table on the device), in the single-case step's
 // order, so a case's results are the same bits alone and in any mix.  Cases
 // join and leave between runs; the step graphs are captured again after a
 // change.  The extrapolation queue (sweeps launched per step) is the
 // batch's, grown from the most sweeps any case has taken as in the single
 // stepper; the result never depends on it (a longer loop is finished by each
 // case's tail workgroup).
-struct SoftBatch {
+template <typename B>
+struct SoftBatchT {
   struct Entry {
     int id;
-    SoftStepper* st;
+    SoftStepperT<B>* st;
   };
   std::vector<Entry> cases;
   int next_id = 1;
   int max_cases = 0;
-  SoftCase* d_cases = nullptr;
-  std::vector<SoftCase> h_cases;
+  SoftCaseT<B>* d_cases = nullptr;
+  std::vector<SoftCaseT<B>> h_cases;
   SoftBatchDims dims{};
   int sweeps = 12;
   unsigned* need = nullptr;
@@ -669,7 +691,7 @@ struct SoftBatch {
 
   hipError_t init(int cap) {
     max_cases = cap;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_cases), sizeof(SoftCase) * cap);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_cases), sizeof(SoftCaseT<B>) * cap);
     if (e != hipSuccess) return e;
     if ((e = hipMalloc(reinterpret_cast<void**>(&need), sizeof(unsigned))) != hipSuccess) return e;
     if ((e = hipMemset(need, 0, sizeof(unsigned))) != hipSuccess) return e;
@@ -692,7 +714,7 @@ struct SoftBatch {
     chunk_graph = nullptr;
   }
 
-  ~SoftBatch() {
+  ~SoftBatchT() {
     (void)hipDeviceSynchronize();
     drop_graphs();
     for (Entry& c : cases) delete c.st;
@@ -705,8 +727,8 @@ struct SoftBatch {
     if (d_rcnt) (void)hipFree(d_rcnt);
   }
 
-  static SoftCase describe(const SoftStepper& S, unsigned* need) {
-    SoftCase C{};
+  static SoftCaseT<B> describe(const SoftStepperT<B>& S, unsigned* need) {
+    SoftCaseT<B> C{};
     const SoftConfig& g = S.cfg;
     C.nr = g.nr;
     C.nz = g.nz;
@@ -724,7 +746,7 @@ struct SoftBatch {
     C.moll_zone = g.moll_zone;
     C.extrap_zone = g.extrap_zone;
     C.reinit_band = g.reinit_band;
-    const SoftBuffers& b = S.buf;
+    const SoftBuffersT<B>& b = S.buf;
     C.vorticity = b.vorticity;
     C.psi = b.psi;
     C.u_z = b.u_z;
@@ -783,7 +805,7 @@ struct SoftBatch {
     h_cases.clear();
     dims = SoftBatchDims{};
     for (const Entry& c : cases) {
-      const SoftStepper& S = *c.st;
+      const SoftStepperT<B>& S = *c.st;
       h_cases.push_back(describe(S, need));
       const StokesPlan& P = *S.plan;
       dims.max_nr = std::max(dims.max_nr, P.nr);
@@ -793,7 +815,7 @@ struct SoftBatch {
       // per CU than one butterfly per thread (measured 4% faster steps)
       const int half = P.any->args.L / 2;
       dims.any_nth = std::max(dims.any_nth, std::min(1024, (int)(ceil_div(ceil_div(half, 4), 64) * 64)));
-      dims.any_lds = std::max(dims.any_lds, P.any->lds_bytes<double>());
+      dims.any_lds = std::max(dims.any_lds, P.any->lds_bytes<B>());
       dims.tri_lds =
           std::max(dims.tri_lds, tri_lds_bytes(P.nr, kSoftBatchTriL, P.tri_K, P.tri_P, false));
     }
@@ -807,20 +829,20 @@ struct SoftBatch {
     // and over its doubled grid
     std::vector<uint32_t> tab;
     for (int c = 0; c < dims.ncase; ++c) {
-      const SoftCase& C = h_cases[c];
+      const SoftCaseT<B>& C = h_cases[c];
       for (int i = 0; i < C.nr; ++i)
         for (int b = 0; b < (int)ceil_div(C.nz, 256); ++b) tab.push_back(soft_block_entry(c, i, b));
     }
     dims.ncell_blocks = (int)tab.size();
     for (int c = 0; c < dims.ncase; ++c) {
-      const SoftCase& C = h_cases[c];
+      const SoftCaseT<B>& C = h_cases[c];
       for (int i = 0; i < 2 * C.nr; ++i)
         for (int b = 0; b < (int)ceil_div(C.nz, 256); ++b) tab.push_back(soft_block_entry(c, i, b));
     }
     dims.ndbl_blocks = (int)tab.size() - dims.ncell_blocks;
     const size_t tiles0 = tab.size();
     for (int c = 0; c < dims.ncase; ++c) {
-      SoftCase& C = h_cases[c];
+      SoftCaseT<B>& C = h_cases[c];
       C.reinit_tile_base = (int)(tab.size() - tiles0);
       for (int ti = 0; ti < (int)ceil_div(C.nr, kReinitTileRows); ++ti)
         for (int tj = 0; tj < (int)ceil_div(C.nz, kReinitTileCols); ++tj)
@@ -857,13 +879,13 @@ struct SoftBatch {
     dims.cell_blocks = d_blocks;
     dims.dbl_blocks = d_blocks + dims.ncell_blocks;
     dims.reinit_tiles = d_blocks + tiles0;
-    return hipMemcpy(d_cases, h_cases.data(), sizeof(SoftCase) * cases.size(),
+    return hipMemcpy(d_cases, h_cases.data(), sizeof(SoftCaseT<B>) * cases.size(),
                      hipMemcpyHostToDevice);
   }
 
-  hipError_t add(const SoftConfig& c, const SoftBuffers& b, int* id) {
+  hipError_t add(const SoftConfig& c, const SoftBuffersT<B>& b, int* id) {
     if ((int)cases.size() >= max_cases) return hipErrorInvalidValue;
-    SoftStepper* st = new (std::nothrow) SoftStepper();
+    SoftStepperT<B>* st = new (std::nothrow) SoftStepperT<B>();
     if (!st) return hipErrorOutOfMemory;
     SoftConfig cc = c;
     cc.lsq_sweeps = kMaxQueued;  // counters sized for the largest queue
@@ -901,7 +923,7 @@ struct SoftBatch {
   static constexpr int kMaxQueued = SoftStepper::kMaxQueued;
 
   hipError_t enqueue_step(hipStream_t s) {
-    const SoftCase* T = d_cases;
+    const SoftCaseT<B>* T = d_cases;
     const SoftBatchDims& d = dims;
     hipError_t e;
 #define AXS_TRY_STEP(...)                            \
@@ -913,28 +935,28 @@ struct SoftBatch {
     if ((e = hipGetLastError()) != hipSuccess) return e; \
   } while (0)
     int iters = 0;
-    for (const SoftCase& c : h_cases) iters = std::max(iters, c.reinit_iterations);
+    for (const SoftCaseT<B>& c : h_cases) iters = std::max(iters, c.reinit_iterations);
     AXS_TRY_STEP(launch_soft_batch_dct2(T, d, s));
     AXS_TRY_STEP(launch_soft_batch_tridiag(T, d, s));
     AXS_TRY_STEP(launch_soft_batch_dct3(T, d, s));
     AXS_TRY_STEP(launch_soft_batch_velocity(T, d, s));
-    soft_batch_dt_kernel<<<d.ncase, 64, 0, s>>>(T);
+    soft_batch_dt_kernel<B><<<d.ncase, 64, 0, s>>>(T);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_soft_batch_refmap(T, d, s));
-    soft_batch_pin_kernel<<<d.ncell_blocks, 256, 0, s>>>(T, d.cell_blocks);
+    soft_batch_pin_kernel<B><<<d.ncell_blocks, 256, 0, s>>>(T, d.cell_blocks);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_soft_batch_reinit(T, d, iters, s));
-    soft_batch_mirror_kernel<<<d.ndbl_blocks, 256, 0, s>>>(T, d.dbl_blocks, sweeps + 1);
+    soft_batch_mirror_kernel<B><<<d.ndbl_blocks, 256, 0, s>>>(T, d.dbl_blocks, sweeps + 1);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_soft_batch_lsq_list(T, d, s));
     for (int q = 0; q < sweeps; ++q) AXS_TRY_STEP(launch_soft_batch_lsq_sweep(T, d, q, s));
     AXS_TRY_STEP(launch_soft_batch_lsq_tail(T, d, sweeps, s));
-    soft_batch_back_kernel<<<d.ncell_blocks, 256, 0, s>>>(T, d.cell_blocks, sweeps);
+    soft_batch_back_kernel<B><<<d.ncell_blocks, 256, 0, s>>>(T, d.cell_blocks, sweeps);
     AXS_TRY_LAUNCH();
     AXS_TRY_STEP(launch_soft_batch_stress(T, d, s));
     AXS_TRY_STEP(launch_soft_batch_advect(T, d, s));
     AXS_TRY_STEP(launch_soft_batch_diffuse(T, d, s));
-    soft_batch_finalize_kernel<<<d.ncase, 64, 0, s>>>(T, sweeps);
+    soft_batch_finalize_kernel<B><<<d.ncase, 64, 0, s>>>(T, sweeps);
     AXS_TRY_LAUNCH();
 #undef AXS_TRY_STEP
 #undef AXS_TRY_LAUNCH
@@ -1015,17 +1037,31 @@ struct SoftBatch {
 
 using namespace axs;
 
+// A soft batch of one storage type: FP64 (axs_soft_batch_create) or FP32
+// bulk (axs_soft_batch_create_f32); the other member stays empty.
 struct axs_soft_batch {
-  SoftBatch impl;
+  bool f32 = false;
+  SoftBatchT<double> d;
+  SoftBatchT<float> f;
+  template <typename F>
+  auto with(F&& fn) {
+    return f32 ? fn(f) : fn(d);
+  }
+  template <typename F>
+  auto with(F&& fn) const {
+    return f32 ? fn(f) : fn(d);
+  }
 };
 
 struct axs_soft_stepper {
   SoftStepper impl;
 };
 
-// Checked copy of a soft configuration and its buffers.
-static axs_status soft_config_from(const axs_soft_config* c, const axs_soft_buffers* b,
-                                   SoftConfig& cfg, SoftBuffers& bb) {
+// Checked copy of a soft configuration and its buffers (axs_soft_buffers or
+// axs_soft_buffers_f32).
+template <class In, typename B>
+static axs_status soft_config_from(const axs_soft_config* c, const In* b, SoftConfig& cfg,
+                                   SoftBuffersT<B>& bb) {
   AXS_CHECK_ARG(c && b, "null pointer");
   AXS_CHECK_ARG(c->nr >= 4 && c->nz >= 8 && c->nz % 2 == 0, "need nr >= 4 and even nz >= 8");
   AXS_CHECK_ARG(c->nr <= 4096 && stokes_nz_supported(c->nz),
@@ -1075,8 +1111,49 @@ static axs_status soft_config_from(const axs_soft_config* c, const axs_soft_buff
   }
   if (cfg.reinit_iterations < 0) cfg.reinit_iterations = reinit_iterations(cfg.reinit_band, cfg.dx);
   AXS_CHECK_ARG(cfg.reinit_iterations <= 1 << 16, "too many reinit iterations");
-  bb = SoftBuffers{b->vorticity, b->psi,     b->u_z, b->u_r, b->eta1,    b->eta2,   b->phi,
-                   b->char_func, b->scalars, b->r,   b->z,   b->kill_wz, b->kill_wr};
+  bb = SoftBuffersT<B>{b->vorticity, b->psi,     b->u_z, b->u_r, b->eta1,    b->eta2,   b->phi,
+                       b->char_func, b->scalars, b->r,   b->z,   b->kill_wz, b->kill_wr};
+  return AXS_OK;
+}
+
+static axs_status soft_batch_create(int max_cases, bool f32, axs_soft_batch** out) {
+  AXS_CHECK_ARG(out, "null pointer");
+  *out = nullptr;
+  AXS_CHECK_ARG(max_cases >= 1 && max_cases <= kSoftBatchMaxCases,
+                "max_cases must be in [1, 1024]");
+  axs_soft_batch* bt = new (std::nothrow) axs_soft_batch();
+  if (!bt) {
+    set_error("out of host memory");
+    return AXS_ERR_ALLOC;
+  }
+  bt->f32 = f32;
+  hipError_t e = bt->with([&](auto& b) { return b.init(max_cases); });
+  if (e != hipSuccess) {
+    set_error(std::string("soft batch: ") + hipGetErrorString(e));
+    delete bt;
+    return AXS_ERR_HIP;
+  }
+  *out = bt;
+  return AXS_OK;
+}
+
+template <typename B, class In>
+static axs_status soft_batch_add(SoftBatchT<B>& bt, const axs_soft_config* c, const In* b,
+                                 int* case_id) {
+  AXS_CHECK_ARG(case_id, "null pointer");
+  AXS_CHECK_ARG((int)bt.cases.size() < bt.max_cases, "the batch is full");
+  SoftConfig cfg;
+  SoftBuffersT<B> bb;
+  const axs_status st0 = soft_config_from(c, b, cfg, bb);
+  if (st0 != AXS_OK) return st0;
+  AXS_CHECK_ARG(!cfg.streaming, "the soft batch steps the Taylor-Green loop (streaming = 0)");
+  AXS_CHECK_ARG(any_dft_length(cfg.nz, nullptr) <= kAnyMaxL && cfg.nr <= 1024 * kSoftBatchTriL,
+                "grid too large for a soft batch");
+  hipError_t e = bt.add(cfg, bb, case_id);
+  if (e != hipSuccess) {
+    set_error(std::string("soft batch add: ") + hipGetErrorString(e));
+    return AXS_ERR_HIP;
+  }
   return AXS_OK;
 }
 
@@ -1106,50 +1183,36 @@ axs_status axs_soft_stepper_create(const axs_soft_config* c, const axs_soft_buff
 }
 
 axs_status axs_soft_batch_create(int max_cases, axs_soft_batch** out) {
-  AXS_CHECK_ARG(out, "null pointer");
-  *out = nullptr;
-  AXS_CHECK_ARG(max_cases >= 1 && max_cases <= kSoftBatchMaxCases,
-                "max_cases must be in [1, 1024]");
-  axs_soft_batch* bt = new (std::nothrow) axs_soft_batch();
-  if (!bt) {
-    set_error("out of host memory");
-    return AXS_ERR_ALLOC;
-  }
-  hipError_t e = bt->impl.init(max_cases);
-  if (e != hipSuccess) {
-    set_error(std::string("soft batch: ") + hipGetErrorString(e));
-    delete bt;
-    return AXS_ERR_HIP;
-  }
-  *out = bt;
-  return AXS_OK;
+  return soft_batch_create(max_cases, false, out);
+}
+
+axs_status axs_soft_batch_create_f32(int max_cases, axs_soft_batch** out) {
+  return soft_batch_create(max_cases, true, out);
 }
 
 axs_status axs_soft_batch_add(axs_soft_batch* bt, const axs_soft_config* c,
                               const axs_soft_buffers* b, int* case_id) {
-  AXS_CHECK_ARG(bt && case_id, "null pointer");
-  AXS_CHECK_ARG((int)bt->impl.cases.size() < bt->impl.max_cases, "the batch is full");
-  SoftConfig cfg;
-  SoftBuffers bb;
-  const axs_status st0 = soft_config_from(c, b, cfg, bb);
-  if (st0 != AXS_OK) return st0;
-  AXS_CHECK_ARG(!cfg.streaming, "the soft batch steps the Taylor-Green loop (streaming = 0)");
-  AXS_CHECK_ARG(any_dft_length(cfg.nz, nullptr) <= kAnyMaxL && cfg.nr <= 1024 * kSoftBatchTriL,
-                "grid too large for a soft batch");
-  hipError_t e = bt->impl.add(cfg, bb, case_id);
-  if (e != hipSuccess) {
-    set_error(std::string("soft batch add: ") + hipGetErrorString(e));
-    return AXS_ERR_HIP;
-  }
-  return AXS_OK;
+  AXS_CHECK_ARG(bt, "null batch");
+  AXS_CHECK_ARG(!bt->f32, "an FP32-bulk batch takes axs_soft_batch_add_f32");
+  return soft_batch_add(bt->d, c, b, case_id);
+}
+
+axs_status axs_soft_batch_add_f32(axs_soft_batch* bt, const axs_soft_config* c,
+                                  const axs_soft_buffers_f32* b, int* case_id) {
+  AXS_CHECK_ARG(bt, "null batch");
+  AXS_CHECK_ARG(bt->f32, "an FP64 batch takes axs_soft_batch_add");
+  return soft_batch_add(bt->f, c, b, case_id);
 }
 
 axs_status axs_soft_batch_remove(axs_soft_batch* bt, int case_id) {
   AXS_CHECK_ARG(bt, "null batch");
-  bool known = false;
-  for (const SoftBatch::Entry& c : bt->impl.cases) known = known || c.id == case_id;
+  const bool known = bt->with([&](const auto& b) {
+    for (const auto& c : b.cases)
+      if (c.id == case_id) return true;
+    return false;
+  });
   AXS_CHECK_ARG(known, "no such case in the batch");
-  AXS_HIP(bt->impl.remove(case_id));
+  AXS_HIP(bt->with([&](auto& b) { return b.remove(case_id); }));
   return AXS_OK;
 }
 
@@ -1157,17 +1220,20 @@ axs_status axs_soft_batch_run(axs_soft_batch* bt, int nsteps, int use_graph, voi
   AXS_CHECK_ARG(bt, "null batch");
   AXS_CHECK_ARG(nsteps >= 0, "negative step count");
   AXS_CHECK_ARG(!use_graph || stream, "graph capture needs a non-null stream");
-  AXS_HIP(bt->impl.run(nsteps, use_graph != 0, as_stream(stream)));
+  AXS_HIP(bt->with([&](auto& b) { return b.run(nsteps, use_graph != 0, as_stream(stream)); }));
   return AXS_OK;
 }
 
 axs_status axs_soft_batch_info(const axs_soft_batch* bt, int* ncase, int* queued_sweeps,
                                int64_t* graph_launches, int64_t* captures) {
   AXS_CHECK_ARG(bt, "null batch");
-  if (ncase) *ncase = (int)bt->impl.cases.size();
-  if (queued_sweeps) *queued_sweeps = bt->impl.sweeps;
-  if (graph_launches) *graph_launches = bt->impl.chunk_launches + bt->impl.step_launches;
-  if (captures) *captures = bt->impl.captures;
+  bt->with([&](const auto& b) {
+    if (ncase) *ncase = (int)b.cases.size();
+    if (queued_sweeps) *queued_sweeps = b.sweeps;
+    if (graph_launches) *graph_launches = b.chunk_launches + b.step_launches;
+    if (captures) *captures = b.captures;
+    return 0;
+  });
   return AXS_OK;
 }
 

@@ -6,8 +6,17 @@
 // passes take a block table (256 cells of one row of one case per
 // workgroup), the row transforms and the radial solve a grid sized for the
 // largest case (blockIdx.y = case; workgroups beyond a case's rows or mode
-// chunks exit at once), and the per-case passes one workgroup per case.  A case's arithmetic is the single-case step's, so a case gives the
-// same bits alone and beside others.
+// chunks exit at once), and the per-case passes one workgroup per case.  A
+// case's arithmetic is the single-case step's, so a case gives the same
+// bits alone and beside others.
+//
+// A batch stores its flow fields (vorticity, psi, velocity, the
+// spectrum, the stress scratch and the solid mask) in one type B: double
+// for the FP64 step, float for the FP32-bulk step.  Both keep FP64 for the
+// radial elimination (its arithmetic; the spectrum is stored in B), the
+// reference map, the level set, its reinit, the least-squares
+// extrapolation and the scalars: the interface's thresholds decide the
+// mask and the extrapolation's cells, and those stay the FP64 ones.
 #pragma once
 
 #include "axs_anydft.hpp"
@@ -33,16 +42,19 @@ enum : int {
   kSsCount
 };
 
-struct SoftCase {
+template <typename B>
+struct SoftCaseT {
   int nr, nz, kill_width, reinit_iterations;
   double dx, nu, G, rho_f, CFL, eps, r_ball, z_cm, r_cm, moll_zone, extrap_zone, reinit_band;
   // caller-owned state
-  double *vorticity, *psi, *u_z, *u_r, *eta1, *eta2, *phi, *chi, *scalars;
+  B *vorticity, *psi, *u_z, *u_r, *chi;
+  double *eta1, *eta2, *phi, *scalars;
   const double *r, *z, *kill_wz, *kill_wr;
   // step scratch
-  double *w0, *w1, *tmp, *spec, *e1n, *e2n, *slots, *e1d, *e2d;
+  B *w0, *w1, *tmp, *spec;
+  double *e1n, *e2n, *slots, *e1d, *e2d;
   int16_t *cur, *tgt;
-  double *s11, *s12, *s22, *g1z, *g1r, *g2z, *g2r, *tz, *tr;
+  B *s11, *s12, *s22, *g1z, *g1r, *g2z, *g2r, *tz, *tr;
   ReinitWork rw;
   int reinit_tile_base;  // the case's first entry in the batch's reinit tile table
   LsqWork lw;
@@ -55,6 +67,7 @@ struct SoftCase {
   const double *ta, *tb, *tc, *tlam, *tcseg, *rvec;
   int tK, tP;
 };
+using SoftCase = SoftCaseT<double>;
 
 // Launch shapes of a batch (host side): the largest case's dimensions, the
 // any-length transform's block shape over all cases, and the block tables
@@ -98,24 +111,43 @@ __device__ __forceinline__ SoftBlock soft_block(const uint32_t* __restrict__ t) 
 // is compiled for one L).
 constexpr int kSoftBatchTriL = 8;
 
-// Passes in other translation units (one launch each over all cases).
-hipError_t launch_soft_batch_dct2(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s);
-hipError_t launch_soft_batch_dct3(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s);
-hipError_t launch_soft_batch_tridiag(const SoftCase* cases, const SoftBatchDims& d,
+// Passes in other translation units (one launch each over all cases),
+// instantiated for B = double and B = float.
+template <typename B>
+hipError_t launch_soft_batch_dct2(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                  hipStream_t s);
+template <typename B>
+hipError_t launch_soft_batch_dct3(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                  hipStream_t s);
+template <typename B>
+hipError_t launch_soft_batch_tridiag(const SoftCaseT<B>* cases, const SoftBatchDims& d,
                                      hipStream_t s);
-hipError_t launch_soft_batch_velocity(const SoftCase* cases, const SoftBatchDims& d,
+template <typename B>
+hipError_t launch_soft_batch_velocity(const SoftCaseT<B>* cases, const SoftBatchDims& d,
                                       hipStream_t s);
-hipError_t launch_soft_batch_refmap(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s);
-hipError_t launch_soft_batch_reinit(const SoftCase* cases, const SoftBatchDims& d, int iterations,
+template <typename B>
+hipError_t launch_soft_batch_refmap(const SoftCaseT<B>* cases, const SoftBatchDims& d,
                                     hipStream_t s);
-hipError_t launch_soft_batch_lsq_list(const SoftCase* cases, const SoftBatchDims& d,
+template <typename B>
+hipError_t launch_soft_batch_reinit(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                    int iterations, hipStream_t s);
+template <typename B>
+hipError_t launch_soft_batch_lsq_list(const SoftCaseT<B>* cases, const SoftBatchDims& d,
                                       hipStream_t s);
-hipError_t launch_soft_batch_lsq_sweep(const SoftCase* cases, const SoftBatchDims& d, int sweep,
-                                       hipStream_t s);
-hipError_t launch_soft_batch_lsq_tail(const SoftCase* cases, const SoftBatchDims& d, int sweeps,
-                                      hipStream_t s);
-hipError_t launch_soft_batch_stress(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s);
-hipError_t launch_soft_batch_advect(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s);
-hipError_t launch_soft_batch_diffuse(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s);
+template <typename B>
+hipError_t launch_soft_batch_lsq_sweep(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                       int sweep, hipStream_t s);
+template <typename B>
+hipError_t launch_soft_batch_lsq_tail(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                      int sweeps, hipStream_t s);
+template <typename B>
+hipError_t launch_soft_batch_stress(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                    hipStream_t s);
+template <typename B>
+hipError_t launch_soft_batch_advect(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                    hipStream_t s);
+template <typename B>
+hipError_t launch_soft_batch_diffuse(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                     hipStream_t s);
 
 }  // namespace axs

@@ -242,44 +242,47 @@ template hipError_t launch_rk2<float, RRow<float>>(const float*, float*, float*,
 // ---------------------------------------------------------------- A11
 // solid_sigma.py:4-29.  Gradient scratch entries the reference never writes
 // keep their previous contents and still enter sigma.
-template <typename T>
-__device__ __forceinline__ void solid_sigma_at(T* __restrict__ s11, T* __restrict__ s12,
-                                               T* __restrict__ s22, T G, T dx,
+// The reference map and the arithmetic in T; the gradient scratch, the
+// mask and the stresses stored in O (O = T but in the FP32-bulk soft batch,
+// whose reference map stays FP64).
+template <typename T, typename O = T>
+__device__ __forceinline__ void solid_sigma_at(O* __restrict__ s11, O* __restrict__ s12,
+                                               O* __restrict__ s22, T G, T dx,
                                                const T* __restrict__ e1, const T* __restrict__ e2,
-                                               T* __restrict__ e1z, T* __restrict__ e1r,
-                                               T* __restrict__ e2z, T* __restrict__ e2r, int nr,
-                                               int nz, const T* __restrict__ chi, int i, int j) {
+                                               O* __restrict__ e1z, O* __restrict__ e1r,
+                                               O* __restrict__ e2z, O* __restrict__ e2r, int nr,
+                                               int nz, const O* __restrict__ chi, int i, int j) {
   const int64_t c = (int64_t)i * nz + j;
   const T two_dx = T(2) * dx;
-  T a1z = e1z[c], a2z = e2z[c], a1r = e1r[c], a2r = e2r[c];
+  T a1z = T(e1z[c]), a2z = T(e2z[c]), a1r = T(e1r[c]), a2r = T(e2r[c]);
   if (j >= 1 && j <= nz - 2) {
     a1z = (e1[c + 1] - e1[c - 1]) / two_dx;
     a2z = (e2[c + 1] - e2[c - 1]) / two_dx;
-    e1z[c] = a1z;
-    e2z[c] = a2z;
+    e1z[c] = O(a1z);
+    e2z[c] = O(a2z);
   }
   if (i == 0) {
     a1r = (-e1[c + 2 * nz] + T(4) * e1[c + nz] - T(3) * e1[c]) / two_dx;
     a2r = (-e2[c + 2 * nz] + T(4) * e2[c + nz] - T(3) * e2[c]) / two_dx;
-    e1r[c] = a1r;
-    e2r[c] = a2r;
+    e1r[c] = O(a1r);
+    e2r[c] = O(a2r);
   } else if (i <= nr - 2 && j >= 1 && j <= nz - 2) {
     a1r = (e1[c + nz] - e1[c - nz]) / two_dx;
     a2r = (e2[c + nz] - e2[c - nz]) / two_dx;
-    e1r[c] = a1r;
-    e2r[c] = a2r;
+    e1r[c] = O(a1r);
+    e2r[c] = O(a2r);
   }
   const T t12 = -G * (a1z * a1r + a2z * a2r);
   const T s = (T(0.5) * G) * (a1r * a1r + a2r * a2r - a1z * a1z - a2z * a2z);
   if (chi) {
-    const T x = chi[c];
-    s12[c] = x * t12;
-    s11[c] = x * s;
-    s22[c] = x * -s;
+    const T x = T(chi[c]);
+    s12[c] = O(x * t12);
+    s11[c] = O(x * s);
+    s22[c] = O(x * -s);
   } else {
-    s12[c] = t12;
-    s11[c] = s;
-    s22[c] = -s;
+    s12[c] = O(t12);
+    s11[c] = O(s);
+    s22[c] = O(-s);
   }
 }
 
@@ -353,17 +356,18 @@ __global__ void div_tau_stage2_kernel(T* __restrict__ w, const T* __restrict__ t
 
 // u from psi, and the step's max(|u_z| + |u_r|) folded per workgroup into the
 // case's spread atomicMax slots (the maximum does not depend on the order).
+template <typename B>
 __global__ void __launch_bounds__(256) soft_batch_velocity_kernel(
-    const SoftCase* __restrict__ cs, const uint32_t* __restrict__ blocks) {
+    const SoftCaseT<B>* __restrict__ cs, const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   const int i = b.i, j = b.j;
   double m = 0.0;
   if (j < C.nz) {
-    double uz, ur;
-    velocity_from_psi_at<double, RRow<double>>(C.u_z, C.u_r, C.psi, RRow<double>{C.r}, C.nr,
-                                               C.nz, C.dx, i, j, uz, ur);
-    m = fabs(uz) + fabs(ur);
+    B uz, ur;
+    velocity_from_psi_at<B, RRow<B>>(C.u_z, C.u_r, C.psi, RRow<B>{C.r}, C.nr, C.nz, B(C.dx), i,
+                                     j, uz, ur);
+    m = double(fabs(uz) + fabs(ur));
   }
   __shared__ double wm[4];
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
@@ -377,78 +381,100 @@ __global__ void __launch_bounds__(256) soft_batch_velocity_kernel(
   }
 }
 
-__global__ void soft_batch_sigma_kernel(const SoftCase* __restrict__ cs,
+// Stresses from the FP64 reference map, stored in the batch's type.
+template <typename B>
+__global__ void soft_batch_sigma_kernel(const SoftCaseT<B>* __restrict__ cs,
                                         const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   const int i = b.i, j = b.j;
   if (j >= C.nz) return;
-  solid_sigma_at<double>(C.s11, C.s12, C.s22, C.G, C.dx, C.eta1, C.eta2, C.g1z, C.g1r, C.g2z,
-                         C.g2r, C.nr, C.nz, C.chi, i, j);
+  solid_sigma_at<double, B>(C.s11, C.s12, C.s22, C.G, C.dx, C.eta1, C.eta2, C.g1z, C.g1r, C.g2z,
+                            C.g2r, C.nr, C.nz, C.chi, i, j);
 }
 
-__global__ void soft_batch_div_tau1_kernel(const SoftCase* __restrict__ cs,
+template <typename B>
+__global__ void soft_batch_div_tau1_kernel(const SoftCaseT<B>* __restrict__ cs,
                                            const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   const int i = b.i, j = b.j + 1;
   if (i >= C.nr - 1 || j >= C.nz - 1) return;
-  div_tau_stage1_at<double, RRow<double>>(C.tz, C.tr, C.s11, C.s12, C.s22, RRow<double>{C.r},
-                                          C.nz, 2.0 * C.dx, i, j);
+  div_tau_stage1_at<B, RRow<B>>(C.tz, C.tr, C.s11, C.s12, C.s22, RRow<B>{C.r}, C.nz,
+                                B(2.0 * C.dx), i, j);
 }
 
-__global__ void soft_batch_div_tau2_kernel(const SoftCase* __restrict__ cs,
+template <typename B>
+__global__ void soft_batch_div_tau2_kernel(const SoftCaseT<B>* __restrict__ cs,
                                            const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   const int i = b.i, j = b.j + 1;
   if (i < 1 || i >= C.nr - 1 || j >= C.nz - 1) return;
-  div_tau_stage2_at<double>(C.w0, C.tz, C.tr, C.nz, C.scalars[kSsDt], 2.0 * C.dx, i, j);
+  div_tau_stage2_at<B>(C.w0, C.tz, C.tr, C.nz, B(C.scalars[kSsDt]), B(2.0 * C.dx), i, j);
 }
 
-__global__ void soft_batch_rk2_stage1_kernel(const SoftCase* __restrict__ cs,
+template <typename B>
+__global__ void soft_batch_rk2_stage1_kernel(const SoftCaseT<B>* __restrict__ cs,
                                              const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   const int i = b.i, j = b.j;
   if (j >= C.nz) return;
-  rk2_stage1_at<double, RRow<double>>(C.w1, C.tmp, RRow<double>{C.r}, C.nr, C.nz,
-                                      C.scalars[kSsHalfNuDt], C.dx * C.dx, 2.0 * C.dx, i, j);
+  rk2_stage1_at<B, RRow<B>>(C.w1, C.tmp, RRow<B>{C.r}, C.nr, C.nz, B(C.scalars[kSsHalfNuDt]),
+                            B(C.dx * C.dx), B(2.0 * C.dx), i, j);
 }
 
-__global__ void soft_batch_rk2_stage2_kernel(const SoftCase* __restrict__ cs,
+template <typename B>
+__global__ void soft_batch_rk2_stage2_kernel(const SoftCaseT<B>* __restrict__ cs,
                                              const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);
-  const SoftCase& C = cs[b.c];
+  const SoftCaseT<B>& C = cs[b.c];
   const int i = b.i, j = b.j;
   if (j >= C.nz) return;
-  rk2_stage2_out_at<double, RRow<double>>(C.w1, C.tmp, C.vorticity, RRow<double>{C.r}, C.nr, C.nz,
-                                          C.scalars[kSsNuDt], C.dx * C.dx, 2.0 * C.dx, i, j);
+  rk2_stage2_out_at<B, RRow<B>>(C.w1, C.tmp, C.vorticity, RRow<B>{C.r}, C.nr, C.nz,
+                                B(C.scalars[kSsNuDt]), B(C.dx * C.dx), B(2.0 * C.dx), i, j);
 }
 
-hipError_t launch_soft_batch_velocity(const SoftCase* cases, const SoftBatchDims& d,
+template <typename B>
+hipError_t launch_soft_batch_velocity(const SoftCaseT<B>* cases, const SoftBatchDims& d,
                                       hipStream_t s) {
-  soft_batch_velocity_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
+  soft_batch_velocity_kernel<B><<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   return hipGetLastError();
 }
 
-hipError_t launch_soft_batch_stress(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
-  soft_batch_sigma_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
+template <typename B>
+hipError_t launch_soft_batch_stress(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                    hipStream_t s) {
+  soft_batch_sigma_kernel<B><<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  soft_batch_div_tau1_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
+  soft_batch_div_tau1_kernel<B><<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  soft_batch_div_tau2_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
+  soft_batch_div_tau2_kernel<B><<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   return hipGetLastError();
 }
 
-hipError_t launch_soft_batch_diffuse(const SoftCase* cases, const SoftBatchDims& d, hipStream_t s) {
-  soft_batch_rk2_stage1_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
+template <typename B>
+hipError_t launch_soft_batch_diffuse(const SoftCaseT<B>* cases, const SoftBatchDims& d,
+                                     hipStream_t s) {
+  soft_batch_rk2_stage1_kernel<B><<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  soft_batch_rk2_stage2_kernel<<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
+  soft_batch_rk2_stage2_kernel<B><<<d.ncell_blocks, 256, 0, s>>>(cases, d.cell_blocks);
   return hipGetLastError();
 }
+
+#define AXS_SOFT_BATCH_INST(B)                                                              \
+  template hipError_t launch_soft_batch_velocity<B>(const SoftCaseT<B>*, const SoftBatchDims&, \
+                                                    hipStream_t);                           \
+  template hipError_t launch_soft_batch_stress<B>(const SoftCaseT<B>*, const SoftBatchDims&,   \
+                                                  hipStream_t);                             \
+  template hipError_t launch_soft_batch_diffuse<B>(const SoftCaseT<B>*, const SoftBatchDims&,  \
+                                                   hipStream_t);
+AXS_SOFT_BATCH_INST(double)
+AXS_SOFT_BATCH_INST(float)
+#undef AXS_SOFT_BATCH_INST
 
 // ---------------------------------------------------------------- launchers
 // (device-resident steps pass dt-derived scalars from device memory)

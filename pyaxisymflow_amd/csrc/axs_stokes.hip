@@ -1261,20 +1261,27 @@ __global__ void __launch_bounds__(1024) tridiag_ragged_kernel(T* __restrict__ d,
 }
 
 // Soft batches: mode chunk blockIdx.x of case blockIdx.y, segments of
-// kSoftBatchTriL rows (every case's plan is made for that L).
-__global__ void __launch_bounds__(1024) soft_batch_tridiag_kernel(const SoftCase* __restrict__ cs) {
-  const SoftCase& C = cs[blockIdx.y];
+// kSoftBatchTriL rows (every case's plan is made for that L); the spectrum
+// in the batch's storage type, the elimination in FP64.
+template <typename B>
+__global__ void __launch_bounds__(1024) soft_batch_tridiag_kernel(
+    const SoftCaseT<B>* __restrict__ cs) {
+  const SoftCaseT<B>& C = cs[blockIdx.y];
   if ((int)blockIdx.x * C.tK >= C.nz) return;
-  tridiag_columns_body<double, kSoftBatchTriL, false, false>(C.spec, C.ta, C.tb, C.tc, C.tlam,
-                                                             C.tcseg, C.nr, C.nz, C.tK, C.tP,
-                                                             blockIdx.x);
+  tridiag_columns_body<B, kSoftBatchTriL, false, false>(C.spec, C.ta, C.tb, C.tc, C.tlam, C.tcseg,
+                                                        C.nr, C.nz, C.tK, C.tP, blockIdx.x);
 }
 
-hipError_t launch_soft_batch_tridiag(const SoftCase* cases, const SoftBatchDims& d,
+template <typename B>
+hipError_t launch_soft_batch_tridiag(const SoftCaseT<B>* cases, const SoftBatchDims& d,
                                      hipStream_t s) {
-  soft_batch_tridiag_kernel<<<dim3(d.max_chunks, d.ncase), 1024, d.tri_lds, s>>>(cases);
+  soft_batch_tridiag_kernel<B><<<dim3(d.max_chunks, d.ncase), 1024, d.tri_lds, s>>>(cases);
   return hipGetLastError();
 }
+template hipError_t launch_soft_batch_tridiag<double>(const SoftCaseT<double>*,
+                                                      const SoftBatchDims&, hipStream_t);
+template hipError_t launch_soft_batch_tridiag<float>(const SoftCaseT<float>*,
+                                                     const SoftBatchDims&, hipStream_t);
 
 // ------------------------------------------------------------------ launchers
 

@@ -13,6 +13,11 @@ remaining run (at most `chunk` steps), so no request takes a step beyond its
 own count.  A case's arithmetic does not depend on the other cases, so a
 request's results are the same bits alone and in any mix.
 
+precision "f32" (the default) serves with the FP32-bulk soft batch: the
+flow fields and their passes in FP32, the radial elimination, the
+reference map, the level set and the extrapolation in FP64; "f64" runs
+every pass in FP64, as the reference's loop does.
+
 Time-to-solution of a request = completion time - arrival time (queueing,
 set-up, and stepping), measured on the host clock at the end of the chunk
 that completes it.
@@ -41,12 +46,15 @@ def request_set(n, seed=0, rate=1.0, steps=2000, nr_range=(128, 1024)):
     return out
 
 
-def footprint_bytes(req):
-    """Device bytes of one soft request: 9 caller-owned (nr, nz) FP64 fields,
-    18 scratch fields, the doubled grid (eta x2, LS fits x2: FP64; flags x2:
-    int16; pending: int8) and the spectral tables."""
+def footprint_bytes(req, precision="f64"):
+    """Device bytes of one soft request: 27 (nr, nz) fields (caller-owned
+    and scratch; in "f32" the 18 flow fields -- vorticity, psi, u_z, u_r,
+    char_func and 13 scratch -- are float32, the rest float64), the
+    doubled grid (eta x2, LS fits x2: FP64; flags x2: int16; pending: int8)
+    and the spectral tables."""
     n = req["nr"] * req["grid_size_z"]
-    return 8 * n * (9 + 18) + 2 * n * (4 * 8 + 2 * 2 + 1) + 8 * n
+    fields = 4 * n * 18 + 8 * n * 9 if precision == "f32" else 8 * n * (9 + 18)
+    return fields + 2 * n * (4 * 8 + 2 * 2 + 1) + 8 * n
 
 
 class SoftServer:
@@ -54,9 +62,10 @@ class SoftServer:
 
     max_active: slots (concurrent requests); hbm_budget: bytes the active
     requests may hold (default: 80% of free device memory); chunk: most
-    steps run between scheduling decisions."""
+    steps run between scheduling decisions; precision: "f32" (FP32 bulk) or
+    "f64"."""
 
-    def __init__(self, max_active=64, hbm_budget=None, chunk=100):
+    def __init__(self, max_active=64, hbm_budget=None, chunk=100, precision="f32"):
         torch = _lib.torch()
         dev = _lib.device()
         if hbm_budget is None:
@@ -65,6 +74,9 @@ class SoftServer:
         self.max_active = int(max_active)
         self.hbm_budget = int(hbm_budget)
         self.chunk = int(chunk)
+        if precision not in ("f32", "f64"):
+            raise ValueError("precision must be 'f32' or 'f64'")
+        self.precision = precision
 
     def run(self, requests, keep_state=False):
         """Serves `requests` (dicts from request_set) with arrivals relative
@@ -75,11 +87,15 @@ class SoftServer:
         from pyaxisymflow_amd.stepper import SoftSphereBatch
 
         pending = sorted(requests, key=lambda r: r["arrival"])
+
+        def fp(r):
+            return footprint_bytes(r, self.precision)
+
         for r in pending:
-            if footprint_bytes(r) > self.hbm_budget:
-                raise ValueError(f"request {r['id']} needs {footprint_bytes(r)} bytes, more "
+            if fp(r) > self.hbm_budget:
+                raise ValueError(f"request {r['id']} needs {fp(r)} bytes, more "
                                  f"than the budget {self.hbm_budget}")
-        batch = SoftSphereBatch(max_cases=self.max_active)
+        batch = SoftSphereBatch(max_cases=self.max_active, precision=self.precision)
         self.last_batch_info = None
         active, records, used = [], {}, 0
         t0 = time.perf_counter()
@@ -87,11 +103,11 @@ class SoftServer:
             now = time.perf_counter() - t0
             # admission: arrived, slot free, memory fits (first come first served)
             while (pending and pending[0]["arrival"] <= now and len(active) < self.max_active
-                   and used + footprint_bytes(pending[0]) <= self.hbm_budget):
+                   and used + fp(pending[0]) <= self.hbm_budget):
                 r = pending.pop(0)
                 case = batch.add(grid_size_z=r["grid_size_z"], G=r["G"], nu=r["nu"],
                                  rho_f=r["rho_f"], r_ball=r["r_ball"])
-                used += footprint_bytes(r)
+                used += fp(r)
                 active.append(dict(req=r, case=case, done=0,
                                    admitted=time.perf_counter() - t0))
             if not active:
@@ -120,7 +136,7 @@ class SoftServer:
                     rec["state"] = dict(vorticity=c.vorticity(), psi=c.psi(), eta1=e1, eta2=e2,
                                         phi=c.phi(), scalars=c.scalars())
                 records[r["id"]] = rec
-                used -= footprint_bytes(r)
+                used -= fp(r)
                 batch.remove(c)
                 active.remove(a)
         self.last_batch_info = batch.info()

@@ -527,10 +527,12 @@ SS_T, SS_DT, SS_DT_BY_DX, SS_HALF_NU_DT, SS_NU_DT, SS_MAXU, SS_IT, SS_LSQ_SWEEPS
 
 
 def _soft_case_state(obj, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_width,
-                     moll_cells=4):
+                     moll_cells=4, precision="f64"):
     """Device state, configuration and buffer table of one soft-sphere run
     (examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:27-98),
-    set as attributes of obj."""
+    set as attributes of obj.  precision "f32": the flow fields (vorticity,
+    psi, u_z, u_r, char_func) are float32 (the FP32-bulk soft batch); the
+    reference map, level set and scalars stay float64."""
     torch = _lib.torch()
     dev = _lib.device()
     dx = 1.0 / grid_size_z
@@ -546,20 +548,26 @@ def _soft_case_state(obj, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_
     kz = kr = 2 * np.pi
     w = C * (kz**2 + kr**2) * np.sin(kz * Z) * np.sin(kr * R)
 
-    def t(a):
-        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    if precision not in ("f64", "f32"):
+        raise ValueError("precision must be 'f64' or 'f32'")
+    flow_dtype = np.float32 if precision == "f32" else np.float64
 
-    obj.t_vorticity = t(w)
-    obj.t_psi = t(0 * Z)
-    obj.t_u_z = t(0 * Z)
-    obj.t_u_r = t(0 * Z)
+    def t(a, dtype=np.float64):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).to(dev)
+
+    obj.precision = precision
+    obj.t_vorticity = t(w, flow_dtype)
+    obj.t_psi = t(0 * Z, flow_dtype)
+    obj.t_u_z = t(0 * Z, flow_dtype)
+    obj.t_u_r = t(0 * Z, flow_dtype)
     obj.t_eta1 = t(Z)
     obj.t_eta2 = t(R)
     obj.t_phi = t(phi)
-    obj.t_char_func = t(0 * Z)
     from pyaxisymflow_amd.kernels.smooth_Heaviside import smooth_Heaviside
 
-    smooth_Heaviside(obj.t_char_func, obj.t_phi, moll_zone)
+    chi = t(0 * Z)
+    smooth_Heaviside(chi, obj.t_phi, moll_zone)
+    obj.t_char_func = chi.to(torch.float32) if precision == "f32" else chi
     obj.t_scalars = torch.zeros(N_SCALARS, dtype=torch.float64, device=dev)
     obj.t_r, obj.t_z = t(r), t(z)
     wz, wr = kill_weights(z, r, dx, kill_width)
@@ -573,7 +581,7 @@ def _soft_case_state(obj, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_
     cfg.r_ball, cfg.z_cm, cfg.r_cm = r_ball, 0.5, 0.0
     cfg.moll_zone, cfg.extrap_zone, cfg.reinit_band = moll_zone, extrap_zone, extrap_zone
     obj.cfg = cfg
-    bufs = _lib.SoftBuffers()
+    bufs = _lib.SoftBuffersF32() if precision == "f32" else _lib.SoftBuffers()
     for name in ("vorticity", "psi", "u_z", "u_r", "eta1", "eta2", "phi", "char_func", "scalars",
                  "r", "z"):
         setattr(bufs, name, getattr(obj, "t_" + name).data_ptr())
@@ -733,15 +741,24 @@ class SoftSphereBatch:
     step one launch over all of them (axs_soft_batch_*).  ``add`` takes the
     SoftSphereStepper parameters and returns a case whose fields can be read
     like the stepper's; ``remove`` drops it.  A case's results are the same
-    bits alone and in any mix; cases join and leave between ``step`` calls."""
+    bits alone and in any mix; cases join and leave between ``step`` calls.
 
-    def __init__(self, max_cases=64, use_graph=True):
+    precision "f32" is the FP32-bulk batch (axs_soft_batch_create_f32): the
+    flow fields and their passes in float32, the radial elimination, the
+    reference map, the level set, its reinit and the extrapolation in
+    float64."""
+
+    def __init__(self, max_cases=64, use_graph=True, precision="f64"):
         torch = _lib.torch()
         dev = _lib.device()
+        if precision not in ("f64", "f32"):
+            raise ValueError("precision must be 'f64' or 'f32'")
+        self.precision = precision
         self.use_graph = use_graph
         self.stream = torch.cuda.Stream(device=dev)
         handle = ctypes.c_void_p()
-        _lib.call("axs_soft_batch_create", int(max_cases), ctypes.byref(handle))
+        create = "axs_soft_batch_create_f32" if precision == "f32" else "axs_soft_batch_create"
+        _lib.call(create, int(max_cases), ctypes.byref(handle))
         self._h = handle
         self.cases = {}
 
@@ -749,10 +766,12 @@ class SoftSphereBatch:
             kill_width=3):
         torch = _lib.torch()
         case = _SoftCaseView(self)
-        _soft_case_state(case, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_width)
+        _soft_case_state(case, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_width,
+                         precision=self.precision)
         torch.cuda.synchronize(_lib.device())
         cid = ctypes.c_int()
-        _lib.call("axs_soft_batch_add", self._h, ctypes.byref(case.cfg), ctypes.byref(case._bufs),
+        add = "axs_soft_batch_add_f32" if self.precision == "f32" else "axs_soft_batch_add"
+        _lib.call(add, self._h, ctypes.byref(case.cfg), ctypes.byref(case._bufs),
                   ctypes.byref(cid))
         case.id = cid.value
         self.cases[case.id] = case

@@ -1,6 +1,7 @@
 """Serving form (continuous batching of soft-sphere requests).
 
-* A request's results are the same alone and in the mix: the admitted
+* A request's results are the same alone and in the mix (FP32 bulk, the
+  server's default, and FP64): the admitted
   requests are cases of one soft batch (one launch per pass over all of
   them) whose arithmetic per case does not depend on the others, so the
   final fields of a request run beside others, joining and leaving at
@@ -22,12 +23,14 @@ def _reqs():
             dict(id=3, arrival=0.002, grid_size_z=128, nr=64, G=8.0, **base)]
 
 
-def test_request_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_request_same_alone_and_in_the_mix(precision):
     from pyaxisymflow_amd.serve import SoftServer
 
     reqs = _reqs()
-    alone = SoftServer(max_active=1, chunk=7).run([reqs[0]], keep_state=True)[0]
-    mix = SoftServer(max_active=4, chunk=5).run(reqs, keep_state=True)
+    alone = SoftServer(max_active=1, chunk=7, precision=precision).run([reqs[0]],
+                                                                        keep_state=True)[0]
+    mix = SoftServer(max_active=4, chunk=5, precision=precision).run(reqs, keep_state=True)
     got = [r for r in mix if r["id"] == 0][0]
     assert got["steps"] == alone["steps"] == 40
     for k in ("vorticity", "psi", "eta1", "eta2", "phi", "scalars"):
@@ -38,7 +41,7 @@ def test_serving_limits_and_records():
     from pyaxisymflow_amd.serve import SoftServer, footprint_bytes, latency_summary
 
     reqs = _reqs()
-    budget = footprint_bytes(reqs[2]) + footprint_bytes(reqs[0])  # at most two at once
+    budget = footprint_bytes(reqs[2], "f32") + footprint_bytes(reqs[0], "f32")  # two at once
     recs = SoftServer(max_active=3, hbm_budget=budget, chunk=10).run(reqs)
     assert [r["id"] for r in recs] == [0, 1, 2, 3]
     for r in recs:
