@@ -119,7 +119,7 @@ def test_soft_tail_finishes_the_loop(queued):
 
 
 def test_soft_growing_queue_equals_fixed():
-    """The default stepper starts with 24 queued sweeps and grows the queue
+    """The default stepper starts with 12 queued sweeps and grows the queue
     (re-capturing its graphs) as steps need more; over 200 steps every field
     equals a stepper with a fixed queue of 64, bit for bit."""
     from pyaxisymflow_amd.stepper import SoftSphereStepper
