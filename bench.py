@@ -79,12 +79,16 @@ def time_gpu(nz, nr, steps, warmup, stage_reps):
                           U_0=wl["U_0"], T_ramp=None, r_sph=wl["r_sph"],
                           vorticity=wl["vorticity"], history=steps + warmup + 8)
     stream = st.stream
-    st.step(warmup)
+    # the stepper's own stream carries every kernel and the device is
+    # synchronized on both sides, so the steps need no ordering against
+    # torch's current stream (join=False: a cross-stream wait would put
+    # ~27 us of idle device time at the start of the timed region)
+    st.step(warmup, join=False)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    st.step(steps)
+    st.step(steps, join=False)
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
@@ -121,7 +125,8 @@ def time_gpu(nz, nr, steps, warmup, stage_reps):
     b.record(stream)
     torch.cuda.synchronize()
     copy_gbs = 2 * 8 * n * reps / (a.elapsed_time(b) * 1e-3) / 1e9
-    return ms, finite, float(cd[-1]) if len(cd) else None, stages, copy_gbs
+    launch = "graph" if st.use_graph else "eager"
+    return ms, finite, float(cd[-1]) if len(cd) else None, stages, copy_gbs, launch
 
 
 def time_batched(steps, warmup, seeds=range(64)):
@@ -483,7 +488,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     nz, nr = args.nz, args.nr
-    ms, finite, cd, stages, copy_gbs = time_gpu(nz, nr, args.steps, args.warmup, args.stage_reps)
+    ms, finite, cd, stages, copy_gbs, launch = time_gpu(nz, nr, args.steps, args.warmup,
+                                                        args.stage_reps)
     if not finite:
         print("non-finite vorticity after the timed steps", file=sys.stderr)
         sys.exit(1)
@@ -508,7 +514,7 @@ def main():
         "config": {
             "workload": f"plain FP64 step, seeded 64-Gaussian vorticity + 8-sphere mask, {nz}x{nr}",
             "nz": nz, "nr": nr, "domain": "z in [0,2], r in [0,1]", "lambda": 1e4, "nu": 1.5e-3,
-            "U": 1.0, "graph": True, "last_Cd": cd,
+            "U": 1.0, "launch": launch, "last_Cd": cd,
             "stage_us": {k: round(v, 2) for k, v in stages.items()},
             "step_bytes_per_cell": sum(v for k, v in STAGE_BYTES_PER_CELL.items()
                                        if k not in ("body_mask", "implicit_diffusion")),
@@ -529,7 +535,7 @@ def main():
     }
     forms = {}
     if args.large_steps > 0 and (nz, nr) != (4096, 2048):
-        ms2, finite2, _, st2, _ = time_gpu(4096, 2048, args.large_steps, 8, 5)
+        ms2, finite2, _, st2, _, _ = time_gpu(4096, 2048, args.large_steps, 8, 5)
         forms["fp64_4096x2048"] = dict(
             value=round(2048 * 4096 * args.large_steps / (ms2 / 1e3) / 1e6, 2),
             unit="Mcell-steps/s", ms_per_step=round(ms2 / args.large_steps, 5),

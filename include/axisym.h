@@ -365,9 +365,10 @@ axs_status axs_rigid_stepper_destroy(axs_rigid_stepper* st);
  * not be the null stream). */
 axs_status axs_rigid_stepper_run(axs_rigid_stepper* st, int nsteps, int use_graph, void* stream);
 /* Graph launches queued so far by axs_rigid_stepper_run with use_graph:
- * chunks of 16 steps and single steps (a run takes as many chunks as fit,
- * from either ping-pong parity, then single steps).  Diagnostics; no
- * counterpart in the reference. */
+ * graphs of 16 steps ("chunks") and shorter graphs of 8, 4, 2 or 1 steps (a
+ * run of n steps launches n / 16 chunks, from either ping-pong parity, then
+ * one graph per set bit of n mod 16; its last graph also completes the
+ * run).  Diagnostics; no counterpart in the reference. */
 axs_status axs_rigid_stepper_graph_launches(const axs_rigid_stepper* st, int64_t* chunks,
                                             int64_t* singles);
 /* Profiling hook: the step is a fixed sequence of stages (taper + dct2 rows,

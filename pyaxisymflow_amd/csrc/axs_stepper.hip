@@ -94,13 +94,20 @@ __global__ void copy_field_kernel(T* __restrict__ dst, const T* __restrict__ src
   if (k < n) dst[k] = src[k];
 }
 
+void RigidStepperBase::drop_graphs() {
+  for (int k = 0; k < kGraphSizes; ++k)
+    for (int p = 0; p < 2; ++p)
+      for (int end = 0; end < 2; ++end) {
+        if (graph_execs[k][p][end]) (void)hipGraphExecDestroy(graph_execs[k][p][end]);
+        if (graphs[k][p][end]) (void)hipGraphDestroy(graphs[k][p][end]);
+        graph_execs[k][p][end] = nullptr;
+        graphs[k][p][end] = nullptr;
+      }
+  graph_stream = nullptr;
+}
+
 RigidStepperBase::~RigidStepperBase() {
-  for (int q = 0; q < 2; ++q) {
-    if (graph_exec[q]) (void)hipGraphExecDestroy(graph_exec[q]);
-    if (graph[q]) (void)hipGraphDestroy(graph[q]);
-    if (chunk_exec[q]) (void)hipGraphExecDestroy(chunk_exec[q]);
-    if (chunk_graph[q]) (void)hipGraphDestroy(chunk_graph[q]);
-  }
+  drop_graphs();
   if (partials) (void)hipFree(partials);
   if (slots) (void)hipFree(slots);
   if (spec) (void)hipFree(spec);
@@ -243,62 +250,54 @@ hipError_t RigidStepperBase::run(int nsteps, bool use_graph, hipStream_t s) {
     return settle(s);
   }
   if (graph_stream != s) {
-    for (int q = 0; q < 2; ++q) {
-      if (graph_exec[q]) (void)hipGraphExecDestroy(graph_exec[q]);
-      if (graph[q]) (void)hipGraphDestroy(graph[q]);
-      if (chunk_exec[q]) (void)hipGraphExecDestroy(chunk_exec[q]);
-      if (chunk_graph[q]) (void)hipGraphDestroy(chunk_graph[q]);
-      graph_exec[q] = nullptr;
-      graph[q] = nullptr;
-      chunk_exec[q] = nullptr;
-      chunk_graph[q] = nullptr;
-    }
-    const int saved = parity;
-    // graphs: one step from each parity (q = 0, 1), and kChunkSteps steps
-    // from each parity (q = 2, 3)
-    for (int q = 0; q < 4; ++q) {
-      hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-      if (e != hipSuccess) {
-        parity = saved;
-        return e;
-      }
-      hipError_t le = hipSuccess;
-      if (q < 2) {
-        parity = q;
-        le = enqueue_step(s);
-      } else {
-        parity = q - 2;
-        for (int k = 0; k < kChunkSteps && le == hipSuccess; ++k) {
+    hipError_t e = capture_graphs(s);
+    if (e != hipSuccess) return e;
+  }
+  if (nsteps == 0) return settle(s);
+  int left = nsteps;
+  while (left > 0) {
+    // the largest graph that fits: 16 steps, else the top bit of what is left
+    const int k = left >= kChunkSteps ? kGraphSizes - 1 : 31 - __builtin_clz((unsigned)left);
+    const int n = 1 << k;
+    const int end = left == n ? 1 : 0;
+    hipError_t e = hipGraphLaunch(graph_execs[k][parity][end], s);
+    if (e != hipSuccess) return e;
+    ++(n == kChunkSteps ? chunk_launches : step_launches);
+    left -= n;
+    parity = end ? 0 : (parity ^ (n & 1));
+  }
+  return hipSuccess;
+}
+
+hipError_t RigidStepperBase::capture_graphs(hipStream_t s) {
+  drop_graphs();
+  const int saved = parity;
+  for (int k = 0; k < kGraphSizes; ++k)
+    for (int p = 0; p < 2; ++p)
+      for (int end = 0; end < 2; ++end) {
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+          parity = saved;
+          return e;
+        }
+        parity = p;
+        hipError_t le = hipSuccess;
+        for (int q = 0; q < (1 << k) && le == hipSuccess; ++q) {
           le = enqueue_step(s);
           parity ^= 1;
         }
+        if (le == hipSuccess && end) le = settle(s);
+        e = hipStreamEndCapture(s, &graphs[k][p][end]);
+        if (le == hipSuccess && e == hipSuccess)
+          e = hipGraphInstantiate(&graph_execs[k][p][end], graphs[k][p][end], nullptr, nullptr, 0);
+        if (le != hipSuccess || e != hipSuccess) {
+          parity = saved;
+          return le != hipSuccess ? le : e;
+        }
       }
-      hipGraph_t* g = (q < 2) ? &graph[q] : &chunk_graph[q - 2];
-      hipGraphExec_t* x = (q < 2) ? &graph_exec[q] : &chunk_exec[q - 2];
-      e = hipStreamEndCapture(s, g);
-      if (le == hipSuccess && e == hipSuccess) e = hipGraphInstantiate(x, *g, nullptr, nullptr, 0);
-      if (le != hipSuccess || e != hipSuccess) {
-        parity = saved;
-        return le != hipSuccess ? le : e;
-      }
-    }
-    parity = saved;
-    graph_stream = s;
-  }
-  int q = 0;
-  // a chunk leaves the parity as it found it (kChunkSteps is even)
-  for (; q + kChunkSteps <= nsteps; q += kChunkSteps) {
-    hipError_t e = hipGraphLaunch(chunk_exec[parity], s);
-    if (e != hipSuccess) return e;
-    ++chunk_launches;
-  }
-  for (; q < nsteps; ++q) {
-    hipError_t e = hipGraphLaunch(graph_exec[parity], s);
-    if (e != hipSuccess) return e;
-    ++step_launches;
-    parity ^= 1;
-  }
-  return settle(s);
+  parity = saved;
+  graph_stream = s;
+  return hipSuccess;
 }
 
 // ------------------------------------------------------------ uneven batches

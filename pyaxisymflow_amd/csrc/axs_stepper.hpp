@@ -73,18 +73,19 @@ struct RigidStepperBase {
   // The step reads the stored vorticity from one buffer of the pair
   // (vorticity, work0) and writes the other; parity says which holds it.
   int parity = 0;
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
-  // kChunkSteps consecutive steps (an even count, so the vorticity pair
-  // returns to the parity it started from) in one graph, one graph per
-  // starting parity: a graph launch costs microseconds, as much as a small
-  // kernel.
-  static constexpr int kChunkSteps = 16;
-  static_assert(kChunkSteps % 2 == 0, "a chunk must preserve the ping-pong parity");
-  hipGraph_t chunk_graph[2] = {nullptr, nullptr};
-  hipGraphExec_t chunk_exec[2] = {nullptr, nullptr};
+  // Graphs of 1, 2, 4, 8 and 16 consecutive steps, one per starting parity,
+  // each also in a form that ends the run (the last step's completion and,
+  // at an odd final parity, the copy home).  A run launches n / 16 graphs of
+  // 16 steps and then one graph per set bit of n mod 16, the last in its
+  // ending form: each graph launch leaves the device idle for several
+  // microseconds before its first kernel (~8 us measured), so a run takes
+  // as few launches as its step count allows.
+  static constexpr int kGraphSizes = 5;  // 2^k steps, k < kGraphSizes
+  static constexpr int kChunkSteps = 1 << (kGraphSizes - 1);
+  hipGraph_t graphs[kGraphSizes][2][2] = {};  // [k][starting parity][ends the run]
+  hipGraphExec_t graph_execs[kGraphSizes][2][2] = {};
   hipStream_t graph_stream = nullptr;
-  // graph launches so far: kChunkSteps-step chunks and single steps
+  // graph launches so far: 16-step graphs and shorter ones
   int64_t chunk_launches = 0, step_launches = 0;
 
   hipError_t init(const RigidConfig* cfgs, int nbatch, const RigidBuffers& b);
@@ -92,6 +93,8 @@ struct RigidStepperBase {
   bool stage_enabled(int stage) const;
   hipError_t enqueue_step(hipStream_t s);
   hipError_t run(int nsteps, bool use_graph, hipStream_t s);
+  hipError_t capture_graphs(hipStream_t s);
+  void drop_graphs();
   // Brings the current vorticity back to the home buffer (after odd counts).
   virtual hipError_t settle(hipStream_t s) = 0;
   // u_pen of the last step into the u_pen buffers (on demand).

@@ -121,9 +121,18 @@ class RigidBatchStepper:
     ``force_scale``).  Every case keeps its own t, dt and force history;
     one launch per stage advances all of them.  Accessors take the case
     index.
+
+    ``use_graph``: True replays captured hipGraphs, False launches the
+    stage kernels one by one; "auto" (the default) takes graphs below
+    EAGER_CELLS cells per step and eager launches from there on, where each
+    kernel runs long enough (~8 us and more) for the host to stay ahead and
+    eager launches avoid the graphs' start latency and the device's idle
+    gap between consecutive graphs (~8 us each on this runtime).
     """
 
-    def __init__(self, nz, nr, z_len, cases, kill_width=3, r_len=None, use_graph=True,
+    EAGER_CELLS = 1 << 21
+
+    def __init__(self, nz, nr, z_len, cases, kill_width=3, r_len=None, use_graph="auto",
                  precision="f64", history=1 << 16):
         torch = _lib.torch()
         dev = _lib.device()
@@ -141,7 +150,9 @@ class RigidBatchStepper:
         self.nz, self.nr = int(nz), int(nr)
         self.dx, self.z, self.r = rigid_grid(nz, nr, z_len, r_len)
         dx = self.dx
-        self.use_graph = use_graph
+        if use_graph == "auto":
+            use_graph = B * self.nr * self.nz < self.EAGER_CELLS
+        self.use_graph = bool(use_graph)
         self.stream = torch.cuda.Stream(device=dev)
         ft = torch.float64 if precision == "f64" else torch.float32
         self.field_dtype = ft
@@ -221,7 +232,8 @@ class RigidBatchStepper:
             cur.wait_stream(self.stream)
 
     def graph_launches(self):
-        """(chunk graph launches, single-step graph launches) queued so far."""
+        """(16-step graph launches, launches of the shorter 8/4/2/1-step
+        graphs) queued so far."""
         c, s = ctypes.c_int64(), ctypes.c_int64()
         _lib.call("axs_rigid_stepper_graph_launches", self._h, ctypes.byref(c), ctypes.byref(s))
         return c.value, s.value
@@ -474,7 +486,7 @@ class RigidBodyStepper(RigidBatchStepper):
 
     def __init__(self, nz, nr, z_len, nu, brink_lam, char_func=None, U_0=1.0, T_ramp=None,
                  r_sph=None, CFL=0.1, vorticity=None, oscillation=None, history=1 << 16,
-                 kill_width=3, r_len=None, use_graph=True, precision="f64", moving_body=None,
+                 kill_width=3, r_len=None, use_graph="auto", precision="f64", moving_body=None,
                  average_psi=False, dt_cap=None, period=None, t_end=None, force_scale=None,
                  implicit_diffusion=None):
         case = dict(nu=nu, brink_lam=brink_lam, char_func=char_func, U_0=U_0, T_ramp=T_ramp,

@@ -155,9 +155,11 @@ def test_graph_replay_matches_eager():
 
 
 def test_chunk_graphs_after_an_odd_step_count():
-    """step(5) leaves the vorticity pair at the odd parity; the following
-    step(20) must still take a 16-step chunk graph (one per starting parity)
-    and stay bit-equal to eager stepping."""
+    """A run of n steps launches n // 16 graphs of 16 steps and one graph per
+    set bit of n % 16 (the last one completing the run): step(5) is 4 + 1
+    and leaves the vorticity pair at the odd parity, step(20) is 16 + 4 from
+    that parity, step(32) is 16 + 16; every run stays bit-equal to eager
+    stepping."""
     case = sphere_case(seed=4)
     from pyaxisymflow_amd.stepper import RigidBodyStepper
 
@@ -168,13 +170,13 @@ def test_chunk_graphs_after_an_odd_step_count():
     b = RigidBodyStepper(*args, use_graph=False, **kw)
     a.step(5)
     b.step(5)
-    assert a.graph_launches() == (0, 5)
+    assert a.graph_launches() == (0, 2)
     a.step(20)
     b.step(20)
-    assert a.graph_launches() == (1, 9)
+    assert a.graph_launches() == (1, 3)
     a.step(32)
     b.step(32)
-    assert a.graph_launches() == (3, 9)
+    assert a.graph_launches() == (3, 3)
     assert a.steps_done() == b.steps_done() == 57
     assert np.array_equal(a.vorticity(), b.vorticity())
     assert np.array_equal(a.psi(), b.psi())
