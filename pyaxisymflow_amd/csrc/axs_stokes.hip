@@ -201,6 +201,9 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
     // spectrum, as in the step: timed alone, back to back, the radial pass
     // finds its modes still in each XCD's L2 and ranks the layouts
     // differently (a layout 10% slower in the step measured fastest alone).
+    // A plan without row tables (the implicit diffusion's radial plan, whose
+    // rows are DST-I passes of its own) times the radial pass alone.
+    const bool row_tables = general ? any != nullptr : (tw && e1 && e2);
     const size_t cells = (size_t)nr * nz;
     double* rows_in = nullptr;
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&rows_in), sizeof(double) * cells);
@@ -214,7 +217,8 @@ hipError_t StokesPlan::tune_tridiag(const std::vector<double>& ha, const std::ve
       return e;
     }
     auto in_step = [&]() {
-      hipError_t le = launch_dct2_rows<double>(*this, rows_in, spec, nullptr, nr, 0);
+      hipError_t le =
+          row_tables ? launch_dct2_rows<double>(*this, rows_in, spec, nullptr, nr, 0) : hipSuccess;
       return le == hipSuccess ? launch_tridiag<double>(*this, spec, 0) : le;
     };
     hipEvent_t t0, t1;
@@ -1315,9 +1319,16 @@ static int fft_block(int M) {
   return std::min(256, std::max(64, M / 2));
 }
 
+// A plan whose row tables were never built (a radial-only plan) refuses
+// the row passes instead of handing their kernels null tables.
+static bool has_row_tables(const StokesPlan& pl) {
+  return pl.general ? pl.any != nullptr : (pl.tw && pl.e1 && pl.e2);
+}
+
 template <typename T>
 hipError_t launch_dct2_rows(const StokesPlan& pl, const T* src, T* dst, const double* rscale,
                             int rows, hipStream_t s) {
+  if (!has_row_tables(pl)) return hipErrorInvalidValue;
   if (pl.general)
     return launch_any_dct2_rows<T>(pl, src, nullptr, dst, rscale, nullptr, nullptr, 0, false, 1,
                                    nullptr, s);
@@ -1359,6 +1370,7 @@ template <typename T>
 hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered, T* dst,
                                   const double* wz, const double* wr, int w, hipStream_t s,
                                   int batch, const StepFinalize* fin) {
+  if (!has_row_tables(pl)) return hipErrorInvalidValue;
   if (pl.general)
     return launch_any_dct2_rows<T>(pl, src, tapered, dst, pl.rvec, wz, wr, w, true, batch, fin,
                                    s);
@@ -1404,6 +1416,7 @@ hipError_t launch_taper_dct2_rows(const StokesPlan& pl, const T* src, T* tapered
 template <typename T>
 hipError_t launch_dct3_rows(const StokesPlan& pl, const T* src, T* dst, int rows, hipStream_t s,
                             int batch) {
+  if (!has_row_tables(pl)) return hipErrorInvalidValue;
   if (pl.general) return launch_any_dct3_rows<T>(pl, src, dst, rows, batch, s);
   const int M = pl.nz / 2;
   const int bs = fft_block(M);
