@@ -86,6 +86,16 @@ __device__ __forceinline__ T div_by(T x, T d, T y) {
   }
 }
 
+// Maximum of two values that are non-negative or NaN, NaN winning, as
+// numpy's amax propagates NaN (fmax drops it): with the sign bit cleared,
+// non-negative doubles order as their bit patterns do and every NaN lies
+// above +inf.  Equal to fmax on non-negative numbers, bit for bit.
+__device__ __forceinline__ double max_nan(double a, double b) {
+  const long long x = __double_as_longlong(a) & 0x7fffffffffffffffLL;
+  const long long y = __double_as_longlong(b) & 0x7fffffffffffffffLL;
+  return __longlong_as_double(x > y ? x : y);
+}
+
 // Appends value to list for the lanes with take set: one atomic per wave
 // for all of them (a counter that every lane of a large grid bumps alone
 // serialises the whole pass), lanes in lane order after the wave's base.

@@ -73,12 +73,12 @@ __global__ void soft_velocity_max_kernel(const double* __restrict__ uz,
   double m = 0.0;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * blockDim.x)
-    m = fmax(m, fabs(uz[k]) + fabs(ur[k]));
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    m = max_nan(m, fabs(uz[k]) + fabs(ur[k]));
+  for (int off = 32; off > 0; off >>= 1) m = max_nan(m, __shfl_xor(m, off));
   if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+    m = max_nan(max_nan(wm[0], wm[1]), max_nan(wm[2], wm[3]));
     atomicMax(reinterpret_cast<unsigned long long*>(slots + (blockIdx.x % kMaxSlots) * kSlotStride),
               static_cast<unsigned long long>(__double_as_longlong(m)));
   }
@@ -91,7 +91,7 @@ __device__ __forceinline__ void soft_dt_body(double* __restrict__ sc, double* __
   double m = __longlong_as_double(static_cast<long long>(
       __hip_atomic_load(reinterpret_cast<unsigned long long*>(slots + threadIdx.x * kSlotStride),
                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  for (int off = 32; off > 0; off >>= 1) m = max_nan(m, __shfl_xor(m, off));
   slots[threadIdx.x * kSlotStride] = 0.0;
   if (threadIdx.x != 0) return;
   double dt = CFL * dx / (m + eps);

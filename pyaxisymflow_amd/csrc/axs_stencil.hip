@@ -369,12 +369,14 @@ __global__ void __launch_bounds__(256) soft_batch_velocity_kernel(
                                      j, uz, ur);
     m = double(fabs(uz) + fabs(ur));
   }
+  // NaN-propagating maximum (a NaN velocity makes the case's dt NaN, as
+  // numpy's amax does in the reference's dt rule)
   __shared__ double wm[4];
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  for (int off = 32; off > 0; off >>= 1) m = max_nan(m, __shfl_xor(m, off));
   if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+    m = max_nan(max_nan(wm[0], wm[1]), max_nan(wm[2], wm[3]));
     const int slot = blockIdx.x % kMaxSlots;
     atomicMax(reinterpret_cast<unsigned long long*>(C.slots + slot * kSlotStride),
               static_cast<unsigned long long>(__double_as_longlong(m)));

@@ -21,6 +21,12 @@ every pass in FP64, as the reference's loop does.
 Time-to-solution of a request = completion time - arrival time (queueing,
 set-up, and stepping), measured on the host clock at the end of the chunk
 that completes it.
+
+Failure handling: after every chunk the server reads each active request's
+time and dt; a request whose time or dt is no longer finite and positive
+(a NaN or Inf velocity makes dt NaN, as in the reference's dt rule) is
+evicted at once and recorded as failed.  The other requests are unaffected:
+no pass mixes values of different cases.
 """
 import time
 
@@ -78,11 +84,14 @@ class SoftServer:
             raise ValueError("precision must be 'f32' or 'f64'")
         self.precision = precision
 
-    def run(self, requests, keep_state=False):
+    def run(self, requests, keep_state=False, on_admit=None):
         """Serves `requests` (dicts from request_set) with arrivals relative
         to the start of the call; returns one record per request (arrival,
-        admitted, finished, time_to_solution, steps, final time) and, with
-        keep_state, each request's final fields."""
+        admitted, finished, time_to_solution, steps, final time, failed)
+        and, with keep_state, each request's final fields.  on_admit(request,
+        case), if given, is called once a request has joined the batch and
+        before its first step (to set a custom initial state in the case's
+        tensors)."""
         torch = _lib.torch()
         from pyaxisymflow_amd.stepper import SoftSphereBatch
 
@@ -107,6 +116,9 @@ class SoftServer:
                 r = pending.pop(0)
                 case = batch.add(grid_size_z=r["grid_size_z"], G=r["G"], nu=r["nu"],
                                  rho_f=r["rho_f"], r_ball=r["r_ball"])
+                if on_admit is not None:
+                    on_admit(r, case)
+                    torch.cuda.synchronize()  # its writes land before the first step
                 used += fp(r)
                 active.append(dict(req=r, case=case, done=0,
                                    admitted=time.perf_counter() - t0))
@@ -121,15 +133,18 @@ class SoftServer:
             ev.record(batch.stream)
             ev.synchronize()
             finished = time.perf_counter() - t0
-            for a in list(active):
+            # time and dt of every active request in one read
+            health = torch.stack([a["case"].t_scalars[:2] for a in active]).cpu().numpy()
+            for a, (t_now, dt_now) in zip(list(active), health):
                 a["done"] += k
                 r = a["req"]
-                if a["done"] < r["steps"]:
+                failed = not (np.isfinite(t_now) and np.isfinite(dt_now) and dt_now > 0)
+                if a["done"] < r["steps"] and not failed:
                     continue
                 c = a["case"]
                 rec = dict(id=r["id"], nr=r["nr"], arrival=r["arrival"], admitted=a["admitted"],
-                           finished=finished, steps=a["done"], sim_time=c.time(),
-                           lsq_unconverged=c.lsq_unconverged())
+                           finished=finished, steps=a["done"], sim_time=float(t_now),
+                           lsq_unconverged=c.lsq_unconverged(), failed=failed)
                 rec["time_to_solution"] = rec["finished"] - rec["arrival"]
                 if keep_state:
                     e1, e2 = c.eta()
@@ -145,7 +160,10 @@ class SoftServer:
 
 
 def latency_summary(records):
-    tts = np.array([r["time_to_solution"] for r in records])
-    return dict(requests=len(records), p50_s=float(np.percentile(tts, 50)),
-                p99_s=float(np.percentile(tts, 99)), mean_s=float(tts.mean()),
-                max_s=float(tts.max()))
+    """Percentiles over the requests that completed (failed ones counted
+    apart)."""
+    done = [r for r in records if not r.get("failed")]
+    tts = np.array([r["time_to_solution"] for r in done])
+    return dict(requests=len(done), failed=len(records) - len(done),
+                p50_s=float(np.percentile(tts, 50)), p99_s=float(np.percentile(tts, 99)),
+                mean_s=float(tts.mean()), max_s=float(tts.max()))

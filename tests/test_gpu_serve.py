@@ -8,6 +8,11 @@
   different chunks, equal, bit for bit, those of the same request run alone.
 * Every request completes its steps; time-to-solution >= its stepping; the
   slot and memory limits hold (admission waits instead of overcommitting).
+* A request that fails (a NaN planted in its vorticity at admission: its
+  first step's velocity maximum, and so its dt, is NaN, as numpy's amax
+  makes it in the reference's dt rule) is evicted after its first chunk and
+  recorded as failed; the requests beside it complete with the bits of
+  their solo runs.
 """
 import numpy as np
 import pytest
@@ -76,3 +81,28 @@ def test_request_chunks_do_not_wait_on_each_other():
     assert not ev_long.query(), "the short chunk waited for the long one"
     ev_long.synchronize()
     assert long_run.steps_done() == 401 and short_run.steps_done() == 3
+
+
+def test_failed_request_is_evicted_and_the_rest_unaffected():
+    from pyaxisymflow_amd.serve import SoftServer, latency_summary
+
+    reqs = _reqs()
+
+    def plant(r, case):
+        if r["id"] == 1:
+            case.t_vorticity[case.nr // 2, case.nz // 2] = float("nan")
+
+    print("serving: NaN planted in request 1 of 4", flush=True)
+    alone = {q["id"]: SoftServer(max_active=1, chunk=7).run([q], keep_state=True)[0]
+             for q in reqs if q["id"] != 1}
+    mix = {r["id"]: r for r in SoftServer(max_active=4, chunk=5).run(reqs, keep_state=True,
+                                                                      on_admit=plant)}
+    bad = mix[1]
+    assert bad["failed"] and bad["steps"] == 5 and not np.isfinite(bad["sim_time"])
+    for i, ref in alone.items():
+        got = mix[i]
+        assert not got["failed"] and got["steps"] == 40
+        for k in ("vorticity", "psi", "eta1", "eta2", "phi", "scalars"):
+            assert np.array_equal(got["state"][k], ref["state"][k]), (i, k)
+    s = latency_summary(list(mix.values()))
+    assert s["requests"] == 3 and s["failed"] == 1
