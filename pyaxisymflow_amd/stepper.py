@@ -110,7 +110,58 @@ def _case_config(nr, nz, dx, kill_width, history, precision, nu, brink_lam, char
     return cfg
 
 
-class RigidBatchStepper:
+class _Checkpoint:
+    """Checkpoint and resume of a stepper's device state.
+
+    ``state()`` returns host copies of the arrays the next step depends on
+    (the class's _STATE tensors: fields, scalars, histories); ``load_state``
+    writes such a dict back, and a stepper built with the same parameters
+    then continues bit for bit as the one it was taken from.  Everything
+    else a step uses is rebuilt within the step, or cleared at its end (the
+    completion of a run zeroes the velocity-maximum slots and the pending
+    flag, as at creation).  ``save`` / ``load`` keep the dict in an .npz
+    file (read back without pickle)."""
+
+    _STATE = ()
+
+    def state(self):
+        self.synchronize()
+        out = {}
+        for n in self._STATE:
+            t = getattr(self, "t_" + n, None)
+            if t is not None:
+                out[n] = t.cpu().numpy().copy()
+        return out
+
+    def load_state(self, state):
+        torch = _lib.torch()
+        self.synchronize()
+        tensors = {n: getattr(self, "t_" + n, None) for n in self._STATE}
+        for n, t in tensors.items():
+            if t is None:
+                if n in state:
+                    raise ValueError(f"{n}: this stepper keeps no such array")
+                continue
+            if n not in state:
+                raise KeyError(f"state lacks {n}")
+            a = np.asarray(state[n])
+            want = str(t.dtype).split(".")[-1]
+            if a.shape != tuple(t.shape) or a.dtype != np.dtype(want):
+                raise ValueError(f"{n}: expected {want} {tuple(t.shape)}, got {a.dtype} {a.shape}")
+        for n, t in tensors.items():
+            if t is not None:
+                t.copy_(torch.from_numpy(np.ascontiguousarray(state[n])).to(t.device))
+        torch.cuda.synchronize(_lib.device())
+
+    def save(self, path):
+        np.savez(path, **self.state())
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as z:
+            self.load_state({k: z[k] for k in z.files})
+
+
+class RigidBatchStepper(_Checkpoint):
     """Many rigid-body cases of one grid, stepped together on the GPU (the
     batched form of examples/FlowPastSphere/flow_past_sphere.py:107-188).
 
@@ -131,6 +182,7 @@ class RigidBatchStepper:
     """
 
     EAGER_CELLS = 1 << 21
+    _STATE = ("vorticity", "psi", "char_func", "avg_psi", "scalars", "drag")
 
     def __init__(self, nz, nr, z_len, cases, kill_width=3, r_len=None, use_graph="auto",
                  precision="f64", history=1 << 16):
@@ -292,7 +344,7 @@ class RigidBatchStepper:
             self._h = None
 
 
-class UnevenBatchStepper:
+class UnevenBatchStepper(_Checkpoint):
     """Rigid-body cases of several grids, stepped together with one launch
     per pass over all of them (the batched form of
     examples/FlowPastSphere/flow_past_sphere.py:107-188 over uneven grids).
@@ -418,6 +470,7 @@ class UnevenBatchStepper:
         o = self.offset[case]
         return t[o:o + nr * nz].view(nr, nz)
 
+    _STATE = RigidBatchStepper._STATE
     step = RigidBatchStepper.step
     graph_launches = RigidBatchStepper.graph_launches
     synchronize = RigidBatchStepper.synchronize
@@ -536,6 +589,8 @@ class OscillatingSpheroidStepper(RigidBodyStepper):
 # soft-step scalar slots (axs_soft.hip)
 SS_T, SS_DT, SS_DT_BY_DX, SS_HALF_NU_DT, SS_NU_DT, SS_MAXU, SS_IT, SS_LSQ_SWEEPS, \
     SS_LSQ_UNCONVERGED = range(9)
+# the arrays a soft step's successor depends on (checkpoints)
+_SOFT_STATE = ("vorticity", "psi", "u_z", "u_r", "eta1", "eta2", "phi", "char_func", "scalars")
 
 
 def _soft_case_state(obj, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_width,
@@ -601,7 +656,7 @@ def _soft_case_state(obj, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_
     obj._bufs = bufs
 
 
-class SoftSphereStepper:
+class SoftSphereStepper(_Checkpoint):
     """examples/SoftSphereInTaylorGreenVortex/soft_sphere_in_taylor_green_vortex.py:27-215
     on the GPU: an elastic sphere (reference map eta, shear modulus G) in an
     axisymmetric Taylor-Green-like vortex, every stage of the loop body
@@ -619,6 +674,8 @@ class SoftSphereStepper:
     deform; a loop longer than the queue is finished by one workgroup, so
     the result is always the reference loop's. ``lsq_unconverged()`` counts
     the steps the one-workgroup tail finished."""
+
+    _STATE = _SOFT_STATE
 
     def __init__(self, grid_size_z=256, G=1.0, nu=1e-3, rho_f=1.0, r_ball=0.15, domain_AR=0.5,
                  C=5e-2, lsq_sweeps=None, use_graph=True, kill_width=3):
@@ -706,12 +763,19 @@ class SoftSphereStepper:
             self._h = None
 
 
-class _SoftCaseView:
+class _SoftCaseView(_Checkpoint):
     """Host views of one case of a SoftSphereBatch (its fields live in
-    tensors the batch steps in place)."""
+    tensors the batch steps in place); state() / load_state() checkpoint
+    the case alone, and a state may be loaded into a case of another batch
+    built with the same parameters."""
+
+    _STATE = _SOFT_STATE
 
     def __init__(self, batch):
         self._batch = batch
+
+    def synchronize(self):
+        self._batch.synchronize()
 
     def _host(self, t):
         self._batch.synchronize()
