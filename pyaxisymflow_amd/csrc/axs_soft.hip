@@ -208,19 +208,26 @@ __device__ __forceinline__ void soft_lsq_mirror_at(
   if (upper) chi[c] = O(h);
   const double in = (h > 0.5) ? 1.0 : 0.0;
   const double pd = -p;
-  const double a = in * e1[c];
-  const double b = in * e2[c];
   const int64_t d = (int64_t)I * nz + j;
   const int16_t known = (pd < 0.0) ? 1 : 0;
+  const int16_t target = (pd < extrap_zone) ? 1 : 0;
+  cur[d] = known;
+  tgt[d] = target;
+  cur2[d] = known;
+  // Values only inside the target zone: a fit weights each value of its
+  // window by the cell's known flag and sums from +0, so a cell outside the
+  // zone (never known in this step) adds +-0 whatever finite value it holds
+  // (the buffers start zeroed and only ever receive finite fits); the copy
+  // back gives such cells in * eta (in = 0 there) itself.
+  if (!target) return;
+  const double a = in * e1[c];
+  const double b = in * e2[c];
   e1d[d] = a;
   e2d[d] = upper ? b : -b;
-  cur[d] = known;
-  tgt[d] = (pd < extrap_zone) ? 1 : 0;
   // the second buffers of the ping-pong sweeps (cells off the list never
   // change, so both copies must hold them)
   e1d2[d] = a;
   e2d2[d] = upper ? b : -b;
-  cur2[d] = known;
 }
 
 __global__ void soft_lsq_mirror_kernel(const double* __restrict__ phi,
@@ -248,6 +255,7 @@ __global__ void soft_lsq_back_kernel(double* __restrict__ e1, double* __restrict
                                      const double* __restrict__ e2d,
                                      const double* __restrict__ e1d2,
                                      const double* __restrict__ e2d2,
+                                     const int16_t* __restrict__ tgt,
                                      const unsigned* __restrict__ counters, int sweeps, int64_t n,
                                      int64_t off) {
   __shared__ int s_second;  // the same for every cell: once per workgroup
@@ -255,6 +263,11 @@ __global__ void soft_lsq_back_kernel(double* __restrict__ e1, double* __restrict
   __syncthreads();
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
+  if (!tgt[off + c]) {  // outside the target zone: in * eta with in = 0
+    e1[c] = 0.0 * e1[c];
+    e2[c] = 0.0 * e2[c];
+    return;
+  }
   const bool second = s_second != 0;
   e1[c] = (second ? e1d2 : e1d)[off + c];
   e2[c] = (second ? e2d2 : e2d)[off + c];
@@ -406,6 +419,13 @@ struct SoftStepperT {
     }
     soft_zero_kernel<double><<<1, 64>>>(slots, kMaxSlots, kSlotStride);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    // the doubled grid's values: cells outside a step's target zone keep
+    // what they hold, which must be finite
+    for (double* f : {e1d, e2d, lw.ex2, lw.ey2}) {
+      soft_zero_kernel<double><<<ceil_div((int64_t)nd, 256), 256>>>(f, (int64_t)nd, 1);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipDeviceSynchronize();
   }
 
@@ -473,7 +493,7 @@ struct SoftStepperT {
     AXS_TRY_STEP(launch_lsq_sweep_tail(cur, e1d, e2d, tgt, buf.z, buf.z, 2 * nr, nz, lw,
                                        cfg.lsq_sweeps, s));
     soft_lsq_back_kernel<<<ceil_div(n, 256), 256, 0, s>>>(
-        buf.eta1, buf.eta2, e1d, e2d, lw.ex2, lw.ey2, lw.counters, cfg.lsq_sweeps, n, n);
+        buf.eta1, buf.eta2, e1d, e2d, lw.ex2, lw.ey2, tgt, lw.counters, cfg.lsq_sweeps, n, n);
     AXS_TRY_LAUNCH();
     // the TG driver (:186-213) adds the stress to the tapered vorticity and
     // then advects it; the streaming driver (:198-264) advects it first
@@ -639,6 +659,11 @@ __global__ void soft_batch_back_kernel(const SoftCaseT<B>* __restrict__ cs,
   __syncthreads();
   if (b.j >= C.nz) return;
   const int64_t n = (int64_t)C.nr * C.nz, c = (int64_t)b.i * C.nz + b.j;
+  if (!C.tgt[n + c]) {  // outside the target zone: in * eta with in = 0
+    C.eta1[c] = 0.0 * C.eta1[c];
+    C.eta2[c] = 0.0 * C.eta2[c];
+    return;
+  }
   const bool second = s_second != 0;
   C.eta1[c] = (second ? C.lw.ex2 : C.e1d)[n + c];
   C.eta2[c] = (second ? C.lw.ey2 : C.e2d)[n + c];
