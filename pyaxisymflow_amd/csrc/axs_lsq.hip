@@ -362,6 +362,9 @@ __global__ void soft_batch_lsq_list_kernel(const SoftCaseT<B>* __restrict__ cs,
                                            const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);  // a row of the doubled grid
   const SoftCaseT<B>& C = cs[b.c];
+  // the doubled row's physical row (both halves mirror it); outside the
+  // step's zone every flag is 0 (the whole workgroup leaves together)
+  if (!zone_row(C, b.i >= C.nr ? b.i - C.nr : C.nr - 1 - b.i)) return;
   const int j = b.j;
   const int64_t c = (int64_t)b.i * C.nz + j;
   list_append(j < C.nz && (C.cur[c] ^ C.tgt[c]), (int)c, C.lw.list, C.lw.count);

@@ -192,9 +192,10 @@ __global__ void soft_pin_kernel(double* __restrict__ e1, double* __restrict__ e2
 // doubled rows), the doubled grid of
 // extrapolate_eta_using_least_squares_unb.py:19-24 and the flags of
 // extrapolate_using_least_squares.py:33-35 (lower bound 0); block (0, 0)
-// also clears the extrapolation sweeps' counters.
+// also clears the extrapolation sweeps' counters.  Returns whether the cell
+// lies in the extrapolation's target zone.
 template <typename O>
-__device__ __forceinline__ void soft_lsq_mirror_at(
+__device__ __forceinline__ bool soft_lsq_mirror_at(
     const double* __restrict__ phi, O* __restrict__ chi, double moll_zone,
     const double* __restrict__ e1, const double* __restrict__ e2, double* __restrict__ e1d,
     double* __restrict__ e2d, int16_t* __restrict__ cur, int16_t* __restrict__ tgt,
@@ -219,7 +220,7 @@ __device__ __forceinline__ void soft_lsq_mirror_at(
   // zone (never known in this step) adds +-0 whatever finite value it holds
   // (the buffers start zeroed and only ever receive finite fits); the copy
   // back gives such cells in * eta (in = 0 there) itself.
-  if (!target) return;
+  if (!target) return false;
   const double a = in * e1[c];
   const double b = in * e2[c];
   e1d[d] = a;
@@ -228,6 +229,7 @@ __device__ __forceinline__ void soft_lsq_mirror_at(
   // change, so both copies must hold them)
   e1d2[d] = a;
   e2d2[d] = upper ? b : -b;
+  return true;
 }
 
 __global__ void soft_lsq_mirror_kernel(const double* __restrict__ phi,
@@ -347,6 +349,7 @@ struct SoftStepperT {
   hipEvent_t need_ev = nullptr;
   bool need_pending = false;
   int since_request = 0;  // graph launches queued after the pending copy
+  int* zone = nullptr;    // batch cases: the target zone's rows (zone_row)
 
   template <typename P>
   hipError_t alloc(P** p, size_t bytes) {
@@ -396,6 +399,8 @@ struct SoftStepperT {
     if ((e = alloc(&lw.counters, sizeof(unsigned) * (kMaxQueued + 1))) != hipSuccess) return e;
     if ((e = alloc(&need, sizeof(unsigned))) != hipSuccess) return e;
     if ((e = hipMemset(need, 0, sizeof(unsigned))) != hipSuccess) return e;
+    if ((e = alloc(&zone, 2 * sizeof(int))) != hipSuccess) return e;
+    if ((e = hipMemset(zone, 0, 2 * sizeof(int))) != hipSuccess) return e;
     if (cfg.lsq_grow) {
       if ((e = hipHostMalloc(reinterpret_cast<void**>(&need_host), sizeof(unsigned),
                              hipHostMallocDefault)) != hipSuccess)
@@ -624,6 +629,10 @@ using SoftStepper = SoftStepperT<double>;
 template <typename B>
 __global__ void soft_batch_dt_kernel(const SoftCaseT<B>* __restrict__ cs) {
   const SoftCaseT<B>& C = cs[blockIdx.x];
+  if (threadIdx.x == 0) {  // an empty zone, for this step's mirror to record
+    C.zone[0] = C.nr;
+    C.zone[1] = -1;
+  }
   soft_dt_body(C.scalars, C.slots, C.CFL, C.dx, C.eps, C.G, C.rho_f, C.nu);
 }
 
@@ -644,9 +653,17 @@ __global__ void soft_batch_mirror_kernel(const SoftCaseT<B>* __restrict__ cs,
   const SoftCaseT<B>& C = cs[b.c];
   if (b.i == 0 && b.j < 256)  // the case's first workgroup clears its sweep counters
     for (int k = threadIdx.x; k < ncounters; k += blockDim.x) C.lw.counters[k] = 0u;
-  if (b.j >= C.nz) return;
-  soft_lsq_mirror_at(C.phi, C.chi, C.moll_zone, C.eta1, C.eta2, C.e1d, C.e2d, C.cur, C.tgt,
-                     C.lw.ex2, C.lw.ey2, C.lw.cur2, C.nr, C.nz, C.extrap_zone, b.i, b.j);
+  bool target = false;
+  if (b.j < C.nz)
+    target = soft_lsq_mirror_at(C.phi, C.chi, C.moll_zone, C.eta1, C.eta2, C.e1d, C.e2d, C.cur,
+                                C.tgt, C.lw.ex2, C.lw.ey2, C.lw.cur2, C.nr, C.nz,
+                                C.extrap_zone, b.i, b.j);
+  // the workgroup is one row segment: its row joins the zone if any cell
+  // of the segment is a target (upper half: the physical rows)
+  if (__syncthreads_or(target) && b.i >= C.nr && threadIdx.x == 0) {
+    atomicMin(C.zone, b.i - C.nr);
+    atomicMax(C.zone + 1, b.i - C.nr);
+  }
 }
 
 template <typename B>
@@ -655,6 +672,7 @@ __global__ void soft_batch_back_kernel(const SoftCaseT<B>* __restrict__ cs,
   __shared__ int s_second;
   const SoftBlock b = soft_block(blocks);
   const SoftCaseT<B>& C = cs[b.c];
+  if (!zone_row(C, b.i)) return;  // the whole workgroup (one row)
   if (threadIdx.x == 0) s_second = lsq_result_is_second(C.lw.counters, sweeps) ? 1 : 0;
   __syncthreads();
   if (b.j >= C.nz) return;
@@ -808,6 +826,7 @@ struct SoftBatchT {
     C.rw = S.rw;
     C.lw = S.lw;
     C.need = need;
+    C.zone = S.zone;
     const StokesPlan& P = *S.plan;
     C.any = P.any->args;
     C.ge2 = P.ge2;

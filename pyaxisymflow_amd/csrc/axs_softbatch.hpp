@@ -59,6 +59,7 @@ struct SoftCaseT {
   int reinit_tile_base;  // the case's first entry in the batch's reinit tile table
   LsqWork lw;
   unsigned* need;
+  int* zone;  // rows [zone[0], zone[1]] hold the step's extrapolation target zone
   // row transforms (any length) and the radial solve's tables
   AnyDftArgs any;
   const Cplx<double>* ge2;
@@ -105,6 +106,22 @@ __device__ __forceinline__ SoftBlock soft_block(const uint32_t* __restrict__ t) 
   const uint32_t v = t[blockIdx.x];
   return SoftBlock{int(v & 1023u), int((v >> 10) & 8191u),
                    int(v >> 23) * 256 + int(threadIdx.x)};
+}
+
+// Rows of a case that the passes after the mirror (copy back, stress,
+// stress divergence, extrapolation list) visit: the extrapolation's target
+// zone, recorded by the mirror pass of the same step, and kZoneMargin rows
+// either side; every row in the case's first step.  Beyond them the
+// reference map is zero (the copy back zeroes it outside the target zone,
+// and one step's ENO3 spreads it by two rows while the zone moves by less
+// than one), so the stress and its divergence are +-0 there and leave the
+// vorticity's value unchanged: skipping those rows changes no value (only,
+// at most, the sign of a zero in the reference map or the stress scratch).
+constexpr int kZoneMargin = 6;
+template <class CaseT>
+__device__ __forceinline__ bool zone_row(const CaseT& C, int i) {
+  if (C.scalars[kSsIt] == 0.0) return true;
+  return i >= C.zone[0] - kZoneMargin && i <= C.zone[1] + kZoneMargin;
 }
 
 // Radial segment length of every case of a batch (the tridiagonal launch

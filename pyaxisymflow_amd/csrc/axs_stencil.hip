@@ -390,7 +390,7 @@ __global__ void soft_batch_sigma_kernel(const SoftCaseT<B>* __restrict__ cs,
   const SoftBlock b = soft_block(blocks);
   const SoftCaseT<B>& C = cs[b.c];
   const int i = b.i, j = b.j;
-  if (j >= C.nz) return;
+  if (j >= C.nz || !zone_row(C, i)) return;
   solid_sigma_at<double, B>(C.s11, C.s12, C.s22, C.G, C.dx, C.eta1, C.eta2, C.g1z, C.g1r, C.g2z,
                             C.g2r, C.nr, C.nz, C.chi, i, j);
 }
@@ -401,7 +401,7 @@ __global__ void soft_batch_div_tau1_kernel(const SoftCaseT<B>* __restrict__ cs,
   const SoftBlock b = soft_block(blocks);
   const SoftCaseT<B>& C = cs[b.c];
   const int i = b.i, j = b.j + 1;
-  if (i >= C.nr - 1 || j >= C.nz - 1) return;
+  if (i >= C.nr - 1 || j >= C.nz - 1 || !zone_row(C, i)) return;
   div_tau_stage1_at<B, RRow<B>>(C.tz, C.tr, C.s11, C.s12, C.s22, RRow<B>{C.r}, C.nz,
                                 B(2.0 * C.dx), i, j);
 }
@@ -412,7 +412,7 @@ __global__ void soft_batch_div_tau2_kernel(const SoftCaseT<B>* __restrict__ cs,
   const SoftBlock b = soft_block(blocks);
   const SoftCaseT<B>& C = cs[b.c];
   const int i = b.i, j = b.j + 1;
-  if (i < 1 || i >= C.nr - 1 || j >= C.nz - 1) return;
+  if (i < 1 || i >= C.nr - 1 || j >= C.nz - 1 || !zone_row(C, i)) return;
   div_tau_stage2_at<B>(C.w0, C.tz, C.tr, C.nz, B(C.scalars[kSsDt]), B(2.0 * C.dx), i, j);
 }
 

@@ -5,6 +5,10 @@ launch per pass over all of them.
 * One step of a case in a mixed batch against the oracle's exact route
   (oracle/drivers.py SoftSphereCase): every field within 1e-12 relative L2,
   dt within 1e-12 (the single-case stepper's bar).
+* Twelve steps (from the second step on, the passes after the mirror visit
+  only the extrapolation zone's rows and a margin): every field no further
+  from the oracle's exact route than twice the oracle's own exact-vs-eig
+  spread (the single-case stepper's bar; the loop amplifies round-off).
 * A case gives the same bits alone and in a mix of other grids and solids,
   also when cases join and leave between runs, and graph replay equals
   eager stepping.
@@ -57,6 +61,30 @@ def test_batch_case_one_step_matches_oracle():
     assert case.lsq_unconverged() == 0
     for k, v in errs.items():
         assert v < 1e-12, k
+
+
+def test_batch_case_12_steps_within_oracle_route_spread():
+    from pyaxisymflow_amd.stepper import SoftSphereBatch
+
+    steps = 12
+    print("soft batch: 12 steps vs oracle routes, nz 128 among 96 and 160", flush=True)
+    exact = SoftSphereCase(grid_size_z=128, solver="exact")
+    eig = SoftSphereCase(grid_size_z=128, solver="eig")
+    b = SoftSphereBatch(max_cases=3)
+    b.add(grid_size_z=96, G=0.5)
+    case = b.add(grid_size_z=128)
+    b.add(grid_size_z=160, G=5.0)
+    for _ in range(steps):
+        exact.step()
+        eig.step()
+    b.step(steps)
+    g, c, e = _fields(case), _fields(exact), _fields(eig)
+    for k in c:
+        got, spread = rel_l2(g[k], c[k]), rel_l2(e[k], c[k])
+        print(f"soft batch {steps} steps {k}: gpu {got:.2e} oracle routes {spread:.2e}", flush=True)
+        assert got <= 2 * spread + 1e-14, k
+    assert abs(case.time() - exact.t) <= 1e-12 * exact.t
+    assert case.lsq_unconverged() == 0
 
 
 def test_batch_case_alone_equals_in_mix():
