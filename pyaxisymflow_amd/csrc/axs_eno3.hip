@@ -165,7 +165,9 @@ __global__ void soft_batch_refmap_kernel(const SoftCaseT<B>* __restrict__ cs,
                                          const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);
   const SoftCaseT<B>& C = cs[b.c];
-  if (b.j >= C.nz) return;
+  // beyond the last zone the reference map is zero and stays so this step
+  // (the pin skips the same rows, so e1n / e2n there are never read)
+  if (b.j >= C.nz || !prev_zone_row(C, b.i)) return;
   eno3_nonconservative2_at<double, B>(C.eta1, C.eta2, C.u_z, C.u_r, C.e1n, C.e2n, C.nr, C.nz,
                                       C.scalars[kSsDtByDx], b.i, b.j);
 }

@@ -399,8 +399,13 @@ struct SoftStepperT {
     if ((e = alloc(&lw.counters, sizeof(unsigned) * (kMaxQueued + 1))) != hipSuccess) return e;
     if ((e = alloc(&need, sizeof(unsigned))) != hipSuccess) return e;
     if ((e = hipMemset(need, 0, sizeof(unsigned))) != hipSuccess) return e;
-    if ((e = alloc(&zone, 2 * sizeof(int))) != hipSuccess) return e;
-    if ((e = hipMemset(zone, 0, 2 * sizeof(int))) != hipSuccess) return e;
+    // every row: the first step a case takes here visits the whole grid
+    // (its flags and values are written nowhere else before it), also when
+    // its state was loaded from another run
+    const int all_rows[4] = {0, cfg.nr - 1, 0, cfg.nr - 1};
+    if ((e = alloc(&zone, sizeof(all_rows))) != hipSuccess) return e;
+    if ((e = hipMemcpy(zone, all_rows, sizeof(all_rows), hipMemcpyHostToDevice)) != hipSuccess)
+      return e;
     if (cfg.lsq_grow) {
       if ((e = hipHostMalloc(reinterpret_cast<void**>(&need_host), sizeof(unsigned),
                              hipHostMallocDefault)) != hipSuccess)
@@ -629,7 +634,9 @@ using SoftStepper = SoftStepperT<double>;
 template <typename B>
 __global__ void soft_batch_dt_kernel(const SoftCaseT<B>* __restrict__ cs) {
   const SoftCaseT<B>& C = cs[blockIdx.x];
-  if (threadIdx.x == 0) {  // an empty zone, for this step's mirror to record
+  if (threadIdx.x == 0) {  // the last step's zone kept, an empty one to record
+    C.zone[2] = C.zone[0];
+    C.zone[3] = C.zone[1];
     C.zone[0] = C.nr;
     C.zone[1] = -1;
   }
@@ -641,7 +648,7 @@ __global__ void soft_batch_pin_kernel(const SoftCaseT<B>* __restrict__ cs,
                                       const uint32_t* __restrict__ blocks) {
   const SoftBlock b = soft_block(blocks);
   const SoftCaseT<B>& C = cs[b.c];
-  if (b.j >= C.nz) return;
+  if (b.j >= C.nz || !prev_zone_row(C, b.i)) return;
   soft_pin_at(C.eta1, C.eta2, C.e1n, C.e2n, C.phi, (int64_t)b.i * C.nz + b.j, C.z_cm, C.r_cm,
               C.r_ball, C.dx);
 }
@@ -653,6 +660,9 @@ __global__ void soft_batch_mirror_kernel(const SoftCaseT<B>* __restrict__ cs,
   const SoftCaseT<B>& C = cs[b.c];
   if (b.i == 0 && b.j < 256)  // the case's first workgroup clears its sweep counters
     for (int k = threadIdx.x; k < ncounters; k += blockDim.x) C.lw.counters[k] = 0u;
+  // rows beyond the last zone keep their flags (0) and chi (0) from the
+  // step that last visited them (the whole workgroup leaves together)
+  if (!prev_zone_row(C, b.i >= C.nr ? b.i - C.nr : C.nr - 1 - b.i)) return;
   bool target = false;
   if (b.j < C.nz)
     target = soft_lsq_mirror_at(C.phi, C.chi, C.moll_zone, C.eta1, C.eta2, C.e1d, C.e2d, C.cur,

@@ -59,7 +59,9 @@ struct SoftCaseT {
   int reinit_tile_base;  // the case's first entry in the batch's reinit tile table
   LsqWork lw;
   unsigned* need;
-  int* zone;  // rows [zone[0], zone[1]] hold the step's extrapolation target zone
+  // rows [zone[0], zone[1]] hold the step's extrapolation target zone,
+  // [zone[2], zone[3]] the previous step's
+  int* zone;
   // row transforms (any length) and the radial solve's tables
   AnyDftArgs any;
   const Cplx<double>* ge2;
@@ -122,6 +124,15 @@ template <class CaseT>
 __device__ __forceinline__ bool zone_row(const CaseT& C, int i) {
   if (C.scalars[kSsIt] == 0.0) return true;
   return i >= C.zone[0] - kZoneMargin && i <= C.zone[1] + kZoneMargin;
+}
+// The passes before the mirror (reference-map ENO3, pin, the mirror
+// itself) go by the previous step's zone, two rows wider: the zone moves by
+// less than a row per step.  A case's zone starts as every row, so its first
+// step in a batch visits the whole grid even when its state was loaded.
+template <class CaseT>
+__device__ __forceinline__ bool prev_zone_row(const CaseT& C, int i) {
+  if (C.scalars[kSsIt] == 0.0) return true;
+  return i >= C.zone[2] - kZoneMargin - 2 && i <= C.zone[3] + kZoneMargin + 2;
 }
 
 // Radial segment length of every case of a batch (the tridiagonal launch
