@@ -10,6 +10,9 @@ bit as that one does.
 * Soft sphere (single stepper, 128 x 64): 17 steps, checkpoint, 12 more.
 * Soft batch, FP32 bulk: a case checkpointed inside a three-case batch
   resumes in a batch of its own and matches the case left in the mix.
+* Uneven batch (three grids, FP32): 13 steps, checkpoint, 9 more; the set-up
+  layout searches are pinned (they time their candidates, and two steppers
+  may pick different layouts, whose round-off differs).
 * A state of the wrong shape or type is refused before anything is written.
 """
 import numpy as np
@@ -44,6 +47,34 @@ def test_rigid_resume_is_bit_identical(tmp_path):
     assert np.array_equal(a.psi(), b.psi())
     assert np.array_equal(a.scalars(), b.scalars())
     assert np.array_equal(a.drag_history(), b.drag_history())
+
+
+def test_uneven_batch_resume_is_bit_identical(monkeypatch):
+    from pyaxisymflow_amd import workloads
+    from pyaxisymflow_amd.stepper import UnevenBatchStepper
+
+    monkeypatch.setenv("AXS_TRIDIAG_LK", "8")
+    monkeypatch.setenv("AXS_VMAX_TR", "16")
+    cases = []
+    for seed, (nz, nr) in enumerate([(128, 64), (192, 96), (96, 48)]):
+        sp = workloads.case_spec(seed)
+        sp.update(nz=nz, nr=nr)
+        _, chi = workloads.case_inputs(sp, nz, nr)
+        args = workloads.rigid_case_args(sp, chi)
+        args.update(nz=nz, nr=nr, z_len=1.0, r_len=0.5)
+        cases.append(args)
+    a = UnevenBatchStepper(cases, precision="f32")
+    a.step(13)
+    snap = a.state()
+    b = UnevenBatchStepper(cases, precision="f32")
+    b.load_state(snap)
+    a.step(9)
+    b.step(9)
+    for q in range(len(cases)):
+        assert a.steps_done(q) == b.steps_done(q) == 22
+        assert np.array_equal(a.vorticity(q), b.vorticity(q))
+        assert np.array_equal(a.psi(q), b.psi(q))
+        assert np.array_equal(a.drag_history(q), b.drag_history(q))
 
 
 def test_soft_resume_is_bit_identical():
