@@ -70,6 +70,34 @@ def test_precision_form_error_vs_fp64(nz):
     assert err_u < 1e-4 and err_psi < 1e-4
 
 
+def test_precision_form_at_its_own_size_vs_fp64_oracle():
+    """The precision form at config 3's grid (1024 x 512, Womersley 10,
+    amplitude 0.01) against the FP64 oracle for its first 60 steps (a full
+    period there is ~2900 oracle steps of 0.3 s; the whole-period checks
+    run below at 128 and 256): penalised velocity and psi within 1e-4
+    relative L2, the force within 1e-4 of its maximum, the same time."""
+    from pyaxisymflow_amd.stepper import OscillatingSpheroidStepper
+
+    n = 60
+    print(f"f32 precision form 1024x512: {n} steps vs oracle", flush=True)
+    cpu = OscillatingSpheroidCase(1024, solver="exact", no_cycles=1)
+    for _ in range(n):
+        cpu.step()
+    gpu = OscillatingSpheroidStepper(1024, precision="f32", no_cycles=1)
+    gpu.step(n)
+    assert gpu.steps_done() == cpu.it == n
+    assert abs(gpu.time() - cpu.t) <= 1e-12 * cpu.t
+    uz, ur = gpu.velocity()
+    err_u = rel_l2(np.hypot(uz, ur), np.hypot(cpu.u_z, cpu.u_r))
+    err_psi = rel_l2(gpu.psi(), cpu.psi)
+    f = gpu.drag_history()
+    ref = np.array(cpu.force)
+    err_f = np.abs(f - ref).max() / np.abs(ref).max()
+    print(f"f32 1024x512, {n} steps: u {err_u:.2e} psi {err_psi:.2e} force {err_f:.2e}",
+          flush=True)
+    assert err_u < 1e-4 and err_psi < 1e-4 and err_f < 1e-4
+
+
 @pytest.mark.parametrize("nz,periods", [(128, 2), (256, 1)])
 def test_precision_form_error_vs_fp64_oracle(nz, periods):
     """The precision form against the FP64 oracle (the reference's loop
