@@ -865,10 +865,12 @@ struct SoftBatchT {
       dims.max_nr = std::max(dims.max_nr, P.nr);
       dims.max_nz = std::max(dims.max_nz, P.nz);
       dims.max_chunks = std::max(dims.max_chunks, (int)ceil_div(P.nz, P.tri_K));
-      // row workgroups of ~4 butterflies per thread: more rows in flight
-      // per CU than one butterfly per thread (measured 4% faster steps)
+      // row workgroups of ~16 butterflies per thread (looped past the
+      // unroll of 4): small workgroups, many rows in flight per CU; against
+      // 4 per thread, 6-12% faster steps (FP32 and FP64), 32 no better
       const int half = P.any->args.L / 2;
-      dims.any_nth = std::max(dims.any_nth, std::min(1024, (int)(ceil_div(ceil_div(half, 4), 64) * 64)));
+      dims.any_nth =
+          std::max(dims.any_nth, std::min(1024, (int)(ceil_div(ceil_div(half, 16), 64) * 64)));
       dims.any_lds = std::max(dims.any_lds, P.any->lds_bytes<B>());
       dims.tri_lds =
           std::max(dims.tri_lds, tri_lds_bytes(P.nr, kSoftBatchTriL, P.tri_K, P.tri_P, false));
