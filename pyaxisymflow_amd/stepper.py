@@ -600,6 +600,8 @@ def _soft_case_state(obj, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_
     set as attributes of obj.  precision "f32": the flow fields (vorticity,
     psi, u_z, u_r, char_func) are float32 (the FP32-bulk soft batch); the
     reference map, level set and scalars stay float64."""
+    if precision not in ("f64", "f32"):
+        raise ValueError("precision must be 'f64' or 'f32'")
     torch = _lib.torch()
     dev = _lib.device()
     dx = 1.0 / grid_size_z
@@ -610,29 +612,31 @@ def _soft_case_state(obj, grid_size_z, G, nu, rho_f, r_ball, domain_AR, C, kill_
     z = np.linspace(0 + dx / 2, 1 - dx / 2, nz)
     r = np.linspace(0 + dx / 2, domain_AR - dx / 2, nr)
     obj.z, obj.r = z, r
-    Z, R = np.meshgrid(z, r)
-    phi = -np.sqrt((Z - 0.5) ** 2 + (R - 0.0) ** 2) + r_ball
+    # the driver's fields on its meshgrid (Z[i, j] = z[j], R[i, j] = r[i]);
+    # built from the 1-D coordinates with the driver's operations in its
+    # order, so every value is the same bits: phi on the host (numpy's
+    # sqrt), the vorticity's sines on the 1-D arrays and their product on
+    # the device (correctly rounded multiplies)
+    phi = -np.sqrt((z[None, :] - 0.5) ** 2 + (r[:, None] - 0.0) ** 2) + r_ball
     kz = kr = 2 * np.pi
-    w = C * (kz**2 + kr**2) * np.sin(kz * Z) * np.sin(kr * R)
+    sin_z = torch.from_numpy(C * (kz**2 + kr**2) * np.sin(kz * z)).to(dev)
+    sin_r = torch.from_numpy(np.sin(kr * r)).to(dev)
 
-    if precision not in ("f64", "f32"):
-        raise ValueError("precision must be 'f64' or 'f32'")
-    flow_dtype = np.float32 if precision == "f32" else np.float64
-
-    def t(a, dtype=np.float64):
-        return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).to(dev)
+    def t(a):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
 
     obj.precision = precision
-    obj.t_vorticity = t(w, flow_dtype)
-    obj.t_psi = t(0 * Z, flow_dtype)
-    obj.t_u_z = t(0 * Z, flow_dtype)
-    obj.t_u_r = t(0 * Z, flow_dtype)
-    obj.t_eta1 = t(Z)
-    obj.t_eta2 = t(R)
+    tdt = torch.float32 if precision == "f32" else torch.float64
+    obj.t_vorticity = (sin_z[None, :] * sin_r[:, None]).to(tdt).contiguous()
+    obj.t_psi = torch.zeros((nr, nz), dtype=tdt, device=dev)
+    obj.t_u_z = torch.zeros((nr, nz), dtype=tdt, device=dev)
+    obj.t_u_r = torch.zeros((nr, nz), dtype=tdt, device=dev)
+    obj.t_eta1 = t(z)[None, :].expand(nr, nz).contiguous()
+    obj.t_eta2 = t(r)[:, None].expand(nr, nz).contiguous()
     obj.t_phi = t(phi)
     from pyaxisymflow_amd.kernels.smooth_Heaviside import smooth_Heaviside
 
-    chi = t(0 * Z)
+    chi = torch.zeros((nr, nz), dtype=torch.float64, device=dev)
     smooth_Heaviside(chi, obj.t_phi, moll_zone)
     obj.t_char_func = chi.to(torch.float32) if precision == "f32" else chi
     obj.t_scalars = torch.zeros(N_SCALARS, dtype=torch.float64, device=dev)

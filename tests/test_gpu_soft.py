@@ -38,6 +38,23 @@ def _fields(case_or_stepper):
     return dict(w=c.vorticity(), psi=c.psi(), e1=e1, e2=e2, phi=c.phi(), chi=c.char_func())
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_initial_state_is_the_oracle_s_bit_for_bit(precision):
+    """The device-built initial state (the driver's fields from the 1-D
+    coordinates) equals the oracle's numpy meshgrid fields bit for bit (FP32
+    flow fields: their round-to-nearest casts)."""
+    from pyaxisymflow_amd.stepper import SoftSphereBatch
+
+    cpu = SoftSphereCase(grid_size_z=194, solver="exact")
+    b = SoftSphereBatch(max_cases=1, precision=precision)
+    case = b.add(grid_size_z=194)
+    w = cpu.vorticity.astype(np.float32) if precision == "f32" else cpu.vorticity
+    assert np.array_equal(case.vorticity(), w)
+    assert np.array_equal(case.phi(), cpu.ball_phi)
+    e1, e2 = case.eta()
+    assert np.array_equal(e1, cpu.eta1) and np.array_equal(e2, cpu.eta2)
+
+
 def test_soft_one_step_matches_oracle():
     from pyaxisymflow_amd.stepper import SoftSphereStepper
 
