@@ -47,6 +47,15 @@ struct AnyDft {
   ~AnyDft();
 };
 
+// Butterflies of the plan's widest Stockham stage: L / (its smallest radix).
+// A workgroup of nth threads with PER butterflies each covers the length-L
+// FFT only if PER * nth reaches this.
+inline int widest_stage(const AnyDftArgs& a) {
+  int rmin = a.L > 1 ? a.L : 2;
+  for (int s = 0; s < a.fp.nstages; ++s) rmin = a.fp.radix[s] < rmin ? a.fp.radix[s] : rmin;
+  return a.L / rmin;
+}
+
 // Largest L one workgroup can hold in LDS (FP64: 8192 x 16 B = 128 KiB of
 // the CU's 160 KiB); every length N with L <= kAnyMaxL is supported.
 constexpr int kAnyMaxL = 8192;

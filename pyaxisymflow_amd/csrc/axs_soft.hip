@@ -865,19 +865,19 @@ struct SoftBatchT {
       dims.max_nr = std::max(dims.max_nr, P.nr);
       dims.max_nz = std::max(dims.max_nz, P.nz);
       dims.max_chunks = std::max(dims.max_chunks, (int)ceil_div(P.nz, P.tri_K));
-      // row workgroups of ~16 butterflies per thread (looped past the
-      // unroll of 4): small workgroups, many rows in flight per CU; against
-      // 4 per thread, 6-12% faster steps (FP32 and FP64), 32 no better
-      const int half = P.any->args.L / 2;
+      // a Stockham stage gives every butterfly its own lane slot in one
+      // pass (at most 4 per thread, no loop): the widest stage, L / (the
+      // smallest radix) butterflies, sets the row workgroup, ~4 per thread
+      const int wide = widest_stage(P.any->args);
       dims.any_nth =
-          std::max(dims.any_nth, std::min(1024, (int)(ceil_div(ceil_div(half, 16), 64) * 64)));
+          std::max(dims.any_nth, std::min(1024, (int)(ceil_div(ceil_div(wide, 4), 64) * 64)));
       dims.any_lds = std::max(dims.any_lds, P.any->lds_bytes<B>());
       dims.tri_lds =
           std::max(dims.tri_lds, tri_lds_bytes(P.nr, kSoftBatchTriL, P.tri_K, P.tri_P, false));
     }
     for (const Entry& c : cases) {
-      const int half = c.st->plan->any->args.L / 2;
-      const int per = (int)ceil_div(half, dims.any_nth);
+      const int per = (int)ceil_div(widest_stage(c.st->plan->any->args), dims.any_nth);
+      if (per > 4) return hipErrorInvalidValue;  // a stage would leave butterflies out
       dims.any_per = std::max(dims.any_per, per <= 1 ? 1 : (per <= 2 ? 2 : 4));
     }
     dims.ncase = (int)cases.size();

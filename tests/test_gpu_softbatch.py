@@ -12,7 +12,16 @@ launch per pass over all of them.
 * A case gives the same bits alone and in a mix of other grids and solids,
   also when cases join and leave between runs, and graph replay equals
   eager stepping.
-Grids 48 x 24 to 160 x 80 (nz = 2 nr, as the extrapolation requires).
+* Long rows: one step of cases whose row FFT has a stage of 300 butterflies
+  (nz 580: the chirp-z length 600; nz 1200: the direct length 600, both
+  with a radix-2 stage), in FP64 and FP32 against the oracle: the batch's
+  row workgroup covers every butterfly of the widest stage (a butterfly
+  left out puts an O(1) error into psi).  Bars: FP64 1e-10 relative L2,
+  below the oracle's own exact-vs-eig spread after one step at these grids
+  (omega 6.0e-10 at nz 580, 2.6e-10 at nz 1200; measured on the GPU:
+  omega 8.8e-12, psi 9.8e-14 at nz 580); FP32 1e-4.
+Grids 48 x 24 to 160 x 80 (nz = 2 nr, as the extrapolation requires), and
+580 x 290 and 1200 x 600 for the long rows.
 """
 import numpy as np
 import pytest
@@ -61,6 +70,27 @@ def test_batch_case_one_step_matches_oracle():
     assert case.lsq_unconverged() == 0
     for k, v in errs.items():
         assert v < 1e-12, k
+
+
+@pytest.mark.parametrize("precision,bar", [("f64", 1e-10), ("f32", 1e-4)])
+def test_batch_long_rows_one_step_match_oracle(precision, bar):
+    from pyaxisymflow_amd.stepper import SoftSphereBatch
+
+    print(f"soft batch {precision}: 1 step vs oracle, nz 580 and 1200 among 96", flush=True)
+    b = SoftSphereBatch(max_cases=3, precision=precision)
+    b.add(grid_size_z=96, G=0.5)
+    cases = {nz: b.add(grid_size_z=nz, G=2.0) for nz in (580, 1200)}
+    b.step(1)
+    for nz, case in cases.items():
+        cpu = SoftSphereCase(grid_size_z=nz, G=2.0, solver="exact")
+        cpu.step()
+        g, c = _fields(case), _fields(cpu)
+        errs = {k: rel_l2(np.asarray(g[k], np.float64), c[k]) for k in c}
+        print(f"soft batch {precision} nz {nz}: " + " ".join(f"{k} {v:.2e}" for k, v in errs.items()),
+              flush=True)
+        assert case.lsq_unconverged() == 0
+        for k, v in errs.items():
+            assert v < bar, (nz, k)
 
 
 def test_batch_case_12_steps_within_oracle_route_spread():
