@@ -387,13 +387,17 @@ def time_serving(n_requests, steps, max_active=32, seed=0, precision="f32", open
     recs = srv.run(closed)
     t_closed = time.perf_counter() - t0
     capacity = n_requests / t_closed
-    cells = sum(r["nr"] * r["grid_size_z"] for r in closed) * steps
+    # cell-steps actually done: a request evicted for a non-finite dt counts
+    # only the steps it ran
+    size = {r["id"]: r["nr"] * r["grid_size_z"] for r in closed}
+    cells = sum(size[r["id"]] * r["steps"] for r in recs)
+    failed = sum(1 for r in recs if r.get("failed"))
     if not open_loop:
         return dict(closed_loop_wall_s=round(t_closed, 3),
                     closed_loop_mcell_steps_per_s=round(cells / t_closed / 1e6, 2),
                     capacity_requests_per_s=round(capacity, 3),
                     lsq_tail_steps=int(sum(r["lsq_unconverged"] for r in recs)),
-                    precision=precision)
+                    failed_requests=failed, precision=precision)
     rate = 0.8 * capacity
     open_set = request_set(n_requests, seed=seed, rate=rate, steps=steps)
     t0 = time.perf_counter()
@@ -409,6 +413,7 @@ def time_serving(n_requests, steps, max_active=32, seed=0, precision="f32", open
         offered_load=0.8, open_loop_wall_s=round(t_open, 3),
         closed_loop_wall_s=round(t_closed, 3),
         closed_loop_mcell_steps_per_s=round(cells / t_closed / 1e6, 2),
+        failed_requests=failed + s["failed"],
         # steps whose extrapolation loop outran the queued sweeps and was
         # finished by the one-workgroup tail (results unchanged)
         lsq_tail_steps=int(sum(r["lsq_unconverged"] for r in recs + recs2)),
