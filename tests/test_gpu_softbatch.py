@@ -10,8 +10,8 @@ launch per pass over all of them.
   from the oracle's exact route than twice the oracle's own exact-vs-eig
   spread (the single-case stepper's bar; the loop amplifies round-off).
 * A case gives the same bits alone and in a mix of other grids and solids,
-  also when cases join and leave between runs, and graph replay equals
-  eager stepping.
+  also when cases join and leave between runs and beside a long-row case
+  that widens the row workgroups, and graph replay equals eager stepping.
 * Long rows: one step of cases whose row FFT has a stage of 300 butterflies
   (nz 580: the chirp-z length 600; nz 1200: the direct length 600, both
   with a radix-2 stage), in FP64 and FP32 against the oracle: the batch's
@@ -115,6 +115,25 @@ def test_batch_case_12_steps_within_oracle_route_spread():
         assert got <= 2 * spread + 1e-14, k
     assert abs(case.time() - exact.t) <= 1e-12 * exact.t
     assert case.lsq_unconverged() == 0
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_small_case_same_bits_beside_a_long_row_case(precision):
+    """A long-row case (nz 1200: a row stage of 300 butterflies) widens the
+    batch's row workgroups from 64 to 128 threads; a small case's rows give
+    the same bits either way (each butterfly's arithmetic does not depend
+    on which thread computes it)."""
+    from pyaxisymflow_amd.stepper import SoftSphereBatch
+
+    print(f"soft batch {precision}: nz 96 alone vs beside nz 1200, 6 steps", flush=True)
+    alone = SoftSphereBatch(max_cases=1, precision=precision)
+    a = alone.add(grid_size_z=96, G=0.5)
+    mix = SoftSphereBatch(max_cases=2, precision=precision)
+    mix.add(grid_size_z=1200, G=2.0)
+    m = mix.add(grid_size_z=96, G=0.5)
+    alone.step(6)
+    mix.step(6)
+    _assert_same_bits(a, m, "alone vs beside a long-row case")
 
 
 def test_batch_case_alone_equals_in_mix():
